@@ -1,0 +1,255 @@
+/*
+ * simit_hip.h -- C ABI of the MI355X execution backend for Simit's
+ * sparsity / assembly / blocked-SpMV / CG path.
+ *
+ * Plain C: opaque handles, plain device pointers, sizes and int status
+ * codes.  Every call is stream-ordered on the context's HIP stream unless it
+ * says it synchronises.  All pointers marked "device" must be device memory
+ * of the context's device (hipMalloc'd by the caller, or by smh_malloc).
+ *
+ * Each entry point names the reference interface it replaces (paths are
+ * relative to the simit-lang/simit source tree).  The reference's GPU path
+ * generated these loops at run time as NVVM IR (src/backend/gpu/); here they
+ * are fixed, hand-written gfx950 kernels behind this ABI, called by the C++
+ * backend::Function work-alike in simit_amd/csrc/simit/.
+ */
+#ifndef SIMIT_HIP_H
+#define SIMIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes -------------------------------------------------------
+ * The C++ layer turns any non-zero status into simit::SimitException, as the
+ * reference does for user/internal errors (src/error.cpp:87-97) and CUDA
+ * driver errors (src/backend/gpu/gpu_codegen.h:15-51). */
+#define SMH_OK 0
+#define SMH_ERR_INVALID 1       /* bad argument (null, size, precision ...)  */
+#define SMH_ERR_HIP 2           /* HIP runtime error                          */
+#define SMH_ERR_UNSUPPORTED 3   /* valid but outside this backend's limits    */
+#define SMH_ERR_NOMEM 4         /* device allocation failed                   */
+#define SMH_ERR_BREAKDOWN 5     /* CG breakdown: p'Ap <= 0, z'r <= 0, NaN     */
+#define SMH_ERR_NOT_CONVERGED 6 /* hit maxiters before the tolerance          */
+
+/* value precisions of a blocked-CSR matrix */
+#define SMH_F64 0   /* double values                                         */
+#define SMH_F32 1   /* float values                                          */
+#define SMH_F32X2 2 /* float hi + float lo = the FP64 value to ~2^-48        */
+
+/* index build flags */
+#define SMH_INDEX_FORCE_DIAG 1u /* every row holds its own vertex            */
+
+typedef struct smh_ctx smh_ctx;
+typedef struct smh_index smh_index;
+
+/* ---- context -------------------------------------------------------------
+ * Replaces GPUFunction's cuInit/cuCtxCreate on device 0
+ * (src/backend/gpu/gpu_function.cpp:55-144). */
+int smh_ctx_create(int device, smh_ctx **out);
+int smh_ctx_destroy(smh_ctx *ctx);
+/* Message of the last failed call on this context ("" if none). */
+const char *smh_ctx_last_error(const smh_ctx *ctx);
+/* Use an existing hipStream_t (NULL: the context's own stream). */
+int smh_ctx_set_stream(smh_ctx *ctx, void *hip_stream);
+void *smh_ctx_get_stream(const smh_ctx *ctx);
+int smh_ctx_synchronize(smh_ctx *ctx);
+
+/* ---- device memory: replaces cuMemAlloc / cuMemcpyHtoD / cuMemcpyDtoH in
+ * GPUFunction::pushSetData, pushGlobalTensor, mapArgs and unmapArgs
+ * (gpu_function.cpp:174-197, :219-418).  The copies synchronise. */
+int smh_malloc(smh_ctx *ctx, size_t bytes, void **dptr);
+int smh_free(smh_ctx *ctx, void *dptr);
+int smh_memcpy_htod(smh_ctx *ctx, void *dst, const void *src, size_t bytes);
+int smh_memcpy_dtoh(smh_ctx *ctx, void *dst, const void *src, size_t bytes);
+int smh_memset(smh_ctx *ctx, void *dst, int value, size_t bytes);
+
+/* ---- sparsity index ------------------------------------------------------
+ * Replaces PathIndexBuilder::buildSegmented for the vev path expression
+ * (src/path_indices.cpp:148-530; pexpr from path_expression_analysis.cpp:
+ * 84-91), which the reference evaluates on the host with std::map/std::set
+ * at every Function::init (llvm_function.cpp:438-465, gpu_function.cpp:651).
+ * Built on the device, bit-identical: row v holds the sorted, unique
+ * endpoints of every element v belongs to; a vertex with no element has an
+ * empty row (unless SMH_INDEX_FORCE_DIAG).  nsets edge sets are united, as
+ * an Or of their vev expressions is (path_indices.cpp:432-471).
+ * d_endpoints[s] is the device copy of Set::getEndpointsData() of set s:
+ * int32[ne[s] * card[s]] (graph.h:529).  Also keeps the vertex->element
+ * incidence (Link::ve, path_indices.cpp:248-276) of every set, ascending. */
+int smh_index_build(smh_ctx *ctx, uint32_t nverts, uint32_t nsets,
+                    const uint64_t *ne, const uint32_t *card,
+                    const int32_t *const *d_endpoints, uint32_t flags,
+                    smh_index **out);
+typedef struct {
+  uint32_t nrows;
+  uint64_t nnz;
+  const uint32_t *rowptr; /* device, [nrows+1] ("<idx>.coords")            */
+  const uint32_t *colidx; /* device, [nnz]     ("<idx>.sinks")             */
+} smh_index_view;
+int smh_index_get(const smh_index *idx, smh_index_view *view);
+/* Link::ve of set s: device ve_ptr[nrows+1], ve_elem[ve_ptr[nrows]]. */
+int smh_index_get_ve(const smh_index *idx, uint32_t set,
+                     const uint32_t **ve_ptr, const uint32_t **ve_elem,
+                     uint64_t *n_incidences);
+int smh_index_destroy(smh_ctx *ctx, smh_index *idx);
+
+/* gatherVVLocs (src/inline.cpp:524-577) with loc() (src/runtime.cpp:19-23):
+ * locs[e*card*card + a*card + b] = position of endpoint b in endpoint a's
+ * row, for every element of set s.  Device int32 output. */
+int smh_index_locs(smh_ctx *ctx, const smh_index *idx, uint32_t set,
+                   int32_t *d_locs);
+
+/* ---- blocked-CSR matrix ----------------------------------------------------
+ * The reference's Indexed tensor storage: vals[ij*b*b + bi*b + bj]
+ * (src/runtime.h:21-44, src/ffi.h:23-74, lower_accesses.cpp:164-192). */
+typedef struct {
+  uint32_t nrows;
+  uint32_t bdim;          /* 1 or 3                                        */
+  uint64_t nnz;           /* blocks                                        */
+  const uint32_t *rowptr; /* device                                        */
+  const uint32_t *colidx; /* device                                        */
+  int32_t precision;      /* SMH_F64 / SMH_F32 / SMH_F32X2                 */
+  const void *vals;       /* device: double or float (hi part for F32X2)   */
+  const void *vals_lo;    /* device: float lo part (SMH_F32X2 only)        */
+} smh_bsr;
+
+/* ---- assembly: map ... reduce + over an edge set ---------------------------
+ * Replaces the element loop that inlineMap (src/inline.cpp:821-888) and
+ * LowerMapFunctionRewriter (src/lower/lower_maps.cpp:35-120) emit, zero-init
+ * included (inline.cpp:878-885), and the reference GPU path's f32 atomicAdd
+ * scatter (src/backend/gpu/gpu_backend.cpp:501-520, :788-846).  Here each
+ * output block is owned by one lane that sums its elements in ascending
+ * element order: no atomics, deterministic, and with SMH_F64 bit-identical
+ * to the CPU order.  Outputs: vals (precision as given; vals_lo only for
+ * SMH_F32X2) on idx's pattern, and the dense right-hand side rhs (double). */
+
+/* test/input/program/isprings.sim:19-69 (and apps/springs/isprings.sim):
+ * A = MD + K, f = fmg + fe. */
+typedef struct {
+  double h;
+  double viscous;
+  double grav[3];
+  double fixed_fill;      /* value a fixed point's MD block is set to      */
+  int32_t fixed_identity; /* 1: fixed_fill*I; 0: broadcast to all nine     */
+} smh_springs_params;
+typedef struct {
+  uint32_t npoints;
+  uint64_t nsprings;
+  const int32_t *endpoints; /* device int32[nsprings*2]                    */
+  const double *x, *v;      /* device double[npoints*3]  (Point.x, .v)     */
+  const double *m;          /* device double[npoints]    (Point.m)         */
+  const uint8_t *fixed;     /* device bool[npoints]      (Point.fixed)     */
+  const double *k, *l0;     /* device double[nsprings]   (Spring.k, .l0)   */
+} smh_springs_set;
+int smh_assemble_springs(smh_ctx *ctx, const smh_index *idx,
+                         const smh_springs_params *p, const smh_springs_set *s,
+                         int32_t precision, void *vals, void *vals_lo,
+                         double *rhs);
+
+/* apps/fem/fem_linear.sim:95-194: per-tet B = inv(M), W = -det(M)/6,
+ * Ke = W*B'*E*B scaled by stiff_scale on free-free pairs, lumped mass
+ * rho*W/4 on each diagonal, body load W/4*load on free rows, and
+ * fixed_diag*I added on constrained rows (c > 0). */
+typedef struct {
+  double stiff_scale;
+  double rho;
+  double load[3];
+  double fixed_diag;
+} smh_tet_params;
+typedef struct {
+  uint32_t nverts;
+  uint64_t ntets;
+  const int32_t *endpoints; /* device int32[ntets*4]                       */
+  const double *x;          /* device double[nverts*3]                     */
+  const int32_t *c;         /* device int32[nverts]; > 0 means constrained */
+  const double *mu, *lam;   /* device double[ntets] (Tet.u, Tet.l)         */
+} smh_tet_set;
+int smh_assemble_tet_linear(smh_ctx *ctx, const smh_index *idx,
+                            const smh_tet_params *p, const smh_tet_set *s,
+                            int32_t precision, void *vals, void *vals_lo,
+                            double *rhs);
+
+/* Scalar trilinear hex Poisson (1x1 blocks): kappa_e*h*Kref on free pairs,
+ * source*h^3/8 on free rows, fixed_diag on Dirichlet rows.  Corner order
+ * a = dx + 2*dy + 4*dz. */
+typedef struct {
+  double h;
+  double source;
+  double fixed_diag;
+} smh_hex_params;
+typedef struct {
+  uint32_t nverts;
+  uint64_t nhex;
+  const int32_t *endpoints; /* device int32[nhex*8]                        */
+  const int32_t *dirichlet; /* device int32[nverts]; > 0 means Dirichlet   */
+  const double *kappa;      /* device double[nhex]                         */
+} smh_hex_set;
+int smh_assemble_hex_poisson(smh_ctx *ctx, const smh_index *idx,
+                             const smh_hex_params *p, const smh_hex_set *s,
+                             int32_t precision, void *vals, void *vals_lo,
+                             double *rhs);
+
+/* ---- linear algebra --------------------------------------------------------
+ * y = A*x: the blocked row loop of lower_indexexprs.cpp:630-667.  x and y
+ * are float for SMH_F32 and double for SMH_F64 / SMH_F32X2.  exact_order=1
+ * (SMH_F64 only) sums each row in the reference's ascending ij, bj order
+ * with no fused multiply-add: bit-identical to the CPU loop. */
+int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
+              int32_t exact_order);
+/* r'r style inner product (flatten.cpp:286-306; the reference GPU path used
+ * one global atomicAdd, gpu_backend.cpp:339-365).  Deterministic two-level
+ * reduction in FP64; synchronises and writes *result on the host.
+ * precision SMH_F64: double inputs; SMH_F32: float inputs. */
+int smh_dot(smh_ctx *ctx, int32_t precision, uint64_t n, const void *x,
+            const void *y, double *result);
+/* Block-diagonal copy and 3x3 inverse per row (Jacobi, thermal.sim:229-233;
+ * inv3 of runtime.cpp:135-160): d_dinv is double[nrows*bdim*bdim]. */
+int smh_block_jacobi(smh_ctx *ctx, const smh_bsr *A, double *d_dinv);
+
+/* ---- CG solve ---------------------------------------------------------------
+ * The CG loops Simit programs write inline (apps/springs/isprings.sim:77-93,
+ * apps/fem/fem_linear.sim:211-230, test/input/program/cg.sim:25-53,
+ * apps/thermal/thermal.sim:75-130) as one device-resident solver: no
+ * per-operation host sync (the reference ran the loop on a one-thread
+ * harness with a device sync after every child launch,
+ * gpu_backend.cpp:564-567, :985).  b and x are device double[nrows*bdim];
+ * x holds the initial guess and receives the solution. */
+#define SMH_PRECOND_NONE 0
+#define SMH_PRECOND_JACOBI 1       /* scalar 1/A(i,i)                        */
+#define SMH_PRECOND_BLOCK_JACOBI 2 /* 3x3 inverse of the diagonal block      */
+#define SMH_PRECOND_LEARNED 3      /* block Jacobi + learned correction      */
+typedef struct {
+  int32_t maxiters;
+  double tol;
+  int32_t rel;           /* 0: stop when r'r <= tol; 1: ||r|| <= tol*||b||  */
+  int32_t norm_form;     /* cg.sim form: norm(r) > tol                      */
+  int32_t replace_every; /* > 0: r = b - A x every k iterations             */
+  int32_t precond;
+  int32_t mixed;         /* 1: FP32 inner iteration on vals (hi part) with
+                            FP64 reliable updates r = b - A x on hi+lo
+                            (needs SMH_F32X2)                               */
+  double reliable_delta; /* mixed: update when ||r|| < delta*max ||r||      */
+  int32_t check_every;   /* iterations between host convergence checks     */
+  const float *learned;  /* SMH_PRECOND_LEARNED: device float[nrows*9]
+                            symmetric corrections C_i, applied as
+                            z_i = (Dinv_i + C_i) r_i                        */
+} smh_cg_opts;
+typedef struct {
+  int32_t iters;
+  int32_t reliable_updates;
+  int32_t status;        /* SMH_OK / SMH_ERR_BREAKDOWN / SMH_ERR_NOT_CONVERGED */
+  double rr;             /* final iterated r'r                              */
+  double true_rel;       /* ||b - A x|| / ||b|| in FP64 at exit             */
+  double seconds;        /* wall time of the solve (host clock, synced)     */
+} smh_cg_stats;
+void smh_cg_default_opts(smh_cg_opts *o);
+int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
+                 const smh_cg_opts *opts, smh_cg_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,323 @@
+"""simit_amd -- MI355X execution backend for Simit's sparsity / assembly /
+blocked-SpMV / CG path.
+
+The product is the C ABI in include/simit_hip.h, implemented by hand-written
+gfx950 kernels in simit_amd/csrc/ and built into simit_amd/lib/libsimit_hip.so.
+This module is a thin ctypes binding of that ABI for Python callers (tests,
+the benchmark).  Device memory is handed over as raw device pointers; torch
+tensors are accepted for convenience (torch is plumbing here, never compute).
+
+There is no fallback: if the HIP library is missing or fails to load, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+__all__ = [
+    "SimitError", "load", "Context", "Index", "Bsr", "CGOptions", "CGStats",
+    "SpringsParams", "TetParams", "HexParams",
+    "F64", "F32", "F32X2",
+    "PRECOND_NONE", "PRECOND_JACOBI", "PRECOND_BLOCK_JACOBI", "PRECOND_LEARNED",
+    "INDEX_FORCE_DIAG", "LIB_PATH", "HEADER_PATH",
+]
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libsimit_hip.so"
+HEADER_PATH = PKG.parent / "include" / "simit_hip.h"
+
+OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM, ERR_BREAKDOWN, ERR_NOT_CONVERGED = range(7)
+F64, F32, F32X2 = 0, 1, 2
+PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2, 3
+INDEX_FORCE_DIAG = 1
+
+_STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "UNSUPPORTED", 4: "NOMEM",
+           5: "BREAKDOWN", 6: "NOT_CONVERGED"}
+
+
+class SimitError(RuntimeError):
+    """A non-zero status from the C ABI (the C++ layer throws SimitException)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{_STATUS.get(code, code)}] {message}")
+        self.code = code
+
+
+# ---- ABI structs (must match include/simit_hip.h) -----------------------------
+class IndexView(C.Structure):
+    _fields_ = [("nrows", C.c_uint32), ("nnz", C.c_uint64),
+                ("rowptr", C.c_void_p), ("colidx", C.c_void_p)]
+
+
+class Bsr(C.Structure):
+    _fields_ = [("nrows", C.c_uint32), ("bdim", C.c_uint32), ("nnz", C.c_uint64),
+                ("rowptr", C.c_void_p), ("colidx", C.c_void_p), ("precision", C.c_int32),
+                ("vals", C.c_void_p), ("vals_lo", C.c_void_p)]
+
+
+class SpringsParams(C.Structure):
+    _fields_ = [("h", C.c_double), ("viscous", C.c_double), ("grav", C.c_double * 3),
+                ("fixed_fill", C.c_double), ("fixed_identity", C.c_int32)]
+
+
+class SpringsSet(C.Structure):
+    _fields_ = [("npoints", C.c_uint32), ("nsprings", C.c_uint64), ("endpoints", C.c_void_p),
+                ("x", C.c_void_p), ("v", C.c_void_p), ("m", C.c_void_p),
+                ("fixed", C.c_void_p), ("k", C.c_void_p), ("l0", C.c_void_p)]
+
+
+class TetParams(C.Structure):
+    _fields_ = [("stiff_scale", C.c_double), ("rho", C.c_double), ("load", C.c_double * 3),
+                ("fixed_diag", C.c_double)]
+
+
+class TetSet(C.Structure):
+    _fields_ = [("nverts", C.c_uint32), ("ntets", C.c_uint64), ("endpoints", C.c_void_p),
+                ("x", C.c_void_p), ("c", C.c_void_p), ("mu", C.c_void_p), ("lam", C.c_void_p)]
+
+
+class HexParams(C.Structure):
+    _fields_ = [("h", C.c_double), ("source", C.c_double), ("fixed_diag", C.c_double)]
+
+
+class HexSet(C.Structure):
+    _fields_ = [("nverts", C.c_uint32), ("nhex", C.c_uint64), ("endpoints", C.c_void_p),
+                ("dirichlet", C.c_void_p), ("kappa", C.c_void_p)]
+
+
+class CGOptions(C.Structure):
+    _fields_ = [("maxiters", C.c_int32), ("tol", C.c_double), ("rel", C.c_int32),
+                ("norm_form", C.c_int32), ("replace_every", C.c_int32),
+                ("precond", C.c_int32), ("mixed", C.c_int32),
+                ("reliable_delta", C.c_double), ("check_every", C.c_int32),
+                ("learned", C.c_void_p)]
+
+
+class CGStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("reliable_updates", C.c_int32), ("status", C.c_int32),
+                ("rr", C.c_double), ("true_rel", C.c_double), ("seconds", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def load(path: os.PathLike | None = None) -> C.CDLL:
+    """Load libsimit_hip.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise SimitError(ERR_UNSUPPORTED,
+                         f"{p} is not built; run simit_amd.build or __graft_entry__.build()")
+    lib = C.CDLL(str(p))
+    vp, u32, u64, i32 = C.c_void_p, C.c_uint32, C.c_uint64, C.c_int32
+    sig = {
+        "smh_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "smh_ctx_destroy": (C.c_int, [vp]),
+        "smh_ctx_last_error": (C.c_char_p, [vp]),
+        "smh_ctx_set_stream": (C.c_int, [vp, vp]),
+        "smh_ctx_get_stream": (vp, [vp]),
+        "smh_ctx_synchronize": (C.c_int, [vp]),
+        "smh_malloc": (C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
+        "smh_free": (C.c_int, [vp, vp]),
+        "smh_memcpy_htod": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "smh_memcpy_dtoh": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "smh_memset": (C.c_int, [vp, vp, C.c_int, C.c_size_t]),
+        "smh_index_build": (C.c_int, [vp, u32, u32, C.POINTER(u64), C.POINTER(u32),
+                                      C.POINTER(vp), u32, C.POINTER(vp)]),
+        "smh_index_get": (C.c_int, [vp, C.POINTER(IndexView)]),
+        "smh_index_get_ve": (C.c_int, [vp, u32, C.POINTER(vp), C.POINTER(vp), C.POINTER(u64)]),
+        "smh_index_destroy": (C.c_int, [vp, vp]),
+        "smh_index_locs": (C.c_int, [vp, vp, u32, vp]),
+        "smh_assemble_springs": (C.c_int, [vp, vp, C.POINTER(SpringsParams),
+                                           C.POINTER(SpringsSet), i32, vp, vp, vp]),
+        "smh_assemble_tet_linear": (C.c_int, [vp, vp, C.POINTER(TetParams), C.POINTER(TetSet),
+                                              i32, vp, vp, vp]),
+        "smh_assemble_hex_poisson": (C.c_int, [vp, vp, C.POINTER(HexParams), C.POINTER(HexSet),
+                                               i32, vp, vp, vp]),
+        "smh_bsrmv": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, i32]),
+        "smh_dot": (C.c_int, [vp, i32, u64, vp, vp, C.POINTER(C.c_double)]),
+        "smh_block_jacobi": (C.c_int, [vp, C.POINTER(Bsr), vp]),
+        "smh_cg_default_opts": (None, [C.POINTER(CGOptions)]),
+        "smh_cg_solve": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, C.POINTER(CGOptions),
+                                   C.POINTER(CGStats)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _ptr(t) -> int:
+    """Raw device pointer of a torch tensor, an int, or None."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+class Context:
+    """One smh_ctx on one device, ordered on torch's current stream."""
+
+    def __init__(self, device: int = 0, use_torch_stream: bool = True):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.smh_ctx_create(device, C.byref(h))
+        if rc:
+            raise SimitError(rc, f"smh_ctx_create({device}) failed")
+        self.h = h
+        self.device = device
+        if use_torch_stream:
+            import torch
+            s = torch.cuda.current_stream(device).cuda_stream
+            self._check(self.lib.smh_ctx_set_stream(self.h, C.c_void_p(s)))
+
+    def _check(self, rc: int):
+        if rc:
+            msg = self.lib.smh_ctx_last_error(self.h).decode(errors="replace")
+            raise SimitError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.smh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        self._check(self.lib.smh_ctx_synchronize(self.h))
+
+    # ---- sparsity -----------------------------------------------------------
+    def build_index(self, nverts: int, endpoint_sets, force_diag: bool = False) -> "Index":
+        """endpoint_sets: list of int32 device tensors of shape [ne, card]."""
+        ns = len(endpoint_sets)
+        ne = (C.c_uint64 * ns)(*[int(e.shape[0]) for e in endpoint_sets])
+        card = (C.c_uint32 * ns)(*[int(e.shape[1]) if e.dim() == 2 else 1 for e in endpoint_sets])
+        eps = (C.c_void_p * ns)(*[_ptr(e) for e in endpoint_sets])
+        out = C.c_void_p()
+        self._check(self.lib.smh_index_build(self.h, nverts, ns, ne, card, eps,
+                                             INDEX_FORCE_DIAG if force_diag else 0, C.byref(out)))
+        return Index(self, out, list(endpoint_sets))
+
+    # ---- assembly -----------------------------------------------------------
+    def assemble_springs(self, idx, params: SpringsParams, pts: dict, springs: dict,
+                         precision: int, vals, vals_lo, rhs):
+        s = SpringsSet(idx.nrows, int(springs["endpoints"].shape[0]), _ptr(springs["endpoints"]),
+                       _ptr(pts["x"]), _ptr(pts["v"]), _ptr(pts["m"]), _ptr(pts["fixed"]),
+                       _ptr(springs["k"]), _ptr(springs["l0"]))
+        self._check(self.lib.smh_assemble_springs(self.h, idx.h, C.byref(params), C.byref(s),
+                                                  precision, _ptr(vals), _ptr(vals_lo), _ptr(rhs)))
+
+    def assemble_tets(self, idx, params: TetParams, verts: dict, tets: dict, precision: int,
+                      vals, vals_lo, rhs):
+        s = TetSet(idx.nrows, int(tets["endpoints"].shape[0]), _ptr(tets["endpoints"]),
+                   _ptr(verts["x"]), _ptr(verts["c"]), _ptr(tets["mu"]), _ptr(tets["lam"]))
+        self._check(self.lib.smh_assemble_tet_linear(self.h, idx.h, C.byref(params), C.byref(s),
+                                                     precision, _ptr(vals), _ptr(vals_lo),
+                                                     _ptr(rhs)))
+
+    def assemble_hex(self, idx, params: HexParams, verts: dict, hexes: dict, precision: int,
+                     vals, vals_lo, rhs):
+        s = HexSet(idx.nrows, int(hexes["endpoints"].shape[0]), _ptr(hexes["endpoints"]),
+                   _ptr(verts["dirichlet"]), _ptr(hexes["kappa"]))
+        self._check(self.lib.smh_assemble_hex_poisson(self.h, idx.h, C.byref(params), C.byref(s),
+                                                      precision, _ptr(vals), _ptr(vals_lo),
+                                                      _ptr(rhs)))
+
+    # ---- linear algebra -----------------------------------------------------
+    def bsrmv(self, A: Bsr, x, y, exact_order: bool = False):
+        self._check(self.lib.smh_bsrmv(self.h, C.byref(A), _ptr(x), _ptr(y), int(exact_order)))
+
+    def dot(self, x, y, precision: int | None = None) -> float:
+        import torch
+        if precision is None:
+            precision = F64 if x.dtype == torch.float64 else F32
+        out = C.c_double()
+        self._check(self.lib.smh_dot(self.h, precision, x.numel(), _ptr(x), _ptr(y),
+                                     C.byref(out)))
+        return out.value
+
+    def block_jacobi(self, A: Bsr, dinv):
+        self._check(self.lib.smh_block_jacobi(self.h, C.byref(A), _ptr(dinv)))
+
+    def cg(self, A: Bsr, b, x, opts: CGOptions) -> CGStats:
+        st = CGStats()
+        self._check(self.lib.smh_cg_solve(self.h, C.byref(A), _ptr(b), _ptr(x), C.byref(opts),
+                                          C.byref(st)))
+        return st
+
+    def cg_options(self, **kw) -> CGOptions:
+        o = CGOptions()
+        self.lib.smh_cg_default_opts(C.byref(o))
+        for k, v in kw.items():
+            if k == "learned":
+                v = _ptr(v)
+            setattr(o, k, v)
+        return o
+
+
+class Index:
+    """A device-built vev index (smh_index)."""
+
+    def __init__(self, ctx: Context, h, endpoint_sets):
+        self.ctx = ctx
+        self.h = h
+        self._eps = endpoint_sets  # keep the borrowed endpoint buffers alive
+        v = IndexView()
+        ctx._check(ctx.lib.smh_index_get(h, C.byref(v)))
+        self.nrows, self.nnz = v.nrows, v.nnz
+        self.rowptr_ptr, self.colidx_ptr = v.rowptr, v.colidx
+
+    def to_host(self):
+        """(rowptr, colidx) as numpy uint32 arrays (synchronises)."""
+        import numpy as np
+        rp = np.empty(self.nrows + 1, dtype=np.uint32)
+        ci = np.empty(max(self.nnz, 1), dtype=np.uint32)
+        lib, h = self.ctx.lib, self.ctx.h
+        self.ctx._check(lib.smh_memcpy_dtoh(h, rp.ctypes.data, self.rowptr_ptr, rp.nbytes))
+        if self.nnz:
+            self.ctx._check(lib.smh_memcpy_dtoh(h, ci.ctypes.data, self.colidx_ptr, 4 * self.nnz))
+        return rp, ci[: self.nnz]
+
+    def ve_to_host(self, s: int = 0):
+        import numpy as np
+        p, e, n = C.c_void_p(), C.c_void_p(), C.c_uint64()
+        lib, h = self.ctx.lib, self.ctx.h
+        self.ctx._check(lib.smh_index_get_ve(self.h, s, C.byref(p), C.byref(e), C.byref(n)))
+        ptr = np.empty(self.nrows + 1, dtype=np.uint32)
+        el = np.empty(max(n.value, 1), dtype=np.uint32)
+        self.ctx._check(lib.smh_memcpy_dtoh(h, ptr.ctypes.data, p, ptr.nbytes))
+        if n.value:
+            self.ctx._check(lib.smh_memcpy_dtoh(h, el.ctypes.data, e, 4 * n.value))
+        return ptr, el[: n.value]
+
+    def locs(self, s, out):
+        self.ctx._check(self.ctx.lib.smh_index_locs(self.ctx.h, self.h, s, _ptr(out)))
+
+    def bsr(self, bdim: int, precision: int, vals, vals_lo=None) -> Bsr:
+        return Bsr(self.nrows, bdim, self.nnz, self.rowptr_ptr, self.colidx_ptr, precision,
+                   _ptr(vals), _ptr(vals_lo))
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.smh_index_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,450 @@
+// Deterministic, atomic-free assembly of `map f to E reduce +` into the
+// blocked-CSR pattern of the vev index.
+//
+// Reference semantics (src/inline.cpp:821-888, src/lower/lower_maps.cpp:
+// 35-120): zero the result, then for every element e in ascending order run
+// the element function and add each written block into A[loc(i,j)].  Each
+// stored value is therefore  0 + c(e1) + c(e2) + ...  over the elements that
+// touch that block, in ascending element order.
+//
+// MI355X design ("pull"): one 64-lane wavefront owns 64 consecutive stored
+// blocks, one lane per block (i, j).  The wave walks the ascending element
+// lists (Link::ve) of its rows in chunks of 64 (row, element) pairs: every
+// lane computes one element's state (gather + geometry) into LDS, then every
+// lane adds, in list order, the contributions of the elements of its own row
+// that also hold j.  Each block is summed by exactly one lane in exactly the
+// CPU order, in FP64, with no fused multiply-add: bit-identical to the CPU
+// loop for SMH_F64.  The 64 results are staged through LDS so the stores are
+// coalesced.  No atomics, so repeated runs are bitwise identical.
+#include "smh_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int kWave = 64;
+
+// ---- element programs --------------------------------------------------------
+
+// test/input/program/isprings.sim:19-69
+struct SpringArgs {
+  smh_springs_params p;
+  const int32_t *ep;
+  const double *x, *v, *m, *k, *l0;
+  const uint8_t *fixed;
+};
+struct SpringOp {
+  static constexpr int CARD = 2, BD = 3;
+  using Args = SpringArgs;
+  struct State {
+    int32_t vid[2];
+    int32_t free_[2];
+    double fe0[3];
+    double kb[9];
+  };
+  __device__ static void load(const Args &A, uint64_t e, State &st) {
+    const int32_t a = A.ep[2 * e], b = A.ep[2 * e + 1];
+    st.vid[0] = a;
+    st.vid[1] = b;
+    st.free_[0] = !A.fixed[a];
+    st.free_[1] = !A.fixed[b];
+    const double h = A.p.h, stiffness = A.k[e], l0 = A.l0[e];
+    double dx[3];
+    for (int q = 0; q < 3; ++q) dx[q] = A.x[3 * (int64_t)b + q] - A.x[3 * (int64_t)a + q];
+    const double dxtdx = dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2];
+    const double l = sqrt(dxtdx);
+    const double s = h * stiffness / (l0 * l0) * (l - l0);
+    for (int q = 0; q < 3; ++q) st.fe0[q] = s * dx[q] / l;
+    const double c = h * h * stiffness / (l0 * l0 * l * l);
+    const double r = (l - l0) / l;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        const double o = dx[i] * dx[j];
+        const double d = (i == j ? dxtdx : 0.0) - o;
+        st.kb[3 * i + j] = c * (o + r * d);
+      }
+  }
+  // K(p(a),p(b)) written iff free_a and (a == b or free_b) (isprings.sim:58-68)
+  __device__ static bool block(const Args &, const State &st, int a, int b, double *out) {
+    if (!st.free_[a] || (a != b && !st.free_[b])) return false;
+    for (int q = 0; q < 9; ++q) out[q] = a == b ? st.kb[q] : -st.kb[q];
+    return true;
+  }
+  __device__ static bool diag_extra(const Args &, const State &, int, double *) { return false; }
+  __device__ static void rhs(const Args &, const State &st, int a, double *f) {
+    if (!st.free_[a]) return;
+    for (int q = 0; q < 3; ++q) f[q] += a == 0 ? st.fe0[q] : -st.fe0[q];
+  }
+  // MDK = MD + K and f = fmg + fe, with MD/fmg from compute_mass_damping
+  // (isprings.sim:19-36), for the lane that owns row i's diagonal block.
+  __device__ static void finish_diag(const Args &A, uint32_t i, double *blk, double *f) {
+    double md[9], fmg[3] = {0.0, 0.0, 0.0};
+    const double mass = A.m[i];
+    if (A.fixed[i]) {
+      for (int q = 0; q < 9; ++q)
+        md[q] = (A.p.fixed_identity && (q % 4 != 0)) ? 0.0 : A.p.fixed_fill;
+    } else {
+      const double dm = mass + A.p.h * A.p.viscous;
+      for (int q = 0; q < 9; ++q) md[q] = (q % 4 == 0) ? dm : 0.0;
+      for (int q = 0; q < 3; ++q)
+        fmg[q] = mass * A.v[3 * (int64_t)i + q] + A.p.h * mass * A.p.grav[q];
+    }
+    for (int q = 0; q < 9; ++q) blk[q] = md[q] + blk[q];
+    for (int q = 0; q < 3; ++q) f[q] = fmg[q] + f[q];
+  }
+  __device__ static void isolated_rhs(const Args &A, uint32_t i, double *f) {
+    f[0] = f[1] = f[2] = 0.0;
+    if (A.fixed[i]) return;
+    const double mass = A.m[i];
+    for (int q = 0; q < 3; ++q)
+      f[q] = mass * A.v[3 * (int64_t)i + q] + A.p.h * mass * A.p.grav[q] + 0.0;
+  }
+};
+
+// apps/fem/fem_linear.sim:95-194
+struct TetArgs {
+  smh_tet_params p;
+  const int32_t *ep;
+  const double *x, *mu, *lam;
+  const int32_t *c;
+};
+__device__ __forceinline__ double det3(const double *a) {
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+         a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+__device__ __forceinline__ void inv3(const double *a, double *inv) {
+  const double cof00 = a[4] * a[8] - a[5] * a[7];
+  const double cof01 = -a[3] * a[8] + a[5] * a[6];
+  const double cof02 = a[3] * a[7] - a[4] * a[6];
+  const double cof10 = -a[1] * a[8] + a[2] * a[7];
+  const double cof11 = a[0] * a[8] - a[2] * a[6];
+  const double cof12 = -a[0] * a[7] + a[1] * a[6];
+  const double cof20 = a[1] * a[5] - a[2] * a[4];
+  const double cof21 = -a[0] * a[5] + a[2] * a[3];
+  const double cof22 = a[0] * a[4] - a[1] * a[3];
+  double d = a[0] * cof00 + a[1] * cof01 + a[2] * cof02;
+  d = 1.0 / d;
+  inv[0] = cof00 * d; inv[1] = cof10 * d; inv[2] = cof20 * d;
+  inv[3] = cof01 * d; inv[4] = cof11 * d; inv[5] = cof21 * d;
+  inv[6] = cof02 * d; inv[7] = cof12 * d; inv[8] = cof22 * d;
+}
+struct TetOp {
+  static constexpr int CARD = 4, BD = 3;
+  using Args = TetArgs;
+  struct State {
+    int32_t vid[4];
+    int32_t free_[4];
+    double dN[12];
+    double W, mu, lam;
+  };
+  __device__ static void load(const Args &A, uint64_t e, State &st) {
+    double xs[12];
+    for (int a = 0; a < 4; ++a) {
+      const int32_t v = A.ep[4 * e + a];
+      st.vid[a] = v;
+      st.free_[a] = A.c[v] <= 0;
+      for (int q = 0; q < 3; ++q) xs[3 * a + q] = A.x[3 * (int64_t)v + q];
+    }
+    double M[9], B[9];
+    for (int ii = 0; ii < 3; ++ii)
+      for (int jj = 0; jj < 3; ++jj) M[3 * jj + ii] = xs[3 * ii + jj] - xs[9 + jj];
+    inv3(M, B);
+    st.W = -(1.0 / 6.0) * det3(M);
+    for (int di = 0; di < 3; ++di) st.dN[9 + di] = 0.0;
+    for (int vi = 0; vi < 3; ++vi)
+      for (int di = 0; di < 3; ++di) {
+        st.dN[3 * vi + di] = B[3 * vi + di];
+        st.dN[9 + di] = st.dN[9 + di] - B[3 * vi + di];
+      }
+    st.mu = A.mu[e];
+    st.lam = A.lam[e];
+  }
+  __device__ static bool block(const Args &A, const State &st, int a, int b, double *out) {
+    if (!st.free_[a] || !st.free_[b]) return false;
+    const double *ga = st.dN + 3 * a, *gb = st.dN + 3 * b;
+    const double gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double t = st.lam * ga[i] * gb[j] + st.mu * ga[j] * gb[i];
+        if (i == j) t = t + st.mu * gg;
+        out[3 * i + j] = A.p.stiff_scale * (st.W * t);
+      }
+    return true;
+  }
+  // lumped mass after the element's stiffness writes (fem_linear.sim:168-173)
+  __device__ static bool diag_extra(const Args &A, const State &st, int, double *out) {
+    if (A.p.rho == 0.0) return false;
+    const double mm = 0.25 * A.p.rho * st.W;
+    for (int q = 0; q < 9; ++q) out[q] = (q % 4 == 0) ? mm : 0.0;
+    return true;
+  }
+  __device__ static void rhs(const Args &A, const State &st, int a, double *f) {
+    if (!st.free_[a]) return;
+    const double wq = 0.25 * st.W;
+    for (int q = 0; q < 3; ++q) f[q] += wq * A.p.load[q];
+  }
+  __device__ static void finish_diag(const Args &A, uint32_t i, double *blk, double *) {
+    if (A.p.fixed_diag == 0.0 || A.c[i] <= 0) return;
+    for (int q = 0; q < 9; ++q) blk[q] = ((q % 4 == 0) ? A.p.fixed_diag : 0.0) + blk[q];
+  }
+  __device__ static void isolated_rhs(const Args &, uint32_t, double *f) {
+    f[0] = f[1] = f[2] = 0.0;
+  }
+};
+
+// scalar trilinear hex Poisson, corner a = dx + 2dy + 4dz
+struct HexArgs {
+  smh_hex_params p;
+  const int32_t *ep;
+  const int32_t *dir;
+  const double *kappa;
+};
+struct HexOp {
+  static constexpr int CARD = 8, BD = 1;
+  using Args = HexArgs;
+  struct State {
+    int32_t vid[8];
+    int32_t free_[8];
+    double ke;
+  };
+  __device__ static void load(const Args &A, uint64_t e, State &st) {
+    for (int a = 0; a < 8; ++a) {
+      const int32_t v = A.ep[8 * e + a];
+      st.vid[a] = v;
+      st.free_[a] = A.dir[v] <= 0;
+    }
+    st.ke = A.kappa[e] * A.p.h;
+  }
+  __device__ static bool block(const Args &, const State &st, int a, int b, double *out) {
+    if (!st.free_[a] || !st.free_[b]) return false;
+    const int d = __popc((unsigned)(a ^ b));
+    const double kr = d == 0 ? (1.0 / 3.0) : (d == 1 ? 0.0 : -(1.0 / 12.0));
+    out[0] = st.ke * kr;
+    return true;
+  }
+  __device__ static bool diag_extra(const Args &, const State &, int, double *) { return false; }
+  __device__ static void rhs(const Args &A, const State &st, int a, double *f) {
+    if (!st.free_[a]) return;
+    f[0] += A.p.source * (A.p.h * A.p.h * A.p.h) * 0.125;
+  }
+  __device__ static void finish_diag(const Args &A, uint32_t i, double *blk, double *) {
+    if (A.dir[i] <= 0) return;
+    blk[0] = A.p.fixed_diag + blk[0];
+  }
+  __device__ static void isolated_rhs(const Args &, uint32_t, double *f) { f[0] = 0.0; }
+};
+
+// ---- the pull kernel --------------------------------------------------------
+template <int PREC>
+__device__ __forceinline__ void store_val(void *vals, void *lo, uint64_t i, double v) {
+  if (PREC == SMH_F64) {
+    ((double *)vals)[i] = v;
+  } else if (PREC == SMH_F32) {
+    ((float *)vals)[i] = (float)v;
+  } else {
+    const float h = (float)v;
+    ((float *)vals)[i] = h;
+    ((float *)lo)[i] = (float)(v - (double)h);
+  }
+}
+
+template <class Op, int PREC>
+__global__ void __launch_bounds__(kWave)
+k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
+           const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ colidx,
+           const uint32_t *__restrict__ ve_ptr, const uint32_t *__restrict__ ve_elem,
+           void *__restrict__ vals, void *__restrict__ vals_lo, double *__restrict__ rhs) {
+  constexpr int BD = Op::BD, BB = BD * BD, CARD = Op::CARD;
+  // the element states and, after the last chunk, the output staging area
+  constexpr size_t kStBytes = sizeof(typename Op::State) * kWave;
+  constexpr size_t kOutBytes = sizeof(double) * kWave * BB;
+  __shared__ __align__(16) unsigned char smem[kStBytes > kOutBytes ? kStBytes : kOutBytes];
+  typename Op::State *st = reinterpret_cast<typename Op::State *>(smem);
+  double *outbuf = reinterpret_cast<double *>(smem);
+  __shared__ uint32_t R[kWave + 1];
+  __shared__ uint32_t rfirst_s;
+
+  const int lane = threadIdx.x;
+  const uint64_t b0 = (uint64_t)blockIdx.x * kWave;
+  const uint64_t bk = b0 + lane;
+  const bool valid = bk < nnz;
+  const uint64_t blast = min(b0 + kWave, nnz) - 1;
+
+  // row of b0: last r with rowptr[r] <= b0
+  if (lane == 0) {
+    uint32_t lo = 0, hi = nrows;  // rowptr[lo] <= b0 < rowptr[hi]
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if ((uint64_t)rowptr[mid] <= b0) lo = mid; else hi = mid;
+    }
+    rfirst_s = lo;
+  }
+  __syncthreads();
+  const uint32_t rfirst = rfirst_s;
+  R[lane] = rowptr[min(rfirst + (uint32_t)lane, nrows)];
+  if (lane == 0) R[kWave] = rowptr[min(rfirst + (uint32_t)kWave, nrows)];
+  __syncthreads();
+  uint32_t row;
+  if ((uint64_t)R[kWave] > blast) {
+    uint32_t c = 0;
+    for (int l = 1; l <= kWave; ++l) c += ((uint64_t)R[l] <= bk && rfirst + l <= nrows) ? 1u : 0u;
+    row = rfirst + c;
+    if (row >= nrows) row = nrows - 1;
+  } else {
+    uint32_t lo = rfirst, hi = nrows;
+    while (hi - lo > 1) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if ((uint64_t)rowptr[mid] <= bk) lo = mid; else hi = mid;
+    }
+    row = lo;
+  }
+  // the row of the wave's last block bounds the incidence range
+  uint32_t rlast = valid ? row : rfirst;
+  for (int off = 32; off > 0; off >>= 1) rlast = max(rlast, (uint32_t)__shfl_xor((int)rlast, off, 64));
+  const uint32_t col = valid ? colidx[bk] : 0xffffffffu;
+  const bool is_diag = valid && col == row;
+
+  double acc[BB];
+  for (int q = 0; q < BB; ++q) acc[q] = 0.0;
+  double f[BD];
+  for (int q = 0; q < BD; ++q) f[q] = 0.0;
+
+  const uint32_t P0 = ve_ptr[rfirst], P1 = ve_ptr[rlast + 1];
+  const uint32_t myb = valid ? ve_ptr[row] : 0, mye = valid ? ve_ptr[row + 1] : 0;
+  uint32_t prev_e = 0xffffffffu;
+  int occ = 0;
+  for (uint32_t c0 = P0; c0 < P1; c0 += kWave) {
+    const uint32_t p = c0 + lane;
+    if (p < P1) Op::load(A, ve_elem[p], st[lane]);
+    __syncthreads();
+    if (valid) {
+      const uint32_t lo = max(myb, c0), hi = min(mye, c0 + kWave);
+      for (uint32_t q = lo; q < hi; ++q) {
+        const typename Op::State &s = st[q - c0];
+        const uint32_t e = ve_elem[q];
+        occ = (e == prev_e) ? occ + 1 : 0;
+        prev_e = e;
+        // slot a: the occ-th endpoint equal to row
+        int a = -1, seen = 0;
+        for (int t = 0; t < CARD; ++t)
+          if ((uint32_t)s.vid[t] == row) {
+            if (seen == occ) { a = t; break; }
+            ++seen;
+          }
+        if (a < 0) continue;
+        double blk[BB];
+        for (int b = 0; b < CARD; ++b)
+          if ((uint32_t)s.vid[b] == col && Op::block(A, s, a, b, blk))
+            for (int k = 0; k < BB; ++k) acc[k] += blk[k];
+        if (is_diag) {
+          if (Op::diag_extra(A, s, a, blk))
+            for (int k = 0; k < BB; ++k) acc[k] += blk[k];
+          Op::rhs(A, s, a, f);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (is_diag) {
+    Op::finish_diag(A, row, acc, f);
+    for (int q = 0; q < BD; ++q) rhs[(uint64_t)row * BD + q] = f[q];
+  }
+  // coalesced store through LDS
+  for (int q = 0; q < BB; ++q) outbuf[lane * BB + q] = acc[q];
+  __syncthreads();
+  const uint64_t nval = (min(b0 + kWave, nnz) - b0) * BB;
+  for (int q = 0; q < BB; ++q) {
+    const uint64_t i = (uint64_t)q * kWave + lane;
+    if (i < nval) store_val<PREC>(vals, vals_lo, b0 * BB + i, outbuf[i]);
+  }
+}
+
+// right-hand side of vertices that belong to no element (empty rows)
+template <class Op>
+__global__ void k_isolated(typename Op::Args A, uint32_t nrows, const uint32_t *__restrict__ rowptr,
+                           double *__restrict__ rhs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows || rowptr[i] != rowptr[i + 1]) return;
+  double f[Op::BD];
+  Op::isolated_rhs(A, i, f);
+  for (int q = 0; q < Op::BD; ++q) rhs[(uint64_t)i * Op::BD + q] = f[q];
+}
+
+template <class Op>
+int launch_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, const typename Op::Args &A,
+                    int32_t prec, void *vals, void *vals_lo, double *rhs, const char *name) {
+  using namespace smh;
+  SMH_CHECK(ctx, idx && vals && rhs, SMH_ERR_INVALID, "%s: null index, vals or rhs", name);
+  SMH_CHECK(ctx, prec == SMH_F64 || prec == SMH_F32 || prec == SMH_F32X2, SMH_ERR_INVALID,
+            "%s: bad precision %d", name, prec);
+  SMH_CHECK(ctx, prec != SMH_F32X2 || vals_lo, SMH_ERR_INVALID, "%s: SMH_F32X2 needs vals_lo",
+            name);
+  int set = -1;
+  for (uint32_t s = 0; s < idx->nsets; ++s)
+    if (idx->ep[s] == ep && idx->card[s] == (uint32_t)Op::CARD) set = (int)s;
+  SMH_CHECK(ctx, set >= 0, SMH_ERR_INVALID,
+            "%s: the element set's endpoints are not one of the index's edge sets", name);
+  SMH_CHECK(ctx, idx->nnz * Op::BD * Op::BD < 0xffffffffull * 16, SMH_ERR_UNSUPPORTED,
+            "%s: too many values", name);
+  const uint32_t nrows = idx->nrows;
+  if (nrows == 0) return SMH_OK;
+  const unsigned nb = (unsigned)((idx->nnz + kWave - 1) / kWave);
+  if (nb) {
+#define SMH_ASM(P)                                                                          \
+  hipLaunchKernelGGL((k_assemble<Op, P>), dim3(nb), dim3(kWave), 0, ctx->stream, A, nrows,  \
+                     idx->nnz, idx->rowptr, idx->colidx, idx->ve_ptr[set], idx->ve_elem[set], \
+                     vals, vals_lo, rhs)
+    if (prec == SMH_F64) SMH_ASM(SMH_F64);
+    else if (prec == SMH_F32) SMH_ASM(SMH_F32);
+    else SMH_ASM(SMH_F32X2);
+#undef SMH_ASM
+  }
+  hipLaunchKernelGGL(k_isolated<Op>, dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream, A,
+                     nrows, idx->rowptr, rhs);
+  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
+}
+
+}  // namespace
+
+extern "C" int smh_assemble_springs(smh_ctx *ctx, const smh_index *idx, const smh_springs_params *p,
+                                    const smh_springs_set *s, int32_t prec, void *vals,
+                                    void *vals_lo, double *rhs) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, p && s, SMH_ERR_INVALID, "smh_assemble_springs: null params or set");
+  SMH_CHECK(ctx, idx && s->npoints == idx->nrows, SMH_ERR_INVALID,
+            "smh_assemble_springs: point count does not match the index rows");
+  SMH_CHECK(ctx, s->x && s->v && s->m && s->fixed && (s->nsprings == 0 || (s->k && s->l0)),
+            SMH_ERR_INVALID, "smh_assemble_springs: null field");
+  SpringArgs A{*p, s->endpoints, s->x, s->v, s->m, s->k, s->l0, s->fixed};
+  return launch_assembly<SpringOp>(ctx, idx, s->endpoints, A, prec, vals, vals_lo, rhs,
+                                   "smh_assemble_springs");
+}
+
+extern "C" int smh_assemble_tet_linear(smh_ctx *ctx, const smh_index *idx, const smh_tet_params *p,
+                                       const smh_tet_set *s, int32_t prec, void *vals,
+                                       void *vals_lo, double *rhs) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, p && s, SMH_ERR_INVALID, "smh_assemble_tet_linear: null params or set");
+  SMH_CHECK(ctx, idx && s->nverts == idx->nrows, SMH_ERR_INVALID,
+            "smh_assemble_tet_linear: vertex count does not match the index rows");
+  SMH_CHECK(ctx, s->x && s->c && (s->ntets == 0 || (s->mu && s->lam)), SMH_ERR_INVALID,
+            "smh_assemble_tet_linear: null field");
+  TetArgs A{*p, s->endpoints, s->x, s->mu, s->lam, s->c};
+  return launch_assembly<TetOp>(ctx, idx, s->endpoints, A, prec, vals, vals_lo, rhs,
+                                "smh_assemble_tet_linear");
+}
+
+extern "C" int smh_assemble_hex_poisson(smh_ctx *ctx, const smh_index *idx, const smh_hex_params *p,
+                                        const smh_hex_set *s, int32_t prec, void *vals,
+                                        void *vals_lo, double *rhs) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, p && s, SMH_ERR_INVALID, "smh_assemble_hex_poisson: null params or set");
+  SMH_CHECK(ctx, idx && s->nverts == idx->nrows, SMH_ERR_INVALID,
+            "smh_assemble_hex_poisson: vertex count does not match the index rows");
+  SMH_CHECK(ctx, s->dirichlet && (s->nhex == 0 || s->kappa), SMH_ERR_INVALID,
+            "smh_assemble_hex_poisson: null field");
+  HexArgs A{*p, s->endpoints, s->dirichlet, s->kappa};
+  return launch_assembly<HexOp>(ctx, idx, s->endpoints, A, prec, vals, vals_lo, rhs,
+                                "smh_assemble_hex_poisson");
+}

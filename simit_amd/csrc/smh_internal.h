@@ -1,0 +1,93 @@
+// Internal definitions shared by the HIP translation units of libsimit_hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/simit_hip.h"
+
+struct smh_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  std::string last_error;
+  // small device scratch for reductions: [0..kRedSlots) partials + counters
+  double *red = nullptr;
+  unsigned *red_count = nullptr;
+  double *host_pinned = nullptr;  // pinned host mirror for scalar readbacks
+};
+
+struct smh_index {
+  uint32_t nrows = 0;
+  uint64_t nnz = 0;
+  uint32_t *rowptr = nullptr;
+  uint32_t *colidx = nullptr;
+  uint32_t nsets = 0;
+  std::vector<uint32_t *> ve_ptr;   // per set, [nrows+1]
+  std::vector<uint32_t *> ve_elem;  // per set
+  std::vector<uint64_t> n_inc;      // per set incidences
+  std::vector<uint64_t> ne;         // per set element count
+  std::vector<uint32_t> card;       // per set cardinality
+  std::vector<const int32_t *> ep;  // per set endpoints (borrowed, device)
+};
+
+namespace smh {
+
+constexpr int kRedSlots = 4096;  // max partial sums per reduction
+constexpr int kRedCounters = 16;
+
+int fail(smh_ctx *ctx, int code, const char *fmt, ...);
+
+#define SMH_HIP(ctx, call)                                                   \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return smh::fail((ctx), SMH_ERR_HIP, "%s failed: %s (%s:%d)", #call,  \
+                       hipGetErrorString(e_), __FILE__, __LINE__);           \
+  } while (0)
+
+#define SMH_CHECK(ctx, cond, code, ...)                           \
+  do {                                                            \
+    if (!(cond)) return smh::fail((ctx), (code), __VA_ARGS__);    \
+  } while (0)
+
+#define SMH_LAUNCH_CHECK(ctx)                                               \
+  do {                                                                      \
+    hipError_t e_ = hipGetLastError();                                      \
+    if (e_ != hipSuccess)                                                   \
+      return smh::fail((ctx), SMH_ERR_HIP, "kernel launch failed: %s (%s:%d)", \
+                       hipGetErrorString(e_), __FILE__, __LINE__);          \
+  } while (0)
+
+inline unsigned grid_for(uint64_t n, unsigned block, unsigned cap = 1u << 30) {
+  uint64_t g = (n + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+
+// exclusive scan of uint32 counts into out[0..n] (out[n] = total); returns
+// SMH status; total (uint64) written to *total on the host (synchronises).
+int scan_u32(smh_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n,
+             uint64_t *total);
+
+// device allocation helpers that report through ctx
+int dmalloc(smh_ctx *ctx, void **p, size_t bytes);
+void dfree(void *p);
+
+}  // namespace smh
+
+// ---- wave-level helpers (64-wide wavefronts) ------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+__device__ __forceinline__ float wave_sumf(float v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}

@@ -1,0 +1,156 @@
+"""Synthetic inputs for the hot-path workloads (numpy, host side).
+
+Orderings follow the reference where it has one:
+  * create_box: points x-major (x*ny*nz + y*nz + z) and springs x-edges, then
+    y-edges, then z-edges (src/graph.cpp:39-42, :66-115);
+  * tet and hex grids use the same x-major vertex numbering, cubes x-major.
+
+The 5-tet split uses global parity so that neighbouring cubes agree on their
+face diagonals: in cube (i,j,k) the four corners with (i+dx+j+dy+k+dz) even
+form the central tet and every odd corner forms a corner tet with its three
+(even) neighbours.  Tets are oriented so that fem_linear.sim's
+W = -(1/6) det([x0-x3, x1-x3, x2-x3]) is positive.  Hex corner a is
+dx + 2*dy + 4*dz.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# ---- lattices ------------------------------------------------------------------
+
+
+def create_box(nx: int, ny: int, nz: int):
+    """Point count and int32[ns, 2] spring endpoints in createBox order."""
+    idx = np.arange(nx * ny * nz, dtype=np.int64).reshape(nx, ny, nz)
+    ex = np.stack([idx[:-1, :, :].ravel(), idx[1:, :, :].ravel()], axis=1)
+    ey = np.stack([idx[:, :-1, :].ravel(), idx[:, 1:, :].ravel()], axis=1)
+    ez = np.stack([idx[:, :, :-1].ravel(), idx[:, :, 1:].ravel()], axis=1)
+    return nx * ny * nz, np.concatenate([ex, ey, ez]).astype(np.int32)
+
+
+def grid_coords(nx: int, ny: int, nz: int, spacing=1.0):
+    x, y, z = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], axis=1).astype(np.float64) * spacing
+
+
+def springs_lattice(n: int, seed: int = 0, jitter: float = 0.1, k: float = 1e3,
+                    l0: float = 1.0, mass: float = 1.0):
+    """Jittered n^3 lattice, 6-neighbour springs, z=0 layer fixed."""
+    rng = np.random.default_rng(seed)
+    npts, ep = create_box(n, n, n)
+    x = grid_coords(n, n, n) + rng.uniform(-jitter, jitter, size=(npts, 3))
+    ijk = grid_coords(n, n, n)
+    return {
+        "npoints": npts,
+        "x": x, "v": np.zeros((npts, 3)), "m": np.full(npts, mass),
+        "fixed": (ijk[:, 2] == 0).astype(np.uint8),
+        "endpoints": ep,
+        "k": np.full(ep.shape[0], k), "l0": np.full(ep.shape[0], l0),
+    }
+
+
+# ---- tet and hex grids ------------------------------------------------------------
+_TET5 = {
+    0: [[0, 3, 5, 6], [0, 3, 5, 1], [3, 0, 6, 2], [5, 6, 0, 4], [6, 5, 3, 7]],
+    1: [[1, 2, 4, 7], [1, 2, 4, 0], [2, 1, 7, 3], [4, 7, 1, 5], [7, 4, 2, 6]],
+}
+
+
+def _corner_offsets():
+    return np.array([[a & 1, (a >> 1) & 1, (a >> 2) & 1] for a in range(8)], dtype=np.int64)
+
+
+def _oriented_templates():
+    off = _corner_offsets().astype(np.float64)
+    out = {}
+    for par, tets in _TET5.items():
+        fixed = []
+        for t in tets:
+            t = list(t)
+            M = np.stack([off[t[i]] - off[t[3]] for i in range(3)], axis=1)
+            if np.linalg.det(M) > 0:
+                t[0], t[1] = t[1], t[0]
+            fixed.append(t)
+        out[par] = np.array(fixed, dtype=np.int64)
+    return out
+
+
+def cube_corners(n: int):
+    """int64[n^3, 8] vertex ids of every cube's corners (corner a = dx+2dy+4dz)."""
+    N1 = n + 1
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    off = _corner_offsets()
+    return ((i[:, None] + off[:, 0]) * N1 * N1 + (j[:, None] + off[:, 1]) * N1
+            + (k[:, None] + off[:, 2])), (i + j + k) & 1
+
+
+def tet5_grid(n: int):
+    """Unit cube split into n^3 cubes of 5 tets: coords [(n+1)^3, 3], tets int32[5n^3, 4]."""
+    corners, par = cube_corners(n)
+    tmpl = _oriented_templates()
+    tets = np.empty((n ** 3, 5, 4), dtype=np.int64)
+    for p in (0, 1):
+        sel = par == p
+        tets[sel] = corners[sel][:, tmpl[p]]
+    x = grid_coords(n + 1, n + 1, n + 1, 1.0 / n)
+    return x, tets.reshape(-1, 4).astype(np.int32)
+
+
+def hex_grid(n: int):
+    corners, _ = cube_corners(n)
+    x = grid_coords(n + 1, n + 1, n + 1, 1.0 / n)
+    return x, corners.astype(np.int32)
+
+
+# ---- random material fields ------------------------------------------------------------
+
+
+def gaussian_random_field(n: int, corr_len: float, seed: int):
+    """Zero-mean, unit-std periodic field on n^3 cells: seeded white noise
+    smoothed in Fourier space by a Gaussian kernel of std corr_len cells."""
+    rng = np.random.default_rng(seed)
+    w = rng.standard_normal((n, n, n))
+    f = np.fft.fftfreq(n)
+    kx, ky, kz = np.meshgrid(f, f, f, indexing="ij")
+    k2 = (2 * np.pi) ** 2 * (kx ** 2 + ky ** 2 + kz ** 2)
+    g = np.real(np.fft.ifftn(np.fft.fftn(w) * np.exp(-0.5 * k2 * corr_len ** 2)))
+    g -= g.mean()
+    s = g.std()
+    return g / s if s > 1e-12 else np.zeros_like(g)
+
+
+def lame(E, nu):
+    mu = E / (2.0 * (1.0 + nu))
+    lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return mu, lam
+
+
+def tet_problem(n: int, seed: int = 0, field: str = "iid", mean_log10=5.0, std_log10=0.5,
+                corr_len=8.0, nu=0.3, load=(0.0, 0.0, -9.81e3)):
+    """Clamped-bottom elasticity on the 5-tet grid (z=0 face constrained)."""
+    x, tets = tet5_grid(n)
+    if field == "iid":
+        rng = np.random.default_rng(seed)
+        log10E = rng.normal(mean_log10, std_log10, size=tets.shape[0])
+    else:
+        g = gaussian_random_field(n, corr_len, seed).ravel()
+        log10E = np.repeat(mean_log10 + std_log10 * g, 5)
+    E = 10.0 ** log10E
+    mu, lam = lame(E, nu)
+    nv = x.shape[0]
+    c = (np.arange(nv) % (n + 1) == 0).astype(np.int32)  # z index 0
+    return {"nverts": nv, "x": x, "c": c, "endpoints": tets, "mu": mu, "lam": lam,
+            "load": load}
+
+
+def hex_problem(n: int, seed: int = 0, std_log10=1.5, corr_len=12.0, source=1.0):
+    """Scalar Poisson on n^3 trilinear hexes, T=0 on the x=0 face."""
+    x, hexes = hex_grid(n)
+    g = gaussian_random_field(n, corr_len, seed).ravel() if n > 1 else np.zeros(1)
+    kappa = 10.0 ** (std_log10 * g)
+    nv = x.shape[0]
+    N1 = n + 1
+    dirichlet = (np.arange(nv) // (N1 * N1) == 0).astype(np.int32)
+    return {"nverts": nv, "x": x, "dirichlet": dirichlet, "endpoints": hexes, "kappa": kappa,
+            "h": 1.0 / n, "source": source}
