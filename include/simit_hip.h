@@ -205,6 +205,11 @@ int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
  * precision SMH_F64: double inputs; SMH_F32: float inputs. */
 int smh_dot(smh_ctx *ctx, int32_t precision, uint64_t n, const void *x,
             const void *y, double *result);
+/* y = a*x + b*y on double vectors: the dense vector statements of Simit
+ * programs (e.g. points.x = points.x + h*points.v, isprings.sim:94; the
+ * reference GPU path ran each as its own kernel, gpu_backend.cpp:1150-1215). */
+int smh_axpby(smh_ctx *ctx, uint64_t n, double a, const double *x, double b,
+              double *y);
 /* Block-diagonal copy and 3x3 inverse per row (Jacobi, thermal.sim:229-233;
  * inv3 of runtime.cpp:135-160): d_dinv is double[nrows*bdim*bdim]. */
 int smh_block_jacobi(smh_ctx *ctx, const smh_bsr *A, double *d_dinv);
@@ -236,6 +241,8 @@ typedef struct {
   const float *learned;  /* SMH_PRECOND_LEARNED: device float[nrows*9]
                             symmetric corrections C_i, applied as
                             z_i = (Dinv_i + C_i) r_i                        */
+  int32_t zr_stop;       /* 1: also stop when z'r <= the threshold, as
+                            thermal.sim's pre_conjgrad does (:115)          */
 } smh_cg_opts;
 typedef struct {
   int32_t iters;

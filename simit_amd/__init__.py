@@ -92,7 +92,7 @@ class CGOptions(C.Structure):
                 ("norm_form", C.c_int32), ("replace_every", C.c_int32),
                 ("precond", C.c_int32), ("mixed", C.c_int32),
                 ("reliable_delta", C.c_double), ("check_every", C.c_int32),
-                ("learned", C.c_void_p)]
+                ("learned", C.c_void_p), ("zr_stop", C.c_int32)]
 
 
 class CGStats(C.Structure):
@@ -144,6 +144,7 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_bsrmv": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, i32]),
         "smh_dot": (C.c_int, [vp, i32, u64, vp, vp, C.POINTER(C.c_double)]),
         "smh_block_jacobi": (C.c_int, [vp, C.POINTER(Bsr), vp]),
+        "smh_axpby": (C.c_int, [vp, u64, C.c_double, vp, C.c_double, vp]),
         "smh_cg_default_opts": (None, [C.POINTER(CGOptions)]),
         "smh_cg_solve": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, C.POINTER(CGOptions),
                                    C.POINTER(CGStats)]),
@@ -249,6 +250,10 @@ class Context:
         self._check(self.lib.smh_dot(self.h, precision, x.numel(), _ptr(x), _ptr(y),
                                      C.byref(out)))
         return out.value
+
+    def axpby(self, a: float, x, b: float, y):
+        """y = a*x + b*y (double vectors)."""
+        self._check(self.lib.smh_axpby(self.h, y.numel(), a, _ptr(x), b, _ptr(y)))
 
     def block_jacobi(self, A: Bsr, dinv):
         self._check(self.lib.smh_block_jacobi(self.h, C.byref(A), _ptr(dinv)))

@@ -488,7 +488,8 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const double *b, double *x, const sm
     double bb = 0.0;
     TRY(smh_dot(ctx, SMH_F64, n, b, b, &bb));
     hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(1), 0, ctx->stream, w.S, bb, o->tol, o->rel,
-                       o->norm_form, o->maxiters, pc != SMH_PRECOND_NONE ? 1 : 0,
+                       o->norm_form, o->maxiters,
+                       (pc != SMH_PRECOND_NONE && o->zr_stop) ? 1 : 0,
                        o->reliable_delta);
     if (pc != SMH_PRECOND_NONE)
       hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream,
@@ -572,8 +573,9 @@ extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
   if (!ctx) return SMH_ERR_INVALID;
   SMH_CHECK(ctx, A && x && y && A->rowptr && (A->nnz == 0 || (A->colidx && A->vals)),
             SMH_ERR_INVALID, "smh_bsrmv: null argument");
-  SMH_CHECK(ctx, A->bdim == 1 || A->bdim == 3, SMH_ERR_UNSUPPORTED,
-            "smh_bsrmv: block size %u (1 or 3 supported)", A->bdim);
+  SMH_CHECK(ctx, A->bdim == 1 || A->bdim == 3 || (exact_order && A->bdim >= 1 && A->bdim <= 8),
+            SMH_ERR_UNSUPPORTED,
+            "smh_bsrmv: block size %u (1 or 3; 1..8 with exact_order)", A->bdim);
   SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
             "smh_bsrmv: SMH_F32X2 needs vals_lo");
   if (A->nrows == 0) return SMH_OK;

@@ -154,6 +154,13 @@ k_dot_partial(uint64_t n, const T *__restrict__ x, const T *__restrict__ y,
 }
 
 __global__ void __launch_bounds__(256)
+k_axpby(uint64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    y[i] = a * x[i] + b * y[i];
+}
+
+__global__ void __launch_bounds__(256)
 k_sum_partials(const double *__restrict__ part, int np, double *__restrict__ out) {
   __shared__ double s[4];
   double acc = 0.0;
@@ -289,5 +296,16 @@ extern "C" int smh_dot(smh_ctx *ctx, int32_t precision, uint64_t n, const void *
                               hipMemcpyDeviceToHost, ctx->stream));
   SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *result = ctx->host_pinned[0];
+  return SMH_OK;
+}
+
+extern "C" int smh_axpby(smh_ctx *ctx, uint64_t n, double a, const double *x, double b,
+                         double *y) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, n == 0 || (x && y), SMH_ERR_INVALID, "smh_axpby: null argument");
+  if (!n) return SMH_OK;
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n, 256, 4096)), dim3(256), 0, ctx->stream, n, a, x, b,
+                     y);
+  SMH_LAUNCH_CHECK(ctx);
   return SMH_OK;
 }

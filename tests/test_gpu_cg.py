@@ -1,0 +1,140 @@
+"""The device CG solver against the reference's program fixtures, the oracle
+and a certified FP64 residual."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import simit_amd as S
+from conftest import to_dev
+from simit_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+def test_program_cg_fixture(ctx, oracle, fixtures):
+    """test/program-cg-test.cpp: A = I - 0.01*(map f to springs), cg.sim's loop."""
+    from test_oracle import cg_program_matrix
+    fx = fixtures["program_cg"]
+    rp, ci, vals = cg_program_matrix(oracle, fx)
+    idx = ctx.build_index(3, [to_dev(np.array(fx["springs"], np.int32))])
+    grp, gci = idx.to_host()
+    np.testing.assert_array_equal(grp, rp)
+    b = to_dev(np.array(fx["b"]))
+    x = torch.zeros(3, dtype=torch.float64, device="cuda")
+    o = ctx.cg_options(maxiters=fx["maxiters"], tol=fx["tol"], rel=0, norm_form=1)
+    st = ctx.cg(idx.bsr(1, S.F64, to_dev(vals)), b, x, o)
+    np.testing.assert_allclose(x.cpu().numpy(), fx["expected_c"], atol=fx["abs_tol"], rtol=0)
+
+
+def test_program_isprings_fixture(ctx, oracle, fixtures):
+    """test/program-isprings.cpp:91-175: ten implicit steps, assembly + CG on the GPU."""
+    from test_oracle import isprings_setup
+    fx = fixtures["program_isprings"]
+    pts, sp = isprings_setup(fx)
+    d = {k: to_dev(v) for k, v in pts.items()}
+    d.update({k: to_dev(v) for k, v in sp.items()})
+    idx = ctx.build_index(3, [d["endpoints"]])
+    P = S.SpringsParams(fx["h"], fx["viscous"], (C.c_double * 3)(*fx["grav"]), 1.0, 0)
+    vals = torch.empty(idx.nnz * 9, dtype=torch.float64, device="cuda")
+    rhs = torch.empty(9, dtype=torch.float64, device="cuda")
+    A = idx.bsr(3, S.F64, vals)
+    o = ctx.cg_options(maxiters=fx["maxiters"], tol=fx["tol"], rel=0)
+    for _ in range(fx["steps"]):
+        ctx.assemble_springs(idx, P, d, d, S.F64, vals, None, rhs)
+        ctx.cg(A, rhs, d["v"], o)                 # vguess = points.v, v = cg(...)
+        ctx.axpby(fx["h"], d["v"], 1.0, d["x"])   # points.x = points.x + h*points.v
+    np.testing.assert_allclose(d["v"].cpu().numpy(), fx["expected_v"], atol=fx["abs_tol"], rtol=0)
+    np.testing.assert_allclose(d["x"].cpu().numpy(), fx["expected_x"], atol=fx["abs_tol"], rtol=0)
+
+
+def tet_system(ctx, n, seed=0, field="iid", std=0.5):
+    prob = meshes.tet_problem(n, seed=seed, field=field, std_log10=std, corr_len=3.0)
+    d = {"x": to_dev(prob["x"]), "c": to_dev(prob["c"]), "mu": to_dev(prob["mu"]),
+         "lam": to_dev(prob["lam"]), "endpoints": to_dev(prob["endpoints"])}
+    idx = ctx.build_index(prob["nverts"], [d["endpoints"]])
+    P = S.TetParams(1.0, 0.0, (C.c_double * 3)(*prob["load"]), 1.0)
+    v64 = torch.empty(idx.nnz * 9, dtype=torch.float64, device="cuda")
+    hi = torch.empty(idx.nnz * 9, dtype=torch.float32, device="cuda")
+    lo = torch.empty(idx.nnz * 9, dtype=torch.float32, device="cuda")
+    rhs = torch.empty(prob["nverts"] * 3, dtype=torch.float64, device="cuda")
+    ctx.assemble_tets(idx, P, d, d, S.F64, v64, None, rhs)
+    ctx.assemble_tets(idx, P, d, d, S.F32X2, hi, lo, rhs)
+    return prob, idx, v64, hi, lo, rhs, d
+
+
+@pytest.mark.parametrize("mode", ["plain64", "jacobi64", "block64", "mixed_block", "mixed_none",
+                                  "hilo64"])
+def test_tet_cg_modes_reach_1e8(ctx, oracle, mode):
+    prob, idx, v64, hi, lo, rhs, _ = tet_system(ctx, 10, seed=3)
+    rp, ci = idx.to_host()
+    n = prob["nverts"] * 3
+    pc = {"plain64": 0, "jacobi64": 1, "block64": 2, "mixed_block": 2, "mixed_none": 0,
+          "hilo64": 2}[mode]
+    mixed = mode.startswith("mixed")
+    A = idx.bsr(3, S.F32X2, hi, lo) if (mixed or mode == "hilo64") else idx.bsr(3, S.F64, v64)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    o = ctx.cg_options(maxiters=20000, tol=1e-8, rel=1, precond=pc, mixed=int(mixed))
+    st = ctx.cg(A, rhs, x, o)
+    print(mode, st.as_dict(), flush=True)
+    assert st.status == 0, st.as_dict()
+    assert st.true_rel <= 1e-8
+    # independent FP64 check of the residual on the oracle's matrix
+    ov, of = oracle.tet_assemble(oracle.tet_params(), prob, rp, ci)
+    xv = x.cpu().numpy()
+    r = of - oracle.bsrmv(rp, ci, ov, xv, 3)
+    assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(of)
+    # and the same solution as the oracle's CG
+    xo, _ = oracle.cg(rp, ci, ov, of, np.zeros(n), 3, oracle.cg_opts(maxiters=20000, tol=1e-10))
+    assert np.linalg.norm(xv - xo) <= 1e-5 * np.linalg.norm(xo)
+
+
+def test_cg_repeatable_bitwise(ctx):
+    _, idx, v64, hi, lo, rhs, _ = tet_system(ctx, 8, seed=1)
+    outs = []
+    for _ in range(2):
+        x = torch.zeros_like(rhs)
+        st = ctx.cg(idx.bsr(3, S.F32X2, hi, lo), rhs, x,
+                    ctx.cg_options(maxiters=5000, tol=1e-8, precond=2, mixed=1))
+        outs.append((x.clone(), st.iters))
+    assert torch.equal(outs[0][0], outs[1][0]) and outs[0][1] == outs[1][1]
+
+
+def test_hex_poisson_mixed(ctx, oracle):
+    prob = meshes.hex_problem(12, seed=2, corr_len=3.0)
+    d = {"dirichlet": to_dev(prob["dirichlet"]), "kappa": to_dev(prob["kappa"]),
+         "endpoints": to_dev(prob["endpoints"])}
+    idx = ctx.build_index(prob["nverts"], [d["endpoints"]])
+    hi = torch.empty(idx.nnz, dtype=torch.float32, device="cuda")
+    lo = torch.empty(idx.nnz, dtype=torch.float32, device="cuda")
+    rhs = torch.empty(prob["nverts"], dtype=torch.float64, device="cuda")
+    ctx.assemble_hex(idx, S.HexParams(prob["h"], 1.0, 1.0), d, d, S.F32X2, hi, lo, rhs)
+    x = torch.zeros_like(rhs)
+    st = ctx.cg(idx.bsr(1, S.F32X2, hi, lo), rhs, x,
+                ctx.cg_options(maxiters=5000, tol=1e-8, precond=1, mixed=1))
+    assert st.status == 0 and st.true_rel <= 1e-8, st.as_dict()
+    rp, ci = idx.to_host()
+    ov, of = oracle.hex_assemble(oracle.HexParams(prob["h"], 1.0, 1.0), prob, rp, ci)
+    r = of - oracle.bsrmv(rp, ci, ov, x.cpu().numpy(), 1)
+    assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(of)
+
+
+def test_maxiters_reports_not_converged(ctx):
+    _, idx, v64, hi, lo, rhs, _ = tet_system(ctx, 6)
+    x = torch.zeros_like(rhs)
+    st = ctx.cg(idx.bsr(3, S.F64, v64), rhs, x, ctx.cg_options(maxiters=3, tol=1e-12))
+    assert st.iters == 3 and st.status == S.ERR_NOT_CONVERGED
+
+
+def test_indefinite_matrix_breaks_down(ctx):
+    """p'Ap <= 0 is detected and reported, not iterated on."""
+    n = 4
+    ep = np.array([[0, 1], [1, 2], [2, 3]], np.int32)
+    idx = ctx.build_index(n, [to_dev(ep)])
+    rp, ci = idx.to_host()
+    vals = np.array([-1.0 if ci[l] == i else 0.0 for i in range(n) for l in range(rp[i], rp[i + 1])])
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    st = ctx.cg(idx.bsr(1, S.F64, to_dev(vals)), to_dev(np.ones(n)), x,
+                ctx.cg_options(maxiters=10, tol=1e-10))
+    assert st.status == S.ERR_BREAKDOWN
