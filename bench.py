@@ -40,6 +40,7 @@ def parse():
     ap.add_argument("--n", type=int, default=96, help="cubes per edge of the tet grid")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--plain-maxiters", type=int, default=40000)
+    ap.add_argument("--cg-maxiters", type=int, default=20000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
@@ -93,7 +94,7 @@ def main():
     lo = torch.empty_like(hi)
     rhs = torch.empty(ndof, dtype=torch.float64, device="cuda")
     A = idx.bsr(3, S.F32X2, hi, lo)
-    opts = ctx.cg_options(maxiters=20000, tol=1e-8, rel=1, precond=S.PRECOND_BLOCK_JACOBI,
+    opts = ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, precond=S.PRECOND_BLOCK_JACOBI,
                           mixed=1, check_every=16)
 
     def assemble():

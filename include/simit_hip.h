@@ -53,7 +53,8 @@ int smh_ctx_create(int device, smh_ctx **out);
 int smh_ctx_destroy(smh_ctx *ctx);
 /* Message of the last failed call on this context ("" if none). */
 const char *smh_ctx_last_error(const smh_ctx *ctx);
-/* Use an existing hipStream_t (NULL: the context's own stream). */
+/* Order all later work on an existing hipStream_t (NULL: HIP's default
+ * stream).  A new context runs on a stream of its own. */
 int smh_ctx_set_stream(smh_ctx *ctx, void *hip_stream);
 void *smh_ctx_get_stream(const smh_ctx *ctx);
 int smh_ctx_synchronize(smh_ctx *ctx);
@@ -192,6 +193,25 @@ int smh_assemble_hex_poisson(smh_ctx *ctx, const smh_index *idx,
                              int32_t precision, void *vals, void *vals_lo,
                              double *rhs);
 
+/* An edge set whose element function writes one per-edge block w_e into the
+ * four blocks of its two endpoints with signs s[a*2+b] (all +1:
+ * test/input/assembly/matrix_vv.sim, test/input/system/gemv_blocked.sim;
+ * +1,-1,-1,+1: test/input/program/cg.sim:13-18), followed by the affine
+ * system expression A = alpha*I + beta*A (cg.sim:57: I - 0.01*(map ...)).
+ * bdim 1, 2 or 3; w is device double[nedges*bdim*bdim]; no right-hand side. */
+typedef struct {
+  uint32_t nverts;
+  uint64_t nedges;
+  uint32_t bdim;
+  const int32_t *endpoints; /* device int32[nedges*2]                      */
+  const double *w;          /* device double[nedges*bdim*bdim]             */
+  double sign[4];
+  double alpha, beta;
+} smh_edge_blocks;
+int smh_assemble_edge_blocks(smh_ctx *ctx, const smh_index *idx,
+                             const smh_edge_blocks *e, int32_t precision,
+                             void *vals, void *vals_lo);
+
 /* ---- linear algebra --------------------------------------------------------
  * y = A*x: the blocked row loop of lower_indexexprs.cpp:630-667.  x and y
  * are float for SMH_F32 and double for SMH_F64 / SMH_F32X2.  exact_order=1
@@ -205,6 +225,11 @@ int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
  * precision SMH_F64: double inputs; SMH_F32: float inputs. */
 int smh_dot(smh_ctx *ctx, int32_t precision, uint64_t n, const void *x,
             const void *y, double *result);
+/* smh_bsrmv and smh_cg_solve cut a pattern into row tiles on first use and
+ * cache that plan per context, keyed by (rowptr pointer, nrows, nnz).  Call
+ * this before a rowptr buffer is freed or refilled with another pattern of
+ * the same size (smh_index_destroy does it for index-owned buffers). */
+int smh_bsr_forget(smh_ctx *ctx, const uint32_t *rowptr);
 /* y = a*x + b*y on double vectors: the dense vector statements of Simit
  * programs (e.g. points.x = points.x + h*points.v, isprings.sim:94; the
  * reference GPU path ran each as its own kernel, gpu_backend.cpp:1150-1215). */

@@ -95,7 +95,8 @@ def _c(a, dt):
 
 # ---- sparsity ----------------------------------------------------------------------
 def index_vev(nv, endpoint_sets, force_diag=False):
-    sets = [_c(e, np.int32).reshape(e.shape[0], -1) for e in endpoint_sets]
+    sets = [_c(e, np.int32) for e in endpoint_sets]
+    sets = [s if s.ndim == 2 else s.reshape(s.shape[0], -1) for s in sets]
     ns = len(sets)
     ne = np.array([s.shape[0] for s in sets], dtype=np.int64)
     card = np.array([s.shape[1] for s in sets], dtype=np.int32)

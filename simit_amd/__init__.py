@@ -87,6 +87,12 @@ class HexSet(C.Structure):
                 ("dirichlet", C.c_void_p), ("kappa", C.c_void_p)]
 
 
+class EdgeBlocks(C.Structure):
+    _fields_ = [("nverts", C.c_uint32), ("nedges", C.c_uint64), ("bdim", C.c_uint32),
+                ("endpoints", C.c_void_p), ("w", C.c_void_p), ("sign", C.c_double * 4),
+                ("alpha", C.c_double), ("beta", C.c_double)]
+
+
 class CGOptions(C.Structure):
     _fields_ = [("maxiters", C.c_int32), ("tol", C.c_double), ("rel", C.c_int32),
                 ("norm_form", C.c_int32), ("replace_every", C.c_int32),
@@ -141,10 +147,12 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
                                               i32, vp, vp, vp]),
         "smh_assemble_hex_poisson": (C.c_int, [vp, vp, C.POINTER(HexParams), C.POINTER(HexSet),
                                                i32, vp, vp, vp]),
+        "smh_assemble_edge_blocks": (C.c_int, [vp, vp, C.POINTER(EdgeBlocks), i32, vp, vp]),
         "smh_bsrmv": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, i32]),
         "smh_dot": (C.c_int, [vp, i32, u64, vp, vp, C.POINTER(C.c_double)]),
         "smh_block_jacobi": (C.c_int, [vp, C.POINTER(Bsr), vp]),
         "smh_axpby": (C.c_int, [vp, u64, C.c_double, vp, C.c_double, vp]),
+        "smh_bsr_forget": (C.c_int, [vp, vp]),
         "smh_cg_default_opts": (None, [C.POINTER(CGOptions)]),
         "smh_cg_solve": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, C.POINTER(CGOptions),
                                    C.POINTER(CGStats)]),
@@ -238,6 +246,14 @@ class Context:
                                                       precision, _ptr(vals), _ptr(vals_lo),
                                                       _ptr(rhs)))
 
+    def assemble_edge_blocks(self, idx, endpoints, w, bdim: int, sign=(1, 1, 1, 1),
+                             alpha: float = 0.0, beta: float = 1.0, precision: int = F64,
+                             vals=None, vals_lo=None):
+        e = EdgeBlocks(idx.nrows, int(endpoints.shape[0]), bdim, _ptr(endpoints), _ptr(w),
+                       (C.c_double * 4)(*sign), alpha, beta)
+        self._check(self.lib.smh_assemble_edge_blocks(self.h, idx.h, C.byref(e), precision,
+                                                      _ptr(vals), _ptr(vals_lo)))
+
     # ---- linear algebra -----------------------------------------------------
     def bsrmv(self, A: Bsr, x, y, exact_order: bool = False):
         self._check(self.lib.smh_bsrmv(self.h, C.byref(A), _ptr(x), _ptr(y), int(exact_order)))
@@ -313,8 +329,12 @@ class Index:
         self.ctx._check(self.ctx.lib.smh_index_locs(self.ctx.h, self.h, s, _ptr(out)))
 
     def bsr(self, bdim: int, precision: int, vals, vals_lo=None) -> Bsr:
-        return Bsr(self.nrows, bdim, self.nnz, self.rowptr_ptr, self.colidx_ptr, precision,
-                   _ptr(vals), _ptr(vals_lo))
+        b = Bsr(self.nrows, bdim, self.nnz, self.rowptr_ptr, self.colidx_ptr, precision,
+                _ptr(vals), _ptr(vals_lo))
+        # the struct holds raw pointers: keep the buffers (and the index)
+        # alive for as long as it is
+        b._keep = (vals, vals_lo, self)
+        return b
 
     def close(self):
         if self.h:

@@ -71,9 +71,10 @@ struct SpringOp {
     return true;
   }
   __device__ static bool diag_extra(const Args &, const State &, int, double *) { return false; }
-  __device__ static void rhs(const Args &, const State &st, int a, double *f) {
-    if (!st.free_[a]) return;
-    for (int q = 0; q < 3; ++q) f[q] += a == 0 ? st.fe0[q] : -st.fe0[q];
+  __device__ static bool rhs(const Args &, const State &st, int a, double *f) {
+    if (!st.free_[a]) return false;
+    for (int q = 0; q < 3; ++q) f[q] = a == 0 ? st.fe0[q] : -st.fe0[q];
+    return true;
   }
   // MDK = MD + K and f = fmg + fe, with MD/fmg from compute_mass_damping
   // (isprings.sim:19-36), for the lane that owns row i's diagonal block.
@@ -99,6 +100,9 @@ struct SpringOp {
     for (int q = 0; q < 3; ++q)
       f[q] = mass * A.v[3 * (int64_t)i + q] + A.p.h * mass * A.p.grav[q] + 0.0;
   }
+  __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
+  static constexpr bool kHasRhs = true;
+  static constexpr bool kExtra = false;
 };
 
 // apps/fem/fem_linear.sim:95-194
@@ -178,10 +182,11 @@ struct TetOp {
     for (int q = 0; q < 9; ++q) out[q] = (q % 4 == 0) ? mm : 0.0;
     return true;
   }
-  __device__ static void rhs(const Args &A, const State &st, int a, double *f) {
-    if (!st.free_[a]) return;
+  __device__ static bool rhs(const Args &A, const State &st, int a, double *f) {
+    if (!st.free_[a]) return false;
     const double wq = 0.25 * st.W;
-    for (int q = 0; q < 3; ++q) f[q] += wq * A.p.load[q];
+    for (int q = 0; q < 3; ++q) f[q] = wq * A.p.load[q];
+    return true;
   }
   __device__ static void finish_diag(const Args &A, uint32_t i, double *blk, double *) {
     if (A.p.fixed_diag == 0.0 || A.c[i] <= 0) return;
@@ -190,6 +195,9 @@ struct TetOp {
   __device__ static void isolated_rhs(const Args &, uint32_t, double *f) {
     f[0] = f[1] = f[2] = 0.0;
   }
+  __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
+  static constexpr bool kHasRhs = true;
+  static constexpr bool kExtra = true;
 };
 
 // scalar trilinear hex Poisson, corner a = dx + 2dy + 4dz
@@ -223,15 +231,63 @@ struct HexOp {
     return true;
   }
   __device__ static bool diag_extra(const Args &, const State &, int, double *) { return false; }
-  __device__ static void rhs(const Args &A, const State &st, int a, double *f) {
-    if (!st.free_[a]) return;
-    f[0] += A.p.source * (A.p.h * A.p.h * A.p.h) * 0.125;
+  __device__ static bool rhs(const Args &A, const State &st, int a, double *f) {
+    if (!st.free_[a]) return false;
+    f[0] = A.p.source * (A.p.h * A.p.h * A.p.h) * 0.125;
+    return true;
   }
   __device__ static void finish_diag(const Args &A, uint32_t i, double *blk, double *) {
     if (A.dir[i] <= 0) return;
     blk[0] = A.p.fixed_diag + blk[0];
   }
   __device__ static void isolated_rhs(const Args &, uint32_t, double *f) { f[0] = 0.0; }
+  __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
+  static constexpr bool kHasRhs = true;
+  static constexpr bool kExtra = false;
+};
+
+// one per-edge block written into the four endpoint blocks with signs, then
+// A = alpha*I + beta*A (test/input/program/cg.sim, assembly/matrix_vv.sim,
+// system/gemv_blocked.sim)
+struct EdgeArgs {
+  const int32_t *ep;
+  const double *w;
+  double sign[4];
+  double alpha, beta;
+  int32_t affine;
+};
+template <int BDIM>
+struct EdgeOp {
+  static constexpr int CARD = 2, BD = BDIM;
+  static constexpr bool kHasRhs = false;
+  static constexpr bool kExtra = false;
+  using Args = EdgeArgs;
+  struct State {
+    int32_t vid[2];
+    double w[BDIM * BDIM];
+  };
+  __device__ static void load(const Args &A, uint64_t e, State &st) {
+    st.vid[0] = A.ep[2 * e];
+    st.vid[1] = A.ep[2 * e + 1];
+    for (int q = 0; q < BDIM * BDIM; ++q) st.w[q] = A.w[e * BDIM * BDIM + q];
+  }
+  __device__ static bool block(const Args &A, const State &st, int a, int b, double *out) {
+    const double s = A.sign[2 * a + b];
+    for (int q = 0; q < BDIM * BDIM; ++q) out[q] = s * st.w[q];
+    return true;
+  }
+  __device__ static bool diag_extra(const Args &, const State &, int, double *) { return false; }
+  __device__ static bool rhs(const Args &, const State &, int, double *) { return false; }
+  __device__ static void finish_diag(const Args &, uint32_t, double *, double *) {}
+  __device__ static void isolated_rhs(const Args &, uint32_t, double *f) {
+    for (int q = 0; q < BDIM; ++q) f[q] = 0.0;
+  }
+  __device__ static void finish_block(const Args &A, uint32_t row, uint32_t col, double *blk) {
+    if (!A.affine) return;
+    for (int i = 0; i < BDIM; ++i)
+      for (int j = 0; j < BDIM; ++j)
+        blk[i * BDIM + j] = ((row == col && i == j) ? A.alpha : 0.0) + A.beta * blk[i * BDIM + j];
+  }
 };
 
 // ---- the pull kernel --------------------------------------------------------
@@ -252,17 +308,22 @@ template <class Op, int PREC>
 __global__ void __launch_bounds__(kWave)
 k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
            const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ colidx,
+           const uint32_t *__restrict__ wave_row,
            const uint32_t *__restrict__ ve_ptr, const uint32_t *__restrict__ ve_elem,
            void *__restrict__ vals, void *__restrict__ vals_lo, double *__restrict__ rhs) {
   constexpr int BD = Op::BD, BB = BD * BD, CARD = Op::CARD;
-  // the element states and, after the last chunk, the output staging area
-  constexpr size_t kStBytes = sizeof(typename Op::State) * kWave;
-  constexpr size_t kOutBytes = sizeof(double) * kWave * BB;
-  __shared__ __align__(16) unsigned char smem[kStBytes > kOutBytes ? kStBytes : kOutBytes];
-  typename Op::State *st = reinterpret_cast<typename Op::State *>(smem);
-  double *outbuf = reinterpret_cast<double *>(smem);
-  __shared__ uint32_t R[kWave + 1];
-  __shared__ uint32_t rfirst_s;
+  constexpr int NE = Op::kExtra ? kWave * BB : 1;
+  // per staged (row, element) pair: its CARD blocks of the row, the lumped
+  // extra on the diagonal and the right-hand side; after the last chunk the
+  // block area is reused to stage the coalesced output stores
+  __shared__ double C[kWave * CARD * BB];
+  __shared__ double E[NE];
+  __shared__ double F[kWave * BD];
+  __shared__ int32_t V[kWave * CARD];
+  __shared__ uint32_t M[kWave];        // bits 0..CARD-1 blocks, CARD extra, CARD+1 rhs
+  __shared__ uint32_t se[kWave];       // element of each staged pair
+  __shared__ uint32_t R[kWave + 1];    // rowptr window
+  __shared__ uint32_t VP[kWave + 2];   // ve_ptr window of the wave's rows
 
   const int lane = threadIdx.x;
   const uint64_t b0 = (uint64_t)blockIdx.x * kWave;
@@ -270,22 +331,14 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
   const bool valid = bk < nnz;
   const uint64_t blast = min(b0 + kWave, nnz) - 1;
 
-  // row of b0: last r with rowptr[r] <= b0
-  if (lane == 0) {
-    uint32_t lo = 0, hi = nrows;  // rowptr[lo] <= b0 < rowptr[hi]
-    while (hi - lo > 1) {
-      const uint32_t mid = lo + (hi - lo) / 2;
-      if ((uint64_t)rowptr[mid] <= b0) lo = mid; else hi = mid;
-    }
-    rfirst_s = lo;
-  }
-  __syncthreads();
-  const uint32_t rfirst = rfirst_s;
+  // row of b0 (last r with rowptr[r] <= b0), precomputed with the index
+  const uint32_t rfirst = wave_row[blockIdx.x];
   R[lane] = rowptr[min(rfirst + (uint32_t)lane, nrows)];
   if (lane == 0) R[kWave] = rowptr[min(rfirst + (uint32_t)kWave, nrows)];
   __syncthreads();
   uint32_t row;
-  if ((uint64_t)R[kWave] > blast) {
+  const bool window = (uint64_t)R[kWave] > blast;  // the wave's rows are rfirst .. rfirst+63
+  if (window) {
     uint32_t c = 0;
     for (int l = 1; l <= kWave; ++l) c += ((uint64_t)R[l] <= bk && rfirst + l <= nrows) ? 1u : 0u;
     row = rfirst + c;
@@ -298,58 +351,101 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
     }
     row = lo;
   }
-  // the row of the wave's last block bounds the incidence range
   uint32_t rlast = valid ? row : rfirst;
   for (int off = 32; off > 0; off >>= 1) rlast = max(rlast, (uint32_t)__shfl_xor((int)rlast, off, 64));
   const uint32_t col = valid ? colidx[bk] : 0xffffffffu;
   const bool is_diag = valid && col == row;
+  const uint32_t nwr = rlast - rfirst + 1;  // rows of the wave
+  const bool vp_lds = nwr <= (uint32_t)kWave;
+  if (vp_lds)
+    for (uint32_t l = lane; l <= nwr; l += kWave) VP[l] = ve_ptr[rfirst + l];
+  __syncthreads();
 
   double acc[BB];
   for (int q = 0; q < BB; ++q) acc[q] = 0.0;
   double f[BD];
   for (int q = 0; q < BD; ++q) f[q] = 0.0;
 
-  const uint32_t P0 = ve_ptr[rfirst], P1 = ve_ptr[rlast + 1];
+  const uint32_t P0 = vp_lds ? VP[0] : ve_ptr[rfirst];
+  const uint32_t P1 = vp_lds ? VP[nwr] : ve_ptr[rlast + 1];
   const uint32_t myb = valid ? ve_ptr[row] : 0, mye = valid ? ve_ptr[row + 1] : 0;
-  uint32_t prev_e = 0xffffffffu;
-  int occ = 0;
   for (uint32_t c0 = P0; c0 < P1; c0 += kWave) {
+    // phase A: lane l evaluates pair p = c0 + l: all blocks of its row
     const uint32_t p = c0 + lane;
-    if (p < P1) Op::load(A, ve_elem[p], st[lane]);
+    uint32_t mask = 0;
+    if (p < P1) {
+      const uint32_t e = ve_elem[p];
+      se[lane] = e;
+      // the row this pair belongs to: last r with ve_ptr[r] <= p
+      uint32_t pr = rfirst;
+      if (vp_lds) {
+        for (uint32_t l = 1; l < nwr; ++l) pr += (VP[l] <= p) ? 1u : 0u;
+      } else {
+        uint32_t lo = rfirst, hi = rlast + 1;
+        while (hi - lo > 1) {
+          const uint32_t mid = lo + (hi - lo) / 2;
+          if (ve_ptr[mid] <= p) lo = mid; else hi = mid;
+        }
+        pr = lo;
+      }
+      const uint32_t prb = vp_lds ? VP[pr - rfirst] : ve_ptr[pr];
+      // an element holding the row k times is listed k times in a row: the
+      // occ-th copy stands for the occ-th endpoint slot equal to the row
+      int occ = 0;
+      while (p - occ > prb && ve_elem[p - occ - 1] == e) ++occ;
+      typename Op::State st;
+      Op::load(A, e, st);
+      int a = -1, seen = 0;
+      for (int t = 0; t < CARD; ++t) {
+        V[lane * CARD + t] = st.vid[t];
+        if ((uint32_t)st.vid[t] == pr && a < 0) {
+          if (seen == occ) a = t;
+          ++seen;
+        }
+      }
+      // slot a is dispatched to compile-time copies so that the element
+      // state stays in registers (a dynamic index would spill it)
+#pragma unroll
+      for (int aa = 0; aa < CARD; ++aa) {
+        if (aa != a) continue;
+#pragma unroll
+        for (int b = 0; b < CARD; ++b)
+          if (Op::block(A, st, aa, b, C + (lane * CARD + b) * BB)) mask |= 1u << b;
+        if (Op::kExtra && Op::diag_extra(A, st, aa, E + (Op::kExtra ? lane * BB : 0)))
+          mask |= 1u << CARD;
+        if (Op::rhs(A, st, aa, F + lane * BD)) mask |= 1u << (CARD + 1);
+      }
+    }
+    M[lane] = mask;
     __syncthreads();
+    // phase B: every block lane adds its contributions in list order
     if (valid) {
       const uint32_t lo = max(myb, c0), hi = min(mye, c0 + kWave);
       for (uint32_t q = lo; q < hi; ++q) {
-        const typename Op::State &s = st[q - c0];
-        const uint32_t e = ve_elem[q];
-        occ = (e == prev_e) ? occ + 1 : 0;
-        prev_e = e;
-        // slot a: the occ-th endpoint equal to row
-        int a = -1, seen = 0;
-        for (int t = 0; t < CARD; ++t)
-          if ((uint32_t)s.vid[t] == row) {
-            if (seen == occ) { a = t; break; }
-            ++seen;
-          }
-        if (a < 0) continue;
-        double blk[BB];
+        const uint32_t l = q - c0;
+        const uint32_t m = M[l];
+        if (!m) continue;
         for (int b = 0; b < CARD; ++b)
-          if ((uint32_t)s.vid[b] == col && Op::block(A, s, a, b, blk))
-            for (int k = 0; k < BB; ++k) acc[k] += blk[k];
+          if ((m >> b & 1u) && (uint32_t)V[l * CARD + b] == col)
+            for (int k = 0; k < BB; ++k) acc[k] += C[(l * CARD + b) * BB + k];
         if (is_diag) {
-          if (Op::diag_extra(A, s, a, blk))
-            for (int k = 0; k < BB; ++k) acc[k] += blk[k];
-          Op::rhs(A, s, a, f);
+          if (Op::kExtra && (m >> CARD & 1u))
+            for (int k = 0; k < BB; ++k) acc[k] += E[Op::kExtra ? l * BB + k : 0];
+          if (m >> (CARD + 1) & 1u)
+            for (int k = 0; k < BD; ++k) f[k] += F[l * BD + k];
         }
       }
     }
     __syncthreads();
   }
+  if (valid) Op::finish_block(A, row, col, acc);
   if (is_diag) {
     Op::finish_diag(A, row, acc, f);
-    for (int q = 0; q < BD; ++q) rhs[(uint64_t)row * BD + q] = f[q];
+    if (rhs)
+      for (int q = 0; q < BD; ++q) rhs[(uint64_t)row * BD + q] = f[q];
   }
   // coalesced store through LDS
+  double *outbuf = C;
   for (int q = 0; q < BB; ++q) outbuf[lane * BB + q] = acc[q];
   __syncthreads();
   const uint64_t nval = (min(b0 + kWave, nnz) - b0) * BB;
@@ -364,7 +460,7 @@ template <class Op>
 __global__ void k_isolated(typename Op::Args A, uint32_t nrows, const uint32_t *__restrict__ rowptr,
                            double *__restrict__ rhs) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrows || rowptr[i] != rowptr[i + 1]) return;
+  if (!rhs || i >= nrows || rowptr[i] != rowptr[i + 1]) return;
   double f[Op::BD];
   Op::isolated_rhs(A, i, f);
   for (int q = 0; q < Op::BD; ++q) rhs[(uint64_t)i * Op::BD + q] = f[q];
@@ -374,7 +470,8 @@ template <class Op>
 int launch_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, const typename Op::Args &A,
                     int32_t prec, void *vals, void *vals_lo, double *rhs, const char *name) {
   using namespace smh;
-  SMH_CHECK(ctx, idx && vals && rhs, SMH_ERR_INVALID, "%s: null index, vals or rhs", name);
+  SMH_CHECK(ctx, idx && vals && (rhs || !Op::kHasRhs), SMH_ERR_INVALID,
+            "%s: null index, vals or rhs", name);
   SMH_CHECK(ctx, prec == SMH_F64 || prec == SMH_F32 || prec == SMH_F32X2, SMH_ERR_INVALID,
             "%s: bad precision %d", name, prec);
   SMH_CHECK(ctx, prec != SMH_F32X2 || vals_lo, SMH_ERR_INVALID, "%s: SMH_F32X2 needs vals_lo",
@@ -392,7 +489,8 @@ int launch_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, const
   if (nb) {
 #define SMH_ASM(P)                                                                          \
   hipLaunchKernelGGL((k_assemble<Op, P>), dim3(nb), dim3(kWave), 0, ctx->stream, A, nrows,  \
-                     idx->nnz, idx->rowptr, idx->colidx, idx->ve_ptr[set], idx->ve_elem[set], \
+                     idx->nnz, idx->rowptr, idx->colidx, idx->wave_row, idx->ve_ptr[set],      \
+                     idx->ve_elem[set],                                                        \
                      vals, vals_lo, rhs)
     if (prec == SMH_F64) SMH_ASM(SMH_F64);
     else if (prec == SMH_F32) SMH_ASM(SMH_F32);
@@ -447,4 +545,26 @@ extern "C" int smh_assemble_hex_poisson(smh_ctx *ctx, const smh_index *idx, cons
   HexArgs A{*p, s->endpoints, s->dirichlet, s->kappa};
   return launch_assembly<HexOp>(ctx, idx, s->endpoints, A, prec, vals, vals_lo, rhs,
                                 "smh_assemble_hex_poisson");
+}
+
+extern "C" int smh_assemble_edge_blocks(smh_ctx *ctx, const smh_index *idx, const smh_edge_blocks *e,
+                                        int32_t prec, void *vals, void *vals_lo) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, e && idx && e->nverts == idx->nrows, SMH_ERR_INVALID,
+            "smh_assemble_edge_blocks: null argument or vertex count mismatch");
+  SMH_CHECK(ctx, e->nedges == 0 || e->w, SMH_ERR_INVALID, "smh_assemble_edge_blocks: null w");
+  EdgeArgs A{e->endpoints, e->w, {e->sign[0], e->sign[1], e->sign[2], e->sign[3]}, e->alpha,
+             e->beta, !(e->alpha == 0.0 && e->beta == 1.0)};
+  switch (e->bdim) {
+    case 1:
+      return launch_assembly<EdgeOp<1>>(ctx, idx, e->endpoints, A, prec, vals, vals_lo, nullptr,
+                                        "smh_assemble_edge_blocks");
+    case 2:
+      return launch_assembly<EdgeOp<2>>(ctx, idx, e->endpoints, A, prec, vals, vals_lo, nullptr,
+                                        "smh_assemble_edge_blocks");
+    case 3:
+      return launch_assembly<EdgeOp<3>>(ctx, idx, e->endpoints, A, prec, vals, vals_lo, nullptr,
+                                        "smh_assemble_edge_blocks");
+  }
+  return smh::fail(ctx, SMH_ERR_UNSUPPORTED, "smh_assemble_edge_blocks: bdim %u (1..3)", e->bdim);
 }

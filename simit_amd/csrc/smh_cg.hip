@@ -2,16 +2,17 @@
 //
 // The CG loop is what Simit programs write inline (apps/springs/isprings.sim:
 // 77-93, apps/fem/fem_linear.sim:211-230, test/input/program/cg.sim:25-53,
-// apps/thermal/thermal.sim:75-130).  One iteration here is three kernels:
-//   K1  q = A p  (wavefront per block row) + FP64 partial of p'q; the last
-//       workgroup to finish reduces the partials in a fixed order and
-//       computes alpha = r'z / p'q on the device;
-//   K2  x += alpha p; r -= alpha q; z = M r; FP64 partials of z'r and r'r;
-//       the last workgroup computes beta and the stopping test;
+// apps/thermal/thermal.sim:75-130).  One iteration here is five launches on
+// one stream, with every scalar kept on the device:
+//   K1  q = A p (row tiles) + FP64 partial of p'q per workgroup;
+//   R1  one workgroup: p'q in a fixed order, alpha = r'z / p'q;
+//   K2  x += alpha p; r -= alpha q; z = M r; partials of z'r and r'r;
+//   R2  one workgroup: z'r, r'r, beta, the stopping test;
 //   K3  p = z + beta p.
-// Scalars never leave the device; the host only looks at the state every
-// check_every iterations.  Every reduction has a fixed shape, so the solve
-// is bitwise repeatable.
+// Cross-workgroup results only pass through kernel boundaries (no
+// device-wide fences inside a kernel), and every reduction has a fixed
+// shape, so a solve is bitwise repeatable.  The host reads the state every
+// check_every iterations; iterations enqueued past convergence exit at once.
 //
 // Mixed precision: the inner iteration streams the FP32 matrix (hi part)
 // with FP32 vectors and FP64 dots; x is accumulated in FP64.  Whenever the
@@ -21,32 +22,103 @@
 // certified by that FP64 residual.
 #include "smh_linalg.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 
-namespace {
-using namespace smh;
-
-// float values promoted to double (an FP32 matrix driven by FP64 vectors)
-constexpr int kF32D = 3;
-}  // namespace
-
 namespace smh {
-template <> struct Val<kF32D> {
-  using T = double;
-  __device__ static __forceinline__ double ld(const void *v, const void *, uint64_t i) {
-    return (double)__builtin_nontemporal_load(((const float *)v) + i);
+
+// ---- row-tile plans ---------------------------------------------------------------
+__global__ void k_rowmax(const uint32_t *__restrict__ rowptr, uint32_t nrows, unsigned *out) {
+  unsigned m = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nrows;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    m = max(m, rowptr[i + 1] - rowptr[i]);
+  for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+__global__ void k_tile_rows(const uint32_t *__restrict__ rowptr, uint32_t nrows, uint32_t tprime,
+                            uint32_t ntiles, uint32_t *__restrict__ tile_row) {
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > ntiles) return;
+  if (k == ntiles) {
+    tile_row[k] = nrows;
+    return;
   }
-};
+  const uint64_t key = (uint64_t)k * tprime;
+  uint32_t lo = 0, hi = nrows;  // first r in [0, nrows] with rowptr[r] >= key
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if ((uint64_t)rowptr[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  tile_row[k] = k == 0 ? 0 : lo;
+}
+
+int tile_plan(smh_ctx *ctx, const uint32_t *rowptr, uint32_t nrows, uint64_t nnz,
+              uint32_t tile_blocks, smh_tile_plan *out) {
+  for (auto &p : ctx->plans)
+    if (p.rowptr == rowptr && p.nrows == nrows && p.nnz == nnz && p.tile_blocks == tile_blocks) {
+      *out = p;
+      return SMH_OK;
+    }
+  unsigned *dmax = nullptr;
+  int rc = dmalloc(ctx, (void **)&dmax, sizeof(unsigned));
+  if (rc) return rc;
+  SMH_HIP(ctx, hipMemsetAsync(dmax, 0, sizeof(unsigned), ctx->stream));
+  if (nrows)
+    hipLaunchKernelGGL(k_rowmax, dim3(grid_for(nrows, 256, 2048)), dim3(256), 0, ctx->stream,
+                       rowptr, nrows, dmax);
+  unsigned maxrow = 0;
+  SMH_HIP(ctx, hipMemcpyAsync(&maxrow, dmax, sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  dfree(dmax);
+  smh_tile_plan p;
+  p.rowptr = rowptr;
+  p.nrows = nrows;
+  p.nnz = nnz;
+  p.maxrow = maxrow;
+  p.tile_blocks = tile_blocks;
+  if (maxrow <= tile_blocks / 2) {
+    p.tprime = tile_blocks - (maxrow ? maxrow : 1) + 1;
+    uint64_t nt = (nnz + p.tprime - 1) / p.tprime;
+    if (nt < 1) nt = 1;
+    SMH_CHECK(ctx, nt < 0xffffffffull, SMH_ERR_UNSUPPORTED, "tile plan: too many tiles");
+    p.ntiles = (uint32_t)nt;
+    rc = dmalloc(ctx, (void **)&p.tile_row, sizeof(uint32_t) * (p.ntiles + 1));
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(p.ntiles + 1, 256)), dim3(256), 0, ctx->stream,
+                       rowptr, nrows, p.tprime, p.ntiles, p.tile_row);
+    SMH_LAUNCH_CHECK(ctx);
+  }
+  ctx->plans.push_back(p);
+  *out = p;
+  return SMH_OK;
+}
+
+void forget_plans(smh_ctx *ctx, const uint32_t *rowptr) {
+  for (size_t i = 0; i < ctx->plans.size();) {
+    if (ctx->plans[i].rowptr == rowptr) {
+      hipStreamSynchronize(ctx->stream);
+      dfree(ctx->plans[i].tile_row);
+      ctx->plans.erase(ctx->plans.begin() + i);
+    } else {
+      ++i;
+    }
+  }
+}
+
 }  // namespace smh
 
 namespace {
+using namespace smh;
 
 // ---- plain SpMV ---------------------------------------------------------------
 #pragma clang fp contract(off)
 // thread per scalar row, ascending ij then bj, separate multiply and add:
-// bit-identical to the CPU loop nest.
+// bit-identical to the CPU loop nest.  Also the path for rows longer than
+// half a tile.
 __global__ void k_bsrmv_exact(uint32_t nrows, int bd, const uint32_t *__restrict__ rowptr,
                               const uint32_t *__restrict__ colidx, const double *__restrict__ vals,
                               const double *__restrict__ x, double *__restrict__ y) {
@@ -67,161 +139,123 @@ __global__ void k_bsrmv_exact(uint32_t nrows, int bd, const uint32_t *__restrict
 }
 #pragma clang fp contract(fast)
 
-template <int PREC>
-__global__ void __launch_bounds__(256)
-k_bsrmv3(BsrDev A, uint32_t per, const typename Val<PREC>::T *__restrict__ x,
-         typename Val<PREC>::T *__restrict__ y) {
-  const int lane = threadIdx.x & 63;
-  uint32_t r0, r1;
-  wave_rows(A.nrows, per, r0, r1);
-  for (uint32_t r = r0; r < r1; ++r) {
-    const auto s = row3<PREC>(A, x, r, lane);
-    if (lane == 0 || lane == 3 || lane == 6) y[(uint64_t)r * 3 + lane / 3] = s;
-  }
-}
-
-template <int PREC>
-__global__ void __launch_bounds__(256)
-k_bsrmv1(BsrDev A, uint32_t per, const typename Val<PREC>::T *__restrict__ x,
-         typename Val<PREC>::T *__restrict__ y) {
-  const int lane = threadIdx.x & 63, hl = lane & 31, half = lane >> 5;
-  uint32_t r0, r1;
-  wave_rows(A.nrows, per, r0, r1);
-  for (uint32_t r = r0; r < r1; r += 2) {
-    const uint32_t rr = r + half;
-    const bool act = rr < r1;
-    const auto s = row1<PREC>(A, x, rr, act, hl);
-    if (act && hl == 0) y[rr] = s;
-  }
-}
-
-struct Launch {
-  unsigned blocks;
-  uint32_t per;  // rows per wavefront (3x3) or per wavefront pair of rows (1x1)
+template <typename T>
+struct XPlain {
+  const T *x;
+  int bd;
+  __device__ T operator()(uint32_t j, int b) const { return x[(uint64_t)j * bd + b]; }
 };
-Launch spmv_launch(uint32_t nrows, int bd) {
-  const uint64_t waves_wanted = bd == 3 ? nrows : (nrows + 1) / 2;
-  uint64_t blocks = (waves_wanted + 3) / 4;
-  const uint64_t cap = 2048;  // 8 workgroups of 4 waves per CU on 256 CUs
-  if (blocks > cap) blocks = cap;
-  if (blocks >= 8) blocks &= ~7ull;
-  if (blocks < 1) blocks = 1;
-  const uint64_t waves = blocks * 4;
-  uint64_t per = (nrows + waves - 1) / waves;
-  if (bd == 1 && (per & 1)) ++per;
-  if (per < 1) per = 1;
-  return {(unsigned)blocks, (uint32_t)per};
+
+template <typename T, int BD>
+struct EpiStore {
+  T *y;
+  __device__ void operator()(uint32_t r, const T *s) {
+    for (int a = 0; a < BD; ++a) y[(uint64_t)r * BD + a] = s[a];
+  }
+};
+
+template <int PREC, int BD>
+__global__ void __launch_bounds__(TileShape<BD>::kThreads)
+k_bsrmv_tiles(TileDev A, const typename Val<PREC>::T *__restrict__ x,
+              typename Val<PREC>::T *__restrict__ y) {
+  using T = typename Val<PREC>::T;
+  __shared__ TileSmem<PREC, BD> sm;
+  uint32_t k0, k1;
+  wg_tiles(A, k0, k1);
+  XPlain<T> xg{x, BD};
+  EpiStore<T, BD> epi{y};
+  spmv_tiles<PREC, BD>(A, k0, k1, sm, xg, epi);
 }
 
-// ---- CG device state -------------------------------------------------------------
+// ---- CG device state ------------------------------------------------------------
 struct CGState {
-  double rz, rz_old, pq, alpha, beta, rr, rr_true, thr, rr_ref, bb, delta2;
-  int iter, done, status, want_ru, claimed, maxiters, precond_stop;
+  double rz, pq, alpha, beta, rr, rr_true, thr, rr_ref, bb, delta2;
+  int iter, done, status, want_ru, claimed, maxiters, precond_stop, mixed;
 };
-
-__device__ bool last_block(unsigned *counter) {
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
-// fixed-order sum of np partials by one 256-thread block; valid on all threads
-__device__ double sum_partials(const double *part, int np, double *s4) {
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += 256) acc += __builtin_nontemporal_load(part + i);
-  acc = wave_sum(acc);
-  if ((threadIdx.x & 63) == 0) s4[threadIdx.x >> 6] = acc;
-  __syncthreads();
-  const double t = ((s4[0] + s4[1]) + s4[2]) + s4[3];
-  __syncthreads();
-  return t;
-}
 
 __device__ __forceinline__ bool finite_pos(double v) { return v > 0.0 && v < INFINITY; }
 
-// K1: q = A p, p'q, alpha
-template <int IP, int BD>
-__global__ void __launch_bounds__(256)
-k_cg_spmv(BsrDev A, uint32_t per, const typename Val<IP>::T *__restrict__ p,
-          typename Val<IP>::T *__restrict__ q, CGState *S, double *part, unsigned *cnt) {
-  __shared__ double s4[4];
-  if (S->done) return;
-  const int lane = threadIdx.x & 63;
-  uint32_t r0, r1;
-  wave_rows(A.nrows, per, r0, r1);
-  double d = 0.0;
-  if (BD == 3) {
-    for (uint32_t r = r0; r < r1; ++r) {
-      const auto s = row3<IP>(A, p, r, lane);
-      if (lane == 0 || lane == 3 || lane == 6) {
-        const uint64_t o = (uint64_t)r * 3 + lane / 3;
-        q[o] = s;
-        d += (double)s * (double)p[o];
-      }
-    }
-  } else {
-    const int hl = lane & 31, half = lane >> 5;
-    for (uint32_t r = r0; r < r1; r += 2) {
-      const uint32_t rr = r + half;
-      const bool act = rr < r1;
-      const auto s = row1<IP>(A, p, rr, act, hl);
-      if (act && hl == 0) {
-        q[rr] = s;
-        d += (double)s * (double)p[rr];
-      }
-    }
-  }
-  const double t = block_sum256(d, s4);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
-  if (last_block(cnt)) {
-    const double pq = sum_partials(part, gridDim.x, s4);
-    if (threadIdx.x == 0) {
-      S->pq = pq;
-      if (!finite_pos(pq)) {
-        S->status = SMH_ERR_BREAKDOWN;
-        S->done = 1;
-      } else {
-        S->alpha = S->rz / pq;
-      }
-      *cnt = 0;
-    }
-  }
+// fixed-order sum of np values (stride st) by one 256-thread workgroup
+__device__ double wg_sum(const double *part, int np, int st, double *s4) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) acc += part[(size_t)i * st];
+  return block_sum256(acc, s4);
 }
 
-__device__ __forceinline__ void apply_precond(int pc, int BD, const double *r, double *z,
-                                              const void *dinv, bool dinv_f32, uint64_t i) {
+template <int BD>
+__device__ __forceinline__ void apply_precond(int pc, const double *r, double *z,
+                                              const void *dinv, bool f32, uint64_t i) {
   if (pc == SMH_PRECOND_NONE) {
     for (int a = 0; a < BD; ++a) z[a] = r[a];
   } else if (BD == 1 || pc == SMH_PRECOND_JACOBI) {
     for (int a = 0; a < BD; ++a)
-      z[a] = r[a] * (dinv_f32 ? (double)((const float *)dinv)[i * BD + a]
-                              : ((const double *)dinv)[i * BD + a]);
+      z[a] = r[a] * (f32 ? (double)((const float *)dinv)[i * BD + a]
+                         : ((const double *)dinv)[i * BD + a]);
   } else {
     for (int a = 0; a < 3; ++a) {
       double s = 0.0;
       for (int b = 0; b < 3; ++b)
-        s += (dinv_f32 ? (double)((const float *)dinv)[i * 9 + 3 * a + b]
-                       : ((const double *)dinv)[i * 9 + 3 * a + b]) * r[b];
+        s += (f32 ? (double)((const float *)dinv)[i * 9 + 3 * a + b]
+                  : ((const double *)dinv)[i * 9 + 3 * a + b]) * r[b];
       z[a] = s;
     }
   }
 }
 
-// stopping rule of the reference loops: r'r > thr (and z'r > thr for PCG)
-__device__ __forceinline__ bool stop_test(const CGState *S, double rr, double rz) {
-  return !(rr > S->thr) || (S->precond_stop && !(rz > S->thr));
+// K1: q = A p and the workgroup partial of p'q
+template <typename VT>
+struct EpiPq {
+  VT *q;
+  const VT *p;
+  double d;
+  int bd;
+  __device__ void operator()(uint32_t r, const VT *s) {
+    for (int a = 0; a < bd; ++a) {
+      const uint64_t o = (uint64_t)r * bd + a;
+      q[o] = s[a];
+      d += (double)s[a] * (double)p[o];
+    }
+  }
+};
+
+template <int IP, int BD>
+__global__ void __launch_bounds__(TileShape<BD>::kThreads)
+k_cg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP>::T *__restrict__ q,
+          const CGState *S, double *part) {
+  using VT = typename Val<IP>::T;
+  __shared__ TileSmem<IP, BD> sm;
+  if (S->done) return;
+  uint32_t k0, k1;
+  wg_tiles(A, k0, k1);
+  XPlain<VT> xg{p, BD};
+  EpiPq<VT> epi{q, p, 0.0, BD};
+  spmv_tiles<IP, BD>(A, k0, k1, sm, xg, epi);
+  const double t = block_sum<TileShape<BD>::kWaves>(epi.d, sm.red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// K2: x += alpha p; r -= alpha q; z = M r; z'r, r'r; beta; stopping test
+// R1: alpha
+__global__ void __launch_bounds__(256) k_cg_r1(CGState *S, const double *part, int np) {
+  __shared__ double s4[4];
+  if (S->done) return;
+  const double pq = wg_sum(part, np, 1, s4);
+  if (threadIdx.x == 0) {
+    S->pq = pq;
+    if (!finite_pos(pq)) {
+      S->status = SMH_ERR_BREAKDOWN;
+      S->done = 1;
+    } else {
+      S->alpha = S->rz / pq;
+    }
+  }
+}
+
+// K2: x += alpha p; r -= alpha q; z = M r; partials of z'r and r'r
 template <typename VT, int BD>
 __global__ void __launch_bounds__(256)
 k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT *__restrict__ r,
-            VT *__restrict__ z, const VT *__restrict__ p, const VT *__restrict__ q, CGState *S,
-            double *part, unsigned *cnt, int mixed) {
+            VT *__restrict__ z, const VT *__restrict__ p, const VT *__restrict__ q,
+            const CGState *S, double *part) {
   __shared__ double s4[4];
   if (S->done) return;
   const double alpha = S->alpha;
@@ -231,13 +265,12 @@ k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT
     double rv[BD], zv[BD];
     for (int a = 0; a < BD; ++a) {
       const uint64_t o = i * BD + a;
-      const double pv = (double)p[o];
-      x[o] = x[o] + alpha * pv;
+      x[o] = x[o] + alpha * (double)p[o];
       const VT rn = (VT)((double)r[o] - alpha * (double)q[o]);
       r[o] = rn;
       rv[a] = (double)rn;
     }
-    apply_precond(pc, BD, rv, zv, dinv, sizeof(VT) == 4, i);
+    apply_precond<BD>(pc, rv, zv, dinv, sizeof(VT) == 4, i);
     for (int a = 0; a < BD; ++a) {
       const VT zz = (VT)zv[a];
       z[i * BD + a] = zz;
@@ -251,41 +284,36 @@ k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT
     part[2 * blockIdx.x] = t1;
     part[2 * blockIdx.x + 1] = t2;
   }
-  if (last_block(cnt)) {
-    double a1 = 0.0, a2 = 0.0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += 256) {
-      a1 += __builtin_nontemporal_load(part + 2 * k);
-      a2 += __builtin_nontemporal_load(part + 2 * k + 1);
+}
+
+// stopping rule of the reference loops: r'r > thr (and z'r > thr for PCG)
+__device__ __forceinline__ bool stop_test(const CGState *S, double rr, double rz) {
+  return !(rr > S->thr) || (S->precond_stop && !(rz > S->thr));
+}
+
+// R2: z'r, r'r, beta and the stopping test
+__global__ void __launch_bounds__(256) k_cg_r2(CGState *S, const double *part, int np) {
+  __shared__ double s4[4];
+  if (S->done) return;
+  const double rz = wg_sum(part, np, 2, s4);
+  const double rr = wg_sum(part + 1, np, 2, s4);
+  if (threadIdx.x == 0) {
+    S->iter += 1;
+    S->beta = rz / S->rz;
+    S->rz = rz;
+    S->rr = rr;
+    if (!(rr < INFINITY) || !(rz == rz)) {
+      S->status = SMH_ERR_BREAKDOWN;
+      S->done = 1;
+    } else if (stop_test(S, rr, rz)) {
+      S->claimed = 1;  // verified against the FP64 residual by the host
+      S->done = 1;
+    } else if (S->iter >= S->maxiters) {
+      S->done = 1;
+    } else if (S->mixed && rr < S->delta2 * S->rr_ref) {
+      S->want_ru = 1;
     }
-    a1 = wave_sum(a1);
-    if ((threadIdx.x & 63) == 0) s4[threadIdx.x >> 6] = a1;
-    __syncthreads();
-    const double rz = ((s4[0] + s4[1]) + s4[2]) + s4[3];
-    __syncthreads();
-    a2 = wave_sum(a2);
-    if ((threadIdx.x & 63) == 0) s4[threadIdx.x >> 6] = a2;
-    __syncthreads();
-    const double rr = ((s4[0] + s4[1]) + s4[2]) + s4[3];
-    if (threadIdx.x == 0) {
-      S->iter += 1;
-      S->rz_old = S->rz;
-      S->beta = rz / S->rz;
-      S->rz = rz;
-      S->rr = rr;
-      if (!(rr < INFINITY) || !(rz == rz)) {
-        S->status = SMH_ERR_BREAKDOWN;
-        S->done = 1;
-      } else if (stop_test(S, rr, rz)) {
-        S->claimed = 1;  // verified against the FP64 residual by the host
-        S->done = 1;
-      } else if (S->iter >= S->maxiters) {
-        S->done = 1;
-      } else if (mixed && rr < S->delta2 * S->rr_ref) {
-        S->want_ru = 1;
-      }
-      if (rr > S->rr_ref) S->rr_ref = rr;
-      *cnt = 0;
-    }
+    if (rr > S->rr_ref) S->rr_ref = rr;
   }
 }
 
@@ -300,91 +328,76 @@ k_cg_pupd(uint64_t n, VT *__restrict__ p, const VT *__restrict__ z, const CGStat
     p[i] = (VT)((double)z[i] + beta * (double)p[i]);
 }
 
-// r = b - A x in FP64 (A in precision RP), stored as VT; partial ||r||^2
-template <int RP, int BD, typename VT>
-__global__ void __launch_bounds__(256)
-k_residual(BsrDev A, uint32_t per, const double *__restrict__ b, const double *__restrict__ x,
-           VT *__restrict__ r, CGState *S, double *part, unsigned *cnt) {
-  __shared__ double s4[4];
-  const int lane = threadIdx.x & 63;
-  uint32_t r0, r1;
-  wave_rows(A.nrows, per, r0, r1);
-  double d = 0.0;
-  if (BD == 3) {
-    for (uint32_t rw = r0; rw < r1; ++rw) {
-      const double s = row3<RP>(A, x, rw, lane);
-      if (lane == 0 || lane == 3 || lane == 6) {
-        const uint64_t o = (uint64_t)rw * 3 + lane / 3;
-        const double rv = b[o] - s;
-        r[o] = (VT)rv;
-        d += rv * rv;
-      }
-    }
-  } else {
-    const int hl = lane & 31, half = lane >> 5;
-    for (uint32_t rw = r0; rw < r1; rw += 2) {
-      const uint32_t rr = rw + half;
-      const bool act = rr < r1;
-      const double s = row1<RP>(A, x, rr, act, hl);
-      if (act && hl == 0) {
-        const double rv = b[rr] - s;
-        r[rr] = (VT)rv;
-        d += rv * rv;
-      }
-    }
-  }
-  const double t = block_sum256(d, s4);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
-  if (last_block(cnt)) {
-    const double rr = sum_partials(part, gridDim.x, s4);
-    if (threadIdx.x == 0) {
-      S->rr_true = rr;
-      *cnt = 0;
-    }
-  }
-}
-
-// after a residual replacement: z = M r, z'r, r'r; reset the update reference;
-// init=1 also sets p = z (start of the solve)
+// Residual replacement: r = b - A x in FP64 (A in precision RP), stored as
+// VT, z = M r, and (init) p = z; workgroup partials of z'r and ||r||^2.
 template <typename VT, int BD>
-__global__ void __launch_bounds__(256)
-k_cg_restart(uint32_t nrows, int pc, const void *dinv, const VT *__restrict__ r,
-             VT *__restrict__ z, VT *__restrict__ p, int init, CGState *S, double *part,
-             unsigned *cnt) {
-  __shared__ double s4[4];
-  double drz = 0.0;
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nrows;
-       i += (uint64_t)gridDim.x * 256) {
+struct EpiResidual {
+  const double *b;
+  VT *r, *z, *p;
+  const void *dinv;
+  int pc, init;
+  double drz, drr;
+  __device__ void operator()(uint32_t row, const double *s) {
     double rv[BD], zv[BD];
-    for (int a = 0; a < BD; ++a) rv[a] = (double)r[i * BD + a];
-    apply_precond(pc, BD, rv, zv, dinv, sizeof(VT) == 4, i);
     for (int a = 0; a < BD; ++a) {
+      const uint64_t o = (uint64_t)row * BD + a;
+      const double v = b[o] - s[a];
+      drr += v * v;
+      r[o] = (VT)v;
+      rv[a] = (double)(VT)v;
+    }
+    apply_precond<BD>(pc, rv, zv, dinv, sizeof(VT) == 4, row);
+    for (int a = 0; a < BD; ++a) {
+      const uint64_t o = (uint64_t)row * BD + a;
       const VT zz = (VT)zv[a];
-      z[i * BD + a] = zz;
-      if (init) p[i * BD + a] = zz;
+      z[o] = zz;
+      if (init) p[o] = zz;
       drz += (double)zz * rv[a];
     }
   }
-  const double t = block_sum256(drz, s4);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
-  if (last_block(cnt)) {
-    const double rz = sum_partials(part, gridDim.x, s4);
-    if (threadIdx.x == 0) {
-      S->rz = rz;
-      S->rr = S->rr_true;
-      S->rr_ref = S->rr_true;
-      S->want_ru = 0;
-      S->claimed = 0;
-      if (!(S->rr_true < INFINITY)) {
-        S->status = SMH_ERR_BREAKDOWN;
-        S->done = 1;
-      } else if (stop_test(S, S->rr_true, rz)) {
-        S->done = 1;
-      } else if (S->iter < S->maxiters && S->status == SMH_OK) {
-        S->done = 0;
-      }
-      *cnt = 0;
-    }
+};
+
+template <int RP, int BD, typename VT>
+__global__ void __launch_bounds__(TileShape<BD>::kThreads)
+k_residual(TileDev A, const double *__restrict__ b, const double *__restrict__ x,
+           EpiResidual<VT, BD> epi, double *part) {
+  __shared__ TileSmem<RP, BD> sm;
+  uint32_t k0, k1;
+  wg_tiles(A, k0, k1);
+  XPlain<double> xg{x, BD};
+  epi.drz = 0.0;
+  epi.drr = 0.0;
+  spmv_tiles<RP, BD>(A, k0, k1, sm, xg, epi);
+  const double t1 = block_sum<TileShape<BD>::kWaves>(epi.drz, sm.red);
+  const double t2 = block_sum<TileShape<BD>::kWaves>(epi.drr, sm.red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t1;
+    part[2 * blockIdx.x + 1] = t2;
+  }
+}
+
+// after a replacement: z'r, the true ||r||^2, the stopping test; the search
+// direction is kept (reliable update) or set to z (start of the solve)
+__global__ void __launch_bounds__(256) k_cg_rres(CGState *S, const double *part, int np,
+                                                 int certify_only) {
+  __shared__ double s4[4];
+  const double rz = wg_sum(part, np, 2, s4);
+  const double rr = wg_sum(part + 1, np, 2, s4);
+  if (threadIdx.x) return;
+  S->rr_true = rr;
+  if (certify_only) return;
+  S->rz = rz;
+  S->rr = rr;
+  S->rr_ref = rr;
+  S->want_ru = 0;
+  S->claimed = 0;
+  if (!(rr < INFINITY) || !(rz == rz)) {
+    S->status = SMH_ERR_BREAKDOWN;
+    S->done = 1;
+  } else if (stop_test(S, rr, rz)) {
+    S->done = 1;
+  } else if (S->status == SMH_OK) {
+    S->done = S->iter >= S->maxiters ? 1 : 0;
   }
 }
 
@@ -392,15 +405,22 @@ k_cg_restart(uint32_t nrows, int pc, const void *dinv, const VT *__restrict__ r,
 // inv3 of runtime.cpp:135-160), from the matrix in its best precision,
 // plus an optional learned symmetric correction per block.
 template <int RP, typename OT>
-__global__ void k_dinv(BsrDev A, int bd, int pc, const float *learned, OT *out) {
+__global__ void k_dinv(uint32_t nrows, const uint32_t *__restrict__ rowptr,
+                       const uint32_t *__restrict__ colidx, const void *vals, const void *vals_lo,
+                       int bd, int pc, const float *learned, OT *out) {
+  using S = typename Val<RP>::S;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= A.nrows) return;
+  if (i >= nrows) return;
   const int bb = bd * bd;
   double d[9];
   for (int q = 0; q < bb; ++q) d[q] = 0.0;
-  for (uint32_t ij = A.rowptr[i]; ij < A.rowptr[i + 1]; ++ij)
-    if (A.colidx[ij] == i) {
-      for (int q = 0; q < bb; ++q) d[q] = Val<RP>::ld(A.vals, A.vals_lo, (uint64_t)ij * bb + q);
+  for (uint32_t ij = rowptr[i]; ij < rowptr[i + 1]; ++ij)
+    if (colidx[ij] == i) {
+      for (int q = 0; q < bb; ++q) {
+        const uint64_t o = (uint64_t)ij * bb + q;
+        d[q] = (double)((const S *)vals)[o];
+        if (Val<RP>::kLo) d[q] += (double)((const float *)vals_lo)[o];
+      }
       break;
     }
   if (bd == 1 || pc == SMH_PRECOND_JACOBI) {
@@ -426,9 +446,10 @@ __global__ void k_dinv(BsrDev A, int bd, int pc, const float *learned, OT *out) 
   }
 }
 
-__global__ void k_cg_init_state(CGState *S, double bb, double tol, int rel, int norm_form,
-                                int maxiters, int precond_stop, double delta) {
+__global__ void k_cg_init_state(CGState *S, double tol, int rel, int norm_form, int maxiters,
+                                int precond_stop, double delta, int mixed, const double *bbp) {
   if (threadIdx.x || blockIdx.x) return;
+  const double bb = *bbp;
   memset(S, 0, sizeof(CGState));
   S->bb = bb;
   double thr = rel ? tol * tol * bb : tol;
@@ -437,11 +458,35 @@ __global__ void k_cg_init_state(CGState *S, double bb, double tol, int rel, int 
   S->maxiters = maxiters;
   S->precond_stop = precond_stop;
   S->delta2 = delta * delta;
+  S->mixed = mixed;
 }
 
-__global__ void k_fill_zero(double *p, uint64_t n) {
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256)
-    p[i] = 0.0;
+__global__ void __launch_bounds__(256) k_dot_wg(uint64_t n, const double *__restrict__ x,
+                                                double *part) {
+  __shared__ double s4[4];
+  double acc = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    acc += x[i] * x[i];
+  const double t = block_sum256(acc, s4);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(256) k_sum_to(const double *part, int np, double *out) {
+  __shared__ double s4[4];
+  const double t = wg_sum(part, np, 1, s4);
+  if (threadIdx.x == 0) *out = t;
+}
+
+TileDev tile_dev(const smh_bsr *A, const smh_tile_plan *p, unsigned *grid) {
+  // persistent workgroups: up to 10 per CU on 256 CUs
+  uint64_t g = std::min<uint64_t>(p->ntiles, 2560);
+  if (g >= 8) g &= ~7ull;
+  if (g < 1) g = 1;
+  *grid = (unsigned)g;
+  TileDev d{A->nrows, A->nnz, A->rowptr, A->colidx, A->vals, A->vals_lo, p->tile_row, p->ntiles,
+            (uint32_t)((p->ntiles + g - 1) / g)};
+  return d;
 }
 
 // ---- host driver ---------------------------------------------------------------
@@ -449,31 +494,44 @@ struct Work {
   void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr;
   CGState *S = nullptr;
   double *part = nullptr;
-  unsigned *cnt = nullptr;
   void release() {
-    dfree(r); dfree(z); dfree(p); dfree(q); dfree(dinv); dfree(S); dfree(part); dfree(cnt);
+    dfree(r); dfree(z); dfree(p); dfree(q); dfree(dinv); dfree(S); dfree(part);
   }
 };
 
+constexpr int kPartSlots = 8192;
+
 template <int IP, int RP, int BD>
-int cg_run(smh_ctx *ctx, const smh_bsr *Am, const double *b, double *x, const smh_cg_opts *o,
-           smh_cg_stats *st) {
+int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh_tile_plan *rplan,
+           const double *b, double *x, const smh_cg_opts *o, smh_cg_stats *st) {
   using VT = typename Val<IP>::T;
   const uint32_t nrows = Am->nrows;
   const uint64_t n = (uint64_t)nrows * BD;
-  BsrDev Ain{nrows, Am->rowptr, Am->colidx, Am->vals, Am->vals_lo};
-  BsrDev Ares = Ain;
-  const Launch L = spmv_launch(nrows, BD);
+  unsigned gt = 0, grt = 0;
+  const TileDev T = tile_dev(Am, plan, &gt);
+  const TileDev TR = tile_dev(Am, rplan, &grt);
   const unsigned nbe = grid_for(nrows, 256, 2048);
+  const unsigned nbv = grid_for(n, 256, 2048);
   const int pc = o->precond;
-  const int mixed = o->mixed;
+  const int check = o->check_every > 0 ? o->check_every : 16;
   Work w;
   int rc = SMH_OK;
   auto T0 = std::chrono::steady_clock::now();
+  CGState hs{};
+  int nru = 0, last_rep = 0;
+  EpiResidual<VT, BD> epi{};
 #define TRY(x)            \
   do {                    \
     rc = (x);             \
     if (rc) goto done;    \
+  } while (0)
+#define LCHECK()                                                                  \
+  do {                                                                            \
+    hipError_t e_ = hipGetLastError();                                            \
+    if (e_ != hipSuccess) {                                                       \
+      rc = fail(ctx, SMH_ERR_HIP, "CG launch failed: %s", hipGetErrorString(e_)); \
+      goto done;                                                                  \
+    }                                                                             \
   } while (0)
   TRY(dmalloc(ctx, &w.r, sizeof(VT) * n));
   TRY(dmalloc(ctx, &w.z, sizeof(VT) * n));
@@ -481,92 +539,101 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const double *b, double *x, const sm
   TRY(dmalloc(ctx, &w.q, sizeof(VT) * n));
   if (pc != SMH_PRECOND_NONE) TRY(dmalloc(ctx, &w.dinv, sizeof(VT) * (uint64_t)nrows * BD * BD));
   TRY(dmalloc(ctx, (void **)&w.S, sizeof(CGState)));
-  TRY(dmalloc(ctx, (void **)&w.part, sizeof(double) * 2 * 4096));
-  TRY(dmalloc(ctx, (void **)&w.cnt, sizeof(unsigned) * 8));
-  hipMemsetAsync(w.cnt, 0, sizeof(unsigned) * 8, ctx->stream);
-  {
-    double bb = 0.0;
-    TRY(smh_dot(ctx, SMH_F64, n, b, b, &bb));
-    hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(1), 0, ctx->stream, w.S, bb, o->tol, o->rel,
-                       o->norm_form, o->maxiters,
-                       (pc != SMH_PRECOND_NONE && o->zr_stop) ? 1 : 0,
-                       o->reliable_delta);
-    if (pc != SMH_PRECOND_NONE)
-      hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream,
-                         Ares, BD, pc, o->learned, (VT *)w.dinv);
-    // r = b - A x, z = M r, p = z
-    hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(L.blocks), dim3(256), 0, ctx->stream, Ares,
-                       L.per, b, (const double *)x, (VT *)w.r, w.S, w.part, w.cnt);
-    hipLaunchKernelGGL((k_cg_restart<VT, BD>), dim3(nbe), dim3(256), 0, ctx->stream, nrows, pc,
-                       (const void *)w.dinv, (const VT *)w.r, (VT *)w.z, (VT *)w.p, 1, w.S,
-                       w.part, w.cnt + 1);
-    SMH_LAUNCH_CHECK(ctx);
-    CGState hs{};
-    const int check = o->check_every > 0 ? o->check_every : 16;
-    int launched = 0, nru = 0;
-    for (;;) {
-      SMH_HIP(ctx, hipMemcpyAsync(&hs, w.S, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
-      SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-      const bool need_ru =
-          hs.status == SMH_OK && (hs.want_ru || hs.claimed ||
-                                  (o->replace_every > 0 && hs.iter > 0 &&
-                                   hs.iter / o->replace_every > nru &&
-                                   hs.iter % o->replace_every == 0));
-      if (need_ru) {
-        ++nru;
-        hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(L.blocks), dim3(256), 0, ctx->stream,
-                           Ares, L.per, b, (const double *)x, (VT *)w.r, w.S, w.part, w.cnt);
-        hipLaunchKernelGGL((k_cg_restart<VT, BD>), dim3(nbe), dim3(256), 0, ctx->stream, nrows,
-                           pc, (const void *)w.dinv, (const VT *)w.r, (VT *)w.z, (VT *)w.p, 0,
-                           w.S, w.part, w.cnt + 1);
-        SMH_LAUNCH_CHECK(ctx);
-        continue;
-      }
-      if (hs.done || hs.status != SMH_OK) break;
-      const int batch = o->replace_every > 0
-                            ? std::min(check, o->replace_every - hs.iter % o->replace_every)
-                            : check;
-      for (int k = 0; k < batch; ++k) {
-        hipLaunchKernelGGL((k_cg_spmv<IP, BD>), dim3(L.blocks), dim3(256), 0, ctx->stream, Ain,
-                           L.per, (const VT *)w.p, (VT *)w.q, w.S, w.part, w.cnt + 2);
-        hipLaunchKernelGGL((k_cg_update<VT, BD>), dim3(nbe), dim3(256), 0, ctx->stream, nrows, pc,
-                           (const void *)w.dinv, x, (VT *)w.r, (VT *)w.z, (const VT *)w.p,
-                           (const VT *)w.q, w.S, w.part + 4096, w.cnt + 3, mixed);
-        hipLaunchKernelGGL((k_cg_pupd<VT>), dim3(grid_for(n, 256, 2048)), dim3(256), 0,
-                           ctx->stream, n, (VT *)w.p, (const VT *)w.z, (const CGState *)w.S);
-      }
-      SMH_LAUNCH_CHECK(ctx);
-      launched += batch;
-    }
-    (void)launched;
-    // certify: FP64 residual of the final x
-    hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(L.blocks), dim3(256), 0, ctx->stream, Ares,
-                       L.per, b, (const double *)x, (VT *)w.r, w.S, w.part, w.cnt);
-    SMH_LAUNCH_CHECK(ctx);
+  TRY(dmalloc(ctx, (void **)&w.part, sizeof(double) * 2 * kPartSlots + 64));
+  // b'b, state, preconditioner
+  hipLaunchKernelGGL(k_dot_wg, dim3(nbv), dim3(256), 0, ctx->stream, n, b, w.part);
+  hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, ctx->stream, w.part, (int)nbv,
+                     w.part + 2 * kPartSlots);
+  hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(1), 0, ctx->stream, w.S, o->tol, o->rel,
+                     o->norm_form, o->maxiters, (pc != SMH_PRECOND_NONE && o->zr_stop) ? 1 : 0,
+                     o->reliable_delta, o->mixed, (const double *)(w.part + 2 * kPartSlots));
+  if (pc != SMH_PRECOND_NONE)
+    hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream,
+                       nrows, Am->rowptr, Am->colidx, Am->vals, Am->vals_lo, BD, pc, o->learned,
+                       (VT *)w.dinv);
+  epi.b = b;
+  epi.r = (VT *)w.r;
+  epi.z = (VT *)w.z;
+  epi.p = (VT *)w.p;
+  epi.dinv = w.dinv;
+  epi.pc = pc;
+  // r = b - A x, z = M r, p = z
+  epi.init = 1;
+  hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
+                     (const double *)x, epi, w.part);
+  hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
+                     (int)grt, 0);
+  LCHECK();
+  epi.init = 0;
+  for (;;) {
     SMH_HIP(ctx, hipMemcpyAsync(&hs, w.S, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
     SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (st) {
-      st->iters = hs.iter;
-      st->reliable_updates = nru;
-      st->rr = hs.rr;
-      st->true_rel = hs.bb > 0 ? std::sqrt(hs.rr_true / hs.bb) : std::sqrt(hs.rr_true);
-      int status = hs.status;
-      if (status == SMH_OK && !(hs.rr_true <= hs.thr * (1.0 + 1e-12)) &&
-          !(hs.precond_stop && hs.rz <= hs.thr))
-        status = SMH_ERR_NOT_CONVERGED;
-      st->status = status;
-      st->seconds =
-          std::chrono::duration<double>(std::chrono::steady_clock::now() - T0).count();
+    const bool replace = o->replace_every > 0 && hs.iter > 0 && hs.iter != last_rep &&
+                         hs.iter % o->replace_every == 0;
+    if (hs.status == SMH_OK && (hs.want_ru || hs.claimed || replace)) {
+      ++nru;
+      last_rep = hs.iter;
+      hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
+                         (const double *)x, epi, w.part);
+      hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
+                         (const double *)w.part, (int)grt, 0);
+      LCHECK();
+      continue;
     }
+    if (hs.done || hs.status != SMH_OK) break;
+    const int batch = o->replace_every > 0
+                          ? std::min(check, o->replace_every - hs.iter % o->replace_every)
+                          : check;
+    for (int k = 0; k < batch; ++k) {
+      hipLaunchKernelGGL((k_cg_spmv<IP, BD>), dim3(gt), dim3(tile_threads_host(BD)), 0, ctx->stream, T,
+                         (const VT *)w.p, (VT *)w.q, (const CGState *)w.S, w.part);
+      hipLaunchKernelGGL(k_cg_r1, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
+                         (int)gt);
+      hipLaunchKernelGGL((k_cg_update<VT, BD>), dim3(nbe), dim3(256), 0, ctx->stream, nrows, pc,
+                         (const void *)w.dinv, x, (VT *)w.r, (VT *)w.z, (const VT *)w.p,
+                         (const VT *)w.q, (const CGState *)w.S, w.part + kPartSlots);
+      hipLaunchKernelGGL(k_cg_r2, dim3(1), dim3(256), 0, ctx->stream, w.S,
+                         (const double *)(w.part + kPartSlots), (int)nbe);
+      hipLaunchKernelGGL((k_cg_pupd<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
+                         (const VT *)w.z, (const CGState *)w.S);
+    }
+    LCHECK();
+  }
+  // certify: FP64 residual of the final x (r, z are scratch from here on)
+  hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
+                     (const double *)x, epi, w.part);
+  hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
+                     (int)grt, 1);
+  LCHECK();
+  SMH_HIP(ctx, hipMemcpyAsync(&hs, w.S, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (st) {
+    st->iters = hs.iter;
+    st->reliable_updates = nru;
+    st->rr = hs.rr;
+    st->true_rel = hs.bb > 0 ? std::sqrt(hs.rr_true / hs.bb) : std::sqrt(hs.rr_true);
+    int status = hs.status;
+    if (status == SMH_OK && !(hs.rr_true <= hs.thr * (1.0 + 1e-12)) &&
+        !(hs.precond_stop && hs.rz <= hs.thr))
+      status = SMH_ERR_NOT_CONVERGED;
+    st->status = status;
+    st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - T0).count();
   }
 done:
 #undef TRY
+#undef LCHECK
   hipStreamSynchronize(ctx->stream);
   w.release();
   return rc;
 }
 
 }  // namespace
+
+extern "C" int smh_bsr_forget(smh_ctx *ctx, const uint32_t *rowptr) {
+  if (!ctx) return SMH_ERR_INVALID;
+  forget_plans(ctx, rowptr);
+  return SMH_OK;
+}
 
 extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
                          int32_t exact_order) {
@@ -589,17 +656,24 @@ extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
     SMH_LAUNCH_CHECK(ctx);
     return SMH_OK;
   }
-  BsrDev D{A->nrows, A->rowptr, A->colidx, A->vals, A->vals_lo};
-  const Launch L = spmv_launch(A->nrows, (int)A->bdim);
-#define SMH_MV(P)                                                                             \
-  do {                                                                                        \
-    using T = typename Val<P>::T;                                                             \
-    if (A->bdim == 3)                                                                         \
-      hipLaunchKernelGGL(k_bsrmv3<P>, dim3(L.blocks), dim3(256), 0, ctx->stream, D, L.per,    \
-                         (const T *)x, (T *)y);                                               \
-    else                                                                                      \
-      hipLaunchKernelGGL(k_bsrmv1<P>, dim3(L.blocks), dim3(256), 0, ctx->stream, D, L.per,    \
-                         (const T *)x, (T *)y);                                               \
+  smh_tile_plan plan;
+  const uint32_t nb = tile_blocks_host((int)A->bdim);
+  int rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, nb, &plan);
+  if (rc) return rc;
+  SMH_CHECK(ctx, plan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
+            "smh_bsrmv: a row holds %u blocks (at most %u outside exact_order)", plan.maxrow,
+            nb / 2);
+  unsigned g = 0;
+  const TileDev T = tile_dev(A, &plan, &g);
+#define SMH_MV(P)                                                                            \
+  do {                                                                                       \
+    using VT = typename Val<P>::T;                                                           \
+    if (A->bdim == 3)                                                                        \
+      hipLaunchKernelGGL((k_bsrmv_tiles<P, 3>), dim3(g), dim3(192), 0, ctx->stream, T,     \
+                         (const VT *)x, (VT *)y);                                            \
+    else                                                                                     \
+      hipLaunchKernelGGL((k_bsrmv_tiles<P, 1>), dim3(g), dim3(256), 0, ctx->stream, T,     \
+                         (const VT *)x, (VT *)y);                                            \
   } while (0)
   if (A->precision == SMH_F64) SMH_MV(SMH_F64);
   else if (A->precision == SMH_F32) SMH_MV(SMH_F32);
@@ -613,18 +687,20 @@ extern "C" int smh_block_jacobi(smh_ctx *ctx, const smh_bsr *A, double *d_dinv) 
   if (!ctx) return SMH_ERR_INVALID;
   SMH_CHECK(ctx, A && d_dinv && (A->bdim == 1 || A->bdim == 3), SMH_ERR_INVALID,
             "smh_block_jacobi: bad argument");
-  BsrDev D{A->nrows, A->rowptr, A->colidx, A->vals, A->vals_lo};
   const int pc = A->bdim == 3 ? SMH_PRECOND_BLOCK_JACOBI : SMH_PRECOND_JACOBI;
   const unsigned g = grid_for(A->nrows, 256);
   if (A->precision == SMH_F64)
-    hipLaunchKernelGGL((k_dinv<SMH_F64, double>), dim3(g), dim3(256), 0, ctx->stream, D,
-                       (int)A->bdim, pc, (const float *)nullptr, d_dinv);
+    hipLaunchKernelGGL((k_dinv<SMH_F64, double>), dim3(g), dim3(256), 0, ctx->stream, A->nrows,
+                       A->rowptr, A->colidx, A->vals, A->vals_lo, (int)A->bdim, pc,
+                       (const float *)nullptr, d_dinv);
   else if (A->precision == SMH_F32X2)
-    hipLaunchKernelGGL((k_dinv<SMH_F32X2, double>), dim3(g), dim3(256), 0, ctx->stream, D,
-                       (int)A->bdim, pc, (const float *)nullptr, d_dinv);
+    hipLaunchKernelGGL((k_dinv<SMH_F32X2, double>), dim3(g), dim3(256), 0, ctx->stream, A->nrows,
+                       A->rowptr, A->colidx, A->vals, A->vals_lo, (int)A->bdim, pc,
+                       (const float *)nullptr, d_dinv);
   else
-    hipLaunchKernelGGL((k_dinv<kF32D, double>), dim3(g), dim3(256), 0, ctx->stream, D,
-                       (int)A->bdim, pc, (const float *)nullptr, d_dinv);
+    hipLaunchKernelGGL((k_dinv<kF32D, double>), dim3(g), dim3(256), 0, ctx->stream, A->nrows,
+                       A->rowptr, A->colidx, A->vals, A->vals_lo, (int)A->bdim, pc,
+                       (const float *)nullptr, d_dinv);
   SMH_LAUNCH_CHECK(ctx);
   return SMH_OK;
 }
@@ -662,19 +738,23 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
     if (st) std::memset(st, 0, sizeof(*st));
     return SMH_OK;
   }
-  int rc;
-  if (o->mixed) {
-    rc = A->bdim == 3 ? cg_run<SMH_F32, SMH_F32X2, 3>(ctx, A, b, x, o, st)
-                      : cg_run<SMH_F32, SMH_F32X2, 1>(ctx, A, b, x, o, st);
-  } else if (A->precision == SMH_F64) {
-    rc = A->bdim == 3 ? cg_run<SMH_F64, SMH_F64, 3>(ctx, A, b, x, o, st)
-                      : cg_run<SMH_F64, SMH_F64, 1>(ctx, A, b, x, o, st);
-  } else if (A->precision == SMH_F32X2) {
-    rc = A->bdim == 3 ? cg_run<SMH_F32X2, SMH_F32X2, 3>(ctx, A, b, x, o, st)
-                      : cg_run<SMH_F32X2, SMH_F32X2, 1>(ctx, A, b, x, o, st);
-  } else {
-    rc = A->bdim == 3 ? cg_run<kF32D, kF32D, 3>(ctx, A, b, x, o, st)
-                      : cg_run<kF32D, kF32D, 1>(ctx, A, b, x, o, st);
-  }
+  const int bd = (int)A->bdim;
+  const int ip = o->mixed ? SMH_F32 : (A->precision == SMH_F32 ? kF32D : A->precision);
+  const int rp = o->mixed ? SMH_F32X2 : ip;
+  smh_tile_plan plan, rplan;
+  int rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(bd), &plan);
+  if (!rc) rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(bd), &rplan);
+  if (rc) return rc;
+  SMH_CHECK(ctx, plan.tile_row != nullptr && rplan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
+            "smh_cg_solve: a row holds %u blocks (at most %u supported)", plan.maxrow,
+            tile_blocks_host(bd) / 2);
+#define SMH_CG(IP, RP)                                                                  \
+  (A->bdim == 3 ? cg_run<IP, RP, 3>(ctx, A, &plan, &rplan, b, x, o, st)                 \
+                : cg_run<IP, RP, 1>(ctx, A, &plan, &rplan, b, x, o, st))
+  if (o->mixed) rc = SMH_CG(SMH_F32, SMH_F32X2);
+  else if (A->precision == SMH_F64) rc = SMH_CG(SMH_F64, SMH_F64);
+  else if (A->precision == SMH_F32X2) rc = SMH_CG(SMH_F32X2, SMH_F32X2);
+  else rc = SMH_CG(kF32D, kF32D);
+#undef SMH_CG
   return rc;
 }

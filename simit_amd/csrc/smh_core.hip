@@ -215,6 +215,7 @@ extern "C" int smh_ctx_destroy(smh_ctx *ctx) {
   if (!ctx) return SMH_ERR_INVALID;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  for (auto &p : ctx->plans) dfree(p.tile_row);
   if (ctx->red) hipFree(ctx->red);
   if (ctx->red_count) hipFree(ctx->red_count);
   if (ctx->host_pinned) hipHostFree(ctx->host_pinned);
@@ -229,7 +230,7 @@ extern "C" const char *smh_ctx_last_error(const smh_ctx *ctx) {
 
 extern "C" int smh_ctx_set_stream(smh_ctx *ctx, void *s) {
   if (!ctx) return SMH_ERR_INVALID;
-  ctx->stream = s ? (hipStream_t)s : ctx->own_stream;
+  ctx->stream = (hipStream_t)s;
   return SMH_OK;
 }
 

@@ -234,10 +234,25 @@ __global__ void k_locs(const int32_t *__restrict__ ep, uint64_t ne, uint32_t car
   }
 }
 
+// row of the first block of every 64-block assembly wave
+__global__ void k_wave_rows(const uint32_t *__restrict__ rowptr, uint32_t nrows, uint64_t nwaves,
+                            uint32_t *__restrict__ wave_row) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwaves) return;
+  const uint64_t b0 = w * 64;
+  uint32_t lo = 0, hi = nrows;  // rowptr[lo] <= b0 < rowptr[hi]
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if ((uint64_t)rowptr[mid] <= b0) lo = mid; else hi = mid;
+  }
+  wave_row[w] = lo;
+}
+
 void free_index(smh_index *ix) {
   if (!ix) return;
   smh::dfree(ix->rowptr);
   smh::dfree(ix->colidx);
+  smh::dfree(ix->wave_row);
   for (auto p : ix->ve_ptr) smh::dfree(p);
   for (auto p : ix->ve_elem) smh::dfree(p);
   delete ix;
@@ -359,6 +374,11 @@ extern "C" int smh_index_build(smh_ctx *ctx, uint32_t nv, uint32_t nsets, const 
       hipLaunchKernelGGL(k_vev_big<true>, dim3(hbig), dim3(256), 0, ctx->stream, S, fd,
                          (const uint32_t *)big, cnt, (const uint32_t *)ix->rowptr, ix->colidx,
                          err);
+    const uint64_t nwaves = (nnz + 63) / 64;
+    TRY(dmalloc(ctx, (void **)&ix->wave_row, sizeof(uint32_t) * (nwaves ? nwaves : 1)));
+    if (nwaves && nv)
+      hipLaunchKernelGGL(k_wave_rows, dim3(grid_for(nwaves, 256)), dim3(256), 0, ctx->stream,
+                         (const uint32_t *)ix->rowptr, nv, nwaves, ix->wave_row);
     int herr = 0;
     hipError_t le = hipGetLastError();
     if (le != hipSuccess) {
@@ -414,6 +434,7 @@ extern "C" int smh_index_destroy(smh_ctx *ctx, smh_index *idx) {
   if (!ctx) return SMH_ERR_INVALID;
   if (!idx) return SMH_OK;
   hipStreamSynchronize(ctx->stream);
+  smh::forget_plans(ctx, idx->rowptr);
   free_index(idx);
   return SMH_OK;
 }

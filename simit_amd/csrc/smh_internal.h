@@ -11,11 +11,25 @@
 
 #include "../../include/simit_hip.h"
 
+// Row-tile partition of one blocked-CSR pattern (see smh_linalg.h), cached
+// per context and keyed by the pattern's rowptr buffer, row and block count.
+struct smh_tile_plan {
+  const uint32_t *rowptr = nullptr;
+  uint32_t nrows = 0;
+  uint64_t nnz = 0;
+  uint32_t maxrow = 0;
+  uint32_t tile_blocks = 0;
+  uint32_t tprime = 0;
+  uint32_t ntiles = 0;
+  uint32_t *tile_row = nullptr;  // device [ntiles+1]
+};
+
 struct smh_ctx {
   int device = 0;
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   std::string last_error;
+  std::vector<smh_tile_plan> plans;
   // small device scratch for reductions: [0..kRedSlots) partials + counters
   double *red = nullptr;
   unsigned *red_count = nullptr;
@@ -27,6 +41,7 @@ struct smh_index {
   uint64_t nnz = 0;
   uint32_t *rowptr = nullptr;
   uint32_t *colidx = nullptr;
+  uint32_t *wave_row = nullptr;     // row holding block 64*w (assembly waves)
   uint32_t nsets = 0;
   std::vector<uint32_t *> ve_ptr;   // per set, [nrows+1]
   std::vector<uint32_t *> ve_elem;  // per set
@@ -79,6 +94,12 @@ int scan_u32(smh_ctx *ctx, const uint32_t *in, uint32_t *out, uint32_t n,
 // device allocation helpers that report through ctx
 int dmalloc(smh_ctx *ctx, void **p, size_t bytes);
 void dfree(void *p);
+
+// the row-tile plan of a pattern (built on first use, synchronises then)
+int tile_plan(smh_ctx *ctx, const uint32_t *rowptr, uint32_t nrows, uint64_t nnz,
+              uint32_t tile_blocks, smh_tile_plan *out);
+// drop cached plans of a rowptr buffer (before it is freed or reused)
+void forget_plans(smh_ctx *ctx, const uint32_t *rowptr);
 
 }  // namespace smh
 

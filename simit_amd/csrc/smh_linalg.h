@@ -6,123 +6,224 @@
 // The reference GPU path ran one thread per row over that serial loop, with
 // 16-thread blocks (gpu_backend.h:49-51).
 //
-// MI355X layout of the work (3x3 blocks): one 64-lane wavefront per block
-// row; lane l < 63 has the fixed role (slot = l/9, e = l%9): it reads value
-// e of blocks rowptr[i] + slot, + 7, + 14, ...  A row's values are
-// contiguous, so each load instruction reads 63 consecutive values (fully
-// coalesced), and no lane ever divides.  The 3 row sums come out of a
-// 5-step shuffle tree on lanes 0, 3, 6.  1x1 blocks: a half-wave per row.
+// MI355X layout of the work ("row tiles"): the stored blocks are cut into
+// tiles of whole rows holding at most NB blocks: tile k holds the rows
+// whose first block lies in [k*T', (k+1)*T'), T' = NB - maxrow + 1.  One
+// persistent workgroup streams a contiguous range of tiles:
+//   1. lane (s, bi) owns block row bi of blocks s, s+S, s+2S, s+3S of a tile
+//      (S = blocks per round): it reads those b contiguous values each, so a
+//      wavefront's loads cover one contiguous range of the value array;
+//   2. it gathers x[colidx*b .. +b] (L2/MALL resident) and writes one partial
+//      row sum per block into LDS;
+//   3. the next tile's values and columns are issued into registers right
+//      away, so HBM reads stay in flight while this tile finishes;
+//   4. one thread per (row, component) adds its blocks' partial sums in
+//      ascending order, then one thread per row hands the row to an epilogue
+//      (store, dot partial, residual, ...).
+// Tiles are dealt so that each XCD (workgroups go round-robin over 8 XCDs)
+// owns a contiguous eighth of the rows and its L2 holds the x entries those
+// rows gather.
 #pragma once
 #include "smh_internal.h"
 
 namespace smh {
 
+constexpr int kRounds = 4;  // blocks per lane per tile
+
+template <int BD>
+struct TileShape {
+  static constexpr int kThreads = BD == 3 ? 192 : 256;  // 3 or 4 wavefronts
+  static constexpr int kWaves = kThreads / 64;
+  static constexpr int kPerRound = kThreads / BD;        // blocks per round
+  static constexpr int kBlocks = kPerRound * kRounds;    // blocks per tile
+};
+inline uint32_t tile_blocks_host(int bd) { return bd == 3 ? 64 * kRounds : 256 * kRounds; }
+inline uint32_t tile_threads_host(int bd) { return bd == 3 ? 192 : 256; }
+
 template <int PREC> struct Val;
 template <> struct Val<SMH_F64> {
-  using T = double;
-  __device__ static __forceinline__ double ld(const void *v, const void *, uint64_t i) {
-    return __builtin_nontemporal_load(((const double *)v) + i);
-  }
+  using T = double;  // vector / compute type
+  using S = double;  // stored value type
+  static constexpr bool kLo = false;
 };
 template <> struct Val<SMH_F32> {
   using T = float;
-  __device__ static __forceinline__ float ld(const void *v, const void *, uint64_t i) {
-    return __builtin_nontemporal_load(((const float *)v) + i);
-  }
+  using S = float;
+  static constexpr bool kLo = false;
 };
 template <> struct Val<SMH_F32X2> {
   using T = double;
-  __device__ static __forceinline__ double ld(const void *v, const void *lo, uint64_t i) {
-    return (double)__builtin_nontemporal_load(((const float *)v) + i) +
-           (double)__builtin_nontemporal_load(((const float *)lo) + i);
-  }
+  using S = float;
+  static constexpr bool kLo = true;
+};
+constexpr int kF32D = 3;  // float values driven by double vectors
+template <> struct Val<kF32D> {
+  using T = double;
+  using S = float;
+  static constexpr bool kLo = false;
 };
 
-struct BsrDev {
+struct TileDev {
   uint32_t nrows;
+  uint64_t nnz;
   const uint32_t *rowptr;
   const uint32_t *colidx;
   const void *vals;
   const void *vals_lo;
+  const uint32_t *tile_row;  // [ntiles+1]
+  uint32_t ntiles;
+  uint32_t tiles_per_wg;
 };
 
-__device__ __forceinline__ double shfl_down_zero(double v, int d) {
-  const int lane = threadIdx.x & 63;
-  const int src = lane + d;
-  const double o = __shfl(v, src & 63, 64);
-  return src < 64 ? o : 0.0;
-}
-__device__ __forceinline__ float shfl_down_zero(float v, int d) {
-  const int lane = threadIdx.x & 63;
-  const int src = lane + d;
-  const float o = __shfl(v, src & 63, 64);
-  return src < 64 ? o : 0.0f;
-}
-
-// One 3x3 block row with the whole wavefront.  Returns the partial sum; the
-// row sums y[bi] end up on lanes 3*bi (bi = 0, 1, 2).
-template <int PREC>
-__device__ __forceinline__ typename Val<PREC>::T
-row3(const BsrDev &A, const typename Val<PREC>::T *__restrict__ x, uint32_t r, int lane) {
+template <int PREC, int BD>
+struct TileSmem {
   using T = typename Val<PREC>::T;
-  const uint32_t rb = A.rowptr[r], re = A.rowptr[r + 1];
-  const int slot = lane / 9, e = lane - 9 * slot, bj = e % 3;
-  T acc = 0;
-  if (lane < 63) {
-    for (uint32_t t = rb + slot; t < re; t += 7) {
-      const T v = Val<PREC>::ld(A.vals, A.vals_lo, (uint64_t)t * 9 + e);
-      const uint32_t j = A.colidx[t];
-      acc += v * x[(uint64_t)j * 3 + bj];
-    }
-  }
-  acc += shfl_down_zero(acc, 1) + shfl_down_zero(acc, 2);
-  acc += shfl_down_zero(acc, 9);
-  acc += shfl_down_zero(acc, 18);
-  acc += shfl_down_zero(acc, 36);
-  return acc;
-}
+  static constexpr int kBlocks = TileShape<BD>::kBlocks;
+  T part[kBlocks * BD];
+  T rsum[(kBlocks + 1) * BD];
+  uint32_t rp[kBlocks + 2];  // rowptr of the tile's rows, relative
+  uint32_t bnd[2][4];        // R0, R1, base, end of this and the next tile
+  double red[TileShape<BD>::kWaves];
+};
 
-// One 1x1 row per half-wave; the sum ends up on lane 0 of the half.
-template <int PREC>
-__device__ __forceinline__ typename Val<PREC>::T
-row1(const BsrDev &A, const typename Val<PREC>::T *__restrict__ x, uint32_t r, bool active,
-     int hl) {
-  using T = typename Val<PREC>::T;
-  T acc = 0;
-  if (active) {
-    const uint32_t rb = A.rowptr[r], re = A.rowptr[r + 1];
-    for (uint32_t t = rb + hl; t < re; t += 32) {
-      const T v = Val<PREC>::ld(A.vals, A.vals_lo, t);
-      acc += v * x[A.colidx[t]];
-    }
-  }
-  for (int off = 16; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  return acc;
-}
-
-// Contiguous row range of wavefront w out of W, with the workgroup order
-// remapped so that each XCD (blocks are dealt round-robin over 8 XCDs)
-// works on one contiguous eighth of the rows: its L2 then holds the x
-// entries its rows gather.
-__device__ __forceinline__ void wave_rows(uint32_t nrows, uint32_t per, uint32_t &r0,
-                                          uint32_t &r1) {
+// Map workgroup g to its tile range (XCD-contiguous when the grid is a
+// multiple of 8).
+__device__ __forceinline__ void wg_tiles(const TileDev &A, uint32_t &k0, uint32_t &k1) {
   const uint32_t nb = gridDim.x;
   uint32_t lb = blockIdx.x;
   if ((nb & 7) == 0) lb = (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
-  const uint32_t w = lb * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const uint64_t a = (uint64_t)w * per, b = a + per;
-  r0 = (uint32_t)min<uint64_t>(a, nrows);
-  r1 = (uint32_t)min<uint64_t>(b, nrows);
+  const uint64_t a = (uint64_t)lb * A.tiles_per_wg;
+  k0 = (uint32_t)min<uint64_t>(a, A.ntiles);
+  k1 = (uint32_t)min<uint64_t>(a + A.tiles_per_wg, A.ntiles);
 }
 
-// block-wide FP64 sum (256 threads), fixed order; result valid on thread 0
-__device__ __forceinline__ double block_sum256(double v, double *s4) {
-  v = wave_sum(v);
-  if ((threadIdx.x & 63) == 0) s4[threadIdx.x >> 6] = v;
+// Registers holding one tile's values and columns for one lane.
+template <int PREC, int BD>
+struct TileRegs {
+  using S = typename Val<PREC>::S;
+  S v[kRounds][BD];
+  float vl[Val<PREC>::kLo ? kRounds : 1][BD];
+  uint32_t col[kRounds];
+
+  __device__ __forceinline__ void load(const TileDev &A, uint32_t base, uint32_t end, int s,
+                                       int bi) {
+    using SH = TileShape<BD>;
+    constexpr int BB = BD * BD;
+#pragma unroll
+    for (int u = 0; u < kRounds; ++u) {
+      const uint32_t blk = base + s + u * SH::kPerRound;
+      const bool act = blk < end;
+      col[u] = act ? A.colidx[blk] : 0u;
+      const uint64_t o = (uint64_t)blk * BB + bi * BD;
+#pragma unroll
+      for (int b = 0; b < BD; ++b) {
+        v[u][b] = act ? __builtin_nontemporal_load((const S *)A.vals + o + b) : (S)0;
+        if (Val<PREC>::kLo)
+          vl[Val<PREC>::kLo ? u : 0][b] =
+              act ? __builtin_nontemporal_load((const float *)A.vals_lo + o + b) : 0.0f;
+      }
+    }
+  }
+};
+
+// Tile boundaries (R0, R1, base, end) of tile k into slot.
+__device__ __forceinline__ void tile_bounds(const TileDev &A, uint32_t k, uint32_t *slot) {
+  const uint32_t R0 = A.tile_row[k], R1 = A.tile_row[k + 1];
+  slot[0] = R0;
+  slot[1] = R1;
+  slot[2] = A.rowptr[R0];
+  slot[3] = A.rowptr[R1];
+}
+
+// Stream tiles [k0, k1): rows handed to epi(row, s[BD]) by one thread per
+// row.  XG gathers x: xg(j, bj).
+template <int PREC, int BD, class XG, class EPI>
+__device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32_t k1,
+                                           TileSmem<PREC, BD> &sm, const XG &xg, EPI &epi) {
+  using T = typename Val<PREC>::T;
+  using SH = TileShape<BD>;
+  const int t = threadIdx.x;
+  const int s = t / BD, bi = t - (t / BD) * BD;
+  if (k0 >= k1) return;
+  if (t == 0) {
+    tile_bounds(A, k0, sm.bnd[0]);
+    if (k0 + 1 < k1) tile_bounds(A, k0 + 1, sm.bnd[1]);
+  }
   __syncthreads();
-  double t = ((s4[0] + s4[1]) + s4[2]) + s4[3];
+  TileRegs<PREC, BD> cur, nxt;
+  cur.load(A, sm.bnd[0][2], sm.bnd[0][3], s, bi);
+  for (uint32_t k = k0; k < k1; ++k) {
+    const uint32_t R0 = sm.bnd[0][0], R1 = sm.bnd[0][1], base = sm.bnd[0][2];
+    const uint32_t nbase = sm.bnd[1][2], nend = sm.bnd[1][3];
+    const uint32_t nr = R1 - R0;
+    const bool small = nr + 1 <= (uint32_t)(SH::kBlocks + 1);
+    // read ahead the boundaries of tile k+2
+    uint32_t ahead[4] = {0, 0, 0, 0};
+    if (t == 0 && k + 2 < k1) tile_bounds(A, k + 2, ahead);
+    if (small)
+      for (uint32_t i = t; i <= nr; i += SH::kThreads) sm.rp[i] = A.rowptr[R0 + i] - base;
+    // partial row sums of this lane's blocks
+#pragma unroll
+    for (int u = 0; u < kRounds; ++u) {
+      T acc = 0;
+#pragma unroll
+      for (int b = 0; b < BD; ++b) {
+        T val = (T)cur.v[u][b];
+        if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][b];
+        acc += val * xg(cur.col[u], b);
+      }
+      sm.part[(s + u * SH::kPerRound) * BD + bi] = acc;
+    }
+    // the next tile's loads go out before this tile is reduced
+    if (k + 1 < k1) nxt.load(A, nbase, nend, s, bi);
+    __syncthreads();
+    if (small) {
+      for (uint32_t i = t; i < nr * BD; i += SH::kThreads) {
+        const uint32_t r = i / BD, a = i - (i / BD) * BD;
+        const uint32_t q0 = sm.rp[r], q1 = sm.rp[r + 1];
+        T acc = 0;
+        for (uint32_t q = q0; q < q1; ++q) acc += sm.part[q * BD + a];
+        sm.rsum[i] = acc;
+      }
+      __syncthreads();
+      for (uint32_t i = t; i < nr; i += SH::kThreads) {
+        T r[BD];
+        for (int a = 0; a < BD; ++a) r[a] = sm.rsum[i * BD + a];
+        epi(R0 + i, r);
+      }
+    } else {
+      for (uint32_t i = t; i < nr; i += SH::kThreads) {
+        const uint32_t q0 = A.rowptr[R0 + i] - base, q1 = A.rowptr[R0 + i + 1] - base;
+        T r[BD];
+        for (int a = 0; a < BD; ++a) r[a] = 0;
+        for (uint32_t q = q0; q < q1; ++q)
+          for (int a = 0; a < BD; ++a) r[a] += sm.part[q * BD + a];
+        epi(R0 + i, r);
+      }
+    }
+    __syncthreads();
+    if (t == 0) {
+      for (int i = 0; i < 4; ++i) {
+        sm.bnd[0][i] = sm.bnd[1][i];
+        sm.bnd[1][i] = ahead[i];
+      }
+    }
+    cur = nxt;
+    __syncthreads();
+  }
+}
+
+// workgroup-wide FP64 sum (NW wavefronts), fixed order; valid on every thread
+template <int NW>
+__device__ __forceinline__ double block_sum(double v, double *s) {
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double t = s[0];
+  for (int w = 1; w < NW; ++w) t += s[w];
   __syncthreads();
   return t;
 }
+__device__ __forceinline__ double block_sum256(double v, double *s4) { return block_sum<4>(v, s4); }
 
 }  // namespace smh

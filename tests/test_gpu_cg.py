@@ -138,3 +138,38 @@ def test_indefinite_matrix_breaks_down(ctx):
     st = ctx.cg(idx.bsr(1, S.F64, to_dev(vals)), to_dev(np.ones(n)), x,
                 ctx.cg_options(maxiters=10, tol=1e-10))
     assert st.status == S.ERR_BREAKDOWN
+
+
+def test_program_cg_fully_on_device(ctx, fixtures):
+    """cg.sim end to end on the GPU: A = I - 0.01*(map f to springs) assembled
+    by the device edge-block kernel, then the device CG."""
+    fx = fixtures["program_cg"]
+    ep = to_dev(np.array(fx["springs"], np.int32))
+    idx = ctx.build_index(3, [ep])
+    w = to_dev(np.array(fx["a"], dtype=np.float64))
+    vals = torch.empty(idx.nnz, dtype=torch.float64, device="cuda")
+    ctx.assemble_edge_blocks(idx, ep, w, 1, sign=(1, -1, -1, 1), alpha=1.0, beta=-0.01,
+                             vals=vals)
+    x = torch.zeros(3, dtype=torch.float64, device="cuda")
+    ctx.cg(idx.bsr(1, S.F64, vals), to_dev(np.array(fx["b"])), x,
+           ctx.cg_options(maxiters=fx["maxiters"], tol=fx["tol"], rel=0, norm_form=1))
+    np.testing.assert_allclose(x.cpu().numpy(), fx["expected_c"], atol=fx["abs_tol"], rtol=0)
+
+
+def test_matrix_vv_and_gemv_blocked_on_device(ctx, fixtures):
+    fx = fixtures["assembly_matrix_vv"]
+    ep = to_dev(np.array(fx["springs"], np.int32))
+    idx = ctx.build_index(3, [ep])
+    vals = torch.empty(idx.nnz, dtype=torch.float64, device="cuda")
+    ctx.assemble_edge_blocks(idx, ep, to_dev(np.ones(2)), 1, vals=vals)
+    y = torch.empty(3, dtype=torch.float64, device="cuda")
+    ctx.bsrmv(idx.bsr(1, S.F64, vals), to_dev(np.array(fx["a"], dtype=np.float64)), y)
+    np.testing.assert_array_equal(y.cpu().numpy(), fx["expected_b"])
+    fx = fixtures["gemv_blocked"]
+    ep = to_dev(np.array(fx["springs"], np.int32))
+    idx = ctx.build_index(3, [ep])
+    vals = torch.empty(idx.nnz * 4, dtype=torch.float64, device="cuda")
+    ctx.assemble_edge_blocks(idx, ep, to_dev(np.array(fx["a"]).ravel()), 2, vals=vals)
+    y = torch.empty(6, dtype=torch.float64, device="cuda")
+    ctx.bsrmv(idx.bsr(2, S.F64, vals), to_dev(np.array(fx["b"]).ravel()), y, exact_order=True)
+    np.testing.assert_array_equal(y.cpu().numpy().reshape(3, 2), fx["expected_c"])

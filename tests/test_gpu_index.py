@@ -11,7 +11,11 @@ pytestmark = pytest.mark.gpu
 
 
 def dev_sets(sets):
-    return [to_dev(np.ascontiguousarray(s, dtype=np.int32).reshape(len(s), -1)) for s in sets]
+    out = []
+    for s in sets:
+        a = np.ascontiguousarray(s, dtype=np.int32)
+        out.append(to_dev(a if a.ndim == 2 else a.reshape(len(a), -1)))
+    return out
 
 
 def build(ctx, nv, sets, force_diag=False):
