@@ -252,17 +252,25 @@ static void tet_geom(const double *xs, double dN[12], double *W) {
     }
 }
 
-/* Block (a,b) of Ke = W * B6' * E * B6 (fem_linear.sim:95-174) in closed form:
- * Ke_ab[i][j] = W*(lam*ga_i*gb_j + mu*ga_j*gb_i + [i==j]*mu*(ga.gb)). */
+/* Block (a,b) of s * Ke, Ke = W * B6' * E * B6 (fem_linear.sim:95-174), in
+ * closed form.  With u = (s*W*lam) g_a and w = (s*W*mu) g_a:
+ *   blk[i][j] = u_i gb_j + w_j gb_i + [i==j] (w . gb)
+ * (the order of operations below is the one the device kernel uses). */
 static void tet_block_from(const double dN[12], double W, double mu, double lam,
-                           int a, int b, double *blk) {
+                           double s, int a, int b, double *blk) {
   const double *ga = dN + 3 * a, *gb = dN + 3 * b;
-  double gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+  const double sw = s * W, cl = sw * lam, cm = sw * mu;
+  double u[3], w[3];
+  for (int i = 0; i < 3; ++i) {
+    u[i] = cl * ga[i];
+    w[i] = cm * ga[i];
+  }
+  const double wg = w[0] * gb[0] + w[1] * gb[1] + w[2] * gb[2];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      double t = lam * ga[i] * gb[j] + mu * ga[j] * gb[i];
-      if (i == j) t = t + mu * gg;
-      blk[3 * i + j] = W * t;
+      double t = u[i] * gb[j] + w[j] * gb[i];
+      if (i == j) t = t + wg;
+      blk[3 * i + j] = t;
     }
 }
 
@@ -270,7 +278,7 @@ void or_tet_block(const double *xs, double mu, double lam, int a, int b,
                   double *blk, double *W) {
   double dN[12];
   tet_geom(xs, dN, W);
-  tet_block_from(dN, *W, mu, lam, a, b, blk);
+  tet_block_from(dN, *W, mu, lam, 1.0, a, b, blk);
 }
 
 void or_tet_assemble(const or_tet_params *P, int32_t nv, int64_t nt,
@@ -292,9 +300,9 @@ void or_tet_assemble(const or_tet_params *P, int32_t nv, int64_t nt,
     for (int a = 0; a < 4; ++a)
       for (int b = 0; b < 4; ++b) {
         if (c[v[a]] <= 0 && c[v[b]] <= 0) {
-          tet_block_from(dN, W, mu[t], lam[t], a, b, blk);
+          tet_block_from(dN, W, mu[t], lam[t], P->stiff_scale, a, b, blk);
           int32_t l = or_loc(rowptr, colidx, v[a], v[b]);
-          for (int q = 0; q < 9; ++q) vals[9 * (int64_t)l + q] += P->stiff_scale * blk[q];
+          for (int q = 0; q < 9; ++q) vals[9 * (int64_t)l + q] += blk[q];
         }
       }
     if (P->rho != 0.0) {

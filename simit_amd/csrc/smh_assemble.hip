@@ -9,13 +9,17 @@
 //
 // MI355X design ("pull"): one 64-lane wavefront owns 64 consecutive stored
 // blocks, one lane per block (i, j).  The wave walks the ascending element
-// lists (Link::ve) of its rows in chunks of 64 (row, element) pairs: every
-// lane computes one element's state (gather + geometry) into LDS, then every
-// lane adds, in list order, the contributions of the elements of its own row
-// that also hold j.  Each block is summed by exactly one lane in exactly the
-// CPU order, in FP64, with no fused multiply-add: bit-identical to the CPU
-// loop for SMH_F64.  The 64 results are staged through LDS so the stores are
-// coalesced.  No atomics, so repeated runs are bitwise identical.
+// lists (Link::ve) of its rows in chunks of 64 (row, element) pairs:
+//   A. every lane takes one pair, gathers the element and computes what the
+//      row's blocks need (geometry, scaled gradients ...) into a compact
+//      per-pair record in LDS;
+//   B. every block lane walks its row's pairs in list order and, for each
+//      element that also holds j, forms block (i, j) from the record and adds
+//      it.
+// Each block is summed by exactly one lane in exactly the CPU order, in FP64,
+// with no fused multiply-add: bit-identical to the CPU loop for SMH_F64.  The
+// 64 results are staged through LDS so the stores are coalesced.  No atomics,
+// so repeated runs are bitwise identical.
 #include "smh_internal.h"
 
 #pragma clang fp contract(off)
@@ -25,6 +29,10 @@ namespace {
 constexpr int kWave = 64;
 
 // ---- element programs --------------------------------------------------------
+// Each op gives: State (registers, phase A) loaded from the element; Pair
+// (LDS, one per staged pair) made for the row's endpoint slot a; block()
+// forms the row's block with endpoint slot b from a Pair; extra() and rhs()
+// the diagonal extra term and the right-hand-side contribution of the row.
 
 // test/input/program/isprings.sim:19-69
 struct SpringArgs {
@@ -35,12 +43,19 @@ struct SpringArgs {
 };
 struct SpringOp {
   static constexpr int CARD = 2, BD = 3;
+  static constexpr bool kHasRhs = true, kExtra = false;
   using Args = SpringArgs;
   struct State {
     int32_t vid[2];
     int32_t free_[2];
     double fe0[3];
     double kb[9];
+  };
+  struct Pair {
+    int32_t vid[2];
+    int32_t a, fa, fb;
+    double kb[9];
+    double fe0[3];
   };
   __device__ static void load(const Args &A, uint64_t e, State &st) {
     const int32_t a = A.ep[2 * e], b = A.ep[2 * e + 1];
@@ -64,16 +79,25 @@ struct SpringOp {
         st.kb[3 * i + j] = c * (o + r * d);
       }
   }
+  __device__ static void pair(const Args &, const State &st, int a, Pair &P) {
+    P.vid[0] = st.vid[0];
+    P.vid[1] = st.vid[1];
+    P.a = a;
+    P.fa = a == 0 ? st.free_[0] : st.free_[1];
+    P.fb = a == 0 ? st.free_[1] : st.free_[0];
+    for (int q = 0; q < 9; ++q) P.kb[q] = st.kb[q];
+    for (int q = 0; q < 3; ++q) P.fe0[q] = st.fe0[q];
+  }
   // K(p(a),p(b)) written iff free_a and (a == b or free_b) (isprings.sim:58-68)
-  __device__ static bool block(const Args &, const State &st, int a, int b, double *out) {
-    if (!st.free_[a] || (a != b && !st.free_[b])) return false;
-    for (int q = 0; q < 9; ++q) out[q] = a == b ? st.kb[q] : -st.kb[q];
+  __device__ static bool block(const Args &, const Pair &P, int b, double *out) {
+    if (!P.fa || (P.a != b && !P.fb)) return false;
+    for (int q = 0; q < 9; ++q) out[q] = P.a == b ? P.kb[q] : -P.kb[q];
     return true;
   }
-  __device__ static bool diag_extra(const Args &, const State &, int, double *) { return false; }
-  __device__ static bool rhs(const Args &, const State &st, int a, double *f) {
-    if (!st.free_[a]) return false;
-    for (int q = 0; q < 3; ++q) f[q] = a == 0 ? st.fe0[q] : -st.fe0[q];
+  __device__ static bool extra(const Args &, const Pair &, double *) { return false; }
+  __device__ static bool rhs(const Args &, const Pair &P, double *f) {
+    if (!P.fa) return false;
+    for (int q = 0; q < 3; ++q) f[q] = P.a == 0 ? P.fe0[q] : -P.fe0[q];
     return true;
   }
   // MDK = MD + K and f = fmg + fe, with MD/fmg from compute_mass_damping
@@ -101,8 +125,6 @@ struct SpringOp {
       f[q] = mass * A.v[3 * (int64_t)i + q] + A.p.h * mass * A.p.grav[q] + 0.0;
   }
   __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
-  static constexpr bool kHasRhs = true;
-  static constexpr bool kExtra = false;
 };
 
 // apps/fem/fem_linear.sim:95-194
@@ -132,14 +154,32 @@ __device__ __forceinline__ void inv3(const double *a, double *inv) {
   inv[3] = cof01 * d; inv[4] = cof11 * d; inv[5] = cof21 * d;
   inv[6] = cof02 * d; inv[7] = cof12 * d; inv[8] = cof22 * d;
 }
+// element a of a 4-slot array (stride apart) by selects, for a runtime a
+template <typename T>
+__device__ __forceinline__ T pick4(const T *v, int a, int stride = 1) {
+  const T x01 = a == 0 ? v[0] : v[stride];
+  const T x23 = a == 2 ? v[2 * stride] : v[3 * stride];
+  return a < 2 ? x01 : x23;
+}
 struct TetOp {
   static constexpr int CARD = 4, BD = 3;
+  static constexpr bool kHasRhs = true, kExtra = true;
   using Args = TetArgs;
   struct State {
     int32_t vid[4];
     int32_t free_[4];
     double dN[12];
     double W, mu, lam;
+  };
+  // u = (s*W*lam) g_a and w = (s*W*mu) g_a for the row's slot a, and all four
+  // shape gradients g_b
+  struct Pair {
+    int32_t vid[4];
+    uint32_t freemask;
+    int32_t fa;
+    double u[3], w[3];
+    double g[12];
+    double W;
   };
   __device__ static void load(const Args &A, uint64_t e, State &st) {
     double xs[12];
@@ -163,28 +203,46 @@ struct TetOp {
     st.mu = A.mu[e];
     st.lam = A.lam[e];
   }
-  __device__ static bool block(const Args &A, const State &st, int a, int b, double *out) {
-    if (!st.free_[a] || !st.free_[b]) return false;
-    const double *ga = st.dN + 3 * a, *gb = st.dN + 3 * b;
-    const double gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+  __device__ static void pair(const Args &A, const State &st, int a, Pair &P) {
+    uint32_t fm = 0;
+    for (int t = 0; t < 4; ++t) {
+      P.vid[t] = st.vid[t];
+      fm |= (st.free_[t] ? 1u : 0u) << t;
+    }
+    P.freemask = fm;
+    P.fa = (fm >> a) & 1u;
+    const double sw = A.p.stiff_scale * st.W, cl = sw * st.lam, cm = sw * st.mu;
+    for (int i = 0; i < 3; ++i) {
+      const double ga = pick4(st.dN + i, a, 3);
+      P.u[i] = cl * ga;
+      P.w[i] = cm * ga;
+    }
+    for (int q = 0; q < 12; ++q) P.g[q] = st.dN[q];
+    P.W = st.W;
+  }
+  // blk[i][j] = u_i gb_j + w_j gb_i + [i==j] (w . gb)  (s * W * B'EB, closed form)
+  __device__ static bool block(const Args &, const Pair &P, int b, double *out) {
+    if (!P.fa || !((P.freemask >> b) & 1u)) return false;
+    const double *gb = P.g + 3 * b;
+    const double wg = P.w[0] * gb[0] + P.w[1] * gb[1] + P.w[2] * gb[2];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
-        double t = st.lam * ga[i] * gb[j] + st.mu * ga[j] * gb[i];
-        if (i == j) t = t + st.mu * gg;
-        out[3 * i + j] = A.p.stiff_scale * (st.W * t);
+        double t = P.u[i] * gb[j] + P.w[j] * gb[i];
+        if (i == j) t = t + wg;
+        out[3 * i + j] = t;
       }
     return true;
   }
   // lumped mass after the element's stiffness writes (fem_linear.sim:168-173)
-  __device__ static bool diag_extra(const Args &A, const State &st, int, double *out) {
+  __device__ static bool extra(const Args &A, const Pair &P, double *out) {
     if (A.p.rho == 0.0) return false;
-    const double mm = 0.25 * A.p.rho * st.W;
+    const double mm = 0.25 * A.p.rho * P.W;
     for (int q = 0; q < 9; ++q) out[q] = (q % 4 == 0) ? mm : 0.0;
     return true;
   }
-  __device__ static bool rhs(const Args &A, const State &st, int a, double *f) {
-    if (!st.free_[a]) return false;
-    const double wq = 0.25 * st.W;
+  __device__ static bool rhs(const Args &A, const Pair &P, double *f) {
+    if (!P.fa) return false;
+    const double wq = 0.25 * P.W;
     for (int q = 0; q < 3; ++q) f[q] = wq * A.p.load[q];
     return true;
   }
@@ -196,8 +254,6 @@ struct TetOp {
     f[0] = f[1] = f[2] = 0.0;
   }
   __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
-  static constexpr bool kHasRhs = true;
-  static constexpr bool kExtra = true;
 };
 
 // scalar trilinear hex Poisson, corner a = dx + 2dy + 4dz
@@ -209,30 +265,44 @@ struct HexArgs {
 };
 struct HexOp {
   static constexpr int CARD = 8, BD = 1;
+  static constexpr bool kHasRhs = true, kExtra = false;
   using Args = HexArgs;
   struct State {
     int32_t vid[8];
-    int32_t free_[8];
+    uint32_t freemask;
+    double ke;
+  };
+  struct Pair {
+    int32_t vid[8];
+    uint32_t freemask;
+    int32_t a;
     double ke;
   };
   __device__ static void load(const Args &A, uint64_t e, State &st) {
+    st.freemask = 0;
     for (int a = 0; a < 8; ++a) {
       const int32_t v = A.ep[8 * e + a];
       st.vid[a] = v;
-      st.free_[a] = A.dir[v] <= 0;
+      st.freemask |= (A.dir[v] <= 0 ? 1u : 0u) << a;
     }
     st.ke = A.kappa[e] * A.p.h;
   }
-  __device__ static bool block(const Args &, const State &st, int a, int b, double *out) {
-    if (!st.free_[a] || !st.free_[b]) return false;
-    const int d = __popc((unsigned)(a ^ b));
+  __device__ static void pair(const Args &, const State &st, int a, Pair &P) {
+    for (int t = 0; t < 8; ++t) P.vid[t] = st.vid[t];
+    P.freemask = st.freemask;
+    P.a = a;
+    P.ke = st.ke;
+  }
+  __device__ static bool block(const Args &, const Pair &P, int b, double *out) {
+    if (!((P.freemask >> P.a) & 1u) || !((P.freemask >> b) & 1u)) return false;
+    const int d = __popc((unsigned)(P.a ^ b));
     const double kr = d == 0 ? (1.0 / 3.0) : (d == 1 ? 0.0 : -(1.0 / 12.0));
-    out[0] = st.ke * kr;
+    out[0] = P.ke * kr;
     return true;
   }
-  __device__ static bool diag_extra(const Args &, const State &, int, double *) { return false; }
-  __device__ static bool rhs(const Args &A, const State &st, int a, double *f) {
-    if (!st.free_[a]) return false;
+  __device__ static bool extra(const Args &, const Pair &, double *) { return false; }
+  __device__ static bool rhs(const Args &A, const Pair &P, double *f) {
+    if (!((P.freemask >> P.a) & 1u)) return false;
     f[0] = A.p.source * (A.p.h * A.p.h * A.p.h) * 0.125;
     return true;
   }
@@ -242,8 +312,6 @@ struct HexOp {
   }
   __device__ static void isolated_rhs(const Args &, uint32_t, double *f) { f[0] = 0.0; }
   __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
-  static constexpr bool kHasRhs = true;
-  static constexpr bool kExtra = false;
 };
 
 // one per-edge block written into the four endpoint blocks with signs, then
@@ -259,11 +327,15 @@ struct EdgeArgs {
 template <int BDIM>
 struct EdgeOp {
   static constexpr int CARD = 2, BD = BDIM;
-  static constexpr bool kHasRhs = false;
-  static constexpr bool kExtra = false;
+  static constexpr bool kHasRhs = false, kExtra = false;
   using Args = EdgeArgs;
   struct State {
     int32_t vid[2];
+    double w[BDIM * BDIM];
+  };
+  struct Pair {
+    int32_t vid[2];
+    int32_t a, pad;
     double w[BDIM * BDIM];
   };
   __device__ static void load(const Args &A, uint64_t e, State &st) {
@@ -271,13 +343,19 @@ struct EdgeOp {
     st.vid[1] = A.ep[2 * e + 1];
     for (int q = 0; q < BDIM * BDIM; ++q) st.w[q] = A.w[e * BDIM * BDIM + q];
   }
-  __device__ static bool block(const Args &A, const State &st, int a, int b, double *out) {
-    const double s = A.sign[2 * a + b];
-    for (int q = 0; q < BDIM * BDIM; ++q) out[q] = s * st.w[q];
+  __device__ static void pair(const Args &, const State &st, int a, Pair &P) {
+    P.vid[0] = st.vid[0];
+    P.vid[1] = st.vid[1];
+    P.a = a;
+    for (int q = 0; q < BDIM * BDIM; ++q) P.w[q] = st.w[q];
+  }
+  __device__ static bool block(const Args &A, const Pair &P, int b, double *out) {
+    const double s = A.sign[2 * P.a + b];
+    for (int q = 0; q < BDIM * BDIM; ++q) out[q] = s * P.w[q];
     return true;
   }
-  __device__ static bool diag_extra(const Args &, const State &, int, double *) { return false; }
-  __device__ static bool rhs(const Args &, const State &, int, double *) { return false; }
+  __device__ static bool extra(const Args &, const Pair &, double *) { return false; }
+  __device__ static bool rhs(const Args &, const Pair &, double *) { return false; }
   __device__ static void finish_diag(const Args &, uint32_t, double *, double *) {}
   __device__ static void isolated_rhs(const Args &, uint32_t, double *f) {
     for (int q = 0; q < BDIM; ++q) f[q] = 0.0;
@@ -312,16 +390,14 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
            const uint32_t *__restrict__ ve_ptr, const uint32_t *__restrict__ ve_elem,
            void *__restrict__ vals, void *__restrict__ vals_lo, double *__restrict__ rhs) {
   constexpr int BD = Op::BD, BB = BD * BD, CARD = Op::CARD;
-  constexpr int NE = Op::kExtra ? kWave * BB : 1;
-  // per staged (row, element) pair: its CARD blocks of the row, the lumped
-  // extra on the diagonal and the right-hand side; after the last chunk the
-  // block area is reused to stage the coalesced output stores
-  __shared__ double C[kWave * CARD * BB];
-  __shared__ double E[NE];
-  __shared__ double F[kWave * BD];
-  __shared__ int32_t V[kWave * CARD];
-  __shared__ uint32_t M[kWave];        // bits 0..CARD-1 blocks, CARD extra, CARD+1 rhs
-  __shared__ uint32_t se[kWave];       // element of each staged pair
+  using Pair = typename Op::Pair;
+  // per staged pair: its record and flags (bit 0: the row's slot was found);
+  // after the last chunk the record area stages the coalesced output stores
+  constexpr size_t kPairBytes = sizeof(Pair) * kWave;
+  constexpr size_t kOutBytes = sizeof(double) * kWave * BB;
+  __shared__ __align__(16) unsigned char smem[kPairBytes > kOutBytes ? kPairBytes : kOutBytes];
+  Pair *PP = reinterpret_cast<Pair *>(smem);
+  __shared__ uint32_t M[kWave];
   __shared__ uint32_t R[kWave + 1];    // rowptr window
   __shared__ uint32_t VP[kWave + 2];   // ve_ptr window of the wave's rows
 
@@ -337,11 +413,14 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
   if (lane == 0) R[kWave] = rowptr[min(rfirst + (uint32_t)kWave, nrows)];
   __syncthreads();
   uint32_t row;
-  const bool window = (uint64_t)R[kWave] > blast;  // the wave's rows are rfirst .. rfirst+63
-  if (window) {
-    uint32_t c = 0;
-    for (int l = 1; l <= kWave; ++l) c += ((uint64_t)R[l] <= bk && rfirst + l <= nrows) ? 1u : 0u;
-    row = rfirst + c;
+  if ((uint64_t)R[kWave] > blast) {
+    // the wave's rows lie in rfirst .. rfirst+63: last l with R[l] <= bk
+    int lo = 0, hi = kWave + 1;
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if ((uint64_t)R[mid] <= bk) lo = mid; else hi = mid;
+    }
+    row = rfirst + (uint32_t)lo;
     if (row >= nrows) row = nrows - 1;
   } else {
     uint32_t lo = rfirst, hi = nrows;
@@ -370,16 +449,20 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
   const uint32_t P1 = vp_lds ? VP[nwr] : ve_ptr[rlast + 1];
   const uint32_t myb = valid ? ve_ptr[row] : 0, mye = valid ? ve_ptr[row + 1] : 0;
   for (uint32_t c0 = P0; c0 < P1; c0 += kWave) {
-    // phase A: lane l evaluates pair p = c0 + l: all blocks of its row
+    // phase A: lane l builds the record of pair p = c0 + l
     const uint32_t p = c0 + lane;
-    uint32_t mask = 0;
+    uint32_t ok = 0;
     if (p < P1) {
       const uint32_t e = ve_elem[p];
-      se[lane] = e;
       // the row this pair belongs to: last r with ve_ptr[r] <= p
-      uint32_t pr = rfirst;
+      uint32_t pr;
       if (vp_lds) {
-        for (uint32_t l = 1; l < nwr; ++l) pr += (VP[l] <= p) ? 1u : 0u;
+        int lo = 0, hi = (int)nwr;
+        while (hi - lo > 1) {
+          const int mid = (lo + hi) >> 1;
+          if (VP[mid] <= p) lo = mid; else hi = mid;
+        }
+        pr = rfirst + (uint32_t)lo;
       } else {
         uint32_t lo = rfirst, hi = rlast + 1;
         while (hi - lo > 1) {
@@ -396,43 +479,35 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
       typename Op::State st;
       Op::load(A, e, st);
       int a = -1, seen = 0;
-      for (int t = 0; t < CARD; ++t) {
-        V[lane * CARD + t] = st.vid[t];
+      for (int t = 0; t < CARD; ++t)
         if ((uint32_t)st.vid[t] == pr && a < 0) {
           if (seen == occ) a = t;
           ++seen;
         }
-      }
-      // slot a is dispatched to compile-time copies so that the element
-      // state stays in registers (a dynamic index would spill it)
-#pragma unroll
-      for (int aa = 0; aa < CARD; ++aa) {
-        if (aa != a) continue;
-#pragma unroll
-        for (int b = 0; b < CARD; ++b)
-          if (Op::block(A, st, aa, b, C + (lane * CARD + b) * BB)) mask |= 1u << b;
-        if (Op::kExtra && Op::diag_extra(A, st, aa, E + (Op::kExtra ? lane * BB : 0)))
-          mask |= 1u << CARD;
-        if (Op::rhs(A, st, aa, F + lane * BD)) mask |= 1u << (CARD + 1);
+      if (a >= 0) {
+        Op::pair(A, st, a, PP[lane]);
+        ok = 1;
       }
     }
-    M[lane] = mask;
+    M[lane] = ok;
     __syncthreads();
     // phase B: every block lane adds its contributions in list order
     if (valid) {
       const uint32_t lo = max(myb, c0), hi = min(mye, c0 + kWave);
       for (uint32_t q = lo; q < hi; ++q) {
         const uint32_t l = q - c0;
-        const uint32_t m = M[l];
-        if (!m) continue;
+        if (!M[l]) continue;
+        const Pair &P = PP[l];
+        double blk[BB];
         for (int b = 0; b < CARD; ++b)
-          if ((m >> b & 1u) && (uint32_t)V[l * CARD + b] == col)
-            for (int k = 0; k < BB; ++k) acc[k] += C[(l * CARD + b) * BB + k];
+          if ((uint32_t)P.vid[b] == col && Op::block(A, P, b, blk))
+            for (int k = 0; k < BB; ++k) acc[k] += blk[k];
         if (is_diag) {
-          if (Op::kExtra && (m >> CARD & 1u))
-            for (int k = 0; k < BB; ++k) acc[k] += E[Op::kExtra ? l * BB + k : 0];
-          if (m >> (CARD + 1) & 1u)
-            for (int k = 0; k < BD; ++k) f[k] += F[l * BD + k];
+          if (Op::kExtra && Op::extra(A, P, blk))
+            for (int k = 0; k < BB; ++k) acc[k] += blk[k];
+          double fr[BD];
+          if (Op::rhs(A, P, fr))
+            for (int k = 0; k < BD; ++k) f[k] += fr[k];
         }
       }
     }
@@ -445,7 +520,7 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
       for (int q = 0; q < BD; ++q) rhs[(uint64_t)row * BD + q] = f[q];
   }
   // coalesced store through LDS
-  double *outbuf = C;
+  double *outbuf = reinterpret_cast<double *>(smem);
   for (int q = 0; q < BB; ++q) outbuf[lane * BB + q] = acc[q];
   __syncthreads();
   const uint64_t nval = (min(b0 + kWave, nnz) - b0) * BB;
