@@ -20,6 +20,9 @@
 // with no fused multiply-add: bit-identical to the CPU loop for SMH_F64.  The
 // 64 results are staged through LDS so the stores are coalesced.  No atomics,
 // so repeated runs are bitwise identical.
+#include <algorithm>
+#include <cstdlib>
+
 #include "smh_internal.h"
 
 #pragma clang fp contract(off)
@@ -388,7 +391,8 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
            const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ colidx,
            const uint32_t *__restrict__ wave_row,
            const uint32_t *__restrict__ ve_ptr, const uint32_t *__restrict__ ve_elem,
-           void *__restrict__ vals, void *__restrict__ vals_lo, double *__restrict__ rhs) {
+           void *__restrict__ vals, void *__restrict__ vals_lo, double *__restrict__ rhs,
+           uint64_t nwaves) {
   constexpr int BD = Op::BD, BB = BD * BD, CARD = Op::CARD;
   using Pair = typename Op::Pair;
   // per staged pair: its record and flags (bit 0: the row's slot was found);
@@ -402,13 +406,20 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
   __shared__ uint32_t VP[kWave + 2];   // ve_ptr window of the wave's rows
 
   const int lane = threadIdx.x;
-  const uint64_t b0 = (uint64_t)blockIdx.x * kWave;
+  // Persistent grid (a multiple of 8 workgroups).  Workgroups are dealt to
+  // the 8 XCDs round-robin; XCD x works through the x-th eighth of the waves,
+  // so neighbouring waves, which gather the same vertices and elements, share
+  // that XCD's L2.
+  const uint32_t xcd = blockIdx.x & 7u, per = gridDim.x >> 3;
+  const uint64_t w1 = nwaves * (xcd + 1) / 8;
+  for (uint64_t w = nwaves * xcd / 8 + (blockIdx.x >> 3); w < w1; w += per) {
+  const uint64_t b0 = w * kWave;
   const uint64_t bk = b0 + lane;
   const bool valid = bk < nnz;
   const uint64_t blast = min(b0 + kWave, nnz) - 1;
 
   // row of b0 (last r with rowptr[r] <= b0), precomputed with the index
-  const uint32_t rfirst = wave_row[blockIdx.x];
+  const uint32_t rfirst = wave_row[w];
   R[lane] = rowptr[min(rfirst + (uint32_t)lane, nrows)];
   if (lane == 0) R[kWave] = rowptr[min(rfirst + (uint32_t)kWave, nrows)];
   __syncthreads();
@@ -499,9 +510,17 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
         if (!M[l]) continue;
         const Pair &P = PP[l];
         double blk[BB];
-        for (int b = 0; b < CARD; ++b)
-          if ((uint32_t)P.vid[b] == col && Op::block(A, P, b, blk))
+        // slots of the element that hold this lane's column, ascending (the
+        // element function's b order); normally exactly one, so the wave
+        // forms one block per pair instead of one per slot
+        uint32_t match = 0;
+        for (int b = 0; b < CARD; ++b) match |= ((uint32_t)P.vid[b] == col ? 1u : 0u) << b;
+        while (match) {
+          const int b = __builtin_ctz(match);
+          match &= match - 1;
+          if (Op::block(A, P, b, blk))
             for (int k = 0; k < BB; ++k) acc[k] += blk[k];
+        }
         if (is_diag) {
           if (Op::kExtra && Op::extra(A, P, blk))
             for (int k = 0; k < BB; ++k) acc[k] += blk[k];
@@ -528,6 +547,8 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
     const uint64_t i = (uint64_t)q * kWave + lane;
     if (i < nval) store_val<PREC>(vals, vals_lo, b0 * BB + i, outbuf[i]);
   }
+  __syncthreads();  // LDS is reused by the next wave of blocks
+  }
 }
 
 // right-hand side of vertices that belong to no element (empty rows)
@@ -539,6 +560,28 @@ __global__ void k_isolated(typename Op::Args A, uint32_t nrows, const uint32_t *
   double f[Op::BD];
   Op::isolated_rhs(A, i, f);
   for (int q = 0; q < Op::BD; ++q) rhs[(uint64_t)i * Op::BD + q] = f[q];
+}
+
+// Resident workgroups per CU for the persistent assembly grid: the
+// occupancy limit of the instantiation (registers, LDS), or
+// SMH_ASSEMBLY_WGS_PER_CU when set (for tuning).
+template <class Op, int P>
+unsigned assembly_wgs_per_cu() {
+  static const unsigned n = [] {
+    if (const char *e = getenv("SMH_ASSEMBLY_WGS_PER_CU")) {
+      const int v = atoi(e);
+      if (v > 0) return (unsigned)v;
+    }
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_assemble<Op, P>, kWave, 0) !=
+            hipSuccess ||
+        occ < 1) {
+      (void)hipGetLastError();
+      occ = 4;
+    }
+    return (unsigned)occ;
+  }();
+  return n;
 }
 
 template <class Op>
@@ -560,13 +603,19 @@ int launch_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, const
             "%s: too many values", name);
   const uint32_t nrows = idx->nrows;
   if (nrows == 0) return SMH_OK;
-  const unsigned nb = (unsigned)((idx->nnz + kWave - 1) / kWave);
-  if (nb) {
+  const uint64_t nwaves = (idx->nnz + kWave - 1) / kWave;
+  if (nwaves) {
+    // grid: a multiple of 8 (one share per XCD), no more than fills the chip
+    auto grid = [&](unsigned per_cu) {
+      const uint64_t full = (uint64_t)ctx->num_cu * per_cu;
+      const uint64_t g = std::min<uint64_t>(full, nwaves);
+      return (unsigned)((g + 7) / 8 * 8);
+    };
 #define SMH_ASM(P)                                                                          \
-  hipLaunchKernelGGL((k_assemble<Op, P>), dim3(nb), dim3(kWave), 0, ctx->stream, A, nrows,  \
-                     idx->nnz, idx->rowptr, idx->colidx, idx->wave_row, idx->ve_ptr[set],      \
-                     idx->ve_elem[set],                                                        \
-                     vals, vals_lo, rhs)
+  hipLaunchKernelGGL((k_assemble<Op, P>), dim3(grid(assembly_wgs_per_cu<Op, P>())),         \
+                     dim3(kWave), 0, ctx->stream, A, nrows, idx->nnz, idx->rowptr,          \
+                     idx->colidx, idx->wave_row, idx->ve_ptr[set], idx->ve_elem[set], vals, \
+                     vals_lo, rhs, nwaves)
     if (prec == SMH_F64) SMH_ASM(SMH_F64);
     else if (prec == SMH_F32) SMH_ASM(SMH_F32);
     else SMH_ASM(SMH_F32X2);

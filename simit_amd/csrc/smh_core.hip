@@ -188,6 +188,10 @@ extern "C" int smh_ctx_create(int device, smh_ctx **out) {
   if (hipSetDevice(device) != hipSuccess) return SMH_ERR_HIP;
   smh_ctx *c = new smh_ctx();
   c->device = device;
+  int ncu = 0;
+  if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
+      ncu > 0)
+    c->num_cu = ncu;
   if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess) {
     delete c;
     return SMH_ERR_HIP;

@@ -26,6 +26,7 @@ struct smh_tile_plan {
 
 struct smh_ctx {
   int device = 0;
+  int num_cu = 256;  // compute units of the device (persistent grids)
   hipStream_t own_stream = nullptr;
   hipStream_t stream = nullptr;
   std::string last_error;
