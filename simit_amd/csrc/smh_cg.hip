@@ -170,7 +170,7 @@ k_bsrmv_tiles(TileDev A, const typename Val<PREC>::T *__restrict__ x,
 // ---- CG device state ------------------------------------------------------------
 struct CGState {
   double rz, pq, alpha, beta, rr, rr_true, thr, rr_ref, bb, delta2;
-  int iter, done, status, want_ru, claimed, maxiters, precond_stop, mixed;
+  int iter, done, status, want_ru, claimed, maxiters, precond_stop, mixed, restart;
 };
 
 __device__ __forceinline__ bool finite_pos(double v) { return v > 0.0 && v < INFINITY; }
@@ -386,6 +386,12 @@ __global__ void __launch_bounds__(256) k_cg_rres(CGState *S, const double *part,
   if (threadIdx.x) return;
   S->rr_true = rr;
   if (certify_only) return;
+  // Keeping the search direction is only sound while the replaced residual
+  // is close to the iterated one.  After a false convergence claim, or when
+  // the true residual is more than 10x the iterated one, restart from p = z:
+  // an old direction scaled for a far smaller residual would make the next
+  // alpha = z'r / p'Ap huge and throw x off.
+  S->restart = (S->claimed || !(rr <= 100.0 * S->rr)) ? 1 : 0;
   S->rz = rz;
   S->rr = rr;
   S->rr_ref = rr;
@@ -399,6 +405,16 @@ __global__ void __launch_bounds__(256) k_cg_rres(CGState *S, const double *part,
   } else if (S->status == SMH_OK) {
     S->done = S->iter >= S->maxiters ? 1 : 0;
   }
+}
+
+// p = z after a residual replacement that asked for a restart
+template <typename VT>
+__global__ void __launch_bounds__(256)
+k_cg_restart(uint64_t n, VT *__restrict__ p, const VT *__restrict__ z, const CGState *S) {
+  if (!S->restart) return;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    p[i] = z[i];
 }
 
 // Jacobi / block-Jacobi inverse of the diagonal (thermal.sim:229-233 and
@@ -577,6 +593,8 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
                          (const double *)x, epi, w.part);
       hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
                          (const double *)w.part, (int)grt, 0);
+      hipLaunchKernelGGL((k_cg_restart<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
+                         (const VT *)w.z, (const CGState *)w.S);
       LCHECK();
       continue;
     }

@@ -119,6 +119,22 @@ def test_tets_fp64_bit_exact(ctx, oracle, kw):
     np.testing.assert_array_equal(rhs.cpu().numpy(), of)
 
 
+def test_tets_full_size_bit_exact(ctx, oracle):
+    # the 96^3 x 5 workload: 182,750 waves of blocks, so every persistent
+    # workgroup walks many of them
+    prob = meshes.tet_problem(96, seed=0, field="grf", mean_log10=5.0, std_log10=1.0,
+                              corr_len=8.0)
+    po, pp = tet_params(oracle, load=prob["load"])
+    idx, vals, _, rhs = run_tets(ctx, prob, S.F64, pp)
+    rp, ci = idx.to_host()
+    assert idx.nnz == 11695969
+    ov, of = oracle.tet_assemble(po, prob, rp, ci)
+    got = vals.cpu().numpy()
+    bad = np.nonzero(got != ov)[0]
+    assert bad.size == 0, f"{bad.size} values differ, first at {bad[:8]}"
+    np.testing.assert_array_equal(rhs.cpu().numpy(), of)
+
+
 def test_tets_hilo_and_determinism(ctx, oracle):
     prob = meshes.tet_problem(12, seed=5, field="grf", std_log10=1.0, corr_len=3.0)
     po, pp = tet_params(oracle)
