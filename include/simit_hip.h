@@ -250,7 +250,8 @@ int smh_block_jacobi(smh_ctx *ctx, const smh_bsr *A, double *d_dinv);
 #define SMH_PRECOND_NONE 0
 #define SMH_PRECOND_JACOBI 1       /* scalar 1/A(i,i)                        */
 #define SMH_PRECOND_BLOCK_JACOBI 2 /* 3x3 inverse of the diagonal block      */
-#define SMH_PRECOND_LEARNED 3      /* block Jacobi + learned correction      */
+#define SMH_PRECOND_LEARNED 3      /* block Jacobi + learned block-sparse
+                                      correction (smh_gnn_precond)          */
 typedef struct {
   int32_t maxiters;
   double tol;
@@ -260,14 +261,20 @@ typedef struct {
   int32_t precond;
   int32_t mixed;         /* 1: FP32 inner iteration on vals (hi part) with
                             FP64 reliable updates r = b - A x on hi+lo
-                            (needs SMH_F32X2)                               */
+                            (needs SMH_F32X2); 2: the same with the inner
+                            vectors (p, q, r, z) kept in FP64            */
   double reliable_delta; /* mixed: update when ||r|| < delta*max ||r||      */
   int32_t check_every;   /* iterations between host convergence checks     */
-  const float *learned;  /* SMH_PRECOND_LEARNED: device float[nrows*9]
-                            symmetric corrections C_i, applied as
-                            z_i = (Dinv_i + C_i) r_i                        */
+  const float *learned;  /* SMH_PRECOND_LEARNED: device float[nrows*9],
+                            Linv_i = L_i^-1 for A_ii = L_i L_i^T (row-major,
+                            lower triangular), as smh_gnn_precond writes */
   int32_t zr_stop;       /* 1: also stop when z'r <= the threshold, as
                             thermal.sim's pre_conjgrad does (:115)          */
+  const uint16_t *learned_corr; /* SMH_PRECOND_LEARNED: device bfloat16
+                            [nnz*9], the normalised correction blocks C_k
+                            on A's pattern (zero on the diagonal); the
+                            preconditioner is
+                            z_i = Linv_i^T (Linv_i r_i + sum_k C_k Linv_j r_j) */
 } smh_cg_opts;
 typedef struct {
   int32_t iters;
@@ -280,6 +287,39 @@ typedef struct {
 void smh_cg_default_opts(smh_cg_opts *o);
 int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
                  const smh_cg_opts *opts, smh_cg_stats *stats);
+
+/* ---- learned block-Jacobi correction (no counterpart in the reference) ----
+ * A message-passing network over the matrix graph (3 layers, width 32, BF16
+ * weights and MFMA inputs, FP32 accumulation) predicts a symmetric correction
+ * C to the block-Jacobi preconditioner on A's own sparsity pattern, in the
+ * block-Jacobi-normalised space Ahat = Linv A Linv^T.  C is scaled so that
+ * every row's off-diagonal block norms sum below kappa < 1, so I + C and the
+ * preconditioner Linv^T (I + C) Linv are symmetric positive definite.
+ * Training minimises the stochastic Frobenius loss
+ *   L = sum (V - (I + C) Ahat V)^2 / (K * ndof)
+ * over K Gaussian probe vectors V, with Adam, entirely in the library's own
+ * forward and backward kernels.  A must be 3x3 blocks with a symmetric
+ * pattern whose rows hold their diagonal block. */
+typedef struct smh_gnn smh_gnn;
+#define SMH_GNN_NPARAMS 18192
+int smh_gnn_create(smh_ctx *ctx, uint64_t seed, double kappa, smh_gnn **out);
+int smh_gnn_destroy(smh_ctx *ctx, smh_gnn *g);
+/* flat float32 parameters: W_ev[16x32] b_ev W_ee[16x32] b_ee, per layer
+ * W_m[96x32] b_m W_u[64x32] b_u, then W_d[96x16] b_d (row-major, in x out) */
+int smh_gnn_get_params(smh_ctx *ctx, smh_gnn *g, float *host);
+int smh_gnn_set_params(smh_ctx *ctx, smh_gnn *g, const float *host);
+/* the preconditioner of A: linv = device float[nrows*9], corr = device
+ * bfloat16 bits [nnz*9] (pass both as opts.learned / opts.learned_corr) */
+int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float *linv, uint16_t *corr);
+/* loss and gradient on A for nprobes probe vectors: probes = device
+ * float[3*nrows*nprobes] (row-major [dof][probe]) or NULL to draw them from
+ * seed; grads (host float[SMH_GNN_NPARAMS]) may be NULL */
+int smh_gnn_loss_grad(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int32_t nprobes,
+                      const float *probes, uint64_t seed, double *loss, float *grads);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on A with fresh probes;
+ * loss (host) may be NULL, which avoids a synchronisation */
+int smh_gnn_train_step(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int32_t nprobes,
+                       uint64_t seed, double lr, double *loss);
 
 #ifdef __cplusplus
 }

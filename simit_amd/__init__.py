@@ -98,7 +98,8 @@ class CGOptions(C.Structure):
                 ("norm_form", C.c_int32), ("replace_every", C.c_int32),
                 ("precond", C.c_int32), ("mixed", C.c_int32),
                 ("reliable_delta", C.c_double), ("check_every", C.c_int32),
-                ("learned", C.c_void_p), ("zr_stop", C.c_int32)]
+                ("learned", C.c_void_p), ("zr_stop", C.c_int32),
+                ("learned_corr", C.c_void_p)]
 
 
 class CGStats(C.Structure):
@@ -156,6 +157,15 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_cg_default_opts": (None, [C.POINTER(CGOptions)]),
         "smh_cg_solve": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, C.POINTER(CGOptions),
                                    C.POINTER(CGStats)]),
+        "smh_gnn_create": (C.c_int, [vp, u64, C.c_double, C.POINTER(vp)]),
+        "smh_gnn_destroy": (C.c_int, [vp, vp]),
+        "smh_gnn_get_params": (C.c_int, [vp, vp, vp]),
+        "smh_gnn_set_params": (C.c_int, [vp, vp, vp]),
+        "smh_gnn_precond": (C.c_int, [vp, vp, C.POINTER(Bsr), vp, vp]),
+        "smh_gnn_loss_grad": (C.c_int, [vp, vp, C.POINTER(Bsr), i32, vp, u64,
+                                        C.POINTER(C.c_double), vp]),
+        "smh_gnn_train_step": (C.c_int, [vp, vp, C.POINTER(Bsr), i32, u64, C.c_double,
+                                         C.POINTER(C.c_double)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -284,10 +294,78 @@ class Context:
         o = CGOptions()
         self.lib.smh_cg_default_opts(C.byref(o))
         for k, v in kw.items():
-            if k == "learned":
+            if k in ("learned", "learned_corr"):
                 v = _ptr(v)
             setattr(o, k, v)
         return o
+
+    def gnn(self, seed: int = 0, kappa: float = 0.9) -> "Gnn":
+        return Gnn(self, seed, kappa)
+
+
+GNN_NPARAMS = 18192
+
+
+class Gnn:
+    """The learned block-Jacobi correction network (smh_gnn)."""
+
+    def __init__(self, ctx: Context, seed: int = 0, kappa: float = 0.9):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx._check(ctx.lib.smh_gnn_create(ctx.h, seed, kappa, C.byref(h)))
+        self.h = h
+        self.kappa = kappa
+
+    def get_params(self):
+        import numpy as np
+        out = np.empty(GNN_NPARAMS, dtype=np.float32)
+        self.ctx._check(self.ctx.lib.smh_gnn_get_params(self.ctx.h, self.h,
+                                                       out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    def set_params(self, p):
+        import numpy as np
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        assert a.size == GNN_NPARAMS
+        self.ctx._check(self.ctx.lib.smh_gnn_set_params(self.ctx.h, self.h,
+                                                       a.ctypes.data_as(C.c_void_p)))
+
+    def precond(self, A: Bsr):
+        """(linv float32[nrows*9], corr bfloat16-bits int16[nnz*9]) device tensors."""
+        import torch
+        linv = torch.empty(A.nrows * 9, dtype=torch.float32, device="cuda")
+        corr = torch.empty(A.nnz * 9, dtype=torch.int16, device="cuda")
+        self.ctx._check(self.ctx.lib.smh_gnn_precond(self.ctx.h, self.h, C.byref(A),
+                                                    _ptr(linv), _ptr(corr)))
+        return linv, corr
+
+    def loss_grad(self, A: Bsr, nprobes: int = 4, probes=None, seed: int = 0, grads=True):
+        import numpy as np
+        loss = C.c_double()
+        g = np.empty(GNN_NPARAMS, dtype=np.float32) if grads else None
+        self.ctx._check(self.ctx.lib.smh_gnn_loss_grad(
+            self.ctx.h, self.h, C.byref(A), nprobes, _ptr(probes), seed, C.byref(loss),
+            g.ctypes.data_as(C.c_void_p) if g is not None else None))
+        return loss.value, g
+
+    def train_step(self, A: Bsr, nprobes: int = 4, seed: int = 0, lr: float = 1e-3,
+                   want_loss: bool = True):
+        loss = C.c_double()
+        self.ctx._check(self.ctx.lib.smh_gnn_train_step(
+            self.ctx.h, self.h, C.byref(A), nprobes, seed, lr,
+            C.byref(loss) if want_loss else None))
+        return loss.value if want_loss else None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.smh_gnn_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Index:

@@ -17,7 +17,7 @@ LIB = LIBDIR / "libsimit_hip.so"
 HOST_LIB = LIBDIR / "libsimit.so"
 PROGRAM_TESTS = BINDIR / "simit_program_tests"
 
-HIP_SOURCES = ["smh_core.hip", "smh_index.hip", "smh_assemble.hip", "smh_cg.hip"]
+HIP_SOURCES = ["smh_core.hip", "smh_index.hip", "smh_assemble.hip", "smh_cg.hip", "smh_gnn.hip"]
 HOST_SOURCES = ["simit/graph.cpp", "simit/program.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -187,6 +187,15 @@ __device__ __forceinline__ void apply_precond(int pc, const double *r, double *z
                                               const void *dinv, bool f32, uint64_t i) {
   if (pc == SMH_PRECOND_NONE) {
     for (int a = 0; a < BD; ++a) z[a] = r[a];
+  } else if (BD == 3 && pc == SMH_PRECOND_LEARNED) {
+    // first half of the learned preconditioner: u = Linv r (the rest,
+    // z = Linv^T (u + C u), needs the neighbours' u: k_cg_lapply)
+    const float *L = (const float *)dinv + i * 9;
+    for (int a = 0; a < 3; ++a) {
+      double s = 0.0;
+      for (int b = 0; b <= a; ++b) s += (double)L[3 * a + b] * r[b];
+      z[a] = s;
+    }
   } else if (BD == 1 || pc == SMH_PRECOND_JACOBI) {
     for (int a = 0; a < BD; ++a)
       z[a] = r[a] * (f32 ? (double)((const float *)dinv)[i * BD + a]
@@ -280,6 +289,54 @@ k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT
   }
   const double t1 = block_sum256(drz, s4);
   const double t2 = block_sum256(drr, s4);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t1;
+    part[2 * blockIdx.x + 1] = t2;
+  }
+}
+
+// K2b (learned preconditioner): z_i = Linv_i^T (u_i + sum_k C_k u_j) over
+// the pattern of A (C in BF16, zero on the diagonal), and the workgroup
+// partials of z'r and r'r.  check_done = 0 on the residual-replacement path,
+// which runs after the stopping test already set done.
+template <typename VT>
+struct EpiLearned {
+  const VT *u, *r;
+  VT *z;
+  const float *linv;
+  double drz, drr;
+  __device__ void operator()(uint32_t row, const VT *s) {
+    double y[3];
+    for (int a = 0; a < 3; ++a) y[a] = (double)u[(uint64_t)row * 3 + a] + (double)s[a];
+    const float *L = linv + (uint64_t)row * 9;
+    for (int a = 0; a < 3; ++a) {
+      double t = 0.0;
+      for (int b = a; b < 3; ++b) t += (double)L[3 * b + a] * y[b];
+      const uint64_t o = (uint64_t)row * 3 + a;
+      const VT zz = (VT)t;
+      z[o] = zz;
+      const double rv = (double)r[o];
+      drz += (double)zz * rv;
+      drr += rv * rv;
+    }
+  }
+};
+
+template <int LP>
+__global__ void __launch_bounds__(TileShape<3>::kThreads)
+k_cg_lapply(TileDev C, const typename Val<LP>::T *__restrict__ u,
+            const typename Val<LP>::T *__restrict__ r, typename Val<LP>::T *__restrict__ z,
+            const float *__restrict__ linv, const CGState *S, int check_done, double *part) {
+  using VT = typename Val<LP>::T;
+  __shared__ TileSmem<LP, 3> sm;
+  if (check_done && S->done) return;
+  uint32_t k0, k1;
+  wg_tiles(C, k0, k1);
+  XPlain<VT> xg{u, 3};
+  EpiLearned<VT> epi{u, r, z, linv, 0.0, 0.0};
+  spmv_tiles<LP, 3>(C, k0, k1, sm, xg, epi);
+  const double t1 = block_sum<TileShape<3>::kWaves>(epi.drz, sm.red);
+  const double t2 = block_sum<TileShape<3>::kWaves>(epi.drr, sm.red);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = t1;
     part[2 * blockIdx.x + 1] = t2;
@@ -418,12 +475,11 @@ k_cg_restart(uint64_t n, VT *__restrict__ p, const VT *__restrict__ z, const CGS
 }
 
 // Jacobi / block-Jacobi inverse of the diagonal (thermal.sim:229-233 and
-// inv3 of runtime.cpp:135-160), from the matrix in its best precision,
-// plus an optional learned symmetric correction per block.
+// inv3 of runtime.cpp:135-160), from the matrix in its best precision.
 template <int RP, typename OT>
 __global__ void k_dinv(uint32_t nrows, const uint32_t *__restrict__ rowptr,
                        const uint32_t *__restrict__ colidx, const void *vals, const void *vals_lo,
-                       int bd, int pc, const float *learned, OT *out) {
+                       int bd, int pc, OT *out) {
   using S = typename Val<RP>::S;
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows) return;
@@ -455,11 +511,7 @@ __global__ void k_dinv(uint32_t nrows, const uint32_t *__restrict__ rowptr,
   inv[0] = c00 * det; inv[1] = c10 * det; inv[2] = c20 * det;
   inv[3] = c01 * det; inv[4] = c11 * det; inv[5] = c21 * det;
   inv[6] = c02 * det; inv[7] = c12 * det; inv[8] = c22 * det;
-  for (int q = 0; q < 9; ++q) {
-    double v = inv[q];
-    if (pc == SMH_PRECOND_LEARNED && learned) v += (double)learned[(uint64_t)i * 9 + q];
-    out[(uint64_t)i * 9 + q] = (OT)v;
-  }
+  for (int q = 0; q < 9; ++q) out[(uint64_t)i * 9 + q] = (OT)inv[q];
 }
 
 __global__ void k_cg_init_state(CGState *S, double tol, int rel, int norm_form, int maxiters,
@@ -507,11 +559,11 @@ TileDev tile_dev(const smh_bsr *A, const smh_tile_plan *p, unsigned *grid) {
 
 // ---- host driver ---------------------------------------------------------------
 struct Work {
-  void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr;
+  void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *u = nullptr;
   CGState *S = nullptr;
   double *part = nullptr;
   void release() {
-    dfree(r); dfree(z); dfree(p); dfree(q); dfree(dinv); dfree(S); dfree(part);
+    dfree(r); dfree(z); dfree(p); dfree(q); dfree(dinv); dfree(u); dfree(S); dfree(part);
   }
 };
 
@@ -529,7 +581,15 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   const unsigned nbe = grid_for(nrows, 256, 2048);
   const unsigned nbv = grid_for(n, 256, 2048);
   const int pc = o->precond;
+  const bool learned = BD == 3 && pc == SMH_PRECOND_LEARNED;
+  constexpr int LP = sizeof(VT) == 4 ? kBF16F : kBF16D;
+  smh_bsr Cb = *Am;
+  Cb.vals = learned ? (const void *)o->learned_corr : Am->vals;
+  Cb.vals_lo = nullptr;
+  unsigned gc = 0;
+  const TileDev TC = tile_dev(&Cb, plan, &gc);
   const int check = o->check_every > 0 ? o->check_every : 16;
+  const void *pdata = nullptr;  // the preconditioner data K2 reads
   Work w;
   int rc = SMH_OK;
   auto T0 = std::chrono::steady_clock::now();
@@ -553,7 +613,13 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   TRY(dmalloc(ctx, &w.z, sizeof(VT) * n));
   TRY(dmalloc(ctx, &w.p, sizeof(VT) * n));
   TRY(dmalloc(ctx, &w.q, sizeof(VT) * n));
-  if (pc != SMH_PRECOND_NONE) TRY(dmalloc(ctx, &w.dinv, sizeof(VT) * (uint64_t)nrows * BD * BD));
+  if (pc != SMH_PRECOND_NONE && !learned)
+    TRY(dmalloc(ctx, &w.dinv, sizeof(VT) * (uint64_t)nrows * BD * BD));
+  if (learned) {
+    TRY(dmalloc(ctx, &w.u, sizeof(VT) * n));
+    w.dinv = nullptr;
+  }
+  pdata = learned ? (const void *)o->learned : (const void *)w.dinv;
   TRY(dmalloc(ctx, (void **)&w.S, sizeof(CGState)));
   TRY(dmalloc(ctx, (void **)&w.part, sizeof(double) * 2 * kPartSlots + 64));
   // b'b, state, preconditioner
@@ -563,22 +629,32 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(1), 0, ctx->stream, w.S, o->tol, o->rel,
                      o->norm_form, o->maxiters, (pc != SMH_PRECOND_NONE && o->zr_stop) ? 1 : 0,
                      o->reliable_delta, o->mixed, (const double *)(w.part + 2 * kPartSlots));
-  if (pc != SMH_PRECOND_NONE)
+  if (pc != SMH_PRECOND_NONE && !learned)
     hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream,
-                       nrows, Am->rowptr, Am->colidx, Am->vals, Am->vals_lo, BD, pc, o->learned,
+                       nrows, Am->rowptr, Am->colidx, Am->vals, Am->vals_lo, BD, pc,
                        (VT *)w.dinv);
   epi.b = b;
   epi.r = (VT *)w.r;
-  epi.z = (VT *)w.z;
+  epi.z = (VT *)(learned ? w.u : w.z);  // learned: u = Linv r, z follows in k_cg_lapply
   epi.p = (VT *)w.p;
-  epi.dinv = w.dinv;
+  epi.dinv = pdata;
   epi.pc = pc;
   // r = b - A x, z = M r, p = z
-  epi.init = 1;
+  epi.init = learned ? 0 : 1;
   hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
                      (const double *)x, epi, w.part);
-  hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
-                     (int)grt, 0);
+  if (learned) {
+    hipLaunchKernelGGL((k_cg_lapply<LP>), dim3(gc), dim3(tile_threads_host(3)), 0, ctx->stream, TC,
+                       (const VT *)w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
+                       (const CGState *)w.S, 0, w.part);
+    hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
+                       (const double *)w.part, (int)gc, 0);
+    hipLaunchKernelGGL((k_cg_restart<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
+                       (const VT *)w.z, (const CGState *)w.S);
+  } else {
+    hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
+                       (const double *)w.part, (int)grt, 0);
+  }
   LCHECK();
   epi.init = 0;
   for (;;) {
@@ -591,8 +667,12 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
       last_rep = hs.iter;
       hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
                          (const double *)x, epi, w.part);
+      if (learned)
+        hipLaunchKernelGGL((k_cg_lapply<LP>), dim3(gc), dim3(tile_threads_host(3)), 0,
+                           ctx->stream, TC, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
+                           (const float *)o->learned, (const CGState *)w.S, 0, w.part);
       hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
-                         (const double *)w.part, (int)grt, 0);
+                         (const double *)w.part, (int)(learned ? gc : grt), 0);
       hipLaunchKernelGGL((k_cg_restart<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
                          (const VT *)w.z, (const CGState *)w.S);
       LCHECK();
@@ -608,10 +688,15 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
       hipLaunchKernelGGL(k_cg_r1, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
                          (int)gt);
       hipLaunchKernelGGL((k_cg_update<VT, BD>), dim3(nbe), dim3(256), 0, ctx->stream, nrows, pc,
-                         (const void *)w.dinv, x, (VT *)w.r, (VT *)w.z, (const VT *)w.p,
+                         pdata, x, (VT *)w.r, (VT *)(learned ? w.u : w.z), (const VT *)w.p,
                          (const VT *)w.q, (const CGState *)w.S, w.part + kPartSlots);
+      if (learned)
+        hipLaunchKernelGGL((k_cg_lapply<LP>), dim3(gc), dim3(tile_threads_host(3)), 0,
+                           ctx->stream, TC, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
+                           (const float *)o->learned, (const CGState *)w.S, 1,
+                           w.part + kPartSlots);
       hipLaunchKernelGGL(k_cg_r2, dim3(1), dim3(256), 0, ctx->stream, w.S,
-                         (const double *)(w.part + kPartSlots), (int)nbe);
+                         (const double *)(w.part + kPartSlots), (int)(learned ? gc : nbe));
       hipLaunchKernelGGL((k_cg_pupd<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
                          (const VT *)w.z, (const CGState *)w.S);
     }
@@ -710,15 +795,15 @@ extern "C" int smh_block_jacobi(smh_ctx *ctx, const smh_bsr *A, double *d_dinv) 
   if (A->precision == SMH_F64)
     hipLaunchKernelGGL((k_dinv<SMH_F64, double>), dim3(g), dim3(256), 0, ctx->stream, A->nrows,
                        A->rowptr, A->colidx, A->vals, A->vals_lo, (int)A->bdim, pc,
-                       (const float *)nullptr, d_dinv);
+                       d_dinv);
   else if (A->precision == SMH_F32X2)
     hipLaunchKernelGGL((k_dinv<SMH_F32X2, double>), dim3(g), dim3(256), 0, ctx->stream, A->nrows,
                        A->rowptr, A->colidx, A->vals, A->vals_lo, (int)A->bdim, pc,
-                       (const float *)nullptr, d_dinv);
+                       d_dinv);
   else
     hipLaunchKernelGGL((k_dinv<kF32D, double>), dim3(g), dim3(256), 0, ctx->stream, A->nrows,
                        A->rowptr, A->colidx, A->vals, A->vals_lo, (int)A->bdim, pc,
-                       (const float *)nullptr, d_dinv);
+                       d_dinv);
   SMH_LAUNCH_CHECK(ctx);
   return SMH_OK;
 }
@@ -742,9 +827,11 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
             "smh_cg_solve: block size %u (1 or 3 supported)", A->bdim);
   SMH_CHECK(ctx, o->precond >= 0 && o->precond <= 3, SMH_ERR_INVALID,
             "smh_cg_solve: unknown preconditioner %d", o->precond);
-  SMH_CHECK(ctx, o->precond != SMH_PRECOND_LEARNED || (A->bdim == 3 && o->learned),
-            SMH_ERR_INVALID, "smh_cg_solve: the learned preconditioner needs 3x3 blocks and "
-                             "opts.learned");
+  SMH_CHECK(ctx,
+            o->precond != SMH_PRECOND_LEARNED ||
+                (A->bdim == 3 && o->learned && o->learned_corr),
+            SMH_ERR_INVALID, "smh_cg_solve: the learned preconditioner needs 3x3 blocks, "
+                             "opts.learned and opts.learned_corr (smh_gnn_precond)");
   SMH_CHECK(ctx, A->bdim == 3 || o->precond <= SMH_PRECOND_JACOBI, SMH_ERR_INVALID,
             "smh_cg_solve: block Jacobi needs 3x3 blocks");
   SMH_CHECK(ctx, !o->mixed || A->precision == SMH_F32X2, SMH_ERR_INVALID,
@@ -752,13 +839,13 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
   SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
             "smh_cg_solve: SMH_F32X2 needs vals_lo");
   SMH_CHECK(ctx, o->maxiters >= 0, SMH_ERR_INVALID, "smh_cg_solve: negative maxiters");
+  SMH_CHECK(ctx, o->mixed >= 0 && o->mixed <= 2, SMH_ERR_INVALID,
+            "smh_cg_solve: mixed must be 0, 1 or 2 (got %d)", o->mixed);
   if (A->nrows == 0) {
     if (st) std::memset(st, 0, sizeof(*st));
     return SMH_OK;
   }
   const int bd = (int)A->bdim;
-  const int ip = o->mixed ? SMH_F32 : (A->precision == SMH_F32 ? kF32D : A->precision);
-  const int rp = o->mixed ? SMH_F32X2 : ip;
   smh_tile_plan plan, rplan;
   int rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(bd), &plan);
   if (!rc) rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(bd), &rplan);
@@ -769,7 +856,8 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
 #define SMH_CG(IP, RP)                                                                  \
   (A->bdim == 3 ? cg_run<IP, RP, 3>(ctx, A, &plan, &rplan, b, x, o, st)                 \
                 : cg_run<IP, RP, 1>(ctx, A, &plan, &rplan, b, x, o, st))
-  if (o->mixed) rc = SMH_CG(SMH_F32, SMH_F32X2);
+  if (o->mixed == 2) rc = SMH_CG(kF32D, SMH_F32X2);
+  else if (o->mixed) rc = SMH_CG(SMH_F32, SMH_F32X2);
   else if (A->precision == SMH_F64) rc = SMH_CG(SMH_F64, SMH_F64);
   else if (A->precision == SMH_F32X2) rc = SMH_CG(SMH_F32X2, SMH_F32X2);
   else rc = SMH_CG(kF32D, kF32D);

@@ -1,0 +1,1147 @@
+// Learned block-Jacobi correction: a message-passing network over the graph
+// of a 3x3 blocked matrix, with hand-written forward (BF16 MFMA, FP32
+// accumulation) and backward (FP32) kernels and an Adam optimiser.
+//
+// There is no counterpart in the reference; the model is:
+//   D_i = A_ii = L_i L_i^T,  Linv_i = L_i^-1,  Ahat_k = Linv_i A_k Linv_j^T
+//   edge features   f_e = [Ahat_k (9), |Ahat_k|_F, [i == j], 0 x 5]
+//   vertex features f_v = [sum_{j!=i} Ahat_ij (9), sum_{j!=i} |Ahat_ij|_F, deg_i/16, 0 x 5]
+//   h = relu(f_v W_ev + b_ev),  e = relu(f_e W_ee + b_ee)
+//   3 x { m_k = relu([h_i h_j e_k] W_m + b_m);  a_i = mean_{k in row i} m_k;
+//         h_i += relu([h_i a_i] W_u + b_u) }
+//   R_k = [h_i h_j e_k] W_d + b_d (first 9 outputs, a 3x3 block)
+//   S_k = (R_k + R_t(k)^T)/2,  C_k = kappa S_k / (max(deg_i, deg_j) sqrt(1 + |S_k|_F^2))
+//   preconditioner z_i = Linv_i^T (Linv_i r_i + sum_{k in row i, k off-diagonal} C_k Linv_j r_j)
+// where t(k) is the position of block (j, i).  Every row of I + C has
+// off-diagonal block norms summing below kappa < 1, so I + C is SPD.
+//
+// Layout in HBM: per-block arrays follow the CSR order of the pattern; h is
+// FP32 [nrows][32]; e and m are BF16 [nnz][32]; weights are FP32 masters
+// with BF16 copies stored transposed ([out][in]) so one lane's four
+// consecutive k values of an MFMA B fragment are one 8-byte load.
+//
+// MFMA: __builtin_amdgcn_mfma_f32_16x16x16bf16_1k.  Lane l feeds
+// A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16] (i = 0..3) and
+// receives D[4 (l / 16) + i][l % 16].  One wave multiplies 16 rows by a
+// K x 32 (or K x 16) weight matrix per tile.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "smh_internal.h"
+#include "smh_linalg.h"
+
+namespace {
+
+using namespace smh;
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kH = 32;
+constexpr int kF = 16;
+constexpr int kLayers = 3;
+constexpr int kNParams = SMH_GNN_NPARAMS;
+
+// flat parameter offsets (row-major W[in][out])
+constexpr int oWev = 0, oBev = 512, oWee = 544, oBee = 1056;
+constexpr int kLayerBase = 1088, kLayerSize = 5184;
+__host__ __device__ constexpr int oWm(int l) { return kLayerBase + l * kLayerSize; }
+__host__ __device__ constexpr int oBm(int l) { return oWm(l) + 3072; }
+__host__ __device__ constexpr int oWu(int l) { return oWm(l) + 3104; }
+__host__ __device__ constexpr int oBu(int l) { return oWm(l) + 5152; }
+constexpr int oWd = kLayerBase + kLayers * kLayerSize, oBd = oWd + 1536;
+static_assert(oBd + 16 == kNParams, "parameter layout");
+
+struct WeightMat {
+  int off, in, out;
+};
+const WeightMat kMats[] = {{oWev, 16, 32}, {oWee, 16, 32}, {oWm(0), 96, 32}, {oWu(0), 64, 32},
+                           {oWm(1), 96, 32}, {oWu(1), 64, 32}, {oWm(2), 96, 32},
+                           {oWu(2), 64, 32}, {oWd, 96, 16}};
+
+__device__ __forceinline__ uint16_t f2bf(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ float bfr(float x) { return bf2f(f2bf(x)); }
+
+// ---- graph ---------------------------------------------------------------------------
+__global__ void k_graph_rows(uint32_t n, const uint32_t *__restrict__ rp,
+                             const uint32_t *__restrict__ ci, uint32_t *__restrict__ brow,
+                             uint32_t *__restrict__ deg, int *__restrict__ err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t nd = 0;
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+    brow[k] = i;
+    nd += ci[k] == i;
+  }
+  if (nd != 1 && rp[i + 1] > rp[i]) atomicOr(err, 1);  // rows must hold their diagonal
+  deg[i] = rp[i + 1] - rp[i] - nd;
+}
+
+// t(k): position of block (j, i) for block k = (i, j)
+__global__ void k_graph_transpose(uint64_t nnz, const uint32_t *__restrict__ rp,
+                                  const uint32_t *__restrict__ ci,
+                                  const uint32_t *__restrict__ brow, uint32_t *__restrict__ t,
+                                  int *__restrict__ err) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint32_t i = brow[k], j = ci[k];
+  uint32_t lo = rp[j], hi = rp[j + 1];
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (ci[mid] < i) lo = mid + 1; else hi = mid;
+  }
+  if (lo < rp[j + 1] && ci[lo] == i) {
+    t[k] = lo;
+  } else {
+    t[k] = (uint32_t)k;
+    atomicOr(err, 2);  // pattern not symmetric
+  }
+}
+
+// ---- features --------------------------------------------------------------------------
+template <int PREC>
+__device__ __forceinline__ double aval(const void *v, const void *lo, uint64_t o) {
+  if (PREC == SMH_F64) return ((const double *)v)[o];
+  double x = (double)((const float *)v)[o];
+  if (PREC == SMH_F32X2) x += (double)((const float *)lo)[o];
+  return x;
+}
+
+// Linv_i from the diagonal block (Cholesky of its lower triangle); I when
+// the row has no diagonal block or the block is not positive definite
+template <int PREC>
+__global__ void k_gnn_linv(uint32_t n, const uint32_t *__restrict__ rp,
+                           const uint32_t *__restrict__ ci, const void *vals, const void *vlo,
+                           float *__restrict__ linv) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
+    if (ci[k] == i) {
+      for (int q = 0; q < 9; ++q) d[q] = aval<PREC>(vals, vlo, (uint64_t)k * 9 + q);
+      break;
+    }
+  double L[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  const double p0 = d[0];
+  bool ok = p0 > 0.0;
+  if (ok) {
+    const double l00 = sqrt(p0), l10 = d[3] / l00, l20 = d[6] / l00;
+    const double p1 = d[4] - l10 * l10;
+    ok = p1 > 0.0;
+    if (ok) {
+      const double l11 = sqrt(p1), l21 = (d[7] - l20 * l10) / l11;
+      const double p2 = d[8] - l20 * l20 - l21 * l21;
+      ok = p2 > 0.0;
+      if (ok) {
+        const double l22 = sqrt(p2);
+        const double i00 = 1.0 / l00, i11 = 1.0 / l11, i22 = 1.0 / l22;
+        const double i10 = -l10 * i00 * i11, i21 = -l21 * i11 * i22;
+        const double i20 = -(l20 * i00 + l21 * i10) * i22;
+        L[0] = i00; L[3] = i10; L[4] = i11; L[6] = i20; L[7] = i21; L[8] = i22;
+        L[1] = L[2] = L[5] = 0.0;
+      }
+    }
+  }
+  if (!ok) {
+    for (int q = 0; q < 9; ++q) L[q] = (q % 4 == 0) ? 1.0 : 0.0;
+  }
+  for (int q = 0; q < 9; ++q) linv[(uint64_t)i * 9 + q] = (float)L[q];
+}
+
+// normalised blocks, edge features (and FP32 Ahat for training), vertex
+// features; one thread per row, blocks in CSR order
+template <int PREC>
+__global__ void k_gnn_features(uint32_t n, const uint32_t *__restrict__ rp,
+                               const uint32_t *__restrict__ ci, const void *vals,
+                               const void *vlo, const float *__restrict__ linv,
+                               const uint32_t *__restrict__ deg, uint16_t *__restrict__ fe,
+                               uint16_t *__restrict__ fv, float *__restrict__ ahat) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double Li[9];
+  for (int q = 0; q < 9; ++q) Li[q] = (double)linv[(uint64_t)i * 9 + q];
+  double sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sn = 0.0;
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+    const uint32_t j = ci[k];
+    double Ah[9];
+    const bool dg = j == i;
+    if (dg) {
+      for (int q = 0; q < 9; ++q) Ah[q] = (q % 4 == 0) ? 1.0 : 0.0;
+    } else {
+      double Lj[9], A[9], T[9];
+      for (int q = 0; q < 9; ++q) {
+        Lj[q] = (double)linv[(uint64_t)j * 9 + q];
+        A[q] = aval<PREC>(vals, vlo, (uint64_t)k * 9 + q);
+      }
+      // T = Li A ; Ah = T Lj^T
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          T[3 * a + b] = Li[3 * a] * A[b] + Li[3 * a + 1] * A[3 + b] + Li[3 * a + 2] * A[6 + b];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          Ah[3 * a + b] = T[3 * a] * Lj[3 * b] + T[3 * a + 1] * Lj[3 * b + 1] +
+                          T[3 * a + 2] * Lj[3 * b + 2];
+    }
+    double nr = 0.0;
+    for (int q = 0; q < 9; ++q) nr += Ah[q] * Ah[q];
+    nr = sqrt(nr);
+    uint16_t *f = fe + (uint64_t)k * kF;
+    for (int q = 0; q < 9; ++q) f[q] = f2bf((float)Ah[q]);
+    f[9] = f2bf((float)nr);
+    f[10] = f2bf(dg ? 1.0f : 0.0f);
+    for (int q = 11; q < kF; ++q) f[q] = 0;
+    if (ahat)
+      for (int q = 0; q < 9; ++q) ahat[(uint64_t)k * 9 + q] = (float)Ah[q];
+    if (!dg) {
+      for (int q = 0; q < 9; ++q) sv[q] += Ah[q];
+      sn += nr;
+    }
+  }
+  uint16_t *f = fv + (uint64_t)i * kF;
+  for (int q = 0; q < 9; ++q) f[q] = f2bf((float)sv[q]);
+  f[9] = f2bf((float)sn);
+  f[10] = f2bf((float)deg[i] / 16.0f);
+  for (int q = 11; q < kF; ++q) f[q] = 0;
+}
+
+// ---- MFMA tiles ---------------------------------------------------------------------------
+// acc[nb] += A(16 x 16 KB) * W(16 KB x 16 NB); W given transposed (Wt[c][k])
+template <int KB, int NB>
+struct BFrag {
+  bf16x4 b[NB][KB];
+  __device__ void load(const uint16_t *Wt, int lane) {
+    for (int nb = 0; nb < NB; ++nb)
+      for (int kb = 0; kb < KB; ++kb)
+        b[nb][kb] = *reinterpret_cast<const bf16x4 *>(
+            Wt + (16 * nb + (lane & 15)) * (16 * KB) + 16 * kb + 4 * (lane >> 4));
+  }
+};
+
+__device__ __forceinline__ bf16x4 pack4(float a, float b, float c, float d) {
+  bf16x4 r;
+  r[0] = (short)f2bf(a);
+  r[1] = (short)f2bf(b);
+  r[2] = (short)f2bf(c);
+  r[3] = (short)f2bf(d);
+  return r;
+}
+__device__ __forceinline__ bf16x4 load_f32_as_bf(const float *p) {
+  const float4 v = *reinterpret_cast<const float4 *>(p);
+  return pack4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ bf16x4 load_bf(const uint16_t *p) {
+  return *reinterpret_cast<const bf16x4 *>(p);
+}
+
+template <int KB, int NB>
+__device__ __forceinline__ void tile_mma(const bf16x4 (&a)[KB], const BFrag<KB, NB> &w,
+                                         f32x4 (&acc)[NB]) {
+  for (int nb = 0; nb < NB; ++nb)
+    for (int kb = 0; kb < KB; ++kb)
+      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kb], w.b[nb][kb], acc[nb], 0, 0, 0);
+}
+
+// encoders: out = relu(X W + b) for X = bf16 [rows][16]
+template <bool EDGE>
+__global__ void __launch_bounds__(256)
+k_gnn_encode(uint64_t rows, const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt,
+             const float *__restrict__ bias, float *__restrict__ hout,
+             uint16_t *__restrict__ eout) {
+  const int lane = threadIdx.x & 63;
+  BFrag<1, 2> w;
+  w.load(Wt, lane);
+  const uint64_t ntiles = (rows + 15) / 16;
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles;
+       tile += (uint64_t)gridDim.x * 4) {
+    const uint64_t r = tile * 16 + (lane & 15);
+    bf16x4 a[1];
+    a[0] = r < rows ? load_bf(X + r * kF + 4 * (lane >> 4)) : bf16x4{0, 0, 0, 0};
+    f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    tile_mma<1, 2>(a, w, acc);
+    for (int v = 0; v < 4; ++v) {
+      const uint64_t row = tile * 16 + 4 * (lane >> 4) + v;
+      if (row >= rows) continue;
+      for (int nb = 0; nb < 2; ++nb) {
+        const int c = 16 * nb + (lane & 15);
+        const float z = fmaxf(acc[nb][v] + bias[c], 0.0f);
+        if (EDGE) eout[row * kH + c] = f2bf(z);
+        else hout[row * kH + c] = z;
+      }
+    }
+  }
+}
+
+// A fragment of [h_i h_j e_k] for edge row r
+__device__ __forceinline__ void gather_edge_x(bf16x4 (&a)[6], uint64_t r, bool valid,
+                                              const uint32_t *brow, const uint32_t *ci,
+                                              const float *h, const uint16_t *e, int lane) {
+  const int k4 = 4 * (lane >> 4);
+  if (!valid) {
+    for (int kb = 0; kb < 6; ++kb) a[kb] = bf16x4{0, 0, 0, 0};
+    return;
+  }
+  const uint32_t i = brow[r], j = ci[r];
+  a[0] = load_f32_as_bf(h + (uint64_t)i * kH + k4);
+  a[1] = load_f32_as_bf(h + (uint64_t)i * kH + 16 + k4);
+  a[2] = load_f32_as_bf(h + (uint64_t)j * kH + k4);
+  a[3] = load_f32_as_bf(h + (uint64_t)j * kH + 16 + k4);
+  a[4] = load_bf(e + r * kH + k4);
+  a[5] = load_bf(e + r * kH + 16 + k4);
+}
+
+// m_k = relu([h_i h_j e_k] W_m + b_m) (BF16 out)
+__global__ void __launch_bounds__(256)
+k_gnn_message(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *__restrict__ ci,
+              const float *__restrict__ h, const uint16_t *__restrict__ e,
+              const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
+              uint16_t *__restrict__ m) {
+  const int lane = threadIdx.x & 63;
+  BFrag<6, 2> w;
+  w.load(Wt, lane);
+  const uint64_t ntiles = (nnz + 15) / 16;
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles;
+       tile += (uint64_t)gridDim.x * 4) {
+    const uint64_t r = tile * 16 + (lane & 15);
+    bf16x4 a[6];
+    gather_edge_x(a, r, r < nnz, brow, ci, h, e, lane);
+    f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    tile_mma<6, 2>(a, w, acc);
+    for (int v = 0; v < 4; ++v) {
+      const uint64_t row = tile * 16 + 4 * (lane >> 4) + v;
+      if (row >= nnz) continue;
+      for (int nb = 0; nb < 2; ++nb) {
+        const int c = 16 * nb + (lane & 15);
+        m[row * kH + c] = f2bf(fmaxf(acc[nb][v] + bias[c], 0.0f));
+      }
+    }
+  }
+}
+
+// a_i = mean of row i's messages; U = [h_i a_i] W_u + b_u; hn = h + relu(U)
+__global__ void __launch_bounds__(256)
+k_gnn_update(uint32_t n, const uint32_t *__restrict__ rp, const float *__restrict__ h,
+             const uint16_t *__restrict__ m, const uint16_t *__restrict__ Wt,
+             const float *__restrict__ bias, float *__restrict__ hn, float *__restrict__ Uout,
+             float *__restrict__ aout) {
+  const int lane = threadIdx.x & 63;
+  BFrag<4, 2> w;
+  w.load(Wt, lane);
+  const int k4 = 4 * (lane >> 4);
+  const uint32_t ntiles = (n + 15) / 16;
+  for (uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles;
+       tile += gridDim.x * 4) {
+    const uint32_t r = tile * 16 + (lane & 15);
+    bf16x4 a[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    if (r < n) {
+      a[0] = load_f32_as_bf(h + (uint64_t)r * kH + k4);
+      a[1] = load_f32_as_bf(h + (uint64_t)r * kH + 16 + k4);
+      float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      const uint32_t k0 = rp[r], k1 = rp[r + 1];
+      for (uint32_t k = k0; k < k1; ++k) {
+        const bf16x4 x0 = load_bf(m + (uint64_t)k * kH + k4);
+        const bf16x4 x1 = load_bf(m + (uint64_t)k * kH + 16 + k4);
+        for (int q = 0; q < 4; ++q) {
+          s[q] += bf2f((uint16_t)x0[q]);
+          s[4 + q] += bf2f((uint16_t)x1[q]);
+        }
+      }
+      const float inv = 1.0f / (float)(k1 > k0 ? k1 - k0 : 1u);
+      for (int q = 0; q < 8; ++q) s[q] *= inv;
+      a[2] = pack4(s[0], s[1], s[2], s[3]);
+      a[3] = pack4(s[4], s[5], s[6], s[7]);
+      if (aout) {
+        for (int q = 0; q < 4; ++q) {
+          aout[(uint64_t)r * kH + k4 + q] = s[q];
+          aout[(uint64_t)r * kH + 16 + k4 + q] = s[4 + q];
+        }
+      }
+    }
+    f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+    tile_mma<4, 2>(a, w, acc);
+    for (int v = 0; v < 4; ++v) {
+      const uint32_t row = tile * 16 + 4 * (lane >> 4) + v;
+      if (row >= n) continue;
+      for (int nb = 0; nb < 2; ++nb) {
+        const int c = 16 * nb + (lane & 15);
+        const float U = acc[nb][v] + bias[c];
+        const uint64_t o = (uint64_t)row * kH + c;
+        if (Uout) Uout[o] = U;
+        hn[o] = h[o] + fmaxf(U, 0.0f);
+      }
+    }
+  }
+}
+
+// R_k = [h_i h_j e_k] W_d + b_d, first 9 outputs
+__global__ void __launch_bounds__(256)
+k_gnn_decode(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *__restrict__ ci,
+             const float *__restrict__ h, const uint16_t *__restrict__ e,
+             const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
+             float *__restrict__ R) {
+  const int lane = threadIdx.x & 63;
+  BFrag<6, 1> w;
+  w.load(Wt, lane);
+  const uint64_t ntiles = (nnz + 15) / 16;
+  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles;
+       tile += (uint64_t)gridDim.x * 4) {
+    const uint64_t r = tile * 16 + (lane & 15);
+    bf16x4 a[6];
+    gather_edge_x(a, r, r < nnz, brow, ci, h, e, lane);
+    f32x4 acc[1] = {{0, 0, 0, 0}};
+    tile_mma<6, 1>(a, w, acc);
+    const int c = lane & 15;
+    if (c >= 9) continue;
+    for (int v = 0; v < 4; ++v) {
+      const uint64_t row = tile * 16 + 4 * (lane >> 4) + v;
+      if (row < nnz) R[row * 9 + c] = acc[0][v] + bias[c];
+    }
+  }
+}
+
+// ---- output: symmetric, scaled correction --------------------------------------------------
+struct OutScale {
+  float S[9], phi, c;
+};
+__device__ __forceinline__ OutScale out_scale(const float *R, uint64_t k, uint64_t tk,
+                                              uint32_t di, uint32_t dj, float kappa) {
+  OutScale o;
+  float n2 = 0.0f;
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      const float s = 0.5f * (R[k * 9 + 3 * a + b] + R[tk * 9 + 3 * b + a]);
+      o.S[3 * a + b] = s;
+      n2 += s * s;
+    }
+  o.phi = 1.0f / sqrtf(1.0f + n2);
+  const uint32_t md = max(max(di, dj), 1u);
+  o.c = kappa / (float)md;
+  return o;
+}
+
+// C_k (the block of the lower position of each pair is the transpose of the
+// upper one's, so the stored correction is exactly symmetric)
+__global__ void k_gnn_out(uint64_t nnz, const uint32_t *__restrict__ brow,
+                          const uint32_t *__restrict__ ci, const uint32_t *__restrict__ t,
+                          const uint32_t *__restrict__ deg, const float *__restrict__ R,
+                          float kappa, uint16_t *__restrict__ corr, float *__restrict__ c32) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint32_t i = brow[k], j = ci[k];
+  float C[9];
+  if (i == j) {
+    for (int q = 0; q < 9; ++q) C[q] = 0.0f;
+  } else {
+    const uint64_t tk = t[k];
+    const bool lower = tk < k;
+    const uint64_t own = lower ? tk : k, oth = lower ? k : tk;
+    const OutScale o = out_scale(R, own, oth, deg[brow[own]], deg[ci[own]], kappa);
+    const float f = o.c * o.phi;
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) C[3 * a + b] = f * (lower ? o.S[3 * b + a] : o.S[3 * a + b]);
+  }
+  for (int q = 0; q < 9; ++q) {
+    corr[k * 9 + q] = f2bf(C[q]);
+    if (c32) c32[k * 9 + q] = C[q];
+  }
+}
+
+// ---- training: probes, loss, output backward ------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+// standard normal probes (Box-Muller on a counter hash)
+__global__ void k_gnn_probes(uint64_t n, uint64_t seed, float *__restrict__ V) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint64_t h = mix64(seed * 0x100000001B3ull + i);
+  const double u1 = ((double)(h >> 11) + 1.0) * (1.0 / 9007199254740993.0);
+  const double u2 = (double)(mix64(h) >> 11) * (1.0 / 9007199254740992.0);
+  V[i] = (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
+}
+
+// W = Ahat V (one thread per row; V, W [3 n][K])
+__global__ void k_gnn_ahat_apply(uint32_t n, int K, const uint32_t *__restrict__ rp,
+                                 const uint32_t *__restrict__ ci, const float *__restrict__ ahat,
+                                 const float *__restrict__ V, float *__restrict__ W) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int p = 0; p < K; ++p) {
+    float s[3] = {0, 0, 0};
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+      const uint32_t j = ci[k];
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+          s[a] += ahat[(uint64_t)k * 9 + 3 * a + b] * V[((uint64_t)3 * j + b) * K + p];
+    }
+    for (int a = 0; a < 3; ++a) W[((uint64_t)3 * i + a) * K + p] = s[a];
+  }
+}
+
+// res = V - W - C W; loss partials per workgroup; G = dL/dY = -2 res / N
+__global__ void __launch_bounds__(256)
+k_gnn_loss(uint32_t n, int K, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+           const float *__restrict__ c32, const float *__restrict__ V,
+           const float *__restrict__ W, float *__restrict__ G, double *__restrict__ part) {
+  __shared__ double s4[4];
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double N = (double)K * 3.0 * n;
+  double acc = 0.0;
+  if (i < n) {
+    for (int p = 0; p < K; ++p) {
+      float y[3];
+      for (int a = 0; a < 3; ++a) y[a] = W[((uint64_t)3 * i + a) * K + p];
+      for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+        const uint32_t j = ci[k];
+        if (j == i) continue;
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b)
+            y[a] += c32[(uint64_t)k * 9 + 3 * a + b] * W[((uint64_t)3 * j + b) * K + p];
+      }
+      for (int a = 0; a < 3; ++a) {
+        const uint64_t o = ((uint64_t)3 * i + a) * K + p;
+        const double res = (double)V[o] - (double)y[a];
+        acc += res * res;
+        G[o] = (float)(-2.0 * res / N);
+      }
+    }
+  }
+  const double t = block_sum256(acc, s4);
+  if (threadIdx.x == 0) part[blockIdx.x] = t / N;
+}
+
+// dR_k = (dS_k + dS_t(k)^T) / 2 with dS from dC = G W^T (per block)
+__device__ __forceinline__ void dS_of(uint64_t k, const uint32_t *brow, const uint32_t *ci,
+                                      const uint32_t *t, const uint32_t *deg, const float *R,
+                                      const float *G, const float *W, int K, float kappa,
+                                      float *dS) {
+  const uint32_t i = brow[k], j = ci[k];
+  if (i == j) {
+    for (int q = 0; q < 9; ++q) dS[q] = 0.0f;
+    return;
+  }
+  float dC[9];
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b) {
+      float s = 0.0f;
+      for (int p = 0; p < K; ++p)
+        s += G[((uint64_t)3 * i + a) * K + p] * W[((uint64_t)3 * j + b) * K + p];
+      dC[3 * a + b] = s;
+    }
+  const OutScale o = out_scale(R, k, t[k], deg[i], deg[j], kappa);
+  float inner = 0.0f;
+  for (int q = 0; q < 9; ++q) inner += dC[q] * o.S[q];
+  const float f = o.c * o.phi, g = o.c * o.phi * o.phi * o.phi * inner;
+  for (int q = 0; q < 9; ++q) dS[q] = f * dC[q] - g * o.S[q];
+}
+
+__global__ void k_gnn_out_bwd(uint64_t nnz, const uint32_t *__restrict__ brow,
+                              const uint32_t *__restrict__ ci, const uint32_t *__restrict__ t,
+                              const uint32_t *__restrict__ deg, const float *__restrict__ R,
+                              const float *__restrict__ G, const float *__restrict__ W, int K,
+                              float kappa, float *__restrict__ dR) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  float d1[9], d2[9];
+  dS_of(k, brow, ci, t, deg, R, G, W, K, kappa, d1);
+  dS_of(t[k], brow, ci, t, deg, R, G, W, K, kappa, d2);
+  for (int a = 0; a < 3; ++a)
+    for (int b = 0; b < 3; ++b)
+      dR[k * 16 + 3 * a + b] = 0.5f * d1[3 * a + b] + 0.5f * d2[3 * b + a];
+  for (int q = 9; q < 16; ++q) dR[k * 16 + q] = 0.0f;
+}
+
+// ---- backward building blocks (FP32) ------------------------------------------------------------
+// out[r][o] = sum_c dZ[r][c] W[o][c]  (W row-major [NIN][NOUT] as BF16 transposed Wt[c][o])
+__global__ void k_gnn_rowmat(uint64_t rows, int nin, int nout, const float *__restrict__ dZ,
+                             const uint16_t *__restrict__ Wt, float *__restrict__ out) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * nin) return;
+  const uint64_t r = idx / nin;
+  const int o = (int)(idx % nin);
+  float s = 0.0f;
+  for (int c = 0; c < nout; ++c) s += dZ[r * nout + c] * bf2f(Wt[c * nin + o]);
+  out[idx] = s;
+}
+
+// dh[i][f] (=|+=) sum_{k in row i} dX[k][f] + sum_{k in row i} dX[t(k)][32 + f]
+__global__ void k_gnn_scatter_h(uint32_t n, const uint32_t *__restrict__ rp,
+                                const uint32_t *__restrict__ t, const float *__restrict__ dX,
+                                float *__restrict__ dh, int add) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (uint64_t)n * kH) return;
+  const uint32_t i = (uint32_t)(idx / kH);
+  const int f = (int)(idx % kH);
+  float s = 0.0f;
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) s += dX[(uint64_t)k * 96 + f];
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) s += dX[(uint64_t)t[k] * 96 + 32 + f];
+  dh[idx] = add ? dh[idx] + s : s;
+}
+
+// de[k][f] (=|+=) dX[k][64 + f]
+__global__ void k_gnn_scatter_e(uint64_t nnz, const float *__restrict__ dX, float *__restrict__ de,
+                                int add) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nnz * kH) return;
+  const float v = dX[(idx / kH) * 96 + 64 + idx % kH];
+  de[idx] = add ? de[idx] + v : v;
+}
+
+// dU = dh * [U > 0]
+__global__ void k_gnn_upd_pre(uint64_t n, const float *__restrict__ dh, const float *__restrict__ U,
+                              float *__restrict__ dU) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dU[i] = U[i] > 0.0f ? dh[i] : 0.0f;
+}
+// dh += dXu[:, :32]; da = dXu[:, 32:]
+__global__ void k_gnn_upd_post(uint32_t n, const float *__restrict__ dXu, float *__restrict__ dh,
+                               float *__restrict__ da) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (uint64_t)n * kH) return;
+  const uint64_t i = idx / kH, f = idx % kH;
+  dh[idx] += dXu[i * 64 + f];
+  da[idx] = dXu[i * 64 + 32 + f];
+}
+// dz_k = da_i / rowlen_i * [m_k > 0]
+__global__ void k_gnn_msg_dz(uint64_t nnz, const uint32_t *__restrict__ rp,
+                             const uint32_t *__restrict__ brow, const float *__restrict__ da,
+                             const uint16_t *__restrict__ m, float *__restrict__ dz) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nnz * kH) return;
+  const uint64_t k = idx / kH;
+  const uint32_t i = brow[k];
+  const float inv = 1.0f / (float)max(rp[i + 1] - rp[i], 1u);
+  dz[idx] = bf2f(m[idx]) > 0.0f ? da[(uint64_t)i * kH + idx % kH] * inv : 0.0f;
+}
+// dz = d * [x > 0] with x FP32 or BF16
+__global__ void k_gnn_mask_f32(uint64_t n, const float *__restrict__ d, const float *__restrict__ x,
+                               float *__restrict__ dz) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dz[i] = x[i] > 0.0f ? d[i] : 0.0f;
+}
+__global__ void k_gnn_mask_bf(uint64_t n, const float *__restrict__ d, const uint16_t *__restrict__ x,
+                              float *__restrict__ dz) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dz[i] = bf2f(x[i]) > 0.0f ? d[i] : 0.0f;
+}
+
+// rows of the inputs of each product, as the forward pass fed them (BF16 values)
+struct XEdge {  // [h_i h_j e_k]
+  const uint32_t *brow, *ci;
+  const float *h;
+  const uint16_t *e;
+  static constexpr int NIN = 96;
+  __device__ float operator()(uint64_t k, int c) const {
+    if (c < 32) return bfr(h[(uint64_t)brow[k] * kH + c]);
+    if (c < 64) return bfr(h[(uint64_t)ci[k] * kH + c - 32]);
+    return bf2f(e[k * kH + c - 64]);
+  }
+};
+struct XUpd {  // [h_i a_i]
+  const float *h, *a;
+  static constexpr int NIN = 64;
+  __device__ float operator()(uint64_t i, int c) const {
+    return c < 32 ? bfr(h[i * kH + c]) : bfr(a[i * kH + c - 32]);
+  }
+};
+struct XFeat {  // f (BF16 [rows][16])
+  const uint16_t *f;
+  static constexpr int NIN = 16;
+  __device__ float operator()(uint64_t i, int c) const { return bf2f(f[i * kF + c]); }
+};
+
+// per-workgroup partial of X^T dZ (NIN x NOUT) and of the bias gradient
+template <class X, int NOUT>
+__global__ void __launch_bounds__(256)
+k_gnn_wgrad(X x, uint64_t rows, uint64_t chunk, const float *__restrict__ dZ,
+            float *__restrict__ part) {
+  constexpr int NIN = X::NIN;
+  constexpr int kSub = 32;
+  constexpr int RG = 256 / NOUT;            // row groups
+  constexpr int NJ = (NIN + RG - 1) / RG;   // rows per thread
+  __shared__ float xs[kSub][NIN];
+  __shared__ float zs[kSub][NOUT];
+  const int tid = threadIdx.x;
+  const int c = tid % NOUT, r0 = tid / NOUT;
+  float acc[NJ];
+  for (int j = 0; j < NJ; ++j) acc[j] = 0.0f;
+  float bacc = 0.0f;
+  const uint64_t b0 = (uint64_t)blockIdx.x * chunk, b1 = min(b0 + chunk, rows);
+  for (uint64_t s0 = b0; s0 < b1; s0 += kSub) {
+    const int ns = (int)min<uint64_t>(kSub, b1 - s0);
+    for (int q = tid; q < kSub * NIN; q += 256) {
+      const int e = q / NIN, col = q % NIN;
+      xs[e][col] = e < ns ? x(s0 + e, col) : 0.0f;
+    }
+    for (int q = tid; q < kSub * NOUT; q += 256) {
+      const int e = q / NOUT, col = q % NOUT;
+      zs[e][col] = e < ns ? dZ[(s0 + e) * NOUT + col] : 0.0f;
+    }
+    __syncthreads();
+    for (int e = 0; e < ns; ++e) {
+      const float z = zs[e][c];
+      for (int j = 0; j < NJ; ++j) {
+        const int r = r0 + j * RG;
+        if (r < NIN) acc[j] += xs[e][r] * z;
+      }
+      if (r0 == 0) bacc += z;
+    }
+    __syncthreads();
+  }
+  float *P = part + (uint64_t)blockIdx.x * (NIN * NOUT + NOUT);
+  for (int j = 0; j < NJ; ++j) {
+    const int r = r0 + j * RG;
+    if (r < NIN) P[r * NOUT + c] = acc[j];
+  }
+  if (r0 == 0) P[NIN * NOUT + c] = bacc;
+}
+
+// grad[w_off + q] = sum_b part[b][q] (q < NIN*NOUT), grad[b_off + c] likewise
+__global__ void k_gnn_wgrad_reduce(int nparts, int nw, int nb, const float *__restrict__ part,
+                                   float *__restrict__ grad, int w_off, int b_off) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nw + nb) return;
+  double s = 0.0;
+  for (int b = 0; b < nparts; ++b) s += (double)part[(uint64_t)b * (nw + nb) + q];
+  if (q < nw) grad[w_off + q] = (float)s;
+  else grad[b_off + q - nw] = (float)s;
+}
+
+__global__ void k_gnn_adam(int n, float *__restrict__ p, const float *__restrict__ g,
+                           float *__restrict__ m, float *__restrict__ v, float lr, float c1,
+                           float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = 0.9f * m[i] + 0.1f * gi;
+  const float vi = 0.999f * v[i] + 0.001f * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + 1e-8f);
+}
+
+// Wt[c][r] = bf16(W[r][c]) for one weight matrix
+__global__ void k_gnn_wt(int in, int out, const float *__restrict__ W, uint16_t *__restrict__ Wt) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= in * out) return;
+  const int r = q / out, c = q % out;
+  Wt[c * in + r] = f2bf(W[q]);
+}
+
+__global__ void k_gnn_sum(int n, const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double s4[4];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) a += part[i];
+  const double t = block_sum256(a, s4);
+  if (threadIdx.x == 0) *out = t;
+}
+
+}  // namespace
+
+// ---- the network object and its workspace --------------------------------------------------------
+struct smh_gnn {
+  float kappa = 0.9f;
+  float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  uint16_t *wt = nullptr;
+  int64_t step = 0;
+  // workspace, sized for (nrows, nnz, probes)
+  uint32_t cap_n = 0;
+  uint64_t cap_nnz = 0;
+  int cap_k = 0;
+  bool cap_train = false;
+  std::vector<void *> bufs;
+  uint32_t *brow = nullptr, *tmap = nullptr, *deg = nullptr;
+  int *err = nullptr;
+  float *linv = nullptr, *ahat = nullptr, *R = nullptr, *c32 = nullptr;
+  uint16_t *fe = nullptr, *fv = nullptr, *e = nullptr, *corr_tmp = nullptr;
+  float *h[kLayers + 1] = {}, *a[kLayers] = {}, *U[kLayers] = {};
+  uint16_t *m[kLayers] = {};
+  float *V = nullptr, *W = nullptr, *G = nullptr, *dR = nullptr, *dX = nullptr, *dh = nullptr,
+        *de = nullptr, *dz = nullptr, *dU = nullptr, *da = nullptr, *dXu = nullptr,
+        *wpart = nullptr;
+  double *lpart = nullptr, *loss = nullptr;
+};
+
+namespace {
+
+constexpr int kWgradParts = 256;
+
+void free_work(smh_gnn *g) {
+  for (void *p : g->bufs) dfree(p);
+  g->bufs.clear();
+  g->cap_n = 0;
+  g->cap_nnz = 0;
+  g->cap_k = 0;
+  g->cap_train = false;
+}
+
+template <typename T>
+int walloc(smh_ctx *ctx, smh_gnn *g, T **p, uint64_t count) {
+  void *q = nullptr;
+  const int rc = dmalloc(ctx, &q, sizeof(T) * (count ? count : 1));
+  if (rc) return rc;
+  g->bufs.push_back(q);
+  *p = (T *)q;
+  return SMH_OK;
+}
+
+int ensure_work(smh_ctx *ctx, smh_gnn *g, uint32_t n, uint64_t nnz, int K, bool train) {
+  if (n <= g->cap_n && nnz <= g->cap_nnz && K <= g->cap_k && (!train || g->cap_train))
+    return SMH_OK;
+  free_work(g);
+  const uint64_t N = std::max<uint64_t>(n, 1), E = std::max<uint64_t>(nnz, 1);
+  const int Kk = std::max(K, 1);
+  int rc = 0;
+#define WA(p, c) \
+  if ((rc = walloc(ctx, g, &(p), (c)))) return rc
+  WA(g->brow, E);
+  WA(g->tmap, E);
+  WA(g->deg, N);
+  WA(g->err, 4);
+  WA(g->linv, N * 9);
+  WA(g->R, E * 9);
+  WA(g->fe, E * kF);
+  WA(g->fv, N * kF);
+  WA(g->e, E * kH);
+  WA(g->corr_tmp, E * 9);
+  const int nh = train ? kLayers + 1 : 2;
+  for (int l = 0; l < nh; ++l) WA(g->h[l], N * kH);
+  const int nm = train ? kLayers : 1;
+  for (int l = 0; l < nm; ++l) WA(g->m[l], E * kH);
+  if (train) {
+    WA(g->ahat, E * 9);
+    WA(g->c32, E * 9);
+    for (int l = 0; l < kLayers; ++l) {
+      WA(g->a[l], N * kH);
+      WA(g->U[l], N * kH);
+    }
+    WA(g->V, N * 3 * Kk);
+    WA(g->W, N * 3 * Kk);
+    WA(g->G, N * 3 * Kk);
+    WA(g->dR, E * 16);
+    WA(g->dX, E * 96);
+    WA(g->dh, N * kH);
+    WA(g->de, E * kH);
+    WA(g->dz, E * kH);
+    WA(g->dU, N * kH);
+    WA(g->da, N * kH);
+    WA(g->dXu, N * 64);
+    WA(g->wpart, (uint64_t)kWgradParts * (96 * 32 + 32));
+    WA(g->lpart, (N + 255) / 256 + 1);
+    WA(g->loss, 1);
+  }
+#undef WA
+  g->cap_n = n;
+  g->cap_nnz = nnz;
+  g->cap_k = K;
+  g->cap_train = train;
+  return SMH_OK;
+}
+
+int refresh_wt(smh_ctx *ctx, smh_gnn *g) {
+  for (const WeightMat &w : kMats)
+    hipLaunchKernelGGL(k_gnn_wt, dim3(grid_for(w.in * w.out, 256)), dim3(256), 0, ctx->stream,
+                       w.in, w.out, g->params + w.off, g->wt + w.off);
+  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
+}
+
+int check_A(smh_ctx *ctx, const smh_bsr *A, const char *who) {
+  SMH_CHECK(ctx, A && A->rowptr && A->colidx && A->vals, SMH_ERR_INVALID, "%s: null matrix", who);
+  SMH_CHECK(ctx, A->bdim == 3, SMH_ERR_UNSUPPORTED, "%s: needs 3x3 blocks (got %u)", who,
+            A->bdim);
+  SMH_CHECK(ctx, A->precision == SMH_F64 || A->precision == SMH_F32 ||
+                     A->precision == SMH_F32X2,
+            SMH_ERR_INVALID, "%s: bad precision %d", who, A->precision);
+  SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
+            "%s: SMH_F32X2 needs vals_lo", who);
+  SMH_CHECK(ctx, A->nnz < (1ull << 32), SMH_ERR_UNSUPPORTED, "%s: too many blocks", who);
+  return SMH_OK;
+}
+
+unsigned mfma_grid(uint64_t rows, const smh_ctx *ctx) {
+  const uint64_t tiles = (rows + 15) / 16;
+  const uint64_t wgs = (tiles + 3) / 4;
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wgs, (uint64_t)ctx->num_cu * 8));
+}
+
+// graph, features and the forward pass; leaves R (and the layer caches when
+// training) in the workspace
+int forward(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, bool train) {
+  const uint32_t n = A->nrows;
+  const uint64_t nnz = A->nnz;
+  hipStream_t s = ctx->stream;
+  SMH_HIP(ctx, hipMemsetAsync(g->err, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_graph_rows, dim3(grid_for(n, 256)), dim3(256), 0, s, n, A->rowptr,
+                     A->colidx, g->brow, g->deg, g->err);
+  hipLaunchKernelGGL(k_graph_transpose, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz,
+                     A->rowptr, A->colidx, g->brow, g->tmap, g->err);
+#define SMH_FEAT(P)                                                                           \
+  do {                                                                                        \
+    hipLaunchKernelGGL(k_gnn_linv<P>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, A->rowptr,   \
+                       A->colidx, A->vals, A->vals_lo, g->linv);                              \
+    hipLaunchKernelGGL(k_gnn_features<P>, dim3(grid_for(n, 128)), dim3(128), 0, s, n,          \
+                       A->rowptr, A->colidx, A->vals, A->vals_lo, g->linv, g->deg, g->fe,      \
+                       g->fv, train ? g->ahat : nullptr);                                     \
+  } while (0)
+  if (A->precision == SMH_F64) SMH_FEAT(SMH_F64);
+  else if (A->precision == SMH_F32) SMH_FEAT(SMH_F32);
+  else SMH_FEAT(SMH_F32X2);
+#undef SMH_FEAT
+  int herr = 0;
+  SMH_HIP(ctx, hipMemcpyAsync(&herr, g->err, sizeof(int), hipMemcpyDeviceToHost, s));
+  SMH_HIP(ctx, hipStreamSynchronize(s));
+  SMH_CHECK(ctx, herr == 0, SMH_ERR_INVALID,
+            "learned preconditioner: the pattern must be symmetric with every non-empty row "
+            "holding its diagonal block (code %d)", herr);
+  hipLaunchKernelGGL(k_gnn_encode<false>, dim3(mfma_grid(n, ctx)), dim3(256), 0, s, (uint64_t)n,
+                     g->fv, g->wt + oWev, g->params + oBev, g->h[0], nullptr);
+  hipLaunchKernelGGL(k_gnn_encode<true>, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz, g->fe,
+                     g->wt + oWee, g->params + oBee, nullptr, g->e);
+  float *hin = g->h[0];
+  for (int l = 0; l < kLayers; ++l) {
+    uint16_t *m = train ? g->m[l] : g->m[0];
+    float *hout = train ? g->h[l + 1] : (hin == g->h[0] ? g->h[1] : g->h[0]);
+    hipLaunchKernelGGL(k_gnn_message, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz, g->brow,
+                       A->colidx, hin, g->e, g->wt + oWm(l), g->params + oBm(l), m);
+    hipLaunchKernelGGL(k_gnn_update, dim3(mfma_grid(n, ctx)), dim3(256), 0, s, n, A->rowptr, hin,
+                       m, g->wt + oWu(l), g->params + oBu(l), hout, train ? g->U[l] : nullptr,
+                       train ? g->a[l] : nullptr);
+    hin = hout;
+  }
+  hipLaunchKernelGGL(k_gnn_decode, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz, g->brow,
+                     A->colidx, hin, g->e, g->wt + oWd, g->params + oBd, g->R);
+  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
+}
+
+template <class X, int NOUT>
+int wgrad(smh_ctx *ctx, smh_gnn *g, X x, uint64_t rows, const float *dZ, int w_off, int b_off) {
+  const uint64_t chunk = std::max<uint64_t>(32, (rows + kWgradParts - 1) / kWgradParts);
+  const int parts = (int)std::max<uint64_t>(1, (rows + chunk - 1) / chunk);
+  hipLaunchKernelGGL((k_gnn_wgrad<X, NOUT>), dim3(parts), dim3(256), 0, ctx->stream, x, rows,
+                     chunk, dZ, g->wpart);
+  const int nw = X::NIN * NOUT;
+  hipLaunchKernelGGL(k_gnn_wgrad_reduce, dim3(grid_for(nw + NOUT, 256)), dim3(256), 0,
+                     ctx->stream, parts, nw, NOUT, (const float *)g->wpart, g->grads, w_off,
+                     b_off);
+  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
+}
+
+// loss and gradients into g->grads (device); loss value left in g->loss
+int loss_grad(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int K, const float *probes,
+              uint64_t seed) {
+  const uint32_t n = A->nrows;
+  const uint64_t nnz = A->nnz;
+  hipStream_t s = ctx->stream;
+  int rc = ensure_work(ctx, g, n, nnz, K, true);
+  if (rc) return rc;
+  rc = forward(ctx, g, A, true);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_gnn_out, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, g->brow,
+                     A->colidx, g->tmap, g->deg, g->R, g->kappa, g->corr_tmp, g->c32);
+  const uint64_t nv = (uint64_t)3 * n * K;
+  const float *V = probes;
+  if (!V) {
+    hipLaunchKernelGGL(k_gnn_probes, dim3(grid_for(nv, 256)), dim3(256), 0, s, nv, seed, g->V);
+    V = g->V;
+  }
+  hipLaunchKernelGGL(k_gnn_ahat_apply, dim3(grid_for(n, 128)), dim3(128), 0, s, n, K, A->rowptr,
+                     A->colidx, g->ahat, V, g->W);
+  const unsigned nl = grid_for(n, 256);
+  hipLaunchKernelGGL(k_gnn_loss, dim3(nl), dim3(256), 0, s, n, K, A->rowptr, A->colidx, g->c32, V,
+                     g->W, g->G, g->lpart);
+  hipLaunchKernelGGL(k_gnn_sum, dim3(1), dim3(256), 0, s, (int)nl, (const double *)g->lpart,
+                     g->loss);
+  hipLaunchKernelGGL(k_gnn_out_bwd, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, g->brow,
+                     A->colidx, g->tmap, g->deg, g->R, g->G, g->W, K, g->kappa, g->dR);
+  // decoder
+  float *hfin = g->h[kLayers];
+  XEdge xd{g->brow, A->colidx, hfin, g->e};
+  if ((rc = wgrad<XEdge, 16>(ctx, g, xd, nnz, g->dR, oWd, oBd))) return rc;
+  hipLaunchKernelGGL(k_gnn_rowmat, dim3(grid_for(nnz * 96, 256)), dim3(256), 0, s, nnz, 96, 16,
+                     g->dR, g->wt + oWd, g->dX);
+  hipLaunchKernelGGL(k_gnn_scatter_h, dim3(grid_for((uint64_t)n * kH, 256)), dim3(256), 0, s, n,
+                     A->rowptr, g->tmap, g->dX, g->dh, 0);
+  hipLaunchKernelGGL(k_gnn_scatter_e, dim3(grid_for(nnz * kH, 256)), dim3(256), 0, s, nnz, g->dX,
+                     g->de, 0);
+  for (int l = kLayers - 1; l >= 0; --l) {
+    hipLaunchKernelGGL(k_gnn_upd_pre, dim3(grid_for((uint64_t)n * kH, 256)), dim3(256), 0, s,
+                       (uint64_t)n * kH, g->dh, g->U[l], g->dU);
+    XUpd xu{g->h[l], g->a[l]};
+    if ((rc = wgrad<XUpd, 32>(ctx, g, xu, n, g->dU, oWu(l), oBu(l)))) return rc;
+    hipLaunchKernelGGL(k_gnn_rowmat, dim3(grid_for((uint64_t)n * 64, 256)), dim3(256), 0, s,
+                       (uint64_t)n, 64, 32, g->dU, g->wt + oWu(l), g->dXu);
+    hipLaunchKernelGGL(k_gnn_upd_post, dim3(grid_for((uint64_t)n * kH, 256)), dim3(256), 0, s, n,
+                       g->dXu, g->dh, g->da);
+    hipLaunchKernelGGL(k_gnn_msg_dz, dim3(grid_for(nnz * kH, 256)), dim3(256), 0, s, nnz,
+                       A->rowptr, g->brow, g->da, g->m[l], g->dz);
+    XEdge xm{g->brow, A->colidx, g->h[l], g->e};
+    if ((rc = wgrad<XEdge, 32>(ctx, g, xm, nnz, g->dz, oWm(l), oBm(l)))) return rc;
+    hipLaunchKernelGGL(k_gnn_rowmat, dim3(grid_for(nnz * 96, 256)), dim3(256), 0, s, nnz, 96, 32,
+                       g->dz, g->wt + oWm(l), g->dX);
+    hipLaunchKernelGGL(k_gnn_scatter_h, dim3(grid_for((uint64_t)n * kH, 256)), dim3(256), 0, s,
+                       n, A->rowptr, g->tmap, g->dX, g->dh, 1);
+    hipLaunchKernelGGL(k_gnn_scatter_e, dim3(grid_for(nnz * kH, 256)), dim3(256), 0, s, nnz,
+                       g->dX, g->de, 1);
+  }
+  // encoders
+  hipLaunchKernelGGL(k_gnn_mask_bf, dim3(grid_for(nnz * kH, 256)), dim3(256), 0, s, nnz * kH,
+                     g->de, g->e, g->dz);
+  XFeat xe{g->fe};
+  if ((rc = wgrad<XFeat, 32>(ctx, g, xe, nnz, g->dz, oWee, oBee))) return rc;
+  hipLaunchKernelGGL(k_gnn_mask_f32, dim3(grid_for((uint64_t)n * kH, 256)), dim3(256), 0, s,
+                     (uint64_t)n * kH, g->dh, g->h[0], g->dU);
+  XFeat xv{g->fv};
+  if ((rc = wgrad<XFeat, 32>(ctx, g, xv, n, g->dU, oWev, oBev))) return rc;
+  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
+}
+
+}  // namespace
+
+extern "C" int smh_gnn_create(smh_ctx *ctx, uint64_t seed, double kappa, smh_gnn **out) {
+  if (!ctx || !out) return SMH_ERR_INVALID;
+  *out = nullptr;
+  SMH_CHECK(ctx, kappa > 0.0 && kappa < 1.0, SMH_ERR_INVALID,
+            "smh_gnn_create: kappa must lie in (0, 1), got %g", kappa);
+  smh_gnn *g = new smh_gnn();
+  g->kappa = (float)kappa;
+  int rc = 0;
+  if ((rc = dmalloc(ctx, (void **)&g->params, sizeof(float) * kNParams)) ||
+      (rc = dmalloc(ctx, (void **)&g->grads, sizeof(float) * kNParams)) ||
+      (rc = dmalloc(ctx, (void **)&g->adam_m, sizeof(float) * kNParams)) ||
+      (rc = dmalloc(ctx, (void **)&g->adam_v, sizeof(float) * kNParams)) ||
+      (rc = dmalloc(ctx, (void **)&g->wt, sizeof(uint16_t) * kNParams))) {
+    smh_gnn_destroy(ctx, g);
+    return rc;
+  }
+  // Glorot-uniform weights (decoder scaled by 0.1 so training starts next to
+  // block Jacobi), zero biases, from a counter hash of the seed
+  std::vector<float> p(kNParams, 0.0f);
+  for (const WeightMat &w : kMats) {
+    const double a = std::sqrt(6.0 / (w.in + w.out)) * (w.off == oWd ? 0.1 : 1.0);
+    for (int q = 0; q < w.in * w.out; ++q) {
+      uint64_t x = seed * 0x9E3779B97F4A7C15ull + (uint64_t)(w.off + q) + 1;
+      x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+      x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+      x ^= x >> 31;
+      const double u = (double)(x >> 11) * (1.0 / 9007199254740992.0);
+      p[w.off + q] = (float)((2.0 * u - 1.0) * a);
+    }
+  }
+  rc = smh_gnn_set_params(ctx, g, p.data());
+  if (rc) {
+    smh_gnn_destroy(ctx, g);
+    return rc;
+  }
+  *out = g;
+  return SMH_OK;
+}
+
+extern "C" int smh_gnn_destroy(smh_ctx *ctx, smh_gnn *g) {
+  if (!g) return SMH_OK;
+  if (ctx) (void)hipStreamSynchronize(ctx->stream);
+  free_work(g);
+  dfree(g->params);
+  dfree(g->grads);
+  dfree(g->adam_m);
+  dfree(g->adam_v);
+  dfree(g->wt);
+  delete g;
+  return SMH_OK;
+}
+
+extern "C" int smh_gnn_get_params(smh_ctx *ctx, smh_gnn *g, float *host) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, g && host, SMH_ERR_INVALID, "smh_gnn_get_params: null argument");
+  SMH_HIP(ctx, hipMemcpyAsync(host, g->params, sizeof(float) * kNParams, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SMH_OK;
+}
+
+extern "C" int smh_gnn_set_params(smh_ctx *ctx, smh_gnn *g, const float *host) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, g && host, SMH_ERR_INVALID, "smh_gnn_set_params: null argument");
+  SMH_HIP(ctx, hipMemcpyAsync(g->params, host, sizeof(float) * kNParams, hipMemcpyHostToDevice,
+                              ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SMH_HIP(ctx, hipMemsetAsync(g->adam_m, 0, sizeof(float) * kNParams, ctx->stream));
+  SMH_HIP(ctx, hipMemsetAsync(g->adam_v, 0, sizeof(float) * kNParams, ctx->stream));
+  SMH_HIP(ctx, hipMemsetAsync(g->wt, 0, sizeof(uint16_t) * kNParams, ctx->stream));
+  g->step = 0;
+  return refresh_wt(ctx, g);
+}
+
+extern "C" int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float *linv,
+                               uint16_t *corr) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, g && linv && corr, SMH_ERR_INVALID, "smh_gnn_precond: null argument");
+  int rc = check_A(ctx, A, "smh_gnn_precond");
+  if (rc) return rc;
+  if (A->nrows == 0) return SMH_OK;
+  if ((rc = ensure_work(ctx, g, A->nrows, A->nnz, g->cap_k, false))) return rc;
+  if ((rc = forward(ctx, g, A, false))) return rc;
+  hipLaunchKernelGGL(k_gnn_out, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream, A->nnz,
+                     g->brow, A->colidx, g->tmap, g->deg, g->R, g->kappa, corr, nullptr);
+  SMH_HIP(ctx, hipMemcpyAsync(linv, g->linv, sizeof(float) * 9 * (size_t)A->nrows,
+                              hipMemcpyDeviceToDevice, ctx->stream));
+  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
+}
+
+extern "C" int smh_gnn_loss_grad(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int32_t nprobes,
+                                 const float *probes, uint64_t seed, double *loss, float *grads) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, g, SMH_ERR_INVALID, "smh_gnn_loss_grad: null network");
+  SMH_CHECK(ctx, nprobes >= 1 && nprobes <= 64, SMH_ERR_INVALID,
+            "smh_gnn_loss_grad: nprobes must be 1..64 (got %d)", nprobes);
+  int rc = check_A(ctx, A, "smh_gnn_loss_grad");
+  if (rc) return rc;
+  SMH_CHECK(ctx, A->nrows > 0, SMH_ERR_INVALID, "smh_gnn_loss_grad: empty matrix");
+  if ((rc = loss_grad(ctx, g, A, nprobes, probes, seed))) return rc;
+  if (loss)
+    SMH_HIP(ctx, hipMemcpyAsync(loss, g->loss, sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  if (grads)
+    SMH_HIP(ctx, hipMemcpyAsync(grads, g->grads, sizeof(float) * kNParams,
+                                hipMemcpyDeviceToHost, ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return SMH_OK;
+}
+
+extern "C" int smh_gnn_train_step(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int32_t nprobes,
+                                  uint64_t seed, double lr, double *loss) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, g, SMH_ERR_INVALID, "smh_gnn_train_step: null network");
+  SMH_CHECK(ctx, nprobes >= 1 && nprobes <= 64, SMH_ERR_INVALID,
+            "smh_gnn_train_step: nprobes must be 1..64 (got %d)", nprobes);
+  SMH_CHECK(ctx, lr > 0.0, SMH_ERR_INVALID, "smh_gnn_train_step: lr must be positive");
+  int rc = check_A(ctx, A, "smh_gnn_train_step");
+  if (rc) return rc;
+  SMH_CHECK(ctx, A->nrows > 0, SMH_ERR_INVALID, "smh_gnn_train_step: empty matrix");
+  if ((rc = loss_grad(ctx, g, A, nprobes, nullptr, seed))) return rc;
+  g->step += 1;
+  const float c1 = (float)(1.0 - std::pow(0.9, (double)g->step));
+  const float c2 = (float)(1.0 - std::pow(0.999, (double)g->step));
+  hipLaunchKernelGGL(k_gnn_adam, dim3(grid_for(kNParams, 256)), dim3(256), 0, ctx->stream,
+                     kNParams, g->params, (const float *)g->grads, g->adam_m, g->adam_v,
+                     (float)lr, c1, c2);
+  if ((rc = refresh_wt(ctx, g))) return rc;
+  if (loss) {
+    SMH_HIP(ctx, hipMemcpyAsync(loss, g->loss, sizeof(double), hipMemcpyDeviceToHost,
+                                ctx->stream));
+    SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
+  return SMH_OK;
+}

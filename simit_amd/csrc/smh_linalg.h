@@ -45,23 +45,50 @@ template <> struct Val<SMH_F64> {
   using T = double;  // vector / compute type
   using S = double;  // stored value type
   static constexpr bool kLo = false;
+  static constexpr bool kBF16 = false;
 };
 template <> struct Val<SMH_F32> {
   using T = float;
   using S = float;
   static constexpr bool kLo = false;
+  static constexpr bool kBF16 = false;
 };
 template <> struct Val<SMH_F32X2> {
   using T = double;
   using S = float;
   static constexpr bool kLo = true;
+  static constexpr bool kBF16 = false;
 };
 constexpr int kF32D = 3;  // float values driven by double vectors
 template <> struct Val<kF32D> {
   using T = double;
   using S = float;
   static constexpr bool kLo = false;
+  static constexpr bool kBF16 = false;
 };
+constexpr int kBF16F = 4;  // bfloat16 values (bits in uint16), float vectors
+template <> struct Val<kBF16F> {
+  using T = float;
+  using S = uint16_t;
+  static constexpr bool kLo = false;
+  static constexpr bool kBF16 = true;
+};
+constexpr int kBF16D = 5;  // bfloat16 values, double vectors
+template <> struct Val<kBF16D> {
+  using T = double;
+  using S = uint16_t;
+  static constexpr bool kLo = false;
+  static constexpr bool kBF16 = true;
+};
+
+// stored value -> compute type
+template <int PREC>
+__device__ __forceinline__ typename Val<PREC>::T val_of(typename Val<PREC>::S v) {
+  if constexpr (Val<PREC>::kBF16)
+    return (typename Val<PREC>::T)__uint_as_float((uint32_t)v << 16);
+  else
+    return (typename Val<PREC>::T)v;
+}
 
 struct TileDev {
   uint32_t nrows;
@@ -168,7 +195,7 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
       T acc = 0;
 #pragma unroll
       for (int b = 0; b < BD; ++b) {
-        T val = (T)cur.v[u][b];
+        T val = val_of<PREC>(cur.v[u][b]);
         if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][b];
         acc += val * xg(cur.col[u], b);
       }

@@ -65,17 +65,17 @@ def tet_system(ctx, n, seed=0, field="iid", std=0.5):
 
 
 @pytest.mark.parametrize("mode", ["plain64", "jacobi64", "block64", "mixed_block", "mixed_none",
-                                  "hilo64"])
+                                  "mixed64v_block", "hilo64"])
 def test_tet_cg_modes_reach_1e8(ctx, oracle, mode):
     prob, idx, v64, hi, lo, rhs, _ = tet_system(ctx, 10, seed=3)
     rp, ci = idx.to_host()
     n = prob["nverts"] * 3
     pc = {"plain64": 0, "jacobi64": 1, "block64": 2, "mixed_block": 2, "mixed_none": 0,
-          "hilo64": 2}[mode]
-    mixed = mode.startswith("mixed")
+          "mixed64v_block": 2, "hilo64": 2}[mode]
+    mixed = 2 if mode.startswith("mixed64v") else int(mode.startswith("mixed"))
     A = idx.bsr(3, S.F32X2, hi, lo) if (mixed or mode == "hilo64") else idx.bsr(3, S.F64, v64)
     x = torch.zeros(n, dtype=torch.float64, device="cuda")
-    o = ctx.cg_options(maxiters=20000, tol=1e-8, rel=1, precond=pc, mixed=int(mixed))
+    o = ctx.cg_options(maxiters=20000, tol=1e-8, rel=1, precond=pc, mixed=mixed)
     st = ctx.cg(A, rhs, x, o)
     print(mode, st.as_dict(), flush=True)
     assert st.status == 0, st.as_dict()

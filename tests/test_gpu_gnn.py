@@ -1,0 +1,201 @@
+"""The learned block-Jacobi correction on the GPU against the CPU oracle.
+
+The oracle (oracle/gnn.py, pinned by finite differences and torch autograd in
+test_gnn_oracle.py) runs with mimic=True: it rounds to bfloat16 where the
+kernels store or feed BF16.  The kernels accumulate in FP32 where the oracle
+uses float64, so a value next to a BF16 rounding boundary can land one BF16
+step (2^-8 relative) apart; the tolerances below allow for that and nothing
+more.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import gnn as G
+import simit_amd as S
+from simit_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+
+def to_dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def tet_system(ctx, n, seed, corr=3.0):
+    prob = meshes.tet_problem(n, seed=seed, field="grf", std_log10=1.0, corr_len=corr)
+    d = {"x": to_dev(prob["x"]), "c": to_dev(prob["c"].astype(np.int32)),
+         "mu": to_dev(prob["mu"]), "lam": to_dev(prob["lam"]),
+         "endpoints": to_dev(prob["endpoints"].astype(np.int32))}
+    idx = ctx.build_index(prob["nverts"], [d["endpoints"]])
+    P = S.TetParams(1.0, 0.0, (C.c_double * 3)(*prob["load"]), 1.0)
+    v64 = torch.empty(idx.nnz * 9, dtype=torch.float64, device="cuda")
+    hi = torch.empty(idx.nnz * 9, dtype=torch.float32, device="cuda")
+    lo = torch.empty_like(hi)
+    rhs = torch.empty(prob["nverts"] * 3, dtype=torch.float64, device="cuda")
+    ctx.assemble_tets(idx, P, d, d, S.F32X2, hi, lo, rhs)
+    ctx.assemble_tets(idx, P, d, d, S.F64, v64, None, rhs)
+    torch.cuda.synchronize()
+    return {"idx": idx, "v64": v64, "hi": hi, "lo": lo, "rhs": rhs, "dev": d, "prob": prob}
+
+
+def bf16_bits_to_f32(t):
+    u = t.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+    return u.view(np.float32).astype(np.float64)
+
+
+def random_params(seed, scale=1.0):
+    p = G.init_params(seed, dec_scale=1.0)
+    rng = np.random.default_rng(seed + 100)
+    for k in p:
+        p[k] = rng.normal(0, 0.1, p[k].shape) if k.startswith("b") else p[k] * scale
+    return G.flatten(p).astype(np.float32)
+
+
+def test_forward_matches_oracle(ctx):
+    sy = tet_system(ctx, 4, seed=1)
+    idx = sy["idx"]
+    rp, ci = idx.to_host()
+    vals = sy["v64"].cpu().numpy()
+    net = ctx.gnn(seed=3, kappa=0.9)
+    flat = random_params(5)
+    net.set_params(flat)
+    np.testing.assert_array_equal(net.get_params(), flat)
+    linv, corr = net.precond(idx.bsr(3, S.F64, sy["v64"]))
+    torch.cuda.synchronize()
+    B, aux = G.precond_blocks(G.unflatten(flat.astype(np.float64)), rp, ci, vals, kappa=0.9,
+                              mimic=True)
+    Lo = aux["ft"]["Linv"].reshape(-1, 9)
+    np.testing.assert_allclose(linv.cpu().numpy().reshape(-1, 9), Lo, rtol=2e-6,
+                               atol=2e-6 * np.abs(Lo).max())
+    Cg = bf16_bits_to_f32(corr).reshape(-1, 3, 3)
+    Co = aux["C"]
+    g = aux["g"]
+    # exact symmetry of the stored correction, zero diagonal blocks
+    assert np.array_equal(Cg, Cg[g["t"]].transpose(0, 2, 1))
+    assert np.all(Cg[g["diag"]] == 0.0)
+    err = np.abs(Cg - Co)
+    scale = np.abs(Co).max()
+    assert scale > 1e-3
+    assert np.linalg.norm(Cg - Co) <= 1e-2 * np.linalg.norm(Co), np.linalg.norm(Cg - Co)
+    assert err.max() <= 3e-2 * scale, err.max() / scale
+    # Gershgorin bound of every row (kappa = 0.9) survives the BF16 storage
+    rows = np.zeros(rp.size - 1)
+    np.add.at(rows, g["brow"], np.sqrt((Cg * Cg).sum(axis=(1, 2))))
+    assert rows.max() < 0.9 * (1 + 2 ** -7)
+    net.close()
+
+
+def test_gradients_match_oracle(ctx):
+    sy = tet_system(ctx, 3, seed=2)
+    idx = sy["idx"]
+    rp, ci = idx.to_host()
+    vals = sy["v64"].cpu().numpy()
+    n = rp.size - 1
+    net = ctx.gnn(seed=0, kappa=0.9)
+    flat = random_params(7, scale=0.8)
+    net.set_params(flat)
+    rng = np.random.default_rng(3)
+    V = rng.normal(size=(3 * n, 4)).astype(np.float32)
+    L, g = net.loss_grad(idx.bsr(3, S.F64, sy["v64"]), nprobes=4, probes=to_dev(V))
+    Lo, go, _ = G.loss_and_grads(G.unflatten(flat.astype(np.float64)), rp, ci, vals,
+                                 V.astype(np.float64), kappa=0.9, mimic=True)
+    assert abs(L - Lo) <= 2e-3 * Lo, (L, Lo)
+    gof = G.flatten(go)
+    o = 0
+    for name, shape in G.param_layout():
+        k = int(np.prod(shape))
+        a, b = g[o:o + k].astype(np.float64), gof[o:o + k]
+        o += k
+        nb = np.linalg.norm(b)
+        if nb < 1e-7 * np.linalg.norm(gof):
+            assert np.linalg.norm(a) <= 1e-5 * np.linalg.norm(gof), name
+            continue
+        assert np.linalg.norm(a - b) <= 3e-2 * nb, (name, np.linalg.norm(a - b) / nb)
+    # the loss is what the probes say: same probes, same bits
+    L2, g2 = net.loss_grad(idx.bsr(3, S.F64, sy["v64"]), nprobes=4, probes=to_dev(V))
+    assert L2 == L and np.array_equal(g, g2)
+    net.close()
+
+
+def test_training_reduces_loss_and_is_deterministic(ctx):
+    systems = [tet_system(ctx, 6, seed=s) for s in range(3)]
+    mats = [s["idx"].bsr(3, S.F32X2, s["hi"], s["lo"]) for s in systems]
+    losses = []
+    for run in range(2):
+        net = ctx.gnn(seed=11, kappa=0.9)
+        first = [net.loss_grad(A, nprobes=8, seed=99, grads=False)[0] for A in mats]
+        for step in range(120):
+            net.train_step(mats[step % 3], nprobes=4, seed=1000 + step, lr=3e-3,
+                           want_loss=False)
+        last = [net.loss_grad(A, nprobes=8, seed=99, grads=False)[0] for A in mats]
+        losses.append((first, last, net.get_params()))
+        net.close()
+    (f0, l0, p0), (f1, l1, p1) = losses
+    assert np.mean(l0) < 0.9 * np.mean(f0), (f0, l0)
+    assert f0 == f1 and l0 == l1 and np.array_equal(p0, p1)
+
+
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+def test_learned_pcg_converges(ctx, oracle, mixed):
+    sy = tet_system(ctx, 8, seed=4)
+    idx = sy["idx"]
+    net = ctx.gnn(seed=2, kappa=0.9)
+    net.set_params(random_params(9))
+    A = idx.bsr(3, S.F32X2, sy["hi"], sy["lo"]) if mixed else idx.bsr(3, S.F64, sy["v64"])
+    linv, corr = net.precond(A)
+    n = sy["rhs"].numel()
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    o = ctx.cg_options(maxiters=5000, tol=1e-8, rel=1, precond=S.PRECOND_LEARNED, mixed=mixed,
+                       learned=linv, learned_corr=corr)
+    st = ctx.cg(A, sy["rhs"], x, o)
+    assert st.status == 0 and st.true_rel <= 1e-8, st.as_dict()
+    rp, ci = idx.to_host()
+    vals = sy["v64"].cpu().numpy()
+    f = sy["rhs"].cpu().numpy()
+    r = f - oracle.bsrmv(rp, ci, vals, x.cpu().numpy(), 3)
+    assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(f)
+    net.close()
+
+
+def test_learned_pcg_rejects_missing_buffers(ctx):
+    sy = tet_system(ctx, 3, seed=0)
+    A = sy["idx"].bsr(3, S.F64, sy["v64"])
+    x = torch.zeros_like(sy["rhs"])
+    with pytest.raises(S.SimitError):
+        ctx.cg(A, sy["rhs"], x, ctx.cg_options(precond=S.PRECOND_LEARNED))
+
+
+def test_trained_network_beats_block_jacobi(ctx):
+    # ablation: the same PCG with the trained correction against block Jacobi
+    # alone (the network with its output zeroed) on a held-out system
+    train = [tet_system(ctx, 10, seed=100 + s, corr=4.0) for s in range(8)]
+    mats = [s["idx"].bsr(3, S.F32X2, s["hi"], s["lo"]) for s in train]
+    net = ctx.gnn(seed=0, kappa=0.9)
+    for step in range(400):
+        net.train_step(mats[step % len(mats)], nprobes=4, seed=step, lr=3e-3, want_loss=False)
+    test = tet_system(ctx, 16, seed=7, corr=4.0)
+    A = test["idx"].bsr(3, S.F64, test["v64"])
+    n = test["rhs"].numel()
+
+    def iters(linv, corr):
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        st = ctx.cg(A, test["rhs"], x, ctx.cg_options(
+            maxiters=20000, tol=1e-8, rel=1, precond=S.PRECOND_LEARNED, learned=linv,
+            learned_corr=corr))
+        assert st.status == 0 and st.true_rel <= 1e-8
+        return st.iters
+
+    linv, corr = net.precond(A)
+    it_learned = iters(linv, corr)
+    zero = torch.zeros_like(corr)
+    it_bj = iters(linv, zero)
+    x = torch.zeros(n, dtype=torch.float64, device="cuda")
+    st = ctx.cg(A, test["rhs"], x, ctx.cg_options(maxiters=20000, tol=1e-8, rel=1,
+                                                  precond=S.PRECOND_BLOCK_JACOBI))
+    print(f"block Jacobi {st.iters} / zero correction {it_bj} / learned {it_learned}")
+    assert abs(it_bj - st.iters) <= max(3, st.iters // 50)
+    assert it_learned < 0.92 * it_bj, (it_learned, it_bj)
+    net.close()
