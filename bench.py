@@ -5,16 +5,20 @@ cube (912,673 vertices, 2,738,019 DOF, 4,423,680 tets, 11,695,969 3x3
 blocks), log10 E = 5 + 1 * Gaussian random field (correlation length 8
 cells), nu = 0.3, clamped z = 0 face, uniform body load.  Synthetic, seeded.
 
-One step = assemble the FP32 hi/lo blocked-CSR matrix and the right-hand
-side from the resident mesh, then solve to a 1e-8 relative FP64 residual with
-the mixed-precision block-Jacobi PCG (FP32 SpMV, FP64 dots and reliable
-updates), from x = 0.  The metric is the time of that step; plain FP64 CG on
-the same GPU is timed beside it.
+The learned correction is trained first, on the GPU by the library's own
+forward/backward kernels, on 2000 seeded 32^3 samples of the same physics.
 
-The roofline object is for the dominant kernel, the FP32 blocked SpMV:
-algorithmic bytes per launch = 36*nnz (values) + 4*nnz (colidx) + 4*(N+1)
-(rowptr) + 12*N (x) + 12*N (y), timed with HIP events on the stream the
-kernel runs on.
+One step (the metric) = assemble the FP32 hi/lo blocked-CSR matrix and the
+right-hand side from the resident mesh, run the network to get the learned
+preconditioner, then solve to a 1e-8 relative FP64 residual with the
+mixed-precision learned PCG (FP32 SpMV, FP64 dots and reliable updates) from
+x = 0.  Block-Jacobi PCG (mixed and FP64) and plain FP64 CG on the same GPU
+are timed beside it, each from assembly to a certified 1e-8 residual.
+
+The roofline object is for the dominant kernel, the FP32 3x3 blocked SpMV
+(k_bsrmv_tiles; the CG's k_cg_spmv runs the same tile loop): algorithmic
+bytes per launch = 36*nnz (values) + 4*nnz (colidx) + 4*(N+1) (rowptr) +
+12*N (x) + 12*N (y), timed with HIP events on the stream it runs on.
 """
 import argparse
 import ctypes as C
@@ -30,6 +34,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
 
 
 def parse():
@@ -39,11 +44,15 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=96, help="cubes per edge of the tet grid")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--plain-maxiters", type=int, default=40000)
-    ap.add_argument("--cg-maxiters", type=int, default=20000)
+    ap.add_argument("--train-samples", type=int, default=2000)
+    ap.add_argument("--train-n", type=int, default=32)
+    ap.add_argument("--cg-maxiters", type=int, default=40000)
+    ap.add_argument("--plain-maxiters", type=int, default=250000)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="index, assembly, SpMV and a copy of known size only (profiling)")
     return ap.parse_args()
 
 
@@ -55,11 +64,11 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(multi-GPU sharding is design-only)"}))
+                                   "(the multi-GPU sharding is a design, not built)"}))
         sys.exit(2)
     import torch
     import simit_amd as S
-    from simit_amd import meshes
+    from simit_amd import meshes, learned
 
     torch.cuda.init()
     ctx = S.Context(0)
@@ -68,8 +77,8 @@ def main():
     t0 = time.time()
     prob = meshes.tet_problem(n, seed=args.seed, field="grf", mean_log10=5.0, std_log10=1.0,
                               corr_len=8.0)
-    log(f"[bench] generated mesh n={n} nverts={prob['nverts']} ntets={prob['endpoints'].shape[0]}"
-        f" in {time.time() - t0:.1f}s")
+    log(f"[bench] mesh n={n} nverts={prob['nverts']} ntets={prob['endpoints'].shape[0]}"
+        f" generated in {time.time() - t0:.1f}s")
     dev = {k: torch.from_numpy(np.ascontiguousarray(prob[k])).cuda()
            for k in ("x", "c", "mu", "lam", "endpoints")}
     nv, nt = prob["nverts"], prob["endpoints"].shape[0]
@@ -78,14 +87,23 @@ def main():
     def ev():
         return torch.cuda.Event(enable_timing=True)
 
-    # ---- sparsity index (device) ---------------------------------------------------
-    torch.cuda.synchronize()
-    e0, e1 = ev(), ev()
-    e0.record(stream)
-    idx = ctx.build_index(nv, [dev["endpoints"]])
-    e1.record(stream)
-    torch.cuda.synchronize()
-    index_ms = e0.elapsed_time(e1)
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        e0, e1 = ev(), ev()
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    # ---- sparsity index (device) ---------------------------------------------------------
+    holder = {}
+
+    def build():
+        holder["idx"] = ctx.build_index(nv, [dev["endpoints"]])
+    index_ms = timed(build, 1)
+    idx = holder["idx"]
     nnz = idx.nnz
     log(f"[bench] index nnz={nnz} in {index_ms:.2f} ms")
 
@@ -93,84 +111,114 @@ def main():
     hi = torch.empty(nnz * 9, dtype=torch.float32, device="cuda")
     lo = torch.empty_like(hi)
     rhs = torch.empty(ndof, dtype=torch.float64, device="cuda")
-    A = idx.bsr(3, S.F32X2, hi, lo)
-    opts = ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, precond=S.PRECOND_BLOCK_JACOBI,
-                          mixed=1, check_every=16)
+    A2 = idx.bsr(3, S.F32X2, hi, lo)
 
     def assemble():
         ctx.assemble_tets(idx, P, dev, dev, S.F32X2, hi, lo, rhs)
 
-    # ---- assembly kernel time ----------------------------------------------------------
+    # ---- assembly kernel -------------------------------------------------------------------
     assemble()
-    torch.cuda.synchronize()
-    reps = 5
-    e0.record(stream)
-    for _ in range(reps):
-        assemble()
-    e1.record(stream)
-    torch.cuda.synchronize()
-    asm_ms = e0.elapsed_time(e1) / reps
-    asm_bytes = 8 * 9 * nnz + 8 * ndof + 16 * nt + 16 * nt + 24 * nv
-    log(f"[bench] assembly {asm_ms:.3f} ms  {asm_bytes / asm_ms / 1e6:.0f} GB/s")
+    asm_ms = timed(assemble, 5)
+    # FP32 hi + lo values, FP64 rhs, tet endpoints, mu and lam, vertex x and c
+    asm_bytes = 8 * 9 * nnz + 8 * ndof + 16 * nt + 16 * nt + 24 * nv + 4 * nv
+    asm_gbs = asm_bytes / asm_ms / 1e6
+    log(f"[bench] assembly {asm_ms:.3f} ms  {asm_gbs:.0f} GB/s")
 
-    # ---- SpMV kernel time (FP32 values, the inner-iteration operator) ------------------
+    # ---- FP32 SpMV (the operator of the inner iterations) --------------------------------------
     A32 = idx.bsr(3, S.F32, hi)
     xs = torch.randn(ndof, dtype=torch.float32, device="cuda")
     ys = torch.empty_like(xs)
     ctx.bsrmv(A32, xs, ys)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    for _ in range(args.spmv_reps):
-        ctx.bsrmv(A32, xs, ys)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    spmv_ms = e0.elapsed_time(e1) / args.spmv_reps
+    spmv_ms = timed(lambda: ctx.bsrmv(A32, xs, ys), args.spmv_reps)
     spmv_bytes = 40 * nnz + 4 * (nv + 1) + 24 * nv
     spmv_gbs = spmv_bytes / spmv_ms / 1e6
     log(f"[bench] spmv f32 {spmv_ms * 1e3:.1f} us  {spmv_gbs:.0f} GB/s")
 
-    # ---- the step: assemble + mixed PCG to 1e-8 ---------------------------------------------
+    if args.kernels_only:
+        # a copy of known size for the counter calibration: y = 1*x + 0*y on
+        # 2^27 doubles reads 2 GiB and writes 1 GiB
+        xa = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        ya = torch.zeros_like(xa)
+        copy_ms = timed(lambda: ctx.axpby(1.0, xa, 0.0, ya), 5)
+        print(json.dumps({"kernels_only": True, "index_ms": index_ms, "assembly_ms": asm_ms,
+                          "assembly_GBps": asm_gbs, "spmv_us": spmv_ms * 1e3,
+                          "spmv_GBps": spmv_gbs, "axpby_2GiB_read_1GiB_write_ms": copy_ms}))
+        ctx.close()
+        return
+
+    # ---- the learned correction: training on 32^3 samples ----------------------------------------
+    net = ctx.gnn(seed=0, kappa=0.9)
+    tr = learned.train(ctx, net, n=args.train_n, nsamples=args.train_samples, log_every=500,
+                       log=log)
+    log(f"[bench] trained on {tr['steps']} samples of {args.train_n}^3 in {tr['seconds']:.1f}s "
+        f"(host field draws {tr['sample_seconds']:.1f}s, GPU steps {tr['step_seconds']:.1f}s)")
+
+    def opts(**kw):
+        return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
+
+    # ---- the step: assemble + network + learned mixed PCG to 1e-8 ---------------------------------
     def step():
         assemble()
+        linv, corr = net.precond(A2)
         x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
-        st = ctx.cg(A, rhs, x, opts)
-        return st, x
+        st = ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=1, learned=linv,
+                                     learned_corr=corr))
+        return st
 
     for w in range(args.warmup):
-        st, _ = step()
-        log(f"[bench] warmup {w}: iters={st.iters} ru={st.reliable_updates} "
-            f"true_rel={st.true_rel:.2e} {st.seconds:.3f}s")
+        st = step()
+        log(f"[bench] warmup {w}: iters={st.iters} true_rel={st.true_rel:.2e}")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     stats = []
     for s in range(args.steps):
-        st, x = step()
+        st = step()
         stats.append(st.as_dict())
         log(f"[bench] step {s}: iters={st.iters} ru={st.reliable_updates} "
             f"true_rel={st.true_rel:.2e} status={st.status}")
     torch.cuda.synchronize()
     step_s = (time.perf_counter() - t0) / args.steps
-    iters = stats[-1]["iters"]
     converged = all(s["status"] == 0 and s["true_rel"] <= 1e-8 for s in stats)
 
-    # ---- plain FP64 CG on the same GPU (the comparison the metric names) ----------------
-    plain = None
-    if not args.no_plain:
-        v64 = torch.empty(nnz * 9, dtype=torch.float64, device="cuda")
-        A64 = idx.bsr(3, S.F64, v64)
+    # precond inference time alone
+    pre_ms = timed(lambda: net.precond(A2), 3)
+
+    # ---- block-Jacobi PCG (mixed, FP64) and plain FP64 CG on the same GPU ----------------------------
+    def leg(name, fn):
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
+        t = time.perf_counter()
+        st = fn()
+        torch.cuda.synchronize()
+        r = {"seconds": round(time.perf_counter() - t, 4), "iters": st.iters,
+             "true_rel": st.true_rel, "converged": st.status == 0 and st.true_rel <= 1e-8}
+        log(f"[bench] {name}: {r}")
+        return r
+
+    def bj_mixed():
+        assemble()
+        x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
+        return ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_BLOCK_JACOBI, mixed=1))
+
+    v64 = torch.empty(nnz * 9, dtype=torch.float64, device="cuda")
+    A64 = idx.bsr(3, S.F64, v64)
+
+    def bj64():
         ctx.assemble_tets(idx, P, dev, dev, S.F64, v64, None, rhs)
         x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
-        st = ctx.cg(A64, rhs, x, ctx.cg_options(maxiters=args.plain_maxiters, tol=1e-8, rel=1,
-                                                 precond=S.PRECOND_NONE, check_every=64))
-        torch.cuda.synchronize()
-        plain = {"seconds": time.perf_counter() - t0, "iters": st.iters,
-                 "true_rel": st.true_rel, "converged": st.status == 0}
-        log(f"[bench] plain fp64 CG: {plain}")
-        del v64
+        return ctx.cg(A64, rhs, x, opts(precond=S.PRECOND_BLOCK_JACOBI))
 
-    # ---- CPU baseline: the oracle on a bounded sample of the same workload --------------------
+    def plain64():
+        ctx.assemble_tets(idx, P, dev, dev, S.F64, v64, None, rhs)
+        x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
+        return ctx.cg(A64, rhs, x, ctx.cg_options(maxiters=args.plain_maxiters, tol=1e-8, rel=1,
+                                                  precond=S.PRECOND_NONE, check_every=256))
+
+    legs = {"block_jacobi_pcg_mixed": leg("block-Jacobi PCG mixed", bj_mixed),
+            "block_jacobi_pcg_f64": leg("block-Jacobi PCG FP64", bj64)}
+    plain = leg("plain CG FP64", plain64) if not args.no_plain else None
+    assemble()  # restore the FP32 system (rhs was overwritten by the FP64 assembly)
+
+    # ---- CPU baseline: the oracle on a bounded sample of the same workload ----------------------------
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
@@ -179,57 +227,66 @@ def main():
         t0 = time.perf_counter()
         ov, of = O.tet_assemble(O.tet_params(load=prob["load"]), prob, rp, ci)
         t_asm = time.perf_counter() - t0
-        o2 = O.cg_opts(maxiters=2, tol=1e-8, precond=2)
-        o12 = O.cg_opts(maxiters=12, tol=1e-8, precond=2)
         t0 = time.perf_counter()
-        O.cg(rp, ci, ov, of, np.zeros(ndof), 3, o2)
+        O.cg(rp, ci, ov, of, np.zeros(ndof), 3, O.cg_opts(maxiters=2, tol=1e-8, precond=2))
         t2 = time.perf_counter() - t0
         t0 = time.perf_counter()
-        O.cg(rp, ci, ov, of, np.zeros(ndof), 3, o12)
-        t12 = time.perf_counter() - t0
-        per_iter = (t12 - t2) / 10
-        cpu_total = t_asm + t2 + per_iter * (iters - 2)
-        cpu = {"value": round(cpu_total, 3), "unit": "s", "cores": 1, "kind": "port",
-               "sample": (f"oracle (single-thread FP64 C restatement) on the full mesh: one "
-                          f"assembly ({t_asm:.2f}s) and 12 block-Jacobi PCG iterations "
-                          f"({per_iter * 1e3:.0f} ms/iter), extrapolated to the {iters} "
-                          f"iterations the GPU solve took"),
+        O.cg(rp, ci, ov, of, np.zeros(ndof), 3, O.cg_opts(maxiters=22, tol=1e-8, precond=2))
+        t22 = time.perf_counter() - t0
+        per_iter = (t22 - t2) / 20
+        it_cpu = legs["block_jacobi_pcg_f64"]["iters"]
+        cpu_total = t_asm + t2 + per_iter * (it_cpu - 2)
+        cpu = {"value": round(cpu_total, 2), "unit": "s", "cores": 1, "kind": "port",
+               "sample": (f"oracle (single-thread FP64 C restatement of the reference loops) "
+                          f"on the full {n}^3 mesh: one assembly ({t_asm:.2f}s) and 22 "
+                          f"block-Jacobi PCG iterations ({per_iter * 1e3:.0f} ms/iter), "
+                          f"extrapolated to the {it_cpu} iterations FP64 block-Jacobi PCG "
+                          f"needs on this system"),
                "nproc": os.cpu_count()}
         log(f"[bench] cpu baseline {cpu}")
 
     out = {
-        "metric": "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s",
-        "value": round(step_s, 5),
+        "metric": METRIC,
+        "value": round(step_s, 4),
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(step_s * 1e3, 3),
+        "ms_per_step": round(step_s * 1e3, 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 SpMV / f64 dots+refinement",
+        "dtype": "fp32 matrix/SpMV, fp64 dots+refinement, bf16 network",
         "data": "synthetic (seeded Gaussian random field material, generated on the host)",
-        "config": {"workload": f"tet5 elasticity {n}^3 cubes, {ndof} DOF, Jacobi-PCG mixed "
-                               f"precision to 1e-8", "nverts": nv, "ntets": nt, "nnz_blocks": nnz,
-                   "solver": "block-Jacobi PCG, FP32 SpMV, FP64 dots + reliable updates"},
-        "iterations": iters,
+        "config": {"workload": f"tet5 linear elasticity {n}^3 cubes x 5 tets, {ndof} DOF, "
+                               f"log10 E = 5 + GRF(std 1, corr 8 cells), learned mixed PCG "
+                               f"to a 1e-8 FP64 relative residual",
+                   "nverts": nv, "ntets": nt, "nnz_blocks": nnz,
+                   "train": f"{args.train_samples} seeded {args.train_n}^3 samples"},
+        "iterations": stats[-1]["iters"],
         "converged": converged,
         "true_rel": stats[-1]["true_rel"],
         "plain_cg_f64": plain,
-        "speedup_vs_plain_cg_f64": (round(plain["seconds"] / step_s, 2) if plain else None),
+        "speedup_vs_plain_cg_f64": (round(plain["seconds"] / step_s, 2)
+                                    if plain and plain["converged"] else None),
+        **legs,
+        "speedup_vs_block_jacobi_mixed": round(legs["block_jacobi_pcg_mixed"]["seconds"]
+                                               / step_s, 3),
+        "gnn_train_seconds": round(tr["seconds"], 2),
+        "gnn_precond_ms": round(pre_ms, 2),
         "index_build_ms": round(index_ms, 3),
         "assembly_ms": round(asm_ms, 4),
-        "assembly_GBps": round(asm_bytes / asm_ms / 1e6, 1),
+        "assembly_GBps": round(asm_gbs, 1),
         "spmv_us": round(spmv_ms * 1e3, 2),
         "spmv_GBps": round(spmv_gbs, 1),
-        "roofline": {"bound": "hbm", "kernel": "k_bsrmv3<SMH_F32> (FP32 3x3 BSR SpMV)",
+        "roofline": {"bound": "hbm", "kernel": "k_bsrmv_tiles<SMH_F32,3> (FP32 3x3 BSR SpMV)",
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": spmv_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    net.close()
     ctx.close()
 
 

@@ -263,16 +263,19 @@ typedef struct {
                             FP64 reliable updates r = b - A x on hi+lo
                             (needs SMH_F32X2); 2: the same with the inner
                             vectors (p, q, r, z) kept in FP64            */
-  double reliable_delta; /* mixed: update when ||r|| < delta*max ||r||      */
+  double reliable_delta; /* mixed: update when ||r|| < delta*max ||r||
+                            (default 0.3)                                   */
   int32_t check_every;   /* iterations between host convergence checks     */
   const float *learned;  /* SMH_PRECOND_LEARNED: device float[nrows*9],
                             Linv_i = L_i^-1 for A_ii = L_i L_i^T (row-major,
                             lower triangular), as smh_gnn_precond writes */
   int32_t zr_stop;       /* 1: also stop when z'r <= the threshold, as
                             thermal.sim's pre_conjgrad does (:115)          */
-  const uint16_t *learned_corr; /* SMH_PRECOND_LEARNED: device bfloat16
-                            [nnz*9], the normalised correction blocks C_k
-                            on A's pattern (zero on the diagonal); the
+  const uint8_t *learned_corr; /* SMH_PRECOND_LEARNED: device bytes
+                            [nnz*12], 4-byte aligned: the normalised
+                            correction blocks C_k on A's pattern (zero on
+                            the diagonal) as FP8 e4m3 of 32 C_k, three
+                            values and a zero pad byte per block row; the
                             preconditioner is
                             z_i = Linv_i^T (Linv_i r_i + sum_k C_k Linv_j r_j) */
 } smh_cg_opts;
@@ -309,8 +312,9 @@ int smh_gnn_destroy(smh_ctx *ctx, smh_gnn *g);
 int smh_gnn_get_params(smh_ctx *ctx, smh_gnn *g, float *host);
 int smh_gnn_set_params(smh_ctx *ctx, smh_gnn *g, const float *host);
 /* the preconditioner of A: linv = device float[nrows*9], corr = device
- * bfloat16 bits [nnz*9] (pass both as opts.learned / opts.learned_corr) */
-int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float *linv, uint16_t *corr);
+ * bytes [nnz*12] in the FP8 layout of opts.learned_corr (pass both as
+ * opts.learned / opts.learned_corr) */
+int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float *linv, uint8_t *corr);
 /* loss and gradient on A for nprobes probe vectors: probes = device
  * float[3*nrows*nprobes] (row-major [dof][probe]) or NULL to draw them from
  * seed; grads (host float[SMH_GNN_NPARAMS]) may be NULL */

@@ -331,10 +331,10 @@ class Gnn:
                                                        a.ctypes.data_as(C.c_void_p)))
 
     def precond(self, A: Bsr):
-        """(linv float32[nrows*9], corr bfloat16-bits int16[nnz*9]) device tensors."""
+        """(linv float32[nrows*9], corr uint8[nnz*12] FP8 blocks) device tensors."""
         import torch
         linv = torch.empty(A.nrows * 9, dtype=torch.float32, device="cuda")
-        corr = torch.empty(A.nnz * 9, dtype=torch.int16, device="cuda")
+        corr = torch.empty(A.nnz * 12, dtype=torch.uint8, device="cuda")
         self.ctx._check(self.ctx.lib.smh_gnn_precond(self.ctx.h, self.h, C.byref(A),
                                                     _ptr(linv), _ptr(corr)))
         return linv, corr

@@ -211,33 +211,56 @@ __device__ __forceinline__ void apply_precond(int pc, const double *r, double *z
   }
 }
 
-// K1: q = A p and the workgroup partial of p'q
+// K1: q = A p and the workgroup partial of p'q.  With dlo (mixed precision),
+// the inner operator is hi + blockdiag(sum_j lo_ij): the FP32 values plus
+// each row's rounding residuals moved onto its diagonal block, so its block
+// row sums equal the exact matrix's and the difference to the exact matrix
+// annihilates constant (rigid translation) vectors.
 template <typename VT>
 struct EpiPq {
   VT *q;
   const VT *p;
+  const float *dlo;
   double d;
   int bd;
   __device__ void operator()(uint32_t r, const VT *s) {
+    VT sv[3];
+    for (int a = 0; a < bd; ++a) sv[a] = s[a];
+    if (dlo && bd == 3) {
+      const float *D = dlo + (uint64_t)r * 9;
+      for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) sv[a] += (VT)D[3 * a + b] * p[(uint64_t)r * 3 + b];
+    }
     for (int a = 0; a < bd; ++a) {
       const uint64_t o = (uint64_t)r * bd + a;
-      q[o] = s[a];
-      d += (double)s[a] * (double)p[o];
+      q[o] = sv[a];
+      d += (double)sv[a] * (double)p[o];
     }
   }
 };
 
+// dlo_i = sum over the blocks of row i of the low parts of an SMH_F32X2 matrix
+__global__ void k_lo_rowsum(uint32_t n, const uint32_t *__restrict__ rp,
+                            const float *__restrict__ lo, float *__restrict__ dlo) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
+    for (int q = 0; q < 9; ++q) s[q] += (double)lo[(uint64_t)k * 9 + q];
+  for (int q = 0; q < 9; ++q) dlo[(uint64_t)i * 9 + q] = (float)s[q];
+}
+
 template <int IP, int BD>
 __global__ void __launch_bounds__(TileShape<BD>::kThreads)
 k_cg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP>::T *__restrict__ q,
-          const CGState *S, double *part) {
+          const CGState *S, const float *dlo, double *part) {
   using VT = typename Val<IP>::T;
   __shared__ TileSmem<IP, BD> sm;
   if (S->done) return;
   uint32_t k0, k1;
   wg_tiles(A, k0, k1);
   XPlain<VT> xg{p, BD};
-  EpiPq<VT> epi{q, p, 0.0, BD};
+  EpiPq<VT> epi{q, p, dlo, 0.0, BD};
   spmv_tiles<IP, BD>(A, k0, k1, sm, xg, epi);
   const double t = block_sum<TileShape<BD>::kWaves>(epi.d, sm.red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
@@ -296,7 +319,7 @@ k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT
 }
 
 // K2b (learned preconditioner): z_i = Linv_i^T (u_i + sum_k C_k u_j) over
-// the pattern of A (C in BF16, zero on the diagonal), and the workgroup
+// the pattern of A (C in FP8, zero on the diagonal), and the workgroup
 // partials of z'r and r'r.  check_done = 0 on the residual-replacement path,
 // which runs after the stopping test already set done.
 template <typename VT>
@@ -559,11 +582,13 @@ TileDev tile_dev(const smh_bsr *A, const smh_tile_plan *p, unsigned *grid) {
 
 // ---- host driver ---------------------------------------------------------------
 struct Work {
-  void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *u = nullptr;
+  void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *u = nullptr,
+       *dlo = nullptr;
   CGState *S = nullptr;
   double *part = nullptr;
   void release() {
-    dfree(r); dfree(z); dfree(p); dfree(q); dfree(dinv); dfree(u); dfree(S); dfree(part);
+    dfree(r); dfree(z); dfree(p); dfree(q); dfree(dinv); dfree(u); dfree(dlo); dfree(S);
+    dfree(part);
   }
 };
 
@@ -582,7 +607,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   const unsigned nbv = grid_for(n, 256, 2048);
   const int pc = o->precond;
   const bool learned = BD == 3 && pc == SMH_PRECOND_LEARNED;
-  constexpr int LP = sizeof(VT) == 4 ? kBF16F : kBF16D;
+  constexpr int LP = sizeof(VT) == 4 ? kFP8F : kFP8D;
   smh_bsr Cb = *Am;
   Cb.vals = learned ? (const void *)o->learned_corr : Am->vals;
   Cb.vals_lo = nullptr;
@@ -618,6 +643,12 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   if (learned) {
     TRY(dmalloc(ctx, &w.u, sizeof(VT) * n));
     w.dinv = nullptr;
+  }
+  // mixed precision: the inner operator keeps the exact block row sums
+  if (o->mixed && BD == 3 && Am->precision == SMH_F32X2 && Am->vals_lo) {
+    TRY(dmalloc(ctx, &w.dlo, sizeof(float) * (uint64_t)nrows * 9));
+    hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream, nrows,
+                       Am->rowptr, (const float *)Am->vals_lo, (float *)w.dlo);
   }
   pdata = learned ? (const void *)o->learned : (const void *)w.dinv;
   TRY(dmalloc(ctx, (void **)&w.S, sizeof(CGState)));
@@ -684,7 +715,8 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
                           : check;
     for (int k = 0; k < batch; ++k) {
       hipLaunchKernelGGL((k_cg_spmv<IP, BD>), dim3(gt), dim3(tile_threads_host(BD)), 0, ctx->stream, T,
-                         (const VT *)w.p, (VT *)w.q, (const CGState *)w.S, w.part);
+                         (const VT *)w.p, (VT *)w.q, (const CGState *)w.S,
+                         (const float *)w.dlo, w.part);
       hipLaunchKernelGGL(k_cg_r1, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
                          (int)gt);
       hipLaunchKernelGGL((k_cg_update<VT, BD>), dim3(nbe), dim3(256), 0, ctx->stream, nrows, pc,
@@ -815,7 +847,7 @@ extern "C" void smh_cg_default_opts(smh_cg_opts *o) {
   o->tol = 1e-8;
   o->rel = 1;
   o->precond = SMH_PRECOND_NONE;
-  o->reliable_delta = 0.1;
+  o->reliable_delta = 0.3;
   o->check_every = 16;
 }
 
@@ -829,7 +861,8 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
             "smh_cg_solve: unknown preconditioner %d", o->precond);
   SMH_CHECK(ctx,
             o->precond != SMH_PRECOND_LEARNED ||
-                (A->bdim == 3 && o->learned && o->learned_corr),
+                (A->bdim == 3 && o->learned && o->learned_corr &&
+                 ((uintptr_t)o->learned_corr & 3u) == 0),
             SMH_ERR_INVALID, "smh_cg_solve: the learned preconditioner needs 3x3 blocks, "
                              "opts.learned and opts.learned_corr (smh_gnn_precond)");
   SMH_CHECK(ctx, A->bdim == 3 || o->precond <= SMH_PRECOND_JACOBI, SMH_ERR_INVALID,

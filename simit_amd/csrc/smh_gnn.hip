@@ -68,6 +68,29 @@ __device__ __forceinline__ uint16_t f2bf(float x) {
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 __device__ __forceinline__ float bfr(float x) { return bf2f(f2bf(x)); }
 
+// float -> e4m3 (bias 7, 3 mantissa bits, subnormals below 2^-6), round to
+// nearest even, saturating at 448; the inverse is e4m3_to_float
+__device__ __forceinline__ uint32_t float_to_e4m3(float x) {
+  const uint32_t u = __float_as_uint(x);
+  const uint32_t sign = (u >> 24) & 0x80u;
+  const float a = fabsf(x);
+  if (!(a > 0.0f)) return sign;
+  if (a >= 448.0f) return sign | 0x7Eu;
+  int ea = (int)((u >> 23) & 0xFFu) - 127;
+  if (ea < -6) {  // subnormal: multiples of 2^-9
+    const uint32_t m = (uint32_t)rintf(a * 512.0f);
+    return sign | (m >= 8u ? 0x08u : m);
+  }
+  const uint32_t man = u & 0x7FFFFFu;
+  uint32_t r = (man + 0x7FFFFu + ((man >> 20) & 1u)) >> 20;
+  if (r == 8u) {
+    r = 0;
+    ++ea;
+  }
+  if (ea > 8) return sign | 0x7Eu;
+  return sign | ((uint32_t)(ea + 7) << 3) | r;
+}
+
 // ---- graph ---------------------------------------------------------------------------
 __global__ void k_graph_rows(uint32_t n, const uint32_t *__restrict__ rp,
                              const uint32_t *__restrict__ ci, uint32_t *__restrict__ brow,
@@ -429,7 +452,7 @@ __device__ __forceinline__ OutScale out_scale(const float *R, uint64_t k, uint64
 __global__ void k_gnn_out(uint64_t nnz, const uint32_t *__restrict__ brow,
                           const uint32_t *__restrict__ ci, const uint32_t *__restrict__ t,
                           const uint32_t *__restrict__ deg, const float *__restrict__ R,
-                          float kappa, uint16_t *__restrict__ corr, float *__restrict__ c32) {
+                          float kappa, uint32_t *__restrict__ corr, float *__restrict__ c32) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
   const uint32_t i = brow[k], j = ci[k];
@@ -445,10 +468,14 @@ __global__ void k_gnn_out(uint64_t nnz, const uint32_t *__restrict__ brow,
     for (int a = 0; a < 3; ++a)
       for (int b = 0; b < 3; ++b) C[3 * a + b] = f * (lower ? o.S[3 * b + a] : o.S[3 * a + b]);
   }
-  for (int q = 0; q < 9; ++q) {
-    corr[k * 9 + q] = f2bf(C[q]);
-    if (c32) c32[k * 9 + q] = C[q];
-  }
+  // FP8 storage of 2^5 C, three values and a zero pad byte per block row
+  if (corr)
+    for (int a = 0; a < 3; ++a)
+      corr[k * 3 + a] = float_to_e4m3(32.0f * C[3 * a]) |
+                        (float_to_e4m3(32.0f * C[3 * a + 1]) << 8) |
+                        (float_to_e4m3(32.0f * C[3 * a + 2]) << 16);
+  if (c32)
+    for (int q = 0; q < 9; ++q) c32[k * 9 + q] = C[q];
 }
 
 // ---- training: probes, loss, output backward ------------------------------------------------
@@ -761,7 +788,7 @@ struct smh_gnn {
   uint32_t *brow = nullptr, *tmap = nullptr, *deg = nullptr;
   int *err = nullptr;
   float *linv = nullptr, *ahat = nullptr, *R = nullptr, *c32 = nullptr;
-  uint16_t *fe = nullptr, *fv = nullptr, *e = nullptr, *corr_tmp = nullptr;
+  uint16_t *fe = nullptr, *fv = nullptr, *e = nullptr;
   float *h[kLayers + 1] = {}, *a[kLayers] = {}, *U[kLayers] = {};
   uint16_t *m[kLayers] = {};
   float *V = nullptr, *W = nullptr, *G = nullptr, *dR = nullptr, *dX = nullptr, *dh = nullptr,
@@ -811,7 +838,6 @@ int ensure_work(smh_ctx *ctx, smh_gnn *g, uint32_t n, uint64_t nnz, int K, bool 
   WA(g->fe, E * kF);
   WA(g->fv, N * kF);
   WA(g->e, E * kH);
-  WA(g->corr_tmp, E * 9);
   const int nh = train ? kLayers + 1 : 2;
   for (int l = 0; l < nh; ++l) WA(g->h[l], N * kH);
   const int nm = train ? kLayers : 1;
@@ -948,7 +974,7 @@ int loss_grad(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int K, const float *pr
   rc = forward(ctx, g, A, true);
   if (rc) return rc;
   hipLaunchKernelGGL(k_gnn_out, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, g->brow,
-                     A->colidx, g->tmap, g->deg, g->R, g->kappa, g->corr_tmp, g->c32);
+                     A->colidx, g->tmap, g->deg, g->R, g->kappa, (uint32_t *)nullptr, g->c32);
   const uint64_t nv = (uint64_t)3 * n * K;
   const float *V = probes;
   if (!V) {
@@ -1084,16 +1110,19 @@ extern "C" int smh_gnn_set_params(smh_ctx *ctx, smh_gnn *g, const float *host) {
 }
 
 extern "C" int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float *linv,
-                               uint16_t *corr) {
+                               uint8_t *corr) {
   if (!ctx) return SMH_ERR_INVALID;
   SMH_CHECK(ctx, g && linv && corr, SMH_ERR_INVALID, "smh_gnn_precond: null argument");
+  SMH_CHECK(ctx, ((uintptr_t)corr & 3u) == 0, SMH_ERR_INVALID,
+            "smh_gnn_precond: corr must be 4-byte aligned");
   int rc = check_A(ctx, A, "smh_gnn_precond");
   if (rc) return rc;
   if (A->nrows == 0) return SMH_OK;
   if ((rc = ensure_work(ctx, g, A->nrows, A->nnz, g->cap_k, false))) return rc;
   if ((rc = forward(ctx, g, A, false))) return rc;
   hipLaunchKernelGGL(k_gnn_out, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream, A->nnz,
-                     g->brow, A->colidx, g->tmap, g->deg, g->R, g->kappa, corr, nullptr);
+                     g->brow, A->colidx, g->tmap, g->deg, g->R, g->kappa, (uint32_t *)corr,
+                     nullptr);
   SMH_HIP(ctx, hipMemcpyAsync(linv, g->linv, sizeof(float) * 9 * (size_t)A->nrows,
                               hipMemcpyDeviceToDevice, ctx->stream));
   SMH_LAUNCH_CHECK(ctx);

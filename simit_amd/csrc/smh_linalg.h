@@ -46,18 +46,21 @@ template <> struct Val<SMH_F64> {
   using S = double;  // stored value type
   static constexpr bool kLo = false;
   static constexpr bool kBF16 = false;
+  static constexpr bool kFP8 = false;
 };
 template <> struct Val<SMH_F32> {
   using T = float;
   using S = float;
   static constexpr bool kLo = false;
   static constexpr bool kBF16 = false;
+  static constexpr bool kFP8 = false;
 };
 template <> struct Val<SMH_F32X2> {
   using T = double;
   using S = float;
   static constexpr bool kLo = true;
   static constexpr bool kBF16 = false;
+  static constexpr bool kFP8 = false;
 };
 constexpr int kF32D = 3;  // float values driven by double vectors
 template <> struct Val<kF32D> {
@@ -65,6 +68,7 @@ template <> struct Val<kF32D> {
   using S = float;
   static constexpr bool kLo = false;
   static constexpr bool kBF16 = false;
+  static constexpr bool kFP8 = false;
 };
 constexpr int kBF16F = 4;  // bfloat16 values (bits in uint16), float vectors
 template <> struct Val<kBF16F> {
@@ -72,6 +76,7 @@ template <> struct Val<kBF16F> {
   using S = uint16_t;
   static constexpr bool kLo = false;
   static constexpr bool kBF16 = true;
+  static constexpr bool kFP8 = false;
 };
 constexpr int kBF16D = 5;  // bfloat16 values, double vectors
 template <> struct Val<kBF16D> {
@@ -79,7 +84,45 @@ template <> struct Val<kBF16D> {
   using S = uint16_t;
   static constexpr bool kLo = false;
   static constexpr bool kBF16 = true;
+  static constexpr bool kFP8 = false;
 };
+
+// 8-bit floats (e4m3: bias 7, 3 mantissa bits, subnormals below 2^-6) of
+// 3x3 blocks stored as 3 block rows of 4 bytes (3 values and a zero pad), so
+// one lane reads one aligned 32-bit word per block row.  The stored values
+// are the block's entries times 2^5 (kFP8Scale undoes that).
+constexpr int kFP8F = 6;  // FP8 values, float vectors
+constexpr int kFP8D = 7;  // FP8 values, double vectors
+constexpr float kFP8Scale = 1.0f / 32.0f;
+template <> struct Val<kFP8F> {
+  using T = float;
+  using S = uint32_t;
+  static constexpr bool kLo = false;
+  static constexpr bool kBF16 = false;
+  static constexpr bool kFP8 = true;
+};
+template <> struct Val<kFP8D> {
+  using T = double;
+  using S = uint32_t;
+  static constexpr bool kLo = false;
+  static constexpr bool kBF16 = false;
+  static constexpr bool kFP8 = true;
+};
+__host__ __device__ __forceinline__ float e4m3_to_float(uint32_t b) {
+  const uint32_t e = (b >> 3) & 15u, m = b & 7u;
+#ifdef __HIP_DEVICE_COMPILE__
+  const float v = e ? __uint_as_float(((e + 120u) << 23) | (m << 20)) : (float)m * 0.001953125f;
+#else
+  float v;
+  if (e) {
+    const uint32_t bits = ((e + 120u) << 23) | (m << 20);
+    __builtin_memcpy(&v, &bits, 4);
+  } else {
+    v = (float)m * 0.001953125f;
+  }
+#endif
+  return (b & 0x80u) ? -v : v;
+}
 
 // stored value -> compute type
 template <int PREC>
@@ -141,6 +184,11 @@ struct TileRegs {
       const uint32_t blk = base + s + u * SH::kPerRound;
       const bool act = blk < end;
       col[u] = act ? A.colidx[blk] : 0u;
+      if constexpr (Val<PREC>::kFP8) {
+        v[u][0] = act ? __builtin_nontemporal_load((const uint32_t *)A.vals + (uint64_t)blk * BD + bi)
+                      : 0u;
+        continue;
+      }
       const uint64_t o = (uint64_t)blk * BB + bi * BD;
 #pragma unroll
       for (int b = 0; b < BD; ++b) {
@@ -195,7 +243,11 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
       T acc = 0;
 #pragma unroll
       for (int b = 0; b < BD; ++b) {
-        T val = val_of<PREC>(cur.v[u][b]);
+        T val;
+        if constexpr (Val<PREC>::kFP8)
+          val = (T)(e4m3_to_float((uint32_t)cur.v[u][0] >> (8 * b)) * kFP8Scale);
+        else
+          val = val_of<PREC>(cur.v[u][b]);
         if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][b];
         acc += val * xg(cur.col[u], b);
       }

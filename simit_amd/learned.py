@@ -1,0 +1,91 @@
+"""Training driver of the learned block-Jacobi correction (smh_gnn).
+
+The samples are the tet-elasticity systems this library assembles: one
+n^3 x 5 tet grid (clamped z = 0 face, uniform body load) whose Young's
+modulus is redrawn per sample as log10 E = mean + std * GRF (seeded,
+FFT-smoothed white noise, correlation length in cells), Poisson ratio 0.3.
+Each sample is assembled on the GPU (FP32 hi/lo) and fed to one Adam step of
+the network's own forward and backward kernels.  Everything but the
+material draw (host numpy, seeded) runs on the device.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import meshes
+
+
+def tet_material(n: int, seed: int, mean_log10=5.0, std_log10=1.0, corr_len=8.0, nu=0.3):
+    """(mu, lam) per tet of the n^3 x 5 grid for one seeded random field."""
+    g = meshes.gaussian_random_field(n, corr_len, seed).ravel()
+    E = 10.0 ** np.repeat(mean_log10 + std_log10 * g, 5)
+    return meshes.lame(E, nu)
+
+
+class TetSampler:
+    """One resident n^3 x 5 tet mesh whose material is redrawn per sample."""
+
+    def __init__(self, ctx, n: int = 32, mean_log10=5.0, std_log10=1.0, corr_len=8.0):
+        import torch
+        from . import TetParams, F32X2
+        self.ctx, self.n = ctx, n
+        self.field = (mean_log10, std_log10, corr_len)
+        prob = meshes.tet_problem(n, seed=0)
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        self.dev = {"x": dev(prob["x"]), "c": dev(prob["c"].astype(np.int32)),
+                    "endpoints": dev(prob["endpoints"].astype(np.int32)),
+                    "mu": dev(prob["mu"]), "lam": dev(prob["lam"])}
+        self.idx = ctx.build_index(prob["nverts"], [self.dev["endpoints"]])
+        self.params = TetParams(1.0, 0.0, (C.c_double * 3)(*prob["load"]), 1.0)
+        self.hi = torch.empty(self.idx.nnz * 9, dtype=torch.float32, device="cuda")
+        self.lo = torch.empty_like(self.hi)
+        self.rhs = torch.empty(prob["nverts"] * 3, dtype=torch.float64, device="cuda")
+        self.A = self.idx.bsr(3, F32X2, self.hi, self.lo)
+        self._prec = F32X2
+
+    def sample(self, seed: int):
+        """Assemble the system of material seed (returns the resident Bsr)."""
+        import torch
+        mu, lam = tet_material(self.n, seed, *self.field)
+        self.dev["mu"].copy_(torch.from_numpy(mu), non_blocking=False)
+        self.dev["lam"].copy_(torch.from_numpy(lam), non_blocking=False)
+        self.ctx.assemble_tets(self.idx, self.params, self.dev, self.dev, self._prec, self.hi,
+                               self.lo, self.rhs)
+        return self.A
+
+
+def train(ctx, net, n: int = 32, nsamples: int = 2000, epochs: int = 1, lr: float = 3e-3,
+          lr_final: float = 3e-4, nprobes: int = 4, seed0: int = 10_000, log_every: int = 0,
+          log=None, sampler: TetSampler | None = None):
+    """Train net on nsamples seeded n^3 systems; returns a dict of timings and
+    the loss curve (the loss is read back only every log_every steps)."""
+    smp = sampler or TetSampler(ctx, n)
+    steps = nsamples * epochs
+    curve = []
+    t_gen = t_gpu = 0.0
+    import torch
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        seed = seed0 + (s % nsamples)
+        a = time.perf_counter()
+        A = smp.sample(seed)
+        b = time.perf_counter()
+        # cosine decay from lr to lr_final
+        cur = lr_final + 0.5 * (lr - lr_final) * (1.0 + math.cos(math.pi * s / max(steps - 1, 1)))
+        want = bool(log_every) and (s % log_every == 0 or s == steps - 1)
+        loss = net.train_step(A, nprobes=nprobes, seed=seed * 7919 + s, lr=cur, want_loss=want)
+        c = time.perf_counter()
+        t_gen += b - a
+        t_gpu += c - b
+        if want:
+            curve.append((s, loss))
+            if log:
+                log(f"[train] step {s} loss {loss:.4f} lr {cur:.2e}")
+    torch.cuda.synchronize()
+    return {"steps": steps, "seconds": time.perf_counter() - t0, "sample_seconds": t_gen,
+            "step_seconds": t_gpu, "curve": curve}

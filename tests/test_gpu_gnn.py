@@ -41,9 +41,22 @@ def tet_system(ctx, n, seed, corr=3.0):
     return {"idx": idx, "v64": v64, "hi": hi, "lo": lo, "rhs": rhs, "dev": d, "prob": prob}
 
 
-def bf16_bits_to_f32(t):
-    u = t.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
-    return u.view(np.float32).astype(np.float64)
+def fp8_blocks(t):
+    """Decode the [nnz*12] FP8 correction (e4m3 of 32 C, 3 values + pad per
+    block row) into [nnz, 3, 3] float64."""
+    b = t.cpu().numpy().reshape(-1, 3, 4)[:, :, :3].astype(np.int64)
+    e, m = (b >> 3) & 15, b & 7
+    v = np.where(e > 0, (1.0 + m / 8.0) * np.exp2(e - 7.0), m * 2.0 ** -9)
+    return np.where(b & 0x80, -v, v) / 32.0
+
+
+def fp8_round(x):
+    """x rounded to the FP8 grid the kernels store (e4m3 of 32 x) / 32."""
+    y = np.abs(32.0 * x)
+    ex = np.floor(np.log2(np.maximum(y, 2.0 ** -6)))
+    step = np.exp2(ex - 3)
+    q = np.round(y / step) * step          # numpy rounds half to even
+    return np.sign(x) * np.minimum(q, 448.0) / 32.0
 
 
 def random_params(seed, scale=1.0):
@@ -70,21 +83,25 @@ def test_forward_matches_oracle(ctx):
     Lo = aux["ft"]["Linv"].reshape(-1, 9)
     np.testing.assert_allclose(linv.cpu().numpy().reshape(-1, 9), Lo, rtol=2e-6,
                                atol=2e-6 * np.abs(Lo).max())
-    Cg = bf16_bits_to_f32(corr).reshape(-1, 3, 3)
+    Cg = fp8_blocks(corr)
     Co = aux["C"]
     g = aux["g"]
     # exact symmetry of the stored correction, zero diagonal blocks
     assert np.array_equal(Cg, Cg[g["t"]].transpose(0, 2, 1))
     assert np.all(Cg[g["diag"]] == 0.0)
-    err = np.abs(Cg - Co)
+    # FP8 storage: each value within half an e4m3 step of the oracle's value
+    # rounded the same way, up to the BF16-chain noise of the network
+    Cr = fp8_round(Co)
     scale = np.abs(Co).max()
     assert scale > 1e-3
-    assert np.linalg.norm(Cg - Co) <= 1e-2 * np.linalg.norm(Co), np.linalg.norm(Cg - Co)
-    assert err.max() <= 3e-2 * scale, err.max() / scale
-    # Gershgorin bound of every row (kappa = 0.9) survives the BF16 storage
+    close = np.abs(Cg - Cr) <= 1e-6 * scale
+    assert close.mean() > 0.95, close.mean()
+    assert np.linalg.norm(Cg - Co) <= 5e-2 * np.linalg.norm(Co), np.linalg.norm(Cg - Co)
+    assert np.abs(Cg - Co).max() <= 0.07 * scale
+    # Gershgorin bound of every row (kappa = 0.9) survives the FP8 storage
     rows = np.zeros(rp.size - 1)
     np.add.at(rows, g["brow"], np.sqrt((Cg * Cg).sum(axis=(1, 2))))
-    assert rows.max() < 0.9 * (1 + 2 ** -7)
+    assert rows.max() < 0.9 * (1 + 2 ** -4)
     net.close()
 
 
@@ -157,6 +174,37 @@ def test_learned_pcg_converges(ctx, oracle, mixed):
     f = sy["rhs"].cpu().numpy()
     r = f - oracle.bsrmv(rp, ci, vals, x.cpu().numpy(), 3)
     assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(f)
+    if mixed == 0:
+        # the same PCG on the CPU with the preconditioner decoded from the
+        # stored Linv and FP8 blocks takes the same number of iterations
+        import scipy.sparse as sp
+        nr = rp.size - 1
+        Li = linv.cpu().numpy().astype(np.float64).reshape(-1, 3, 3)
+        Cb = fp8_blocks(corr)
+        brow = np.repeat(np.arange(nr), np.diff(rp.astype(np.int64)))
+        Bh = Cb + np.where((brow == ci)[:, None, None], np.eye(3), 0.0)
+        Bk = Li[brow].transpose(0, 2, 1) @ Bh @ Li[ci]
+        shape = (3 * nr, 3 * nr)
+        Bm = sp.bsr_matrix((Bk, ci.astype(np.int64), rp.astype(np.int64)), shape=shape)
+        Am = sp.bsr_matrix((vals.reshape(-1, 3, 3), ci.astype(np.int64), rp.astype(np.int64)),
+                           shape=shape)
+        xx = np.zeros_like(f)
+        rr = f.copy()
+        zz = Bm @ rr
+        pp = zz.copy()
+        rz = rr @ zz
+        it = 0
+        while np.linalg.norm(rr) > 1e-8 * np.linalg.norm(f) and it < 5000:
+            qq = Am @ pp
+            al = rz / (pp @ qq)
+            xx += al * pp
+            rr -= al * qq
+            zz = Bm @ rr
+            rzn = rr @ zz
+            pp = zz + (rzn / rz) * pp
+            rz = rzn
+            it += 1
+        assert abs(it - st.iters) <= max(2, st.iters // 50), (it, st.iters)
     net.close()
 
 
