@@ -34,6 +34,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per FP32 SpMV launch on the 96^3 x 5 system, from two
+# `rocprofv3 --pmc` runs of `bench.py --kernels-only`: FETCH_SIZE 256.05 MiB
+# x 2.0 + WRITE_SIZE 11.75 MiB x 1.0 (factors from the same run's copy of
+# known size; summary and commands in profiles/spmv_f32_tet96.txt)
+SPMV_TRAFFIC_96 = 549_300_000
 METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
 
 
@@ -281,7 +286,8 @@ def main():
         "spmv_GBps": round(spmv_gbs, 1),
         "roofline": {"bound": "hbm", "kernel": "k_bsrmv_tiles<SMH_F32,3> (FP32 3x3 BSR SpMV)",
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": SPMV_TRAFFIC_96 if n == 96 else None,
                      "algorithmic_bytes": spmv_bytes},
         "cpu_baseline": cpu,
     }
