@@ -58,6 +58,10 @@ def parse():
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--kernels-only", action="store_true",
                     help="index, assembly, SpMV and a copy of known size only (profiling)")
+    ap.add_argument("--workload", choices=("tet", "hex"), default="tet",
+                    help="tet: 96^3 x 5 learned-PCG elasticity (default); hex: 192^3 trilinear "
+                         "hex Poisson, mixed Jacobi-PCG")
+    ap.add_argument("--hex-n", type=int, default=192)
     return ap.parse_args()
 
 
@@ -77,6 +81,10 @@ def main():
 
     torch.cuda.init()
     ctx = S.Context(0)
+    if args.workload == "hex":
+        run_hex(args, ctx)
+        ctx.close()
+        return
     stream = torch.cuda.current_stream()
     n = args.n
     t0 = time.time()
@@ -294,6 +302,146 @@ def main():
     print(json.dumps(out), flush=True)
     net.close()
     ctx.close()
+
+
+def run_hex(args, ctx):
+    """192^3 trilinear-hex scalar Poisson (7.15M unknowns, 27-point rows): log10 k
+    lognormal (std 1.5, correlation length 12 cells), T = 0 on x = 0, uniform
+    source.  Step = assemble FP32 hi/lo + mixed Jacobi-PCG (FP32 SpMV, FP64
+    dots and reliable updates) to a certified 1e-8.  The learned correction
+    works on 3x3 blocks only, so this workload runs without it."""
+    import torch
+    import simit_amd as S
+    from simit_amd import meshes
+    stream = torch.cuda.current_stream()
+    n = args.hex_n
+    t0 = time.time()
+    prob = meshes.hex_problem(n, seed=args.seed, std_log10=1.5, corr_len=12.0, source=1.0)
+    log(f"[bench] hex mesh n={n} nverts={prob['nverts']} generated in {time.time() - t0:.1f}s")
+    dev = {"dirichlet": torch.from_numpy(prob["dirichlet"]).cuda(),
+           "kappa": torch.from_numpy(prob["kappa"]).cuda(),
+           "endpoints": torch.from_numpy(np.ascontiguousarray(prob["endpoints"])).cuda()}
+    nv, nh = prob["nverts"], prob["endpoints"].shape[0]
+
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    holder = {}
+    index_ms = timed(lambda: holder.__setitem__("idx", ctx.build_index(nv, [dev["endpoints"]])), 1)
+    idx = holder["idx"]
+    nnz = idx.nnz
+    log(f"[bench] index nnz={nnz} in {index_ms:.1f} ms")
+    P = S.HexParams(prob["h"], prob["source"], 1.0)
+    hi = torch.empty(nnz, dtype=torch.float32, device="cuda")
+    lo = torch.empty_like(hi)
+    rhs = torch.empty(nv, dtype=torch.float64, device="cuda")
+    A2 = idx.bsr(1, S.F32X2, hi, lo)
+
+    def assemble():
+        ctx.assemble_hex(idx, P, dev, dev, S.F32X2, hi, lo, rhs)
+    assemble()
+    asm_ms = timed(assemble, 3)
+    asm_bytes = 8 * nnz + 8 * nv + 32 * nh + 8 * nh + 4 * nv
+    asm_gbs = asm_bytes / asm_ms / 1e6
+    A32 = idx.bsr(1, S.F32, hi)
+    xs = torch.randn(nv, dtype=torch.float32, device="cuda")
+    ys = torch.empty_like(xs)
+    ctx.bsrmv(A32, xs, ys)
+    spmv_ms = timed(lambda: ctx.bsrmv(A32, xs, ys), 20)
+    spmv_bytes = 8 * nnz + 4 * (nv + 1) + 8 * nv
+    spmv_gbs = spmv_bytes / spmv_ms / 1e6
+    log(f"[bench] assembly {asm_ms:.2f} ms {asm_gbs:.0f} GB/s; spmv {spmv_ms * 1e3:.1f} us "
+        f"{spmv_gbs:.0f} GB/s")
+
+    def opts(**kw):
+        return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
+
+    def step():
+        assemble()
+        x = torch.zeros(nv, dtype=torch.float64, device="cuda")
+        return ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=1))
+
+    for w in range(args.warmup):
+        st = step()
+        log(f"[bench] warmup {w}: iters={st.iters} true_rel={st.true_rel:.2e}")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = [step().as_dict() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    step_s = (time.perf_counter() - t0) / args.steps
+    log(f"[bench] step {step_s:.3f}s iters={stats[-1]['iters']} true_rel={stats[-1]['true_rel']:.2e}")
+    v64 = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    A64 = idx.bsr(1, S.F64, v64)
+
+    def leg(name, pc, maxit):
+        ctx.assemble_hex(idx, P, dev, dev, S.F64, v64, None, rhs)
+        x = torch.zeros(nv, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = ctx.cg(A64, rhs, x, ctx.cg_options(maxiters=maxit, tol=1e-8, rel=1, precond=pc,
+                                                 check_every=256))
+        torch.cuda.synchronize()
+        r = {"seconds": round(time.perf_counter() - t, 3), "iters": st.iters,
+             "true_rel": st.true_rel, "converged": st.status == 0 and st.true_rel <= 1e-8}
+        log(f"[bench] {name}: {r}")
+        return r
+    jac64 = leg("Jacobi PCG FP64", S.PRECOND_JACOBI, args.cg_maxiters)
+    # plain CG does not reach 1e-8 on this system within minutes; it is run
+    # for a bounded number of iterations and reported as not converged
+    plain = None if args.no_plain else leg("plain CG FP64", S.PRECOND_NONE,
+                                           min(args.plain_maxiters, 20000))
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import pyoracle as O
+        rp, ci = idx.to_host()
+        t0 = time.perf_counter()
+        ov, of = O.hex_assemble(O.HexParams(prob["h"], prob["source"], 1.0), prob, rp, ci)
+        t_asm = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.cg(rp, ci, ov, of, np.zeros(nv), 1, O.cg_opts(maxiters=2, tol=1e-8, precond=1))
+        t2 = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        O.cg(rp, ci, ov, of, np.zeros(nv), 1, O.cg_opts(maxiters=12, tol=1e-8, precond=1))
+        per = (time.perf_counter() - t0 - t2) / 10
+        it = jac64["iters"]
+        cpu = {"value": round(t_asm + t2 + per * (it - 2), 2), "unit": "s", "cores": 1,
+               "kind": "port",
+               "sample": (f"oracle (single-thread FP64 C) on the full {n}^3 mesh: one assembly "
+                          f"({t_asm:.2f}s) and 12 Jacobi PCG iterations ({per * 1e3:.0f} ms/iter), "
+                          f"extrapolated to the {it} iterations FP64 Jacobi PCG needs"),
+               "nproc": os.cpu_count()}
+        log(f"[bench] cpu baseline {cpu}")
+    print(json.dumps({
+        "metric": METRIC, "value": round(step_s, 4), "unit": "s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 matrix/SpMV, fp64 dots+refinement",
+        "data": "synthetic (seeded Gaussian random field conductivity, generated on the host)",
+        "config": {"workload": f"trilinear hex Poisson {n}^3 cells, {nv} unknowns "
+                               f"({int(prob['dirichlet'].sum())} Dirichlet), mixed Jacobi-PCG "
+                               f"to a 1e-8 FP64 relative residual (no learned correction: "
+                               f"3x3 blocks only)", "nverts": nv, "nhex": nh, "nnz": nnz},
+        "iterations": stats[-1]["iters"], "true_rel": stats[-1]["true_rel"],
+        "converged": all(s["status"] == 0 and s["true_rel"] <= 1e-8 for s in stats),
+        "jacobi_pcg_f64": jac64, "plain_cg_f64": plain,
+        "speedup_vs_plain_cg_f64": (round(plain["seconds"] / step_s, 2)
+                                    if plain and plain["converged"] else None),
+        "index_build_ms": round(index_ms, 2), "assembly_ms": round(asm_ms, 3),
+        "assembly_GBps": round(asm_gbs, 1), "spmv_us": round(spmv_ms * 1e3, 1),
+        "spmv_GBps": round(spmv_gbs, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_bsrmv_tiles<SMH_F32,1> (FP32 scalar CSR SpMV)",
+                     "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": spmv_bytes},
+        "cpu_baseline": cpu}), flush=True)
 
 
 if __name__ == "__main__":

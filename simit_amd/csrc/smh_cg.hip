@@ -226,10 +226,10 @@ struct EpiPq {
   __device__ void operator()(uint32_t r, const VT *s) {
     VT sv[3];
     for (int a = 0; a < bd; ++a) sv[a] = s[a];
-    if (dlo && bd == 3) {
-      const float *D = dlo + (uint64_t)r * 9;
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b) sv[a] += (VT)D[3 * a + b] * p[(uint64_t)r * 3 + b];
+    if (dlo) {
+      const float *D = dlo + (uint64_t)r * bd * bd;
+      for (int a = 0; a < bd; ++a)
+        for (int b = 0; b < bd; ++b) sv[a] += (VT)D[bd * a + b] * p[(uint64_t)r * bd + b];
     }
     for (int a = 0; a < bd; ++a) {
       const uint64_t o = (uint64_t)r * bd + a;
@@ -240,14 +240,14 @@ struct EpiPq {
 };
 
 // dlo_i = sum over the blocks of row i of the low parts of an SMH_F32X2 matrix
-__global__ void k_lo_rowsum(uint32_t n, const uint32_t *__restrict__ rp,
+__global__ void k_lo_rowsum(uint32_t n, int bb, const uint32_t *__restrict__ rp,
                             const float *__restrict__ lo, float *__restrict__ dlo) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
-    for (int q = 0; q < 9; ++q) s[q] += (double)lo[(uint64_t)k * 9 + q];
-  for (int q = 0; q < 9; ++q) dlo[(uint64_t)i * 9 + q] = (float)s[q];
+    for (int q = 0; q < bb; ++q) s[q] += (double)lo[(uint64_t)k * bb + q];
+  for (int q = 0; q < bb; ++q) dlo[(uint64_t)i * bb + q] = (float)s[q];
 }
 
 template <int IP, int BD>
@@ -569,9 +569,17 @@ __global__ void __launch_bounds__(256) k_sum_to(const double *part, int np, doub
   if (threadIdx.x == 0) *out = t;
 }
 
+// the CG's per-workgroup partial sums: two per workgroup of a tile kernel
+constexpr int kMaxTileGrid = 16384;
+constexpr int kPartSlots = 2 * kMaxTileGrid;
+
 TileDev tile_dev(const smh_bsr *A, const smh_tile_plan *p, unsigned *grid) {
-  // persistent workgroups: up to 10 per CU on 256 CUs
-  uint64_t g = std::min<uint64_t>(p->ntiles, 2560);
+  // persistent workgroups (16 per CU on 256 CUs)
+#ifndef SMH_TILE_GRID
+#define SMH_TILE_GRID 4096
+#endif
+  static_assert(SMH_TILE_GRID <= kMaxTileGrid, "tile grid above the partial-sum slots");
+  uint64_t g = std::min<uint64_t>(p->ntiles, SMH_TILE_GRID);
   if (g >= 8) g &= ~7ull;
   if (g < 1) g = 1;
   *grid = (unsigned)g;
@@ -592,7 +600,6 @@ struct Work {
   }
 };
 
-constexpr int kPartSlots = 8192;
 
 template <int IP, int RP, int BD>
 int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh_tile_plan *rplan,
@@ -645,10 +652,10 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
     w.dinv = nullptr;
   }
   // mixed precision: the inner operator keeps the exact block row sums
-  if (o->mixed && BD == 3 && Am->precision == SMH_F32X2 && Am->vals_lo) {
-    TRY(dmalloc(ctx, &w.dlo, sizeof(float) * (uint64_t)nrows * 9));
+  if (o->mixed && Am->precision == SMH_F32X2 && Am->vals_lo) {
+    TRY(dmalloc(ctx, &w.dlo, sizeof(float) * (uint64_t)nrows * BD * BD));
     hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream, nrows,
-                       Am->rowptr, (const float *)Am->vals_lo, (float *)w.dlo);
+                       BD * BD, Am->rowptr, (const float *)Am->vals_lo, (float *)w.dlo);
   }
   pdata = learned ? (const void *)o->learned : (const void *)w.dinv;
   TRY(dmalloc(ctx, (void **)&w.S, sizeof(CGState)));
@@ -804,7 +811,8 @@ extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
   do {                                                                                       \
     using VT = typename Val<P>::T;                                                           \
     if (A->bdim == 3)                                                                        \
-      hipLaunchKernelGGL((k_bsrmv_tiles<P, 3>), dim3(g), dim3(192), 0, ctx->stream, T,     \
+      hipLaunchKernelGGL((k_bsrmv_tiles<P, 3>), dim3(g), dim3(tile_threads_host(3)), 0,   \
+                         ctx->stream, T,                                                    \
                          (const VT *)x, (VT *)y);                                            \
     else                                                                                     \
       hipLaunchKernelGGL((k_bsrmv_tiles<P, 1>), dim3(g), dim3(256), 0, ctx->stream, T,     \

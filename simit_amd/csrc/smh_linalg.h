@@ -28,17 +28,36 @@
 
 namespace smh {
 
-constexpr int kRounds = 4;  // blocks per lane per tile
+// Tile shapes (measured on MI355X at 96^3 x 5 tets: one wavefront of 21
+// 3x3 blocks per round and 16 rounds per tile beat three wavefronts x 4
+// rounds by 5-7%); scalar (1x1) tiles keep 4 rounds of 256 lanes.
+#ifndef SMH_TILE_ROUNDS3
+#define SMH_TILE_ROUNDS3 16
+#endif
+#ifndef SMH_TILE_ROUNDS1
+#define SMH_TILE_ROUNDS1 4
+#endif
+// matrix values are read once per product: streaming (non-temporal) loads
+#ifndef SMH_STREAM_LOAD
+#define SMH_STREAM_LOAD(p) __builtin_nontemporal_load(p)
+#endif
 
 template <int BD>
+#ifndef SMH_TILE_THREADS3
+#define SMH_TILE_THREADS3 64
+#endif
 struct TileShape {
-  static constexpr int kThreads = BD == 3 ? 192 : 256;  // 3 or 4 wavefronts
+  static constexpr int kThreads = BD == 3 ? SMH_TILE_THREADS3 : 256;
+  static constexpr int kRounds = BD == 3 ? SMH_TILE_ROUNDS3 : SMH_TILE_ROUNDS1;  // per lane
   static constexpr int kWaves = kThreads / 64;
   static constexpr int kPerRound = kThreads / BD;        // blocks per round
+  static constexpr int kActive = kPerRound * BD;         // lanes holding a block row
   static constexpr int kBlocks = kPerRound * kRounds;    // blocks per tile
 };
-inline uint32_t tile_blocks_host(int bd) { return bd == 3 ? 64 * kRounds : 256 * kRounds; }
-inline uint32_t tile_threads_host(int bd) { return bd == 3 ? 192 : 256; }
+inline uint32_t tile_blocks_host(int bd) {
+  return bd == 3 ? (SMH_TILE_THREADS3 / 3) * SMH_TILE_ROUNDS3 : 256 * SMH_TILE_ROUNDS1;
+}
+inline uint32_t tile_threads_host(int bd) { return bd == 3 ? SMH_TILE_THREADS3 : 256; }
 
 template <int PREC> struct Val;
 template <> struct Val<SMH_F64> {
@@ -171,6 +190,7 @@ __device__ __forceinline__ void wg_tiles(const TileDev &A, uint32_t &k0, uint32_
 template <int PREC, int BD>
 struct TileRegs {
   using S = typename Val<PREC>::S;
+  static constexpr int kRounds = TileShape<BD>::kRounds;
   S v[kRounds][BD];
   float vl[Val<PREC>::kLo ? kRounds : 1][BD];
   uint32_t col[kRounds];
@@ -182,20 +202,19 @@ struct TileRegs {
 #pragma unroll
     for (int u = 0; u < kRounds; ++u) {
       const uint32_t blk = base + s + u * SH::kPerRound;
-      const bool act = blk < end;
+      const bool act = s < SH::kPerRound && blk < end;
       col[u] = act ? A.colidx[blk] : 0u;
       if constexpr (Val<PREC>::kFP8) {
-        v[u][0] = act ? __builtin_nontemporal_load((const uint32_t *)A.vals + (uint64_t)blk * BD + bi)
-                      : 0u;
+        v[u][0] = act ? SMH_STREAM_LOAD((const uint32_t *)A.vals + (uint64_t)blk * BD + bi) : 0u;
         continue;
       }
       const uint64_t o = (uint64_t)blk * BB + bi * BD;
 #pragma unroll
       for (int b = 0; b < BD; ++b) {
-        v[u][b] = act ? __builtin_nontemporal_load((const S *)A.vals + o + b) : (S)0;
+        v[u][b] = act ? SMH_STREAM_LOAD((const S *)A.vals + o + b) : (S)0;
         if (Val<PREC>::kLo)
           vl[Val<PREC>::kLo ? u : 0][b] =
-              act ? __builtin_nontemporal_load((const float *)A.vals_lo + o + b) : 0.0f;
+              act ? SMH_STREAM_LOAD((const float *)A.vals_lo + o + b) : 0.0f;
       }
     }
   }
@@ -239,7 +258,7 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
       for (uint32_t i = t; i <= nr; i += SH::kThreads) sm.rp[i] = A.rowptr[R0 + i] - base;
     // partial row sums of this lane's blocks
 #pragma unroll
-    for (int u = 0; u < kRounds; ++u) {
+    for (int u = 0; u < SH::kRounds; ++u) {
       T acc = 0;
 #pragma unroll
       for (int b = 0; b < BD; ++b) {
@@ -251,7 +270,7 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
         if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][b];
         acc += val * xg(cur.col[u], b);
       }
-      sm.part[(s + u * SH::kPerRound) * BD + bi] = acc;
+      if (t < SH::kActive) sm.part[(s + u * SH::kPerRound) * BD + bi] = acc;
     }
     // the next tile's loads go out before this tile is reduced
     if (k + 1 < k1) nxt.load(A, nbase, nend, s, bi);
@@ -261,7 +280,17 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
         const uint32_t r = i / BD, a = i - (i / BD) * BD;
         const uint32_t q0 = sm.rp[r], q1 = sm.rp[r + 1];
         T acc = 0;
-        for (uint32_t q = q0; q < q1; ++q) acc += sm.part[q * BD + a];
+        uint32_t q = q0;
+        // four independent LDS reads in flight, added in ascending order
+        for (; q + 4 <= q1; q += 4) {
+          const T v0 = sm.part[q * BD + a], v1 = sm.part[(q + 1) * BD + a];
+          const T v2 = sm.part[(q + 2) * BD + a], v3 = sm.part[(q + 3) * BD + a];
+          acc += v0;
+          acc += v1;
+          acc += v2;
+          acc += v3;
+        }
+        for (; q < q1; ++q) acc += sm.part[q * BD + a];
         sm.rsum[i] = acc;
       }
       __syncthreads();
