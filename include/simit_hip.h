@@ -291,6 +291,24 @@ void smh_cg_default_opts(smh_cg_opts *o);
 int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
                  const smh_cg_opts *opts, smh_cg_stats *stats);
 
+/* ---- batched CG: many independent systems in one launch ------------------
+ * The systems are the diagonal blocks of one matrix: system m owns rows
+ * [rows[m], rows[m+1]) (host array of nbatch+1 entries from 0 to nrows) and
+ * no block couples two systems (checked).  Each system runs its own CG
+ * (own alpha, beta and stopping test on its own ||b_m||, as if solved alone
+ * by smh_cg_solve) in lockstep; opts->check_every iterations are captured
+ * once as a hipGraph and replayed.  The SpMV's workgroup count is
+ * autotuned on first use of a pattern.  stats: host array of nbatch
+ * entries (may be NULL); seconds (may be NULL): wall time of the solve. */
+typedef struct {
+  int32_t iters;
+  int32_t status;        /* SMH_OK / SMH_ERR_BREAKDOWN / SMH_ERR_NOT_CONVERGED */
+  double true_rel;       /* ||b_m - A_m x_m|| / ||b_m|| in FP64 at exit     */
+} smh_cg_batch_stats;
+int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch, const uint32_t *rows,
+                         const double *b, double *x, const smh_cg_opts *opts,
+                         smh_cg_batch_stats *stats, double *seconds);
+
 /* ---- learned block-Jacobi correction (no counterpart in the reference) ----
  * A message-passing network over the matrix graph (3 layers, width 32, BF16
  * weights and MFMA inputs, FP32 accumulation) predicts a symmetric correction

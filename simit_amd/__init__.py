@@ -102,6 +102,13 @@ class CGOptions(C.Structure):
                 ("learned_corr", C.c_void_p)]
 
 
+class CGBatchStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("status", C.c_int32), ("true_rel", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class CGStats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("reliable_updates", C.c_int32), ("status", C.c_int32),
                 ("rr", C.c_double), ("true_rel", C.c_double), ("seconds", C.c_double)]
@@ -157,6 +164,9 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_cg_default_opts": (None, [C.POINTER(CGOptions)]),
         "smh_cg_solve": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, C.POINTER(CGOptions),
                                    C.POINTER(CGStats)]),
+        "smh_cg_solve_batched": (C.c_int, [vp, C.POINTER(Bsr), u32, C.POINTER(u32), vp, vp,
+                                           C.POINTER(CGOptions), C.POINTER(CGBatchStats),
+                                           C.POINTER(C.c_double)]),
         "smh_gnn_create": (C.c_int, [vp, u64, C.c_double, C.POINTER(vp)]),
         "smh_gnn_destroy": (C.c_int, [vp, vp]),
         "smh_gnn_get_params": (C.c_int, [vp, vp, vp]),
@@ -289,6 +299,19 @@ class Context:
         self._check(self.lib.smh_cg_solve(self.h, C.byref(A), _ptr(b), _ptr(x), C.byref(opts),
                                           C.byref(st)))
         return st
+
+    def cg_batched(self, A: Bsr, rows, b, x, opts: CGOptions):
+        """Solve the independent diagonal-block systems rows[m]..rows[m+1];
+        returns (per-system stats dicts, seconds)."""
+        import numpy as np
+        r = np.ascontiguousarray(rows, dtype=np.uint32)
+        nb = r.size - 1
+        st = (CGBatchStats * max(nb, 1))()
+        sec = C.c_double()
+        self._check(self.lib.smh_cg_solve_batched(
+            self.h, C.byref(A), nb, r.ctypes.data_as(C.POINTER(C.c_uint32)), _ptr(b), _ptr(x),
+            C.byref(opts), st, C.byref(sec)))
+        return [st[m].as_dict() for m in range(nb)], sec.value
 
     def cg_options(self, **kw) -> CGOptions:
         o = CGOptions()

@@ -17,9 +17,10 @@
 // Mixed precision: the inner iteration streams the FP32 matrix (hi part)
 // with FP32 vectors and FP64 dots; x is accumulated in FP64.  Whenever the
 // iterated residual fell by reliable_delta since the last update, or claims
-// convergence, a reliable update recomputes r = b - A x in FP64 from
-// hi + lo (~2^-48 relative to the FP64 matrix).  Convergence is always
-// certified by that FP64 residual.
+// convergence, the iteration pauses right after its x/r update and a
+// reliable update recomputes r = b - A x in FP64 from hi + lo (~2^-48
+// relative to the FP64 matrix); beta and p then continue from the replaced
+// residual.  Convergence is always certified by that FP64 residual.
 #include "smh_linalg.h"
 
 #include <algorithm>
@@ -170,7 +171,7 @@ k_bsrmv_tiles(TileDev A, const typename Val<PREC>::T *__restrict__ x,
 // ---- CG device state ------------------------------------------------------------
 struct CGState {
   double rz, pq, alpha, beta, rr, rr_true, thr, rr_ref, bb, delta2;
-  int iter, done, status, want_ru, claimed, maxiters, precond_stop, mixed, restart;
+  int iter, done, status, want_ru, claimed, maxiters, precond_stop, mixed, restart, replace_every;
 };
 
 __device__ __forceinline__ bool finite_pos(double v) { return v > 0.0 && v < INFINITY; }
@@ -379,19 +380,25 @@ __global__ void __launch_bounds__(256) k_cg_r2(CGState *S, const double *part, i
   const double rr = wg_sum(part + 1, np, 2, s4);
   if (threadIdx.x == 0) {
     S->iter += 1;
-    S->beta = rz / S->rz;
-    S->rz = rz;
     S->rr = rr;
     if (!(rr < INFINITY) || !(rz == rz)) {
       S->status = SMH_ERR_BREAKDOWN;
       S->done = 1;
     } else if (stop_test(S, rr, rz)) {
+      S->rz = rz;
       S->claimed = 1;  // verified against the FP64 residual by the host
       S->done = 1;
     } else if (S->iter >= S->maxiters) {
       S->done = 1;
-    } else if (S->mixed && rr < S->delta2 * S->rr_ref) {
+    } else if ((S->mixed && rr < S->delta2 * S->rr_ref) ||
+               (S->replace_every > 0 && S->iter % S->replace_every == 0)) {
+      // pause before beta and p: the host replaces r (and z) by the true
+      // residual, then k_cg_rres forms beta from the replaced z'r
       S->want_ru = 1;
+      S->done = 1;
+    } else {
+      S->beta = rz / S->rz;
+      S->rz = rz;
     }
     if (rr > S->rr_ref) S->rr_ref = rr;
   }
@@ -466,12 +473,20 @@ __global__ void __launch_bounds__(256) k_cg_rres(CGState *S, const double *part,
   if (threadIdx.x) return;
   S->rr_true = rr;
   if (certify_only) return;
-  // Keeping the search direction is only sound while the replaced residual
-  // is close to the iterated one.  After a false convergence claim, or when
-  // the true residual is more than 10x the iterated one, restart from p = z:
-  // an old direction scaled for a far smaller residual would make the next
-  // alpha = z'r / p'Ap huge and throw x off.
-  S->restart = (S->claimed || !(rr <= 100.0 * S->rr)) ? 1 : 0;
+  // The replacement sits between the x/r update and beta (as in the
+  // reference loop's periodic r = b - A x), so the search direction
+  // continues as p = z + beta p with beta = (z'r)_true / (z'r)_old.  After a
+  // false convergence claim, or when the true residual is more than 10x the
+  // iterated one, restart from p = z instead: the old direction belongs to a
+  // different residual.
+  if (S->claimed || !(rr <= 100.0 * S->rr)) {
+    S->restart = 1;
+  } else if (S->want_ru) {
+    S->restart = 2;
+    S->beta = rz / S->rz;
+  } else {
+    S->restart = 0;
+  }
   S->rz = rz;
   S->rr = rr;
   S->rr_ref = rr;
@@ -487,14 +502,16 @@ __global__ void __launch_bounds__(256) k_cg_rres(CGState *S, const double *part,
   }
 }
 
-// p = z after a residual replacement that asked for a restart
+// after a residual replacement: p = z (restart 1) or p = z + beta p (restart 2)
 template <typename VT>
 __global__ void __launch_bounds__(256)
 k_cg_restart(uint64_t n, VT *__restrict__ p, const VT *__restrict__ z, const CGState *S) {
-  if (!S->restart) return;
+  const int mode = S->restart;
+  if (!mode) return;
+  const double beta = S->beta;
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
        i += (uint64_t)gridDim.x * 256)
-    p[i] = z[i];
+    p[i] = mode == 1 ? z[i] : (VT)((double)z[i] + beta * (double)p[i]);
 }
 
 // Jacobi / block-Jacobi inverse of the diagonal (thermal.sim:229-233 and
@@ -538,7 +555,8 @@ __global__ void k_dinv(uint32_t nrows, const uint32_t *__restrict__ rowptr,
 }
 
 __global__ void k_cg_init_state(CGState *S, double tol, int rel, int norm_form, int maxiters,
-                                int precond_stop, double delta, int mixed, const double *bbp) {
+                                int precond_stop, double delta, int mixed, int replace_every,
+                                const double *bbp) {
   if (threadIdx.x || blockIdx.x) return;
   const double bb = *bbp;
   memset(S, 0, sizeof(CGState));
@@ -550,6 +568,7 @@ __global__ void k_cg_init_state(CGState *S, double tol, int rel, int norm_form, 
   S->precond_stop = precond_stop;
   S->delta2 = delta * delta;
   S->mixed = mixed;
+  S->replace_every = replace_every;
 }
 
 __global__ void __launch_bounds__(256) k_dot_wg(uint64_t n, const double *__restrict__ x,
@@ -626,7 +645,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   int rc = SMH_OK;
   auto T0 = std::chrono::steady_clock::now();
   CGState hs{};
-  int nru = 0, last_rep = 0;
+  int nru = 0;
   EpiResidual<VT, BD> epi{};
 #define TRY(x)            \
   do {                    \
@@ -666,7 +685,8 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
                      w.part + 2 * kPartSlots);
   hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(1), 0, ctx->stream, w.S, o->tol, o->rel,
                      o->norm_form, o->maxiters, (pc != SMH_PRECOND_NONE && o->zr_stop) ? 1 : 0,
-                     o->reliable_delta, o->mixed, (const double *)(w.part + 2 * kPartSlots));
+                     o->reliable_delta, o->mixed, o->replace_every,
+                     (const double *)(w.part + 2 * kPartSlots));
   if (pc != SMH_PRECOND_NONE && !learned)
     hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream,
                        nrows, Am->rowptr, Am->colidx, Am->vals, Am->vals_lo, BD, pc,
@@ -698,11 +718,8 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   for (;;) {
     SMH_HIP(ctx, hipMemcpyAsync(&hs, w.S, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
     SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    const bool replace = o->replace_every > 0 && hs.iter > 0 && hs.iter != last_rep &&
-                         hs.iter % o->replace_every == 0;
-    if (hs.status == SMH_OK && (hs.want_ru || hs.claimed || replace)) {
+    if (hs.status == SMH_OK && (hs.want_ru || hs.claimed)) {
       ++nru;
-      last_rep = hs.iter;
       hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
                          (const double *)x, epi, w.part);
       if (learned)
@@ -717,10 +734,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
       continue;
     }
     if (hs.done || hs.status != SMH_OK) break;
-    const int batch = o->replace_every > 0
-                          ? std::min(check, o->replace_every - hs.iter % o->replace_every)
-                          : check;
-    for (int k = 0; k < batch; ++k) {
+    for (int k = 0; k < check; ++k) {
       hipLaunchKernelGGL((k_cg_spmv<IP, BD>), dim3(gt), dim3(tile_threads_host(BD)), 0, ctx->stream, T,
                          (const VT *)w.p, (VT *)w.q, (const CGState *)w.S,
                          (const float *)w.dlo, w.part);
@@ -766,6 +780,590 @@ done:
 #undef LCHECK
   hipStreamSynchronize(ctx->stream);
   w.release();
+  return rc;
+}
+
+// ---- batched CG: many independent systems in one block-diagonal matrix ---------
+// System m owns rows [rows[m], rows[m+1]).  Every system runs its own CG
+// recurrence (own alpha, beta and stopping test) in lockstep: the SpMV and the
+// vector updates stream the whole union once per iteration, and the scalar
+// reductions are segmented by system (one workgroup per system, fixed order).
+// One iteration is seven launches, all on the device and captured once into a
+// hipGraph that the host replays check_every times between state reads:
+//   K1  q = A_in p, per-row p'q          R1  alpha per system
+//   K2  x, r, z update, per-row z'r, r'r R2  stopping test, beta or a request
+//   RU  r = b - A x in FP64 for the tiles of systems that claimed convergence
+//       or asked for a reliable update (other workgroups exit at once)
+//   RR  certify the claims, beta from the replaced residual
+//   K3  p = z + beta p, or p = z after a restart
+// The replacement thus sits between the x/r update and beta, as in the
+// reference loop's periodic r = b - A x.
+enum BState : int {
+  kBActive = 0,
+  kBClaimed = 1,   // r'r <= thr in the iteration: certified by RR
+  kBDoneOk = 2,
+  kBDoneMaxit = 3,
+  kBBreakdown = 4,
+  kBWant = 5       // asked for a residual replacement: served by RU/RR
+};
+
+__device__ __forceinline__ bool bflagged(int st) { return st == kBClaimed || st == kBWant; }
+
+struct BDev {
+  double *rz, *alpha, *beta, *rr, *rr_true, *thr, *bb, *rr_ref;
+  int *state, *iter, *restart;
+};
+
+__global__ void k_bcg_mesh_rows(uint32_t nb, const uint32_t *__restrict__ rows,
+                                uint32_t *__restrict__ mesh) {
+  const uint32_t m = blockIdx.x;
+  if (m >= nb) return;
+  for (uint32_t i = rows[m] + threadIdx.x; i < rows[m + 1]; i += blockDim.x) mesh[i] = m;
+}
+
+// every block of a row of system m must stay inside system m
+__global__ void k_bcg_check(uint32_t n, const uint32_t *__restrict__ rowptr,
+                            const uint32_t *__restrict__ colidx, const uint32_t *__restrict__ mesh,
+                            const uint32_t *__restrict__ rows, int *__restrict__ err) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t m = mesh[i], lo = rows[m], hi = rows[m + 1];
+  for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+    if (colidx[k] < lo || colidx[k] >= hi) {
+      atomicOr(err, 1);
+      return;
+    }
+}
+
+// K1: q = A_in p; prow = p'q per row
+template <typename VT>
+struct EpiPqRow {
+  VT *q;
+  const VT *p;
+  const float *dlo;
+  double *prow;
+  int bd;
+  __device__ void operator()(uint32_t r, const VT *s) {
+    VT sv[3];
+    for (int a = 0; a < bd; ++a) sv[a] = s[a];
+    if (dlo) {
+      const float *D = dlo + (uint64_t)r * bd * bd;
+      for (int a = 0; a < bd; ++a)
+        for (int b = 0; b < bd; ++b) sv[a] += (VT)D[bd * a + b] * p[(uint64_t)r * bd + b];
+    }
+    double d = 0.0;
+    for (int a = 0; a < bd; ++a) {
+      const uint64_t o = (uint64_t)r * bd + a;
+      q[o] = sv[a];
+      d += (double)sv[a] * (double)p[o];
+    }
+    prow[r] = d;
+  }
+};
+
+template <int IP, int BD>
+__global__ void __launch_bounds__(TileShape<BD>::kThreads)
+k_bcg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP>::T *__restrict__ q,
+           const float *dlo, double *prow) {
+  using VT = typename Val<IP>::T;
+  __shared__ TileSmem<IP, BD> sm;
+  uint32_t k0, k1;
+  wg_tiles(A, k0, k1);
+  XPlain<VT> xg{p, BD};
+  EpiPqRow<VT> epi{q, p, dlo, prow, BD};
+  spmv_tiles<IP, BD>(A, k0, k1, sm, xg, epi);
+}
+
+// segmented sums of one or two per-row vectors over system m (workgroup m)
+__device__ __forceinline__ void seg_sums(const uint32_t *rows, const double *v1,
+                                         const double *v2, double *s4, double &a, double &b) {
+  const uint32_t m = blockIdx.x;
+  double x = 0.0, y = 0.0;
+  for (uint32_t i = rows[m] + threadIdx.x; i < rows[m + 1]; i += 256) {
+    x += v1[i];
+    if (v2) y += v2[i];
+  }
+  a = block_sum256(x, s4);
+  b = v2 ? block_sum256(y, s4) : 0.0;
+}
+
+// R1: alpha per system
+__global__ void __launch_bounds__(256) k_bcg_alpha(const uint32_t *rows, const double *prow,
+                                                   BDev S) {
+  __shared__ double s4[4];
+  if (S.state[blockIdx.x] != kBActive) return;  // uniform over the workgroup
+  double pq, unused;
+  seg_sums(rows, prow, nullptr, s4, pq, unused);
+  if (threadIdx.x) return;
+  const uint32_t m = blockIdx.x;
+  if (finite_pos(pq)) {
+    S.alpha[m] = S.rz[m] / pq;
+  } else {
+    S.alpha[m] = 0.0;
+    S.state[m] = kBBreakdown;
+  }
+}
+
+// K2: x += alpha p, r -= alpha q, z = M r for rows of active systems
+template <typename VT, int BD>
+__global__ void __launch_bounds__(256)
+k_bcg_update(uint32_t nrows, const uint32_t *__restrict__ mesh, int pc, const void *dinv,
+             double *__restrict__ x, VT *__restrict__ r, VT *__restrict__ z,
+             const VT *__restrict__ p, const VT *__restrict__ q, BDev S, double *prz,
+             double *prr) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nrows;
+       i += (uint64_t)gridDim.x * 256) {
+    const uint32_t m = mesh[i];
+    double drz = 0.0, drr = 0.0;
+    if (S.state[m] == kBActive) {
+      const double alpha = S.alpha[m];
+      double rv[BD], zv[BD];
+      for (int a = 0; a < BD; ++a) {
+        const uint64_t o = i * BD + a;
+        x[o] = x[o] + alpha * (double)p[o];
+        const VT rn = (VT)((double)r[o] - alpha * (double)q[o]);
+        r[o] = rn;
+        rv[a] = (double)rn;
+      }
+      apply_precond<BD>(pc, rv, zv, dinv, sizeof(VT) == 4, i);
+      for (int a = 0; a < BD; ++a) {
+        const VT zz = (VT)zv[a];
+        z[i * BD + a] = zz;
+        drz += (double)zz * rv[a];
+        drr += rv[a] * rv[a];
+      }
+    }
+    prz[i] = drz;
+    prr[i] = drr;
+  }
+}
+
+// R2: the stopping test, then beta or a reliable-update request per system
+__global__ void __launch_bounds__(256) k_bcg_beta(const uint32_t *rows, const double *prz,
+                                                  const double *prr, BDev S, int maxiters,
+                                                  int mixed, double delta2, int replace_every) {
+  __shared__ double s4[4];
+  const uint32_t m = blockIdx.x;
+  if (S.state[m] != kBActive) return;  // uniform over the workgroup
+  double rz, rr;
+  seg_sums(rows, prz, prr, s4, rz, rr);
+  if (threadIdx.x) return;
+  const int it = S.iter[m] + 1;
+  S.iter[m] = it;
+  S.rr[m] = rr;
+  S.restart[m] = 0;
+  if (!(rr < INFINITY) || !(rz == rz)) {
+    S.state[m] = kBBreakdown;
+  } else if (!(rr > S.thr[m])) {
+    S.state[m] = kBClaimed;
+  } else if (it >= maxiters) {
+    S.state[m] = kBDoneMaxit;
+  } else if ((mixed && rr < delta2 * S.rr_ref[m]) ||
+             (replace_every > 0 && it % replace_every == 0)) {
+    S.state[m] = kBWant;  // S.rz keeps the previous z'r for RR's beta
+  } else {
+    S.beta[m] = rz / S.rz[m];
+    S.rz[m] = rz;
+  }
+  if (rr > S.rr_ref[m]) S.rr_ref[m] = rr;
+}
+
+// K3: p = z + beta p (restart 0) or p = z (restart 1) for active systems
+template <typename VT, int BD>
+__global__ void __launch_bounds__(256)
+k_bcg_pupd(uint32_t nrows, const uint32_t *__restrict__ mesh, VT *__restrict__ p,
+           const VT *__restrict__ z, BDev S) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nrows;
+       i += (uint64_t)gridDim.x * 256) {
+    const uint32_t m = mesh[i];
+    if (S.state[m] != kBActive) continue;
+    if (S.restart[m]) {
+      for (int a = 0; a < BD; ++a) p[i * BD + a] = z[i * BD + a];
+    } else {
+      const double beta = S.beta[m];
+      for (int a = 0; a < BD; ++a) {
+        const uint64_t o = i * BD + a;
+        p[o] = (VT)((double)z[o] + beta * (double)p[o]);
+      }
+    }
+  }
+}
+
+// RU epilogue: r = b - A x (FP64, A in precision RP), z = M r for the rows of
+// the systems being replaced (all rows on the host-driven passes); per-row
+// z'r and r'r of every row the workgroup computed
+template <typename VT, int BD>
+struct EpiResRow {
+  const double *b;
+  VT *r, *z;
+  const void *dinv;
+  int pc, all;
+  const uint32_t *mesh;
+  const int *state;
+  double *prz, *prr;
+  __device__ void operator()(uint32_t row, const double *s) {
+    double rv[BD], zv[BD], drr = 0.0, drz = 0.0;
+    for (int a = 0; a < BD; ++a) {
+      const double v = b[(uint64_t)row * BD + a] - s[a];
+      drr += v * v;
+      rv[a] = (double)(VT)v;
+    }
+    apply_precond<BD>(pc, rv, zv, dinv, sizeof(VT) == 4, row);
+    const bool mine = all || bflagged(state[mesh[row]]);
+    for (int a = 0; a < BD; ++a) {
+      const uint64_t o = (uint64_t)row * BD + a;
+      const VT zz = (VT)zv[a];
+      if (mine) {
+        r[o] = (VT)rv[a];
+        z[o] = zz;
+      }
+      drz += (double)zz * rv[a];
+    }
+    prz[row] = drz;
+    prr[row] = drr;
+  }
+};
+
+template <int RP, int BD, typename VT>
+__global__ void __launch_bounds__(TileShape<BD>::kThreads)
+k_bcg_residual(TileDev A, const double *__restrict__ x, EpiResRow<VT, BD> epi) {
+  __shared__ TileSmem<RP, BD> sm;
+  uint32_t k0, k1;
+  wg_tiles(A, k0, k1);
+  if (!epi.all) {
+    // skip the workgroup unless one of its rows belongs to a system being
+    // replaced (the same answer on every thread)
+    if (k0 >= k1) return;
+    const uint32_t R0 = A.tile_row[k0], R1 = A.tile_row[k1];
+    if (R0 >= R1) return;
+    bool any = false;
+    for (uint32_t m = epi.mesh[R0]; m <= epi.mesh[R1 - 1] && !any; ++m)
+      any = bflagged(epi.state[m]);
+    if (!any) return;
+  }
+  XPlain<double> xg{x, BD};
+  spmv_tiles<RP, BD>(A, k0, k1, sm, xg, epi);
+}
+
+// RR: after RU.  mode 0 (in the iteration): certify the claims, and beta
+// from the replaced residual for the requests; mode 1: start of the solve;
+// mode 2: the final certification of every system.
+__global__ void __launch_bounds__(256) k_bcg_rres(const uint32_t *rows, const double *prz,
+                                                  const double *prr, BDev S, int mode) {
+  __shared__ double s4[4];
+  const uint32_t m = blockIdx.x;
+  const int st = S.state[m];
+  if (mode == 0 && !bflagged(st)) return;  // uniform over the workgroup
+  double rz, rr;
+  seg_sums(rows, prz, prr, s4, rz, rr);
+  if (threadIdx.x) return;
+  S.rr_true[m] = rr;
+  if (mode == 2) return;
+  if (mode == 1) {
+    S.restart[m] = 1;
+  } else if (st == kBClaimed) {
+    if (rr <= S.thr[m] * (1.0 + 1e-12)) {
+      S.state[m] = kBDoneOk;
+      return;
+    }
+    S.restart[m] = 1;  // a false claim: restart from the true residual
+  } else if (!(rr <= 100.0 * S.rr[m])) {
+    S.restart[m] = 1;  // the old direction belongs to a far smaller residual
+  } else {
+    S.restart[m] = 0;
+    S.beta[m] = rz / S.rz[m];
+  }
+  S.state[m] = kBActive;
+  if (!(rr < INFINITY) || !(rz == rz)) {
+    S.state[m] = kBBreakdown;
+  } else if (!(rr > S.thr[m])) {
+    S.state[m] = kBDoneOk;
+  }
+  S.rz[m] = rz;
+  S.rr[m] = rr;
+  S.rr_ref[m] = rr;
+}
+
+// per-system b'b, thresholds and initial state
+__global__ void __launch_bounds__(256) k_bcg_init(const uint32_t *rows, const double *prr,
+                                                  BDev S, double tol, int rel) {
+  __shared__ double s4[4];
+  double bb, unused;
+  seg_sums(rows, prr, nullptr, s4, bb, unused);
+  if (threadIdx.x) return;
+  const uint32_t m = blockIdx.x;
+  S.bb[m] = bb;
+  S.thr[m] = rel ? tol * tol * bb : tol;
+  S.state[m] = kBActive;
+  S.iter[m] = 0;
+  S.restart[m] = 1;
+  S.rz[m] = S.rr[m] = S.rr_ref[m] = S.rr_true[m] = 0.0;
+  S.alpha[m] = S.beta[m] = 0.0;
+}
+
+template <int BD>
+__global__ void k_bcg_sq(uint32_t nrows, const double *__restrict__ b, double *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  double s = 0.0;
+  for (int a = 0; a < BD; ++a) s += b[(uint64_t)i * BD + a] * b[(uint64_t)i * BD + a];
+  out[i] = s;
+}
+
+// the SpMV grid of a pattern: the fastest of a few candidates, timed once
+unsigned tuned_grid(smh_ctx *ctx, hipStream_t stream, const smh_bsr *A, const smh_tile_plan *plan,
+                    void *xbuf, void *ybuf) {
+  for (const auto &t : ctx->tuned)
+    if (t.rowptr == A->rowptr && t.nrows == A->nrows && t.nnz == A->nnz && t.bdim == A->bdim)
+      return t.grid;
+  smh_bsr Af = *A;
+  if (Af.precision == SMH_F32X2) Af.precision = SMH_F32;
+  const unsigned cands[] = {1024, 2048, 4096};
+  unsigned best = 0;
+  float best_ms = 1e30f;
+  hipEvent_t e0, e1;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 0;
+  for (unsigned g : cands) {
+    unsigned gg = 0;
+    TileDev T = tile_dev(&Af, plan, &gg);
+    gg = std::min<unsigned>(g, plan->ntiles);
+    if (gg >= 8) gg &= ~7u;
+    if (gg < 1) gg = 1;
+    T.tiles_per_wg = (uint32_t)((plan->ntiles + gg - 1) / gg);
+    float ms = 0.0f;
+    for (int rep = 0; rep < 4; ++rep) {
+      (void)hipEventRecord(e0, stream);
+      if (Af.precision == SMH_F64) {
+        if (A->bdim == 3)
+          hipLaunchKernelGGL((k_bsrmv_tiles<SMH_F64, 3>), dim3(gg), dim3(tile_threads_host(3)), 0,
+                             stream, T, (const double *)xbuf, (double *)ybuf);
+        else
+          hipLaunchKernelGGL((k_bsrmv_tiles<SMH_F64, 1>), dim3(gg), dim3(256), 0, stream, T,
+                             (const double *)xbuf, (double *)ybuf);
+      } else {
+        if (A->bdim == 3)
+          hipLaunchKernelGGL((k_bsrmv_tiles<SMH_F32, 3>), dim3(gg), dim3(tile_threads_host(3)), 0,
+                             stream, T, (const float *)xbuf, (float *)ybuf);
+        else
+          hipLaunchKernelGGL((k_bsrmv_tiles<SMH_F32, 1>), dim3(gg), dim3(256), 0, stream, T,
+                             (const float *)xbuf, (float *)ybuf);
+      }
+      (void)hipEventRecord(e1, stream);
+      (void)hipEventSynchronize(e1);
+      float t = 0.0f;
+      (void)hipEventElapsedTime(&t, e0, e1);
+      if (rep) ms += t;  // the first launch warms up
+    }
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = gg;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipGetLastError();
+  ctx->tuned.push_back({A->rowptr, A->nrows, A->bdim, A->nnz, best});
+  return best;
+}
+
+template <int IP, int RP, int BD>
+int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t nb,
+            const uint32_t *hrows, const double *b, double *x, const smh_cg_opts *o,
+            smh_cg_batch_stats *st, double *seconds) {
+  using VT = typename Val<IP>::T;
+  const uint32_t nrows = Am->nrows;
+  const uint64_t n = (uint64_t)nrows * BD;
+  const int pc = o->precond;
+  const int check = o->check_every > 0 ? o->check_every : 16;
+  auto T0 = std::chrono::steady_clock::now();
+  std::vector<void *> bufs;
+  int rc = SMH_OK;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  auto alloc = [&](void **pp, size_t bytes) {
+    const int r = dmalloc(ctx, pp, bytes);
+    if (!r) bufs.push_back(*pp);
+    return r;
+  };
+  void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *dlo = nullptr;
+  uint32_t *rows = nullptr, *mesh = nullptr;
+  double *prz = nullptr, *prr = nullptr, *sbuf = nullptr;
+  int *ibuf = nullptr, *err = nullptr;
+  BDev S{};
+  std::vector<int> hstate(nb), hiter(nb);
+  std::vector<double> hrr(nb), hbb(nb);
+  if ((rc = alloc(&r, sizeof(VT) * n)) || (rc = alloc(&z, sizeof(VT) * n)) ||
+      (rc = alloc(&p, sizeof(VT) * n)) || (rc = alloc(&q, sizeof(VT) * n)) ||
+      (rc = alloc((void **)&rows, sizeof(uint32_t) * (nb + 1))) ||
+      (rc = alloc((void **)&mesh, sizeof(uint32_t) * (nrows ? nrows : 1))) ||
+      (rc = alloc((void **)&prz, sizeof(double) * (nrows ? nrows : 1))) ||
+      (rc = alloc((void **)&prr, sizeof(double) * (nrows ? nrows : 1))) ||
+      (rc = alloc((void **)&sbuf, sizeof(double) * 8 * nb)) ||
+      (rc = alloc((void **)&ibuf, sizeof(int) * 3 * nb)) || (rc = alloc((void **)&err, 64)))
+    goto out;
+  if (pc != SMH_PRECOND_NONE && (rc = alloc(&dinv, sizeof(VT) * (uint64_t)nrows * BD * BD)))
+    goto out;
+  if (o->mixed && Am->precision == SMH_F32X2 && Am->vals_lo &&
+      (rc = alloc(&dlo, sizeof(float) * (uint64_t)nrows * BD * BD)))
+    goto out;
+  S = BDev{sbuf, sbuf + nb, sbuf + 2 * nb, sbuf + 3 * nb, sbuf + 4 * nb, sbuf + 5 * nb,
+           sbuf + 6 * nb, sbuf + 7 * nb, ibuf, ibuf + nb, ibuf + 2 * nb};
+  {
+    // the iteration is captured as a graph, which the legacy default stream
+    // does not allow: run on the context's own stream, ordered after the
+    // caller's work (this solver synchronises with the host anyway)
+    hipStream_t s = ctx->own_stream;
+    (void)hipStreamSynchronize(ctx->stream);
+    rc = SMH_OK;
+    if (hipMemcpyAsync(rows, hrows, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, s) !=
+            hipSuccess ||
+        hipMemsetAsync(err, 0, sizeof(int), s) != hipSuccess) {
+      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: upload failed");
+      goto out;
+    }
+    hipLaunchKernelGGL(k_bcg_mesh_rows, dim3(nb), dim3(256), 0, s, nb, (const uint32_t *)rows,
+                       mesh);
+    hipLaunchKernelGGL(k_bcg_check, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows,
+                       Am->rowptr, Am->colidx, (const uint32_t *)mesh, (const uint32_t *)rows,
+                       err);
+    int herr = 0;
+    if (hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: check failed");
+      goto out;
+    }
+    if (herr) {
+      rc = fail(ctx, SMH_ERR_INVALID,
+                "smh_cg_solve_batched: a block couples two systems (the matrix is not "
+                "block-diagonal by the given row ranges)");
+      goto out;
+    }
+    // the autotuned SpMV grid, then the kernels' shapes
+    unsigned g = tuned_grid(ctx, s, Am, plan, p, q);
+    unsigned g0 = 0;
+    TileDev T = tile_dev(Am, plan, &g0);
+    if (g) {
+      T.tiles_per_wg = (uint32_t)((plan->ntiles + g - 1) / g);
+    } else {
+      g = g0;
+    }
+    const unsigned nbe = grid_for(nrows, 256, 2048);
+    if (pc != SMH_PRECOND_NONE)
+      hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows,
+                         Am->rowptr, Am->colidx, Am->vals, Am->vals_lo, BD, pc, (VT *)dinv);
+    if (dlo)
+      hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows,
+                         BD * BD, Am->rowptr, (const float *)Am->vals_lo, (float *)dlo);
+    hipLaunchKernelGGL(k_bcg_sq<BD>, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows, b, prr);
+    hipLaunchKernelGGL(k_bcg_init, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
+                       (const double *)prr, S, o->tol, o->rel);
+    EpiResRow<VT, BD> epi{b, (VT *)r, (VT *)z, dinv, pc, 0, (const uint32_t *)mesh, S.state,
+                          prz, prr};
+    EpiResRow<VT, BD> epi_all = epi;
+    epi_all.all = 1;
+    auto pupd = [&]() {
+      hipLaunchKernelGGL((k_bcg_pupd<VT, BD>), dim3(nbe), dim3(256), 0, s, nrows,
+                         (const uint32_t *)mesh, (VT *)p, (const VT *)z, S);
+    };
+    // r = b - A x, z = M r, p = z for every system
+    hipLaunchKernelGGL((k_bcg_residual<RP, BD, VT>), dim3(g), dim3(tile_threads_host(BD)), 0, s,
+                       T, (const double *)x, epi_all);
+    hipLaunchKernelGGL(k_bcg_rres, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
+                       (const double *)prz, (const double *)prr, S, 1);
+    pupd();
+    // one captured chunk of `check` iterations
+    auto iterate = [&]() {
+      for (int k = 0; k < check; ++k) {
+        hipLaunchKernelGGL((k_bcg_spmv<IP, BD>), dim3(g), dim3(tile_threads_host(BD)), 0, s, T,
+                           (const VT *)p, (VT *)q, (const float *)dlo, prz);
+        hipLaunchKernelGGL(k_bcg_alpha, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
+                           (const double *)prz, S);
+        hipLaunchKernelGGL((k_bcg_update<VT, BD>), dim3(nbe), dim3(256), 0, s, nrows,
+                           (const uint32_t *)mesh, pc, (const void *)dinv, x, (VT *)r, (VT *)z,
+                           (const VT *)p, (const VT *)q, S, prz, prr);
+        hipLaunchKernelGGL(k_bcg_beta, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
+                           (const double *)prz, (const double *)prr, S, o->maxiters, o->mixed,
+                           o->reliable_delta * o->reliable_delta, o->replace_every);
+        hipLaunchKernelGGL((k_bcg_residual<RP, BD, VT>), dim3(g), dim3(tile_threads_host(BD)),
+                           0, s, T, (const double *)x, epi);
+        hipLaunchKernelGGL(k_bcg_rres, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
+                           (const double *)prz, (const double *)prr, S, 0);
+        pupd();
+      }
+    };
+    if (hipGetLastError() != hipSuccess) {
+      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: launch failed");
+      goto out;
+    }
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: stream capture failed");
+      goto out;
+    }
+    iterate();
+    if (hipStreamEndCapture(s, &graph) != hipSuccess ||
+        hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGetLastError();
+      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: graph instantiation failed");
+      goto out;
+    }
+    // replay until no system is active (every system leaves the active state
+    // by maxiters at the latest)
+    for (;;) {
+      if (hipMemcpyAsync(hstate.data(), S.state, sizeof(int) * nb, hipMemcpyDeviceToHost, s) !=
+              hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess) {
+        rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: state readback failed");
+        goto out;
+      }
+      bool active = false;
+      for (uint32_t m = 0; m < nb; ++m) active |= hstate[m] == kBActive;
+      if (!active) break;
+      if (hipGraphLaunch(gexec, s) != hipSuccess) {
+        rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: graph launch failed");
+        goto out;
+      }
+    }
+    // certify every system against its FP64 residual
+    hipLaunchKernelGGL((k_bcg_residual<RP, BD, VT>), dim3(g), dim3(tile_threads_host(BD)), 0, s,
+                       T, (const double *)x, epi_all);
+    hipLaunchKernelGGL(k_bcg_rres, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
+                       (const double *)prz, (const double *)prr, S, 2);
+    if (hipMemcpyAsync(hstate.data(), S.state, sizeof(int) * nb, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(hiter.data(), S.iter, sizeof(int) * nb, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(hrr.data(), S.rr_true, sizeof(double) * nb, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipMemcpyAsync(hbb.data(), S.bb, sizeof(double) * nb, hipMemcpyDeviceToHost, s) !=
+            hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: result readback failed");
+      goto out;
+    }
+    std::vector<double> hthr(nb);
+    if (hipMemcpy(hthr.data(), S.thr, sizeof(double) * nb, hipMemcpyDeviceToHost) !=
+        hipSuccess) {
+      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: result readback failed");
+      goto out;
+    }
+    if (st)
+      for (uint32_t m = 0; m < nb; ++m) {
+        st[m].iters = hiter[m];
+        st[m].true_rel = hbb[m] > 0 ? std::sqrt(hrr[m] / hbb[m]) : std::sqrt(hrr[m]);
+        int status = SMH_OK;
+        if (hstate[m] == kBBreakdown) status = SMH_ERR_BREAKDOWN;
+        else if (!(hrr[m] <= hthr[m] * (1.0 + 1e-12))) status = SMH_ERR_NOT_CONVERGED;
+        st[m].status = status;
+      }
+  }
+out:
+  if (gexec) (void)hipGraphExecDestroy(gexec);
+  if (graph) (void)hipGraphDestroy(graph);
+  (void)hipStreamSynchronize(ctx->own_stream);
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void *b : bufs) dfree(b);
+  if (seconds)
+    *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - T0).count();
   return rc;
 }
 
@@ -903,5 +1501,48 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
   else if (A->precision == SMH_F32X2) rc = SMH_CG(SMH_F32X2, SMH_F32X2);
   else rc = SMH_CG(kF32D, kF32D);
 #undef SMH_CG
+  return rc;
+}
+
+extern "C" int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch,
+                                    const uint32_t *rows, const double *b, double *x,
+                                    const smh_cg_opts *o, smh_cg_batch_stats *stats,
+                                    double *seconds) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, A && rows && b && x && o && A->rowptr && nbatch > 0, SMH_ERR_INVALID,
+            "smh_cg_solve_batched: null argument");
+  SMH_CHECK(ctx, A->bdim == 1 || A->bdim == 3, SMH_ERR_UNSUPPORTED,
+            "smh_cg_solve_batched: block size %u (1 or 3 supported)", A->bdim);
+  SMH_CHECK(ctx, o->precond >= 0 && o->precond <= SMH_PRECOND_BLOCK_JACOBI, SMH_ERR_INVALID,
+            "smh_cg_solve_batched: preconditioner %d (none, Jacobi or block Jacobi)", o->precond);
+  SMH_CHECK(ctx, A->bdim == 3 || o->precond <= SMH_PRECOND_JACOBI, SMH_ERR_INVALID,
+            "smh_cg_solve_batched: block Jacobi needs 3x3 blocks");
+  SMH_CHECK(ctx, !o->mixed || A->precision == SMH_F32X2, SMH_ERR_INVALID,
+            "smh_cg_solve_batched: mixed precision needs an SMH_F32X2 matrix");
+  SMH_CHECK(ctx, o->mixed == 0 || o->mixed == 1, SMH_ERR_INVALID,
+            "smh_cg_solve_batched: mixed must be 0 or 1");
+  SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
+            "smh_cg_solve_batched: SMH_F32X2 needs vals_lo");
+  SMH_CHECK(ctx, rows[0] == 0 && rows[nbatch] == A->nrows, SMH_ERR_INVALID,
+            "smh_cg_solve_batched: rows must run from 0 to nrows");
+  for (uint32_t m = 0; m < nbatch; ++m)
+    SMH_CHECK(ctx, rows[m] <= rows[m + 1], SMH_ERR_INVALID,
+              "smh_cg_solve_batched: rows must not decrease (system %u)", m);
+  if (A->nrows == 0) return SMH_OK;
+  const int bd = (int)A->bdim;
+  smh_tile_plan plan;
+  int rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(bd), &plan);
+  if (rc) return rc;
+  SMH_CHECK(ctx, plan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
+            "smh_cg_solve_batched: a row holds %u blocks (at most %u supported)", plan.maxrow,
+            tile_blocks_host(bd) / 2);
+#define SMH_BCG(IP, RP)                                                                      \
+  (bd == 3 ? bcg_run<IP, RP, 3>(ctx, A, &plan, nbatch, rows, b, x, o, stats, seconds)        \
+           : bcg_run<IP, RP, 1>(ctx, A, &plan, nbatch, rows, b, x, o, stats, seconds))
+  if (o->mixed) rc = SMH_BCG(SMH_F32, SMH_F32X2);
+  else if (A->precision == SMH_F64) rc = SMH_BCG(SMH_F64, SMH_F64);
+  else if (A->precision == SMH_F32X2) rc = SMH_BCG(SMH_F32X2, SMH_F32X2);
+  else rc = SMH_BCG(kF32D, kF32D);
+#undef SMH_BCG
   return rc;
 }

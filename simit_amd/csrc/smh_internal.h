@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/simit_hip.h"
@@ -31,6 +32,13 @@ struct smh_ctx {
   hipStream_t stream = nullptr;
   std::string last_error;
   std::vector<smh_tile_plan> plans;
+  struct Tuned {
+    const uint32_t *rowptr;
+    uint32_t nrows, bdim;
+    uint64_t nnz;
+    unsigned grid;
+  };
+  std::vector<Tuned> tuned;  // autotuned SpMV grids by pattern
   // small device scratch for reductions: [0..kRedSlots) partials + counters
   double *red = nullptr;
   unsigned *red_count = nullptr;

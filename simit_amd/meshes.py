@@ -49,6 +49,24 @@ def springs_lattice(n: int, seed: int = 0, jitter: float = 0.1, k: float = 1e3,
     }
 
 
+def springs_batch(nmesh: int = 256, nmin: int = 8, nmax: int = 36, seed: int = 0,
+                  klo: float = 1e2, khi: float = 1e4, jitter: float = 0.1):
+    """nmesh independent jittered n^3 spring lattices (n uniform in nmin..nmax,
+    stiffness log-uniform in [klo, khi] per lattice, z=0 layer fixed) stored
+    as one disjoint union; rows[m]..rows[m+1] are lattice m's points."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, size=nmesh)
+    ks = 10.0 ** rng.uniform(np.log10(klo), np.log10(khi), size=nmesh)
+    parts = [springs_lattice(int(n), seed=seed * 100003 + m, jitter=jitter, k=float(k))
+             for m, (n, k) in enumerate(zip(ns, ks))]
+    rows = np.zeros(nmesh + 1, dtype=np.int64)
+    rows[1:] = np.cumsum([p["npoints"] for p in parts])
+    ep = np.concatenate([p["endpoints"] + rows[m] for m, p in enumerate(parts)])
+    cat = {k: np.concatenate([p[k] for p in parts]) for k in ("x", "v", "m", "fixed", "k", "l0")}
+    return {"npoints": int(rows[-1]), "rows": rows.astype(np.uint32), "n": ns, "kmesh": ks,
+            "endpoints": ep.astype(np.int32), **cat}
+
+
 # ---- tet and hex grids ------------------------------------------------------------
 _TET5 = {
     0: [[0, 3, 5, 6], [0, 3, 5, 1], [3, 0, 6, 2], [5, 6, 0, 4], [6, 5, 3, 7]],

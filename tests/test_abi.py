@@ -37,6 +37,7 @@ def test_struct_layouts_match_header():
     # sizes a C compiler gives these structs on x86-64
     assert ctypes.sizeof(simit_amd.Bsr) == 56
     assert ctypes.sizeof(simit_amd.CGOptions) == 80
+    assert ctypes.sizeof(simit_amd.CGBatchStats) == 16
     assert ctypes.sizeof(simit_amd.CGStats) == 40
     assert ctypes.sizeof(simit_amd.SpringsParams) == 56
 

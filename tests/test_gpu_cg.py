@@ -101,6 +101,50 @@ def test_cg_repeatable_bitwise(ctx):
     assert torch.equal(outs[0][0], outs[1][0]) and outs[0][1] == outs[1][1]
 
 
+def test_replace_every_follows_oracle(ctx, oracle):
+    """opts.replace_every = k: r = b - A x every k iterations, between the x/r
+    update and beta, as the oracle's loop does (thermal.sim's periodic
+    replacement); same iteration count and solution as the oracle."""
+    prob, idx, v64, hi, lo, rhs, _ = tet_system(ctx, 8, seed=2)
+    rp, ci = idx.to_host()
+    n = prob["nverts"] * 3
+    ov, of = oracle.tet_assemble(oracle.tet_params(), prob, rp, ci)
+    for k in (0, 7, 1):
+        x = torch.zeros(n, dtype=torch.float64, device="cuda")
+        st = ctx.cg(idx.bsr(3, S.F64, v64), rhs, x,
+                    ctx.cg_options(maxiters=5000, tol=1e-8, rel=1, replace_every=k))
+        xo, so = oracle.cg(rp, ci, ov, of, np.zeros(n), 3,
+                           oracle.cg_opts(maxiters=5000, tol=1e-8, replace_every=k))
+        print(k, st.as_dict(), so, flush=True)
+        assert st.status == 0 and st.true_rel <= 1e-8
+        # FP64 in both, summed in different orders: over ~370 iterations of an
+        # ill-conditioned system the counts drift by a few (plain CG too)
+        assert abs(st.iters - so["iters"]) <= max(2, 0.02 * so["iters"]), (k, st.iters, so)
+        if k:
+            assert st.reliable_updates >= st.iters // k
+        assert np.linalg.norm(x.cpu().numpy() - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_mixed_fast_convergence(ctx, oracle):
+    """A mass-dominated spring lattice (condition number ~1.06): the FP32
+    iterated residual falls 1e4 per iteration, far below the FP32 floor of
+    the true residual within one check_every chunk; the replacement before
+    beta keeps the iteration converging to a certified 1e-8."""
+    pts = meshes.springs_lattice(8, seed=11, k=123.0)
+    d = {k: to_dev(v) for k, v in pts.items() if k != "npoints"}
+    idx = ctx.build_index(pts["npoints"], [d["endpoints"]])
+    hi = torch.empty(idx.nnz * 9, dtype=torch.float32, device="cuda")
+    lo = torch.empty_like(hi)
+    rhs = torch.empty(pts["npoints"] * 3, dtype=torch.float64, device="cuda")
+    P = S.SpringsParams(0.01, 0.0, (C.c_double * 3)(0.0, 0.0, -9.8), 1.0, 1)
+    ctx.assemble_springs(idx, P, d, d, S.F32X2, hi, lo, rhs)
+    for check in (1, 4, 16):
+        x = torch.zeros_like(rhs)
+        st = ctx.cg(idx.bsr(3, S.F32X2, hi, lo), rhs, x,
+                    ctx.cg_options(maxiters=200, tol=1e-8, rel=1, mixed=1, check_every=check))
+        assert st.status == 0 and st.true_rel <= 1e-8 and st.iters <= 12, (check, st.as_dict())
+
+
 def test_hex_poisson_mixed(ctx, oracle):
     prob = meshes.hex_problem(12, seed=2, corr_len=3.0)
     d = {"dirichlet": to_dev(prob["dirichlet"]), "kappa": to_dev(prob["kappa"]),
