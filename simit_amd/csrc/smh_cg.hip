@@ -240,15 +240,18 @@ struct EpiPq {
   }
 };
 
-// dlo_i = sum over the blocks of row i of the low parts of an SMH_F32X2 matrix
+// dlo_i = sum over the blocks of row i of the low parts of an SMH_F32X2
+// matrix; one thread per (row, block entry), so a wave reads whole blocks of
+// consecutive rows (ascending block order, FP64 accumulation)
 __global__ void k_lo_rowsum(uint32_t n, int bb, const uint32_t *__restrict__ rp,
                             const float *__restrict__ lo, float *__restrict__ dlo) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
-    for (int q = 0; q < bb; ++q) s[q] += (double)lo[(uint64_t)k * bb + q];
-  for (int q = 0; q < bb; ++q) dlo[(uint64_t)i * bb + q] = (float)s[q];
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)n * bb) return;
+  const uint32_t i = (uint32_t)(t / bb);
+  const int q = (int)(t - (uint64_t)i * bb);
+  double s = 0.0;
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) s += (double)lo[(uint64_t)k * bb + q];
+  dlo[t] = (float)s;
 }
 
 template <int IP, int BD>
@@ -673,7 +676,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   // mixed precision: the inner operator keeps the exact block row sums
   if (o->mixed && Am->precision == SMH_F32X2 && Am->vals_lo) {
     TRY(dmalloc(ctx, &w.dlo, sizeof(float) * (uint64_t)nrows * BD * BD));
-    hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream, nrows,
+    hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for((uint64_t)nrows * BD * BD, 256)), dim3(256), 0, ctx->stream, nrows,
                        BD * BD, Am->rowptr, (const float *)Am->vals_lo, (float *)w.dlo);
   }
   pdata = learned ? (const void *)o->learned : (const void *)w.dinv;
@@ -787,15 +790,19 @@ done:
 // System m owns rows [rows[m], rows[m+1]).  Every system runs its own CG
 // recurrence (own alpha, beta and stopping test) in lockstep: the SpMV and the
 // vector updates stream the whole union once per iteration, and the scalar
-// reductions are segmented by system (one workgroup per system, fixed order).
-// One iteration is seven launches, all on the device and captured once into a
+// reductions are segmented by system in two fixed-order levels: each system's
+// rows are cut into chunks of kBChunk rows (a chunk never spans two systems),
+// one workgroup sums a chunk, one thread per system sums its chunks.  One
+// iteration is nine launches, all on the device and captured once into a
 // hipGraph that the host replays check_every times between state reads:
-//   K1  q = A_in p, per-row p'q          R1  alpha per system
-//   K2  x, r, z update, per-row z'r, r'r R2  stopping test, beta or a request
+//   K1  q = A_in p, per-row p'q (row tiles)   S1  chunk sums of p'q
+//   R1  alpha per system
+//   K2  x, r, z update (a workgroup per chunk) with chunk sums of z'r, r'r
+//   R2  stopping test, then beta or a reliable-update request
 //   RU  r = b - A x in FP64 for the tiles of systems that claimed convergence
 //       or asked for a reliable update (other workgroups exit at once)
-//   RR  certify the claims, beta from the replaced residual
-//   K3  p = z + beta p, or p = z after a restart
+//   S2  chunk sums of the replaced z'r, r'r     RR  certify claims, beta
+//   K3  p = z + beta p, or p = z after a restart (a workgroup per chunk)
 // The replacement thus sits between the x/r update and beta, as in the
 // reference loop's periodic r = b - A x.
 enum BState : int {
@@ -874,28 +881,49 @@ k_bcg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP
   spmv_tiles<IP, BD>(A, k0, k1, sm, xg, epi);
 }
 
-// segmented sums of one or two per-row vectors over system m (workgroup m)
-__device__ __forceinline__ void seg_sums(const uint32_t *rows, const double *v1,
-                                         const double *v2, double *s4, double &a, double &b) {
-  const uint32_t m = blockIdx.x;
-  double x = 0.0, y = 0.0;
-  for (uint32_t i = rows[m] + threadIdx.x; i < rows[m + 1]; i += 256) {
-    x += v1[i];
-    if (v2) y += v2[i];
+// Chunks: crow[c]..crow[c+1] are the rows of chunk c, all in system cmesh[c];
+// system m owns chunks mchunk[m]..mchunk[m+1].
+constexpr uint32_t kBChunk = 2048;
+
+struct BChunks {
+  const uint32_t *crow, *cmesh, *mchunk;
+  uint32_t nchunks, nb;
+};
+
+// S1/S2: per-chunk sums of one or two per-row vectors (flagged_only: only the
+// chunks of systems being replaced)
+__global__ void __launch_bounds__(256) k_bcg_csum(BChunks C, const double *__restrict__ v1,
+                                                  const double *__restrict__ v2,
+                                                  const int *__restrict__ state,
+                                                  int flagged_only, double *cp1, double *cp2) {
+  __shared__ double s4[4];
+  const uint32_t c = blockIdx.x;
+  if (flagged_only && !bflagged(state[C.cmesh[c]])) return;  // uniform
+  double a = 0.0, b = 0.0;
+  for (uint32_t i = C.crow[c] + threadIdx.x; i < C.crow[c + 1]; i += 256) {
+    a += v1[i];
+    if (v2) b += v2[i];
   }
-  a = block_sum256(x, s4);
-  b = v2 ? block_sum256(y, s4) : 0.0;
+  a = block_sum256(a, s4);
+  if (v2) b = block_sum256(b, s4);
+  if (threadIdx.x == 0) {
+    cp1[c] = a;
+    if (v2) cp2[c] = b;
+  }
 }
 
-// R1: alpha per system
-__global__ void __launch_bounds__(256) k_bcg_alpha(const uint32_t *rows, const double *prow,
-                                                   BDev S) {
-  __shared__ double s4[4];
-  if (S.state[blockIdx.x] != kBActive) return;  // uniform over the workgroup
-  double pq, unused;
-  seg_sums(rows, prow, nullptr, s4, pq, unused);
-  if (threadIdx.x) return;
-  const uint32_t m = blockIdx.x;
+// a system's total over its chunks, in chunk order (one thread)
+__device__ __forceinline__ double sys_sum(const BChunks &C, uint32_t m, const double *cp) {
+  double t = 0.0;
+  for (uint32_t c = C.mchunk[m]; c < C.mchunk[m + 1]; ++c) t += cp[c];
+  return t;
+}
+
+// R1: alpha per system (a thread per system)
+__global__ void __launch_bounds__(256) k_bcg_alpha(BChunks C, const double *cp, BDev S) {
+  const uint32_t m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= C.nb || S.state[m] != kBActive) return;
+  const double pq = sys_sum(C, m, cp);
   if (finite_pos(pq)) {
     S.alpha[m] = S.rz[m] / pq;
   } else {
@@ -904,50 +932,51 @@ __global__ void __launch_bounds__(256) k_bcg_alpha(const uint32_t *rows, const d
   }
 }
 
-// K2: x += alpha p, r -= alpha q, z = M r for rows of active systems
+// K2: x += alpha p, r -= alpha q, z = M r; a workgroup per chunk (one system),
+// with the chunk's z'r and r'r
 template <typename VT, int BD>
 __global__ void __launch_bounds__(256)
-k_bcg_update(uint32_t nrows, const uint32_t *__restrict__ mesh, int pc, const void *dinv,
-             double *__restrict__ x, VT *__restrict__ r, VT *__restrict__ z,
-             const VT *__restrict__ p, const VT *__restrict__ q, BDev S, double *prz,
-             double *prr) {
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nrows;
-       i += (uint64_t)gridDim.x * 256) {
-    const uint32_t m = mesh[i];
-    double drz = 0.0, drr = 0.0;
-    if (S.state[m] == kBActive) {
-      const double alpha = S.alpha[m];
-      double rv[BD], zv[BD];
-      for (int a = 0; a < BD; ++a) {
-        const uint64_t o = i * BD + a;
-        x[o] = x[o] + alpha * (double)p[o];
-        const VT rn = (VT)((double)r[o] - alpha * (double)q[o]);
-        r[o] = rn;
-        rv[a] = (double)rn;
-      }
-      apply_precond<BD>(pc, rv, zv, dinv, sizeof(VT) == 4, i);
-      for (int a = 0; a < BD; ++a) {
-        const VT zz = (VT)zv[a];
-        z[i * BD + a] = zz;
-        drz += (double)zz * rv[a];
-        drr += rv[a] * rv[a];
-      }
+k_bcg_update(BChunks C, int pc, const void *dinv, double *__restrict__ x, VT *__restrict__ r,
+             VT *__restrict__ z, const VT *__restrict__ p, const VT *__restrict__ q, BDev S,
+             double *cpz, double *cpr) {
+  __shared__ double s4[4];
+  const uint32_t c = blockIdx.x, m = C.cmesh[c];
+  if (S.state[m] != kBActive) return;  // uniform; R2 skips this system
+  const double alpha = S.alpha[m];
+  double drz = 0.0, drr = 0.0;
+  for (uint64_t i = C.crow[c] + threadIdx.x; i < C.crow[c + 1]; i += 256) {
+    double rv[BD], zv[BD];
+    for (int a = 0; a < BD; ++a) {
+      const uint64_t o = i * BD + a;
+      x[o] = x[o] + alpha * (double)p[o];
+      const VT rn = (VT)((double)r[o] - alpha * (double)q[o]);
+      r[o] = rn;
+      rv[a] = (double)rn;
     }
-    prz[i] = drz;
-    prr[i] = drr;
+    apply_precond<BD>(pc, rv, zv, dinv, sizeof(VT) == 4, i);
+    for (int a = 0; a < BD; ++a) {
+      const VT zz = (VT)zv[a];
+      z[i * BD + a] = zz;
+      drz += (double)zz * rv[a];
+      drr += rv[a] * rv[a];
+    }
+  }
+  drz = block_sum256(drz, s4);
+  drr = block_sum256(drr, s4);
+  if (threadIdx.x == 0) {
+    cpz[c] = drz;
+    cpr[c] = drr;
   }
 }
 
-// R2: the stopping test, then beta or a reliable-update request per system
-__global__ void __launch_bounds__(256) k_bcg_beta(const uint32_t *rows, const double *prz,
-                                                  const double *prr, BDev S, int maxiters,
+// R2: the stopping test, then beta or a reliable-update request (a thread per
+// system)
+__global__ void __launch_bounds__(256) k_bcg_beta(BChunks C, const double *cpz,
+                                                  const double *cpr, BDev S, int maxiters,
                                                   int mixed, double delta2, int replace_every) {
-  __shared__ double s4[4];
-  const uint32_t m = blockIdx.x;
-  if (S.state[m] != kBActive) return;  // uniform over the workgroup
-  double rz, rr;
-  seg_sums(rows, prz, prr, s4, rz, rr);
-  if (threadIdx.x) return;
+  const uint32_t m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= C.nb || S.state[m] != kBActive) return;
+  const double rz = sys_sum(C, m, cpz), rr = sys_sum(C, m, cpr);
   const int it = S.iter[m] + 1;
   S.iter[m] = it;
   S.rr[m] = rr;
@@ -968,25 +997,17 @@ __global__ void __launch_bounds__(256) k_bcg_beta(const uint32_t *rows, const do
   if (rr > S.rr_ref[m]) S.rr_ref[m] = rr;
 }
 
-// K3: p = z + beta p (restart 0) or p = z (restart 1) for active systems
+// K3: p = z + beta p (restart 0) or p = z (restart 1); a workgroup per chunk
 template <typename VT, int BD>
 __global__ void __launch_bounds__(256)
-k_bcg_pupd(uint32_t nrows, const uint32_t *__restrict__ mesh, VT *__restrict__ p,
-           const VT *__restrict__ z, BDev S) {
-  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nrows;
-       i += (uint64_t)gridDim.x * 256) {
-    const uint32_t m = mesh[i];
-    if (S.state[m] != kBActive) continue;
-    if (S.restart[m]) {
-      for (int a = 0; a < BD; ++a) p[i * BD + a] = z[i * BD + a];
-    } else {
-      const double beta = S.beta[m];
-      for (int a = 0; a < BD; ++a) {
-        const uint64_t o = i * BD + a;
-        p[o] = (VT)((double)z[o] + beta * (double)p[o]);
-      }
-    }
-  }
+k_bcg_pupd(BChunks C, VT *__restrict__ p, const VT *__restrict__ z, BDev S) {
+  const uint32_t c = blockIdx.x, m = C.cmesh[c];
+  if (S.state[m] != kBActive) return;  // uniform
+  const bool restart = S.restart[m] != 0;
+  const double beta = S.beta[m];
+  for (uint64_t o = (uint64_t)C.crow[c] * BD + threadIdx.x; o < (uint64_t)C.crow[c + 1] * BD;
+       o += 256)
+    p[o] = restart ? z[o] : (VT)((double)z[o] + beta * (double)p[o]);
 }
 
 // RU epilogue: r = b - A x (FP64, A in precision RP), z = M r for the rows of
@@ -1045,18 +1066,16 @@ k_bcg_residual(TileDev A, const double *__restrict__ x, EpiResRow<VT, BD> epi) {
   spmv_tiles<RP, BD>(A, k0, k1, sm, xg, epi);
 }
 
-// RR: after RU.  mode 0 (in the iteration): certify the claims, and beta
-// from the replaced residual for the requests; mode 1: start of the solve;
-// mode 2: the final certification of every system.
-__global__ void __launch_bounds__(256) k_bcg_rres(const uint32_t *rows, const double *prz,
-                                                  const double *prr, BDev S, int mode) {
-  __shared__ double s4[4];
-  const uint32_t m = blockIdx.x;
+// RR: after RU and S2.  mode 0 (in the iteration): certify the claims, and
+// beta from the replaced residual for the requests; mode 1: start of the
+// solve; mode 2: the final certification of every system.  A thread per system.
+__global__ void __launch_bounds__(256) k_bcg_rres(BChunks C, const double *cpz,
+                                                  const double *cpr, BDev S, int mode) {
+  const uint32_t m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= C.nb) return;
   const int st = S.state[m];
-  if (mode == 0 && !bflagged(st)) return;  // uniform over the workgroup
-  double rz, rr;
-  seg_sums(rows, prz, prr, s4, rz, rr);
-  if (threadIdx.x) return;
+  if (mode == 0 && !bflagged(st)) return;
+  const double rz = sys_sum(C, m, cpz), rr = sys_sum(C, m, cpr);
   S.rr_true[m] = rr;
   if (mode == 2) return;
   if (mode == 1) {
@@ -1084,14 +1103,26 @@ __global__ void __launch_bounds__(256) k_bcg_rres(const uint32_t *rows, const do
   S.rr_ref[m] = rr;
 }
 
-// per-system b'b, thresholds and initial state
-__global__ void __launch_bounds__(256) k_bcg_init(const uint32_t *rows, const double *prr,
-                                                  BDev S, double tol, int rel) {
+// per-chunk b'b
+template <int BD>
+__global__ void __launch_bounds__(256) k_bcg_bb(BChunks C, const double *__restrict__ b,
+                                                double *cp) {
   __shared__ double s4[4];
-  double bb, unused;
-  seg_sums(rows, prr, nullptr, s4, bb, unused);
-  if (threadIdx.x) return;
-  const uint32_t m = blockIdx.x;
+  const uint32_t c = blockIdx.x;
+  double t = 0.0;
+  for (uint64_t o = (uint64_t)C.crow[c] * BD + threadIdx.x; o < (uint64_t)C.crow[c + 1] * BD;
+       o += 256)
+    t += b[o] * b[o];
+  t = block_sum256(t, s4);
+  if (threadIdx.x == 0) cp[c] = t;
+}
+
+// per-system b'b, thresholds and initial state
+__global__ void __launch_bounds__(256) k_bcg_init(BChunks C, const double *cp, BDev S,
+                                                  double tol, int rel) {
+  const uint32_t m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= C.nb) return;
+  const double bb = sys_sum(C, m, cp);
   S.bb[m] = bb;
   S.thr[m] = rel ? tol * tol * bb : tol;
   S.state[m] = kBActive;
@@ -1099,15 +1130,6 @@ __global__ void __launch_bounds__(256) k_bcg_init(const uint32_t *rows, const do
   S.restart[m] = 1;
   S.rz[m] = S.rr[m] = S.rr_ref[m] = S.rr_true[m] = 0.0;
   S.alpha[m] = S.beta[m] = 0.0;
-}
-
-template <int BD>
-__global__ void k_bcg_sq(uint32_t nrows, const double *__restrict__ b, double *__restrict__ out) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nrows) return;
-  double s = 0.0;
-  for (int a = 0; a < BD; ++a) s += b[(uint64_t)i * BD + a] * b[(uint64_t)i * BD + a];
-  out[i] = s;
 }
 
 // the SpMV grid of a pattern: the fastest of a few candidates, timed once
@@ -1186,12 +1208,28 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
     return r;
   };
   void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *dlo = nullptr;
-  uint32_t *rows = nullptr, *mesh = nullptr;
-  double *prz = nullptr, *prr = nullptr, *sbuf = nullptr;
+  uint32_t *rows = nullptr, *mesh = nullptr, *ctab = nullptr;
+  double *prz = nullptr, *prr = nullptr, *sbuf = nullptr, *cp1 = nullptr, *cp2 = nullptr;
   int *ibuf = nullptr, *err = nullptr;
   BDev S{};
   std::vector<int> hstate(nb), hiter(nb);
   std::vector<double> hrr(nb), hbb(nb);
+  // chunk tables: [crow (nchunks+1) | cmesh (nchunks) | mchunk (nb+1)]
+  std::vector<uint32_t> hcrow, hcmesh, hmchunk(nb + 1);
+  for (uint32_t m = 0; m < nb; ++m) {
+    hmchunk[m] = (uint32_t)hcrow.size();
+    for (uint32_t i = hrows[m]; i < hrows[m + 1]; i += std::min(kBChunk, hrows[m + 1] - i)) {
+      hcrow.push_back(i);
+      hcmesh.push_back(m);
+    }
+  }
+  const uint32_t nch = (uint32_t)hcrow.size();
+  hmchunk[nb] = nch;
+  hcrow.push_back(nrows);
+  std::vector<uint32_t> htab(hcrow);
+  htab.insert(htab.end(), hcmesh.begin(), hcmesh.end());
+  htab.insert(htab.end(), hmchunk.begin(), hmchunk.end());
+  BChunks CH{};
   if ((rc = alloc(&r, sizeof(VT) * n)) || (rc = alloc(&z, sizeof(VT) * n)) ||
       (rc = alloc(&p, sizeof(VT) * n)) || (rc = alloc(&q, sizeof(VT) * n)) ||
       (rc = alloc((void **)&rows, sizeof(uint32_t) * (nb + 1))) ||
@@ -1199,8 +1237,12 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
       (rc = alloc((void **)&prz, sizeof(double) * (nrows ? nrows : 1))) ||
       (rc = alloc((void **)&prr, sizeof(double) * (nrows ? nrows : 1))) ||
       (rc = alloc((void **)&sbuf, sizeof(double) * 8 * nb)) ||
-      (rc = alloc((void **)&ibuf, sizeof(int) * 3 * nb)) || (rc = alloc((void **)&err, 64)))
+      (rc = alloc((void **)&ibuf, sizeof(int) * 3 * nb)) || (rc = alloc((void **)&err, 64)) ||
+      (rc = alloc((void **)&ctab, sizeof(uint32_t) * htab.size())) ||
+      (rc = alloc((void **)&cp1, sizeof(double) * (nch ? nch : 1))) ||
+      (rc = alloc((void **)&cp2, sizeof(double) * (nch ? nch : 1))))
     goto out;
+  CH = BChunks{ctab, ctab + nch + 1, ctab + 2 * nch + 1, nch, nb};
   if (pc != SMH_PRECOND_NONE && (rc = alloc(&dinv, sizeof(VT) * (uint64_t)nrows * BD * BD)))
     goto out;
   if (o->mixed && Am->precision == SMH_F32X2 && Am->vals_lo &&
@@ -1217,6 +1259,8 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
     rc = SMH_OK;
     if (hipMemcpyAsync(rows, hrows, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, s) !=
             hipSuccess ||
+        hipMemcpyAsync(ctab, htab.data(), sizeof(uint32_t) * htab.size(), hipMemcpyHostToDevice,
+                       s) != hipSuccess ||
         hipMemsetAsync(err, 0, sizeof(int), s) != hipSuccess) {
       rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: upload failed");
       goto out;
@@ -1247,47 +1291,52 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
     } else {
       g = g0;
     }
-    const unsigned nbe = grid_for(nrows, 256, 2048);
+    const unsigned gs = grid_for(nb, 256);  // a thread per system
     if (pc != SMH_PRECOND_NONE)
       hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows,
                          Am->rowptr, Am->colidx, Am->vals, Am->vals_lo, BD, pc, (VT *)dinv);
     if (dlo)
-      hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows,
-                         BD * BD, Am->rowptr, (const float *)Am->vals_lo, (float *)dlo);
-    hipLaunchKernelGGL(k_bcg_sq<BD>, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows, b, prr);
-    hipLaunchKernelGGL(k_bcg_init, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
-                       (const double *)prr, S, o->tol, o->rel);
+      hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for((uint64_t)nrows * BD * BD, 256)), dim3(256),
+                         0, s, nrows, BD * BD, Am->rowptr, (const float *)Am->vals_lo,
+                         (float *)dlo);
+    hipLaunchKernelGGL(k_bcg_bb<BD>, dim3(nch), dim3(256), 0, s, CH, b, cp1);
+    hipLaunchKernelGGL(k_bcg_init, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1, S,
+                       o->tol, o->rel);
     EpiResRow<VT, BD> epi{b, (VT *)r, (VT *)z, dinv, pc, 0, (const uint32_t *)mesh, S.state,
                           prz, prr};
     EpiResRow<VT, BD> epi_all = epi;
     epi_all.all = 1;
+    // RU + S2 + RR (all: every system, else only those being replaced)
+    auto replace = [&](int all, int mode) {
+      hipLaunchKernelGGL((k_bcg_residual<RP, BD, VT>), dim3(g), dim3(tile_threads_host(BD)), 0,
+                         s, T, (const double *)x, all ? epi_all : epi);
+      hipLaunchKernelGGL(k_bcg_csum, dim3(nch), dim3(256), 0, s, CH, (const double *)prz,
+                         (const double *)prr, (const int *)S.state, all ? 0 : 1, cp1, cp2);
+      hipLaunchKernelGGL(k_bcg_rres, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1,
+                         (const double *)cp2, S, mode);
+    };
     auto pupd = [&]() {
-      hipLaunchKernelGGL((k_bcg_pupd<VT, BD>), dim3(nbe), dim3(256), 0, s, nrows,
-                         (const uint32_t *)mesh, (VT *)p, (const VT *)z, S);
+      hipLaunchKernelGGL((k_bcg_pupd<VT, BD>), dim3(nch), dim3(256), 0, s, CH, (VT *)p,
+                         (const VT *)z, S);
     };
     // r = b - A x, z = M r, p = z for every system
-    hipLaunchKernelGGL((k_bcg_residual<RP, BD, VT>), dim3(g), dim3(tile_threads_host(BD)), 0, s,
-                       T, (const double *)x, epi_all);
-    hipLaunchKernelGGL(k_bcg_rres, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
-                       (const double *)prz, (const double *)prr, S, 1);
+    replace(1, 1);
     pupd();
     // one captured chunk of `check` iterations
     auto iterate = [&]() {
       for (int k = 0; k < check; ++k) {
         hipLaunchKernelGGL((k_bcg_spmv<IP, BD>), dim3(g), dim3(tile_threads_host(BD)), 0, s, T,
                            (const VT *)p, (VT *)q, (const float *)dlo, prz);
-        hipLaunchKernelGGL(k_bcg_alpha, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
-                           (const double *)prz, S);
-        hipLaunchKernelGGL((k_bcg_update<VT, BD>), dim3(nbe), dim3(256), 0, s, nrows,
-                           (const uint32_t *)mesh, pc, (const void *)dinv, x, (VT *)r, (VT *)z,
-                           (const VT *)p, (const VT *)q, S, prz, prr);
-        hipLaunchKernelGGL(k_bcg_beta, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
-                           (const double *)prz, (const double *)prr, S, o->maxiters, o->mixed,
+        hipLaunchKernelGGL(k_bcg_csum, dim3(nch), dim3(256), 0, s, CH, (const double *)prz,
+                           (const double *)nullptr, (const int *)S.state, 0, cp1, cp2);
+        hipLaunchKernelGGL(k_bcg_alpha, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1, S);
+        hipLaunchKernelGGL((k_bcg_update<VT, BD>), dim3(nch), dim3(256), 0, s, CH, pc,
+                           (const void *)dinv, x, (VT *)r, (VT *)z, (const VT *)p, (const VT *)q,
+                           S, cp1, cp2);
+        hipLaunchKernelGGL(k_bcg_beta, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1,
+                           (const double *)cp2, S, o->maxiters, o->mixed,
                            o->reliable_delta * o->reliable_delta, o->replace_every);
-        hipLaunchKernelGGL((k_bcg_residual<RP, BD, VT>), dim3(g), dim3(tile_threads_host(BD)),
-                           0, s, T, (const double *)x, epi);
-        hipLaunchKernelGGL(k_bcg_rres, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
-                           (const double *)prz, (const double *)prr, S, 0);
+        replace(0, 0);
         pupd();
       }
     };
@@ -1324,10 +1373,7 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
       }
     }
     // certify every system against its FP64 residual
-    hipLaunchKernelGGL((k_bcg_residual<RP, BD, VT>), dim3(g), dim3(tile_threads_host(BD)), 0, s,
-                       T, (const double *)x, epi_all);
-    hipLaunchKernelGGL(k_bcg_rres, dim3(nb), dim3(256), 0, s, (const uint32_t *)rows,
-                       (const double *)prz, (const double *)prr, S, 2);
+    replace(1, 2);
     if (hipMemcpyAsync(hstate.data(), S.state, sizeof(int) * nb, hipMemcpyDeviceToHost, s) !=
             hipSuccess ||
         hipMemcpyAsync(hiter.data(), S.iter, sizeof(int) * nb, hipMemcpyDeviceToHost, s) !=

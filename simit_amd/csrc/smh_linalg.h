@@ -143,6 +143,17 @@ __host__ __device__ __forceinline__ float e4m3_to_float(uint32_t b) {
   return (b & 0x80u) ? -v : v;
 }
 
+// FP8 e4m3 byte (low 8 bits of b) -> float times kFP8Scale, without branches:
+// the sign moved to bit 31 and the seven magnitude bits to bits 26..20 read
+// as 2^(e-127) (1 + m/8) for e > 0 and as the FP32 subnormal m 2^-129 for
+// e = 0 (kernels run with FP32 subnormals preserved); one exact multiply by
+// 2^120 * kFP8Scale gives the e4m3 value times kFP8Scale.  (A select between
+// the normal and subnormal forms compiles to branches that serialise the
+// tile loop's gathers.)
+__device__ __forceinline__ float e4m3_scaled(uint32_t b) {
+  return __uint_as_float(((b & 0x80u) << 24) | ((b & 0x7Fu) << 20)) * (0x1p120f * kFP8Scale);
+}
+
 // stored value -> compute type
 template <int PREC>
 __device__ __forceinline__ typename Val<PREC>::T val_of(typename Val<PREC>::S v) {
@@ -264,7 +275,7 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
       for (int b = 0; b < BD; ++b) {
         T val;
         if constexpr (Val<PREC>::kFP8)
-          val = (T)(e4m3_to_float((uint32_t)cur.v[u][0] >> (8 * b)) * kFP8Scale);
+          val = (T)e4m3_scaled((uint32_t)cur.v[u][0] >> (8 * b));
         else
           val = val_of<PREC>(cur.v[u][b]);
         if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][b];
