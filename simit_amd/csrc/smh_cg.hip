@@ -145,6 +145,11 @@ struct XPlain {
   const T *x;
   int bd;
   __device__ T operator()(uint32_t j, int b) const { return x[(uint64_t)j * bd + b]; }
+  // x_j (BD entries) as one vector load
+  template <int BD>
+  __device__ void row(uint32_t j, T *out) const {
+    load_row<T, BD, false>(x + (uint64_t)j * BD, out);
+  }
 };
 
 template <typename T, int BD>

@@ -24,6 +24,8 @@
 // owns a contiguous eighth of the rows and its L2 holds the x entries those
 // rows gather.
 #pragma once
+#include <type_traits>
+
 #include "smh_internal.h"
 
 namespace smh {
@@ -50,12 +52,52 @@ struct TileShape {
   static constexpr int kThreads = BD == 3 ? SMH_TILE_THREADS3 : 256;
   static constexpr int kRounds = BD == 3 ? SMH_TILE_ROUNDS3 : SMH_TILE_ROUNDS1;  // per lane
   static constexpr int kWaves = kThreads / 64;
-  static constexpr int kPerRound = kThreads / BD;        // blocks per round
-  static constexpr int kActive = kPerRound * BD;         // lanes holding a block row
-  static constexpr int kBlocks = kPerRound * kRounds;    // blocks per tile
+  static constexpr int kPerWave = 64 / BD;                // blocks per wave and round
+  static constexpr int kPerRound = kWaves * kPerWave;     // blocks per round
+  static constexpr int kBlocks = kPerRound * kRounds;     // blocks per tile
+  static_assert(kThreads % 64 == 0, "tiles are made of whole wavefronts");
 };
 inline uint32_t tile_blocks_host(int bd) {
-  return bd == 3 ? (SMH_TILE_THREADS3 / 3) * SMH_TILE_ROUNDS3 : 256 * SMH_TILE_ROUNDS1;
+  return bd == 3 ? (SMH_TILE_THREADS3 / 64) * 21 * SMH_TILE_ROUNDS3 : 256 * SMH_TILE_ROUNDS1;
+}
+
+// column index of a lane that gathers nothing in a round
+constexpr uint32_t kNoCol = 0xFFFFFFFFu;
+
+// BD consecutive entries at p in as few loads as fit them exactly; NT:
+// non-temporal (streamed once).  Three floats: one 12-byte load (a 3-vector
+// type with its alignment lowered to 4 reads exactly 12 bytes).  Three
+// doubles: 16 + 8 bytes (a 3-vector of doubles is lowered to two 16-byte
+// loads, which would read past the row).
+template <typename S, int BD, bool NT>
+__device__ __forceinline__ void load_row(const S *p, S *out) {
+  if constexpr (BD == 3 && std::is_same<S, float>::value) {
+    typedef float v3 __attribute__((ext_vector_type(3), aligned(4)));
+    v3 r;
+    if constexpr (NT) r = __builtin_nontemporal_load((const v3 *)p);
+    else r = *(const v3 *)p;
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+  } else if constexpr (BD == 3 && std::is_same<S, double>::value) {
+    typedef double v2 __attribute__((ext_vector_type(2), aligned(8)));
+    v2 r;
+    if constexpr (NT) {
+      r = __builtin_nontemporal_load((const v2 *)p);
+      out[2] = __builtin_nontemporal_load(p + 2);
+    } else {
+      r = *(const v2 *)p;
+      out[2] = p[2];
+    }
+    out[0] = r.x;
+    out[1] = r.y;
+  } else {
+#pragma unroll
+    for (int b = 0; b < BD; ++b) {
+      if constexpr (NT) out[b] = __builtin_nontemporal_load(p + b);
+      else out[b] = p[b];
+    }
+  }
 }
 inline uint32_t tile_threads_host(int bd) { return bd == 3 ? SMH_TILE_THREADS3 : 256; }
 
@@ -198,34 +240,47 @@ __device__ __forceinline__ void wg_tiles(const TileDev &A, uint32_t &k0, uint32_
 }
 
 // Registers holding one tile's values and columns for one lane.
+//
+// Lane layout within a wavefront, per round: block s of the wave's kPerWave
+// blocks is held by lanes BD*s .. BD*s+BD-1, one block row each (values read
+// fully coalesced).  For BD > 1 the column index and the gathered x of block
+// s are loaded once, by lane s, and handed to the block's lanes by a
+// cross-lane shuffle: the contiguous gathering lanes keep the address unit's
+// work per block at one column and one x request instead of BD of each.
 template <int PREC, int BD>
 struct TileRegs {
   using S = typename Val<PREC>::S;
   static constexpr int kRounds = TileShape<BD>::kRounds;
   S v[kRounds][BD];
   float vl[Val<PREC>::kLo ? kRounds : 1][BD];
-  uint32_t col[kRounds];
+  uint32_t col[kRounds];  // the column this lane gathers (kNoCol: none)
 
-  __device__ __forceinline__ void load(const TileDev &A, uint32_t base, uint32_t end, int s,
-                                       int bi) {
+  __device__ __forceinline__ void load(const TileDev &A, uint32_t base, uint32_t end, int w,
+                                       int lane) {
     using SH = TileShape<BD>;
     constexpr int BB = BD * BD;
+    const int s = w * SH::kPerWave + lane / BD, bi = lane % BD;
+    const bool ok = lane < SH::kPerWave * BD;
 #pragma unroll
     for (int u = 0; u < kRounds; ++u) {
       const uint32_t blk = base + s + u * SH::kPerRound;
-      const bool act = s < SH::kPerRound && blk < end;
-      col[u] = act ? A.colidx[blk] : 0u;
+      const bool act = ok && blk < end;
+      const uint32_t gblk = base + w * SH::kPerWave + lane + u * SH::kPerRound;
+      col[u] = (lane < SH::kPerWave && gblk < end) ? A.colidx[gblk] : kNoCol;
       if constexpr (Val<PREC>::kFP8) {
         v[u][0] = act ? SMH_STREAM_LOAD((const uint32_t *)A.vals + (uint64_t)blk * BD + bi) : 0u;
         continue;
       }
       const uint64_t o = (uint64_t)blk * BB + bi * BD;
+      if (act) {
+        load_row<S, BD, true>((const S *)A.vals + o, v[u]);
+        if constexpr (Val<PREC>::kLo) load_row<float, BD, true>((const float *)A.vals_lo + o, vl[u]);
+      } else {
 #pragma unroll
-      for (int b = 0; b < BD; ++b) {
-        v[u][b] = act ? SMH_STREAM_LOAD((const S *)A.vals + o + b) : (S)0;
-        if (Val<PREC>::kLo)
-          vl[Val<PREC>::kLo ? u : 0][b] =
-              act ? SMH_STREAM_LOAD((const float *)A.vals_lo + o + b) : 0.0f;
+        for (int b = 0; b < BD; ++b) {
+          v[u][b] = (S)0;
+          if constexpr (Val<PREC>::kLo) vl[u][b] = 0.0f;
+        }
       }
     }
   }
@@ -248,7 +303,9 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
   using T = typename Val<PREC>::T;
   using SH = TileShape<BD>;
   const int t = threadIdx.x;
-  const int s = t / BD, bi = t - (t / BD) * BD;
+  const int lane = t & 63, w = t >> 6;
+  const int s = w * SH::kPerWave + lane / BD, bi = lane % BD;
+  const bool ok = lane < SH::kPerWave * BD;
   if (k0 >= k1) return;
   if (t == 0) {
     tile_bounds(A, k0, sm.bnd[0]);
@@ -256,7 +313,7 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
   }
   __syncthreads();
   TileRegs<PREC, BD> cur, nxt;
-  cur.load(A, sm.bnd[0][2], sm.bnd[0][3], s, bi);
+  cur.load(A, sm.bnd[0][2], sm.bnd[0][3], w, lane);
   for (uint32_t k = k0; k < k1; ++k) {
     const uint32_t R0 = sm.bnd[0][0], R1 = sm.bnd[0][1], base = sm.bnd[0][2];
     const uint32_t nbase = sm.bnd[1][2], nend = sm.bnd[1][3];
@@ -268,23 +325,53 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
     if (small)
       for (uint32_t i = t; i <= nr; i += SH::kThreads) sm.rp[i] = A.rowptr[R0 + i] - base;
     // partial row sums of this lane's blocks
+    // gathers in groups of kG rounds: the group's x loads go out together
+    // (a lane past the tile's end gathers row 0, whose product multiplies a
+    // zero value and lands in a partial slot no row reads)
+    constexpr int kG = SH::kRounds < 4 ? SH::kRounds : 4;
+    static_assert(SH::kRounds % kG == 0, "rounds come in whole gather groups");
 #pragma unroll
-    for (int u = 0; u < SH::kRounds; ++u) {
-      T acc = 0;
+    for (int u0 = 0; u0 < SH::kRounds; u0 += kG) {
+      T xo[kG][BD];
+      if (BD == 1 || lane < SH::kPerWave) {
 #pragma unroll
-      for (int b = 0; b < BD; ++b) {
-        T val;
-        if constexpr (Val<PREC>::kFP8)
-          val = (T)e4m3_scaled((uint32_t)cur.v[u][0] >> (8 * b));
-        else
-          val = val_of<PREC>(cur.v[u][b]);
-        if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][b];
-        acc += val * xg(cur.col[u], b);
+        for (int j = 0; j < kG; ++j) {
+          const uint32_t c = cur.col[u0 + j];
+          xg.template row<BD>(c == kNoCol ? 0u : c, xo[j]);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < kG; ++j)
+#pragma unroll
+          for (int b = 0; b < BD; ++b) xo[j][b] = T(0);
       }
-      if (t < SH::kActive) sm.part[(s + u * SH::kPerRound) * BD + bi] = acc;
+#pragma unroll
+      for (int j = 0; j < kG; ++j) {
+        const int u = u0 + j;
+        T xv[BD];
+#pragma unroll
+        for (int b = 0; b < BD; ++b) {
+          if constexpr (BD == 1)
+            xv[b] = xo[j][b];
+          else
+            xv[b] = __shfl(xo[j][b], lane / BD, 64);
+        }
+        T acc = 0;
+#pragma unroll
+        for (int b = 0; b < BD; ++b) {
+          T val;
+          if constexpr (Val<PREC>::kFP8)
+            val = (T)e4m3_scaled((uint32_t)cur.v[u][0] >> (8 * b));
+          else
+            val = val_of<PREC>(cur.v[u][b]);
+          if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][b];
+          acc += val * xv[b];
+        }
+        if (ok) sm.part[(s + u * SH::kPerRound) * BD + bi] = acc;
+      }
     }
     // the next tile's loads go out before this tile is reduced
-    if (k + 1 < k1) nxt.load(A, nbase, nend, s, bi);
+    if (k + 1 < k1) nxt.load(A, nbase, nend, w, lane);
     __syncthreads();
     if (small) {
       for (uint32_t i = t; i < nr * BD; i += SH::kThreads) {
