@@ -251,7 +251,8 @@ template <int PREC, int BD>
 struct TileRegs {
   using S = typename Val<PREC>::S;
   static constexpr int kRounds = TileShape<BD>::kRounds;
-  S v[kRounds][BD];
+  static constexpr int kV = Val<PREC>::kFP8 ? 1 : BD;  // FP8: one word per block row
+  S v[kRounds][kV];
   float vl[Val<PREC>::kLo ? kRounds : 1][BD];
   uint32_t col[kRounds];  // the column this lane gathers (kNoCol: none)
 
@@ -261,26 +262,58 @@ struct TileRegs {
     constexpr int BB = BD * BD;
     const int s = w * SH::kPerWave + lane / BD, bi = lane % BD;
     const bool ok = lane < SH::kPerWave * BD;
+    if (base >= end) {  // a tile without blocks (the matrix may have none)
+#pragma unroll
+      for (int u = 0; u < kRounds; ++u) {
+        col[u] = kNoCol;
+#pragma unroll
+        for (int b = 0; b < kV; ++b) v[u][b] = (S)0;
+        if constexpr (Val<PREC>::kLo)
+#pragma unroll
+          for (int b = 0; b < BD; ++b) vl[u][b] = 0.0f;
+      }
+      return;
+    }
 #pragma unroll
     for (int u = 0; u < kRounds; ++u) {
       const uint32_t blk = base + s + u * SH::kPerRound;
       const bool act = ok && blk < end;
       const uint32_t gblk = base + w * SH::kPerWave + lane + u * SH::kPerRound;
       col[u] = (lane < SH::kPerWave && gblk < end) ? A.colidx[gblk] : kNoCol;
-      if constexpr (Val<PREC>::kFP8) {
-        v[u][0] = act ? SMH_STREAM_LOAD((const uint32_t *)A.vals + (uint64_t)blk * BD + bi) : 0u;
+      // 4-byte values: lanes without a block skip the load (measured ~7%
+      // faster on FP32 than loading a dummy row).  8-byte values: branch-free
+      // (below); the conditional form makes the compiler keep FP64 rows in
+      // scratch memory.
+      if constexpr (sizeof(S) < 8) {
+        if constexpr (Val<PREC>::kFP8) {
+          v[u][0] = act ? SMH_STREAM_LOAD((const uint32_t *)A.vals + (uint64_t)blk * BD + bi) : 0u;
+          continue;
+        }
+        const uint64_t o = (uint64_t)blk * BB + bi * BD;
+        if (act) {
+          load_row<S, BD, true>((const S *)A.vals + o, v[u]);
+          if constexpr (Val<PREC>::kLo) load_row<float, BD, true>((const float *)A.vals_lo + o, vl[u]);
+        } else {
+#pragma unroll
+          for (int b = 0; b < kV; ++b) v[u][b] = (S)0;
+          if constexpr (Val<PREC>::kLo)
+#pragma unroll
+            for (int b = 0; b < BD; ++b) vl[u][b] = 0.0f;
+        }
         continue;
       }
-      const uint64_t o = (uint64_t)blk * BB + bi * BD;
-      if (act) {
-        load_row<S, BD, true>((const S *)A.vals + o, v[u]);
-        if constexpr (Val<PREC>::kLo) load_row<float, BD, true>((const float *)A.vals_lo + o, vl[u]);
-      } else {
+      // branch-free: a lane without a block reads row 0 of block 0 (present
+      // whenever the tile has blocks) and keeps zeros
+      const uint64_t o = act ? (uint64_t)blk * BB + bi * BD : 0;
+      S t[BD];
+      load_row<S, BD, true>((const S *)A.vals + o, t);
 #pragma unroll
-        for (int b = 0; b < BD; ++b) {
-          v[u][b] = (S)0;
-          if constexpr (Val<PREC>::kLo) vl[u][b] = 0.0f;
-        }
+      for (int b = 0; b < BD; ++b) v[u][b] = act ? t[b] : (S)0;
+      if constexpr (Val<PREC>::kLo) {
+        float tl[BD];
+        load_row<float, BD, true>((const float *)A.vals_lo + o, tl);
+#pragma unroll
+        for (int b = 0; b < BD; ++b) vl[u][b] = act ? tl[b] : 0.0f;
       }
     }
   }
