@@ -34,11 +34,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per FP32 SpMV launch on the 96^3 x 5 system, from two
-# `rocprofv3 --pmc` runs of `bench.py --kernels-only`: FETCH_SIZE 256.05 MiB
-# x 2.0 + WRITE_SIZE 11.75 MiB x 1.0 (factors from the same run's copy of
-# known size; summary and commands in profiles/spmv_f32_tet96.txt)
-SPMV_TRAFFIC_96 = 549_300_000
+# HBM bytes per FP32 SpMV launch, from two `rocprofv3 --pmc` runs (FETCH_SIZE,
+# then WRITE_SIZE) of `bench.py --kernels-only`, each counter scaled by what it
+# reads for a copy of known size (y = x over 2^27 doubles: factors 2.0 and 1.0);
+# summaries and commands in profiles/.
+# 96^3 x 5 tets: FETCH 237.27 MiB x 2.0 + WRITE 12.60 MiB x 1.0
+SPMV_TRAFFIC_96 = 510_798_243
+# 192^3 hexes (scalar): FETCH 940.04 MiB x 2.0 + WRITE 35.43 MiB x 1.0
+SPMV_TRAFFIC_HEX192 = 2_008_549_290
 METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
 
 
@@ -365,6 +368,11 @@ def run_hex(args, ctx):
     spmv_gbs = spmv_bytes / spmv_ms / 1e6
     log(f"[bench] assembly {asm_ms:.2f} ms {asm_gbs:.0f} GB/s; spmv {spmv_ms * 1e3:.1f} us "
         f"{spmv_gbs:.0f} GB/s")
+    if args.kernels_only:
+        print(json.dumps({"kernels_only": True, "workload": "hex", "index_ms": index_ms,
+                          "assembly_ms": asm_ms, "assembly_GBps": asm_gbs,
+                          "spmv_us": spmv_ms * 1e3, "spmv_GBps": spmv_gbs}))
+        return
 
     def opts(**kw):
         return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
@@ -445,7 +453,8 @@ def run_hex(args, ctx):
         "spmv_GBps": round(spmv_gbs, 1),
         "roofline": {"bound": "hbm", "kernel": "k_bsrmv_tiles<SMH_F32,1> (FP32 scalar CSR SpMV)",
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": SPMV_TRAFFIC_HEX192 if n == 192 else None,
                      "algorithmic_bytes": spmv_bytes},
         "cpu_baseline": cpu}), flush=True)
 
@@ -479,6 +488,24 @@ def run_batch(args, ctx):
     rhs = torch.empty(3 * npts, dtype=torch.float64, device="cuda")
     A2 = idx.bsr(3, S.F32X2, hi, lo)
     opts = ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, mixed=1, check_every=8)
+    # the dominant kernel: the FP32 tile SpMV over the union, timed with HIP
+    # events on the stream it runs on
+    ctx.assemble_springs(idx, P, dev, dev, S.F32X2, hi, lo, rhs)
+    A32 = idx.bsr(3, S.F32, hi)
+    xs = torch.randn(3 * npts, dtype=torch.float32, device="cuda")
+    ys = torch.empty_like(xs)
+    ctx.bsrmv(A32, xs, ys)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(20):
+        ctx.bsrmv(A32, xs, ys)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    spmv_ms = e0.elapsed_time(e1) / 20
+    spmv_bytes = 40 * nnz + 4 * (npts + 1) + 24 * npts
+    spmv_gbs = spmv_bytes / spmv_ms / 1e6
+    log(f"[bench] spmv f32 {spmv_ms * 1e3:.1f} us {spmv_gbs:.0f} GB/s")
 
     def step():
         ctx.assemble_springs(idx, P, dev, dev, S.F32X2, hi, lo, rhs)
@@ -561,7 +588,12 @@ def run_batch(args, ctx):
         "converged": ok, "worst_true_rel": worst, "iterations_min": min(iters),
         "iterations_max": max(iters), "one_at_a_time_seconds": round(seq_s, 4),
         "speedup_vs_one_at_a_time": round(seq_s / step_s, 2), "index_build_ms": round(index_ms, 2),
-        "roofline": None, "cpu_baseline": cpu}), flush=True)
+        "spmv_us": round(spmv_ms * 1e3, 1), "spmv_GBps": round(spmv_gbs, 1),
+        "roofline": {"bound": "hbm", "kernel": "k_bsrmv_tiles<SMH_F32,3> (FP32 3x3 BSR SpMV)",
+                     "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes": spmv_bytes},
+        "cpu_baseline": cpu}), flush=True)
 
 
 if __name__ == "__main__":
