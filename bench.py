@@ -40,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # summaries and commands in profiles/.
 # 96^3 x 5 tets: FETCH 237.27 MiB x 2.0 + WRITE 12.60 MiB x 1.0
 SPMV_TRAFFIC_96 = 510_798_243
-# 192^3 hexes (scalar): FETCH 940.04 MiB x 2.0 + WRITE 35.43 MiB x 1.0
-SPMV_TRAFFIC_HEX192 = 2_008_549_290
+# 192^3 hexes (scalar, FP32 values, FP64 vectors): not measured yet
+SPMV_TRAFFIC_HEX192 = None
 METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
 
 
@@ -316,9 +316,10 @@ def main():
 def run_hex(args, ctx):
     """192^3 trilinear-hex scalar Poisson (7.15M unknowns, 27-point rows): log10 k
     lognormal (std 1.5, correlation length 12 cells), T = 0 on x = 0, uniform
-    source.  Step = assemble FP32 hi/lo + mixed Jacobi-PCG (FP32 SpMV, FP64
-    dots and reliable updates) to a certified 1e-8.  The learned correction
-    works on 3x3 blocks only, so this workload runs without it."""
+    source.  Step = assemble FP32 hi/lo + mixed Jacobi-PCG to a certified 1e-8:
+    FP32 matrix values in the SpMV, FP64 vectors, dots and reliable updates
+    (FP32 vectors cost this system 55% more iterations).  The learned
+    correction works on 3x3 blocks only, so this workload runs without it."""
     import torch
     import simit_amd as S
     from simit_amd import meshes
@@ -359,12 +360,13 @@ def run_hex(args, ctx):
     asm_ms = timed(assemble, 3)
     asm_bytes = 8 * nnz + 8 * nv + 32 * nh + 8 * nh + 4 * nv
     asm_gbs = asm_bytes / asm_ms / 1e6
-    A32 = idx.bsr(1, S.F32, hi)
-    xs = torch.randn(nv, dtype=torch.float32, device="cuda")
+    # the inner SpMV of the step: FP32 values, FP64 vectors
+    A32 = idx.bsr(1, S.F32D, hi)
+    xs = torch.randn(nv, dtype=torch.float64, device="cuda")
     ys = torch.empty_like(xs)
     ctx.bsrmv(A32, xs, ys)
     spmv_ms = timed(lambda: ctx.bsrmv(A32, xs, ys), 20)
-    spmv_bytes = 8 * nnz + 4 * (nv + 1) + 8 * nv
+    spmv_bytes = 8 * nnz + 4 * (nv + 1) + 16 * nv
     spmv_gbs = spmv_bytes / spmv_ms / 1e6
     log(f"[bench] assembly {asm_ms:.2f} ms {asm_gbs:.0f} GB/s; spmv {spmv_ms * 1e3:.1f} us "
         f"{spmv_gbs:.0f} GB/s")
@@ -380,7 +382,7 @@ def run_hex(args, ctx):
     def step():
         assemble()
         x = torch.zeros(nv, dtype=torch.float64, device="cuda")
-        return ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=1))
+        return ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=2))
 
     for w in range(args.warmup):
         st = step()
@@ -437,12 +439,13 @@ def run_hex(args, ctx):
         "metric": METRIC, "value": round(step_s, 4), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 matrix/SpMV, fp64 dots+refinement",
+        "dtype": "fp32 matrix values, fp64 vectors+dots+refinement",
         "data": "synthetic (seeded Gaussian random field conductivity, generated on the host)",
         "config": {"workload": f"trilinear hex Poisson {n}^3 cells, {nv} unknowns "
-                               f"({int(prob['dirichlet'].sum())} Dirichlet), mixed Jacobi-PCG "
-                               f"to a 1e-8 FP64 relative residual (no learned correction: "
-                               f"3x3 blocks only)", "nverts": nv, "nhex": nh, "nnz": nnz},
+                               f"({int(prob['dirichlet'].sum())} Dirichlet), Jacobi-PCG with FP32 "
+                               f"matrix values and FP64 vectors to a 1e-8 FP64 relative residual "
+                               f"(no learned correction: 3x3 blocks only)", "nverts": nv,
+                   "nhex": nh, "nnz": nnz},
         "iterations": stats[-1]["iters"], "true_rel": stats[-1]["true_rel"],
         "converged": all(s["status"] == 0 and s["true_rel"] <= 1e-8 for s in stats),
         "jacobi_pcg_f64": jac64, "plain_cg_f64": plain,
@@ -451,7 +454,8 @@ def run_hex(args, ctx):
         "index_build_ms": round(index_ms, 2), "assembly_ms": round(asm_ms, 3),
         "assembly_GBps": round(asm_gbs, 1), "spmv_us": round(spmv_ms * 1e3, 1),
         "spmv_GBps": round(spmv_gbs, 1),
-        "roofline": {"bound": "hbm", "kernel": "k_bsrmv_tiles<SMH_F32,1> (FP32 scalar CSR SpMV)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_bsrmv_tiles<SMH_F32D,1> (scalar CSR SpMV: FP32 values, FP64 vectors)",
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
                      "traffic": SPMV_TRAFFIC_HEX192 if n == 192 else None,

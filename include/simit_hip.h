@@ -39,6 +39,7 @@ extern "C" {
 #define SMH_F64 0   /* double values                                         */
 #define SMH_F32 1   /* float values                                          */
 #define SMH_F32X2 2 /* float hi + float lo = the FP64 value to ~2^-48        */
+#define SMH_F32D 3  /* float values, double vectors (smh_bsrmv, smh_cg_solve) */
 
 /* index build flags */
 #define SMH_INDEX_FORCE_DIAG 1u /* every row holds its own vertex            */
@@ -112,7 +113,7 @@ typedef struct {
   uint64_t nnz;           /* blocks                                        */
   const uint32_t *rowptr; /* device                                        */
   const uint32_t *colidx; /* device                                        */
-  int32_t precision;      /* SMH_F64 / SMH_F32 / SMH_F32X2                 */
+  int32_t precision;      /* SMH_F64 / SMH_F32 / SMH_F32X2 / SMH_F32D      */
   const void *vals;       /* device: double or float (hi part for F32X2)   */
   const void *vals_lo;    /* device: float lo part (SMH_F32X2 only)        */
 } smh_bsr;

@@ -19,7 +19,7 @@ from pathlib import Path
 __all__ = [
     "SimitError", "load", "Context", "Index", "Bsr", "CGOptions", "CGStats",
     "SpringsParams", "TetParams", "HexParams",
-    "F64", "F32", "F32X2",
+    "F64", "F32", "F32X2", "F32D",
     "PRECOND_NONE", "PRECOND_JACOBI", "PRECOND_BLOCK_JACOBI", "PRECOND_LEARNED",
     "INDEX_FORCE_DIAG", "LIB_PATH", "HEADER_PATH",
 ]
@@ -29,7 +29,7 @@ LIB_PATH = PKG / "lib" / "libsimit_hip.so"
 HEADER_PATH = PKG.parent / "include" / "simit_hip.h"
 
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM, ERR_BREAKDOWN, ERR_NOT_CONVERGED = range(7)
-F64, F32, F32X2 = 0, 1, 2
+F64, F32, F32X2, F32D = 0, 1, 2, 3  # F32D: float values, double vectors
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2, 3
 INDEX_FORCE_DIAG = 1
 

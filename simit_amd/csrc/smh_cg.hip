@@ -1434,6 +1434,8 @@ extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
   SMH_CHECK(ctx, A->bdim == 1 || A->bdim == 3 || (exact_order && A->bdim >= 1 && A->bdim <= 8),
             SMH_ERR_UNSUPPORTED,
             "smh_bsrmv: block size %u (1 or 3; 1..8 with exact_order)", A->bdim);
+  SMH_CHECK(ctx, A->precision >= SMH_F64 && A->precision <= SMH_F32D, SMH_ERR_INVALID,
+            "smh_bsrmv: unknown precision %d", A->precision);
   SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
             "smh_bsrmv: SMH_F32X2 needs vals_lo");
   if (A->nrows == 0) return SMH_OK;
@@ -1469,6 +1471,7 @@ extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
   } while (0)
   if (A->precision == SMH_F64) SMH_MV(SMH_F64);
   else if (A->precision == SMH_F32) SMH_MV(SMH_F32);
+  else if (A->precision == SMH_F32D) SMH_MV(kF32D);
   else SMH_MV(SMH_F32X2);
 #undef SMH_MV
   SMH_LAUNCH_CHECK(ctx);
@@ -1479,6 +1482,8 @@ extern "C" int smh_block_jacobi(smh_ctx *ctx, const smh_bsr *A, double *d_dinv) 
   if (!ctx) return SMH_ERR_INVALID;
   SMH_CHECK(ctx, A && d_dinv && (A->bdim == 1 || A->bdim == 3), SMH_ERR_INVALID,
             "smh_block_jacobi: bad argument");
+  SMH_CHECK(ctx, A->precision >= SMH_F64 && A->precision <= SMH_F32D, SMH_ERR_INVALID,
+            "smh_block_jacobi: unknown precision %d", A->precision);
   const int pc = A->bdim == 3 ? SMH_PRECOND_BLOCK_JACOBI : SMH_PRECOND_JACOBI;
   const unsigned g = grid_for(A->nrows, 256);
   if (A->precision == SMH_F64)
@@ -1526,6 +1531,8 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
             "smh_cg_solve: block Jacobi needs 3x3 blocks");
   SMH_CHECK(ctx, !o->mixed || A->precision == SMH_F32X2, SMH_ERR_INVALID,
             "smh_cg_solve: mixed precision needs an SMH_F32X2 matrix");
+  SMH_CHECK(ctx, A->precision >= SMH_F64 && A->precision <= SMH_F32D, SMH_ERR_INVALID,
+            "smh_cg_solve: unknown precision %d", A->precision);
   SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
             "smh_cg_solve: SMH_F32X2 needs vals_lo");
   SMH_CHECK(ctx, o->maxiters >= 0, SMH_ERR_INVALID, "smh_cg_solve: negative maxiters");
@@ -1572,6 +1579,8 @@ extern "C" int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nba
             "smh_cg_solve_batched: mixed precision needs an SMH_F32X2 matrix");
   SMH_CHECK(ctx, o->mixed == 0 || o->mixed == 1, SMH_ERR_INVALID,
             "smh_cg_solve_batched: mixed must be 0 or 1");
+  SMH_CHECK(ctx, A->precision >= SMH_F64 && A->precision <= SMH_F32D, SMH_ERR_INVALID,
+            "smh_cg_solve_batched: unknown precision %d", A->precision);
   SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
             "smh_cg_solve_batched: SMH_F32X2 needs vals_lo");
   SMH_CHECK(ctx, rows[0] == 0 && rows[nbatch] == A->nrows, SMH_ERR_INVALID,

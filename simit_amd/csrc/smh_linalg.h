@@ -36,8 +36,11 @@ namespace smh {
 #ifndef SMH_TILE_ROUNDS3
 #define SMH_TILE_ROUNDS3 16
 #endif
+#ifndef SMH_TILE_VEC1
+#define SMH_TILE_VEC1 4
+#endif
 #ifndef SMH_TILE_ROUNDS1
-#define SMH_TILE_ROUNDS1 4
+#define SMH_TILE_ROUNDS1 1
 #endif
 // matrix values are read once per product: streaming (non-temporal) loads
 #ifndef SMH_STREAM_LOAD
@@ -52,17 +55,55 @@ struct TileShape {
   static constexpr int kThreads = BD == 3 ? SMH_TILE_THREADS3 : 256;
   static constexpr int kRounds = BD == 3 ? SMH_TILE_ROUNDS3 : SMH_TILE_ROUNDS1;  // per lane
   static constexpr int kWaves = kThreads / 64;
-  static constexpr int kPerWave = 64 / BD;                // blocks per wave and round
+  static constexpr int kVec = BD == 1 ? SMH_TILE_VEC1 : 1;  // consecutive blocks per lane
+  static constexpr int kPerWave = 64 / BD * kVec;         // blocks per wave and round
   static constexpr int kPerRound = kWaves * kPerWave;     // blocks per round
   static constexpr int kBlocks = kPerRound * kRounds;     // blocks per tile
   static_assert(kThreads % 64 == 0, "tiles are made of whole wavefronts");
 };
 inline uint32_t tile_blocks_host(int bd) {
-  return bd == 3 ? (SMH_TILE_THREADS3 / 64) * 21 * SMH_TILE_ROUNDS3 : 256 * SMH_TILE_ROUNDS1;
+  return bd == 3 ? (SMH_TILE_THREADS3 / 64) * 21 * SMH_TILE_ROUNDS3
+                 : 256 * SMH_TILE_VEC1 * SMH_TILE_ROUNDS1;
 }
 
 // column index of a lane that gathers nothing in a round
 constexpr uint32_t kNoCol = 0xFFFFFFFFu;
+
+// N (4) consecutive entries at p as 16-byte loads (dword-aligned addresses are
+// enough for global loads); other types element by element
+template <typename S, int N, bool NT>
+__device__ __forceinline__ void load_vec(const S *p, S *out) {
+  if constexpr (N == 4 && sizeof(S) == 4) {
+    typedef S v4 __attribute__((ext_vector_type(4), aligned(4)));
+    v4 r;
+    if constexpr (NT) r = __builtin_nontemporal_load((const v4 *)p);
+    else r = *(const v4 *)p;
+    out[0] = r.x;
+    out[1] = r.y;
+    out[2] = r.z;
+    out[3] = r.w;
+  } else if constexpr (N == 4 && std::is_same<S, double>::value) {
+    typedef double v2 __attribute__((ext_vector_type(2), aligned(8)));
+    v2 r0, r1;
+    if constexpr (NT) {
+      r0 = __builtin_nontemporal_load((const v2 *)p);
+      r1 = __builtin_nontemporal_load((const v2 *)(p + 2));
+    } else {
+      r0 = *(const v2 *)p;
+      r1 = *(const v2 *)(p + 2);
+    }
+    out[0] = r0.x;
+    out[1] = r0.y;
+    out[2] = r1.x;
+    out[3] = r1.y;
+  } else {
+#pragma unroll
+    for (int b = 0; b < N; ++b) {
+      if constexpr (NT) out[b] = __builtin_nontemporal_load(p + b);
+      else out[b] = p[b];
+    }
+  }
+}
 
 // BD consecutive entries at p in as few loads as fit them exactly; NT:
 // non-temporal (streamed once).  Three floats: one 12-byte load (a 3-vector
@@ -319,6 +360,41 @@ struct TileRegs {
   }
 };
 
+// Registers of one lane for 1x1 tiles: kVec consecutive entries per round,
+// values and columns loaded as vectors (a 4-byte value per request would
+// leave the address unit, not HBM, as the limit).
+template <int PREC>
+struct TileRegsV {
+  using S = typename Val<PREC>::S;
+  using SH = TileShape<1>;
+  static constexpr int kRounds = SH::kRounds, kVec = SH::kVec;
+  S v[kRounds][kVec];
+  float vl[Val<PREC>::kLo ? kRounds : 1][kVec];
+  uint32_t col[kRounds][kVec];  // kNoCol: past the tile's end
+
+  __device__ __forceinline__ void load(const TileDev &A, uint32_t base, uint32_t end, int w,
+                                       int lane) {
+#pragma unroll
+    for (int u = 0; u < kRounds; ++u) {
+      const uint32_t b0 = base + u * SH::kPerRound + (uint32_t)(w * 64 + lane) * kVec;
+      if (b0 + kVec <= end) {
+        load_vec<S, kVec, true>((const S *)A.vals + b0, v[u]);
+        load_vec<uint32_t, kVec, false>(A.colidx + b0, col[u]);
+        if constexpr (Val<PREC>::kLo)
+          load_vec<float, kVec, true>((const float *)A.vals_lo + b0, vl[u]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < kVec; ++j) {
+          const bool a = b0 + j < end;
+          v[u][j] = a ? ((const S *)A.vals)[b0 + j] : (S)0;
+          col[u][j] = a ? A.colidx[b0 + j] : kNoCol;
+          if constexpr (Val<PREC>::kLo) vl[u][j] = a ? ((const float *)A.vals_lo)[b0 + j] : 0.0f;
+        }
+      }
+    }
+  }
+};
+
 // Tile boundaries (R0, R1, base, end) of tile k into slot.
 __device__ __forceinline__ void tile_bounds(const TileDev &A, uint32_t k, uint32_t *slot) {
   const uint32_t R0 = A.tile_row[k], R1 = A.tile_row[k + 1];
@@ -345,7 +421,8 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
     if (k0 + 1 < k1) tile_bounds(A, k0 + 1, sm.bnd[1]);
   }
   __syncthreads();
-  TileRegs<PREC, BD> cur, nxt;
+  using Regs = typename std::conditional<BD == 1, TileRegsV<PREC>, TileRegs<PREC, BD>>::type;
+  Regs cur, nxt;
   cur.load(A, sm.bnd[0][2], sm.bnd[0][3], w, lane);
   for (uint32_t k = k0; k < k1; ++k) {
     const uint32_t R0 = sm.bnd[0][0], R1 = sm.bnd[0][1], base = sm.bnd[0][2];
@@ -358,49 +435,69 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
     if (small)
       for (uint32_t i = t; i <= nr; i += SH::kThreads) sm.rp[i] = A.rowptr[R0 + i] - base;
     // partial row sums of this lane's blocks
-    // gathers in groups of kG rounds: the group's x loads go out together
-    // (a lane past the tile's end gathers row 0, whose product multiplies a
-    // zero value and lands in a partial slot no row reads)
-    constexpr int kG = SH::kRounds < 4 ? SH::kRounds : 4;
-    static_assert(SH::kRounds % kG == 0, "rounds come in whole gather groups");
+    if constexpr (BD == 1) {
 #pragma unroll
-    for (int u0 = 0; u0 < SH::kRounds; u0 += kG) {
-      T xo[kG][BD];
-      if (BD == 1 || lane < SH::kPerWave) {
+      for (int u = 0; u < SH::kRounds; ++u) {
+        constexpr int kVec = SH::kVec;
+        T xo[kVec];
 #pragma unroll
-        for (int j = 0; j < kG; ++j) {
-          const uint32_t c = cur.col[u0 + j];
-          xg.template row<BD>(c == kNoCol ? 0u : c, xo[j]);
+        for (int q = 0; q < kVec; ++q) {
+          const uint32_t c = cur.col[u][q];
+          xo[q] = xg(c == kNoCol ? 0u : c, 0);
         }
-      } else {
+        const uint32_t s0 = u * SH::kPerRound + (uint32_t)(w * 64 + lane) * kVec;
 #pragma unroll
-        for (int j = 0; j < kG; ++j)
-#pragma unroll
-          for (int b = 0; b < BD; ++b) xo[j][b] = T(0);
+        for (int q = 0; q < kVec; ++q) {
+          T val = val_of<PREC>(cur.v[u][q]);
+          if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][q];
+          sm.part[s0 + q] = val * xo[q];
+        }
       }
-#pragma unroll
-      for (int j = 0; j < kG; ++j) {
-        const int u = u0 + j;
-        T xv[BD];
-#pragma unroll
-        for (int b = 0; b < BD; ++b) {
-          if constexpr (BD == 1)
-            xv[b] = xo[j][b];
-          else
-            xv[b] = __shfl(xo[j][b], lane / BD, 64);
+    } else {
+      // gathers in groups of kG rounds: the group's x loads go out together
+      // (a lane past the tile's end gathers row 0, whose product multiplies a
+      // zero value and lands in a partial slot no row reads)
+      constexpr int kG = SH::kRounds < 4 ? SH::kRounds : 4;
+      static_assert(SH::kRounds % kG == 0, "rounds come in whole gather groups");
+  #pragma unroll
+      for (int u0 = 0; u0 < SH::kRounds; u0 += kG) {
+        T xo[kG][BD];
+        if (BD == 1 || lane < SH::kPerWave) {
+  #pragma unroll
+          for (int j = 0; j < kG; ++j) {
+            const uint32_t c = cur.col[u0 + j];
+            xg.template row<BD>(c == kNoCol ? 0u : c, xo[j]);
+          }
+        } else {
+  #pragma unroll
+          for (int j = 0; j < kG; ++j)
+  #pragma unroll
+            for (int b = 0; b < BD; ++b) xo[j][b] = T(0);
         }
-        T acc = 0;
-#pragma unroll
-        for (int b = 0; b < BD; ++b) {
-          T val;
-          if constexpr (Val<PREC>::kFP8)
-            val = (T)e4m3_scaled((uint32_t)cur.v[u][0] >> (8 * b));
-          else
-            val = val_of<PREC>(cur.v[u][b]);
-          if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][b];
-          acc += val * xv[b];
+  #pragma unroll
+        for (int j = 0; j < kG; ++j) {
+          const int u = u0 + j;
+          T xv[BD];
+  #pragma unroll
+          for (int b = 0; b < BD; ++b) {
+            if constexpr (BD == 1)
+              xv[b] = xo[j][b];
+            else
+              xv[b] = __shfl(xo[j][b], lane / BD, 64);
+          }
+          T acc = 0;
+  #pragma unroll
+          for (int b = 0; b < BD; ++b) {
+            T val;
+            if constexpr (Val<PREC>::kFP8)
+              val = (T)e4m3_scaled((uint32_t)cur.v[u][0] >> (8 * b));
+            else
+              val = val_of<PREC>(cur.v[u][b]);
+            if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][b];
+            acc += val * xv[b];
+          }
+          if (ok) sm.part[(s + u * SH::kPerRound) * BD + bi] = acc;
         }
-        if (ok) sm.part[(s + u * SH::kPerRound) * BD + bi] = acc;
       }
     }
     // the next tile's loads go out before this tile is reduced

@@ -70,6 +70,11 @@ def test_fast_spmv_all_precisions(ctx, oracle, kind, n):
     ctx.bsrmv(idx.bsr(bd, S.F32, to_dev(v32)), to_dev(x32), y32)
     ref32 = oracle.bsrmv(rp, ci, v32.astype(np.float64), x32.astype(np.float64), bd)
     assert (np.abs(y32.cpu().numpy() - ref32) <= 128 * 2.0 ** -24 * absrow + 1e-30).all()
+    # FP32 values, FP64 vectors and accumulation (SMH_F32D)
+    yd = torch.empty(len(xv), dtype=torch.float64, device="cuda")
+    ctx.bsrmv(idx.bsr(bd, S.F32D, to_dev(v32)), to_dev(xv), yd)
+    refd = oracle.bsrmv(rp, ci, v32.astype(np.float64), xv, bd)
+    assert (np.abs(yd.cpu().numpy() - refd) <= 1e-14 * absrow + 1e-300).all()
     # hi + lo split of the FP64 values, FP64 vectors
     hi = vals.astype(np.float32)
     lo = (vals - hi.astype(np.float64)).astype(np.float32)
