@@ -459,26 +459,26 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
       // zero value and lands in a partial slot no row reads)
       constexpr int kG = SH::kRounds < 4 ? SH::kRounds : 4;
       static_assert(SH::kRounds % kG == 0, "rounds come in whole gather groups");
-  #pragma unroll
+#pragma unroll
       for (int u0 = 0; u0 < SH::kRounds; u0 += kG) {
         T xo[kG][BD];
         if (BD == 1 || lane < SH::kPerWave) {
-  #pragma unroll
+#pragma unroll
           for (int j = 0; j < kG; ++j) {
             const uint32_t c = cur.col[u0 + j];
             xg.template row<BD>(c == kNoCol ? 0u : c, xo[j]);
           }
         } else {
-  #pragma unroll
+#pragma unroll
           for (int j = 0; j < kG; ++j)
-  #pragma unroll
+#pragma unroll
             for (int b = 0; b < BD; ++b) xo[j][b] = T(0);
         }
-  #pragma unroll
+#pragma unroll
         for (int j = 0; j < kG; ++j) {
           const int u = u0 + j;
           T xv[BD];
-  #pragma unroll
+#pragma unroll
           for (int b = 0; b < BD; ++b) {
             if constexpr (BD == 1)
               xv[b] = xo[j][b];
@@ -486,7 +486,7 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
               xv[b] = __shfl(xo[j][b], lane / BD, 64);
           }
           T acc = 0;
-  #pragma unroll
+#pragma unroll
           for (int b = 0; b < BD; ++b) {
             T val;
             if constexpr (Val<PREC>::kFP8)
