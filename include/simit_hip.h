@@ -269,15 +269,16 @@ typedef struct {
   int32_t check_every;   /* iterations between host convergence checks     */
   const float *learned;  /* SMH_PRECOND_LEARNED: device float[nrows*9],
                             Linv_i = L_i^-1 for A_ii = L_i L_i^T (row-major,
-                            lower triangular), as smh_gnn_precond writes */
+                            lower triangular); float[nrows], 1/sqrt(a_ii),
+                            for 1x1 blocks; as smh_gnn_precond writes */
   int32_t zr_stop;       /* 1: also stop when z'r <= the threshold, as
                             thermal.sim's pre_conjgrad does (:115)          */
-  const uint8_t *learned_corr; /* SMH_PRECOND_LEARNED: device bytes
-                            [nnz*12], 4-byte aligned: the normalised
-                            correction blocks C_k on A's pattern (zero on
-                            the diagonal) as FP8 e4m3 of 32 C_k, three
-                            values and a zero pad byte per block row; the
-                            preconditioner is
+  const uint8_t *learned_corr; /* SMH_PRECOND_LEARNED: device bytes,
+                            4-byte aligned: the normalised correction blocks
+                            C_k on A's pattern (zero on the diagonal); 3x3
+                            blocks: [nnz*12], FP8 e4m3 of 32 C_k, three
+                            values and a zero pad byte per block row; 1x1
+                            blocks: float[nnz]; the preconditioner is
                             z_i = Linv_i^T (Linv_i r_i + sum_k C_k Linv_j r_j) */
 } smh_cg_opts;
 typedef struct {
@@ -330,13 +331,14 @@ int smh_gnn_destroy(smh_ctx *ctx, smh_gnn *g);
  * W_m[96x32] b_m W_u[64x32] b_u, then W_d[96x16] b_d (row-major, in x out) */
 int smh_gnn_get_params(smh_ctx *ctx, smh_gnn *g, float *host);
 int smh_gnn_set_params(smh_ctx *ctx, smh_gnn *g, const float *host);
-/* the preconditioner of A: linv = device float[nrows*9], corr = device
- * bytes [nnz*12] in the FP8 layout of opts.learned_corr (pass both as
+/* the preconditioner of A (3x3 or 1x1 blocks): linv = device
+ * float[nrows*bdim*bdim], corr = device bytes [nnz*12] (3x3, FP8) or
+ * [nnz*4] (1x1, float) in the layout of opts.learned_corr (pass both as
  * opts.learned / opts.learned_corr) */
 int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float *linv, uint8_t *corr);
 /* loss and gradient on A for nprobes probe vectors: probes = device
- * float[3*nrows*nprobes] (row-major [dof][probe]) or NULL to draw them from
- * seed; grads (host float[SMH_GNN_NPARAMS]) may be NULL */
+ * float[bdim*nrows*nprobes] (row-major [dof][probe]) or NULL to draw them
+ * from seed; grads (host float[SMH_GNN_NPARAMS]) may be NULL */
 int smh_gnn_loss_grad(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int32_t nprobes,
                       const float *probes, uint64_t seed, double *loss, float *grads);
 /* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on A with fresh probes;

@@ -26,6 +26,9 @@ Model (all sizes fixed: F = 16 input features, H = 32 hidden):
   loss            L = sum (V - (I + C) Ahat V)^2 / (K * ndof) over K probe
                   vectors V: the stochastic Frobenius norm of I - (I + C) Ahat.
 
+Scalar systems (1x1 blocks, bd = 1): Linv_i = 1 / sqrt(a_ii); Ahat_k and the
+sums take feature slot 0 (slots 1-8 zero); the decoder's first output is R_k.
+
 With mimic=True the oracle rounds to bfloat16 where the GPU kernels store or
 feed BF16 (weights, MFMA inputs h / a / features, stored e and m); gradients
 pass those roundings straight through, as the GPU's do.
@@ -129,26 +132,29 @@ def graph(rowptr, colidx):
             "rowlen": rowlen, "deg": deg}
 
 
-def features(g, vals):
-    """Linv, normalised blocks and the input features of the network."""
-    V = np.asarray(vals, dtype=np.float64).reshape(-1, 3, 3)
+def features(g, vals, bd=3):
+    """Linv, normalised blocks and the input features of the network.  For
+    1x1 blocks (bd = 1) the scalar Ahat_k takes feature slot 0 and slots 1-8
+    stay zero; Linv_i = 1 / sqrt(a_ii)."""
+    bb = bd * bd
+    V = np.asarray(vals, dtype=np.float64).reshape(-1, bd, bd)
     n, brow, ci, diag = g["n"], g["brow"], g["ci"], g["diag"]
-    D = np.tile(np.eye(3), (n, 1, 1))
+    D = np.tile(np.eye(bd), (n, 1, 1))
     D[brow[diag]] = V[diag]
-    Linv = chol_inv3(D)
+    Linv = chol_inv3(D) if bd == 3 else 1.0 / np.sqrt(D)
     Ah = Linv[brow] @ V @ Linv[ci].transpose(0, 2, 1)
-    Ah[diag] = np.eye(3)
+    Ah[diag] = np.eye(bd)
     nrm = np.sqrt((Ah * Ah).sum(axis=(1, 2)))
     fe = np.zeros((V.shape[0], F))
-    fe[:, :9] = Ah.reshape(-1, 9)
+    fe[:, :bb] = Ah.reshape(-1, bb)
     fe[:, 9] = nrm
     fe[:, 10] = diag.astype(np.float64)
     off = ~diag
     fv = np.zeros((n, F))
-    np.add.at(fv[:, :9], brow[off], Ah[off].reshape(-1, 9))
+    np.add.at(fv[:, :bb], brow[off], Ah[off].reshape(-1, bb))
     np.add.at(fv[:, 9], brow[off], nrm[off])
     fv[:, 10] = g["deg"] / 16.0
-    return {"Linv": Linv, "Ah": Ah, "fe": fe, "fv": fv}
+    return {"Linv": Linv, "Ah": Ah, "fe": fe, "fv": fv, "bd": bd}
 
 
 # ---- network ------------------------------------------------------------------------------
@@ -184,12 +190,14 @@ def forward(params, g, ft, mimic=False):
     Xd = np.concatenate([hq[brow], hq[ci], e], axis=1)
     R = Xd @ W["W_d"] + params["b_d"]
     cache.update(h_out=h, Xd=Xd, W=W)
-    return R[:, :9].copy(), cache
+    bb = ft.get("bd", 3) ** 2
+    return R[:, :bb].copy(), cache
 
 
 def output(R, g, ft, kappa):
     """C (normalised correction) and B (the preconditioner blocks)."""
-    Rm = R.reshape(-1, 3, 3)
+    bd = ft.get("bd", 3)
+    Rm = R.reshape(-1, bd, bd)
     S = 0.5 * (Rm + Rm[g["t"]].transpose(0, 2, 1))
     deg = g["deg"]
     md = np.maximum(np.maximum(deg[g["brow"]], deg[g["ci"]]), 1).astype(np.float64)
@@ -199,7 +207,7 @@ def output(R, g, ft, kappa):
     C = (c * phi)[:, None, None] * S
     C[g["diag"]] = 0.0
     Bh = C.copy()
-    Bh[g["diag"]] = np.eye(3)
+    Bh[g["diag"]] = np.eye(bd)
     Linv = ft["Linv"]
     B = Linv[g["brow"]].transpose(0, 2, 1) @ Bh @ Linv[g["ci"]]
     # exact symmetry: the block of the lower position of each pair is the
@@ -218,13 +226,13 @@ def apply_blocks(g, blocks, x3):
 
 def loss(C, g, ft, V):
     """L, dL/dC and W = Ahat V for probes V [ndof, K]."""
-    n = g["n"]
+    n, bd = g["n"], ft.get("bd", 3)
     K = V.shape[1]
-    V3 = V.reshape(n, 3, K)
+    V3 = V.reshape(n, bd, K)
     W3 = apply_blocks(g, ft["Ah"], V3)
     Y3 = W3 + apply_blocks(g, C, W3)
     res = V3 - Y3
-    N = K * 3 * n
+    N = K * bd * n
     L = float((res * res).sum() / N)
     G = -2.0 * res / N
     dC = G[g["brow"]] @ W3[g["ci"]].transpose(0, 2, 1)
@@ -238,7 +246,7 @@ def output_backward(dC, g, oc):
     dS = (c * phi)[:, None, None] * dC - (c * phi ** 3 * inner)[:, None, None] * S
     dS[g["diag"]] = 0.0
     dR = 0.5 * dS + 0.5 * dS[g["t"]].transpose(0, 2, 1)
-    return dR.reshape(-1, 9)
+    return dR.reshape(dR.shape[0], -1)
 
 
 def backward(params, g, cache, dR9):
@@ -246,7 +254,7 @@ def backward(params, g, cache, dR9):
     brow, ci, n = g["brow"], g["ci"], g["n"]
     grads = {k: np.zeros_like(v) for k, v in params.items()}
     dR = np.zeros((dR9.shape[0], 16))
-    dR[:, :9] = dR9
+    dR[:, :dR9.shape[1]] = dR9
     grads["W_d"] = cache["Xd"].T @ dR
     grads["b_d"] = dR.sum(axis=0)
     dX = dR @ W["W_d"].T
@@ -279,9 +287,9 @@ def backward(params, g, cache, dR9):
     return grads
 
 
-def loss_and_grads(params, rowptr, colidx, vals, V, kappa=0.9, mimic=False):
+def loss_and_grads(params, rowptr, colidx, vals, V, kappa=0.9, mimic=False, bd=3):
     g = graph(rowptr, colidx)
-    ft = features(g, vals)
+    ft = features(g, vals, bd)
     R, cache = forward(params, g, ft, mimic)
     C, B, oc = output(R, g, ft, kappa)
     L, dC, _ = loss(C, g, ft, V)
@@ -289,9 +297,9 @@ def loss_and_grads(params, rowptr, colidx, vals, V, kappa=0.9, mimic=False):
     return L, backward(params, g, cache, dR), {"R": R, "C": C, "B": B, "g": g, "ft": ft}
 
 
-def precond_blocks(params, rowptr, colidx, vals, kappa=0.9, mimic=False):
+def precond_blocks(params, rowptr, colidx, vals, kappa=0.9, mimic=False, bd=3):
     g = graph(rowptr, colidx)
-    ft = features(g, vals)
+    ft = features(g, vals, bd)
     R, _ = forward(params, g, ft, mimic)
     C, B, _ = output(R, g, ft, kappa)
     return B, {"R": R, "C": C, "g": g, "ft": ft}

@@ -354,10 +354,12 @@ class Gnn:
                                                        a.ctypes.data_as(C.c_void_p)))
 
     def precond(self, A: Bsr):
-        """(linv float32[nrows*9], corr uint8[nnz*12] FP8 blocks) device tensors."""
+        """(linv, corr) device tensors: 3x3 blocks: float32[nrows*9] and uint8[nnz*12]
+        (FP8 blocks); 1x1 blocks: float32[nrows] and uint8[nnz*4] (float32 values)."""
         import torch
-        linv = torch.empty(A.nrows * 9, dtype=torch.float32, device="cuda")
-        corr = torch.empty(A.nnz * 12, dtype=torch.uint8, device="cuda")
+        bb = A.bdim * A.bdim
+        linv = torch.empty(A.nrows * bb, dtype=torch.float32, device="cuda")
+        corr = torch.empty(A.nnz * (12 if A.bdim == 3 else 4), dtype=torch.uint8, device="cuda")
         self.ctx._check(self.ctx.lib.smh_gnn_precond(self.ctx.h, self.h, C.byref(A),
                                                     _ptr(linv), _ptr(corr)))
         return linv, corr

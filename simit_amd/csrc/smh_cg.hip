@@ -193,6 +193,8 @@ __device__ __forceinline__ void apply_precond(int pc, const double *r, double *z
                                               const void *dinv, bool f32, uint64_t i) {
   if (pc == SMH_PRECOND_NONE) {
     for (int a = 0; a < BD; ++a) z[a] = r[a];
+  } else if (BD == 1 && pc == SMH_PRECOND_LEARNED) {
+    z[0] = (double)((const float *)dinv)[i] * r[0];  // u = Linv r, Linv = 1/sqrt(a_ii)
   } else if (BD == 3 && pc == SMH_PRECOND_LEARNED) {
     // first half of the learned preconditioner: u = Linv r (the rest,
     // z = Linv^T (u + C u), needs the neighbours' u: k_cg_lapply)
@@ -328,16 +330,26 @@ k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT
 }
 
 // K2b (learned preconditioner): z_i = Linv_i^T (u_i + sum_k C_k u_j) over
-// the pattern of A (C in FP8, zero on the diagonal), and the workgroup
-// partials of z'r and r'r.  check_done = 0 on the residual-replacement path,
-// which runs after the stopping test already set done.
-template <typename VT>
+// the pattern of A (C zero on the diagonal; FP8 for 3x3 blocks, FP32 for
+// scalar systems), and the workgroup partials of z'r and r'r.  check_done = 0
+// on the residual-replacement path, which runs after the stopping test
+// already set done.
+template <typename VT, int BD>
 struct EpiLearned {
   const VT *u, *r;
   VT *z;
   const float *linv;
   double drz, drr;
   __device__ void operator()(uint32_t row, const VT *s) {
+    if constexpr (BD == 1) {
+      const double t = (double)linv[row] * ((double)u[row] + (double)s[0]);
+      const VT zz = (VT)t;
+      z[row] = zz;
+      const double rv = (double)r[row];
+      drz += (double)zz * rv;
+      drr += rv * rv;
+      return;
+    }
     double y[3];
     for (int a = 0; a < 3; ++a) y[a] = (double)u[(uint64_t)row * 3 + a] + (double)s[a];
     const float *L = linv + (uint64_t)row * 9;
@@ -354,21 +366,21 @@ struct EpiLearned {
   }
 };
 
-template <int LP>
-__global__ void __launch_bounds__(TileShape<3>::kThreads)
+template <int LP, int BD>
+__global__ void __launch_bounds__(TileShape<BD>::kThreads)
 k_cg_lapply(TileDev C, const typename Val<LP>::T *__restrict__ u,
             const typename Val<LP>::T *__restrict__ r, typename Val<LP>::T *__restrict__ z,
             const float *__restrict__ linv, const CGState *S, int check_done, double *part) {
   using VT = typename Val<LP>::T;
-  __shared__ TileSmem<LP, 3> sm;
+  __shared__ TileSmem<LP, BD> sm;
   if (check_done && S->done) return;
   uint32_t k0, k1;
   wg_tiles(C, k0, k1);
-  XPlain<VT> xg{u, 3};
-  EpiLearned<VT> epi{u, r, z, linv, 0.0, 0.0};
-  spmv_tiles<LP, 3>(C, k0, k1, sm, xg, epi);
-  const double t1 = block_sum<TileShape<3>::kWaves>(epi.drz, sm.red);
-  const double t2 = block_sum<TileShape<3>::kWaves>(epi.drr, sm.red);
+  XPlain<VT> xg{u, BD};
+  EpiLearned<VT, BD> epi{u, r, z, linv, 0.0, 0.0};
+  spmv_tiles<LP, BD>(C, k0, k1, sm, xg, epi);
+  const double t1 = block_sum<TileShape<BD>::kWaves>(epi.drz, sm.red);
+  const double t2 = block_sum<TileShape<BD>::kWaves>(epi.drr, sm.red);
   if (threadIdx.x == 0) {
     part[2 * blockIdx.x] = t1;
     part[2 * blockIdx.x + 1] = t2;
@@ -640,8 +652,10 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   const unsigned nbe = grid_for(nrows, 256, 2048);
   const unsigned nbv = grid_for(n, 256, 2048);
   const int pc = o->precond;
-  const bool learned = BD == 3 && pc == SMH_PRECOND_LEARNED;
-  constexpr int LP = sizeof(VT) == 4 ? kFP8F : kFP8D;
+  const bool learned = pc == SMH_PRECOND_LEARNED;
+  // the correction's storage: FP8 words (3x3 blocks) or FP32 (scalar systems)
+  constexpr int LP = BD == 3 ? (sizeof(VT) == 4 ? kFP8F : kFP8D)
+                             : (sizeof(VT) == 4 ? SMH_F32 : kF32D);
   smh_bsr Cb = *Am;
   Cb.vals = learned ? (const void *)o->learned_corr : Am->vals;
   Cb.vals_lo = nullptr;
@@ -710,7 +724,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
                      (const double *)x, epi, w.part);
   if (learned) {
-    hipLaunchKernelGGL((k_cg_lapply<LP>), dim3(gc), dim3(tile_threads_host(3)), 0, ctx->stream, TC,
+    hipLaunchKernelGGL((k_cg_lapply<LP, BD>), dim3(gc), dim3(tile_threads_host(BD)), 0, ctx->stream, TC,
                        (const VT *)w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
                        (const CGState *)w.S, 0, w.part);
     hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
@@ -731,7 +745,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
       hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
                          (const double *)x, epi, w.part);
       if (learned)
-        hipLaunchKernelGGL((k_cg_lapply<LP>), dim3(gc), dim3(tile_threads_host(3)), 0,
+        hipLaunchKernelGGL((k_cg_lapply<LP, BD>), dim3(gc), dim3(tile_threads_host(BD)), 0,
                            ctx->stream, TC, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
                            (const float *)o->learned, (const CGState *)w.S, 0, w.part);
       hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
@@ -752,7 +766,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
                          pdata, x, (VT *)w.r, (VT *)(learned ? w.u : w.z), (const VT *)w.p,
                          (const VT *)w.q, (const CGState *)w.S, w.part + kPartSlots);
       if (learned)
-        hipLaunchKernelGGL((k_cg_lapply<LP>), dim3(gc), dim3(tile_threads_host(3)), 0,
+        hipLaunchKernelGGL((k_cg_lapply<LP, BD>), dim3(gc), dim3(tile_threads_host(BD)), 0,
                            ctx->stream, TC, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
                            (const float *)o->learned, (const CGState *)w.S, 1,
                            w.part + kPartSlots);
@@ -1523,11 +1537,10 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
             "smh_cg_solve: unknown preconditioner %d", o->precond);
   SMH_CHECK(ctx,
             o->precond != SMH_PRECOND_LEARNED ||
-                (A->bdim == 3 && o->learned && o->learned_corr &&
-                 ((uintptr_t)o->learned_corr & 3u) == 0),
-            SMH_ERR_INVALID, "smh_cg_solve: the learned preconditioner needs 3x3 blocks, "
-                             "opts.learned and opts.learned_corr (smh_gnn_precond)");
-  SMH_CHECK(ctx, A->bdim == 3 || o->precond <= SMH_PRECOND_JACOBI, SMH_ERR_INVALID,
+                (o->learned && o->learned_corr && ((uintptr_t)o->learned_corr & 3u) == 0),
+            SMH_ERR_INVALID, "smh_cg_solve: the learned preconditioner needs opts.learned and "
+                             "opts.learned_corr (smh_gnn_precond)");
+  SMH_CHECK(ctx, A->bdim == 3 || o->precond != SMH_PRECOND_BLOCK_JACOBI, SMH_ERR_INVALID,
             "smh_cg_solve: block Jacobi needs 3x3 blocks");
   SMH_CHECK(ctx, !o->mixed || A->precision == SMH_F32X2, SMH_ERR_INVALID,
             "smh_cg_solve: mixed precision needs an SMH_F32X2 matrix");

@@ -138,12 +138,22 @@ __device__ __forceinline__ double aval(const void *v, const void *lo, uint64_t o
 
 // Linv_i from the diagonal block (Cholesky of its lower triangle); I when
 // the row has no diagonal block or the block is not positive definite
-template <int PREC>
+template <int PREC, int B>
 __global__ void k_gnn_linv(uint32_t n, const uint32_t *__restrict__ rp,
                            const uint32_t *__restrict__ ci, const void *vals, const void *vlo,
                            float *__restrict__ linv) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if constexpr (B == 1) {  // scalar: 1 / sqrt(a_ii), 1 without a positive diagonal
+    double d = 1.0;
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
+      if (ci[k] == i) {
+        d = aval<PREC>(vals, vlo, k);
+        break;
+      }
+    linv[i] = (float)(d > 0.0 ? 1.0 / sqrt(d) : 1.0);
+    return;
+  }
   double d[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
   for (uint32_t k = rp[i]; k < rp[i + 1]; ++k)
     if (ci[k] == i) {
@@ -179,7 +189,7 @@ __global__ void k_gnn_linv(uint32_t n, const uint32_t *__restrict__ rp,
 
 // normalised blocks, edge features (and FP32 Ahat for training), vertex
 // features; one thread per row, blocks in CSR order
-template <int PREC>
+template <int PREC, int B>
 __global__ void k_gnn_features(uint32_t n, const uint32_t *__restrict__ rp,
                                const uint32_t *__restrict__ ci, const void *vals,
                                const void *vlo, const float *__restrict__ linv,
@@ -187,6 +197,34 @@ __global__ void k_gnn_features(uint32_t n, const uint32_t *__restrict__ rp,
                                uint16_t *__restrict__ fv, float *__restrict__ ahat) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  if constexpr (B == 1) {  // scalar: Ahat_k in slot 0, slots 1-8 zero
+    const double li = (double)linv[i];
+    double sv = 0.0, sn = 0.0;
+    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+      const uint32_t j = ci[k];
+      const bool dg = j == i;
+      const double ah = dg ? 1.0 : li * aval<PREC>(vals, vlo, k) * (double)linv[j];
+      const double nr = fabs(ah);
+      uint16_t *f = fe + (uint64_t)k * kF;
+      f[0] = f2bf((float)ah);
+      for (int q = 1; q < 9; ++q) f[q] = 0;
+      f[9] = f2bf((float)nr);
+      f[10] = f2bf(dg ? 1.0f : 0.0f);
+      for (int q = 11; q < kF; ++q) f[q] = 0;
+      if (ahat) ahat[k] = (float)ah;
+      if (!dg) {
+        sv += ah;
+        sn += nr;
+      }
+    }
+    uint16_t *f = fv + (uint64_t)i * kF;
+    f[0] = f2bf((float)sv);
+    for (int q = 1; q < 9; ++q) f[q] = 0;
+    f[9] = f2bf((float)sn);
+    f[10] = f2bf((float)deg[i] / 16.0f);
+    for (int q = 11; q < kF; ++q) f[q] = 0;
+    return;
+  }
   double Li[9];
   for (int q = 0; q < 9; ++q) Li[q] = (double)linv[(uint64_t)i * 9 + q];
   double sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sn = 0.0;
@@ -428,17 +466,21 @@ k_gnn_decode(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *__
 }
 
 // ---- output: symmetric, scaled correction --------------------------------------------------
+// B x B blocks (B = 3, or 1 for scalar systems, which use the decoder's first
+// output); R is [nnz][9]
+template <int B>
 struct OutScale {
-  float S[9], phi, c;
+  float S[B * B], phi, c;
 };
-__device__ __forceinline__ OutScale out_scale(const float *R, uint64_t k, uint64_t tk,
-                                              uint32_t di, uint32_t dj, float kappa) {
-  OutScale o;
+template <int B>
+__device__ __forceinline__ OutScale<B> out_scale(const float *R, uint64_t k, uint64_t tk,
+                                                 uint32_t di, uint32_t dj, float kappa) {
+  OutScale<B> o;
   float n2 = 0.0f;
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) {
+  for (int a = 0; a < B; ++a)
+    for (int b = 0; b < B; ++b) {
       const float s = 0.5f * (R[k * 9 + 3 * a + b] + R[tk * 9 + 3 * b + a]);
-      o.S[3 * a + b] = s;
+      o.S[B * a + b] = s;
       n2 += s * s;
     }
   o.phi = 1.0f / sqrtf(1.0f + n2);
@@ -448,34 +490,42 @@ __device__ __forceinline__ OutScale out_scale(const float *R, uint64_t k, uint64
 }
 
 // C_k (the block of the lower position of each pair is the transpose of the
-// upper one's, so the stored correction is exactly symmetric)
+// upper one's, so the stored correction is exactly symmetric).  Storage for
+// the solver: B = 3, FP8 of 2^5 C, three values and a zero pad byte per block
+// row (corr); B = 1, FP32 (corr_f).
+template <int B>
 __global__ void k_gnn_out(uint64_t nnz, const uint32_t *__restrict__ brow,
                           const uint32_t *__restrict__ ci, const uint32_t *__restrict__ t,
                           const uint32_t *__restrict__ deg, const float *__restrict__ R,
-                          float kappa, uint32_t *__restrict__ corr, float *__restrict__ c32) {
+                          float kappa, uint32_t *__restrict__ corr, float *__restrict__ corr_f,
+                          float *__restrict__ c32) {
+  constexpr int BB = B * B;
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
   const uint32_t i = brow[k], j = ci[k];
-  float C[9];
+  float C[BB];
   if (i == j) {
-    for (int q = 0; q < 9; ++q) C[q] = 0.0f;
+    for (int q = 0; q < BB; ++q) C[q] = 0.0f;
   } else {
     const uint64_t tk = t[k];
     const bool lower = tk < k;
     const uint64_t own = lower ? tk : k, oth = lower ? k : tk;
-    const OutScale o = out_scale(R, own, oth, deg[brow[own]], deg[ci[own]], kappa);
+    const OutScale<B> o = out_scale<B>(R, own, oth, deg[brow[own]], deg[ci[own]], kappa);
     const float f = o.c * o.phi;
-    for (int a = 0; a < 3; ++a)
-      for (int b = 0; b < 3; ++b) C[3 * a + b] = f * (lower ? o.S[3 * b + a] : o.S[3 * a + b]);
+    for (int a = 0; a < B; ++a)
+      for (int b = 0; b < B; ++b) C[B * a + b] = f * (lower ? o.S[B * b + a] : o.S[B * a + b]);
   }
-  // FP8 storage of 2^5 C, three values and a zero pad byte per block row
-  if (corr)
-    for (int a = 0; a < 3; ++a)
-      corr[k * 3 + a] = float_to_e4m3(32.0f * C[3 * a]) |
-                        (float_to_e4m3(32.0f * C[3 * a + 1]) << 8) |
-                        (float_to_e4m3(32.0f * C[3 * a + 2]) << 16);
+  if constexpr (B == 3) {
+    if (corr)
+      for (int a = 0; a < 3; ++a)
+        corr[k * 3 + a] = float_to_e4m3(32.0f * C[3 * a]) |
+                          (float_to_e4m3(32.0f * C[3 * a + 1]) << 8) |
+                          (float_to_e4m3(32.0f * C[3 * a + 2]) << 16);
+  } else {
+    if (corr_f) corr_f[k] = C[0];
+  }
   if (c32)
-    for (int q = 0; q < 9; ++q) c32[k * 9 + q] = C[q];
+    for (int q = 0; q < BB; ++q) c32[k * BB + q] = C[q];
 }
 
 // ---- training: probes, loss, output backward ------------------------------------------------
@@ -495,46 +545,49 @@ __global__ void k_gnn_probes(uint64_t n, uint64_t seed, float *__restrict__ V) {
   V[i] = (float)(sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2));
 }
 
-// W = Ahat V (one thread per row; V, W [3 n][K])
+// W = Ahat V (one thread per row; V, W [B n][K])
+template <int B>
 __global__ void k_gnn_ahat_apply(uint32_t n, int K, const uint32_t *__restrict__ rp,
                                  const uint32_t *__restrict__ ci, const float *__restrict__ ahat,
                                  const float *__restrict__ V, float *__restrict__ W) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   for (int p = 0; p < K; ++p) {
-    float s[3] = {0, 0, 0};
+    float s[B];
+    for (int a = 0; a < B; ++a) s[a] = 0.0f;
     for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
       const uint32_t j = ci[k];
-      for (int a = 0; a < 3; ++a)
-        for (int b = 0; b < 3; ++b)
-          s[a] += ahat[(uint64_t)k * 9 + 3 * a + b] * V[((uint64_t)3 * j + b) * K + p];
+      for (int a = 0; a < B; ++a)
+        for (int b = 0; b < B; ++b)
+          s[a] += ahat[(uint64_t)k * B * B + B * a + b] * V[((uint64_t)B * j + b) * K + p];
     }
-    for (int a = 0; a < 3; ++a) W[((uint64_t)3 * i + a) * K + p] = s[a];
+    for (int a = 0; a < B; ++a) W[((uint64_t)B * i + a) * K + p] = s[a];
   }
 }
 
 // res = V - W - C W; loss partials per workgroup; G = dL/dY = -2 res / N
+template <int B>
 __global__ void __launch_bounds__(256)
 k_gnn_loss(uint32_t n, int K, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
            const float *__restrict__ c32, const float *__restrict__ V,
            const float *__restrict__ W, float *__restrict__ G, double *__restrict__ part) {
   __shared__ double s4[4];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const double N = (double)K * 3.0 * n;
+  const double N = (double)K * B * n;
   double acc = 0.0;
   if (i < n) {
     for (int p = 0; p < K; ++p) {
-      float y[3];
-      for (int a = 0; a < 3; ++a) y[a] = W[((uint64_t)3 * i + a) * K + p];
+      float y[B];
+      for (int a = 0; a < B; ++a) y[a] = W[((uint64_t)B * i + a) * K + p];
       for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
         const uint32_t j = ci[k];
         if (j == i) continue;
-        for (int a = 0; a < 3; ++a)
-          for (int b = 0; b < 3; ++b)
-            y[a] += c32[(uint64_t)k * 9 + 3 * a + b] * W[((uint64_t)3 * j + b) * K + p];
+        for (int a = 0; a < B; ++a)
+          for (int b = 0; b < B; ++b)
+            y[a] += c32[(uint64_t)k * B * B + B * a + b] * W[((uint64_t)B * j + b) * K + p];
       }
-      for (int a = 0; a < 3; ++a) {
-        const uint64_t o = ((uint64_t)3 * i + a) * K + p;
+      for (int a = 0; a < B; ++a) {
+        const uint64_t o = ((uint64_t)B * i + a) * K + p;
         const double res = (double)V[o] - (double)y[a];
         acc += res * res;
         G[o] = (float)(-2.0 * res / N);
@@ -546,30 +599,34 @@ k_gnn_loss(uint32_t n, int K, const uint32_t *__restrict__ rp, const uint32_t *_
 }
 
 // dR_k = (dS_k + dS_t(k)^T) / 2 with dS from dC = G W^T (per block)
+template <int B>
 __device__ __forceinline__ void dS_of(uint64_t k, const uint32_t *brow, const uint32_t *ci,
                                       const uint32_t *t, const uint32_t *deg, const float *R,
                                       const float *G, const float *W, int K, float kappa,
                                       float *dS) {
+  constexpr int BB = B * B;
   const uint32_t i = brow[k], j = ci[k];
   if (i == j) {
-    for (int q = 0; q < 9; ++q) dS[q] = 0.0f;
+    for (int q = 0; q < BB; ++q) dS[q] = 0.0f;
     return;
   }
-  float dC[9];
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b) {
+  float dC[BB];
+  for (int a = 0; a < B; ++a)
+    for (int b = 0; b < B; ++b) {
       float s = 0.0f;
       for (int p = 0; p < K; ++p)
-        s += G[((uint64_t)3 * i + a) * K + p] * W[((uint64_t)3 * j + b) * K + p];
-      dC[3 * a + b] = s;
+        s += G[((uint64_t)B * i + a) * K + p] * W[((uint64_t)B * j + b) * K + p];
+      dC[B * a + b] = s;
     }
-  const OutScale o = out_scale(R, k, t[k], deg[i], deg[j], kappa);
+  const OutScale<B> o = out_scale<B>(R, k, t[k], deg[i], deg[j], kappa);
   float inner = 0.0f;
-  for (int q = 0; q < 9; ++q) inner += dC[q] * o.S[q];
+  for (int q = 0; q < BB; ++q) inner += dC[q] * o.S[q];
   const float f = o.c * o.phi, g = o.c * o.phi * o.phi * o.phi * inner;
-  for (int q = 0; q < 9; ++q) dS[q] = f * dC[q] - g * o.S[q];
+  for (int q = 0; q < BB; ++q) dS[q] = f * dC[q] - g * o.S[q];
 }
 
+// dR [nnz][16]: the block's B x B outputs at 3 a + b, zero elsewhere
+template <int B>
 __global__ void k_gnn_out_bwd(uint64_t nnz, const uint32_t *__restrict__ brow,
                               const uint32_t *__restrict__ ci, const uint32_t *__restrict__ t,
                               const uint32_t *__restrict__ deg, const float *__restrict__ R,
@@ -577,13 +634,13 @@ __global__ void k_gnn_out_bwd(uint64_t nnz, const uint32_t *__restrict__ brow,
                               float kappa, float *__restrict__ dR) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
-  float d1[9], d2[9];
-  dS_of(k, brow, ci, t, deg, R, G, W, K, kappa, d1);
-  dS_of(t[k], brow, ci, t, deg, R, G, W, K, kappa, d2);
-  for (int a = 0; a < 3; ++a)
-    for (int b = 0; b < 3; ++b)
-      dR[k * 16 + 3 * a + b] = 0.5f * d1[3 * a + b] + 0.5f * d2[3 * b + a];
-  for (int q = 9; q < 16; ++q) dR[k * 16 + q] = 0.0f;
+  float d1[B * B], d2[B * B];
+  dS_of<B>(k, brow, ci, t, deg, R, G, W, K, kappa, d1);
+  dS_of<B>(t[k], brow, ci, t, deg, R, G, W, K, kappa, d2);
+  for (int q = 0; q < 16; ++q) dR[k * 16 + q] = 0.0f;
+  for (int a = 0; a < B; ++a)
+    for (int b = 0; b < B; ++b)
+      dR[k * 16 + 3 * a + b] = 0.5f * d1[B * a + b] + 0.5f * d2[B * b + a];
 }
 
 // ---- backward building blocks (FP32) ------------------------------------------------------------
@@ -783,6 +840,7 @@ struct smh_gnn {
   uint32_t cap_n = 0;
   uint64_t cap_nnz = 0;
   int cap_k = 0;
+  int cap_b = 0;  // block size the workspace was sized for (1 or 3)
   bool cap_train = false;
   std::vector<void *> bufs;
   uint32_t *brow = nullptr, *tmap = nullptr, *deg = nullptr;
@@ -807,6 +865,7 @@ void free_work(smh_gnn *g) {
   g->cap_n = 0;
   g->cap_nnz = 0;
   g->cap_k = 0;
+  g->cap_b = 0;
   g->cap_train = false;
 }
 
@@ -820,11 +879,13 @@ int walloc(smh_ctx *ctx, smh_gnn *g, T **p, uint64_t count) {
   return SMH_OK;
 }
 
-int ensure_work(smh_ctx *ctx, smh_gnn *g, uint32_t n, uint64_t nnz, int K, bool train) {
-  if (n <= g->cap_n && nnz <= g->cap_nnz && K <= g->cap_k && (!train || g->cap_train))
+int ensure_work(smh_ctx *ctx, smh_gnn *g, uint32_t n, uint64_t nnz, int K, bool train, int B) {
+  if (n <= g->cap_n && nnz <= g->cap_nnz && K <= g->cap_k && (!train || g->cap_train) &&
+      B <= g->cap_b)
     return SMH_OK;
   free_work(g);
   const uint64_t N = std::max<uint64_t>(n, 1), E = std::max<uint64_t>(nnz, 1);
+  const uint64_t BB = (uint64_t)B * B;
   const int Kk = std::max(K, 1);
   int rc = 0;
 #define WA(p, c) \
@@ -833,7 +894,7 @@ int ensure_work(smh_ctx *ctx, smh_gnn *g, uint32_t n, uint64_t nnz, int K, bool 
   WA(g->tmap, E);
   WA(g->deg, N);
   WA(g->err, 4);
-  WA(g->linv, N * 9);
+  WA(g->linv, N * BB);
   WA(g->R, E * 9);
   WA(g->fe, E * kF);
   WA(g->fv, N * kF);
@@ -843,15 +904,15 @@ int ensure_work(smh_ctx *ctx, smh_gnn *g, uint32_t n, uint64_t nnz, int K, bool 
   const int nm = train ? kLayers : 1;
   for (int l = 0; l < nm; ++l) WA(g->m[l], E * kH);
   if (train) {
-    WA(g->ahat, E * 9);
-    WA(g->c32, E * 9);
+    WA(g->ahat, E * BB);
+    WA(g->c32, E * BB);
     for (int l = 0; l < kLayers; ++l) {
       WA(g->a[l], N * kH);
       WA(g->U[l], N * kH);
     }
-    WA(g->V, N * 3 * Kk);
-    WA(g->W, N * 3 * Kk);
-    WA(g->G, N * 3 * Kk);
+    WA(g->V, N * B * Kk);
+    WA(g->W, N * B * Kk);
+    WA(g->G, N * B * Kk);
     WA(g->dR, E * 16);
     WA(g->dX, E * 96);
     WA(g->dh, N * kH);
@@ -868,6 +929,7 @@ int ensure_work(smh_ctx *ctx, smh_gnn *g, uint32_t n, uint64_t nnz, int K, bool 
   g->cap_n = n;
   g->cap_nnz = nnz;
   g->cap_k = K;
+  g->cap_b = B;
   g->cap_train = train;
   return SMH_OK;
 }
@@ -882,8 +944,8 @@ int refresh_wt(smh_ctx *ctx, smh_gnn *g) {
 
 int check_A(smh_ctx *ctx, const smh_bsr *A, const char *who) {
   SMH_CHECK(ctx, A && A->rowptr && A->colidx && A->vals, SMH_ERR_INVALID, "%s: null matrix", who);
-  SMH_CHECK(ctx, A->bdim == 3, SMH_ERR_UNSUPPORTED, "%s: needs 3x3 blocks (got %u)", who,
-            A->bdim);
+  SMH_CHECK(ctx, A->bdim == 3 || A->bdim == 1, SMH_ERR_UNSUPPORTED,
+            "%s: needs 3x3 or 1x1 blocks (got %u)", who, A->bdim);
   SMH_CHECK(ctx, A->precision == SMH_F64 || A->precision == SMH_F32 ||
                      A->precision == SMH_F32X2,
             SMH_ERR_INVALID, "%s: bad precision %d", who, A->precision);
@@ -910,17 +972,23 @@ int forward(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, bool train) {
                      A->colidx, g->brow, g->deg, g->err);
   hipLaunchKernelGGL(k_graph_transpose, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz,
                      A->rowptr, A->colidx, g->brow, g->tmap, g->err);
-#define SMH_FEAT(P)                                                                           \
+#define SMH_FEAT(P, B)                                                                        \
   do {                                                                                        \
-    hipLaunchKernelGGL(k_gnn_linv<P>, dim3(grid_for(n, 256)), dim3(256), 0, s, n, A->rowptr,   \
-                       A->colidx, A->vals, A->vals_lo, g->linv);                              \
-    hipLaunchKernelGGL(k_gnn_features<P>, dim3(grid_for(n, 128)), dim3(128), 0, s, n,          \
+    hipLaunchKernelGGL((k_gnn_linv<P, B>), dim3(grid_for(n, 256)), dim3(256), 0, s, n,         \
+                       A->rowptr, A->colidx, A->vals, A->vals_lo, g->linv);                   \
+    hipLaunchKernelGGL((k_gnn_features<P, B>), dim3(grid_for(n, 128)), dim3(128), 0, s, n,     \
                        A->rowptr, A->colidx, A->vals, A->vals_lo, g->linv, g->deg, g->fe,      \
                        g->fv, train ? g->ahat : nullptr);                                     \
   } while (0)
-  if (A->precision == SMH_F64) SMH_FEAT(SMH_F64);
-  else if (A->precision == SMH_F32) SMH_FEAT(SMH_F32);
-  else SMH_FEAT(SMH_F32X2);
+#define SMH_FEATP(B)                                                                          \
+  do {                                                                                        \
+    if (A->precision == SMH_F64) SMH_FEAT(SMH_F64, B);                                        \
+    else if (A->precision == SMH_F32X2) SMH_FEAT(SMH_F32X2, B);                               \
+    else SMH_FEAT(SMH_F32, B);                                                                \
+  } while (0)
+  if (A->bdim == 1) SMH_FEATP(1);
+  else SMH_FEATP(3);
+#undef SMH_FEATP
 #undef SMH_FEAT
   int herr = 0;
   SMH_HIP(ctx, hipMemcpyAsync(&herr, g->err, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -969,27 +1037,36 @@ int loss_grad(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int K, const float *pr
   const uint32_t n = A->nrows;
   const uint64_t nnz = A->nnz;
   hipStream_t s = ctx->stream;
-  int rc = ensure_work(ctx, g, n, nnz, K, true);
+  const int B = (int)A->bdim;
+  int rc = ensure_work(ctx, g, n, nnz, K, true, B);
   if (rc) return rc;
   rc = forward(ctx, g, A, true);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_gnn_out, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, g->brow,
-                     A->colidx, g->tmap, g->deg, g->R, g->kappa, (uint32_t *)nullptr, g->c32);
-  const uint64_t nv = (uint64_t)3 * n * K;
+  const uint64_t nv = (uint64_t)B * n * K;
   const float *V = probes;
   if (!V) {
     hipLaunchKernelGGL(k_gnn_probes, dim3(grid_for(nv, 256)), dim3(256), 0, s, nv, seed, g->V);
     V = g->V;
   }
-  hipLaunchKernelGGL(k_gnn_ahat_apply, dim3(grid_for(n, 128)), dim3(128), 0, s, n, K, A->rowptr,
-                     A->colidx, g->ahat, V, g->W);
   const unsigned nl = grid_for(n, 256);
-  hipLaunchKernelGGL(k_gnn_loss, dim3(nl), dim3(256), 0, s, n, K, A->rowptr, A->colidx, g->c32, V,
-                     g->W, g->G, g->lpart);
-  hipLaunchKernelGGL(k_gnn_sum, dim3(1), dim3(256), 0, s, (int)nl, (const double *)g->lpart,
-                     g->loss);
-  hipLaunchKernelGGL(k_gnn_out_bwd, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, g->brow,
-                     A->colidx, g->tmap, g->deg, g->R, g->G, g->W, K, g->kappa, g->dR);
+#define SMH_LOSS(B)                                                                            \
+  do {                                                                                         \
+    hipLaunchKernelGGL(k_gnn_out<B>, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, g->brow,  \
+                       A->colidx, g->tmap, g->deg, g->R, g->kappa, (uint32_t *)nullptr,        \
+                       (float *)nullptr, g->c32);                                              \
+    hipLaunchKernelGGL(k_gnn_ahat_apply<B>, dim3(grid_for(n, 128)), dim3(128), 0, s, n, K,     \
+                       A->rowptr, A->colidx, g->ahat, V, g->W);                                \
+    hipLaunchKernelGGL(k_gnn_loss<B>, dim3(nl), dim3(256), 0, s, n, K, A->rowptr, A->colidx,   \
+                       g->c32, V, g->W, g->G, g->lpart);                                       \
+    hipLaunchKernelGGL(k_gnn_sum, dim3(1), dim3(256), 0, s, (int)nl, (const double *)g->lpart, \
+                       g->loss);                                                               \
+    hipLaunchKernelGGL(k_gnn_out_bwd<B>, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz,       \
+                       g->brow, A->colidx, g->tmap, g->deg, g->R, g->G, g->W, K, g->kappa,     \
+                       g->dR);                                                                 \
+  } while (0)
+  if (B == 1) SMH_LOSS(1);
+  else SMH_LOSS(3);
+#undef SMH_LOSS
   // decoder
   float *hfin = g->h[kLayers];
   XEdge xd{g->brow, A->colidx, hfin, g->e};
@@ -1118,12 +1195,18 @@ extern "C" int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float
   int rc = check_A(ctx, A, "smh_gnn_precond");
   if (rc) return rc;
   if (A->nrows == 0) return SMH_OK;
-  if ((rc = ensure_work(ctx, g, A->nrows, A->nnz, g->cap_k, false))) return rc;
+  const int B = (int)A->bdim;
+  if ((rc = ensure_work(ctx, g, A->nrows, A->nnz, g->cap_k, false, B))) return rc;
   if ((rc = forward(ctx, g, A, false))) return rc;
-  hipLaunchKernelGGL(k_gnn_out, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream, A->nnz,
-                     g->brow, A->colidx, g->tmap, g->deg, g->R, g->kappa, (uint32_t *)corr,
-                     nullptr);
-  SMH_HIP(ctx, hipMemcpyAsync(linv, g->linv, sizeof(float) * 9 * (size_t)A->nrows,
+  if (B == 3)
+    hipLaunchKernelGGL(k_gnn_out<3>, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
+                       A->nnz, g->brow, A->colidx, g->tmap, g->deg, g->R, g->kappa,
+                       (uint32_t *)corr, (float *)nullptr, (float *)nullptr);
+  else
+    hipLaunchKernelGGL(k_gnn_out<1>, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
+                       A->nnz, g->brow, A->colidx, g->tmap, g->deg, g->R, g->kappa,
+                       (uint32_t *)nullptr, (float *)corr, (float *)nullptr);
+  SMH_HIP(ctx, hipMemcpyAsync(linv, g->linv, sizeof(float) * B * B * (size_t)A->nrows,
                               hipMemcpyDeviceToDevice, ctx->stream));
   SMH_LAUNCH_CHECK(ctx);
   return SMH_OK;

@@ -1,6 +1,7 @@
 """Training driver of the learned block-Jacobi correction (smh_gnn).
 
-The samples are the tet-elasticity systems this library assembles: one
+The samples are systems this library assembles (hex Poisson samples for
+scalar systems: HexSampler).  For elasticity: one
 n^3 x 5 tet grid (clamped z = 0 face, uniform body load) whose Young's
 modulus is redrawn per sample as log10 E = mean + std * GRF (seeded,
 FFT-smoothed white noise, correlation length in cells), Poisson ratio 0.3.
@@ -58,11 +59,46 @@ class TetSampler:
         return self.A
 
 
+class HexSampler:
+    """One resident n^3 trilinear-hex Poisson mesh (T = 0 on x = 0, uniform
+    source) whose conductivity is redrawn per sample as 10^(std * GRF); the
+    correlation length is in cells, so a small training mesh sees the same
+    local statistics as the full-size system.  Scalar (1x1) systems."""
+
+    def __init__(self, ctx, n: int = 48, std_log10=1.5, corr_len=12.0):
+        import torch
+        from . import HexParams, F32X2
+        self.ctx, self.n = ctx, n
+        self.field = (std_log10, corr_len)
+        prob = meshes.hex_problem(n, seed=0, std_log10=std_log10, corr_len=corr_len)
+        dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+        self.dev = {"dirichlet": dev(prob["dirichlet"]), "kappa": dev(prob["kappa"]),
+                    "endpoints": dev(prob["endpoints"])}
+        self.idx = ctx.build_index(prob["nverts"], [self.dev["endpoints"]])
+        self.params = HexParams(prob["h"], prob["source"], 1.0)
+        self.hi = torch.empty(self.idx.nnz, dtype=torch.float32, device="cuda")
+        self.lo = torch.empty_like(self.hi)
+        self.rhs = torch.empty(prob["nverts"], dtype=torch.float64, device="cuda")
+        self.A = self.idx.bsr(1, F32X2, self.hi, self.lo)
+        self._prec = F32X2
+
+    def sample(self, seed: int):
+        """Assemble the system of conductivity seed (returns the resident Bsr)."""
+        import torch
+        std, corr = self.field
+        g = meshes.gaussian_random_field(self.n, corr, seed).ravel()
+        self.dev["kappa"].copy_(torch.from_numpy(10.0 ** (std * g)), non_blocking=False)
+        self.ctx.assemble_hex(self.idx, self.params, self.dev, self.dev, self._prec, self.hi,
+                              self.lo, self.rhs)
+        return self.A
+
+
 def train(ctx, net, n: int = 32, nsamples: int = 2000, epochs: int = 1, lr: float = 3e-3,
           lr_final: float = 3e-4, nprobes: int = 4, seed0: int = 10_000, log_every: int = 0,
-          log=None, sampler: TetSampler | None = None):
-    """Train net on nsamples seeded n^3 systems; returns a dict of timings and
-    the loss curve (the loss is read back only every log_every steps)."""
+          log=None, sampler=None):
+    """Train net on nsamples seeded systems (n^3 x 5 tets unless a sampler,
+    e.g. HexSampler, is given); returns a dict of timings and the loss curve
+    (the loss is read back only every log_every steps)."""
     smp = sampler or TetSampler(ctx, n)
     steps = nsamples * epochs
     curve = []

@@ -183,3 +183,77 @@ def test_bf16_rounding_matches_definition():
     assert got[5] == 65536.0
     u = np.asarray(got, np.float32).view(np.uint32)
     assert np.all(u & 0xFFFF == 0)
+
+
+# ---- scalar systems (1x1 blocks): the hex Poisson matrix ------------------------------
+def small_hex(oracle, n=3, seed=0):
+    prob = meshes.hex_problem(n, seed=seed, std_log10=1.0, corr_len=2.0)
+    rp, ci = oracle.index_vev(prob["nverts"], [prob["endpoints"]])
+    vals, _ = oracle.hex_assemble(oracle.HexParams(prob["h"], 1.0, 1.0), prob, rp, ci)
+    return rp, ci, vals
+
+
+def dense1(rp, ci, vals, n):
+    M = np.zeros((n, n))
+    for i in range(n):
+        M[i, ci[rp[i]:rp[i + 1]]] = np.asarray(vals).ravel()[rp[i]:rp[i + 1]]
+    return M
+
+
+def test_scalar_normalisation_and_jacobi_at_zero_output(oracle):
+    rp, ci, vals = small_hex(oracle)
+    n = rp.size - 1
+    g = G.graph(rp, ci)
+    ft = G.features(g, vals, bd=1)
+    d = vals[g["diag"]]
+    np.testing.assert_allclose(ft["Linv"].ravel() ** 2 * d, 1.0, rtol=1e-14)
+    assert np.all(ft["fe"][:, 1:9] == 0.0) and np.all(ft["fv"][:, 1:9] == 0.0)
+    p = random_params(1)
+    p["W_d"][:] = 0.0
+    p["b_d"][:] = 0.0
+    B, aux = G.precond_blocks(p, rp, ci, vals, bd=1)
+    assert np.all(aux["C"] == 0.0)
+    np.testing.assert_allclose(dense1(rp, ci, B, n), np.diag(1.0 / d), rtol=1e-13)
+
+
+def test_scalar_preconditioner_symmetric_positive_definite(oracle):
+    rp, ci, vals = small_hex(oracle, n=3, seed=4)
+    n = rp.size - 1
+    B, aux = G.precond_blocks(random_params(2, 30.0), rp, ci, vals, kappa=0.9, bd=1)
+    g = aux["g"]
+    C = aux["C"].ravel()
+    rowsum = np.zeros(n)
+    np.add.at(rowsum, g["brow"], np.abs(C))
+    assert rowsum.max() <= 0.9 * (1 + 1e-12)
+    Bd = dense1(rp, ci, B, n)
+    assert np.array_equal(Bd, Bd.T)
+    assert np.linalg.eigvalsh(Bd).min() > 0
+
+
+def test_scalar_loss_and_gradients(oracle):
+    rp, ci, vals = small_hex(oracle, n=2, seed=1)
+    n = rp.size - 1
+    rng = np.random.default_rng(5)
+    V = rng.normal(size=(n, 3))
+    p = random_params(7, scale=0.7)
+    L, grads, aux = G.loss_and_grads(p, rp, ci, vals, V, bd=1)
+    # dense formula
+    Li = np.diag(aux["ft"]["Linv"].ravel())
+    Ah = Li @ dense1(rp, ci, vals, n) @ Li
+    C = dense1(rp, ci, aux["C"], n)
+    R = V - (np.eye(n) + C) @ Ah @ V
+    assert abs(L - (R * R).sum() / V.size) <= 1e-12 * L
+    # central finite differences
+    flat, gflat = G.flatten(p), G.flatten(grads)
+    idx = np.concatenate([rng.choice(flat.size, 40, replace=False),
+                          [flat.size - 16, flat.size - 16 - 1536]])  # b_d[0], W_d[0][0]
+    eps = 1e-6
+    for k in idx:
+        fp, fm = flat.copy(), flat.copy()
+        fp[k] += eps
+        fm[k] -= eps
+        Lp = G.loss_and_grads(G.unflatten(fp), rp, ci, vals, V, bd=1)[0]
+        Lm = G.loss_and_grads(G.unflatten(fm), rp, ci, vals, V, bd=1)[0]
+        fd = (Lp - Lm) / (2 * eps)
+        assert abs(fd - gflat[k]) <= 1e-6 * max(1.0, abs(gflat).max()) + 1e-5 * abs(fd), \
+            (k, fd, gflat[k])
