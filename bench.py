@@ -63,10 +63,12 @@ def parse():
                     help="index, assembly, SpMV and a copy of known size only (profiling)")
     ap.add_argument("--workload", choices=("tet", "hex", "batch"), default="tet",
                     help="tet: 96^3 x 5 learned-PCG elasticity (default); hex: 192^3 trilinear "
-                         "hex Poisson, mixed Jacobi-PCG; batch: 256 spring lattices solved "
-                         "together by the batched CG")
+                         "hex Poisson, learned PCG trained on 48^3; batch: 256 spring lattices "
+                         "solved together by the batched CG")
     ap.add_argument("--nmesh", type=int, default=256)
     ap.add_argument("--hex-n", type=int, default=192)
+    ap.add_argument("--hex-train-n", type=int, default=48)
+    ap.add_argument("--hex-train-samples", type=int, default=500)
     return ap.parse_args()
 
 
@@ -316,13 +318,16 @@ def main():
 def run_hex(args, ctx):
     """192^3 trilinear-hex scalar Poisson (7.15M unknowns, 27-point rows): log10 k
     lognormal (std 1.5, correlation length 12 cells), T = 0 on x = 0, uniform
-    source.  Step = assemble FP32 hi/lo + mixed Jacobi-PCG to a certified 1e-8:
-    FP32 matrix values in the SpMV, FP64 vectors, dots and reliable updates
-    (FP32 vectors cost this system 55% more iterations).  The learned
-    correction works on 3x3 blocks only, so this workload runs without it."""
+    source.  The scalar learned correction is trained first on seeded
+    hex_train_n^3 samples of the same statistics (HexSampler), then applied
+    at full size.  Step = assemble FP32 hi/lo + network + learned PCG to a
+    certified 1e-8: FP32 matrix values in the SpMV, FP64 vectors, dots and
+    reliable updates (FP32 vectors cost this system 55% more iterations).
+    Mixed Jacobi-PCG (the same step without the network), FP64 Jacobi-PCG and
+    plain FP64 CG are timed beside it."""
     import torch
     import simit_amd as S
-    from simit_amd import meshes
+    from simit_amd import meshes, learned
     stream = torch.cuda.current_stream()
     n = args.hex_n
     t0 = time.time()
@@ -376,13 +381,24 @@ def run_hex(args, ctx):
                           "spmv_us": spmv_ms * 1e3, "spmv_GBps": spmv_gbs}))
         return
 
+    # ---- the learned correction: training on hex_train_n^3 samples -------------------------------
+    net = ctx.gnn(seed=0, kappa=0.9)
+    smp = learned.HexSampler(ctx, args.hex_train_n, std_log10=1.5, corr_len=12.0)
+    tr = learned.train(ctx, net, nsamples=args.hex_train_samples, log_every=250, log=log,
+                       sampler=smp)
+    del smp
+    log(f"[bench] trained on {tr['steps']} samples of {args.hex_train_n}^3 in "
+        f"{tr['seconds']:.1f}s")
+
     def opts(**kw):
         return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
 
     def step():
         assemble()
+        linv, corr = net.precond(A2)
         x = torch.zeros(nv, dtype=torch.float64, device="cuda")
-        return ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=2))
+        return ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=2, learned=linv,
+                                       learned_corr=corr))
 
     for w in range(args.warmup):
         st = step()
@@ -392,7 +408,25 @@ def run_hex(args, ctx):
     stats = [step().as_dict() for _ in range(args.steps)]
     torch.cuda.synchronize()
     step_s = (time.perf_counter() - t0) / args.steps
-    log(f"[bench] step {step_s:.3f}s iters={stats[-1]['iters']} true_rel={stats[-1]['true_rel']:.2e}")
+    log(f"[bench] step {step_s:.3f}s iters={stats[-1]['iters']} ru={stats[-1]['reliable_updates']} "
+        f"true_rel={stats[-1]['true_rel']:.2e}")
+    pre_ms = timed(lambda: net.precond(A2), 3)
+
+    def jac_mixed():
+        assemble()
+        x = torch.zeros(nv, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        st = ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=2))
+        torch.cuda.synchronize()
+        r = {"seconds": round(time.perf_counter() - t, 3), "iters": st.iters,
+             "reliable_updates": st.reliable_updates, "true_rel": st.true_rel,
+             "converged": st.status == 0 and st.true_rel <= 1e-8}
+        log(f"[bench] Jacobi PCG mixed (assembly + solve): {r}")
+        return r
+    jac_mixed()
+    jacm = jac_mixed()
+    net.close()
     v64 = torch.empty(nnz, dtype=torch.float64, device="cuda")
     A64 = idx.bsr(1, S.F64, v64)
 
@@ -439,16 +473,20 @@ def run_hex(args, ctx):
         "metric": METRIC, "value": round(step_s, 4), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 matrix values, fp64 vectors+dots+refinement",
+        "dtype": "fp32 matrix values, fp64 vectors+dots+refinement, bf16 network",
         "data": "synthetic (seeded Gaussian random field conductivity, generated on the host)",
         "config": {"workload": f"trilinear hex Poisson {n}^3 cells, {nv} unknowns "
-                               f"({int(prob['dirichlet'].sum())} Dirichlet), Jacobi-PCG with FP32 "
-                               f"matrix values and FP64 vectors to a 1e-8 FP64 relative residual "
-                               f"(no learned correction: 3x3 blocks only)", "nverts": nv,
-                   "nhex": nh, "nnz": nnz},
+                               f"({int(prob['dirichlet'].sum())} Dirichlet), learned PCG with FP32 "
+                               f"matrix values and FP64 vectors to a 1e-8 FP64 relative residual",
+                   "nverts": nv, "nhex": nh, "nnz": nnz,
+                   "train": f"{args.hex_train_samples} seeded {args.hex_train_n}^3 samples"},
         "iterations": stats[-1]["iters"], "true_rel": stats[-1]["true_rel"],
+        "reliable_updates": stats[-1]["reliable_updates"],
         "converged": all(s["status"] == 0 and s["true_rel"] <= 1e-8 for s in stats),
+        "jacobi_pcg_mixed": jacm,
+        "speedup_vs_jacobi_mixed": round(jacm["seconds"] / step_s, 3),
         "jacobi_pcg_f64": jac64, "plain_cg_f64": plain,
+        "gnn_train_seconds": round(tr["seconds"], 2), "gnn_precond_ms": round(pre_ms, 2),
         "speedup_vs_plain_cg_f64": (round(plain["seconds"] / step_s, 2)
                                     if plain and plain["converged"] else None),
         "index_build_ms": round(index_ms, 2), "assembly_ms": round(asm_ms, 3),

@@ -667,7 +667,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   int rc = SMH_OK;
   auto T0 = std::chrono::steady_clock::now();
   CGState hs{};
-  int nru = 0;
+  int nru = 0, batch = 0;
   EpiResidual<VT, BD> epi{};
 #define TRY(x)            \
   do {                    \
@@ -737,11 +737,17 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   }
   LCHECK();
   epi.init = 0;
+  // Iterations are queued in batches between host checks of the state.  A
+  // replacement pauses the system until the host runs it, and the paused
+  // rest of a batch still costs its launches, so the batch shrinks after a
+  // replacement and doubles back towards check_every after a clean batch.
+  batch = check;
   for (;;) {
     SMH_HIP(ctx, hipMemcpyAsync(&hs, w.S, sizeof(CGState), hipMemcpyDeviceToHost, ctx->stream));
     SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (hs.status == SMH_OK && (hs.want_ru || hs.claimed)) {
       ++nru;
+      batch = std::max(2, batch / 4);
       hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
                          (const double *)x, epi, w.part);
       if (learned)
@@ -756,7 +762,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
       continue;
     }
     if (hs.done || hs.status != SMH_OK) break;
-    for (int k = 0; k < check; ++k) {
+    for (int k = 0; k < batch; ++k) {
       hipLaunchKernelGGL((k_cg_spmv<IP, BD>), dim3(gt), dim3(tile_threads_host(BD)), 0, ctx->stream, T,
                          (const VT *)w.p, (VT *)w.q, (const CGState *)w.S,
                          (const float *)w.dlo, w.part);
@@ -776,6 +782,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
                          (const VT *)w.z, (const CGState *)w.S);
     }
     LCHECK();
+    batch = std::min(check, 2 * batch);
   }
   // certify: FP64 residual of the final x (r, z are scratch from here on)
   hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
