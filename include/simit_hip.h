@@ -315,14 +315,18 @@ int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch, const 
  * A message-passing network over the matrix graph (3 layers, width 32, BF16
  * weights and MFMA inputs, FP32 accumulation) predicts a symmetric correction
  * C to the block-Jacobi preconditioner on A's own sparsity pattern, in the
- * block-Jacobi-normalised space Ahat = Linv A Linv^T.  C is scaled so that
- * every row's off-diagonal block norms sum below kappa < 1, so I + C and the
- * preconditioner Linv^T (I + C) Linv are symmetric positive definite.
+ * block-Jacobi-normalised space Ahat = Linv A Linv^T.  C is exactly
+ * symmetric; each block is soft-saturated (|C_k|_F < 1) but not capped per
+ * row.  A spectral guard keeps I + C positive definite: a 32-step Lanczos
+ * estimate of lambda_min of the stored C, less 5% of the estimated spectral
+ * width, is held at or above -kappa (kappa in (0, 1)) by a global scale of C,
+ * so lambda_min(I + C) >= 1 - kappa and the preconditioner
+ * Linv^T (I + C) Linv is symmetric positive definite.
  * Training minimises the stochastic Frobenius loss
  *   L = sum (V - (I + C) Ahat V)^2 / (K * ndof)
  * over K Gaussian probe vectors V, with Adam, entirely in the library's own
- * forward and backward kernels.  A must be 3x3 blocks with a symmetric
- * pattern whose rows hold their diagonal block. */
+ * forward and backward kernels.  A must be 3x3 or 1x1 blocks with a
+ * symmetric pattern whose rows hold their diagonal block. */
 typedef struct smh_gnn smh_gnn;
 #define SMH_GNN_NPARAMS 18192
 int smh_gnn_create(smh_ctx *ctx, uint64_t seed, double kappa, smh_gnn **out);
@@ -336,6 +340,12 @@ int smh_gnn_set_params(smh_ctx *ctx, smh_gnn *g, const float *host);
  * [nnz*4] (1x1, float) in the layout of opts.learned_corr (pass both as
  * opts.learned / opts.learned_corr) */
 int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float *linv, uint8_t *corr);
+/* the spectral guard of the last smh_gnn_precond: the Lanczos estimate of
+ * lambda_min and the spectral width of the network's C (before scaling), the
+ * scale applied to C (1 when none was needed) and the Lanczos steps taken;
+ * any pointer may be NULL */
+int smh_gnn_precond_info(smh_gnn *g, double *lambda_min, double *spectral_width, double *scale,
+                         int32_t *steps);
 /* loss and gradient on A for nprobes probe vectors: probes = device
  * float[bdim*nrows*nprobes] (row-major [dof][probe]) or NULL to draw them
  * from seed; grads (host float[SMH_GNN_NPARAMS]) may be NULL */

@@ -19,10 +19,14 @@ Model (all sizes fixed: F = 16 input features, H = 32 hidden):
                   h_i = h_i + relu([h_i, a_i] W_u + b_u)
   decoder         R_k = [h_i, h_j, e_k] W_d + b_d  (first 9 of 16 outputs, a 3x3 block)
   output          S_k = (R_k + R_t(k)^T) / 2  (t(k): the block (j, i))
-                  C_k = kappa S_k / (max(deg_i, deg_j) sqrt(1 + ||S_k||_F^2)),  C_ii = 0
+                  C_k = S_k / sqrt(1 + ||S_k||_F^2),  C_ii = 0
                   B_k = Linv_i^T (I [k diagonal] + C_k) Linv_j
-  Each row of I + C has off-diagonal block norms summing below kappa < 1, so
-  I + C is SPD (block Gershgorin) and so is the preconditioner B.
+  C is exactly symmetric.  Positive definiteness of I + C is not built into
+  the output: the library's spectral guard (a Lanczos estimate of lambda_min
+  of the stored C, kept at or above -kappa by a global scale) enforces it at
+  inference; this oracle returns the unscaled C, which the tests compare with
+  the library's when its guard applied no scale, and against whose exact
+  lambda_min (scipy) the guard's estimate is checked.
   loss            L = sum (V - (I + C) Ahat V)^2 / (K * ndof) over K probe
                   vectors V: the stochastic Frobenius norm of I - (I + C) Ahat.
 
@@ -199,11 +203,9 @@ def output(R, g, ft, kappa):
     bd = ft.get("bd", 3)
     Rm = R.reshape(-1, bd, bd)
     S = 0.5 * (Rm + Rm[g["t"]].transpose(0, 2, 1))
-    deg = g["deg"]
-    md = np.maximum(np.maximum(deg[g["brow"]], deg[g["ci"]]), 1).astype(np.float64)
     nS2 = (S * S).sum(axis=(1, 2))
     phi = 1.0 / np.sqrt(1.0 + nS2)
-    c = kappa / md
+    c = np.ones_like(phi)
     C = (c * phi)[:, None, None] * S
     C[g["diag"]] = 0.0
     Bh = C.copy()

@@ -172,6 +172,8 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_gnn_get_params": (C.c_int, [vp, vp, vp]),
         "smh_gnn_set_params": (C.c_int, [vp, vp, vp]),
         "smh_gnn_precond": (C.c_int, [vp, vp, C.POINTER(Bsr), vp, vp]),
+        "smh_gnn_precond_info": (C.c_int, [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_int32)]),
         "smh_gnn_loss_grad": (C.c_int, [vp, vp, C.POINTER(Bsr), i32, vp, u64,
                                         C.POINTER(C.c_double), vp]),
         "smh_gnn_train_step": (C.c_int, [vp, vp, C.POINTER(Bsr), i32, u64, C.c_double,
@@ -363,6 +365,16 @@ class Gnn:
         self.ctx._check(self.ctx.lib.smh_gnn_precond(self.ctx.h, self.h, C.byref(A),
                                                     _ptr(linv), _ptr(corr)))
         return linv, corr
+
+    def precond_info(self) -> dict:
+        """The spectral guard of the last precond(): Lanczos estimate of
+        lambda_min and the spectral width of the network's C, the scale
+        applied to C and the Lanczos steps."""
+        lm, w, sc, st = C.c_double(), C.c_double(), C.c_double(), C.c_int32()
+        self.ctx._check(self.ctx.lib.smh_gnn_precond_info(self.h, C.byref(lm), C.byref(w),
+                                                         C.byref(sc), C.byref(st)))
+        return {"lambda_min": lm.value, "spectral_width": w.value, "scale": sc.value,
+                "steps": st.value}
 
     def loss_grad(self, A: Bsr, nprobes: int = 4, probes=None, seed: int = 0, grads=True):
         import numpy as np

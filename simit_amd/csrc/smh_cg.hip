@@ -140,17 +140,6 @@ __global__ void k_bsrmv_exact(uint32_t nrows, int bd, const uint32_t *__restrict
 }
 #pragma clang fp contract(fast)
 
-template <typename T>
-struct XPlain {
-  const T *x;
-  int bd;
-  __device__ T operator()(uint32_t j, int b) const { return x[(uint64_t)j * bd + b]; }
-  // x_j (BD entries) as one vector load
-  template <int BD>
-  __device__ void row(uint32_t j, T *out) const {
-    load_row<T, BD, false>(x + (uint64_t)j * BD, out);
-  }
-};
 
 template <typename T, int BD>
 struct EpiStore {
@@ -609,23 +598,7 @@ __global__ void __launch_bounds__(256) k_sum_to(const double *part, int np, doub
 }
 
 // the CG's per-workgroup partial sums: two per workgroup of a tile kernel
-constexpr int kMaxTileGrid = 16384;
 constexpr int kPartSlots = 2 * kMaxTileGrid;
-
-TileDev tile_dev(const smh_bsr *A, const smh_tile_plan *p, unsigned *grid) {
-  // persistent workgroups (16 per CU on 256 CUs)
-#ifndef SMH_TILE_GRID
-#define SMH_TILE_GRID 4096
-#endif
-  static_assert(SMH_TILE_GRID <= kMaxTileGrid, "tile grid above the partial-sum slots");
-  uint64_t g = std::min<uint64_t>(p->ntiles, SMH_TILE_GRID);
-  if (g >= 8) g &= ~7ull;
-  if (g < 1) g = 1;
-  *grid = (unsigned)g;
-  TileDev d{A->nrows, A->nnz, A->rowptr, A->colidx, A->vals, A->vals_lo, p->tile_row, p->ntiles,
-            (uint32_t)((p->ntiles + g - 1) / g)};
-  return d;
-}
 
 // ---- host driver ---------------------------------------------------------------
 struct Work {

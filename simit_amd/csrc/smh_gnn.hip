@@ -10,10 +10,14 @@
 //   3 x { m_k = relu([h_i h_j e_k] W_m + b_m);  a_i = mean_{k in row i} m_k;
 //         h_i += relu([h_i a_i] W_u + b_u) }
 //   R_k = [h_i h_j e_k] W_d + b_d (first 9 outputs, a 3x3 block)
-//   S_k = (R_k + R_t(k)^T)/2,  C_k = kappa S_k / (max(deg_i, deg_j) sqrt(1 + |S_k|_F^2))
+//   S_k = (R_k + R_t(k)^T)/2,  C_k = s S_k / sqrt(1 + |S_k|_F^2)
 //   preconditioner z_i = Linv_i^T (Linv_i r_i + sum_{k in row i, k off-diagonal} C_k Linv_j r_j)
-// where t(k) is the position of block (j, i).  Every row of I + C has
-// off-diagonal block norms summing below kappa < 1, so I + C is SPD.
+// where t(k) is the position of block (j, i).  C is exactly symmetric; I + C
+// is kept positive definite by the spectral guard: a Lanczos estimate of
+// lambda_min of the stored C (with a margin of 5% of the estimated spectral
+// width) and the global scale s <= 1 that holds it at or above -kappa, so
+// lambda_min(I + C) >= 1 - kappa.  No per-entry cap: the blocks are free to
+// approach the least-squares optimum on the pattern of A.
 //
 // Layout in HBM: per-block arrays follow the CSR order of the pattern; h is
 // FP32 [nrows][32]; e and m are BF16 [nnz][32]; weights are FP32 masters
@@ -470,11 +474,10 @@ k_gnn_decode(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *__
 // output); R is [nnz][9]
 template <int B>
 struct OutScale {
-  float S[B * B], phi, c;
+  float S[B * B], phi;
 };
 template <int B>
-__device__ __forceinline__ OutScale<B> out_scale(const float *R, uint64_t k, uint64_t tk,
-                                                 uint32_t di, uint32_t dj, float kappa) {
+__device__ __forceinline__ OutScale<B> out_scale(const float *R, uint64_t k, uint64_t tk) {
   OutScale<B> o;
   float n2 = 0.0f;
   for (int a = 0; a < B; ++a)
@@ -484,21 +487,18 @@ __device__ __forceinline__ OutScale<B> out_scale(const float *R, uint64_t k, uin
       n2 += s * s;
     }
   o.phi = 1.0f / sqrtf(1.0f + n2);
-  const uint32_t md = max(max(di, dj), 1u);
-  o.c = kappa / (float)md;
   return o;
 }
 
-// C_k (the block of the lower position of each pair is the transpose of the
-// upper one's, so the stored correction is exactly symmetric).  Storage for
-// the solver: B = 3, FP8 of 2^5 C, three values and a zero pad byte per block
-// row (corr); B = 1, FP32 (corr_f).
+// C_k = sc S_k phi_k (the block of the lower position of each pair is the
+// transpose of the upper one's, so the stored correction is exactly
+// symmetric).  Storage for the solver: B = 3, FP8 of 2^5 C, three values and
+// a zero pad byte per block row (corr); B = 1, FP32 (corr_f).
 template <int B>
 __global__ void k_gnn_out(uint64_t nnz, const uint32_t *__restrict__ brow,
                           const uint32_t *__restrict__ ci, const uint32_t *__restrict__ t,
-                          const uint32_t *__restrict__ deg, const float *__restrict__ R,
-                          float kappa, uint32_t *__restrict__ corr, float *__restrict__ corr_f,
-                          float *__restrict__ c32) {
+                          const float *__restrict__ R, float sc, uint32_t *__restrict__ corr,
+                          float *__restrict__ corr_f, float *__restrict__ c32) {
   constexpr int BB = B * B;
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
@@ -510,8 +510,8 @@ __global__ void k_gnn_out(uint64_t nnz, const uint32_t *__restrict__ brow,
     const uint64_t tk = t[k];
     const bool lower = tk < k;
     const uint64_t own = lower ? tk : k, oth = lower ? k : tk;
-    const OutScale<B> o = out_scale<B>(R, own, oth, deg[brow[own]], deg[ci[own]], kappa);
-    const float f = o.c * o.phi;
+    const OutScale<B> o = out_scale<B>(R, own, oth);
+    const float f = sc * o.phi;
     for (int a = 0; a < B; ++a)
       for (int b = 0; b < B; ++b) C[B * a + b] = f * (lower ? o.S[B * b + a] : o.S[B * a + b]);
   }
@@ -601,9 +601,8 @@ k_gnn_loss(uint32_t n, int K, const uint32_t *__restrict__ rp, const uint32_t *_
 // dR_k = (dS_k + dS_t(k)^T) / 2 with dS from dC = G W^T (per block)
 template <int B>
 __device__ __forceinline__ void dS_of(uint64_t k, const uint32_t *brow, const uint32_t *ci,
-                                      const uint32_t *t, const uint32_t *deg, const float *R,
-                                      const float *G, const float *W, int K, float kappa,
-                                      float *dS) {
+                                      const uint32_t *t, const float *R, const float *G,
+                                      const float *W, int K, float *dS) {
   constexpr int BB = B * B;
   const uint32_t i = brow[k], j = ci[k];
   if (i == j) {
@@ -618,10 +617,10 @@ __device__ __forceinline__ void dS_of(uint64_t k, const uint32_t *brow, const ui
         s += G[((uint64_t)B * i + a) * K + p] * W[((uint64_t)B * j + b) * K + p];
       dC[B * a + b] = s;
     }
-  const OutScale<B> o = out_scale<B>(R, k, t[k], deg[i], deg[j], kappa);
+  const OutScale<B> o = out_scale<B>(R, k, t[k]);
   float inner = 0.0f;
   for (int q = 0; q < BB; ++q) inner += dC[q] * o.S[q];
-  const float f = o.c * o.phi, g = o.c * o.phi * o.phi * o.phi * inner;
+  const float f = o.phi, g = o.phi * o.phi * o.phi * inner;
   for (int q = 0; q < BB; ++q) dS[q] = f * dC[q] - g * o.S[q];
 }
 
@@ -629,14 +628,13 @@ __device__ __forceinline__ void dS_of(uint64_t k, const uint32_t *brow, const ui
 template <int B>
 __global__ void k_gnn_out_bwd(uint64_t nnz, const uint32_t *__restrict__ brow,
                               const uint32_t *__restrict__ ci, const uint32_t *__restrict__ t,
-                              const uint32_t *__restrict__ deg, const float *__restrict__ R,
-                              const float *__restrict__ G, const float *__restrict__ W, int K,
-                              float kappa, float *__restrict__ dR) {
+                              const float *__restrict__ R, const float *__restrict__ G,
+                              const float *__restrict__ W, int K, float *__restrict__ dR) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
   float d1[B * B], d2[B * B];
-  dS_of<B>(k, brow, ci, t, deg, R, G, W, K, kappa, d1);
-  dS_of<B>(t[k], brow, ci, t, deg, R, G, W, K, kappa, d2);
+  dS_of<B>(k, brow, ci, t, R, G, W, K, d1);
+  dS_of<B>(t[k], brow, ci, t, R, G, W, K, d2);
   for (int q = 0; q < 16; ++q) dR[k * 16 + q] = 0.0f;
   for (int a = 0; a < B; ++a)
     for (int b = 0; b < B; ++b)
@@ -853,11 +851,17 @@ struct smh_gnn {
         *de = nullptr, *dz = nullptr, *dU = nullptr, *da = nullptr, *dXu = nullptr,
         *wpart = nullptr;
   double *lpart = nullptr, *loss = nullptr;
+  // spectral guard (Lanczos on the stored correction)
+  float *lz[3] = {};
+  double *lz_part = nullptr, *lz_ab = nullptr;
+  double lz_theta = 0.0, lz_width = 0.0, lz_scale = 1.0;
+  int lz_steps = 0;
 };
 
 namespace {
 
 constexpr int kWgradParts = 256;
+constexpr int kLzSteps = 32;  // Lanczos steps of the spectral guard
 
 void free_work(smh_gnn *g) {
   for (void *p : g->bufs) dfree(p);
@@ -899,6 +903,9 @@ int ensure_work(smh_ctx *ctx, smh_gnn *g, uint32_t n, uint64_t nnz, int K, bool 
   WA(g->fe, E * kF);
   WA(g->fv, N * kF);
   WA(g->e, E * kH);
+  for (int q = 0; q < 3; ++q) WA(g->lz[q], N * B);
+  WA(g->lz_part, kMaxTileGrid);
+  WA(g->lz_ab, 2 * kLzSteps + 2);
   const int nh = train ? kLayers + 1 : 2;
   for (int l = 0; l < nh; ++l) WA(g->h[l], N * kH);
   const int nm = train ? kLayers : 1;
@@ -1052,8 +1059,8 @@ int loss_grad(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int K, const float *pr
 #define SMH_LOSS(B)                                                                            \
   do {                                                                                         \
     hipLaunchKernelGGL(k_gnn_out<B>, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz, g->brow,  \
-                       A->colidx, g->tmap, g->deg, g->R, g->kappa, (uint32_t *)nullptr,        \
-                       (float *)nullptr, g->c32);                                              \
+                       A->colidx, g->tmap, g->R, 1.0f, (uint32_t *)nullptr, (float *)nullptr,  \
+                       g->c32);                                                                \
     hipLaunchKernelGGL(k_gnn_ahat_apply<B>, dim3(grid_for(n, 128)), dim3(128), 0, s, n, K,     \
                        A->rowptr, A->colidx, g->ahat, V, g->W);                                \
     hipLaunchKernelGGL(k_gnn_loss<B>, dim3(nl), dim3(256), 0, s, n, K, A->rowptr, A->colidx,   \
@@ -1061,8 +1068,7 @@ int loss_grad(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int K, const float *pr
     hipLaunchKernelGGL(k_gnn_sum, dim3(1), dim3(256), 0, s, (int)nl, (const double *)g->lpart, \
                        g->loss);                                                               \
     hipLaunchKernelGGL(k_gnn_out_bwd<B>, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz,       \
-                       g->brow, A->colidx, g->tmap, g->deg, g->R, g->G, g->W, K, g->kappa,     \
-                       g->dR);                                                                 \
+                       g->brow, A->colidx, g->tmap, g->R, g->G, g->W, K, g->dR);               \
   } while (0)
   if (B == 1) SMH_LOSS(1);
   else SMH_LOSS(3);
@@ -1186,6 +1192,216 @@ extern "C" int smh_gnn_set_params(smh_ctx *ctx, smh_gnn *g, const float *host) {
   return refresh_wt(ctx, g);
 }
 
+// ---- spectral guard: Lanczos estimate of lambda_min(C) for the stored correction ---------------
+namespace {
+
+// y = C v over the pattern (the stored C, zero on the diagonal) and the
+// workgroup partials of v'y
+template <int BD>
+struct EpiCmv {
+  const float *v;
+  float *y;
+  double d;
+  __device__ void operator()(uint32_t row, const float *s) {
+    for (int a = 0; a < BD; ++a) {
+      const uint64_t o = (uint64_t)row * BD + a;
+      y[o] = s[a];
+      d += (double)s[a] * (double)v[o];
+    }
+  }
+};
+
+template <int LP, int BD>
+__global__ void __launch_bounds__(TileShape<BD>::kThreads)
+k_gnn_cmv(TileDev C, const float *__restrict__ v, float *__restrict__ y,
+          double *__restrict__ part) {
+  __shared__ TileSmem<LP, BD> sm;
+  uint32_t k0, k1;
+  wg_tiles(C, k0, k1);
+  XPlain<float> xg{v, BD};
+  EpiCmv<BD> epi{v, y, 0.0};
+  spmv_tiles<LP, BD>(C, k0, k1, sm, xg, epi);
+  const double t = block_sum<TileShape<BD>::kWaves>(epi.d, sm.red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// ab[j] = the sum of np partials (root: its square root)
+__global__ void __launch_bounds__(256) k_lz_sum(const double *part, int np, double *ab, int j,
+                                                int root) {
+  __shared__ double s4[4];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) acc += part[i];
+  const double t = block_sum256(acc, s4);
+  if (threadIdx.x == 0) ab[j] = root ? sqrt(t) : t;
+}
+
+// y = y - ab[ja] v - ab[jb] vp (vp optional) and the partials of |y|^2
+__global__ void __launch_bounds__(256) k_lz_orth(uint64_t n, float *__restrict__ y,
+                                                 const float *__restrict__ v,
+                                                 const float *__restrict__ vp,
+                                                 const double *__restrict__ ab, int ja, int jb,
+                                                 double *__restrict__ part) {
+  __shared__ double s4[4];
+  const double al = ja >= 0 ? ab[ja] : 0.0, be = (vp && jb >= 0) ? ab[jb] : 0.0;
+  double acc = 0.0;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    double w = (double)y[i];
+    if (v) w -= al * (double)v[i];
+    if (vp) w -= be * (double)vp[i];
+    y[i] = (float)w;
+    acc += w * w;
+  }
+  const double t = block_sum256(acc, s4);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// y = y / ab[j] (0 when ab[j] is not positive)
+__global__ void __launch_bounds__(256) k_lz_scale(uint64_t n, float *__restrict__ y,
+                                                  const double *__restrict__ ab, int j) {
+  const double b = ab[j];
+  const float inv = b > 0.0 && b < INFINITY ? (float)(1.0 / b) : 0.0f;
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256)
+    y[i] *= inv;
+}
+
+// number of eigenvalues below x of the symmetric tridiagonal matrix with
+// diagonal a[0..L) and off-diagonal b[0..L-1) (Sturm sequence of the LDL^T pivots)
+int sturm_below(const double *a, const double *b, int L, double x) {
+  int c = 0;
+  double d = 1.0;
+  for (int i = 0; i < L; ++i) {
+    const double bb = i ? b[i - 1] * b[i - 1] : 0.0;
+    d = a[i] - x - (i ? bb / d : 0.0);
+    if (d == 0.0) d = -1e-300;
+    if (d < 0.0) ++c;
+  }
+  return c;
+}
+
+// the k-th smallest eigenvalue (k = 1..L) of the tridiagonal matrix, by bisection
+double tri_eig(const double *a, const double *b, int L, int k) {
+  double lo = INFINITY, hi = -INFINITY;
+  for (int i = 0; i < L; ++i) {
+    const double r = (i ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < L ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  for (int it = 0; it < 200 && hi - lo > 1e-13 * (std::fabs(lo) + std::fabs(hi) + 1e-300); ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (sturm_below(a, b, L, mid) >= k) hi = mid;
+    else lo = mid;
+  }
+  return 0.5 * (lo + hi);
+}
+
+// kLzSteps Lanczos steps on the stored correction (FP32 vectors, no host
+// synchronisation inside); returns the extreme Ritz values
+template <int LP, int BD>
+int lanczos(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, const void *corr, double *theta_min,
+            double *theta_max, int *steps) {
+  smh_tile_plan plan;
+  int rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(BD), &plan);
+  if (rc) return rc;
+  smh_bsr Cb = *A;
+  Cb.vals = corr;
+  Cb.vals_lo = nullptr;
+  unsigned gc = 0;
+  const TileDev T = tile_dev(&Cb, &plan, &gc);
+  const uint64_t n = (uint64_t)A->nrows * BD;
+  const unsigned gv = grid_for(n, 256, 2048);
+  const int m = (int)std::min<uint64_t>(kLzSteps, n);
+  hipStream_t s = ctx->stream;
+  float *vp = g->lz[0], *v = g->lz[1], *y = g->lz[2];
+  double *ab = g->lz_ab, *part = g->lz_part;
+  // start: a seeded Gaussian vector, normalised (ab[2m] = its norm)
+  hipLaunchKernelGGL(k_gnn_probes, dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                     (uint64_t)0x5EED0000ull + A->nrows, v);
+  hipLaunchKernelGGL(k_lz_orth, dim3(gv), dim3(256), 0, s, n, v, (const float *)nullptr,
+                     (const float *)nullptr, (const double *)ab, -1, -1, part);
+  hipLaunchKernelGGL(k_lz_sum, dim3(1), dim3(256), 0, s, (const double *)part, (int)gv, ab,
+                     2 * kLzSteps, 1);
+  hipLaunchKernelGGL(k_lz_scale, dim3(gv), dim3(256), 0, s, n, v, (const double *)ab,
+                     2 * kLzSteps);
+  for (int j = 0; j < m; ++j) {
+    // alpha_j = v' C v, then w = C v - alpha_j v - beta_{j-1} v_prev, beta_j = |w|
+    hipLaunchKernelGGL((k_gnn_cmv<LP, BD>), dim3(gc), dim3(tile_threads_host(BD)), 0, s, T,
+                       (const float *)v, y, part);
+    hipLaunchKernelGGL(k_lz_sum, dim3(1), dim3(256), 0, s, (const double *)part, (int)gc, ab, j,
+                       0);
+    hipLaunchKernelGGL(k_lz_orth, dim3(gv), dim3(256), 0, s, n, y, (const float *)v,
+                       (const float *)(j ? vp : nullptr), (const double *)ab, j,
+                       kLzSteps + j - 1, part);
+    hipLaunchKernelGGL(k_lz_sum, dim3(1), dim3(256), 0, s, (const double *)part, (int)gv, ab,
+                       kLzSteps + j, 1);
+    hipLaunchKernelGGL(k_lz_scale, dim3(gv), dim3(256), 0, s, n, y, (const double *)ab,
+                       kLzSteps + j);
+    float *t = vp;
+    vp = v;
+    v = y;
+    y = t;
+  }
+  SMH_LAUNCH_CHECK(ctx);
+  double h[2 * kLzSteps];
+  SMH_HIP(ctx, hipMemcpyAsync(h, ab, sizeof(double) * 2 * kLzSteps, hipMemcpyDeviceToHost, s));
+  SMH_HIP(ctx, hipStreamSynchronize(s));
+  const double *al = h, *be = h + kLzSteps;
+  // the Krylov space ends at the first (numerically) vanishing beta
+  double scale = 0.0;
+  int L = 0;
+  for (int j = 0; j < m; ++j) {
+    SMH_CHECK(ctx, std::isfinite(al[j]) && std::isfinite(be[j]), SMH_ERR_BREAKDOWN,
+              "smh_gnn_precond: non-finite Lanczos coefficient (step %d)", j);
+    scale = std::max(scale, std::max(std::fabs(al[j]), be[j]));
+    L = j + 1;
+    if (!(be[j] > 1e-6 * scale)) break;
+  }
+  *theta_min = tri_eig(al, be, L, 1);
+  *theta_max = tri_eig(al, be, L, L);
+  *steps = L;
+  return SMH_OK;
+}
+
+// Estimate lambda_min(C) of the stored correction and, when lambda_min less
+// 5% of the spectral width falls below -kappa, re-emit C scaled by
+// s = 0.97 kappa / (that margin-adjusted estimate) and check again (the FP8
+// rounding of the scaled C moves the estimate slightly).
+int spectral_guard(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, void *corr) {
+  const int B = (int)A->bdim;
+  double sc = 1.0;
+  for (int round = 0; round < 4; ++round) {
+    double tmin = 0.0, tmax = 0.0;
+    int L = 0;
+    const int rc = B == 3 ? lanczos<kFP8F, 3>(ctx, g, A, corr, &tmin, &tmax, &L)
+                          : lanczos<SMH_F32, 1>(ctx, g, A, corr, &tmin, &tmax, &L);
+    if (rc) return rc;
+    if (round == 0) {
+      g->lz_theta = tmin;
+      g->lz_width = tmax - tmin;
+      g->lz_steps = L;
+    }
+    const double guarded = tmin - 0.05 * (tmax - tmin);
+    if (guarded >= -(double)g->kappa) break;
+    SMH_CHECK(ctx, round < 3, SMH_ERR_BREAKDOWN,
+              "smh_gnn_precond: the spectral guard did not settle (lambda_min %g)", tmin);
+    sc *= 0.97 * (double)g->kappa / -guarded;  // aim inside the bound, so the check settles
+    if (B == 3)
+      hipLaunchKernelGGL(k_gnn_out<3>, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
+                         A->nnz, g->brow, A->colidx, g->tmap, g->R, (float)sc, (uint32_t *)corr,
+                         (float *)nullptr, (float *)nullptr);
+    else
+      hipLaunchKernelGGL(k_gnn_out<1>, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
+                         A->nnz, g->brow, A->colidx, g->tmap, g->R, (float)sc,
+                         (uint32_t *)nullptr, (float *)corr, (float *)nullptr);
+    SMH_LAUNCH_CHECK(ctx);
+  }
+  g->lz_scale = sc;
+  return SMH_OK;
+}
+
+}  // namespace
+
 extern "C" int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float *linv,
                                uint8_t *corr) {
   if (!ctx) return SMH_ERR_INVALID;
@@ -1200,15 +1416,26 @@ extern "C" int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float
   if ((rc = forward(ctx, g, A, false))) return rc;
   if (B == 3)
     hipLaunchKernelGGL(k_gnn_out<3>, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
-                       A->nnz, g->brow, A->colidx, g->tmap, g->deg, g->R, g->kappa,
-                       (uint32_t *)corr, (float *)nullptr, (float *)nullptr);
+                       A->nnz, g->brow, A->colidx, g->tmap, g->R, 1.0f, (uint32_t *)corr,
+                       (float *)nullptr, (float *)nullptr);
   else
     hipLaunchKernelGGL(k_gnn_out<1>, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
-                       A->nnz, g->brow, A->colidx, g->tmap, g->deg, g->R, g->kappa,
-                       (uint32_t *)nullptr, (float *)corr, (float *)nullptr);
+                       A->nnz, g->brow, A->colidx, g->tmap, g->R, 1.0f, (uint32_t *)nullptr,
+                       (float *)corr, (float *)nullptr);
+  SMH_LAUNCH_CHECK(ctx);
+  if ((rc = spectral_guard(ctx, g, A, corr))) return rc;
   SMH_HIP(ctx, hipMemcpyAsync(linv, g->linv, sizeof(float) * B * B * (size_t)A->nrows,
                               hipMemcpyDeviceToDevice, ctx->stream));
-  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
+}
+
+extern "C" int smh_gnn_precond_info(smh_gnn *g, double *lambda_min, double *spectral_width,
+                                    double *scale, int32_t *steps) {
+  if (!g) return SMH_ERR_INVALID;
+  if (lambda_min) *lambda_min = g->lz_theta;
+  if (spectral_width) *spectral_width = g->lz_width;
+  if (scale) *scale = g->lz_scale;
+  if (steps) *steps = g->lz_steps;
   return SMH_OK;
 }
 

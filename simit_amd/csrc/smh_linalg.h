@@ -269,6 +269,37 @@ struct TileSmem {
   double red[TileShape<BD>::kWaves];
 };
 
+// Gather of x for the tile loop: x_j as BD consecutive values.
+template <typename T>
+struct XPlain {
+  const T *x;
+  int bd;
+  __device__ T operator()(uint32_t j, int b) const { return x[(uint64_t)j * bd + b]; }
+  // x_j (BD entries) as one vector load
+  template <int BD>
+  __device__ void row(uint32_t j, T *out) const {
+    load_row<T, BD, false>(x + (uint64_t)j * BD, out);
+  }
+};
+
+// Tile kernels run persistent workgroups (16 per CU on 256 CUs by default),
+// each over a contiguous range of tiles; at most kMaxTileGrid of them.
+constexpr int kMaxTileGrid = 16384;
+#ifndef SMH_TILE_GRID
+#define SMH_TILE_GRID 4096
+#endif
+static_assert(SMH_TILE_GRID <= kMaxTileGrid, "tile grid above kMaxTileGrid");
+
+inline TileDev tile_dev(const smh_bsr *A, const smh_tile_plan *p, unsigned *grid) {
+  uint64_t g = p->ntiles < (uint64_t)SMH_TILE_GRID ? p->ntiles : (uint64_t)SMH_TILE_GRID;
+  if (g >= 8) g &= ~7ull;
+  if (g < 1) g = 1;
+  *grid = (unsigned)g;
+  TileDev d{A->nrows, A->nnz, A->rowptr, A->colidx, A->vals, A->vals_lo, p->tile_row, p->ntiles,
+            (uint32_t)((p->ntiles + g - 1) / g)};
+  return d;
+}
+
 // Map workgroup g to its tile range (XCD-contiguous when the grid is a
 // multiple of 8).
 __device__ __forceinline__ void wg_tiles(const TileDev &A, uint32_t &k0, uint32_t &k1) {

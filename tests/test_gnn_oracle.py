@@ -64,9 +64,11 @@ def test_normalisation_and_block_jacobi_at_zero_output(oracle):
 
 
 @pytest.mark.parametrize("scale", [1.0, 30.0])
-def test_preconditioner_symmetric_positive_definite(oracle, scale):
-    # large weights push the raw output far out; the scaling still keeps
-    # I + C inside the block Gershgorin bound
+def test_preconditioner_symmetric_and_congruent(oracle, scale):
+    # large weights push the raw output far out: every block stays inside the
+    # unit Frobenius ball, C and B are exactly symmetric, and B = L^T (I + C) L
+    # is congruent to I + C (same inertia), so B is SPD exactly when I + C is,
+    # which the library's spectral guard enforces
     rp, ci, vals = small_system(oracle, n=3, seed=4)
     n = rp.size - 1
     B, aux = G.precond_blocks(random_params(2, scale), rp, ci, vals, kappa=0.9)
@@ -74,14 +76,16 @@ def test_preconditioner_symmetric_positive_definite(oracle, scale):
     Bm = B.reshape(-1, 3, 3)
     assert np.array_equal(Bm, Bm[g["t"]].transpose(0, 2, 1))
     C = aux["C"]
-    rowsum = np.zeros(n)
-    np.add.at(rowsum, g["brow"], np.sqrt((C * C).sum(axis=(1, 2))))
-    assert rowsum.max() <= 0.9 * (1 + 1e-12)
-    Ch = dense(rp, ci, C, n) + np.eye(3 * n)
-    assert np.linalg.eigvalsh(0.5 * (Ch + Ch.T)).min() > 0.1 - 1e-12
+    assert np.sqrt((C * C).sum(axis=(1, 2))).max() <= 1.0 + 1e-12
+    Cd = dense(rp, ci, C, n)
+    np.testing.assert_allclose(Cd, Cd.T, rtol=0, atol=1e-15)
     Bd = dense(rp, ci, B, n)
     assert np.array_equal(Bd, Bd.T)
-    assert np.linalg.eigvalsh(Bd).min() > 0
+    ec = np.linalg.eigvalsh(np.eye(3 * n) + Cd)
+    eb = np.linalg.eigvalsh(Bd)
+    tol = 1e-12 * np.abs(eb).max()
+    assert (ec < -1e-12).sum() == (eb < -tol).sum()
+    assert (ec > 1e-12).sum() == (eb > tol).sum()
 
 
 def test_loss_matches_dense_formula(oracle):
@@ -153,9 +157,7 @@ def test_gradients_match_torch_autograd(oracle):
         h = h + torch.relu(torch.cat([h, a], 1) @ T[f"W_u{l}"] + T[f"b_u{l}"])
     R = (torch.cat([h[brow], h[col], e], 1) @ T["W_d"] + T["b_d"])[:, :9].reshape(-1, 3, 3)
     S = 0.5 * (R + R[t].transpose(1, 2))
-    deg = torch.tensor(g["deg"], dtype=torch.float64)
-    md = torch.clamp(torch.maximum(deg[brow], deg[col]), min=1.0)
-    C = (kappa / md / torch.sqrt(1 + (S * S).sum((1, 2))))[:, None, None] * S
+    C = (1.0 / torch.sqrt(1 + (S * S).sum((1, 2))))[:, None, None] * S
     C = C * torch.tensor(~g["diag"], dtype=torch.float64)[:, None, None]
     Ah = torch.tensor(ft["Ah"])
     V3 = torch.tensor(V.reshape(n, 3, -1))
@@ -216,18 +218,22 @@ def test_scalar_normalisation_and_jacobi_at_zero_output(oracle):
     np.testing.assert_allclose(dense1(rp, ci, B, n), np.diag(1.0 / d), rtol=1e-13)
 
 
-def test_scalar_preconditioner_symmetric_positive_definite(oracle):
+def test_scalar_preconditioner_symmetric_and_congruent(oracle):
     rp, ci, vals = small_hex(oracle, n=3, seed=4)
     n = rp.size - 1
     B, aux = G.precond_blocks(random_params(2, 30.0), rp, ci, vals, kappa=0.9, bd=1)
     g = aux["g"]
     C = aux["C"].ravel()
-    rowsum = np.zeros(n)
-    np.add.at(rowsum, g["brow"], np.abs(C))
-    assert rowsum.max() <= 0.9 * (1 + 1e-12)
+    assert np.abs(C).max() <= 1.0 + 1e-12
+    assert np.array_equal(C, C[g["t"]])
     Bd = dense1(rp, ci, B, n)
     assert np.array_equal(Bd, Bd.T)
-    assert np.linalg.eigvalsh(Bd).min() > 0
+    Cd = dense1(rp, ci, aux["C"], n)
+    ec = np.linalg.eigvalsh(np.eye(n) + Cd)
+    eb = np.linalg.eigvalsh(Bd)
+    tol = 1e-12 * np.abs(eb).max()
+    assert (ec < -1e-12).sum() == (eb < -tol).sum()
+    assert (ec > 1e-12).sum() == (eb > tol).sum()
 
 
 def test_scalar_loss_and_gradients(oracle):

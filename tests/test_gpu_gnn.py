@@ -11,6 +11,7 @@ import ctypes as C
 
 import numpy as np
 import pytest
+import scipy.sparse as sp
 import torch
 
 import gnn as G
@@ -83,8 +84,9 @@ def test_forward_matches_oracle(ctx):
     Lo = aux["ft"]["Linv"].reshape(-1, 9)
     np.testing.assert_allclose(linv.cpu().numpy().reshape(-1, 9), Lo, rtol=2e-6,
                                atol=2e-6 * np.abs(Lo).max())
+    info = net.precond_info()
     Cg = fp8_blocks(corr)
-    Co = aux["C"]
+    Co = info["scale"] * aux["C"]  # the spectral guard's global scale (1: none applied)
     g = aux["g"]
     # exact symmetry of the stored correction, zero diagonal blocks
     assert np.array_equal(Cg, Cg[g["t"]].transpose(0, 2, 1))
@@ -98,10 +100,58 @@ def test_forward_matches_oracle(ctx):
     assert close.mean() > 0.95, close.mean()
     assert np.linalg.norm(Cg - Co) <= 5e-2 * np.linalg.norm(Co), np.linalg.norm(Cg - Co)
     assert np.abs(Cg - Co).max() <= 0.07 * scale
-    # Gershgorin bound of every row (kappa = 0.9) survives the FP8 storage
-    rows = np.zeros(rp.size - 1)
-    np.add.at(rows, g["brow"], np.sqrt((Cg * Cg).sum(axis=(1, 2))))
-    assert rows.max() < 0.9 * (1 + 2 ** -4)
+    check_guard(info, dense_blocks(rp, ci, fp8_round(aux["C"])), dense_blocks(rp, ci, Cg), 0.9)
+    net.close()
+
+
+def dense_blocks(rp, ci, blocks):
+    n = rp.size - 1
+    bd = int(round(np.sqrt(blocks[0].size)))
+    return sp.bsr_matrix((np.asarray(blocks).reshape(-1, bd, bd), ci.astype(np.int64),
+                          rp.astype(np.int64)), shape=(n * bd, n * bd)).toarray()
+
+
+def check_guard(info, C0, Cs, kappa):
+    """The spectral guard: its Lanczos estimate of lambda_min against the exact
+    spectrum of the network's unscaled C (C0, the oracle's on the storage
+    grid: the GPU's differs by BF16-chain noise), and the stored C (Cs) keeps
+    lambda_min(I + C) >= 1 - kappa."""
+    ev = np.linalg.eigvalsh(C0)
+    width = ev[-1] - ev[0]
+    assert abs(info["lambda_min"] - ev[0]) <= 0.03 * width + 0.03 * abs(ev[0]), (info, ev[0])
+    assert abs(info["spectral_width"] - width) <= 0.1 * width, (info, width)
+    assert 0 < info["scale"] <= 1.0 and info["steps"] >= 1
+    if ev[0] - 0.1 * width > -kappa:
+        assert info["scale"] == 1.0, info
+    lmin = np.linalg.eigvalsh(np.eye(Cs.shape[0]) + Cs)[0]
+    assert lmin >= 1.0 - kappa - 0.02, (lmin, info)
+
+
+@pytest.mark.parametrize("kappa", [0.9, 0.3])
+def test_spectral_guard_scales_an_indefinite_correction(ctx, kappa):
+    # large random weights saturate the blocks, so lambda_min(C) lies well
+    # below -kappa and the guard must scale C down
+    sy = tet_system(ctx, 4, seed=3)
+    idx = sy["idx"]
+    rp, ci = idx.to_host()
+    vals = sy["v64"].cpu().numpy()
+    net = ctx.gnn(seed=1, kappa=kappa)
+    flat = random_params(8, scale=4.0)
+    net.set_params(flat)
+    linv, corr = net.precond(idx.bsr(3, S.F64, sy["v64"]))
+    info = net.precond_info()
+    _, aux = G.precond_blocks(G.unflatten(flat.astype(np.float64)), rp, ci, vals, mimic=True)
+    C0 = dense_blocks(rp, ci, fp8_round(aux["C"]))
+    assert np.linalg.eigvalsh(C0)[0] < -kappa
+    assert info["scale"] < 1.0, info
+    Cs = dense_blocks(rp, ci, fp8_blocks(corr))
+    check_guard(info, C0, Cs, kappa)
+    # the guarded preconditioner is SPD: learned PCG converges
+    x = torch.zeros_like(sy["rhs"])
+    st = ctx.cg(idx.bsr(3, S.F64, sy["v64"]), sy["rhs"], x,
+                ctx.cg_options(maxiters=5000, tol=1e-8, rel=1, precond=S.PRECOND_LEARNED,
+                               learned=linv, learned_corr=corr))
+    assert st.status == 0 and st.true_rel <= 1e-8, st.as_dict()
     net.close()
 
 

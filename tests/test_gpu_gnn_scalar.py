@@ -14,6 +14,7 @@ import torch
 import gnn as G
 import simit_amd as S
 from simit_amd import meshes
+from test_gpu_gnn import check_guard, dense_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -65,16 +66,15 @@ def test_scalar_forward_matches_oracle(ctx):
                               mimic=True, bd=1)
     Lo = aux["ft"]["Linv"].ravel()
     np.testing.assert_allclose(linv.cpu().numpy(), Lo, rtol=2e-6)
-    Cg, Co, g = corr_values(corr), aux["C"].ravel(), aux["g"]
+    info = net.precond_info()
+    Cg, Co, g = corr_values(corr), info["scale"] * aux["C"].ravel(), aux["g"]
     assert np.array_equal(Cg, Cg[g["t"]])           # exactly symmetric
     assert np.all(Cg[g["diag"]] == 0.0)
     scale = np.abs(Co).max()
     assert scale > 1e-3
     assert np.linalg.norm(Cg - Co) <= 5e-2 * np.linalg.norm(Co), np.linalg.norm(Cg - Co)
     assert np.abs(Cg - Co).max() <= 0.07 * scale
-    rows = np.zeros(rp.size - 1)
-    np.add.at(rows, g["brow"], np.abs(Cg))
-    assert rows.max() < 0.9 * (1 + 1e-6)
+    check_guard(info, dense_blocks(rp, ci, aux["C"].ravel()), dense_blocks(rp, ci, Cg), 0.9)
     net.close()
 
 
