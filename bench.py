@@ -40,8 +40,8 @@ HBM_PEAK_GBS = 8000.0
 # summaries and commands in profiles/.
 # 96^3 x 5 tets: FETCH 237.27 MiB x 2.0 + WRITE 12.60 MiB x 1.0
 SPMV_TRAFFIC_96 = 510_798_243
-# 192^3 hexes (scalar, FP32 values, FP64 vectors): not measured yet
-SPMV_TRAFFIC_HEX192 = None
+# 192^3 hexes (scalar, FP32 values, FP64 vectors): FETCH 989.50 MiB x 2.0 + WRITE 61.65 MiB x 1.0
+SPMV_TRAFFIC_HEX192 = 2_139_761_441
 METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
 
 
