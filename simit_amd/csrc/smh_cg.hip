@@ -632,8 +632,18 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   smh_bsr Cb = *Am;
   Cb.vals = learned ? (const void *)o->learned_corr : Am->vals;
   Cb.vals_lo = nullptr;
+  // the correction pass runs its precision's tile shape (one block per lane
+  // for FP8 3x3), so it may need a plan of its own
+  smh_tile_plan lplan = *plan;
+  if (learned && tile_blocks_of<LP, BD>() != plan->tile_blocks) {
+    const int prc = tile_plan(ctx, Am->rowptr, nrows, Am->nnz, tile_blocks_of<LP, BD>(), &lplan);
+    if (prc) return prc;
+    SMH_CHECK(ctx, lplan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
+              "smh_cg_solve: a row holds %u blocks (the learned correction supports %u)",
+              lplan.maxrow, tile_blocks_of<LP, BD>() / 2);
+  }
   unsigned gc = 0;
-  const TileDev TC = tile_dev(&Cb, plan, &gc);
+  const TileDev TC = tile_dev(&Cb, &lplan, &gc);
   const int check = o->check_every > 0 ? o->check_every : 16;
   const void *pdata = nullptr;  // the preconditioner data K2 reads
   Work w;

@@ -1302,8 +1302,11 @@ template <int LP, int BD>
 int lanczos(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, const void *corr, double *theta_min,
             double *theta_max, int *steps) {
   smh_tile_plan plan;
-  int rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(BD), &plan);
+  int rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_of<LP, BD>(), &plan);
   if (rc) return rc;
+  SMH_CHECK(ctx, plan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
+            "smh_gnn_precond: a row holds %u blocks (at most %u supported)", plan.maxrow,
+            tile_blocks_of<LP, BD>() / 2);
   smh_bsr Cb = *A;
   Cb.vals = corr;
   Cb.vals_lo = nullptr;

@@ -42,6 +42,12 @@ namespace smh {
 #ifndef SMH_TILE_ROUNDS1
 #define SMH_TILE_ROUNDS1 1
 #endif
+// FP8 3x3 tiles (one block per lane): 4 rounds, whose gathers go out as one
+// group (measured at 96^3 x 5: 58 us per correction pass against 67-86 us for
+// 1-3 rounds and 75-93 us for 6-16)
+#ifndef SMH_TILE_ROUNDS8
+#define SMH_TILE_ROUNDS8 4
+#endif
 // matrix values are read once per product: streaming (non-temporal) loads
 #ifndef SMH_STREAM_LOAD
 #define SMH_STREAM_LOAD(p) __builtin_nontemporal_load(p)
@@ -59,7 +65,21 @@ struct TileShape {
   static constexpr int kPerWave = 64 / BD * kVec;         // blocks per wave and round
   static constexpr int kPerRound = kWaves * kPerWave;     // blocks per round
   static constexpr int kBlocks = kPerRound * kRounds;     // blocks per tile
+  static constexpr bool kBPL = false;                     // one block per lane
   static_assert(kThreads % 64 == 0, "tiles are made of whole wavefronts");
+};
+// FP8 3x3 blocks (the learned correction) put a whole block on one lane: its
+// three 4-byte row words are one 12-byte load and 64 blocks go per wave and
+// round, instead of one word per lane and 21 blocks.
+struct TileShapeB {
+  static constexpr int kThreads = SMH_TILE_THREADS3;
+  static constexpr int kRounds = SMH_TILE_ROUNDS8;
+  static constexpr int kWaves = kThreads / 64;
+  static constexpr int kVec = 1;
+  static constexpr int kPerWave = 64;
+  static constexpr int kPerRound = kWaves * kPerWave;
+  static constexpr int kBlocks = kPerRound * kRounds;
+  static constexpr bool kBPL = true;
 };
 inline uint32_t tile_blocks_host(int bd) {
   return bd == 3 ? (SMH_TILE_THREADS3 / 64) * 21 * SMH_TILE_ROUNDS3
@@ -246,6 +266,15 @@ __device__ __forceinline__ typename Val<PREC>::T val_of(typename Val<PREC>::S v)
     return (typename Val<PREC>::T)v;
 }
 
+// the tile shape a precision runs with
+template <int PREC, int BD>
+using TileShapeP =
+    typename std::conditional<Val<PREC>::kFP8 && BD == 3, TileShapeB, TileShape<BD>>::type;
+template <int PREC, int BD>
+inline uint32_t tile_blocks_of() {
+  return (uint32_t)TileShapeP<PREC, BD>::kBlocks;
+}
+
 struct TileDev {
   uint32_t nrows;
   uint64_t nnz;
@@ -258,15 +287,15 @@ struct TileDev {
   uint32_t tiles_per_wg;
 };
 
-template <int PREC, int BD>
+template <int PREC, int BD, class SH = TileShapeP<PREC, BD>>
 struct TileSmem {
   using T = typename Val<PREC>::T;
-  static constexpr int kBlocks = TileShape<BD>::kBlocks;
+  static constexpr int kBlocks = SH::kBlocks;
   T part[kBlocks * BD];
   T rsum[(kBlocks + 1) * BD];
   uint32_t rp[kBlocks + 2];  // rowptr of the tile's rows, relative
   uint32_t bnd[2][4];        // R0, R1, base, end of this and the next tile
-  double red[TileShape<BD>::kWaves];
+  double red[SH::kWaves];
 };
 
 // Gather of x for the tile loop: x_j as BD consecutive values.
@@ -426,6 +455,34 @@ struct TileRegsV {
   }
 };
 
+// Registers of one lane for FP8 3x3 tiles (TileShapeB): one whole block per
+// round, its three row words and its column.  (The same layout for FP32
+// blocks, nine values per lane, measured slower than one block row per lane:
+// 126 us against 106 us at 96^3 x 5.)
+template <int PREC>
+struct TileRegsB {
+  using SH = TileShapeB;
+  static_assert(Val<PREC>::kFP8, "one block per lane: FP8 blocks");
+  static constexpr int kRounds = SH::kRounds;
+  uint32_t v[kRounds][3];
+  uint32_t col[kRounds];  // kNoCol: past the tile's end
+
+  __device__ __forceinline__ void load(const TileDev &A, uint32_t base, uint32_t end, int w,
+                                       int lane) {
+#pragma unroll
+    for (int u = 0; u < kRounds; ++u) {
+      const uint32_t blk = base + u * SH::kPerRound + (uint32_t)(w * 64 + lane);
+      if (blk < end) {
+        load_row<uint32_t, 3, true>((const uint32_t *)A.vals + (uint64_t)blk * 3, v[u]);
+        col[u] = A.colidx[blk];
+      } else {
+        v[u][0] = v[u][1] = v[u][2] = 0u;
+        col[u] = kNoCol;
+      }
+    }
+  }
+};
+
 // Tile boundaries (R0, R1, base, end) of tile k into slot.
 __device__ __forceinline__ void tile_bounds(const TileDev &A, uint32_t k, uint32_t *slot) {
   const uint32_t R0 = A.tile_row[k], R1 = A.tile_row[k + 1];
@@ -437,22 +494,23 @@ __device__ __forceinline__ void tile_bounds(const TileDev &A, uint32_t k, uint32
 
 // Stream tiles [k0, k1): rows handed to epi(row, s[BD]) by one thread per
 // row.  XG gathers x: xg(j, bj).
-template <int PREC, int BD, class XG, class EPI>
+template <int PREC, int BD, class SH, class XG, class EPI>
 __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32_t k1,
-                                           TileSmem<PREC, BD> &sm, const XG &xg, EPI &epi) {
+                                           TileSmem<PREC, BD, SH> &sm, const XG &xg, EPI &epi) {
   using T = typename Val<PREC>::T;
-  using SH = TileShape<BD>;
   const int t = threadIdx.x;
   const int lane = t & 63, w = t >> 6;
-  const int s = w * SH::kPerWave + lane / BD, bi = lane % BD;
-  const bool ok = lane < SH::kPerWave * BD;
+  const int s = SH::kBPL ? w * 64 + lane : w * SH::kPerWave + lane / BD, bi = lane % BD;
+  const bool ok = SH::kBPL || lane < SH::kPerWave * BD;
   if (k0 >= k1) return;
   if (t == 0) {
     tile_bounds(A, k0, sm.bnd[0]);
     if (k0 + 1 < k1) tile_bounds(A, k0 + 1, sm.bnd[1]);
   }
   __syncthreads();
-  using Regs = typename std::conditional<BD == 1, TileRegsV<PREC>, TileRegs<PREC, BD>>::type;
+  using Regs = typename std::conditional<
+      BD == 1, TileRegsV<PREC>,
+      typename std::conditional<SH::kBPL, TileRegsB<PREC>, TileRegs<PREC, BD>>::type>::type;
   Regs cur, nxt;
   cur.load(A, sm.bnd[0][2], sm.bnd[0][3], w, lane);
   for (uint32_t k = k0; k < k1; ++k) {
@@ -482,6 +540,37 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
           T val = val_of<PREC>(cur.v[u][q]);
           if (Val<PREC>::kLo) val += (T)cur.vl[Val<PREC>::kLo ? u : 0][q];
           sm.part[s0 + q] = val * xo[q];
+        }
+      }
+    } else if constexpr (SH::kBPL) {
+      // one block per lane: the lane gathers its own x_j, decodes its three
+      // row words and forms the block's three row products (each in the
+      // order of the one-row-per-lane form, so the results are identical)
+      constexpr int kG = SH::kRounds % 4 == 0 ? 4
+                         : SH::kRounds % 3 == 0 ? 3
+                         : SH::kRounds % 2 == 0 ? 2
+                                                : 1;
+#pragma unroll
+      for (int u0 = 0; u0 < SH::kRounds; u0 += kG) {
+        T xo[kG][3];
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+          const uint32_t c = cur.col[u0 + j];
+          xg.template row<3>(c == kNoCol ? 0u : c, xo[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < kG; ++j) {
+          const int u = u0 + j;
+#pragma unroll
+          for (int a = 0; a < 3; ++a) {
+            T acc = 0;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+              const T val = (T)e4m3_scaled(cur.v[u][a] >> (8 * b));
+              acc += val * xo[j][b];
+            }
+            sm.part[(s + u * SH::kPerRound) * 3 + a] = acc;
+          }
         }
       }
     } else {
