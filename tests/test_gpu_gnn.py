@@ -258,6 +258,40 @@ def test_learned_pcg_converges(ctx, oracle, mixed):
     net.close()
 
 
+def learned_blocks(rp, ci, linv, Cb, bd):
+    """The preconditioner B = Linv^T (I + C) Linv as a dense matrix from the
+    stored buffers (linv float32 [n, bd, bd], C [nnz, bd, bd])."""
+    n = rp.size - 1
+    Li = linv.reshape(n, bd, bd).astype(np.float64)
+    Lm = sp.bsr_matrix((Li, np.arange(n), np.arange(n + 1)), shape=(n * bd, n * bd)).toarray()
+    return Lm.T @ (np.eye(n * bd) + dense_blocks(rp, ci, Cb)) @ Lm
+
+
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+def test_learned_apply_matches_stored_preconditioner(ctx, mixed):
+    # one CG iteration from x = 0 gives x1 = alpha B b: its direction pins the
+    # correction pass (one FP8 block per lane) against B built on the CPU
+    sy = tet_system(ctx, 5, seed=6)
+    idx = sy["idx"]
+    rp, ci = idx.to_host()
+    net = ctx.gnn(seed=4, kappa=0.9)
+    net.set_params(random_params(12, scale=0.5))
+    A = idx.bsr(3, S.F32X2, sy["hi"], sy["lo"]) if mixed else idx.bsr(3, S.F64, sy["v64"])
+    linv, corr = net.precond(A)
+    x = torch.zeros_like(sy["rhs"])
+    ctx.cg(A, sy["rhs"], x, ctx.cg_options(maxiters=1, tol=1e-30, rel=1,
+                                           precond=S.PRECOND_LEARNED, mixed=mixed,
+                                           learned=linv, learned_corr=corr))
+    Bm = learned_blocks(rp, ci, linv.cpu().numpy(), fp8_blocks(corr), 3)
+    z = Bm @ sy["rhs"].cpu().numpy()
+    xg = x.cpu().numpy()
+    cos = (xg @ z) / (np.linalg.norm(xg) * np.linalg.norm(z))
+    err = np.linalg.norm(xg / np.linalg.norm(xg) - z / np.linalg.norm(z))
+    # FP64 vectors: rounding only; FP32 vectors: the float32 r, u and z
+    assert err <= (1e-12 if mixed != 1 else 1e-6), (err, cos)
+    net.close()
+
+
 def test_learned_pcg_rejects_missing_buffers(ctx):
     sy = tet_system(ctx, 3, seed=0)
     A = sy["idx"].bsr(3, S.F64, sy["v64"])

@@ -14,7 +14,7 @@ import torch
 import gnn as G
 import simit_amd as S
 from simit_amd import meshes
-from test_gpu_gnn import check_guard, dense_blocks
+from test_gpu_gnn import check_guard, dense_blocks, learned_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -149,6 +149,27 @@ def test_scalar_learned_pcg_converges(ctx, oracle, mixed):
             rz = rzn
             it += 1
         assert abs(it - st.iters) <= max(2, st.iters // 50), (it, st.iters)
+    net.close()
+
+
+@pytest.mark.parametrize("mixed", [0, 1, 2])
+def test_scalar_learned_apply_matches_stored_preconditioner(ctx, mixed):
+    sy = hex_system(ctx, 10, seed=6)
+    idx = sy["idx"]
+    rp, ci = idx.to_host()
+    net = ctx.gnn(seed=4, kappa=0.9)
+    net.set_params(random_params(12, scale=0.5))
+    A = idx.bsr(1, S.F32X2, sy["hi"], sy["lo"]) if mixed else idx.bsr(1, S.F64, sy["v64"])
+    linv, corr = net.precond(A)
+    x = torch.zeros_like(sy["rhs"])
+    ctx.cg(A, sy["rhs"], x, ctx.cg_options(maxiters=1, tol=1e-30, rel=1,
+                                           precond=S.PRECOND_LEARNED, mixed=mixed,
+                                           learned=linv, learned_corr=corr))
+    Bm = learned_blocks(rp, ci, linv.cpu().numpy(), corr_values(corr).reshape(-1, 1, 1), 1)
+    z = Bm @ sy["rhs"].cpu().numpy()
+    xg = x.cpu().numpy()
+    err = np.linalg.norm(xg / np.linalg.norm(xg) - z / np.linalg.norm(z))
+    assert err <= (1e-12 if mixed != 1 else 1e-6), err
     net.close()
 
 
