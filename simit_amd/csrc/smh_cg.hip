@@ -620,8 +620,9 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   const uint32_t nrows = Am->nrows;
   const uint64_t n = (uint64_t)nrows * BD;
   unsigned gt = 0, grt = 0;
-  const TileDev T = tile_dev(Am, plan, &gt);
-  const TileDev TR = tile_dev(Am, rplan, &grt);
+  constexpr bool kIl = BD == 1 && SMH_TILE_INTERLEAVE1;
+  const TileDev T = tile_dev(Am, plan, &gt, kIl);
+  const TileDev TR = tile_dev(Am, rplan, &grt, kIl);
   const unsigned nbe = grid_for(nrows, 256, 2048);
   const unsigned nbv = grid_for(n, 256, 2048);
   const int pc = o->precond;
@@ -643,7 +644,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
               lplan.maxrow, tile_blocks_of<LP, BD>() / 2);
   }
   unsigned gc = 0;
-  const TileDev TC = tile_dev(&Cb, &lplan, &gc);
+  const TileDev TC = tile_dev(&Cb, &lplan, &gc, kIl);
   const int check = o->check_every > 0 ? o->check_every : 16;
   const void *pdata = nullptr;  // the preconditioner data K2 reads
   Work w;
@@ -1461,7 +1462,7 @@ extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
             "smh_bsrmv: a row holds %u blocks (at most %u outside exact_order)", plan.maxrow,
             nb / 2);
   unsigned g = 0;
-  const TileDev T = tile_dev(A, &plan, &g);
+  const TileDev T = tile_dev(A, &plan, &g, A->bdim == 1 && SMH_TILE_INTERLEAVE1);
 #define SMH_MV(P)                                                                            \
   do {                                                                                       \
     using VT = typename Val<P>::T;                                                           \

@@ -1311,7 +1311,7 @@ int lanczos(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, const void *corr, double
   Cb.vals = corr;
   Cb.vals_lo = nullptr;
   unsigned gc = 0;
-  const TileDev T = tile_dev(&Cb, &plan, &gc);
+  const TileDev T = tile_dev(&Cb, &plan, &gc, BD == 1 && SMH_TILE_INTERLEAVE1);
   const uint64_t n = (uint64_t)A->nrows * BD;
   const unsigned gv = grid_for(n, 256, 2048);
   const int m = (int)std::min<uint64_t>(kLzSteps, n);
