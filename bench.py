@@ -40,8 +40,13 @@ HBM_PEAK_GBS = 8000.0
 # summaries and commands in profiles/.
 # 96^3 x 5 tets: FETCH 237.27 MiB x 2.0 + WRITE 12.60 MiB x 1.0
 SPMV_TRAFFIC_96 = 510_798_243
-# 192^3 hexes (scalar, FP32 values, FP64 vectors): FETCH 989.50 MiB x 2.0 + WRITE 61.65 MiB x 1.0
-SPMV_TRAFFIC_HEX192 = 2_139_761_441
+# 192^3 hexes, blocked CSR (scalar, FP32 values, FP64 vectors): FETCH 803.30 MiB x 2.0 +
+# WRITE 56.56 MiB x 1.0
+SPMV_TRAFFIC_HEX192 = 1_743_948_909
+# the same matrix in 27-point stencil storage: FETCH 626.88 MiB x 2.0 + WRITE 54.85 MiB x 1.0
+# (1.54x the algorithmic bytes: FETCH_SIZE counts the x re-reads the Infinity
+# Cache serves as well as HBM reads)
+STENCIL_TRAFFIC_HEX192 = 1_372_174_950
 METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
 
 
@@ -243,7 +248,7 @@ def main():
     assemble()  # restore the FP32 system (rhs was overwritten by the FP64 assembly)
 
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----------------------------
-    cpu = None
+    cpu = cpu_omp = None
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import pyoracle as O
@@ -260,6 +265,8 @@ def main():
         per_iter = (t22 - t2) / 20
         it_cpu = legs["block_jacobi_pcg_f64"]["iters"]
         cpu_total = t_asm + t2 + per_iter * (it_cpu - 2)
+        cpu_omp = omp_baseline(O, rp, ci, ov, of, 3, 2, t_asm, it_cpu,
+                               "block-Jacobi PCG", f"{n}^3")
         cpu = {"value": round(cpu_total, 2), "unit": "s", "cores": 1, "kind": "port",
                "sample": (f"oracle (single-thread FP64 C restatement of the reference loops) "
                           f"on the full {n}^3 mesh: one assembly ({t_asm:.2f}s) and 22 "
@@ -268,6 +275,7 @@ def main():
                           f"needs on this system"),
                "nproc": os.cpu_count()}
         log(f"[bench] cpu baseline {cpu}")
+        log(f"[bench] cpu baseline (OpenMP) {cpu_omp}")
 
     out = {
         "metric": METRIC,
@@ -309,6 +317,7 @@ def main():
                      "traffic": SPMV_TRAFFIC_96 if n == 96 else None,
                      "algorithmic_bytes": spmv_bytes},
         "cpu_baseline": cpu,
+        "cpu_baseline_omp": cpu_omp,
     }
     print(json.dumps(out), flush=True)
     net.close()
@@ -320,11 +329,13 @@ def run_hex(args, ctx):
     lognormal (std 1.5, correlation length 12 cells), T = 0 on x = 0, uniform
     source.  The scalar learned correction is trained first on seeded
     hex_train_n^3 samples of the same statistics (HexSampler), then applied
-    at full size.  Step = assemble FP32 hi/lo + network + learned PCG to a
-    certified 1e-8: FP32 matrix values in the SpMV, FP64 vectors, dots and
-    reliable updates (FP32 vectors cost this system 55% more iterations).
-    Mixed Jacobi-PCG (the same step without the network), FP64 Jacobi-PCG and
-    plain FP64 CG are timed beside it."""
+    at full size.  Step = assemble FP32 hi/lo (blocked CSR, the index's
+    pattern) + convert to 27-point stencil storage + network + its correction
+    converted likewise + learned PCG to a certified 1e-8 over the stencil
+    storage: FP32 matrix values in the SpMV, FP64 vectors, dots and reliable
+    updates (FP32 vectors cost this system 55% more iterations).  Mixed
+    Jacobi-PCG (the same step without the network, stencil and CSR storage),
+    FP64 Jacobi-PCG and plain FP64 CG are timed beside it."""
     import torch
     import simit_amd as S
     from simit_amd import meshes, learned
@@ -373,12 +384,28 @@ def run_hex(args, ctx):
     spmv_ms = timed(lambda: ctx.bsrmv(A32, xs, ys), 20)
     spmv_bytes = 8 * nnz + 4 * (nv + 1) + 16 * nv
     spmv_gbs = spmv_bytes / spmv_ms / 1e6
-    log(f"[bench] assembly {asm_ms:.2f} ms {asm_gbs:.0f} GB/s; spmv {spmv_ms * 1e3:.1f} us "
+    log(f"[bench] assembly {asm_ms:.2f} ms {asm_gbs:.0f} GB/s; CSR spmv {spmv_ms * 1e3:.1f} us "
         f"{spmv_gbs:.0f} GB/s")
+    # the solver's storage: the same matrix as a 27-point stencil on the
+    # (n+1)^3 vertex grid (no column indices, slot-major values)
+    dims = (n + 1,) * 3
+    st_hi = torch.empty(27 * nv, dtype=torch.float32, device="cuda")
+    st_lo = torch.empty_like(st_hi)
+    St = ctx.stencil27_from_bsr(A2, dims, st_hi, st_lo)
+    conv_ms = timed(lambda: ctx.stencil27_from_bsr(A2, dims, st_hi, st_lo), 3)
+    S32 = S.Stencil27(*dims, S.F32D, st_hi.data_ptr(), None)
+    ctx.stencil27_mv(S32, xs, ys)
+    st_ms = timed(lambda: ctx.stencil27_mv(S32, xs, ys), 20)
+    st_bytes = 108 * nv + 8 * nv + 8 * nv
+    st_gbs = st_bytes / st_ms / 1e6
+    log(f"[bench] CSR -> stencil {conv_ms:.2f} ms; stencil spmv {st_ms * 1e3:.1f} us "
+        f"{st_gbs:.0f} GB/s")
     if args.kernels_only:
         print(json.dumps({"kernels_only": True, "workload": "hex", "index_ms": index_ms,
                           "assembly_ms": asm_ms, "assembly_GBps": asm_gbs,
-                          "spmv_us": spmv_ms * 1e3, "spmv_GBps": spmv_gbs}))
+                          "csr_spmv_us": spmv_ms * 1e3, "csr_spmv_GBps": spmv_gbs,
+                          "stencil_convert_ms": conv_ms, "stencil_spmv_us": st_ms * 1e3,
+                          "stencil_spmv_GBps": st_gbs}))
         return
 
     # ---- the learned correction: training on hex_train_n^3 samples -------------------------------
@@ -393,12 +420,16 @@ def run_hex(args, ctx):
     def opts(**kw):
         return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
 
+    corr_st = torch.empty(27 * nv, dtype=torch.float32, device="cuda")
+
     def step():
         assemble()
+        ctx.stencil27_from_bsr(A2, dims, st_hi, st_lo)
         linv, corr = net.precond(A2)
+        ctx.stencil27_from_bsr(idx.bsr(1, S.F32, corr.view(torch.float32)), dims, corr_st)
         x = torch.zeros(nv, dtype=torch.float64, device="cuda")
-        return ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=2, learned=linv,
-                                       learned_corr=corr))
+        return ctx.cg_stencil27(St, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=2,
+                                                 learned=linv, learned_corr=corr_st))
 
     for w in range(args.warmup):
         st = step()
@@ -412,20 +443,26 @@ def run_hex(args, ctx):
         f"true_rel={stats[-1]['true_rel']:.2e}")
     pre_ms = timed(lambda: net.precond(A2), 3)
 
-    def jac_mixed():
-        assemble()
-        x = torch.zeros(nv, dtype=torch.float64, device="cuda")
+    def jac_mixed(stencil):
         torch.cuda.synchronize()
         t = time.perf_counter()
-        st = ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=2))
+        assemble()
+        x = torch.zeros(nv, dtype=torch.float64, device="cuda")
+        if stencil:
+            ctx.stencil27_from_bsr(A2, dims, st_hi, st_lo)
+            st = ctx.cg_stencil27(St, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=2))
+        else:
+            st = ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=2))
         torch.cuda.synchronize()
         r = {"seconds": round(time.perf_counter() - t, 3), "iters": st.iters,
              "reliable_updates": st.reliable_updates, "true_rel": st.true_rel,
              "converged": st.status == 0 and st.true_rel <= 1e-8}
-        log(f"[bench] Jacobi PCG mixed (assembly + solve): {r}")
+        log(f"[bench] Jacobi PCG mixed, {'stencil' if stencil else 'CSR'} storage (assembly + "
+            f"solve): {r}")
         return r
-    jac_mixed()
-    jacm = jac_mixed()
+    jac_mixed(True)
+    jacm = jac_mixed(True)
+    jacm_csr = jac_mixed(False)
     net.close()
     v64 = torch.empty(nnz, dtype=torch.float64, device="cuda")
     A64 = idx.bsr(1, S.F64, v64)
@@ -447,7 +484,7 @@ def run_hex(args, ctx):
     # for a bounded number of iterations and reported as not converged
     plain = None if args.no_plain else leg("plain CG FP64", S.PRECOND_NONE,
                                            min(args.plain_maxiters, 20000))
-    cpu = None
+    cpu = cpu_omp = None
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "oracle"))
         import pyoracle as O
@@ -462,6 +499,7 @@ def run_hex(args, ctx):
         O.cg(rp, ci, ov, of, np.zeros(nv), 1, O.cg_opts(maxiters=12, tol=1e-8, precond=1))
         per = (time.perf_counter() - t0 - t2) / 10
         it = jac64["iters"]
+        cpu_omp = omp_baseline(O, rp, ci, ov, of, 1, 1, t_asm, it, "Jacobi PCG", f"{n}^3")
         cpu = {"value": round(t_asm + t2 + per * (it - 2), 2), "unit": "s", "cores": 1,
                "kind": "port",
                "sample": (f"oracle (single-thread FP64 C) on the full {n}^3 mesh: one assembly "
@@ -469,6 +507,7 @@ def run_hex(args, ctx):
                           f"extrapolated to the {it} iterations FP64 Jacobi PCG needs"),
                "nproc": os.cpu_count()}
         log(f"[bench] cpu baseline {cpu}")
+        log(f"[bench] cpu baseline (OpenMP) {cpu_omp}")
     print(json.dumps({
         "metric": METRIC, "value": round(step_s, 4), "unit": "s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 2),
@@ -477,28 +516,54 @@ def run_hex(args, ctx):
         "data": "synthetic (seeded Gaussian random field conductivity, generated on the host)",
         "config": {"workload": f"trilinear hex Poisson {n}^3 cells, {nv} unknowns "
                                f"({int(prob['dirichlet'].sum())} Dirichlet), learned PCG with FP32 "
-                               f"matrix values and FP64 vectors to a 1e-8 FP64 relative residual",
+                               f"matrix values and FP64 vectors to a 1e-8 FP64 relative residual, "
+                               f"the matrix and the correction in 27-point stencil storage",
                    "nverts": nv, "nhex": nh, "nnz": nnz,
                    "train": f"{args.hex_train_samples} seeded {args.hex_train_n}^3 samples"},
         "iterations": stats[-1]["iters"], "true_rel": stats[-1]["true_rel"],
         "reliable_updates": stats[-1]["reliable_updates"],
         "converged": all(s["status"] == 0 and s["true_rel"] <= 1e-8 for s in stats),
-        "jacobi_pcg_mixed": jacm,
+        "jacobi_pcg_mixed": jacm, "jacobi_pcg_mixed_csr": jacm_csr,
         "speedup_vs_jacobi_mixed": round(jacm["seconds"] / step_s, 3),
         "jacobi_pcg_f64": jac64, "plain_cg_f64": plain,
         "gnn_train_seconds": round(tr["seconds"], 2), "gnn_precond_ms": round(pre_ms, 2),
         "speedup_vs_plain_cg_f64": (round(plain["seconds"] / step_s, 2)
                                     if plain and plain["converged"] else None),
         "index_build_ms": round(index_ms, 2), "assembly_ms": round(asm_ms, 3),
-        "assembly_GBps": round(asm_gbs, 1), "spmv_us": round(spmv_ms * 1e3, 1),
-        "spmv_GBps": round(spmv_gbs, 1),
+        "assembly_GBps": round(asm_gbs, 1), "stencil_convert_ms": round(conv_ms, 3),
+        "spmv_us": round(st_ms * 1e3, 1), "spmv_GBps": round(st_gbs, 1),
+        "csr_spmv_us": round(spmv_ms * 1e3, 1), "csr_spmv_GBps": round(spmv_gbs, 1),
+        "csr_spmv_traffic": SPMV_TRAFFIC_HEX192 if n == 192 else None,
         "roofline": {"bound": "hbm",
-                     "kernel": "k_bsrmv_tiles<SMH_F32D,1> (scalar CSR SpMV: FP32 values, FP64 vectors)",
-                     "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
-                     "traffic": SPMV_TRAFFIC_HEX192 if n == 192 else None,
-                     "algorithmic_bytes": spmv_bytes},
-        "cpu_baseline": cpu}), flush=True)
+                     "kernel": "k_stmv<SMH_F32D> (27-point stencil SpMV: FP32 values, FP64 "
+                               "vectors; the solver's k_cg_spmv_st runs the same row loop)",
+                     "achieved": round(st_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(st_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": STENCIL_TRAFFIC_HEX192 if n == 192 else None,
+                     "algorithmic_bytes": st_bytes},
+        "cpu_baseline": cpu, "cpu_baseline_omp": cpu_omp}), flush=True)
+
+
+def omp_baseline(O, rp, ci, vals, rhs, bdim, precond, t_asm, iters, what, mesh):
+    """The same oracle built with OpenMP on all the host threads it is given
+    (OMP_NUM_THREADS): 2 and 12 PCG iterations timed, the difference per
+    iteration extrapolated to `iters`; the assembly is the single-thread one."""
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    n = bdim * (len(rp) - 1)
+    t0 = time.perf_counter()
+    O.cg(rp, ci, vals, rhs, np.zeros(n), bdim, O.cg_opts(maxiters=2, tol=1e-8, precond=precond),
+         omp=True)
+    t2 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.cg(rp, ci, vals, rhs, np.zeros(n), bdim, O.cg_opts(maxiters=12, tol=1e-8, precond=precond),
+         omp=True)
+    per = (time.perf_counter() - t0 - t2) / 10
+    return {"value": round(t_asm + t2 + per * (iters - 2), 2), "unit": "s", "cores": threads,
+            "kind": "port",
+            "sample": (f"the oracle built with OpenMP ({threads} threads: SpMV rows, vector "
+                       f"updates and dot reductions in parallel) on the full {mesh} mesh: the "
+                       f"single-thread assembly ({t_asm:.2f}s) and 12 {what} iterations "
+                       f"({per * 1e3:.1f} ms/iter), extrapolated to {iters} iterations")}
 
 
 def run_batch(args, ctx):

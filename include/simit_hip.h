@@ -293,6 +293,39 @@ void smh_cg_default_opts(smh_cg_opts *o);
 int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
                  const smh_cg_opts *opts, smh_cg_stats *stats);
 
+/* ---- scalar 27-point stencil storage ---------------------------------------
+ * Replaces TensorStorage::Stencil (src/storage.h:48) and its SpMV lowering
+ * (src/lower/index_expressions/lower_indexexprs.cpp:669-711) for scalar
+ * systems on a structured n0 x n1 x n2 grid of rows (row id =
+ * i0 + n0*(i1 + n1*i2)): no column indices; vals[s*nrows + i] couples row i
+ * with column i + off_s, off_s = d0 + n0*d1 + n0*n1*d2 for (d2, d1, d0) in
+ * lexicographic order over {-1,0,1}^3 (ascending offsets, s = 13 the
+ * diagonal), zero where there is no coupling.  Any 1x1 matrix whose columns
+ * lie within those offsets of their row is representable (the trilinear hex
+ * Poisson matrix of smh_assemble_hex_poisson is).  Products sum in ascending
+ * column order with each product rounded, so smh_stencil27_mv equals the
+ * blocked-CSR smh_bsrmv of the same matrix bit for bit. */
+typedef struct {
+  uint32_t n0, n1, n2;
+  int32_t precision;     /* SMH_F64, SMH_F32, SMH_F32X2, SMH_F32D          */
+  const void *vals;      /* device [27*nrows] (double for SMH_F64, else float) */
+  const void *vals_lo;   /* SMH_F32X2: device float[27*nrows], else NULL    */
+} smh_stencil27;
+/* scatter a 1x1 blocked-CSR matrix (any precision; SMH_F32X2 copies both
+ * halves) into stencil storage; SMH_ERR_INVALID if a column lies outside its
+ * row's 27 offsets.  Also converts the float[nnz] learned correction of
+ * smh_gnn_precond (pass it as vals of an SMH_F32 matrix) to the float[27*n]
+ * opts.learned_corr of smh_cg_solve_stencil27. */
+int smh_stencil27_from_bsr(smh_ctx *ctx, const smh_bsr *A, uint32_t n0, uint32_t n1, uint32_t n2,
+                           void *vals, void *vals_lo);
+/* y = A*x (vector types as smh_bsrmv) */
+int smh_stencil27_mv(smh_ctx *ctx, const smh_stencil27 *A, const void *x, void *y);
+/* smh_cg_solve over stencil storage: same options, state machine and
+ * certification; preconditioners none, Jacobi (block Jacobi = Jacobi for
+ * 1x1) and learned (opts.learned float[n], opts.learned_corr float[27*n]) */
+int smh_cg_solve_stencil27(smh_ctx *ctx, const smh_stencil27 *A, const double *b, double *x,
+                           const smh_cg_opts *opts, smh_cg_stats *stats);
+
 /* ---- batched CG: many independent systems in one launch ------------------
  * The systems are the diagonal blocks of one matrix: system m owns rows
  * [rows[m], rows[m+1]) (host array of nbatch+1 entries from 0 to nrows) and

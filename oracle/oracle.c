@@ -2,6 +2,12 @@
  * CPU oracle: single-threaded FP64 restatement of the reference's hot-path
  * loop nests.  TEST INFRASTRUCTURE ONLY -- see oracle.h for who may use it
  * and what pins each function.
+ *
+ * The `omp` pragmas are inert in liboracle.so (built without -fopenmp: the
+ * checker, strictly sequential).  liboracle_omp.so is the same source built
+ * with -fopenmp, used only as the multi-core CPU baseline of bench.py: its
+ * SpMV rows and vector updates are the same per element, its dot products
+ * are parallel reductions (a different summation order).
  */
 #include "oracle.h"
 
@@ -380,6 +386,7 @@ void or_bsrmv(int32_t nrows, int32_t bdim, const uint32_t *rowptr,
               const uint32_t *colidx, const double *vals, const double *x,
               double *y) {
   const int bb = bdim * bdim;
+#pragma omp parallel for schedule(static)
   for (int32_t i = 0; i < nrows; ++i) {
     double tmp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t ij = rowptr[i]; ij < rowptr[i + 1]; ++ij) {
@@ -411,6 +418,7 @@ void or_bsrmv_f32(int32_t nrows, int32_t bdim, const uint32_t *rowptr,
 /* flatten.cpp:286-306 -> ir_builder.cpp:130-140: sequential inner product */
 double or_dot(int64_t n, const double *x, const double *y) {
   double s = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : s)
   for (int64_t i = 0; i < n; ++i) s += x[i] * y[i];
   return s;
 }
@@ -425,8 +433,10 @@ static void precond_apply(int32_t nrows, int32_t bdim, int32_t kind,
   if (kind == 0) {
     memcpy(z, r, sizeof(double) * n);
   } else if (kind == 1) {
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
   } else {
+#pragma omp parallel for schedule(static)
     for (int32_t i = 0; i < nrows; ++i)
       for (int a = 0; a < 3; ++a) {
         double s = 0.0;
@@ -476,17 +486,20 @@ void or_cg(int32_t nrows, int32_t bdim, const uint32_t *rowptr,
     or_bsrmv(nrows, bdim, rowptr, colidx, vals, p, Ap);
     double denom = or_dot(n, p, Ap);
     double alpha = (o->norm_form ? or_dot(n, r, r) : rz) / denom;
+#pragma omp parallel for schedule(static)
     for (int64_t q = 0; q < n; ++q) x[q] = x[q] + alpha * p[q];
     iter = iter + 1;
     if (o->replace_every > 0 && iter % o->replace_every == 0) {
       or_bsrmv(nrows, bdim, rowptr, colidx, vals, x, Ap);
       for (int64_t q = 0; q < n; ++q) r[q] = b[q] - Ap[q];
     } else {
+#pragma omp parallel for schedule(static)
       for (int64_t q = 0; q < n; ++q) r[q] = r[q] - alpha * Ap[q];
     }
     precond_apply(nrows, bdim, o->precond, dinv, r, z);
     double rzn = or_dot(n, z, r);
     double beta = rzn / rz;
+#pragma omp parallel for schedule(static)
     for (int64_t q = 0; q < n; ++q) p[q] = z[q] + beta * p[q];
     rz = rzn;
     rr = or_dot(n, r, r);

@@ -15,9 +15,11 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "liboracle.so"
+LIB_OMP = HERE / "liboracle_omp.so"  # the same source with OpenMP (bench baseline only)
 REF_INDEX = HERE / "_ref" / "ref_index"
 
 _lib = None
+_lib_omp = None
 
 
 def build(ref: bool = False):
@@ -33,6 +35,18 @@ def lib():
         _lib = C.CDLL(str(LIB))
         _declare(_lib)
     return _lib
+
+
+def lib_omp():
+    """The OpenMP build of the same restatement: multi-core CPU baseline only
+    (its dot products sum in a different order; never a checker)."""
+    global _lib_omp
+    if _lib_omp is None:
+        if not LIB_OMP.exists():
+            build()
+        _lib_omp = C.CDLL(str(LIB_OMP))
+        _declare(_lib_omp)
+    return _lib_omp
 
 
 class SpringsParams(C.Structure):
@@ -218,12 +232,12 @@ def cg_opts(maxiters=1000, tol=1e-8, rel=1, replace_every=0, precond=0, norm_for
     return CGOpts(maxiters, rel, replace_every, precond, norm_form, tol)
 
 
-def cg(rowptr, colidx, vals, b, x0, bdim, opts):
+def cg(rowptr, colidx, vals, b, x0, bdim, opts, omp=False):
     x = _c(x0, np.float64).copy()
     st = CGStats()
     n = len(rowptr) - 1
-    lib().or_cg(n, bdim, _p(rowptr), _p(colidx), _p(_c(vals, np.float64)), _p(_c(b, np.float64)),
-                _p(x), C.byref(opts), C.byref(st))
+    (lib_omp() if omp else lib()).or_cg(n, bdim, _p(rowptr), _p(colidx), _p(_c(vals, np.float64)),
+                                        _p(_c(b, np.float64)), _p(x), C.byref(opts), C.byref(st))
     return x, {"iters": st.iters, "rr": st.rr, "true_rel": st.true_rel}
 
 

@@ -18,7 +18,7 @@ from pathlib import Path
 
 __all__ = [
     "SimitError", "load", "Context", "Index", "Bsr", "CGOptions", "CGStats",
-    "SpringsParams", "TetParams", "HexParams",
+    "SpringsParams", "TetParams", "HexParams", "Stencil27",
     "F64", "F32", "F32X2", "F32D",
     "PRECOND_NONE", "PRECOND_JACOBI", "PRECOND_BLOCK_JACOBI", "PRECOND_LEARNED",
     "INDEX_FORCE_DIAG", "LIB_PATH", "HEADER_PATH",
@@ -55,6 +55,14 @@ class Bsr(C.Structure):
     _fields_ = [("nrows", C.c_uint32), ("bdim", C.c_uint32), ("nnz", C.c_uint64),
                 ("rowptr", C.c_void_p), ("colidx", C.c_void_p), ("precision", C.c_int32),
                 ("vals", C.c_void_p), ("vals_lo", C.c_void_p)]
+
+
+class Stencil27(C.Structure):
+    """Scalar 27-point stencil matrix of an n0 x n1 x n2 grid of rows:
+    vals[s*n + i] couples row i with column i + off_s (ascending offsets,
+    s = 13 the diagonal)."""
+    _fields_ = [("n0", C.c_uint32), ("n1", C.c_uint32), ("n2", C.c_uint32),
+                ("precision", C.c_int32), ("vals", C.c_void_p), ("vals_lo", C.c_void_p)]
 
 
 class SpringsParams(C.Structure):
@@ -158,6 +166,10 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_assemble_edge_blocks": (C.c_int, [vp, vp, C.POINTER(EdgeBlocks), i32, vp, vp]),
         "smh_bsrmv": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, i32]),
         "smh_dot": (C.c_int, [vp, i32, u64, vp, vp, C.POINTER(C.c_double)]),
+        "smh_stencil27_from_bsr": (C.c_int, [vp, C.POINTER(Bsr), u32, u32, u32, vp, vp]),
+        "smh_stencil27_mv": (C.c_int, [vp, C.POINTER(Stencil27), vp, vp]),
+        "smh_cg_solve_stencil27": (C.c_int, [vp, C.POINTER(Stencil27), vp, vp,
+                                             C.POINTER(CGOptions), C.POINTER(CGStats)]),
         "smh_block_jacobi": (C.c_int, [vp, C.POINTER(Bsr), vp]),
         "smh_axpby": (C.c_int, [vp, u64, C.c_double, vp, C.c_double, vp]),
         "smh_bsr_forget": (C.c_int, [vp, vp]),
@@ -300,6 +312,33 @@ class Context:
         st = CGStats()
         self._check(self.lib.smh_cg_solve(self.h, C.byref(A), _ptr(b), _ptr(x), C.byref(opts),
                                           C.byref(st)))
+        return st
+
+    # ---- 27-point stencil storage ---------------------------------------------
+    def stencil27_from_bsr(self, A: Bsr, dims, vals=None, vals_lo=None) -> Stencil27:
+        """Scatter a 1x1 matrix into stencil storage on the (n0, n1, n2) grid;
+        allocates the [27*n] value arrays unless given."""
+        import torch
+        n0, n1, n2 = (int(d) for d in dims)
+        n = n0 * n1 * n2
+        if vals is None:
+            dt = torch.float64 if A.precision == F64 else torch.float32
+            vals = torch.empty(27 * n, dtype=dt, device="cuda")
+        if vals_lo is None and A.precision == F32X2:
+            vals_lo = torch.empty(27 * n, dtype=torch.float32, device="cuda")
+        self._check(self.lib.smh_stencil27_from_bsr(self.h, C.byref(A), n0, n1, n2, _ptr(vals),
+                                                    _ptr(vals_lo)))
+        S = Stencil27(n0, n1, n2, A.precision, _ptr(vals), _ptr(vals_lo))
+        S._keep = (vals, vals_lo)
+        return S
+
+    def stencil27_mv(self, S: Stencil27, x, y):
+        self._check(self.lib.smh_stencil27_mv(self.h, C.byref(S), _ptr(x), _ptr(y)))
+
+    def cg_stencil27(self, S: Stencil27, b, x, opts: CGOptions) -> CGStats:
+        st = CGStats()
+        self._check(self.lib.smh_cg_solve_stencil27(self.h, C.byref(S), _ptr(b), _ptr(x),
+                                                    C.byref(opts), C.byref(st)))
         return st
 
     def cg_batched(self, A: Bsr, rows, b, x, opts: CGOptions):

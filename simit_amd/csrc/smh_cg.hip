@@ -613,38 +613,297 @@ struct Work {
 };
 
 
-template <int IP, int RP, int BD>
-int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh_tile_plan *rplan,
-           const double *b, double *x, const smh_cg_opts *o, smh_cg_stats *st) {
+// ---- scalar 27-point stencil storage ----------------------------------------------------
+// Row i couples with the columns i + off[s], s = 0..26 in ascending order
+// (s = 13 is the diagonal); values are [27][n] (slot-major, so consecutive
+// rows read consecutive addresses) and zero where there is no coupling.
+struct StDev {
+  uint32_t n;
+  int32_t off[27];
+  const void *vals;
+  const void *vals_lo;
+};
+
+// One thread per row: the 27 products in ascending column order, each
+// rounded before it is added -- the order and rounding of the tile loop, so
+// the result is the blocked-CSR product bit for bit (zero slots add exact
+// zeros; out-of-range columns read a clamped x times a zero value).
+//
+// A plain grid-stride loop over the rows.  (Giving each XCD a contiguous
+// eighth of the rows, so that its x window with the +-n0*n1 halo stays in its
+// L2, measured slower on the 192^3 hex system: 226 us against 212 us; the
+// re-reads are served by the Infinity Cache.)
+template <int PREC, class XG, class EPI>
+__device__ __forceinline__ void stencil_rows(const StDev &A, const XG &xg, EPI &epi) {
+#pragma clang fp contract(off)
+  using T = typename Val<PREC>::T;
+  using S = typename Val<PREC>::S;
+  const uint64_t n = A.n;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    // all 27 values, then all 27 gathers, then the products: the loads go
+    // out together (one slot at a time leaves two loads in flight)
+    S vs[27];
+    float vl[Val<PREC>::kLo ? 27 : 1];
+    T xs[27];
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+      vs[s] = ((const S *)A.vals)[(uint64_t)s * n + i];
+      if constexpr (Val<PREC>::kLo) vl[s] = ((const float *)A.vals_lo)[(uint64_t)s * n + i];
+    }
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+      const int64_t j = (int64_t)i + A.off[s];
+      const uint32_t jc = j < 0 ? 0u : (j >= (int64_t)n ? (uint32_t)(n - 1) : (uint32_t)j);
+      xs[s] = xg(jc, 0);
+    }
+    T acc = 0;
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+      T v = val_of<PREC>(vs[s]);
+      if constexpr (Val<PREC>::kLo) v += (T)vl[s];
+      const T prod = v * xs[s];
+      acc = acc + prod;
+    }
+    epi((uint32_t)i, &acc);
+  }
+}
+
+template <int PREC>
+__global__ void __launch_bounds__(256) k_stmv(StDev A, const typename Val<PREC>::T *__restrict__ x,
+                                             typename Val<PREC>::T *__restrict__ y) {
+  using T = typename Val<PREC>::T;
+  XPlain<T> xg{x, 1};
+  EpiStore<T, 1> epi{y};
+  stencil_rows<PREC>(A, xg, epi);
+}
+
+template <int IP>
+__global__ void __launch_bounds__(256)
+k_cg_spmv_st(StDev A, const typename Val<IP>::T *__restrict__ p,
+             typename Val<IP>::T *__restrict__ q, const CGState *S, const float *dlo,
+             double *part) {
   using VT = typename Val<IP>::T;
-  const uint32_t nrows = Am->nrows;
-  const uint64_t n = (uint64_t)nrows * BD;
-  unsigned gt = 0, grt = 0;
-  constexpr bool kIl = BD == 1 && SMH_TILE_INTERLEAVE1;
-  const TileDev T = tile_dev(Am, plan, &gt, kIl);
-  const TileDev TR = tile_dev(Am, rplan, &grt, kIl);
-  const unsigned nbe = grid_for(nrows, 256, 2048);
-  const unsigned nbv = grid_for(n, 256, 2048);
-  const int pc = o->precond;
-  const bool learned = pc == SMH_PRECOND_LEARNED;
-  // the correction's storage: FP8 words (3x3 blocks) or FP32 (scalar systems)
+  __shared__ double red[4];
+  if (S->done) return;
+  XPlain<VT> xg{p, 1};
+  EpiPq<VT> epi{q, p, dlo, 0.0, 1};
+  stencil_rows<IP>(A, xg, epi);
+  const double t = block_sum256(epi.d, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <int RP, typename VT>
+__global__ void __launch_bounds__(256)
+k_residual_st(StDev A, const double *__restrict__ b, const double *__restrict__ x,
+              EpiResidual<VT, 1> epi, double *part) {
+  __shared__ double red[4];
+  XPlain<double> xg{x, 1};
+  epi.drz = 0.0;
+  epi.drr = 0.0;
+  stencil_rows<RP>(A, xg, epi);
+  const double t1 = block_sum256(epi.drz, red);
+  const double t2 = block_sum256(epi.drr, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t1;
+    part[2 * blockIdx.x + 1] = t2;
+  }
+}
+
+template <int LP>
+__global__ void __launch_bounds__(256)
+k_cg_lapply_st(StDev C, const typename Val<LP>::T *__restrict__ u,
+               const typename Val<LP>::T *__restrict__ r, typename Val<LP>::T *__restrict__ z,
+               const float *__restrict__ linv, const CGState *S, int check_done, double *part) {
+  using VT = typename Val<LP>::T;
+  __shared__ double red[4];
+  if (check_done && S->done) return;
+  XPlain<VT> xg{u, 1};
+  EpiLearned<VT, 1> epi{u, r, z, linv, 0.0, 0.0};
+  stencil_rows<LP>(C, xg, epi);
+  const double t1 = block_sum256(epi.drz, red);
+  const double t2 = block_sum256(epi.drr, red);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = t1;
+    part[2 * blockIdx.x + 1] = t2;
+  }
+}
+
+// dlo_i = sum of the 27 low parts of row i (ascending, FP64: the CSR sum)
+__global__ void k_lo_rowsum_st(uint32_t n, const float *__restrict__ lo,
+                               float *__restrict__ dlo) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int q = 0; q < 27; ++q) s += (double)lo[(uint64_t)q * n + i];
+  dlo[i] = (float)s;
+}
+
+// Jacobi: 1 / the diagonal slot, in the matrix's best precision
+template <int RP, typename OT>
+__global__ void k_dinv_st(uint32_t n, const void *vals, const void *vals_lo, OT *out) {
+  using S = typename Val<RP>::S;
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = (double)((const S *)vals)[13ull * n + i];
+  if (Val<RP>::kLo) d += (double)((const float *)vals_lo)[13ull * n + i];
+  out[i] = (OT)(1.0 / d);
+}
+
+// CSR -> stencil: values of row i land in the slot of their column offset
+__global__ void k_st_fill(uint64_t count, uint32_t *__restrict__ out) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    out[t] = 0u;
+}
+__global__ void k_st_scatter(StDev D, const uint32_t *__restrict__ rowptr,
+                             const uint32_t *__restrict__ colidx, const void *vals,
+                             const void *vals_lo, int esize, void *out, void *out_lo,
+                             int *__restrict__ err) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D.n) return;
+  for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+    const int64_t o = (int64_t)colidx[k] - (int64_t)i;
+    int s = -1;
+    for (int q = 0; q < 27; ++q)
+      if (D.off[q] == o) s = q;
+    if (s < 0) {
+      atomicMax(err, 1);
+      continue;
+    }
+    const uint64_t dst = (uint64_t)s * D.n + i;
+    if (esize == 8)
+      ((double *)out)[dst] = ((const double *)vals)[k];
+    else
+      ((float *)out)[dst] = ((const float *)vals)[k];
+    if (out_lo) ((float *)out_lo)[dst] = ((const float *)vals_lo)[k];
+  }
+}
+
+// The matrix passes of one CG solve over a 27-point stencil matrix.
+template <int IP, int RP, int LP>
+struct StencilOps {
+  using VT = typename Val<IP>::T;
+  StDev A{}, C{};
+  unsigned g = 0;
+  unsigned parts_spmv() const { return g; }
+  unsigned parts_res() const { return g; }
+  unsigned parts_corr() const { return g; }
+  void spmv(hipStream_t s, const VT *p, VT *q, const CGState *S, const float *dlo,
+            double *part) const {
+    hipLaunchKernelGGL((k_cg_spmv_st<IP>), dim3(g), dim3(256), 0, s, A, p, q, S, dlo, part);
+  }
+  void residual(hipStream_t s, const double *b, const double *x, const EpiResidual<VT, 1> &epi,
+                double *part) const {
+    hipLaunchKernelGGL((k_residual_st<RP, VT>), dim3(g), dim3(256), 0, s, A, b, x, epi, part);
+  }
+  void corr(hipStream_t s, const VT *u, const VT *r, VT *z, const float *linv, const CGState *S,
+            int check_done, double *part) const {
+    hipLaunchKernelGGL((k_cg_lapply_st<LP>), dim3(g), dim3(256), 0, s, C, u, r, z, linv, S,
+                       check_done, part);
+  }
+  void lo_rowsum(hipStream_t s, float *dlo) const {
+    hipLaunchKernelGGL(k_lo_rowsum_st, dim3(grid_for(A.n, 256)), dim3(256), 0, s, A.n,
+                       (const float *)A.vals_lo, dlo);
+  }
+  void dinv(hipStream_t s, int, VT *out) const {
+    hipLaunchKernelGGL((k_dinv_st<RP, VT>), dim3(grid_for(A.n, 256)), dim3(256), 0, s, A.n,
+                       A.vals, A.vals_lo, out);
+  }
+};
+
+// the device view of a stencil matrix (offsets of the lexicographic
+// (d2, d1, d0) order, i.e. ascending)
+StDev st_dev(const smh_stencil27 *S, const void *vals, const void *vals_lo) {
+  StDev d{};
+  d.n = S->n0 * S->n1 * S->n2;
+  int q = 0;
+  for (int d2 = -1; d2 <= 1; ++d2)
+    for (int d1 = -1; d1 <= 1; ++d1)
+      for (int d0 = -1; d0 <= 1; ++d0)
+        d.off[q++] = d0 + (int32_t)S->n0 * d1 + (int32_t)(S->n0 * S->n1) * d2;
+  d.vals = vals;
+  d.vals_lo = vals_lo;
+  return d;
+}
+
+// The matrix passes of one CG solve over blocked-CSR row tiles: the inner
+// SpMV (precision IP), the true residual (RP), the learned correction pass
+// (LP) and the two per-row set-ups (row sums of the low parts, the Jacobi
+// diagonal).  cg_run drives the same state machine over any such policy.
+template <int IP, int RP, int LP, int BD>
+struct TileOps {
+  using VT = typename Val<IP>::T;
+  const smh_bsr *A = nullptr;
+  TileDev T{}, TR{}, TC{};
+  unsigned gt = 0, grt = 0, gc = 0;
+  unsigned parts_spmv() const { return gt; }
+  unsigned parts_res() const { return grt; }
+  unsigned parts_corr() const { return gc; }
+  void spmv(hipStream_t s, const VT *p, VT *q, const CGState *S, const float *dlo,
+            double *part) const {
+    hipLaunchKernelGGL((k_cg_spmv<IP, BD>), dim3(gt), dim3(tile_threads_host(BD)), 0, s, T, p,
+                       q, S, dlo, part);
+  }
+  void residual(hipStream_t s, const double *b, const double *x, const EpiResidual<VT, BD> &epi,
+                double *part) const {
+    hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, s, TR,
+                       b, x, epi, part);
+  }
+  void corr(hipStream_t s, const VT *u, const VT *r, VT *z, const float *linv, const CGState *S,
+            int check_done, double *part) const {
+    hipLaunchKernelGGL((k_cg_lapply<LP, BD>), dim3(gc), dim3(tile_threads_host(BD)), 0, s, TC,
+                       u, r, z, linv, S, check_done, part);
+  }
+  void lo_rowsum(hipStream_t s, float *dlo) const {
+    hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for((uint64_t)A->nrows * BD * BD, 256)), dim3(256),
+                       0, s, A->nrows, BD * BD, A->rowptr, (const float *)A->vals_lo, dlo);
+  }
+  void dinv(hipStream_t s, int pc, VT *out) const {
+    hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(A->nrows, 256)), dim3(256), 0, s,
+                       A->nrows, A->rowptr, A->colidx, A->vals, A->vals_lo, BD, pc, out);
+  }
+};
+
+// the tile policy of A: the correction pass runs its precision's tile shape
+// (one block per lane for FP8 3x3), so it may need a plan of its own
+template <int IP, int RP, int BD>
+int make_tile_ops(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan,
+                  const smh_tile_plan *rplan, const smh_cg_opts *o, void *out) {
+  using VT = typename Val<IP>::T;
   constexpr int LP = BD == 3 ? (sizeof(VT) == 4 ? kFP8F : kFP8D)
                              : (sizeof(VT) == 4 ? SMH_F32 : kF32D);
-  smh_bsr Cb = *Am;
-  Cb.vals = learned ? (const void *)o->learned_corr : Am->vals;
+  constexpr bool kIl = BD == 1 && SMH_TILE_INTERLEAVE1;
+  TileOps<IP, RP, LP, BD> &ops = *(TileOps<IP, RP, LP, BD> *)out;
+  ops.A = A;
+  ops.T = tile_dev(A, plan, &ops.gt, kIl);
+  ops.TR = tile_dev(A, rplan, &ops.grt, kIl);
+  smh_bsr Cb = *A;
+  const bool learned = o->precond == SMH_PRECOND_LEARNED;
+  Cb.vals = learned ? (const void *)o->learned_corr : A->vals;
   Cb.vals_lo = nullptr;
-  // the correction pass runs its precision's tile shape (one block per lane
-  // for FP8 3x3), so it may need a plan of its own
   smh_tile_plan lplan = *plan;
   if (learned && tile_blocks_of<LP, BD>() != plan->tile_blocks) {
-    const int prc = tile_plan(ctx, Am->rowptr, nrows, Am->nnz, tile_blocks_of<LP, BD>(), &lplan);
+    const int prc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_of<LP, BD>(), &lplan);
     if (prc) return prc;
     SMH_CHECK(ctx, lplan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
               "smh_cg_solve: a row holds %u blocks (the learned correction supports %u)",
               lplan.maxrow, tile_blocks_of<LP, BD>() / 2);
   }
-  unsigned gc = 0;
-  const TileDev TC = tile_dev(&Cb, &lplan, &gc, kIl);
+  ops.TC = tile_dev(&Cb, &lplan, &ops.gc, kIl);
+  return SMH_OK;
+}
+
+template <int IP, int RP, int BD, class OPS>
+int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const double *b, double *x,
+           const smh_cg_opts *o, smh_cg_stats *st) {
+  using VT = typename Val<IP>::T;
+  const uint64_t n = (uint64_t)nrows * BD;
+  const unsigned nbe = grid_for(nrows, 256, 2048);
+  const unsigned nbv = grid_for(n, 256, 2048);
+  const int pc = o->precond;
+  const bool learned = pc == SMH_PRECOND_LEARNED;
+  const unsigned gt = ops.parts_spmv(), grt = ops.parts_res(), gc = ops.parts_corr();
   const int check = o->check_every > 0 ? o->check_every : 16;
   const void *pdata = nullptr;  // the preconditioner data K2 reads
   Work w;
@@ -677,10 +936,9 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
     w.dinv = nullptr;
   }
   // mixed precision: the inner operator keeps the exact block row sums
-  if (o->mixed && Am->precision == SMH_F32X2 && Am->vals_lo) {
+  if (o->mixed && has_lo) {
     TRY(dmalloc(ctx, &w.dlo, sizeof(float) * (uint64_t)nrows * BD * BD));
-    hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for((uint64_t)nrows * BD * BD, 256)), dim3(256), 0, ctx->stream, nrows,
-                       BD * BD, Am->rowptr, (const float *)Am->vals_lo, (float *)w.dlo);
+    ops.lo_rowsum(ctx->stream, (float *)w.dlo);
   }
   pdata = learned ? (const void *)o->learned : (const void *)w.dinv;
   TRY(dmalloc(ctx, (void **)&w.S, sizeof(CGState)));
@@ -693,10 +951,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
                      o->norm_form, o->maxiters, (pc != SMH_PRECOND_NONE && o->zr_stop) ? 1 : 0,
                      o->reliable_delta, o->mixed, o->replace_every,
                      (const double *)(w.part + 2 * kPartSlots));
-  if (pc != SMH_PRECOND_NONE && !learned)
-    hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream,
-                       nrows, Am->rowptr, Am->colidx, Am->vals, Am->vals_lo, BD, pc,
-                       (VT *)w.dinv);
+  if (pc != SMH_PRECOND_NONE && !learned) ops.dinv(ctx->stream, pc, (VT *)w.dinv);
   epi.b = b;
   epi.r = (VT *)w.r;
   epi.z = (VT *)(learned ? w.u : w.z);  // learned: u = Linv r, z follows in k_cg_lapply
@@ -705,12 +960,10 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
   epi.pc = pc;
   // r = b - A x, z = M r, p = z
   epi.init = learned ? 0 : 1;
-  hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
-                     (const double *)x, epi, w.part);
+  ops.residual(ctx->stream, b, (const double *)x, epi, w.part);
   if (learned) {
-    hipLaunchKernelGGL((k_cg_lapply<LP, BD>), dim3(gc), dim3(tile_threads_host(BD)), 0, ctx->stream, TC,
-                       (const VT *)w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
-                       (const CGState *)w.S, 0, w.part);
+    ops.corr(ctx->stream, (const VT *)w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
+             (const CGState *)w.S, 0, w.part);
     hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
                        (const double *)w.part, (int)gc, 0);
     hipLaunchKernelGGL((k_cg_restart<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
@@ -732,12 +985,10 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
     if (hs.status == SMH_OK && (hs.want_ru || hs.claimed)) {
       ++nru;
       batch = std::max(2, batch / 4);
-      hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
-                         (const double *)x, epi, w.part);
+      ops.residual(ctx->stream, b, (const double *)x, epi, w.part);
       if (learned)
-        hipLaunchKernelGGL((k_cg_lapply<LP, BD>), dim3(gc), dim3(tile_threads_host(BD)), 0,
-                           ctx->stream, TC, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
-                           (const float *)o->learned, (const CGState *)w.S, 0, w.part);
+        ops.corr(ctx->stream, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
+                 (const float *)o->learned, (const CGState *)w.S, 0, w.part);
       hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
                          (const double *)w.part, (int)(learned ? gc : grt), 0);
       hipLaunchKernelGGL((k_cg_restart<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
@@ -747,19 +998,16 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
     }
     if (hs.done || hs.status != SMH_OK) break;
     for (int k = 0; k < batch; ++k) {
-      hipLaunchKernelGGL((k_cg_spmv<IP, BD>), dim3(gt), dim3(tile_threads_host(BD)), 0, ctx->stream, T,
-                         (const VT *)w.p, (VT *)w.q, (const CGState *)w.S,
-                         (const float *)w.dlo, w.part);
+      ops.spmv(ctx->stream, (const VT *)w.p, (VT *)w.q, (const CGState *)w.S,
+               (const float *)w.dlo, w.part);
       hipLaunchKernelGGL(k_cg_r1, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
                          (int)gt);
       hipLaunchKernelGGL((k_cg_update<VT, BD>), dim3(nbe), dim3(256), 0, ctx->stream, nrows, pc,
                          pdata, x, (VT *)w.r, (VT *)(learned ? w.u : w.z), (const VT *)w.p,
                          (const VT *)w.q, (const CGState *)w.S, w.part + kPartSlots);
       if (learned)
-        hipLaunchKernelGGL((k_cg_lapply<LP, BD>), dim3(gc), dim3(tile_threads_host(BD)), 0,
-                           ctx->stream, TC, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
-                           (const float *)o->learned, (const CGState *)w.S, 1,
-                           w.part + kPartSlots);
+        ops.corr(ctx->stream, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
+                 (const float *)o->learned, (const CGState *)w.S, 1, w.part + kPartSlots);
       hipLaunchKernelGGL(k_cg_r2, dim3(1), dim3(256), 0, ctx->stream, w.S,
                          (const double *)(w.part + kPartSlots), (int)(learned ? gc : nbe));
       hipLaunchKernelGGL((k_cg_pupd<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
@@ -769,8 +1017,7 @@ int cg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, const smh
     batch = std::min(check, 2 * batch);
   }
   // certify: FP64 residual of the final x (r, z are scratch from here on)
-  hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, ctx->stream, TR, b,
-                     (const double *)x, epi, w.part);
+  ops.residual(ctx->stream, b, (const double *)x, epi, w.part);
   hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
                      (int)grt, 1);
   LCHECK();
@@ -1140,6 +1387,20 @@ __global__ void __launch_bounds__(256) k_bcg_init(BChunks C, const double *cp, B
   S.restart[m] = 1;
   S.rz[m] = S.rr[m] = S.rr_ref[m] = S.rr_true[m] = 0.0;
   S.alpha[m] = S.beta[m] = 0.0;
+}
+
+// CG over the blocked-CSR row tiles of A
+template <int IP, int RP, int BD>
+int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const smh_tile_plan *rplan,
+             const double *b, double *x, const smh_cg_opts *o, smh_cg_stats *st) {
+  using VT = typename Val<IP>::T;
+  constexpr int LP = BD == 3 ? (sizeof(VT) == 4 ? kFP8F : kFP8D)
+                             : (sizeof(VT) == 4 ? SMH_F32 : kF32D);
+  TileOps<IP, RP, LP, BD> ops;
+  const int rc = make_tile_ops<IP, RP, BD>(ctx, A, plan, rplan, o, &ops);
+  if (rc) return rc;
+  return cg_run<IP, RP, BD>(ctx, ops, A->nrows, A->precision == SMH_F32X2 && A->vals_lo, b, x, o,
+                            st);
 }
 
 // the SpMV grid of a pattern: the fastest of a few candidates, timed once
@@ -1554,9 +1815,8 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
   SMH_CHECK(ctx, plan.tile_row != nullptr && rplan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
             "smh_cg_solve: a row holds %u blocks (at most %u supported)", plan.maxrow,
             tile_blocks_host(bd) / 2);
-#define SMH_CG(IP, RP)                                                                  \
-  (A->bdim == 3 ? cg_run<IP, RP, 3>(ctx, A, &plan, &rplan, b, x, o, st)                 \
-                : cg_run<IP, RP, 1>(ctx, A, &plan, &rplan, b, x, o, st))
+#define SMH_CG(IP, RP) (A->bdim == 3 ? cg_tiles<IP, RP, 3>(ctx, A, &plan, &rplan, b, x, o, st) \
+                                      : cg_tiles<IP, RP, 1>(ctx, A, &plan, &rplan, b, x, o, st))
   if (o->mixed == 2) rc = SMH_CG(kF32D, SMH_F32X2);
   else if (o->mixed) rc = SMH_CG(SMH_F32, SMH_F32X2);
   else if (A->precision == SMH_F64) rc = SMH_CG(SMH_F64, SMH_F64);
@@ -1608,5 +1868,124 @@ extern "C" int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nba
   else if (A->precision == SMH_F32X2) rc = SMH_BCG(SMH_F32X2, SMH_F32X2);
   else rc = SMH_BCG(kF32D, kF32D);
 #undef SMH_BCG
+  return rc;
+}
+
+// ---- scalar 27-point stencil storage: conversion, product and CG -------------------------
+namespace {
+int check_stencil(smh_ctx *ctx, const smh_stencil27 *S, const char *who) {
+  SMH_CHECK(ctx, S && S->vals, SMH_ERR_INVALID, "%s: null matrix", who);
+  SMH_CHECK(ctx, S->n0 >= 1 && S->n1 >= 1 && S->n2 >= 1, SMH_ERR_INVALID, "%s: empty grid", who);
+  SMH_CHECK(ctx, (uint64_t)S->n0 * S->n1 * S->n2 < (1ull << 31), SMH_ERR_UNSUPPORTED,
+            "%s: grid too large", who);
+  SMH_CHECK(ctx, S->precision >= SMH_F64 && S->precision <= SMH_F32D, SMH_ERR_INVALID,
+            "%s: unknown precision %d", who, S->precision);
+  SMH_CHECK(ctx, S->precision != SMH_F32X2 || S->vals_lo, SMH_ERR_INVALID,
+            "%s: SMH_F32X2 needs vals_lo", who);
+  return SMH_OK;
+}
+}  // namespace
+
+extern "C" int smh_stencil27_from_bsr(smh_ctx *ctx, const smh_bsr *A, uint32_t n0, uint32_t n1,
+                                      uint32_t n2, void *vals, void *vals_lo) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, A && A->rowptr && vals, SMH_ERR_INVALID, "smh_stencil27_from_bsr: null argument");
+  SMH_CHECK(ctx, A->bdim == 1, SMH_ERR_UNSUPPORTED,
+            "smh_stencil27_from_bsr: 1x1 blocks only (got %u)", A->bdim);
+  SMH_CHECK(ctx, (uint64_t)n0 * n1 * n2 == A->nrows, SMH_ERR_INVALID,
+            "smh_stencil27_from_bsr: %u x %u x %u grid for %u rows", n0, n1, n2, A->nrows);
+  SMH_CHECK(ctx, A->precision >= SMH_F64 && A->precision <= SMH_F32D, SMH_ERR_INVALID,
+            "smh_stencil27_from_bsr: unknown precision %d", A->precision);
+  SMH_CHECK(ctx, A->nnz == 0 || (A->colidx && A->vals), SMH_ERR_INVALID,
+            "smh_stencil27_from_bsr: null values");
+  const bool lo = A->precision == SMH_F32X2;
+  SMH_CHECK(ctx, !lo || (A->vals_lo && vals_lo), SMH_ERR_INVALID,
+            "smh_stencil27_from_bsr: SMH_F32X2 needs vals_lo in and out");
+  if (A->nrows == 0) return SMH_OK;
+  smh_stencil27 S{n0, n1, n2, A->precision, vals, vals_lo};
+  const StDev D = st_dev(&S, vals, vals_lo);
+  const int esize = A->precision == SMH_F64 ? 8 : 4;
+  const uint64_t words = 27ull * A->nrows * (esize / 4);
+  hipLaunchKernelGGL(k_st_fill, dim3(grid_for(words, 256, 8192)), dim3(256), 0, ctx->stream, words,
+                     (uint32_t *)vals);
+  if (lo)
+    hipLaunchKernelGGL(k_st_fill, dim3(grid_for(27ull * A->nrows, 256, 8192)), dim3(256), 0,
+                       ctx->stream, 27ull * A->nrows, (uint32_t *)vals_lo);
+  int *err = (int *)ctx->red_count;
+  SMH_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_st_scatter, dim3(grid_for(A->nrows, 256)), dim3(256), 0, ctx->stream, D,
+                     A->rowptr, A->colidx, A->vals, A->vals_lo, esize, vals,
+                     lo ? vals_lo : nullptr, err);
+  SMH_LAUNCH_CHECK(ctx);
+  int herr = 0;
+  SMH_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SMH_CHECK(ctx, herr == 0, SMH_ERR_INVALID,
+            "smh_stencil27_from_bsr: a column lies outside the 27-point stencil of its row");
+  return SMH_OK;
+}
+
+extern "C" int smh_stencil27_mv(smh_ctx *ctx, const smh_stencil27 *S, const void *x, void *y) {
+  if (!ctx) return SMH_ERR_INVALID;
+  int rc = check_stencil(ctx, S, "smh_stencil27_mv");
+  if (rc) return rc;
+  SMH_CHECK(ctx, x && y, SMH_ERR_INVALID, "smh_stencil27_mv: null vector");
+  const StDev D = st_dev(S, S->vals, S->vals_lo);
+  const unsigned g = grid_for(D.n, 256, 4096);
+  if (S->precision == SMH_F64)
+    hipLaunchKernelGGL((k_stmv<SMH_F64>), dim3(g), dim3(256), 0, ctx->stream, D,
+                       (const double *)x, (double *)y);
+  else if (S->precision == SMH_F32)
+    hipLaunchKernelGGL((k_stmv<SMH_F32>), dim3(g), dim3(256), 0, ctx->stream, D,
+                       (const float *)x, (float *)y);
+  else if (S->precision == SMH_F32D)
+    hipLaunchKernelGGL((k_stmv<kF32D>), dim3(g), dim3(256), 0, ctx->stream, D,
+                       (const double *)x, (double *)y);
+  else
+    hipLaunchKernelGGL((k_stmv<SMH_F32X2>), dim3(g), dim3(256), 0, ctx->stream, D,
+                       (const double *)x, (double *)y);
+  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
+}
+
+namespace {
+template <int IP, int RP>
+int cg_stencil(smh_ctx *ctx, const smh_stencil27 *S, const double *b, double *x,
+               const smh_cg_opts *o, smh_cg_stats *st) {
+  using VT = typename Val<IP>::T;
+  constexpr int LP = sizeof(VT) == 4 ? SMH_F32 : kF32D;
+  StencilOps<IP, RP, LP> ops;
+  ops.A = st_dev(S, S->vals, S->vals_lo);
+  ops.C = st_dev(S, o->precond == SMH_PRECOND_LEARNED ? o->learned_corr : S->vals, nullptr);
+  ops.g = grid_for(ops.A.n, 256, 4096);
+  return cg_run<IP, RP, 1>(ctx, ops, ops.A.n, S->precision == SMH_F32X2 && S->vals_lo, b, x, o,
+                           st);
+}
+}  // namespace
+
+extern "C" int smh_cg_solve_stencil27(smh_ctx *ctx, const smh_stencil27 *S, const double *b,
+                                      double *x, const smh_cg_opts *o, smh_cg_stats *st) {
+  if (!ctx) return SMH_ERR_INVALID;
+  int rc = check_stencil(ctx, S, "smh_cg_solve_stencil27");
+  if (rc) return rc;
+  SMH_CHECK(ctx, b && x && o, SMH_ERR_INVALID, "smh_cg_solve_stencil27: null argument");
+  SMH_CHECK(ctx, o->precond >= 0 && o->precond <= 3, SMH_ERR_INVALID,
+            "smh_cg_solve_stencil27: unknown preconditioner %d", o->precond);
+  SMH_CHECK(ctx,
+            o->precond != SMH_PRECOND_LEARNED ||
+                (o->learned && o->learned_corr && ((uintptr_t)o->learned_corr & 3u) == 0),
+            SMH_ERR_INVALID, "smh_cg_solve_stencil27: the learned preconditioner needs "
+                             "opts.learned (float[n]) and opts.learned_corr (float[27 n], "
+                             "smh_stencil27_from_bsr of the correction)");
+  SMH_CHECK(ctx, !o->mixed || S->precision == SMH_F32X2, SMH_ERR_INVALID,
+            "smh_cg_solve_stencil27: mixed precision needs an SMH_F32X2 matrix");
+  SMH_CHECK(ctx, o->maxiters >= 0, SMH_ERR_INVALID, "smh_cg_solve_stencil27: negative maxiters");
+  SMH_CHECK(ctx, o->mixed >= 0 && o->mixed <= 2, SMH_ERR_INVALID,
+            "smh_cg_solve_stencil27: mixed must be 0, 1 or 2 (got %d)", o->mixed);
+  if (o->mixed == 2) rc = cg_stencil<kF32D, SMH_F32X2>(ctx, S, b, x, o, st);
+  else if (o->mixed) rc = cg_stencil<SMH_F32, SMH_F32X2>(ctx, S, b, x, o, st);
+  else if (S->precision == SMH_F64) rc = cg_stencil<SMH_F64, SMH_F64>(ctx, S, b, x, o, st);
+  else if (S->precision == SMH_F32X2) rc = cg_stencil<SMH_F32X2, SMH_F32X2>(ctx, S, b, x, o, st);
+  else rc = cg_stencil<kF32D, kF32D>(ctx, S, b, x, o, st);
   return rc;
 }

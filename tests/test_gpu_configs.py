@@ -7,8 +7,9 @@ C2  tets 64^3 x 5, log10 E ~ N(5, 0.5) i.i.d., mixed Jacobi-PCG to 1e-8: FP64
 C3  256 spring lattices n in 8..36 solved together (batched mixed CG): every
     lattice's residual re-checked on the oracle's matrix.
 C5  hex Poisson 192^3: size-independent properties of the full assembly
-    (symmetry, zero row sums away from the Dirichlet face) and a certified mixed
-    Jacobi-PCG solve.
+    (symmetry, zero row sums away from the Dirichlet face, the stencil storage's
+    product equal to the CSR one bit for bit) and certified mixed Jacobi-PCG
+    solves over both storages.
 (C4, tets 96^3 x 5, is covered at full size by test_gpu_assembly and
 test_gpu_index, and solved by bench.py.)
 """
@@ -136,7 +137,12 @@ def test_c5_hex192_properties_and_solve(ctx):
     inner = A1[xi >= 2].abs().max().item()
     diag_scale = (v64.abs().max().item())
     assert inner <= 1e-12 * diag_scale, (inner, diag_scale)
-    del v64, A
+    # the 27-point stencil storage of the same matrix: the product bit for bit
+    St = ctx.stencil27_from_bsr(A, (N1, N1, N1))
+    Su = torch.empty_like(u)
+    ctx.stencil27_mv(St, u, Su)
+    assert torch.equal(Su, Au)
+    del v64, A, St, Su
     hi = torch.empty(idx.nnz, dtype=torch.float32, device="cuda")
     lo = torch.empty_like(hi)
     ctx.assemble_hex(idx, HP, d, d, S.F32X2, hi, lo, rhs)
@@ -146,3 +152,11 @@ def test_c5_hex192_properties_and_solve(ctx):
                                mixed=1, check_every=64))
     print(st.as_dict(), flush=True)
     assert st.status == 0 and st.true_rel <= 1e-8, st.as_dict()
+    # and over the stencil storage (FP64 vectors, the benchmark's solver)
+    St = ctx.stencil27_from_bsr(idx.bsr(1, S.F32X2, hi, lo), (N1, N1, N1))
+    x2 = torch.zeros_like(rhs)
+    s2 = ctx.cg_stencil27(St, rhs, x2, ctx.cg_options(maxiters=40000, tol=1e-8, rel=1,
+                                                      precond=S.PRECOND_JACOBI, mixed=2,
+                                                      check_every=64))
+    print(s2.as_dict(), flush=True)
+    assert s2.status == 0 and s2.true_rel <= 1e-8, s2.as_dict()
