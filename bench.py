@@ -420,13 +420,13 @@ def run_hex(args, ctx):
     def opts(**kw):
         return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
 
-    corr_st = torch.empty(27 * nv, dtype=torch.float32, device="cuda")
+    corr_st = torch.empty(27 * nv, dtype=torch.int16, device="cuda")  # FP16 slots
 
     def step():
         assemble()
         ctx.stencil27_from_bsr(A2, dims, st_hi, st_lo)
         linv, corr = net.precond(A2)
-        ctx.stencil27_from_bsr(idx.bsr(1, S.F32, corr.view(torch.float32)), dims, corr_st)
+        ctx.stencil27_correction(idx, corr, dims, corr_st)
         x = torch.zeros(nv, dtype=torch.float64, device="cuda")
         return ctx.cg_stencil27(St, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=2,
                                                  learned=linv, learned_corr=corr_st))

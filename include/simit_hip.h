@@ -313,16 +313,21 @@ typedef struct {
 } smh_stencil27;
 /* scatter a 1x1 blocked-CSR matrix (any precision; SMH_F32X2 copies both
  * halves) into stencil storage; SMH_ERR_INVALID if a column lies outside its
- * row's 27 offsets.  Also converts the float[nnz] learned correction of
- * smh_gnn_precond (pass it as vals of an SMH_F32 matrix) to the float[27*n]
- * opts.learned_corr of smh_cg_solve_stencil27. */
+ * row's 27 offsets. */
 int smh_stencil27_from_bsr(smh_ctx *ctx, const smh_bsr *A, uint32_t n0, uint32_t n1, uint32_t n2,
                            void *vals, void *vals_lo);
+/* the float[nnz] learned correction of smh_gnn_precond (pass it as the
+ * values of an SMH_F32 1x1 matrix with A's pattern) as IEEE half stencil
+ * slots, out = device uint16[27*n] (4-byte aligned): the opts.learned_corr
+ * of smh_cg_solve_stencil27 */
+int smh_stencil27_correction(smh_ctx *ctx, const smh_bsr *C, uint32_t n0, uint32_t n1, uint32_t n2,
+                             void *out);
 /* y = A*x (vector types as smh_bsrmv) */
 int smh_stencil27_mv(smh_ctx *ctx, const smh_stencil27 *A, const void *x, void *y);
 /* smh_cg_solve over stencil storage: same options, state machine and
  * certification; preconditioners none, Jacobi (block Jacobi = Jacobi for
- * 1x1) and learned (opts.learned float[n], opts.learned_corr float[27*n]) */
+ * 1x1) and learned (opts.learned float[n], opts.learned_corr FP16 [27*n]
+ * from smh_stencil27_correction) */
 int smh_cg_solve_stencil27(smh_ctx *ctx, const smh_stencil27 *A, const double *b, double *x,
                            const smh_cg_opts *opts, smh_cg_stats *stats);
 

@@ -167,6 +167,7 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_bsrmv": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, i32]),
         "smh_dot": (C.c_int, [vp, i32, u64, vp, vp, C.POINTER(C.c_double)]),
         "smh_stencil27_from_bsr": (C.c_int, [vp, C.POINTER(Bsr), u32, u32, u32, vp, vp]),
+        "smh_stencil27_correction": (C.c_int, [vp, C.POINTER(Bsr), u32, u32, u32, vp]),
         "smh_stencil27_mv": (C.c_int, [vp, C.POINTER(Stencil27), vp, vp]),
         "smh_cg_solve_stencil27": (C.c_int, [vp, C.POINTER(Stencil27), vp, vp,
                                              C.POINTER(CGOptions), C.POINTER(CGStats)]),
@@ -331,6 +332,18 @@ class Context:
         S = Stencil27(n0, n1, n2, A.precision, _ptr(vals), _ptr(vals_lo))
         S._keep = (vals, vals_lo)
         return S
+
+    def stencil27_correction(self, idx, corr, dims, out=None):
+        """The float[nnz] learned correction of Gnn.precond (1x1, A's pattern
+        idx) as FP16 stencil slots (uint16[27*n]) for cg_stencil27."""
+        import torch
+        n0, n1, n2 = (int(d) for d in dims)
+        if out is None:
+            out = torch.empty(27 * n0 * n1 * n2, dtype=torch.int16, device="cuda")
+        Cb = idx.bsr(1, F32, corr.view(torch.float32))
+        self._check(self.lib.smh_stencil27_correction(self.h, C.byref(Cb), n0, n1, n2,
+                                                      _ptr(out)))
+        return out
 
     def stencil27_mv(self, S: Stencil27, x, y):
         self._check(self.lib.smh_stencil27_mv(self.h, C.byref(S), _ptr(x), _ptr(y)))

@@ -629,10 +629,12 @@ struct StDev {
 // the result is the blocked-CSR product bit for bit (zero slots add exact
 // zeros; out-of-range columns read a clamped x times a zero value).
 //
-// A plain grid-stride loop over the rows.  (Giving each XCD a contiguous
-// eighth of the rows, so that its x window with the +-n0*n1 halo stays in its
-// L2, measured slower on the 192^3 hex system: 226 us against 212 us; the
-// re-reads are served by the Infinity Cache.)
+// A plain grid-stride loop over the rows with plain loads.  Measured on the
+// 192^3 hex system and slower than it: giving each XCD a contiguous eighth
+// of the rows (226 us against 212 us), staging the nine x pencils of a
+// 256-row block in LDS (9 global loads per row instead of 27 gathers; no
+// faster, the solver iteration slower) and non-temporal value loads
+// (239 us).
 template <int PREC, class XG, class EPI>
 __device__ __forceinline__ void stencil_rows(const StDev &A, const XG &xg, EPI &epi) {
 #pragma clang fp contract(off)
@@ -756,6 +758,11 @@ __global__ void k_st_fill(uint64_t count, uint32_t *__restrict__ out) {
        t += (uint64_t)gridDim.x * blockDim.x)
     out[t] = 0u;
 }
+__global__ void k_st_fill16(uint64_t count, uint16_t *__restrict__ out) {
+  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
+       t += (uint64_t)gridDim.x * blockDim.x)
+    out[t] = 0u;
+}
 __global__ void k_st_scatter(StDev D, const uint32_t *__restrict__ rowptr,
                              const uint32_t *__restrict__ colidx, const void *vals,
                              const void *vals_lo, int esize, void *out, void *out_lo,
@@ -774,6 +781,8 @@ __global__ void k_st_scatter(StDev D, const uint32_t *__restrict__ rowptr,
     const uint64_t dst = (uint64_t)s * D.n + i;
     if (esize == 8)
       ((double *)out)[dst] = ((const double *)vals)[k];
+    else if (esize == 2)  // float in, IEEE half out (round to nearest even)
+      ((uint16_t *)out)[dst] = __builtin_bit_cast(uint16_t, (_Float16)((const float *)vals)[k]);
     else
       ((float *)out)[dst] = ((const float *)vals)[k];
     if (out_lo) ((float *)out_lo)[dst] = ((const float *)vals_lo)[k];
@@ -1925,6 +1934,38 @@ extern "C" int smh_stencil27_from_bsr(smh_ctx *ctx, const smh_bsr *A, uint32_t n
   return SMH_OK;
 }
 
+extern "C" int smh_stencil27_correction(smh_ctx *ctx, const smh_bsr *C, uint32_t n0,
+                                        uint32_t n1, uint32_t n2, void *out) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, C && C->rowptr && out, SMH_ERR_INVALID,
+            "smh_stencil27_correction: null argument");
+  SMH_CHECK(ctx, C->bdim == 1 && C->precision == SMH_F32, SMH_ERR_INVALID,
+            "smh_stencil27_correction: the correction is a 1x1 SMH_F32 matrix");
+  SMH_CHECK(ctx, (uint64_t)n0 * n1 * n2 == C->nrows, SMH_ERR_INVALID,
+            "smh_stencil27_correction: %u x %u x %u grid for %u rows", n0, n1, n2, C->nrows);
+  SMH_CHECK(ctx, ((uintptr_t)out & 3u) == 0, SMH_ERR_INVALID,
+            "smh_stencil27_correction: out must be 4-byte aligned");
+  if (C->nrows == 0) return SMH_OK;
+  SMH_CHECK(ctx, C->nnz == 0 || (C->colidx && C->vals), SMH_ERR_INVALID,
+            "smh_stencil27_correction: null values");
+  smh_stencil27 S{n0, n1, n2, SMH_F32, out, nullptr};
+  const StDev D = st_dev(&S, out, nullptr);
+  hipLaunchKernelGGL(k_st_fill16, dim3(grid_for(27ull * C->nrows, 256, 8192)), dim3(256), 0,
+                     ctx->stream, 27ull * C->nrows, (uint16_t *)out);
+  int *err = (int *)ctx->red_count;
+  SMH_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
+  hipLaunchKernelGGL(k_st_scatter, dim3(grid_for(C->nrows, 256)), dim3(256), 0, ctx->stream, D,
+                     C->rowptr, C->colidx, C->vals, (const void *)nullptr, 2, out, (void *)nullptr,
+                     err);
+  SMH_LAUNCH_CHECK(ctx);
+  int herr = 0;
+  SMH_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SMH_CHECK(ctx, herr == 0, SMH_ERR_INVALID,
+            "smh_stencil27_correction: a column lies outside the 27-point stencil of its row");
+  return SMH_OK;
+}
+
 extern "C" int smh_stencil27_mv(smh_ctx *ctx, const smh_stencil27 *S, const void *x, void *y) {
   if (!ctx) return SMH_ERR_INVALID;
   int rc = check_stencil(ctx, S, "smh_stencil27_mv");
@@ -1953,7 +1994,7 @@ template <int IP, int RP>
 int cg_stencil(smh_ctx *ctx, const smh_stencil27 *S, const double *b, double *x,
                const smh_cg_opts *o, smh_cg_stats *st) {
   using VT = typename Val<IP>::T;
-  constexpr int LP = sizeof(VT) == 4 ? SMH_F32 : kF32D;
+  constexpr int LP = sizeof(VT) == 4 ? kF16F : kF16D;  // the correction's FP16 slots
   StencilOps<IP, RP, LP> ops;
   ops.A = st_dev(S, S->vals, S->vals_lo);
   ops.C = st_dev(S, o->precond == SMH_PRECOND_LEARNED ? o->learned_corr : S->vals, nullptr);
@@ -1975,8 +2016,8 @@ extern "C" int smh_cg_solve_stencil27(smh_ctx *ctx, const smh_stencil27 *S, cons
             o->precond != SMH_PRECOND_LEARNED ||
                 (o->learned && o->learned_corr && ((uintptr_t)o->learned_corr & 3u) == 0),
             SMH_ERR_INVALID, "smh_cg_solve_stencil27: the learned preconditioner needs "
-                             "opts.learned (float[n]) and opts.learned_corr (float[27 n], "
-                             "smh_stencil27_from_bsr of the correction)");
+                             "opts.learned (float[n]) and opts.learned_corr (FP16 [27 n], "
+                             "smh_stencil27_correction)");
   SMH_CHECK(ctx, !o->mixed || S->precision == SMH_F32X2, SMH_ERR_INVALID,
             "smh_cg_solve_stencil27: mixed precision needs an SMH_F32X2 matrix");
   SMH_CHECK(ctx, o->maxiters >= 0, SMH_ERR_INVALID, "smh_cg_solve_stencil27: negative maxiters");

@@ -257,10 +257,31 @@ __device__ __forceinline__ float e4m3_scaled(uint32_t b) {
   return __uint_as_float(((b & 0x80u) << 24) | ((b & 0x7Fu) << 20)) * (0x1p120f * kFP8Scale);
 }
 
+// IEEE half values (bits in uint16): the learned correction in stencil
+// storage, read by one thread per row as 2-byte loads
+constexpr int kF16F = 8;  // FP16 values, float vectors
+constexpr int kF16D = 9;  // FP16 values, double vectors
+template <> struct Val<kF16F> {
+  using T = float;
+  using S = uint16_t;
+  static constexpr bool kLo = false;
+  static constexpr bool kBF16 = false;
+  static constexpr bool kFP8 = false;
+};
+template <> struct Val<kF16D> {
+  using T = double;
+  using S = uint16_t;
+  static constexpr bool kLo = false;
+  static constexpr bool kBF16 = false;
+  static constexpr bool kFP8 = false;
+};
+
 // stored value -> compute type
 template <int PREC>
 __device__ __forceinline__ typename Val<PREC>::T val_of(typename Val<PREC>::S v) {
-  if constexpr (Val<PREC>::kBF16)
+  if constexpr (PREC == kF16F || PREC == kF16D)
+    return (typename Val<PREC>::T)(float)__builtin_bit_cast(_Float16, (uint16_t)v);
+  else if constexpr (Val<PREC>::kBF16)
     return (typename Val<PREC>::T)__uint_as_float((uint32_t)v << 16);
   else
     return (typename Val<PREC>::T)v;

@@ -125,7 +125,9 @@ def test_stencil_cg_matches_csr(ctx, oracle, mixed):
 
 
 @pytest.mark.parametrize("mixed", [0, 1, 2])
-def test_stencil_learned_matches_csr(ctx, mixed):
+def test_stencil_learned_correction(ctx, mixed):
+    # the correction in FP16 stencil slots: one iteration from 0 gives
+    # x1 = alpha B b with B = Linv (I + C16) Linv, C16 the half-rounded C
     sy = hex_system(ctx, 9, seed=3)
     idx = sy["idx"]
     A = idx.bsr(1, S.F32X2, sy["hi"], sy["lo"]) if mixed else idx.bsr(1, S.F64, sy["v64"])
@@ -133,22 +135,34 @@ def test_stencil_learned_matches_csr(ctx, mixed):
     p = G.init_params(5, dec_scale=0.5)
     net.set_params(G.flatten(p).astype(np.float32))
     linv, corr = net.precond(A)
-    Cs = ctx.stencil27_from_bsr(idx.bsr(1, S.F32, corr.view(torch.float32)), sy["dims"])
+    c16 = ctx.stencil27_correction(idx, corr, sy["dims"])
     St = ctx.stencil27_from_bsr(A, sy["dims"])
-    # one iteration from 0: x1 = alpha B b on both storages
-    o1 = ctx.cg_options(maxiters=1, tol=1e-30, rel=1, precond=S.PRECOND_LEARNED, mixed=mixed,
-                        learned=linv, learned_corr=corr)
-    o2 = ctx.cg_options(maxiters=1, tol=1e-30, rel=1, precond=S.PRECOND_LEARNED, mixed=mixed,
-                        learned=linv, learned_corr=Cs._keep[0])
-    x1, x2 = torch.zeros_like(sy["rhs"]), torch.zeros_like(sy["rhs"])
-    ctx.cg(A, sy["rhs"], x1, o1)
-    ctx.cg_stencil27(St, sy["rhs"], x2, o2)
-    tol = 1e-6 if mixed == 1 else 1e-12
-    assert ((x1 - x2).norm() / x1.norm()).item() <= tol
+    o = ctx.cg_options(maxiters=1, tol=1e-30, rel=1, precond=S.PRECOND_LEARNED, mixed=mixed,
+                       learned=linv, learned_corr=c16)
+    x = torch.zeros_like(sy["rhs"])
+    ctx.cg_stencil27(St, sy["rhs"], x, o)
+    rp, ci = idx.to_host()
+    n = rp.size - 1
+    cv = corr.view(torch.float32).cpu().numpy().astype(np.float16).astype(np.float64)
+    Li = linv.cpu().numpy().astype(np.float64)
+    brow = np.repeat(np.arange(n), np.diff(rp.astype(np.int64)))
+    Bk = Li[brow] * (cv + (brow == ci)) * Li[ci]
+    import scipy.sparse as sp
+    Bm = sp.csr_matrix((Bk, ci.astype(np.int64), rp.astype(np.int64)), shape=(n, n))
+    z = Bm @ sy["rhs"].cpu().numpy()
+    xg = x.cpu().numpy()
+    err = np.linalg.norm(xg / np.linalg.norm(xg) - z / np.linalg.norm(z))
+    assert err <= (1e-6 if mixed == 1 else 1e-12), err
+    # the slots hold the half-rounded correction, exactly
+    slots = c16.view(torch.float16).cpu().numpy().astype(np.float64).reshape(27, n)
+    offs = offsets(*sy["dims"][:2])
+    for k in range(0, ci.size, max(1, ci.size // 500)):
+        i, j = brow[k], ci[k]
+        assert slots[offs.index(int(j) - int(i)), i] == cv[k]
     # and the full solve
-    o2.maxiters = 5000
-    o2.tol = 1e-8
-    x2.zero_()
-    st = ctx.cg_stencil27(St, sy["rhs"], x2, o2)
+    o.maxiters = 5000
+    o.tol = 1e-8
+    x.zero_()
+    st = ctx.cg_stencil27(St, sy["rhs"], x, o)
     assert st.status == 0 and st.true_rel <= 1e-8, st.as_dict()
     net.close()
