@@ -172,3 +172,66 @@ def hex_problem(n: int, seed: int = 0, std_log10=1.5, corr_len=12.0, source=1.0)
     dirichlet = (np.arange(nv) // (N1 * N1) == 0).astype(np.int32)
     return {"nverts": nv, "x": x, "dirichlet": dirichlet, "endpoints": hexes, "kappa": kappa,
             "h": 1.0 / n, "source": source}
+
+
+# ---- tetgen meshes -------------------------------------------------------------------------
+def _tetgen_rows(path, ncols):
+    """The data rows of a tetgen file as the reference reads them
+    (src/mesh.cpp:252-360): the first integer of the file is the row count;
+    then, line by line, lines shorter than 3 characters and '#' comments are
+    skipped, and each data line gives its index (ignored) and ncols values
+    (extra columns ignored), until the count is reached."""
+    with open(path) as f:
+        first = f.readline()
+        head = first.split()
+        count = int(head[0])
+        rows = []
+        for line in f:
+            if len(rows) >= count:
+                break
+            line = line.rstrip("\n")
+            if len(line) < 3 or line[0] == "#":
+                continue
+            rows.append(line.split()[1:1 + ncols])
+    return head, rows
+
+
+def load_tetgen(node_path, ele_path, edge_path=None):
+    """A tetgen volume mesh as MeshVol::loadTet / loadTetEdge read it
+    (src/mesh.h:42-95, src/mesh.cpp:252-360): {"x": float64 [nv, 3],
+    "elements": int32 [ne, nodes per element], "edges": int32 [nedge, 2] or
+    None}.  Vertex ids are kept as written (the reference does not rebase)."""
+    _, xr = _tetgen_rows(node_path, 3)
+    x = np.array([[float(v) for v in r] for r in xr], dtype=np.float64).reshape(-1, 3)
+    head, er = _tetgen_rows(ele_path, 0)
+    nper = int(head[1]) if len(head) > 1 else 4
+    _, er = _tetgen_rows(ele_path, nper)
+    el = np.array([[int(v) for v in r] for r in er], dtype=np.int32).reshape(-1, nper)
+    edges = None
+    if edge_path is not None:
+        _, dr = _tetgen_rows(edge_path, 2)
+        edges = np.array([[int(v) for v in r] for r in dr], dtype=np.int32).reshape(-1, 2)
+    return {"x": x, "elements": el, "edges": edges}
+
+
+def tet_signed_volume(x, tets):
+    """W = -det([x_0 - x_3, x_1 - x_3, x_2 - x_3]) / 6 per tet: the volume
+    convention of the linear-tet element (positive for a usable tet)."""
+    d = x[tets[:, :3]] - x[tets[:, 3:4]]
+    return -np.linalg.det(d.transpose(0, 2, 1)) / 6.0
+
+
+def tet_mesh_problem(x, tets, E=1e5, nu=0.3, clamp_frac=0.05, load=(0.0, 0.0, -9.81e3)):
+    """Clamped-bottom linear elasticity on an arbitrary tet mesh: tets whose
+    W is negative get two vertices swapped (so every element is positively
+    oriented for the element formula), vertices within clamp_frac of the
+    height above the lowest one are clamped, E is a scalar or per tet."""
+    t = np.array(tets, dtype=np.int32, copy=True)
+    neg = tet_signed_volume(x, t) < 0
+    t[neg, 0], t[neg, 1] = tets[neg, 1], tets[neg, 0]
+    E = np.broadcast_to(np.asarray(E, dtype=np.float64), (t.shape[0],)).copy()
+    mu, lam = lame(E, nu)
+    z = x[:, 2]
+    c = (z <= z.min() + clamp_frac * (z.max() - z.min())).astype(np.int32)
+    return {"nverts": x.shape[0], "x": np.ascontiguousarray(x), "c": c, "endpoints": t,
+            "mu": mu, "lam": lam, "load": load}

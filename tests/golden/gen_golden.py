@@ -7,6 +7,8 @@
    REFERENCE's own PathIndexBuilder (src/path_indices.cpp), compiled from the
    reference sources by oracle/Makefile into oracle/_ref/ref_index.  Needs the
    reference checkout; the committed .npz files are what tests read.
+3. tet-bunny/: the reference's tetgen bunny mesh (apps/data/tet-bunny, data
+   files copied unchanged) with its vev indices (ref_index_bunny_*.npz).
 
 Run:  python tests/golden/gen_golden.py   (after `make -C oracle all ref`)
 """
@@ -128,6 +130,20 @@ def reference_indices(tmp):
     # the reference's createBox edge order
     ep = O.ref_create_box(4, 3, 5, tmp)
     np.savez_compressed(HERE / "ref_create_box_435.npz", endpoints=ep)
+    # the reference's tetgen bunny (apps/data/tet-bunny: data files, copied
+    # as they are) and its vev indices over the tets and over the edges
+    src = Path("/root/reference/apps/data/tet-bunny")
+    dst = HERE / "tet-bunny"
+    dst.mkdir(exist_ok=True)
+    for f in ("bunny.1.node", "bunny.1.ele", "bunny.1.edge"):
+        (dst / f).write_bytes((src / f).read_bytes())
+    m = meshes.load_tetgen(dst / "bunny.1.node", dst / "bunny.1.ele", dst / "bunny.1.edge")
+    nv = m["x"].shape[0]
+    for name, sets in (("bunny_tets", [m["elements"]]), ("bunny_edges", [m["edges"]])):
+        rp, ci = O.ref_index_vev(nv, sets, tmp)
+        arrays = {"nverts": np.array(nv), "rowptr": rp, "colidx": ci, "set0": sets[0]}
+        np.savez_compressed(HERE / f"ref_index_{name}.npz", **arrays)
+        print(name, nv, "rows", len(ci), "nnz")
 
 
 if __name__ == "__main__":
