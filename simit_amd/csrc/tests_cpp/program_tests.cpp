@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "simit/graph.h"
+#include "simit/mesh.h"
 #include "simit/program.h"
 
 using namespace simit;
@@ -154,10 +155,67 @@ static void springs_lattice() {
   EXPECT_TRUE(v(box(10, 10, 10))(2) < 0.0);   // the rest falls
 }
 
+// The reference's tetgen bunny (apps/data/tet-bunny, a test fixture here)
+// loaded with MeshVol, its edges as springs at rest length, one implicit
+// step: an irregular mesh through the whole C++ pipeline.
+static void mesh_bunny() {
+  MeshVol mesh;
+  const std::string dir = "tests/golden/tet-bunny/";
+  EXPECT_TRUE(mesh.loadTet(dir + "bunny.1.node", dir + "bunny.1.ele") == 0);
+  EXPECT_TRUE(mesh.loadTetEdge(dir + "bunny.1.edge") == 0);
+  EXPECT_TRUE(mesh.v.size() == 3607 && mesh.e.size() == 12687 && mesh.edges.size() == 19233);
+  if (mesh.v.size() != 3607 || mesh.edges.size() != 19233) return;
+  EXPECT_NEAR(-0.89565700000000004, mesh.v[0][0], 0.0);
+  EXPECT_TRUE(mesh.e[0].size() == 4 && mesh.e[0][0] == 2342 && mesh.e[0][3] == 3169);
+  EXPECT_TRUE(mesh.edges[0][0] == 3169 && mesh.edges[0][1] == 2323);
+  EXPECT_TRUE(MeshVol().loadTet(dir + "missing.node", dir + "bunny.1.ele") == -1);
+  Set points;
+  FieldRef<double, 3> x = points.addField<double, 3>("x");
+  FieldRef<double, 3> v = points.addField<double, 3>("v");
+  FieldRef<double> m = points.addField<double>("m");
+  FieldRef<bool> fixed = points.addField<bool>("fixed");
+  Set springs(points, points);
+  FieldRef<double> k = springs.addField<double>("k");
+  FieldRef<double> l0 = springs.addField<double>("l0");
+  double zmin = 1e300, zmax = -1e300;
+  for (const auto &p : mesh.v) {
+    zmin = std::fmin(zmin, p[2]);
+    zmax = std::fmax(zmax, p[2]);
+  }
+  std::vector<ElementRef> pe;
+  for (const auto &p : mesh.v) {
+    ElementRef q = points.add();
+    x(q) = {p[0], p[1], p[2]};
+    v(q) = {0.0, 0.0, 0.0};
+    m(q) = 1.0;
+    fixed(q) = p[2] <= zmin + 0.05 * (zmax - zmin);
+    pe.push_back(q);
+  }
+  for (const auto &ed : mesh.edges) {
+    ElementRef s = springs.add(pe[ed[0]], pe[ed[1]]);
+    const auto &a = mesh.v[ed[0]], &b = mesh.v[ed[1]];
+    k(s) = 1e3;
+    l0(s) = std::sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) +
+                      (a[2] - b[2]) * (a[2] - b[2]));
+  }
+  Function func = load("springs_implicit");
+  if (!func.defined()) return;
+  func.bind("points", &points);
+  func.bind("springs", &springs);
+  func.runSafe();
+  SolveStats st = func.lastSolve();
+  std::printf("  bunny springs_implicit: %d iterations, relative residual %.3e\n", st.iters,
+              st.true_rel);
+  EXPECT_TRUE(st.status == 0);
+  EXPECT_TRUE(st.true_rel <= 1e-8);
+  EXPECT_TRUE(v(pe[0])(2) < 0.0 || fixed(pe[0]));  // free points fall
+}
+
 int main(int argc, char **argv) {
   init("hip", 8);
   std::vector<std::pair<std::string, std::function<void()>>> tests = {
-      {"isprings", isprings}, {"cg", cg}, {"springs_lattice", springs_lattice}};
+      {"isprings", isprings}, {"cg", cg}, {"springs_lattice", springs_lattice},
+      {"mesh_bunny", mesh_bunny}};
   for (auto &t : tests) {
     bool run = argc <= 1;
     for (int i = 1; i < argc; ++i) run |= t.first == argv[i];

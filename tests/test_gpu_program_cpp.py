@@ -14,7 +14,7 @@ BIN = ROOT / "simit_amd" / "bin" / "simit_program_tests"
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["isprings", "cg", "springs_lattice"])
+@pytest.mark.parametrize("name", ["isprings", "cg", "springs_lattice", "mesh_bunny"])
 def test_cpp_program(name):
     import torch
     if not torch.cuda.is_available():
