@@ -18,7 +18,8 @@ HOST_LIB = LIBDIR / "libsimit.so"
 PROGRAM_TESTS = BINDIR / "simit_program_tests"
 
 HIP_SOURCES = ["smh_core.hip", "smh_index.hip", "smh_assemble.hip", "smh_cg.hip", "smh_gnn.hip"]
-HOST_SOURCES = ["simit/graph.cpp", "simit/program.cpp", "simit/mesh.cpp"]
+HOST_SOURCES = ["simit/graph.cpp", "simit/program.cpp", "simit/mesh.cpp",
+                "simit/reorder.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

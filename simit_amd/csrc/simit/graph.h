@@ -219,8 +219,56 @@ class Set {
   Iterator begin() const { return Iterator(0); }
   Iterator end() const { return Iterator(numElements); }
 
-  // bumped whenever elements are added (a bound function must re-init)
+  // bumped whenever elements are added or reordered (a bound function must re-init)
   uint64_t structureVersion() const { return version; }
+
+  // the 3-vector double field holding positions, used by reorder()
+  // (graph.h:255-262 setSpatialField / getSpatialFieldName)
+  void setSpatialField(const std::string &fname) {
+    const FieldData *f = field(fname);
+    if (!f || f->type != FieldType::Float64 || f->ncomp != 3)
+      throw SimitException("spatial field " + fname + " must be a double 3-vector field");
+    spatial = fname;
+  }
+  bool hasSpatialField() const { return !spatial.empty(); }
+  const std::string &getSpatialFieldName() const { return spatial; }
+
+  // Move element i to position newPos[i] (a permutation): every field's
+  // values and, for an edge set, its endpoint rows.
+  void permuteElements(const std::vector<int> &newPos) {
+    checkPermutation(newPos, numElements);
+    for (auto &kv : fields) {
+      FieldData &f = *kv.second;
+      const size_t eb = f.elementBytes();
+      std::vector<unsigned char> nb(f.bytes.size());
+      for (int i = 0; i < numElements; ++i)
+        std::memcpy(nb.data() + (size_t)newPos[i] * eb, f.bytes.data() + (size_t)i * eb, eb);
+      f.bytes.swap(nb);
+    }
+    const int card = getCardinality();
+    if (card) {
+      std::vector<int> ne(endpoints.size());
+      for (int i = 0; i < numElements; ++i)
+        for (int c = 0; c < card; ++c)
+          ne[(size_t)newPos[i] * card + c] = endpoints[(size_t)i * card + c];
+      endpoints.swap(ne);
+    }
+    ++version;
+  }
+  // After the endpoint set `vertices` was permuted by newPos: renumber the
+  // endpoints that refer to it.
+  void remapEndpoints(const Set &vertices, const std::vector<int> &newPos) {
+    checkPermutation(newPos, vertices.getSize());
+    const int card = getCardinality();
+    for (int c = 0; c < card; ++c) {
+      if (endpointSets[c] != &vertices) continue;
+      for (int i = 0; i < numElements; ++i) {
+        int &e = endpoints[(size_t)i * card + c];
+        e = newPos[e];
+      }
+    }
+    ++version;
+  }
 
  private:
   ElementRef grow() {
@@ -236,6 +284,16 @@ class Set {
   std::vector<int> endpoints;
   std::map<std::string, std::unique_ptr<FieldData>> fields;
   std::vector<std::string> order;
+  std::string spatial;
+
+  static void checkPermutation(const std::vector<int> &p, int n) {
+    if ((int)p.size() != n) throw SimitException("reordering: the map must have one entry per element");
+    std::vector<char> seen(n, 0);
+    for (int v : p) {
+      if (v < 0 || v >= n || seen[v]) throw SimitException("reordering: the map is not a permutation");
+      seen[v] = 1;
+    }
+  }
 };
 
 // createBox (src/graph.cpp:66-115): numX*numY*numZ points numbered

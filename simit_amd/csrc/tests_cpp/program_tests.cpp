@@ -5,6 +5,8 @@
 //
 // Usage: simit_program_tests [name ...]; exits non-zero on any failure.
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -14,6 +16,7 @@
 #include "simit/graph.h"
 #include "simit/mesh.h"
 #include "simit/program.h"
+#include "simit/reorder.h"
 
 using namespace simit;
 
@@ -211,11 +214,201 @@ static void mesh_bunny() {
   EXPECT_TRUE(v(pe[0])(2) < 0.0 || fixed(pe[0]));  // free points fall
 }
 
+// ---- reordering (host only) ---------------------------------------------------------------
+// test/reorder-tests.cpp:177-300 (reorderInt, reorderDouble): three vertices
+// x = (i,i,i), a = (3+i) x 4; edges (1,2) with 20s and (0,1) with 10s;
+// vertices moved by {2,0,1}, edges by {1,0}
+template <typename T>
+static void reorder_values() {
+  Set verts;
+  Set edges(verts, verts);
+  FieldRef<T, 3> x = verts.addField<T, 3>("x");
+  FieldRef<T, 4> a = verts.addField<T, 4>("a");
+  FieldRef<T, 5> tx = edges.addField<T, 5>("x");
+  std::vector<ElementRef> vr;
+  for (int i = 0; i < 3; ++i) {
+    vr.push_back(verts.add());
+    x(vr.back()) = {(T)i, (T)i, (T)i};
+    a(vr.back()) = {(T)(3 + i), (T)(3 + i), (T)(3 + i), (T)(3 + i)};
+  }
+  ElementRef e0 = edges.add(vr[1], vr[2]);
+  tx(e0) = {(T)20, (T)20, (T)20, (T)20, (T)20};
+  ElementRef e1 = edges.add(vr[0], vr[1]);
+  tx(e1) = {(T)10, (T)10, (T)10, (T)10, (T)10};
+  std::vector<int> vo{2, 0, 1}, eo{1, 0};
+  reorderVertexSet(edges, verts, vo);
+  reorderEdgeSet(edges, eo);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      EXPECT_NEAR((double)i, (double)x(ElementRef(vo[i]))(j), 0.0);
+      EXPECT_TRUE(x(ElementRef(i))(j) != (T)i);
+    }
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 4; ++j) EXPECT_NEAR(3.0 + i, (double)a(ElementRef(vo[i]))(j), 0.0);
+  const double want[2] = {20, 10};
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 5; ++j) EXPECT_NEAR(want[i], (double)tx(ElementRef(eo[i]))(j), 0.0);
+  // the endpoints followed: old edge 0 = (1,2) -> (vo[1], vo[2]) at position eo[0]
+  EXPECT_TRUE(edges.getEndpoint(ElementRef(eo[0]), 0).getIdent() == vo[1]);
+  EXPECT_TRUE(edges.getEndpoint(ElementRef(eo[0]), 1).getIdent() == vo[2]);
+  EXPECT_TRUE(edges.getEndpoint(ElementRef(eo[1]), 0).getIdent() == vo[0]);
+  EXPECT_TRUE(edges.getEndpoint(ElementRef(eo[1]), 1).getIdent() == vo[1]);
+  bool threw = false;
+  try {
+    std::vector<int> bad{0, 0, 1};
+    reorderVertexSet(edges, verts, bad);
+  } catch (const SimitException &) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+}
+static void reorder_int() { reorder_values<int>(); }
+static void reorder_double() { reorder_values<double>(); }
+
+// the Hilbert curve of order 3: a bijection onto [0, 512) whose consecutive
+// indices are lattice neighbours
+static void hilbert_curve() {
+  const int bits = 3, n = 1 << bits;
+  std::vector<int> at(n * n * n, -1);
+  for (int x = 0; x < n; ++x)
+    for (int y = 0; y < n; ++y)
+      for (int z = 0; z < n; ++z) {
+        const uint64_t h = hilbertIndex3(x, y, z, bits);
+        EXPECT_TRUE(h < (uint64_t)(n * n * n));
+        if (h >= (uint64_t)(n * n * n)) return;
+        EXPECT_TRUE(at[h] < 0);
+        at[h] = (x * n + y) * n + z;
+      }
+  EXPECT_TRUE(hilbertIndex3(0, 0, 0, bits) == 0);
+  for (int h = 1; h < n * n * n; ++h) {
+    const int p = at[h - 1], q = at[h];
+    const int d = std::abs(p / (n * n) - q / (n * n)) + std::abs(p / n % n - q / n % n) +
+                  std::abs(p % n - q % n);
+    EXPECT_TRUE(d == 1);
+    if (d != 1) return;
+  }
+}
+
+struct BunnySets {
+  MeshVol mesh;
+  Set points;
+  Set springs{points, points};
+  std::vector<ElementRef> pe;
+};
+static bool bunny_sets(BunnySets &b) {
+  const std::string dir = "tests/golden/tet-bunny/";
+  if (b.mesh.loadTet(dir + "bunny.1.node", dir + "bunny.1.ele") != 0 ||
+      b.mesh.loadTetEdge(dir + "bunny.1.edge") != 0)
+    return false;
+  FieldRef<double, 3> x = b.points.addField<double, 3>("x");
+  FieldRef<double, 3> v = b.points.addField<double, 3>("v");
+  FieldRef<double> m = b.points.addField<double>("m");
+  FieldRef<bool> fixed = b.points.addField<bool>("fixed");
+  FieldRef<double> k = b.springs.addField<double>("k");
+  FieldRef<double> l0 = b.springs.addField<double>("l0");
+  double zmin = 1e300, zmax = -1e300;
+  for (const auto &p : b.mesh.v) {
+    zmin = std::fmin(zmin, p[2]);
+    zmax = std::fmax(zmax, p[2]);
+  }
+  for (const auto &p : b.mesh.v) {
+    ElementRef q = b.points.add();
+    x(q) = {p[0], p[1], p[2]};
+    v(q) = {0.0, 0.0, 0.0};
+    m(q) = 1.0;
+    fixed(q) = p[2] <= zmin + 0.05 * (zmax - zmin);
+    b.pe.push_back(q);
+  }
+  for (const auto &ed : b.mesh.edges) {
+    ElementRef s = b.springs.add(b.pe[ed[0]], b.pe[ed[1]]);
+    const auto &p = b.mesh.v[ed[0]], &q = b.mesh.v[ed[1]];
+    k(s) = 1e3;
+    l0(s) = std::sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
+                      (p[2] - q[2]) * (p[2] - q[2]));
+  }
+  b.points.setSpatialField("x");
+  return true;
+}
+static double mean_span(const Set &edges) {
+  double s = 0;
+  const int *ep = edges.getEndpointsData();
+  for (int e = 0; e < edges.getSize(); ++e) s += std::abs(ep[2 * e] - ep[2 * e + 1]);
+  return s / edges.getSize();
+}
+
+// Hilbert reordering of the bunny: valid permutations, every field and
+// endpoint moved with its element, and shorter index spans along the edges
+static void reorder_bunny_host() {
+  BunnySets b;
+  EXPECT_TRUE(bunny_sets(b));
+  if (b.points.getSize() != 3607) return;
+  const double span0 = mean_span(b.springs);
+  std::vector<double> x0, l00;
+  FieldRef<double, 3> x = b.points.getField<double, 3>("x");
+  FieldRef<double> l0 = b.springs.getField<double>("l0");
+  for (ElementRef p : b.points)
+    for (int d = 0; d < 3; ++d) x0.push_back(x(p)(d));
+  for (ElementRef s : b.springs) l00.push_back(l0(s));
+  std::vector<int> eo, vo;
+  reorder(b.springs, b.points, eo, vo);
+  const double span1 = mean_span(b.springs);
+  std::printf("  mean |i - j| over the edges: %.1f before, %.1f after\n", span0, span1);
+  EXPECT_TRUE(span1 < 0.5 * span0);
+  for (int i = 0; i < b.points.getSize(); ++i)
+    for (int d = 0; d < 3; ++d) EXPECT_NEAR(x0[3 * i + d], x(ElementRef(vo[i]))(d), 0.0);
+  for (int e = 0; e < b.springs.getSize(); ++e) {
+    EXPECT_NEAR(l00[e], l0(ElementRef(eo[e])), 0.0);
+    const int a = b.springs.getEndpoint(ElementRef(eo[e]), 0).getIdent();
+    const int c = b.springs.getEndpoint(ElementRef(eo[e]), 1).getIdent();
+    EXPECT_TRUE(a == vo[b.mesh.edges[e][0]] && c == vo[b.mesh.edges[e][1]]);
+  }
+}
+
+// host-only part of the loader checks
+static void mesh_load() {
+  MeshVol mesh;
+  const std::string dir = "tests/golden/tet-bunny/";
+  EXPECT_TRUE(mesh.loadTet(dir + "bunny.1.node", dir + "bunny.1.ele") == 0);
+  EXPECT_TRUE(mesh.loadTetEdge(dir + "bunny.1.edge") == 0);
+  EXPECT_TRUE(mesh.v.size() == 3607 && mesh.e.size() == 12687 && mesh.edges.size() == 19233);
+  EXPECT_TRUE(MeshVol().loadTet(dir + "missing.node", dir + "bunny.1.ele") == -1);
+}
+
+// the springs step on the bunny, unreordered and reordered: the same motion
+static void reorder_bunny_springs() {
+  BunnySets a, b;
+  EXPECT_TRUE(bunny_sets(a) && bunny_sets(b));
+  std::vector<int> eo, vo;
+  reorder(b.springs, b.points, eo, vo);
+  Function fa = load("springs_implicit"), fb = load("springs_implicit");
+  if (!fa.defined() || !fb.defined()) return;
+  fa.bind("points", &a.points);
+  fa.bind("springs", &a.springs);
+  fb.bind("points", &b.points);
+  fb.bind("springs", &b.springs);
+  fa.runSafe();
+  fb.runSafe();
+  std::printf("  iterations: %d unreordered, %d reordered\n", fa.lastSolve().iters,
+              fb.lastSolve().iters);
+  FieldRef<double, 3> va = a.points.getField<double, 3>("v"), vb = b.points.getField<double, 3>("v");
+  double scale = 0, diff = 0;
+  for (int i = 0; i < a.points.getSize(); ++i)
+    for (int d = 0; d < 3; ++d) {
+      scale = std::fmax(scale, std::fabs(va(ElementRef(i))(d)));
+      diff = std::fmax(diff, std::fabs(va(ElementRef(i))(d) - vb(ElementRef(vo[i]))(d)));
+    }
+  std::printf("  max |v - v_reordered| = %.3e (max |v| %.3e)\n", diff, scale);
+  EXPECT_TRUE(scale > 0 && diff <= 1e-6 * scale);
+}
+
 int main(int argc, char **argv) {
   init("hip", 8);
   std::vector<std::pair<std::string, std::function<void()>>> tests = {
       {"isprings", isprings}, {"cg", cg}, {"springs_lattice", springs_lattice},
-      {"mesh_bunny", mesh_bunny}};
+      {"mesh_bunny", mesh_bunny}, {"reorder_bunny_springs", reorder_bunny_springs},
+      {"reorder_int", reorder_int}, {"reorder_double", reorder_double},
+      {"hilbert_curve", hilbert_curve}, {"reorder_bunny_host", reorder_bunny_host},
+      {"mesh_load", mesh_load}};
   for (auto &t : tests) {
     bool run = argc <= 1;
     for (int i = 1; i < argc; ++i) run |= t.first == argv[i];

@@ -14,7 +14,8 @@ BIN = ROOT / "simit_amd" / "bin" / "simit_program_tests"
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["isprings", "cg", "springs_lattice", "mesh_bunny"])
+@pytest.mark.parametrize("name", ["isprings", "cg", "springs_lattice", "mesh_bunny",
+                                  "reorder_bunny_springs"])
 def test_cpp_program(name):
     import torch
     if not torch.cuda.is_available():
