@@ -752,40 +752,68 @@ __global__ void k_dinv_st(uint32_t n, const void *vals, const void *vals_lo, OT 
   out[i] = (OT)(1.0 / d);
 }
 
-// CSR -> stencil: values of row i land in the slot of their column offset
-__global__ void k_st_fill(uint64_t count, uint32_t *__restrict__ out) {
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
-       t += (uint64_t)gridDim.x * blockDim.x)
-    out[t] = 0u;
+// CSR -> stencil through an LDS table: a workgroup takes 64 consecutive
+// rows, drops each of their entries into a zeroed [27][64] slot table (row
+// by binary search over the rows' offsets, slot by the first offset equal
+// to column - row), then writes the table out slot by slot, so both the
+// reads of the entries and the writes of each slot are coalesced.  An entry
+// outside its row's 27 offsets raises *err.
+template <typename OT, typename VT>
+__device__ __forceinline__ OT st_out(VT v) {
+  if constexpr (sizeof(OT) == 2)  // IEEE half, round to nearest even
+    return __builtin_bit_cast(uint16_t, (_Float16)v);
+  else
+    return (OT)v;
 }
-__global__ void k_st_fill16(uint64_t count, uint16_t *__restrict__ out) {
-  for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < count;
-       t += (uint64_t)gridDim.x * blockDim.x)
-    out[t] = 0u;
-}
-__global__ void k_st_scatter(StDev D, const uint32_t *__restrict__ rowptr,
-                             const uint32_t *__restrict__ colidx, const void *vals,
-                             const void *vals_lo, int esize, void *out, void *out_lo,
-                             int *__restrict__ err) {
-  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= D.n) return;
-  for (uint32_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-    const int64_t o = (int64_t)colidx[k] - (int64_t)i;
-    int s = -1;
+template <typename VT, typename OT>
+__global__ void __launch_bounds__(256)
+k_st_convert(StDev D, const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ colidx,
+             const VT *__restrict__ vals, const float *__restrict__ vals_lo, OT *__restrict__ out,
+             float *__restrict__ out_lo, int *__restrict__ err) {
+  constexpr int kRows = 64;
+  __shared__ VT tv[27 * kRows];
+  __shared__ float tl[27 * kRows];
+  __shared__ uint32_t rp[kRows + 1];
+  const uint64_t r0 = (uint64_t)blockIdx.x * kRows;
+  if (r0 >= D.n) return;
+  const int nr = (int)(D.n - r0 < (uint64_t)kRows ? D.n - r0 : (uint64_t)kRows);
+  const bool lo = out_lo != nullptr;
+  for (int t = threadIdx.x; t < 27 * kRows; t += 256) {
+    tv[t] = (VT)0;
+    tl[t] = 0.0f;
+  }
+  for (int t = threadIdx.x; t <= nr; t += 256) rp[t] = rowptr[r0 + t];
+  __syncthreads();
+  const uint32_t base = rp[0], cnt = rp[nr] - base;
+  for (uint32_t k = threadIdx.x; k < cnt; k += 256) {
+    const uint32_t g = base + k;
+    int a = 0, b = nr - 1;  // the last row starting at or before entry g
+    while (a < b) {
+      const int mid = (a + b + 1) >> 1;
+      if (rp[mid] <= g) a = mid;
+      else b = mid - 1;
+    }
+    const int64_t o = (int64_t)colidx[g] - (int64_t)(r0 + a);
+    int sl = -1;
     for (int q = 0; q < 27; ++q)
-      if (D.off[q] == o) s = q;
-    if (s < 0) {
+      if (D.off[q] == o) {
+        sl = q;
+        break;
+      }
+    if (sl < 0) {
       atomicMax(err, 1);
       continue;
     }
-    const uint64_t dst = (uint64_t)s * D.n + i;
-    if (esize == 8)
-      ((double *)out)[dst] = ((const double *)vals)[k];
-    else if (esize == 2)  // float in, IEEE half out (round to nearest even)
-      ((uint16_t *)out)[dst] = __builtin_bit_cast(uint16_t, (_Float16)((const float *)vals)[k]);
-    else
-      ((float *)out)[dst] = ((const float *)vals)[k];
-    if (out_lo) ((float *)out_lo)[dst] = ((const float *)vals_lo)[k];
+    tv[sl * kRows + a] = vals[g];
+    if (lo) tl[sl * kRows + a] = vals_lo[g];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < 27 * kRows; t += 256) {
+    const int sl = t / kRows, r = t - sl * kRows;
+    if (r >= nr) continue;
+    const uint64_t dst = (uint64_t)sl * D.n + r0 + r;
+    out[dst] = st_out<OT>(tv[t]);
+    if (lo) out_lo[dst] = tl[t];
   }
 }
 
@@ -1913,18 +1941,20 @@ extern "C" int smh_stencil27_from_bsr(smh_ctx *ctx, const smh_bsr *A, uint32_t n
   if (A->nrows == 0) return SMH_OK;
   smh_stencil27 S{n0, n1, n2, A->precision, vals, vals_lo};
   const StDev D = st_dev(&S, vals, vals_lo);
-  const int esize = A->precision == SMH_F64 ? 8 : 4;
-  const uint64_t words = 27ull * A->nrows * (esize / 4);
-  hipLaunchKernelGGL(k_st_fill, dim3(grid_for(words, 256, 8192)), dim3(256), 0, ctx->stream, words,
-                     (uint32_t *)vals);
-  if (lo)
-    hipLaunchKernelGGL(k_st_fill, dim3(grid_for(27ull * A->nrows, 256, 8192)), dim3(256), 0,
-                       ctx->stream, 27ull * A->nrows, (uint32_t *)vals_lo);
   int *err = (int *)ctx->red_count;
   SMH_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
-  hipLaunchKernelGGL(k_st_scatter, dim3(grid_for(A->nrows, 256)), dim3(256), 0, ctx->stream, D,
-                     A->rowptr, A->colidx, A->vals, A->vals_lo, esize, vals,
-                     lo ? vals_lo : nullptr, err);
+  const unsigned g = (unsigned)((A->nrows + 63) / 64);
+  if (A->precision == SMH_F64)
+    hipLaunchKernelGGL((k_st_convert<double, double>), dim3(g), dim3(256), 0, ctx->stream, D,
+                       A->rowptr,
+                       A->colidx, (const double *)A->vals, (const float *)nullptr,
+                       (double *)vals, (float *)nullptr, err);
+  else
+    hipLaunchKernelGGL((k_st_convert<float, float>), dim3(g), dim3(256), 0, ctx->stream, D,
+                       A->rowptr,
+                       A->colidx, (const float *)A->vals,
+                       lo ? (const float *)A->vals_lo : (const float *)nullptr, (float *)vals,
+                       lo ? (float *)vals_lo : (float *)nullptr, err);
   SMH_LAUNCH_CHECK(ctx);
   int herr = 0;
   SMH_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
@@ -1950,13 +1980,11 @@ extern "C" int smh_stencil27_correction(smh_ctx *ctx, const smh_bsr *C, uint32_t
             "smh_stencil27_correction: null values");
   smh_stencil27 S{n0, n1, n2, SMH_F32, out, nullptr};
   const StDev D = st_dev(&S, out, nullptr);
-  hipLaunchKernelGGL(k_st_fill16, dim3(grid_for(27ull * C->nrows, 256, 8192)), dim3(256), 0,
-                     ctx->stream, 27ull * C->nrows, (uint16_t *)out);
   int *err = (int *)ctx->red_count;
   SMH_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), ctx->stream));
-  hipLaunchKernelGGL(k_st_scatter, dim3(grid_for(C->nrows, 256)), dim3(256), 0, ctx->stream, D,
-                     C->rowptr, C->colidx, C->vals, (const void *)nullptr, 2, out, (void *)nullptr,
-                     err);
+  hipLaunchKernelGGL((k_st_convert<float, uint16_t>), dim3((unsigned)((C->nrows + 63) / 64)),
+                     dim3(256), 0, ctx->stream, D, C->rowptr, C->colidx, (const float *)C->vals,
+                     (const float *)nullptr, (uint16_t *)out, (float *)nullptr, err);
   SMH_LAUNCH_CHECK(ctx);
   int herr = 0;
   SMH_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
