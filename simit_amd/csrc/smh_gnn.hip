@@ -96,18 +96,43 @@ __device__ __forceinline__ uint32_t float_to_e4m3(float x) {
 }
 
 // ---- graph ---------------------------------------------------------------------------
-__global__ void k_graph_rows(uint32_t n, const uint32_t *__restrict__ rp,
-                             const uint32_t *__restrict__ ci, uint32_t *__restrict__ brow,
-                             uint32_t *__restrict__ deg, int *__restrict__ err) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  uint32_t nd = 0;
-  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
-    brow[k] = i;
-    nd += ci[k] == i;
+// brow[k] = the row of entry k, deg[i] = the off-diagonal entries of row i.
+// A workgroup takes 256 consecutive rows, whose entries are contiguous:
+// thread t writes brow for entries base + t, base + t + 256, ... (coalesced),
+// finding the row by binary search over the rows' offsets in LDS, and counts
+// the diagonal entries in LDS; every non-empty row must hold exactly one.
+constexpr int kGraphRows = 256;
+__global__ void __launch_bounds__(kGraphRows)
+k_graph_rows(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+             uint32_t *__restrict__ brow, uint32_t *__restrict__ deg, int *__restrict__ err) {
+  __shared__ uint32_t srp[kGraphRows + 1];
+  __shared__ uint32_t nd[kGraphRows];
+  const uint64_t r0 = (uint64_t)blockIdx.x * kGraphRows;
+  if (r0 >= n) return;
+  const int nr = (int)(n - r0 < (uint64_t)kGraphRows ? n - r0 : (uint64_t)kGraphRows);
+  for (int t = threadIdx.x; t <= nr; t += kGraphRows) srp[t] = rp[r0 + t];
+  nd[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = srp[0], cnt = srp[nr] - base;
+  for (uint32_t k = threadIdx.x; k < cnt; k += kGraphRows) {
+    const uint32_t g = base + k;
+    int a = 0, b = nr - 1;  // the last row starting at or before entry g
+    while (a < b) {
+      const int mid = (a + b + 1) >> 1;
+      if (srp[mid] <= g) a = mid;
+      else b = mid - 1;
+    }
+    const uint32_t row = (uint32_t)r0 + (uint32_t)a;
+    brow[g] = row;
+    if (ci[g] == row) atomicAdd(&nd[a], 1u);
   }
-  if (nd != 1 && rp[i + 1] > rp[i]) atomicOr(err, 1);  // rows must hold their diagonal
-  deg[i] = rp[i + 1] - rp[i] - nd;
+  __syncthreads();
+  if ((int)threadIdx.x < nr) {
+    const int t = threadIdx.x;
+    const uint32_t len = srp[t + 1] - srp[t];
+    if (nd[t] != 1 && len > 0) atomicOr(err, 1);  // rows must hold their diagonal
+    deg[r0 + t] = len - nd[t];
+  }
 }
 
 // t(k): position of block (j, i) for block k = (i, j)
@@ -191,58 +216,24 @@ __global__ void k_gnn_linv(uint32_t n, const uint32_t *__restrict__ rp,
   for (int q = 0; q < 9; ++q) linv[(uint64_t)i * 9 + q] = (float)L[q];
 }
 
-// normalised blocks, edge features (and FP32 Ahat for training), vertex
-// features; one thread per row, blocks in CSR order
+// The normalised block Ahat_k = Linv_i A_k Linv_j^T of entry k = (i, j) (I on
+// the diagonal), in FP64, and its Frobenius norm.
 template <int PREC, int B>
-__global__ void k_gnn_features(uint32_t n, const uint32_t *__restrict__ rp,
-                               const uint32_t *__restrict__ ci, const void *vals,
-                               const void *vlo, const float *__restrict__ linv,
-                               const uint32_t *__restrict__ deg, uint16_t *__restrict__ fe,
-                               uint16_t *__restrict__ fv, float *__restrict__ ahat) {
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  if constexpr (B == 1) {  // scalar: Ahat_k in slot 0, slots 1-8 zero
-    const double li = (double)linv[i];
-    double sv = 0.0, sn = 0.0;
-    for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
-      const uint32_t j = ci[k];
-      const bool dg = j == i;
-      const double ah = dg ? 1.0 : li * aval<PREC>(vals, vlo, k) * (double)linv[j];
-      const double nr = fabs(ah);
-      uint16_t *f = fe + (uint64_t)k * kF;
-      f[0] = f2bf((float)ah);
-      for (int q = 1; q < 9; ++q) f[q] = 0;
-      f[9] = f2bf((float)nr);
-      f[10] = f2bf(dg ? 1.0f : 0.0f);
-      for (int q = 11; q < kF; ++q) f[q] = 0;
-      if (ahat) ahat[k] = (float)ah;
-      if (!dg) {
-        sv += ah;
-        sn += nr;
-      }
-    }
-    uint16_t *f = fv + (uint64_t)i * kF;
-    f[0] = f2bf((float)sv);
-    for (int q = 1; q < 9; ++q) f[q] = 0;
-    f[9] = f2bf((float)sn);
-    f[10] = f2bf((float)deg[i] / 16.0f);
-    for (int q = 11; q < kF; ++q) f[q] = 0;
-    return;
-  }
-  double Li[9];
-  for (int q = 0; q < 9; ++q) Li[q] = (double)linv[(uint64_t)i * 9 + q];
-  double sv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0}, sn = 0.0;
-  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
-    const uint32_t j = ci[k];
-    double Ah[9];
-    const bool dg = j == i;
-    if (dg) {
+__device__ __forceinline__ double ahat_block(const void *vals, const void *vlo,
+                                             const float *__restrict__ linv, uint32_t i,
+                                             uint32_t j, uint64_t k, double (&Ah)[B * B]) {
+  if constexpr (B == 1) {
+    Ah[0] = i == j ? 1.0 : (double)linv[i] * aval<PREC>(vals, vlo, k) * (double)linv[j];
+    return fabs(Ah[0]);
+  } else {
+    if (i == j) {
       for (int q = 0; q < 9; ++q) Ah[q] = (q % 4 == 0) ? 1.0 : 0.0;
     } else {
-      double Lj[9], A[9], T[9];
+      double Li[9], Lj[9], A[9], T[9];
       for (int q = 0; q < 9; ++q) {
+        Li[q] = (double)linv[(uint64_t)i * 9 + q];
         Lj[q] = (double)linv[(uint64_t)j * 9 + q];
-        A[q] = aval<PREC>(vals, vlo, (uint64_t)k * 9 + q);
+        A[q] = aval<PREC>(vals, vlo, k * 9 + q);
       }
       // T = Li A ; Ah = T Lj^T
       for (int a = 0; a < 3; ++a)
@@ -255,24 +246,128 @@ __global__ void k_gnn_features(uint32_t n, const uint32_t *__restrict__ rp,
     }
     double nr = 0.0;
     for (int q = 0; q < 9; ++q) nr += Ah[q] * Ah[q];
-    nr = sqrt(nr);
-    uint16_t *f = fe + (uint64_t)k * kF;
-    for (int q = 0; q < 9; ++q) f[q] = f2bf((float)Ah[q]);
-    f[9] = f2bf((float)nr);
-    f[10] = f2bf(dg ? 1.0f : 0.0f);
-    for (int q = 11; q < kF; ++q) f[q] = 0;
-    if (ahat)
-      for (int q = 0; q < 9; ++q) ahat[(uint64_t)k * 9 + q] = (float)Ah[q];
-    if (!dg) {
-      for (int q = 0; q < 9; ++q) sv[q] += Ah[q];
-      sn += nr;
+    return sqrt(nr);
+  }
+}
+
+// one 16-slot BF16 feature row: v[0..B*B-1] (zeros up to slot 8), then
+// slots 9 and 10, zeros after; two 16-byte stores
+template <int B>
+__device__ __forceinline__ void store_feat(uint16_t *f, const double (&v)[B * B], float s9,
+                                           float s10) {
+  uint16_t h[kF];
+  for (int q = 0; q < kF; ++q) h[q] = 0;
+  for (int q = 0; q < B * B; ++q) h[q] = f2bf((float)v[q]);
+  h[9] = f2bf(s9);
+  h[10] = f2bf(s10);
+  uint32_t w[8];
+  for (int q = 0; q < 8; ++q) w[q] = (uint32_t)h[2 * q] | ((uint32_t)h[2 * q + 1] << 16);
+  uint4 *o = reinterpret_cast<uint4 *>(f);
+  o[0] = make_uint4(w[0], w[1], w[2], w[3]);
+  o[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// edge features (and FP32 Ahat for training): one thread per entry, so the
+// 32-byte feature rows are written contiguously across the wave
+template <int PREC, int B>
+__global__ void __launch_bounds__(256)
+k_gnn_feat_edges(uint64_t nnz, const uint32_t *__restrict__ brow,
+                 const uint32_t *__restrict__ ci, const void *vals, const void *vlo,
+                 const float *__restrict__ linv, uint16_t *__restrict__ fe,
+                 float *__restrict__ ahat) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const uint32_t i = brow[k], j = ci[k];
+  double Ah[B * B];
+  const double nr = ahat_block<PREC, B>(vals, vlo, linv, i, j, k, Ah);
+  store_feat<B>(fe + k * kF, Ah, (float)nr, i == j ? 1.0f : 0.0f);
+  if (ahat)
+    for (int q = 0; q < B * B; ++q) ahat[k * (B * B) + q] = (float)Ah[q];
+}
+
+// vertex features: the sums over the row's off-diagonal blocks of Ahat_k and
+// of its norm (in CSR order, FP64), and deg_i / 16; one thread per row
+template <int PREC, int B>
+__global__ void __launch_bounds__(128)
+k_gnn_feat_rows(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+                const void *vals, const void *vlo, const float *__restrict__ linv,
+                const uint32_t *__restrict__ deg, uint16_t *__restrict__ fv) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double sv[B * B], sn = 0.0;
+  for (int q = 0; q < B * B; ++q) sv[q] = 0.0;
+  for (uint32_t k = rp[i]; k < rp[i + 1]; ++k) {
+    const uint32_t j = ci[k];
+    if (j == i) continue;
+    double Ah[B * B];
+    sn += ahat_block<PREC, B>(vals, vlo, linv, i, j, k, Ah);
+    for (int q = 0; q < B * B; ++q) sv[q] += Ah[q];
+  }
+  store_feat<B>(fv + (uint64_t)i * kF, sv, (float)sn, (float)deg[i] / 16.0f);
+}
+
+// Scalar systems: both feature sets in one pass over 64 consecutive rows
+// per workgroup.  The rows' entries (contiguous) are spread over the
+// threads, which write their edge features (coalesced, as k_gnn_feat_edges)
+// and keep Ahat in LDS; then one thread per row sums its row's off-diagonal
+// Ahat in CSR order in FP64, as k_gnn_feat_rows does.  Entries beyond the
+// LDS capacity (rows far longer than a 27-point stencil's) are recomputed
+// from global memory in that sum.
+constexpr int kFeatRows = 64, kFeatCap = 2048;
+template <int PREC>
+__global__ void __launch_bounds__(256)
+k_gnn_feat_scalar(uint32_t n, const uint32_t *__restrict__ rp, const uint32_t *__restrict__ ci,
+                  const void *vals, const void *vlo, const float *__restrict__ linv,
+                  const uint32_t *__restrict__ deg, uint16_t *__restrict__ fe,
+                  uint16_t *__restrict__ fv, float *__restrict__ ahat) {
+  __shared__ uint32_t srp[kFeatRows + 1];
+  __shared__ double sah[kFeatCap];
+  __shared__ uint8_t sdg[kFeatCap];
+  const uint64_t r0 = (uint64_t)blockIdx.x * kFeatRows;
+  if (r0 >= n) return;
+  const int nr = (int)(n - r0 < (uint64_t)kFeatRows ? n - r0 : (uint64_t)kFeatRows);
+  for (int t = threadIdx.x; t <= nr; t += 256) srp[t] = rp[r0 + t];
+  __syncthreads();
+  const uint32_t base = srp[0], cnt = srp[nr] - base;
+  for (uint32_t k = threadIdx.x; k < cnt; k += 256) {
+    const uint32_t g = base + k;
+    int a = 0, b = nr - 1;  // the last row starting at or before entry g
+    while (a < b) {
+      const int mid = (a + b + 1) >> 1;
+      if (srp[mid] <= g) a = mid;
+      else b = mid - 1;
+    }
+    const uint32_t i = (uint32_t)r0 + (uint32_t)a, j = ci[g];
+    double Ah[1];
+    const double nrm = ahat_block<PREC, 1>(vals, vlo, linv, i, j, g, Ah);
+    store_feat<1>(fe + (uint64_t)g * kF, Ah, (float)nrm, i == j ? 1.0f : 0.0f);
+    if (ahat) ahat[g] = (float)Ah[0];
+    if (k < (uint32_t)kFeatCap) {
+      sah[k] = Ah[0];
+      sdg[k] = i == j;
     }
   }
-  uint16_t *f = fv + (uint64_t)i * kF;
-  for (int q = 0; q < 9; ++q) f[q] = f2bf((float)sv[q]);
-  f[9] = f2bf((float)sn);
-  f[10] = f2bf((float)deg[i] / 16.0f);
-  for (int q = 11; q < kF; ++q) f[q] = 0;
+  __syncthreads();
+  if ((int)threadIdx.x >= nr) return;
+  const uint32_t i = (uint32_t)r0 + threadIdx.x;
+  double sv[1] = {0.0}, sn = 0.0;
+  for (uint32_t g = srp[threadIdx.x]; g < srp[threadIdx.x + 1]; ++g) {
+    const uint32_t k = g - base;
+    double ah;
+    if (k < (uint32_t)kFeatCap) {
+      if (sdg[k]) continue;
+      ah = sah[k];
+    } else {
+      const uint32_t j = ci[g];
+      if (j == i) continue;
+      double Ah[1];
+      ahat_block<PREC, 1>(vals, vlo, linv, i, j, g, Ah);
+      ah = Ah[0];
+    }
+    sn += fabs(ah);
+    sv[0] += ah;
+  }
+  store_feat<1>(fv + (uint64_t)i * kF, sv, (float)sn, (float)deg[i] / 16.0f);
 }
 
 // ---- MFMA tiles ---------------------------------------------------------------------------
@@ -312,13 +407,50 @@ __device__ __forceinline__ void tile_mma(const bf16x4 (&a)[KB], const BFrag<KB, 
       acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kb], w.b[nb][kb], acc[nb], 0, 0, 0);
 }
 
-// encoders: out = relu(X W + b) for X = bf16 [rows][16]
+// Orders a wave's LDS accesses across its lanes: a wave's LDS instructions
+// execute in order, so it only has to keep the compiler from moving them
+// (no workgroup barrier: the waves of a workgroup run different numbers of
+// tiles).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A wave's 16 x 32 output tile, rows [16 tile, 16 tile + 16) of a BF16
+// [rows][32] array: the accumulators (lane: row 4 (lane / 16) + v, column
+// 16 nb + lane % 16) pass through the wave's 1 KB of LDS, so the global
+// store is one 16-byte store per lane over the tile's contiguous 1 KB
+// instead of eight 2-byte stores.
+struct ReluBias {
+  const float *bias;
+  __device__ float operator()(float x, int c) const { return fmaxf(x + bias[c], 0.0f); }
+};
+template <class F>
+__device__ __forceinline__ void store_tile_bf16(uint16_t *lds, const f32x4 (&acc)[2], F f,
+                                                uint16_t *out, uint64_t tile, uint64_t rows,
+                                                int lane) {
+  for (int v = 0; v < 4; ++v)
+    for (int nb = 0; nb < 2; ++nb) {
+      const int c = 16 * nb + (lane & 15);
+      lds[(4 * (lane >> 4) + v) * kH + c] = f2bf(f(acc[nb][v], c));
+    }
+  wave_lds_sync();
+  const uint4 q = reinterpret_cast<const uint4 *>(lds)[lane];  // row lane / 4
+  wave_lds_sync();                                             // read before the next writes
+  if (tile * 16 + (lane >> 2) < rows) reinterpret_cast<uint4 *>(out + tile * 16 * kH)[lane] = q;
+}
+
+// encoders: out = relu(X W + b) for X = bf16 [rows][16]; edges write BF16
+// (eout), vertices FP32 (hout) and its BF16 rounding (eout, the copy the
+// edge kernels gather)
 template <bool EDGE>
 __global__ void __launch_bounds__(256)
 k_gnn_encode(uint64_t rows, const uint16_t *__restrict__ X, const uint16_t *__restrict__ Wt,
              const float *__restrict__ bias, float *__restrict__ hout,
              uint16_t *__restrict__ eout) {
   const int lane = threadIdx.x & 63;
+  __shared__ __align__(16) uint16_t lt[4][16 * kH];
   BFrag<1, 2> w;
   w.load(Wt, lane);
   const uint64_t ntiles = (rows + 15) / 16;
@@ -329,33 +461,57 @@ k_gnn_encode(uint64_t rows, const uint16_t *__restrict__ X, const uint16_t *__re
     a[0] = r < rows ? load_bf(X + r * kF + 4 * (lane >> 4)) : bf16x4{0, 0, 0, 0};
     f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     tile_mma<1, 2>(a, w, acc);
+    if (EDGE) {
+      store_tile_bf16(lt[threadIdx.x >> 6], acc, ReluBias{bias}, eout, tile, rows, lane);
+      continue;
+    }
     for (int v = 0; v < 4; ++v) {
       const uint64_t row = tile * 16 + 4 * (lane >> 4) + v;
       if (row >= rows) continue;
       for (int nb = 0; nb < 2; ++nb) {
         const int c = 16 * nb + (lane & 15);
         const float z = fmaxf(acc[nb][v] + bias[c], 0.0f);
-        if (EDGE) eout[row * kH + c] = f2bf(z);
-        else hout[row * kH + c] = z;
+        hout[row * kH + c] = z;
+        eout[row * kH + c] = f2bf(z);
       }
     }
   }
 }
 
-// A fragment of [h_i h_j e_k] for edge row r
+// The 16-entry tiles of an edge kernel, XCD-aware: the dispatcher deals
+// workgroups to the 8 XCDs in turn, so with a grid that is a multiple of 8,
+// XCD x (workgroups x, x + 8, ...) takes the x-th eighth of the tiles and
+// its workgroups sweep it together.  Tiles that gather the same vertex rows
+// (neighbouring rows of the matrix) then meet in one L2 instead of being
+// dealt 64 entries at a time across all eight.
+struct TileRange {
+  uint64_t t, end, step;
+};
+__device__ __forceinline__ TileRange edge_tiles(uint64_t ntiles) {
+  const uint64_t wave = threadIdx.x >> 6;
+  if (gridDim.x % 8 == 0) {
+    const uint64_t per = gridDim.x / 8, x = blockIdx.x % 8, w = blockIdx.x / 8;
+    const uint64_t chunk = (ntiles + 7) / 8, beg = x * chunk;
+    const uint64_t end = beg + chunk < ntiles ? beg + chunk : ntiles;
+    return {beg + 4 * w + wave, end, 4 * per};
+  }
+  return {(uint64_t)blockIdx.x * 4 + wave, ntiles, (uint64_t)gridDim.x * 4};
+}
+
+// A fragment of [h_i h_j e_k] for edge row r (hb: the BF16 vertex states)
 __device__ __forceinline__ void gather_edge_x(bf16x4 (&a)[6], uint64_t r, bool valid,
                                               const uint32_t *brow, const uint32_t *ci,
-                                              const float *h, const uint16_t *e, int lane) {
+                                              const uint16_t *hb, const uint16_t *e, int lane) {
   const int k4 = 4 * (lane >> 4);
   if (!valid) {
     for (int kb = 0; kb < 6; ++kb) a[kb] = bf16x4{0, 0, 0, 0};
     return;
   }
   const uint32_t i = brow[r], j = ci[r];
-  a[0] = load_f32_as_bf(h + (uint64_t)i * kH + k4);
-  a[1] = load_f32_as_bf(h + (uint64_t)i * kH + 16 + k4);
-  a[2] = load_f32_as_bf(h + (uint64_t)j * kH + k4);
-  a[3] = load_f32_as_bf(h + (uint64_t)j * kH + 16 + k4);
+  a[0] = load_bf(hb + (uint64_t)i * kH + k4);
+  a[1] = load_bf(hb + (uint64_t)i * kH + 16 + k4);
+  a[2] = load_bf(hb + (uint64_t)j * kH + k4);
+  a[3] = load_bf(hb + (uint64_t)j * kH + 16 + k4);
   a[4] = load_bf(e + r * kH + k4);
   a[5] = load_bf(e + r * kH + 16 + k4);
 }
@@ -363,37 +519,31 @@ __device__ __forceinline__ void gather_edge_x(bf16x4 (&a)[6], uint64_t r, bool v
 // m_k = relu([h_i h_j e_k] W_m + b_m) (BF16 out)
 __global__ void __launch_bounds__(256)
 k_gnn_message(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *__restrict__ ci,
-              const float *__restrict__ h, const uint16_t *__restrict__ e,
+              const uint16_t *__restrict__ hb, const uint16_t *__restrict__ e,
               const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
               uint16_t *__restrict__ m) {
   const int lane = threadIdx.x & 63;
+  __shared__ __align__(16) uint16_t lt[4][16 * kH];
   BFrag<6, 2> w;
   w.load(Wt, lane);
-  const uint64_t ntiles = (nnz + 15) / 16;
-  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles;
-       tile += (uint64_t)gridDim.x * 4) {
+  for (TileRange T = edge_tiles((nnz + 15) / 16); T.t < T.end; T.t += T.step) {
+    const uint64_t tile = T.t;
     const uint64_t r = tile * 16 + (lane & 15);
     bf16x4 a[6];
-    gather_edge_x(a, r, r < nnz, brow, ci, h, e, lane);
+    gather_edge_x(a, r, r < nnz, brow, ci, hb, e, lane);
     f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
     tile_mma<6, 2>(a, w, acc);
-    for (int v = 0; v < 4; ++v) {
-      const uint64_t row = tile * 16 + 4 * (lane >> 4) + v;
-      if (row >= nnz) continue;
-      for (int nb = 0; nb < 2; ++nb) {
-        const int c = 16 * nb + (lane & 15);
-        m[row * kH + c] = f2bf(fmaxf(acc[nb][v] + bias[c], 0.0f));
-      }
-    }
+    store_tile_bf16(lt[threadIdx.x >> 6], acc, ReluBias{bias}, m, tile, nnz, lane);
   }
 }
 
 // a_i = mean of row i's messages; U = [h_i a_i] W_u + b_u; hn = h + relu(U)
+// (FP32, and its BF16 rounding hbn)
 __global__ void __launch_bounds__(256)
 k_gnn_update(uint32_t n, const uint32_t *__restrict__ rp, const float *__restrict__ h,
              const uint16_t *__restrict__ m, const uint16_t *__restrict__ Wt,
-             const float *__restrict__ bias, float *__restrict__ hn, float *__restrict__ Uout,
-             float *__restrict__ aout) {
+             const float *__restrict__ bias, float *__restrict__ hn, uint16_t *__restrict__ hbn,
+             float *__restrict__ Uout, float *__restrict__ aout) {
   const int lane = threadIdx.x & 63;
   BFrag<4, 2> w;
   w.load(Wt, lane);
@@ -437,7 +587,9 @@ k_gnn_update(uint32_t n, const uint32_t *__restrict__ rp, const float *__restric
         const float U = acc[nb][v] + bias[c];
         const uint64_t o = (uint64_t)row * kH + c;
         if (Uout) Uout[o] = U;
-        hn[o] = h[o] + fmaxf(U, 0.0f);
+        const float hv = h[o] + fmaxf(U, 0.0f);
+        hn[o] = hv;
+        hbn[o] = f2bf(hv);
       }
     }
   }
@@ -446,26 +598,39 @@ k_gnn_update(uint32_t n, const uint32_t *__restrict__ rp, const float *__restric
 // R_k = [h_i h_j e_k] W_d + b_d, first 9 outputs
 __global__ void __launch_bounds__(256)
 k_gnn_decode(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *__restrict__ ci,
-             const float *__restrict__ h, const uint16_t *__restrict__ e,
+             const uint16_t *__restrict__ hb, const uint16_t *__restrict__ e,
              const uint16_t *__restrict__ Wt, const float *__restrict__ bias,
              float *__restrict__ R) {
   const int lane = threadIdx.x & 63;
+  __shared__ __align__(16) float lt[4][16 * 9];
+  float *l = lt[threadIdx.x >> 6];
   BFrag<6, 1> w;
   w.load(Wt, lane);
-  const uint64_t ntiles = (nnz + 15) / 16;
-  for (uint64_t tile = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles;
-       tile += (uint64_t)gridDim.x * 4) {
+  for (TileRange T = edge_tiles((nnz + 15) / 16); T.t < T.end; T.t += T.step) {
+    const uint64_t tile = T.t;
     const uint64_t r = tile * 16 + (lane & 15);
     bf16x4 a[6];
-    gather_edge_x(a, r, r < nnz, brow, ci, h, e, lane);
+    gather_edge_x(a, r, r < nnz, brow, ci, hb, e, lane);
     f32x4 acc[1] = {{0, 0, 0, 0}};
     tile_mma<6, 1>(a, w, acc);
+    // the tile's 16 x 9 outputs are 144 contiguous floats of R: through LDS,
+    // then 36 lanes store 16 bytes each
     const int c = lane & 15;
-    if (c >= 9) continue;
-    for (int v = 0; v < 4; ++v) {
-      const uint64_t row = tile * 16 + 4 * (lane >> 4) + v;
-      if (row < nnz) R[row * 9 + c] = acc[0][v] + bias[c];
+    if (c < 9)
+      for (int v = 0; v < 4; ++v) l[(4 * (lane >> 4) + v) * 9 + c] = acc[0][v] + bias[c];
+    wave_lds_sync();
+    if (lane < 36) {
+      const float4 q = reinterpret_cast<const float4 *>(l)[lane];
+      float *o = R + tile * 144;
+      if (tile * 16 + 16 <= nnz) {
+        reinterpret_cast<float4 *>(o)[lane] = q;
+      } else {  // the last, partial tile
+        const float qq[4] = {q.x, q.y, q.z, q.w};
+        for (int e = 0; e < 4; ++e)
+          if (tile * 16 + (uint64_t)(4 * lane + e) / 9 < nnz) o[4 * lane + e] = qq[e];
+      }
     }
+    wave_lds_sync();
   }
 }
 
@@ -846,6 +1011,7 @@ struct smh_gnn {
   float *linv = nullptr, *ahat = nullptr, *R = nullptr, *c32 = nullptr;
   uint16_t *fe = nullptr, *fv = nullptr, *e = nullptr;
   float *h[kLayers + 1] = {}, *a[kLayers] = {}, *U[kLayers] = {};
+  uint16_t *hb[kLayers + 1] = {};  // BF16 copies of h, gathered by the edge kernels
   uint16_t *m[kLayers] = {};
   float *V = nullptr, *W = nullptr, *G = nullptr, *dR = nullptr, *dX = nullptr, *dh = nullptr,
         *de = nullptr, *dz = nullptr, *dU = nullptr, *da = nullptr, *dXu = nullptr,
@@ -907,7 +1073,10 @@ int ensure_work(smh_ctx *ctx, smh_gnn *g, uint32_t n, uint64_t nnz, int K, bool 
   WA(g->lz_part, kMaxTileGrid);
   WA(g->lz_ab, 2 * kLzSteps + 2);
   const int nh = train ? kLayers + 1 : 2;
-  for (int l = 0; l < nh; ++l) WA(g->h[l], N * kH);
+  for (int l = 0; l < nh; ++l) {
+    WA(g->h[l], N * kH);
+    WA(g->hb[l], N * kH);
+  }
   const int nm = train ? kLayers : 1;
   for (int l = 0; l < nm; ++l) WA(g->m[l], E * kH);
   if (train) {
@@ -975,17 +1144,19 @@ int forward(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, bool train) {
   const uint64_t nnz = A->nnz;
   hipStream_t s = ctx->stream;
   SMH_HIP(ctx, hipMemsetAsync(g->err, 0, sizeof(int), s));
-  hipLaunchKernelGGL(k_graph_rows, dim3(grid_for(n, 256)), dim3(256), 0, s, n, A->rowptr,
-                     A->colidx, g->brow, g->deg, g->err);
+  hipLaunchKernelGGL(k_graph_rows, dim3(grid_for(n, kGraphRows)), dim3(kGraphRows), 0, s, n,
+                     A->rowptr, A->colidx, g->brow, g->deg, g->err);
   hipLaunchKernelGGL(k_graph_transpose, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz,
                      A->rowptr, A->colidx, g->brow, g->tmap, g->err);
 #define SMH_FEAT(P, B)                                                                        \
   do {                                                                                        \
     hipLaunchKernelGGL((k_gnn_linv<P, B>), dim3(grid_for(n, 256)), dim3(256), 0, s, n,         \
                        A->rowptr, A->colidx, A->vals, A->vals_lo, g->linv);                   \
-    hipLaunchKernelGGL((k_gnn_features<P, B>), dim3(grid_for(n, 128)), dim3(128), 0, s, n,     \
-                       A->rowptr, A->colidx, A->vals, A->vals_lo, g->linv, g->deg, g->fe,      \
-                       g->fv, train ? g->ahat : nullptr);                                     \
+    hipLaunchKernelGGL((k_gnn_feat_edges<P, B>), dim3(grid_for(nnz, 256)), dim3(256), 0, s,   \
+                       nnz, g->brow, A->colidx, A->vals, A->vals_lo, g->linv, g->fe,           \
+                       train ? g->ahat : nullptr);                                            \
+    hipLaunchKernelGGL((k_gnn_feat_rows<P, B>), dim3(grid_for(n, 128)), dim3(128), 0, s, n,    \
+                       A->rowptr, A->colidx, A->vals, A->vals_lo, g->linv, g->deg, g->fv);     \
   } while (0)
 #define SMH_FEATP(B)                                                                          \
   do {                                                                                        \
@@ -993,8 +1164,22 @@ int forward(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, bool train) {
     else if (A->precision == SMH_F32X2) SMH_FEAT(SMH_F32X2, B);                               \
     else SMH_FEAT(SMH_F32, B);                                                                \
   } while (0)
-  if (A->bdim == 1) SMH_FEATP(1);
-  else SMH_FEATP(3);
+  if (A->bdim == 1) {
+#define SMH_FEAT1(P)                                                                          \
+  do {                                                                                        \
+    hipLaunchKernelGGL((k_gnn_linv<P, 1>), dim3(grid_for(n, 256)), dim3(256), 0, s, n,         \
+                       A->rowptr, A->colidx, A->vals, A->vals_lo, g->linv);                   \
+    hipLaunchKernelGGL((k_gnn_feat_scalar<P>), dim3(grid_for(n, kFeatRows)), dim3(256), 0, s,  \
+                       n, A->rowptr, A->colidx, A->vals, A->vals_lo, g->linv, g->deg, g->fe,  \
+                       g->fv, train ? g->ahat : nullptr);                                     \
+  } while (0)
+    if (A->precision == SMH_F64) SMH_FEAT1(SMH_F64);
+    else if (A->precision == SMH_F32X2) SMH_FEAT1(SMH_F32X2);
+    else SMH_FEAT1(SMH_F32);
+#undef SMH_FEAT1
+  } else {
+    SMH_FEATP(3);
+  }
 #undef SMH_FEATP
 #undef SMH_FEAT
   int herr = 0;
@@ -1004,22 +1189,22 @@ int forward(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, bool train) {
             "learned preconditioner: the pattern must be symmetric with every non-empty row "
             "holding its diagonal block (code %d)", herr);
   hipLaunchKernelGGL(k_gnn_encode<false>, dim3(mfma_grid(n, ctx)), dim3(256), 0, s, (uint64_t)n,
-                     g->fv, g->wt + oWev, g->params + oBev, g->h[0], nullptr);
+                     g->fv, g->wt + oWev, g->params + oBev, g->h[0], g->hb[0]);
   hipLaunchKernelGGL(k_gnn_encode<true>, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz, g->fe,
                      g->wt + oWee, g->params + oBee, nullptr, g->e);
-  float *hin = g->h[0];
+  int cur = 0;  // the layer input h[cur], hb[cur]
   for (int l = 0; l < kLayers; ++l) {
     uint16_t *m = train ? g->m[l] : g->m[0];
-    float *hout = train ? g->h[l + 1] : (hin == g->h[0] ? g->h[1] : g->h[0]);
+    const int nxt = train ? l + 1 : 1 - cur;
     hipLaunchKernelGGL(k_gnn_message, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz, g->brow,
-                       A->colidx, hin, g->e, g->wt + oWm(l), g->params + oBm(l), m);
-    hipLaunchKernelGGL(k_gnn_update, dim3(mfma_grid(n, ctx)), dim3(256), 0, s, n, A->rowptr, hin,
-                       m, g->wt + oWu(l), g->params + oBu(l), hout, train ? g->U[l] : nullptr,
-                       train ? g->a[l] : nullptr);
-    hin = hout;
+                       A->colidx, g->hb[cur], g->e, g->wt + oWm(l), g->params + oBm(l), m);
+    hipLaunchKernelGGL(k_gnn_update, dim3(mfma_grid(n, ctx)), dim3(256), 0, s, n, A->rowptr,
+                       g->h[cur], m, g->wt + oWu(l), g->params + oBu(l), g->h[nxt], g->hb[nxt],
+                       train ? g->U[l] : nullptr, train ? g->a[l] : nullptr);
+    cur = nxt;
   }
   hipLaunchKernelGGL(k_gnn_decode, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz, g->brow,
-                     A->colidx, hin, g->e, g->wt + oWd, g->params + oBd, g->R);
+                     A->colidx, g->hb[cur], g->e, g->wt + oWd, g->params + oBd, g->R);
   SMH_LAUNCH_CHECK(ctx);
   return SMH_OK;
 }
