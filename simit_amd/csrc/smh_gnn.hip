@@ -595,7 +595,9 @@ k_gnn_update(uint32_t n, const uint32_t *__restrict__ rp, const float *__restric
   }
 }
 
-// R_k = [h_i h_j e_k] W_d + b_d, first 9 outputs
+// R_k = [h_i h_j e_k] W_d + b_d: the first RS outputs (9; 1 for scalar
+// systems outside training), R[nnz][RS]
+template <int RS>
 __global__ void __launch_bounds__(256)
 k_gnn_decode(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *__restrict__ ci,
              const uint16_t *__restrict__ hb, const uint16_t *__restrict__ e,
@@ -613,41 +615,60 @@ k_gnn_decode(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *__
     gather_edge_x(a, r, r < nnz, brow, ci, hb, e, lane);
     f32x4 acc[1] = {{0, 0, 0, 0}};
     tile_mma<6, 1>(a, w, acc);
-    // the tile's 16 x 9 outputs are 144 contiguous floats of R: through LDS,
-    // then 36 lanes store 16 bytes each
-    const int c = lane & 15;
-    if (c < 9)
-      for (int v = 0; v < 4; ++v) l[(4 * (lane >> 4) + v) * 9 + c] = acc[0][v] + bias[c];
-    wave_lds_sync();
-    if (lane < 36) {
-      const float4 q = reinterpret_cast<const float4 *>(l)[lane];
-      float *o = R + tile * 144;
-      if (tile * 16 + 16 <= nnz) {
-        reinterpret_cast<float4 *>(o)[lane] = q;
-      } else {  // the last, partial tile
-        const float qq[4] = {q.x, q.y, q.z, q.w};
-        for (int e = 0; e < 4; ++e)
-          if (tile * 16 + (uint64_t)(4 * lane + e) / 9 < nnz) o[4 * lane + e] = qq[e];
+    if constexpr (RS == 1) {
+      // output 0 of rows 4 (lane / 16) + v sits in lanes 0, 16, 32, 48: one
+      // 16-byte store each
+      if ((lane & 15) == 0) {
+        const uint64_t r0 = tile * 16 + 4 * (lane >> 4);
+        float *o = R + r0;
+        if (r0 + 4 <= nnz) {
+          *reinterpret_cast<float4 *>(o) = make_float4(acc[0][0] + bias[0], acc[0][1] + bias[0],
+                                                       acc[0][2] + bias[0], acc[0][3] + bias[0]);
+        } else {
+          for (int v = 0; v < 4; ++v)
+            if (r0 + v < nnz) o[v] = acc[0][v] + bias[0];
+        }
       }
+    } else {
+      // the tile's 16 x 9 outputs are 144 contiguous floats of R: through LDS,
+      // then 36 lanes store 16 bytes each
+      const int c = lane & 15;
+      if (c < 9)
+        for (int v = 0; v < 4; ++v) l[(4 * (lane >> 4) + v) * 9 + c] = acc[0][v] + bias[c];
+      wave_lds_sync();
+      if (lane < 36) {
+        const float4 q = reinterpret_cast<const float4 *>(l)[lane];
+        float *o = R + tile * 144;
+        if (tile * 16 + 16 <= nnz) {
+          reinterpret_cast<float4 *>(o)[lane] = q;
+        } else {  // the last, partial tile
+          const float qq[4] = {q.x, q.y, q.z, q.w};
+          for (int e = 0; e < 4; ++e)
+            if (tile * 16 + (uint64_t)(4 * lane + e) / 9 < nnz) o[4 * lane + e] = qq[e];
+        }
+      }
+      wave_lds_sync();
     }
-    wave_lds_sync();
   }
 }
 
 // ---- output: symmetric, scaled correction --------------------------------------------------
 // B x B blocks (B = 3, or 1 for scalar systems, which use the decoder's first
-// output); R is [nnz][9]
+// output); R is [nnz][9], or [nnz][1] (see out_scale)
 template <int B>
 struct OutScale {
   float S[B * B], phi;
 };
-template <int B>
+// RS: the row stride of R, 9, or 1 when the decoder kept only its first
+// output (scalar systems outside training)
+template <int B, int RS = 9>
 __device__ __forceinline__ OutScale<B> out_scale(const float *R, uint64_t k, uint64_t tk) {
+  static_assert(RS == 9 || B == 1, "R rows of one value hold scalar blocks");
   OutScale<B> o;
   float n2 = 0.0f;
   for (int a = 0; a < B; ++a)
     for (int b = 0; b < B; ++b) {
-      const float s = 0.5f * (R[k * 9 + 3 * a + b] + R[tk * 9 + 3 * b + a]);
+      const float s = 0.5f * (R[k * RS + 3 * a + b] + R[tk * RS + 3 * b + a]);
       o.S[B * a + b] = s;
       n2 += s * s;
     }
@@ -659,7 +680,7 @@ __device__ __forceinline__ OutScale<B> out_scale(const float *R, uint64_t k, uin
 // transpose of the upper one's, so the stored correction is exactly
 // symmetric).  Storage for the solver: B = 3, FP8 of 2^5 C, three values and
 // a zero pad byte per block row (corr); B = 1, FP32 (corr_f).
-template <int B>
+template <int B, int RS = 9>
 __global__ void k_gnn_out(uint64_t nnz, const uint32_t *__restrict__ brow,
                           const uint32_t *__restrict__ ci, const uint32_t *__restrict__ t,
                           const float *__restrict__ R, float sc, uint32_t *__restrict__ corr,
@@ -675,7 +696,7 @@ __global__ void k_gnn_out(uint64_t nnz, const uint32_t *__restrict__ brow,
     const uint64_t tk = t[k];
     const bool lower = tk < k;
     const uint64_t own = lower ? tk : k, oth = lower ? k : tk;
-    const OutScale<B> o = out_scale<B>(R, own, oth);
+    const OutScale<B> o = out_scale<B, RS>(R, own, oth);
     const float f = sc * o.phi;
     for (int a = 0; a < B; ++a)
       for (int b = 0; b < B; ++b) C[B * a + b] = f * (lower ? o.S[B * b + a] : o.S[B * a + b]);
@@ -1203,8 +1224,12 @@ int forward(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, bool train) {
                        train ? g->U[l] : nullptr, train ? g->a[l] : nullptr);
     cur = nxt;
   }
-  hipLaunchKernelGGL(k_gnn_decode, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz, g->brow,
-                     A->colidx, g->hb[cur], g->e, g->wt + oWd, g->params + oBd, g->R);
+  if (!train && A->bdim == 1)
+    hipLaunchKernelGGL(k_gnn_decode<1>, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz,
+                       g->brow, A->colidx, g->hb[cur], g->e, g->wt + oWd, g->params + oBd, g->R);
+  else
+    hipLaunchKernelGGL(k_gnn_decode<9>, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz,
+                       g->brow, A->colidx, g->hb[cur], g->e, g->wt + oWd, g->params + oBd, g->R);
   SMH_LAUNCH_CHECK(ctx);
   return SMH_OK;
 }
@@ -1579,7 +1604,7 @@ int spectral_guard(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, void *corr) {
                          A->nnz, g->brow, A->colidx, g->tmap, g->R, (float)sc, (uint32_t *)corr,
                          (float *)nullptr, (float *)nullptr);
     else
-      hipLaunchKernelGGL(k_gnn_out<1>, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
+      hipLaunchKernelGGL((k_gnn_out<1, 1>), dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
                          A->nnz, g->brow, A->colidx, g->tmap, g->R, (float)sc,
                          (uint32_t *)nullptr, (float *)corr, (float *)nullptr);
     SMH_LAUNCH_CHECK(ctx);
@@ -1607,7 +1632,7 @@ extern "C" int smh_gnn_precond(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, float
                        A->nnz, g->brow, A->colidx, g->tmap, g->R, 1.0f, (uint32_t *)corr,
                        (float *)nullptr, (float *)nullptr);
   else
-    hipLaunchKernelGGL(k_gnn_out<1>, dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
+    hipLaunchKernelGGL((k_gnn_out<1, 1>), dim3(grid_for(A->nnz, 256)), dim3(256), 0, ctx->stream,
                        A->nnz, g->brow, A->colidx, g->tmap, g->R, 1.0f, (uint32_t *)nullptr,
                        (float *)corr, (float *)nullptr);
   SMH_LAUNCH_CHECK(ctx);
