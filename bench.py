@@ -40,10 +40,10 @@ HBM_PEAK_GBS = 8000.0
 # summaries and commands in profiles/.
 # 96^3 x 5 tets: FETCH 237.27 MiB x 2.0 + WRITE 12.60 MiB x 1.0
 SPMV_TRAFFIC_96 = 510_798_243
-# 192^3 hexes, blocked CSR (scalar, FP32 values, FP64 vectors): FETCH 803.30 MiB x 2.0 +
-# WRITE 56.56 MiB x 1.0
-SPMV_TRAFFIC_HEX192 = 1_743_948_909
-# the same matrix in 27-point stencil storage: FETCH 626.88 MiB x 2.0 + WRITE 54.85 MiB x 1.0
+# 192^3 hexes, blocked CSR (scalar, FP32 values, FP64 vectors): FETCH 803.20 MiB x 2.0 +
+# WRITE 56.27 MiB x 1.0
+SPMV_TRAFFIC_HEX192 = 1_743_427_658
+# the same matrix in 27-point stencil storage: FETCH 623.03 MiB x 2.0 + WRITE 54.85 MiB x 1.0
 # (1.54x the algorithmic bytes: FETCH_SIZE counts the x re-reads the Infinity
 # Cache serves as well as HBM reads)
 STENCIL_TRAFFIC_HEX192 = 1_364_080_041
