@@ -263,7 +263,9 @@ typedef struct {
   int32_t mixed;         /* 1: FP32 inner iteration on vals (hi part) with
                             FP64 reliable updates r = b - A x on hi+lo
                             (needs SMH_F32X2); 2: the same with the inner
-                            vectors (p, q, r, z) kept in FP64            */
+                            vectors (p, q, r, z) kept in FP64 (over
+                            stencil storage the learned correction's
+                            input u = Linv r is gathered in FP32)         */
   double reliable_delta; /* mixed: update when ||r|| < delta*max ||r||
                             (default 0.3)                                   */
   int32_t check_every;   /* iterations between host convergence checks     */

@@ -282,11 +282,13 @@ __global__ void __launch_bounds__(256) k_cg_r1(CGState *S, const double *part, i
   }
 }
 
-// K2: x += alpha p; r -= alpha q; z = M r; partials of z'r and r'r
+// K2: x += alpha p; r -= alpha q; z = M r; partials of z'r and r'r.  z32:
+// z is stored as float (the learned correction's input u = Linv r when the
+// correction pass gathers FP32 u beside FP64 vectors)
 template <typename VT, int BD>
 __global__ void __launch_bounds__(256)
 k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT *__restrict__ r,
-            VT *__restrict__ z, const VT *__restrict__ p, const VT *__restrict__ q,
+            void *__restrict__ z, int z32, const VT *__restrict__ p, const VT *__restrict__ q,
             const CGState *S, double *part) {
   __shared__ double s4[4];
   if (S->done) return;
@@ -304,8 +306,9 @@ k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT
     }
     apply_precond<BD>(pc, rv, zv, dinv, sizeof(VT) == 4, i);
     for (int a = 0; a < BD; ++a) {
-      const VT zz = (VT)zv[a];
-      z[i * BD + a] = zz;
+      const VT zz = z32 ? (VT)(float)zv[a] : (VT)zv[a];
+      if (z32) ((float *)z)[i * BD + a] = (float)zv[a];
+      else ((VT *)z)[i * BD + a] = zz;
       drz += (double)zz * rv[a];
       drr += rv[a] * rv[a];
     }
@@ -323,13 +326,15 @@ k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT
 // scalar systems), and the workgroup partials of z'r and r'r.  check_done = 0
 // on the residual-replacement path, which runs after the stopping test
 // already set done.
-template <typename VT, int BD>
+template <typename VT, int BD, typename UT = VT>
 struct EpiLearned {
-  const VT *u, *r;
+  const UT *u;
+  const VT *r;
   VT *z;
   const float *linv;
   double drz, drr;
-  __device__ void operator()(uint32_t row, const VT *s) {
+  template <typename AT>  // the row loop's accumulator type
+  __device__ void operator()(uint32_t row, const AT *s) {
     if constexpr (BD == 1) {
       const double t = (double)linv[row] * ((double)u[row] + (double)s[0]);
       const VT zz = (VT)t;
@@ -429,9 +434,11 @@ k_cg_pupd(uint64_t n, VT *__restrict__ p, const VT *__restrict__ z, const CGStat
 template <typename VT, int BD>
 struct EpiResidual {
   const double *b;
-  VT *r, *z, *p;
+  VT *r;
+  void *z;  // VT, or float when z32 (as k_cg_update)
+  VT *p;
   const void *dinv;
-  int pc, init;
+  int pc, init, z32;
   double drz, drr;
   __device__ void operator()(uint32_t row, const double *s) {
     double rv[BD], zv[BD];
@@ -445,8 +452,9 @@ struct EpiResidual {
     apply_precond<BD>(pc, rv, zv, dinv, sizeof(VT) == 4, row);
     for (int a = 0; a < BD; ++a) {
       const uint64_t o = (uint64_t)row * BD + a;
-      const VT zz = (VT)zv[a];
-      z[o] = zz;
+      const VT zz = z32 ? (VT)(float)zv[a] : (VT)zv[a];
+      if (z32) ((float *)z)[o] = (float)zv[a];
+      else ((VT *)z)[o] = zz;
       if (init) p[o] = zz;
       drz += (double)zz * rv[a];
     }
@@ -712,16 +720,18 @@ k_residual_st(StDev A, const double *__restrict__ b, const double *__restrict__ 
   }
 }
 
-template <int LP>
+// u in the correction's vector type (Val<LP>::T: float for kF16F, also under
+// FP64 r and z, which keeps the 27 gathered values in 27 registers)
+template <int LP, typename VT>
 __global__ void __launch_bounds__(256)
-k_cg_lapply_st(StDev C, const typename Val<LP>::T *__restrict__ u,
-               const typename Val<LP>::T *__restrict__ r, typename Val<LP>::T *__restrict__ z,
-               const float *__restrict__ linv, const CGState *S, int check_done, double *part) {
-  using VT = typename Val<LP>::T;
+k_cg_lapply_st(StDev C, const typename Val<LP>::T *__restrict__ u, const VT *__restrict__ r,
+               VT *__restrict__ z, const float *__restrict__ linv, const CGState *S,
+               int check_done, double *part) {
+  using UT = typename Val<LP>::T;
   __shared__ double red[4];
   if (check_done && S->done) return;
-  XPlain<VT> xg{u, 1};
-  EpiLearned<VT, 1> epi{u, r, z, linv, 0.0, 0.0};
+  XPlain<UT> xg{u, 1};
+  EpiLearned<VT, 1, UT> epi{u, r, z, linv, 0.0, 0.0};
   stencil_rows<LP>(C, xg, epi);
   const double t1 = block_sum256(epi.drz, red);
   const double t2 = block_sum256(epi.drr, red);
@@ -834,10 +844,12 @@ struct StencilOps {
                 double *part) const {
     hipLaunchKernelGGL((k_residual_st<RP, VT>), dim3(g), dim3(256), 0, s, A, b, x, epi, part);
   }
-  void corr(hipStream_t s, const VT *u, const VT *r, VT *z, const float *linv, const CGState *S,
+  // the correction pass reads u in the correction's vector type
+  static constexpr bool kU32 = sizeof(typename Val<LP>::T) == 4 && sizeof(VT) == 8;
+  void corr(hipStream_t s, const void *u, const VT *r, VT *z, const float *linv, const CGState *S,
             int check_done, double *part) const {
-    hipLaunchKernelGGL((k_cg_lapply_st<LP>), dim3(g), dim3(256), 0, s, C, u, r, z, linv, S,
-                       check_done, part);
+    hipLaunchKernelGGL((k_cg_lapply_st<LP, VT>), dim3(g), dim3(256), 0, s, C,
+                       (const typename Val<LP>::T *)u, r, z, linv, S, check_done, part);
   }
   void lo_rowsum(hipStream_t s, float *dlo) const {
     hipLaunchKernelGGL(k_lo_rowsum_st, dim3(grid_for(A.n, 256)), dim3(256), 0, s, A.n,
@@ -887,10 +899,11 @@ struct TileOps {
     hipLaunchKernelGGL((k_residual<RP, BD, VT>), dim3(grt), dim3(tile_threads_host(BD)), 0, s, TR,
                        b, x, epi, part);
   }
-  void corr(hipStream_t s, const VT *u, const VT *r, VT *z, const float *linv, const CGState *S,
+  static constexpr bool kU32 = false;
+  void corr(hipStream_t s, const void *u, const VT *r, VT *z, const float *linv, const CGState *S,
             int check_done, double *part) const {
     hipLaunchKernelGGL((k_cg_lapply<LP, BD>), dim3(gc), dim3(tile_threads_host(BD)), 0, s, TC,
-                       u, r, z, linv, S, check_done, part);
+                       (const VT *)u, r, z, linv, S, check_done, part);
   }
   void lo_rowsum(hipStream_t s, float *dlo) const {
     hipLaunchKernelGGL(k_lo_rowsum, dim3(grid_for((uint64_t)A->nrows * BD * BD, 256)), dim3(256),
@@ -948,6 +961,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   auto T0 = std::chrono::steady_clock::now();
   CGState hs{};
   int nru = 0, batch = 0;
+  const int z32 = learned && OPS::kU32;  // the learned correction's u stored as float
   EpiResidual<VT, BD> epi{};
 #define TRY(x)            \
   do {                    \
@@ -991,7 +1005,8 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   if (pc != SMH_PRECOND_NONE && !learned) ops.dinv(ctx->stream, pc, (VT *)w.dinv);
   epi.b = b;
   epi.r = (VT *)w.r;
-  epi.z = (VT *)(learned ? w.u : w.z);  // learned: u = Linv r, z follows in k_cg_lapply
+  epi.z = learned ? w.u : w.z;  // learned: u = Linv r, z follows in k_cg_lapply
+  epi.z32 = z32;
   epi.p = (VT *)w.p;
   epi.dinv = pdata;
   epi.pc = pc;
@@ -999,7 +1014,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   epi.init = learned ? 0 : 1;
   ops.residual(ctx->stream, b, (const double *)x, epi, w.part);
   if (learned) {
-    ops.corr(ctx->stream, (const VT *)w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
+    ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
              (const CGState *)w.S, 0, w.part);
     hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
                        (const double *)w.part, (int)gc, 0);
@@ -1024,8 +1039,8 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
       batch = std::max(2, batch / 4);
       ops.residual(ctx->stream, b, (const double *)x, epi, w.part);
       if (learned)
-        ops.corr(ctx->stream, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
-                 (const float *)o->learned, (const CGState *)w.S, 0, w.part);
+        ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
+                 (const CGState *)w.S, 0, w.part);
       hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
                          (const double *)w.part, (int)(learned ? gc : grt), 0);
       hipLaunchKernelGGL((k_cg_restart<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
@@ -1040,11 +1055,11 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
       hipLaunchKernelGGL(k_cg_r1, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
                          (int)gt);
       hipLaunchKernelGGL((k_cg_update<VT, BD>), dim3(nbe), dim3(256), 0, ctx->stream, nrows, pc,
-                         pdata, x, (VT *)w.r, (VT *)(learned ? w.u : w.z), (const VT *)w.p,
+                         pdata, x, (VT *)w.r, learned ? w.u : w.z, z32, (const VT *)w.p,
                          (const VT *)w.q, (const CGState *)w.S, w.part + kPartSlots);
       if (learned)
-        ops.corr(ctx->stream, (const VT *)w.u, (const VT *)w.r, (VT *)w.z,
-                 (const float *)o->learned, (const CGState *)w.S, 1, w.part + kPartSlots);
+        ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
+                 (const CGState *)w.S, 1, w.part + kPartSlots);
       hipLaunchKernelGGL(k_cg_r2, dim3(1), dim3(256), 0, ctx->stream, w.S,
                          (const double *)(w.part + kPartSlots), (int)(learned ? gc : nbe));
       hipLaunchKernelGGL((k_cg_pupd<VT>), dim3(nbv), dim3(256), 0, ctx->stream, n, (VT *)w.p,
@@ -2022,7 +2037,8 @@ template <int IP, int RP>
 int cg_stencil(smh_ctx *ctx, const smh_stencil27 *S, const double *b, double *x,
                const smh_cg_opts *o, smh_cg_stats *st) {
   using VT = typename Val<IP>::T;
-  constexpr int LP = sizeof(VT) == 4 ? kF16F : kF16D;  // the correction's FP16 slots
+  // the correction's FP16 slots, gathered u in FP32 unless the solve is all FP64
+  constexpr int LP = IP == SMH_F64 ? kF16D : kF16F;
   StencilOps<IP, RP, LP> ops;
   ops.A = st_dev(S, S->vals, S->vals_lo);
   ops.C = st_dev(S, o->precond == SMH_PRECOND_LEARNED ? o->learned_corr : S->vals, nullptr);

@@ -152,7 +152,8 @@ def test_stencil_learned_correction(ctx, mixed):
     z = Bm @ sy["rhs"].cpu().numpy()
     xg = x.cpu().numpy()
     err = np.linalg.norm(xg / np.linalg.norm(xg) - z / np.linalg.norm(z))
-    assert err <= (1e-6 if mixed == 1 else 1e-12), err
+    # mixed 1 and 2 gather u = Linv r in FP32 (mixed 2 keeps FP64 r and z)
+    assert err <= (1e-6 if mixed else 1e-12), err
     # the slots hold the half-rounded correction, exactly
     slots = c16.view(torch.float16).cpu().numpy().astype(np.float64).reshape(27, n)
     offs = offsets(*sy["dims"][:2])
