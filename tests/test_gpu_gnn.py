@@ -68,8 +68,12 @@ def random_params(seed, scale=1.0):
     return G.flatten(p).astype(np.float32)
 
 
-def test_forward_matches_oracle(ctx):
-    sy = tet_system(ctx, 4, seed=1)
+# n = 22: ~148k blocks, so the edge kernels run the full-chip grid (2,048
+# workgroups) and take their tiles by XCD; n = 4 runs a small grid of plain
+# strided tiles.  The dense spectral check is for n = 4.
+@pytest.mark.parametrize("n", [4, 22])
+def test_forward_matches_oracle(ctx, n):
+    sy = tet_system(ctx, n, seed=1)
     idx = sy["idx"]
     rp, ci = idx.to_host()
     vals = sy["v64"].cpu().numpy()
@@ -100,7 +104,9 @@ def test_forward_matches_oracle(ctx):
     assert close.mean() > 0.95, close.mean()
     assert np.linalg.norm(Cg - Co) <= 5e-2 * np.linalg.norm(Co), np.linalg.norm(Cg - Co)
     assert np.abs(Cg - Co).max() <= 0.07 * scale
-    check_guard(info, dense_blocks(rp, ci, fp8_round(aux["C"])), dense_blocks(rp, ci, Cg), 0.9)
+    if n <= 8:
+        check_guard(info, dense_blocks(rp, ci, fp8_round(aux["C"])), dense_blocks(rp, ci, Cg),
+                    0.9)
     net.close()
 
 

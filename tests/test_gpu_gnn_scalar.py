@@ -51,8 +51,12 @@ def corr_values(corr):
     return corr.view(torch.float32).cpu().numpy().astype(np.float64)
 
 
-def test_scalar_forward_matches_oracle(ctx):
-    sy = hex_system(ctx, 6, seed=1)
+# n = 17: 140,608 entries, so the edge kernels run the full-chip grid (2,048
+# workgroups, a multiple of 8) and take their tiles by XCD; n = 6 runs the
+# small grid of plain strided tiles.  The dense spectral check is for n = 6.
+@pytest.mark.parametrize("n", [6, 17])
+def test_scalar_forward_matches_oracle(ctx, n):
+    sy = hex_system(ctx, n, seed=1)
     idx = sy["idx"]
     rp, ci = idx.to_host()
     vals = sy["v64"].cpu().numpy()
@@ -74,7 +78,8 @@ def test_scalar_forward_matches_oracle(ctx):
     assert scale > 1e-3
     assert np.linalg.norm(Cg - Co) <= 5e-2 * np.linalg.norm(Co), np.linalg.norm(Cg - Co)
     assert np.abs(Cg - Co).max() <= 0.07 * scale
-    check_guard(info, dense_blocks(rp, ci, aux["C"].ravel()), dense_blocks(rp, ci, Cg), 0.9)
+    if n <= 8:
+        check_guard(info, dense_blocks(rp, ci, aux["C"].ravel()), dense_blocks(rp, ci, Cg), 0.9)
     net.close()
 
 
