@@ -1155,10 +1155,22 @@ int check_A(smh_ctx *ctx, const smh_bsr *A, const char *who) {
   return SMH_OK;
 }
 
-unsigned mfma_grid(uint64_t rows, const smh_ctx *ctx) {
+// the grid of a 16-row-tile MFMA kernel: one workgroup per 4 tiles, at most
+// per_cu resident workgroups per CU (8: 8 waves per SIMD)
+unsigned mfma_grid(uint64_t rows, const smh_ctx *ctx, unsigned per_cu = 8) {
   const uint64_t tiles = (rows + 15) / 16;
   const uint64_t wgs = (tiles + 3) / 4;
-  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wgs, (uint64_t)ctx->num_cu * 8));
+  return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(wgs, (uint64_t)ctx->num_cu * per_cu));
+}
+
+// 256-thread workgroups of a kernel resident on one CU at once (8 if the
+// runtime cannot say)
+template <typename K>
+unsigned resident_wgs(K kernel) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, 256, 0) != hipSuccess || nb < 1)
+    nb = 8;
+  return (unsigned)std::min(nb, 8);
 }
 
 // graph, features and the forward pass; leaves R (and the layer caches when
@@ -1217,11 +1229,15 @@ int forward(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, bool train) {
   hipLaunchKernelGGL(k_gnn_encode<true>, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz, g->fe,
                      g->wt + oWee, g->params + oBee, nullptr, g->e);
   int cur = 0;  // the layer input h[cur], hb[cur]
+  // k_gnn_message keeps its weights in registers (6 waves per SIMD): a grid
+  // of what is resident, so no second round of workgroups trails the first
+  static const unsigned msg_per_cu = resident_wgs(k_gnn_message);
   for (int l = 0; l < kLayers; ++l) {
     uint16_t *m = train ? g->m[l] : g->m[0];
     const int nxt = train ? l + 1 : 1 - cur;
-    hipLaunchKernelGGL(k_gnn_message, dim3(mfma_grid(nnz, ctx)), dim3(256), 0, s, nnz, g->brow,
-                       A->colidx, g->hb[cur], g->e, g->wt + oWm(l), g->params + oBm(l), m);
+    hipLaunchKernelGGL(k_gnn_message, dim3(mfma_grid(nnz, ctx, msg_per_cu)), dim3(256), 0, s, nnz,
+                       g->brow, A->colidx, g->hb[cur], g->e, g->wt + oWm(l), g->params + oBm(l),
+                       m);
     hipLaunchKernelGGL(k_gnn_update, dim3(mfma_grid(n, ctx)), dim3(256), 0, s, n, A->rowptr,
                        g->h[cur], m, g->wt + oWu(l), g->params + oBu(l), g->h[nxt], g->hb[nxt],
                        train ? g->U[l] : nullptr, train ? g->a[l] : nullptr);
