@@ -2,11 +2,13 @@
 //
 // The CG loop is what Simit programs write inline (apps/springs/isprings.sim:
 // 77-93, apps/fem/fem_linear.sim:211-230, test/input/program/cg.sim:25-53,
-// apps/thermal/thermal.sim:75-130).  One iteration here is five launches on
-// one stream, with every scalar kept on the device:
+// apps/thermal/thermal.sim:75-130).  One iteration here is four launches on
+// one stream (five with the learned correction's pass), with every scalar
+// kept on the device:
 //   K1  q = A p (row tiles) + FP64 partial of p'q per workgroup;
-//   R1  one workgroup: p'q in a fixed order, alpha = r'z / p'q;
-//   K2  x += alpha p; r -= alpha q; z = M r; partials of z'r and r'r;
+//   K2  every workgroup sums the p'q partials in one fixed order (so all
+//       hold the same alpha = r'z / p'q); x += alpha p; r -= alpha q;
+//       z = M r; partials of z'r and r'r;
 //   R2  one workgroup: z'r, r'r, beta, the stopping test;
 //   K3  p = z + beta p.
 // Cross-workgroup results only pass through kernel boundaries (no
@@ -266,33 +268,33 @@ k_cg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP>
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-// R1: alpha
-__global__ void __launch_bounds__(256) k_cg_r1(CGState *S, const double *part, int np) {
-  __shared__ double s4[4];
-  if (S->done) return;
-  const double pq = wg_sum(part, np, 1, s4);
-  if (threadIdx.x == 0) {
-    S->pq = pq;
-    if (!finite_pos(pq)) {
-      S->status = SMH_ERR_BREAKDOWN;
-      S->done = 1;
-    } else {
-      S->alpha = S->rz / pq;
-    }
-  }
-}
-
-// K2: x += alpha p; r -= alpha q; z = M r; partials of z'r and r'r.  z32:
-// z is stored as float (the learned correction's input u = Linv r when the
-// correction pass gathers FP32 u beside FP64 vectors)
+// K2: alpha = z'r / p'q (R1 folded in: every workgroup sums K1's np
+// partials of p'q in the same fixed order, so all hold the same alpha, and
+// workgroup 0 records it); x += alpha p; r -= alpha q; z = M r; partials of
+// z'r and r'r.  z32: z is stored as float (the learned correction's input
+// u = Linv r when the correction pass gathers FP32 u beside FP64 vectors)
 template <typename VT, int BD>
 __global__ void __launch_bounds__(256)
 k_cg_update(uint32_t nrows, int pc, const void *dinv, double *__restrict__ x, VT *__restrict__ r,
             void *__restrict__ z, int z32, const VT *__restrict__ p, const VT *__restrict__ q,
-            const CGState *S, double *part) {
+            CGState *S, const double *pq_part, int np, double *part) {
   __shared__ double s4[4];
   if (S->done) return;
-  const double alpha = S->alpha;
+  const double pq = wg_sum(pq_part, np, 1, s4);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (!finite_pos(pq)) {  // the same verdict in every workgroup
+    if (lead) {
+      S->pq = pq;
+      S->status = SMH_ERR_BREAKDOWN;
+      S->done = 1;
+    }
+    return;
+  }
+  const double alpha = S->rz / pq;
+  if (lead) {
+    S->pq = pq;
+    S->alpha = alpha;
+  }
   double drz = 0.0, drr = 0.0;
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < nrows;
        i += (uint64_t)gridDim.x * 256) {
@@ -1052,11 +1054,10 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
     for (int k = 0; k < batch; ++k) {
       ops.spmv(ctx->stream, (const VT *)w.p, (VT *)w.q, (const CGState *)w.S,
                (const float *)w.dlo, w.part);
-      hipLaunchKernelGGL(k_cg_r1, dim3(1), dim3(256), 0, ctx->stream, w.S, (const double *)w.part,
-                         (int)gt);
       hipLaunchKernelGGL((k_cg_update<VT, BD>), dim3(nbe), dim3(256), 0, ctx->stream, nrows, pc,
                          pdata, x, (VT *)w.r, learned ? w.u : w.z, z32, (const VT *)w.p,
-                         (const VT *)w.q, (const CGState *)w.S, w.part + kPartSlots);
+                         (const VT *)w.q, w.S, (const double *)w.part, (int)gt,
+                         w.part + kPartSlots);
       if (learned)
         ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
                  (const CGState *)w.S, 1, w.part + kPartSlots);
