@@ -186,12 +186,15 @@ def main():
         return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
 
     # ---- the step: assemble + network + learned mixed PCG to 1e-8 ---------------------------------
+    last = {}
+
     def step():
         assemble()
         linv, corr = net.precond(A2)
         x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
         st = ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=1, learned=linv,
                                      learned_corr=corr))
+        last["x"] = x
         return st
 
     for w in range(args.warmup):
@@ -211,6 +214,16 @@ def main():
 
     # precond inference time alone
     pre_ms = timed(lambda: net.precond(A2), 3)
+    # the timed step's solution against the oracle's FP64 matrix and load
+    x_step = last["x"].cpu().numpy()
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    rp, ci = idx.to_host()
+    t0 = time.perf_counter()
+    ov, of = O.tet_assemble(O.tet_params(load=prob["load"]), prob, rp, ci)
+    t_asm = time.perf_counter() - t0
+    oracle_rel = float(np.linalg.norm(of - O.bsrmv(rp, ci, ov, x_step, 3)) / np.linalg.norm(of))
+    log(f"[bench] oracle check of the step's x: ||b - A x|| / ||b|| = {oracle_rel:.3e}")
 
     # ---- block-Jacobi PCG (mixed, FP64) and plain FP64 CG on the same GPU ----------------------------
     def leg(name, fn):
@@ -250,12 +263,6 @@ def main():
     # ---- CPU baseline: the oracle on a bounded sample of the same workload ----------------------------
     cpu = cpu_omp = None
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import pyoracle as O
-        rp, ci = idx.to_host()
-        t0 = time.perf_counter()
-        ov, of = O.tet_assemble(O.tet_params(load=prob["load"]), prob, rp, ci)
-        t_asm = time.perf_counter() - t0
         t0 = time.perf_counter()
         O.cg(rp, ci, ov, of, np.zeros(ndof), 3, O.cg_opts(maxiters=2, tol=1e-8, precond=2))
         t2 = time.perf_counter() - t0
@@ -298,6 +305,10 @@ def main():
         "iterations": stats[-1]["iters"],
         "converged": converged,
         "true_rel": stats[-1]["true_rel"],
+        "oracle_true_rel": oracle_rel,
+        "oracle_check": ("||b - A x|| / ||b|| of the last timed step's x with the oracle's "
+                         "FP64 matrix and load (single-thread C restatement, assembled on "
+                         "the host)"),
         "plain_cg_f64": plain,
         "speedup_vs_plain_cg_f64": (round(plain["seconds"] / step_s, 2)
                                     if plain and plain["converged"] else None),
@@ -421,6 +432,7 @@ def run_hex(args, ctx):
         return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
 
     corr_st = torch.empty(27 * nv, dtype=torch.int16, device="cuda")  # FP16 slots
+    last = {}
 
     def step():
         assemble()
@@ -428,6 +440,7 @@ def run_hex(args, ctx):
         linv, corr = net.precond(A2)
         ctx.stencil27_correction(idx, corr, dims, corr_st)
         x = torch.zeros(nv, dtype=torch.float64, device="cuda")
+        last["x"] = x
         return ctx.cg_stencil27(St, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=2,
                                                  learned=linv, learned_corr=corr_st))
 
@@ -442,6 +455,16 @@ def run_hex(args, ctx):
     log(f"[bench] step {step_s:.3f}s iters={stats[-1]['iters']} ru={stats[-1]['reliable_updates']} "
         f"true_rel={stats[-1]['true_rel']:.2e}")
     pre_ms = timed(lambda: net.precond(A2), 3)
+    # the timed step's solution against the oracle's FP64 matrix and source
+    x_step = last["x"].cpu().numpy()
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import pyoracle as O
+    rp, ci = idx.to_host()
+    t0 = time.perf_counter()
+    ov, of = O.hex_assemble(O.HexParams(prob["h"], prob["source"], 1.0), prob, rp, ci)
+    t_asm = time.perf_counter() - t0
+    oracle_rel = float(np.linalg.norm(of - O.bsrmv(rp, ci, ov, x_step, 1)) / np.linalg.norm(of))
+    log(f"[bench] oracle check of the step's x: ||b - A x|| / ||b|| = {oracle_rel:.3e}")
 
     def jac_mixed(stencil):
         torch.cuda.synchronize()
@@ -486,12 +509,6 @@ def run_hex(args, ctx):
                                            min(args.plain_maxiters, 20000))
     cpu = cpu_omp = None
     if not args.no_cpu:
-        sys.path.insert(0, str(ROOT / "oracle"))
-        import pyoracle as O
-        rp, ci = idx.to_host()
-        t0 = time.perf_counter()
-        ov, of = O.hex_assemble(O.HexParams(prob["h"], prob["source"], 1.0), prob, rp, ci)
-        t_asm = time.perf_counter() - t0
         t0 = time.perf_counter()
         O.cg(rp, ci, ov, of, np.zeros(nv), 1, O.cg_opts(maxiters=2, tol=1e-8, precond=1))
         t2 = time.perf_counter() - t0
@@ -521,6 +538,10 @@ def run_hex(args, ctx):
                    "nverts": nv, "nhex": nh, "nnz": nnz,
                    "train": f"{args.hex_train_samples} seeded {args.hex_train_n}^3 samples"},
         "iterations": stats[-1]["iters"], "true_rel": stats[-1]["true_rel"],
+        "oracle_true_rel": oracle_rel,
+        "oracle_check": ("||b - A x|| / ||b|| of the last timed step's x with the oracle's "
+                         "FP64 matrix and source (single-thread C restatement, assembled on "
+                         "the host)"),
         "reliable_updates": stats[-1]["reliable_updates"],
         "converged": all(s["status"] == 0 and s["true_rel"] <= 1e-8 for s in stats),
         "jacobi_pcg_mixed": jacm, "jacobi_pcg_mixed_csr": jacm_csr,

@@ -282,14 +282,24 @@ typedef struct {
                             values and a zero pad byte per block row; 1x1
                             blocks: float[nnz]; the preconditioner is
                             z_i = Linv_i^T (Linv_i r_i + sum_k C_k Linv_j r_j) */
+  int32_t fallback;      /* 1 (default): when the solve breaks down (p'Ap <= 0,
+                            z'r <= 0 with a preconditioner, NaN) -- or, with
+                            the learned preconditioner, ends unconverged --
+                            solve again from the initial guess with (block)
+                            Jacobi (after the learned one), then with no
+                            preconditioner; each rung is certified by its
+                            own FP64 residual.  0: report the first status */
 } smh_cg_opts;
 typedef struct {
-  int32_t iters;
+  int32_t iters;         /* iterations, summed over the rungs of the chain  */
   int32_t reliable_updates;
   int32_t status;        /* SMH_OK / SMH_ERR_BREAKDOWN / SMH_ERR_NOT_CONVERGED */
   double rr;             /* final iterated r'r                              */
   double true_rel;       /* ||b - A x|| / ||b|| in FP64 at exit             */
   double seconds;        /* wall time of the solve (host clock, synced)     */
+  int32_t precond_used;  /* the preconditioner whose solve produced x       */
+  int32_t fallbacks;     /* rungs of the fallback chain taken (0: none)      */
+  int32_t first_status;  /* status of the requested preconditioner's solve   */
 } smh_cg_stats;
 void smh_cg_default_opts(smh_cg_opts *o);
 int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,

@@ -107,7 +107,7 @@ class CGOptions(C.Structure):
                 ("precond", C.c_int32), ("mixed", C.c_int32),
                 ("reliable_delta", C.c_double), ("check_every", C.c_int32),
                 ("learned", C.c_void_p), ("zr_stop", C.c_int32),
-                ("learned_corr", C.c_void_p)]
+                ("learned_corr", C.c_void_p), ("fallback", C.c_int32)]
 
 
 class CGBatchStats(C.Structure):
@@ -119,7 +119,9 @@ class CGBatchStats(C.Structure):
 
 class CGStats(C.Structure):
     _fields_ = [("iters", C.c_int32), ("reliable_updates", C.c_int32), ("status", C.c_int32),
-                ("rr", C.c_double), ("true_rel", C.c_double), ("seconds", C.c_double)]
+                ("rr", C.c_double), ("true_rel", C.c_double), ("seconds", C.c_double),
+                ("precond_used", C.c_int32), ("fallbacks", C.c_int32),
+                ("first_status", C.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -370,11 +372,20 @@ class Context:
     def cg_options(self, **kw) -> CGOptions:
         o = CGOptions()
         self.lib.smh_cg_default_opts(C.byref(o))
+        keep = []
         for k, v in kw.items():
             if k in ("learned", "learned_corr"):
+                keep.append(v)  # the struct holds a raw pointer: keep the buffer alive
                 v = _ptr(v)
             setattr(o, k, v)
+        o._keep = tuple(keep)
         return o
+
+    def bsr_forget(self, A):
+        """Drop the cached row-tile plan of A's pattern (call before its
+        rowptr buffer is freed or refilled with another pattern)."""
+        rp = A.rowptr if isinstance(A, Bsr) else _ptr(A)
+        self._check(self.lib.smh_bsr_forget(self.h, rp))
 
     def gnn(self, seed: int = 0, kappa: float = 0.9) -> "Gnn":
         return Gnn(self, seed, kappa)

@@ -204,6 +204,8 @@ class HipFunction : public backend::Function {
     stats.status = st.status;
     stats.true_rel = st.true_rel;
     stats.seconds = st.seconds;
+    stats.precond_used = st.precond_used;
+    stats.fallbacks = st.fallbacks;
   }
   void release() {
     smh_ctx *ctx = nullptr;

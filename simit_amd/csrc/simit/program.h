@@ -56,6 +56,8 @@ struct SolveStats {
   int status = 0;
   double true_rel = 0.0;
   double seconds = 0.0;
+  int precond_used = 0;  // the rung of the fallback chain that produced x
+  int fallbacks = 0;
 };
 
 namespace backend {

@@ -168,6 +168,7 @@ k_bsrmv_tiles(TileDev A, const typename Val<PREC>::T *__restrict__ x,
 struct CGState {
   double rz, pq, alpha, beta, rr, rr_true, thr, rr_ref, bb, delta2;
   int iter, done, status, want_ru, claimed, maxiters, precond_stop, mixed, restart, replace_every;
+  int precond;  // a preconditioner is applied: z'r must stay positive
 };
 
 __device__ __forceinline__ bool finite_pos(double v) { return v > 0.0 && v < INFINITY; }
@@ -404,6 +405,11 @@ __global__ void __launch_bounds__(256) k_cg_r2(CGState *S, const double *part, i
       S->rz = rz;
       S->claimed = 1;  // verified against the FP64 residual by the host
       S->done = 1;
+    } else if (S->precond && !(rz > 0.0)) {
+      // an indefinite preconditioner (z'r <= 0): the recurrence has no
+      // meaning from here on
+      S->status = SMH_ERR_BREAKDOWN;
+      S->done = 1;
     } else if (S->iter >= S->maxiters) {
       S->done = 1;
     } else if ((S->mixed && rr < S->delta2 * S->rr_ref) ||
@@ -516,6 +522,9 @@ __global__ void __launch_bounds__(256) k_cg_rres(CGState *S, const double *part,
     S->done = 1;
   } else if (stop_test(S, rr, rz)) {
     S->done = 1;
+  } else if (S->precond && !(rz > 0.0)) {
+    S->status = SMH_ERR_BREAKDOWN;
+    S->done = 1;
   } else if (S->status == SMH_OK) {
     S->done = S->iter >= S->maxiters ? 1 : 0;
   }
@@ -575,7 +584,7 @@ __global__ void k_dinv(uint32_t nrows, const uint32_t *__restrict__ rowptr,
 
 __global__ void k_cg_init_state(CGState *S, double tol, int rel, int norm_form, int maxiters,
                                 int precond_stop, double delta, int mixed, int replace_every,
-                                const double *bbp) {
+                                int precond, const double *bbp) {
   if (threadIdx.x || blockIdx.x) return;
   const double bb = *bbp;
   memset(S, 0, sizeof(CGState));
@@ -588,6 +597,7 @@ __global__ void k_cg_init_state(CGState *S, double tol, int rel, int norm_form, 
   S->delta2 = delta * delta;
   S->mixed = mixed;
   S->replace_every = replace_every;
+  S->precond = precond;
 }
 
 __global__ void __launch_bounds__(256) k_dot_wg(uint64_t n, const double *__restrict__ x,
@@ -1003,7 +1013,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   hipLaunchKernelGGL(k_cg_init_state, dim3(1), dim3(1), 0, ctx->stream, w.S, o->tol, o->rel,
                      o->norm_form, o->maxiters, (pc != SMH_PRECOND_NONE && o->zr_stop) ? 1 : 0,
                      o->reliable_delta, o->mixed, o->replace_every,
-                     (const double *)(w.part + 2 * kPartSlots));
+                     pc != SMH_PRECOND_NONE ? 1 : 0, (const double *)(w.part + 2 * kPartSlots));
   if (pc != SMH_PRECOND_NONE && !learned) ops.dinv(ctx->stream, pc, (VT *)w.dinv);
   epi.b = b;
   epi.r = (VT *)w.r;
@@ -1283,7 +1293,8 @@ k_bcg_update(BChunks C, int pc, const void *dinv, double *__restrict__ x, VT *__
 // system)
 __global__ void __launch_bounds__(256) k_bcg_beta(BChunks C, const double *cpz,
                                                   const double *cpr, BDev S, int maxiters,
-                                                  int mixed, double delta2, int replace_every) {
+                                                  int mixed, double delta2, int replace_every,
+                                                  int precond) {
   const uint32_t m = blockIdx.x * 256 + threadIdx.x;
   if (m >= C.nb || S.state[m] != kBActive) return;
   const double rz = sys_sum(C, m, cpz), rr = sys_sum(C, m, cpr);
@@ -1295,6 +1306,8 @@ __global__ void __launch_bounds__(256) k_bcg_beta(BChunks C, const double *cpz,
     S.state[m] = kBBreakdown;
   } else if (!(rr > S.thr[m])) {
     S.state[m] = kBClaimed;
+  } else if (precond && !(rz > 0.0)) {
+    S.state[m] = kBBreakdown;  // an indefinite preconditioner
   } else if (it >= maxiters) {
     S.state[m] = kBDoneMaxit;
   } else if ((mixed && rr < delta2 * S.rr_ref[m]) ||
@@ -1380,7 +1393,8 @@ k_bcg_residual(TileDev A, const double *__restrict__ x, EpiResRow<VT, BD> epi) {
 // beta from the replaced residual for the requests; mode 1: start of the
 // solve; mode 2: the final certification of every system.  A thread per system.
 __global__ void __launch_bounds__(256) k_bcg_rres(BChunks C, const double *cpz,
-                                                  const double *cpr, BDev S, int mode) {
+                                                  const double *cpr, BDev S, int mode,
+                                                  int precond) {
   const uint32_t m = blockIdx.x * 256 + threadIdx.x;
   if (m >= C.nb) return;
   const int st = S.state[m];
@@ -1407,6 +1421,8 @@ __global__ void __launch_bounds__(256) k_bcg_rres(BChunks C, const double *cpz,
     S.state[m] = kBBreakdown;
   } else if (!(rr > S.thr[m])) {
     S.state[m] = kBDoneOk;
+  } else if (precond && !(rz > 0.0)) {
+    S.state[m] = kBBreakdown;
   }
   S.rz[m] = rz;
   S.rr[m] = rr;
@@ -1637,7 +1653,7 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
       hipLaunchKernelGGL(k_bcg_csum, dim3(nch), dim3(256), 0, s, CH, (const double *)prz,
                          (const double *)prr, (const int *)S.state, all ? 0 : 1, cp1, cp2);
       hipLaunchKernelGGL(k_bcg_rres, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1,
-                         (const double *)cp2, S, mode);
+                         (const double *)cp2, S, mode, pc != SMH_PRECOND_NONE ? 1 : 0);
     };
     auto pupd = [&]() {
       hipLaunchKernelGGL((k_bcg_pupd<VT, BD>), dim3(nch), dim3(256), 0, s, CH, (VT *)p,
@@ -1659,7 +1675,8 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
                            S, cp1, cp2);
         hipLaunchKernelGGL(k_bcg_beta, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1,
                            (const double *)cp2, S, o->maxiters, o->mixed,
-                           o->reliable_delta * o->reliable_delta, o->replace_every);
+                           o->reliable_delta * o->reliable_delta, o->replace_every,
+                           pc != SMH_PRECOND_NONE ? 1 : 0);
         replace(0, 0);
         pupd();
       }
@@ -1830,7 +1847,77 @@ extern "C" void smh_cg_default_opts(smh_cg_opts *o) {
   o->precond = SMH_PRECOND_NONE;
   o->reliable_delta = 0.3;
   o->check_every = 16;
+  o->fallback = 1;
 }
+
+namespace {
+// The fallback chain of the single-system solvers: the requested
+// preconditioner, then (block) Jacobi, then none.  A rung is retried on the
+// next one when it breaks down (p'q <= 0, z'r <= 0, NaN) or, for the learned
+// preconditioner only, when it ends without convergence.  Every rung starts
+// from the caller's initial guess and is certified by its own FP64 residual;
+// the stats report the total iterations, the preconditioner that produced x
+// and the first rung's status.
+template <class RUN>
+int solve_chain(smh_ctx *ctx, uint64_t n, int bdim, double *x, const smh_cg_opts *o,
+                smh_cg_stats *st, RUN &&run) {
+  smh_cg_stats s{};
+  int chain[3] = {o->precond, 0, 0}, nchain = 1;
+  if (o->fallback) {
+    if (o->precond == SMH_PRECOND_LEARNED)
+      chain[nchain++] = bdim == 3 ? SMH_PRECOND_BLOCK_JACOBI : SMH_PRECOND_JACOBI;
+    if (o->precond != SMH_PRECOND_NONE) chain[nchain++] = SMH_PRECOND_NONE;
+  }
+  double *x0 = nullptr;
+  int rc = SMH_OK;
+  if (nchain > 1) {
+    rc = dmalloc(ctx, (void **)&x0, sizeof(double) * (n ? n : 1));
+    if (rc) return rc;
+    if (hipMemcpyAsync(x0, x, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream) !=
+        hipSuccess) {
+      dfree(x0);
+      return fail(ctx, SMH_ERR_HIP, "smh_cg_solve: copy of the initial guess failed");
+    }
+  }
+  smh_cg_opts oo = *o;
+  int total = 0, nru = 0, first = SMH_OK;
+  double secs = 0.0;
+  int k = 0;
+  for (;; ++k) {
+    oo.precond = chain[k];
+    if (k > 0 &&
+        hipMemcpyAsync(x, x0, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream) !=
+            hipSuccess) {
+      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve: reset to the initial guess failed");
+      break;
+    }
+    s = smh_cg_stats{};
+    rc = run(&oo, &s);
+    if (rc) break;
+    total += s.iters;
+    nru += s.reliable_updates;
+    secs += s.seconds;
+    if (k == 0) first = s.status;
+    const bool retry = s.status == SMH_ERR_BREAKDOWN ||
+                       (chain[k] == SMH_PRECOND_LEARNED && s.status == SMH_ERR_NOT_CONVERGED);
+    if (!retry || k + 1 >= nchain) break;
+  }
+  if (x0) {
+    (void)hipStreamSynchronize(ctx->stream);
+    dfree(x0);
+  }
+  if (!rc && st) {
+    *st = s;
+    st->iters = total;
+    st->reliable_updates = nru;
+    st->seconds = secs;
+    st->precond_used = chain[k];
+    st->fallbacks = k;
+    st->first_status = first;
+  }
+  return rc;
+}
+}  // namespace
 
 extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
                             const smh_cg_opts *o, smh_cg_stats *st) {
@@ -1868,15 +1955,17 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
   SMH_CHECK(ctx, plan.tile_row != nullptr && rplan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
             "smh_cg_solve: a row holds %u blocks (at most %u supported)", plan.maxrow,
             tile_blocks_host(bd) / 2);
-#define SMH_CG(IP, RP) (A->bdim == 3 ? cg_tiles<IP, RP, 3>(ctx, A, &plan, &rplan, b, x, o, st) \
-                                      : cg_tiles<IP, RP, 1>(ctx, A, &plan, &rplan, b, x, o, st))
-  if (o->mixed == 2) rc = SMH_CG(kF32D, SMH_F32X2);
-  else if (o->mixed) rc = SMH_CG(SMH_F32, SMH_F32X2);
-  else if (A->precision == SMH_F64) rc = SMH_CG(SMH_F64, SMH_F64);
-  else if (A->precision == SMH_F32X2) rc = SMH_CG(SMH_F32X2, SMH_F32X2);
-  else rc = SMH_CG(kF32D, kF32D);
+  auto run = [&](const smh_cg_opts *oo, smh_cg_stats *so) {
+#define SMH_CG(IP, RP) (A->bdim == 3 ? cg_tiles<IP, RP, 3>(ctx, A, &plan, &rplan, b, x, oo, so) \
+                                      : cg_tiles<IP, RP, 1>(ctx, A, &plan, &rplan, b, x, oo, so))
+    if (oo->mixed == 2) return SMH_CG(kF32D, SMH_F32X2);
+    if (oo->mixed) return SMH_CG(SMH_F32, SMH_F32X2);
+    if (A->precision == SMH_F64) return SMH_CG(SMH_F64, SMH_F64);
+    if (A->precision == SMH_F32X2) return SMH_CG(SMH_F32X2, SMH_F32X2);
+    return SMH_CG(kF32D, kF32D);
 #undef SMH_CG
-  return rc;
+  };
+  return solve_chain(ctx, (uint64_t)A->nrows * bd, bd, x, o, st, run);
 }
 
 extern "C" int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch,
@@ -2068,10 +2157,12 @@ extern "C" int smh_cg_solve_stencil27(smh_ctx *ctx, const smh_stencil27 *S, cons
   SMH_CHECK(ctx, o->maxiters >= 0, SMH_ERR_INVALID, "smh_cg_solve_stencil27: negative maxiters");
   SMH_CHECK(ctx, o->mixed >= 0 && o->mixed <= 2, SMH_ERR_INVALID,
             "smh_cg_solve_stencil27: mixed must be 0, 1 or 2 (got %d)", o->mixed);
-  if (o->mixed == 2) rc = cg_stencil<kF32D, SMH_F32X2>(ctx, S, b, x, o, st);
-  else if (o->mixed) rc = cg_stencil<SMH_F32, SMH_F32X2>(ctx, S, b, x, o, st);
-  else if (S->precision == SMH_F64) rc = cg_stencil<SMH_F64, SMH_F64>(ctx, S, b, x, o, st);
-  else if (S->precision == SMH_F32X2) rc = cg_stencil<SMH_F32X2, SMH_F32X2>(ctx, S, b, x, o, st);
-  else rc = cg_stencil<kF32D, kF32D>(ctx, S, b, x, o, st);
-  return rc;
+  auto run = [&](const smh_cg_opts *oo, smh_cg_stats *so) {
+    if (oo->mixed == 2) return cg_stencil<kF32D, SMH_F32X2>(ctx, S, b, x, oo, so);
+    if (oo->mixed) return cg_stencil<SMH_F32, SMH_F32X2>(ctx, S, b, x, oo, so);
+    if (S->precision == SMH_F64) return cg_stencil<SMH_F64, SMH_F64>(ctx, S, b, x, oo, so);
+    if (S->precision == SMH_F32X2) return cg_stencil<SMH_F32X2, SMH_F32X2>(ctx, S, b, x, oo, so);
+    return cg_stencil<kF32D, kF32D>(ctx, S, b, x, oo, so);
+  };
+  return solve_chain(ctx, (uint64_t)S->n0 * S->n1 * S->n2, 1, x, o, st, run);
 }

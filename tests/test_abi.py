@@ -33,13 +33,41 @@ def test_python_binding_covers_the_abi():
         assert getattr(lib, n).argtypes is not None, f"{n} has no ctypes signature"
 
 
-def test_struct_layouts_match_header():
-    # sizes a C compiler gives these structs on x86-64
-    assert ctypes.sizeof(simit_amd.Bsr) == 56
-    assert ctypes.sizeof(simit_amd.CGOptions) == 80
-    assert ctypes.sizeof(simit_amd.CGBatchStats) == 16
-    assert ctypes.sizeof(simit_amd.CGStats) == 40
-    assert ctypes.sizeof(simit_amd.SpringsParams) == 56
+# ctypes mirror -> C struct name in include/simit_hip.h
+STRUCTS = {"IndexView": "smh_index_view", "Bsr": "smh_bsr", "Stencil27": "smh_stencil27",
+           "SpringsParams": "smh_springs_params", "SpringsSet": "smh_springs_set",
+           "TetParams": "smh_tet_params", "TetSet": "smh_tet_set",
+           "HexParams": "smh_hex_params", "HexSet": "smh_hex_set",
+           "EdgeBlocks": "smh_edge_blocks", "CGOptions": "smh_cg_opts",
+           "CGBatchStats": "smh_cg_batch_stats", "CGStats": "smh_cg_stats"}
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror has the size and field offsets gcc gives the
+    header's struct."""
+    import subprocess
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "simit_hip.h"',
+             "int main(void) {"]
+    for py, cs in STRUCTS.items():
+        lines.append(f'  printf("{py} size %zu\\n", sizeof({cs}));')
+        for f, _ in getattr(simit_amd, py)._fields_:
+            lines.append(f'  printf("{py} {f} %zu\\n", offsetof({cs}, {f}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", str(simit_amd.HEADER_PATH.parent), str(src), "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = {}
+    for line in out.splitlines():
+        py, f, v = line.split()
+        got[(py, f)] = int(v)
+    for py in STRUCTS:
+        cls = getattr(simit_amd, py)
+        assert ctypes.sizeof(cls) == got[(py, "size")], py
+        for f, _ in cls._fields_:
+            assert getattr(cls, f).offset == got[(py, f)], (py, f)
 
 
 def test_context_fails_cleanly_without_gpu():

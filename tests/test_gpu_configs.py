@@ -10,8 +10,9 @@ C5  hex Poisson 192^3: size-independent properties of the full assembly
     (symmetry, zero row sums away from the Dirichlet face, the stencil storage's
     product equal to the CSR one bit for bit) and certified mixed Jacobi-PCG
     solves over both storages.
-(C4, tets 96^3 x 5, is covered at full size by test_gpu_assembly and
-test_gpu_index, and solved by bench.py.)
+C4  tets 96^3 x 5, log10 E = 5 + GRF (std 1, correlation 8 cells): the learned
+    mixed PCG of the benchmark step (a network trained briefly on 16^3
+    samples), its solution re-checked on the oracle's FP64 matrix.
 """
 import ctypes as C
 
@@ -21,7 +22,7 @@ import torch
 
 import simit_amd as S
 from conftest import to_dev
-from simit_amd import meshes
+from simit_amd import learned, meshes
 
 pytestmark = pytest.mark.gpu
 
@@ -83,7 +84,41 @@ def test_c2_tets64_jacobi_mixed(ctx, oracle):
     assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(of)
 
 
-def test_c3_batch256(ctx, oracle):
+def test_c4_tets96_learned_full_size(ctx, oracle):
+    prob = meshes.tet_problem(96, seed=0, field="grf", mean_log10=5.0, std_log10=1.0,
+                              corr_len=8.0)
+    nv = prob["nverts"]
+    assert nv == 912673 and prob["endpoints"].shape[0] == 4423680
+    d = {k: to_dev(prob[k]) for k in ("x", "c", "mu", "lam", "endpoints")}
+    idx = ctx.build_index(nv, [d["endpoints"]])
+    assert idx.nnz == 11695969
+    net = ctx.gnn(seed=0, kappa=0.9)
+    learned.train(ctx, net, n=16, nsamples=300)
+    P = S.TetParams(1.0, 0.0, (C.c_double * 3)(*prob["load"]), 1.0)
+    hi = torch.empty(idx.nnz * 9, dtype=torch.float32, device="cuda")
+    lo = torch.empty_like(hi)
+    rhs = torch.empty(3 * nv, dtype=torch.float64, device="cuda")
+    ctx.assemble_tets(idx, P, d, d, S.F32X2, hi, lo, rhs)
+    A2 = idx.bsr(3, S.F32X2, hi, lo)
+    linv, corr = net.precond(A2)
+    x = torch.zeros_like(rhs)
+    st = ctx.cg(A2, rhs, x, ctx.cg_options(maxiters=40000, tol=1e-8, rel=1, check_every=64,
+                                           precond=S.PRECOND_LEARNED, mixed=1, learned=linv,
+                                           learned_corr=corr))
+    print(st.as_dict(), net.precond_info(), flush=True)
+    assert st.status == 0 and st.true_rel <= 1e-8, st.as_dict()
+    assert st.precond_used == S.PRECOND_LEARNED and st.fallbacks == 0
+    rp, ci = idx.to_host()
+    ov, of = oracle.tet_assemble(oracle.tet_params(), prob, rp, ci)
+    r = of - oracle.bsrmv(rp, ci, ov, x.cpu().numpy(), 3)
+    rel = np.linalg.norm(r) / np.linalg.norm(of)
+    print("oracle relative residual", rel, flush=True)
+    assert rel <= 1.01e-8
+    net.close()
+
+
+@pytest.mark.parametrize("delta", [0.3, 0.01])
+def test_c3_batch256(ctx, oracle, delta):
     batch = meshes.springs_batch(256, 8, 36, seed=0)
     npts = batch["npoints"]
     d = {k: to_dev(batch[k]) for k in ("x", "v", "m", "fixed", "k", "l0", "endpoints")}
@@ -95,7 +130,7 @@ def test_c3_batch256(ctx, oracle):
     x = torch.zeros_like(rhs)
     stats, _ = ctx.cg_batched(idx.bsr(3, S.F32X2, hi, lo), batch["rows"], rhs, x,
                               ctx.cg_options(maxiters=2000, tol=1e-8, rel=1, mixed=1,
-                                             check_every=8))
+                                             check_every=8, reliable_delta=delta))
     assert all(s["status"] == 0 and s["true_rel"] <= 1e-8 for s in stats)
     # every lattice's FP64 residual on the oracle's matrix
     rp, ci = idx.to_host()
