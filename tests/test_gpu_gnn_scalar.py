@@ -167,7 +167,7 @@ def test_scalar_learned_apply_matches_stored_preconditioner(ctx, mixed):
     A = idx.bsr(1, S.F32X2, sy["hi"], sy["lo"]) if mixed else idx.bsr(1, S.F64, sy["v64"])
     linv, corr = net.precond(A)
     x = torch.zeros_like(sy["rhs"])
-    ctx.cg(A, sy["rhs"], x, ctx.cg_options(maxiters=1, tol=1e-30, rel=1,
+    ctx.cg(A, sy["rhs"], x, ctx.cg_options(maxiters=1, tol=1e-30, rel=1, fallback=0,
                                            precond=S.PRECOND_LEARNED, mixed=mixed,
                                            learned=linv, learned_corr=corr))
     Bm = learned_blocks(rp, ci, linv.cpu().numpy(), corr_values(corr).reshape(-1, 1, 1), 1)

@@ -137,8 +137,8 @@ def test_stencil_learned_correction(ctx, mixed):
     linv, corr = net.precond(A)
     c16 = ctx.stencil27_correction(idx, corr, sy["dims"])
     St = ctx.stencil27_from_bsr(A, sy["dims"])
-    o = ctx.cg_options(maxiters=1, tol=1e-30, rel=1, precond=S.PRECOND_LEARNED, mixed=mixed,
-                       learned=linv, learned_corr=c16)
+    o = ctx.cg_options(maxiters=1, tol=1e-30, rel=1, fallback=0, precond=S.PRECOND_LEARNED,
+                       mixed=mixed, learned=linv, learned_corr=c16)
     x = torch.zeros_like(sy["rhs"])
     ctx.cg_stencil27(St, sy["rhs"], x, o)
     rp, ci = idx.to_host()
