@@ -305,6 +305,24 @@ void smh_cg_default_opts(smh_cg_opts *o);
 int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
                  const smh_cg_opts *opts, smh_cg_stats *stats);
 
+/* smh_cg_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels, results (bit for bit) and statuses
+ * as smh_cg_solve, fallback chain included; preconditioners none, Jacobi and
+ * block Jacobi (not the learned one).  Every piece of scratch -- the row-tile
+ * plan (rebuilt on each call), the CG vectors, the (block) Jacobi inverse,
+ * the reductions and the fallback chain's copy of the initial guess -- lives
+ * in one 32 MiB array of device memory that the library holds on each device
+ * until the process ends; a system that needs more is rejected with
+ * SMH_ERR_UNSUPPORTED before any work is queued (e.g. a 3x3 FP64
+ * block-Jacobi solve fits up to ~170,000 rows, a scalar one ~690,000).
+ * All calls on one device share that array and leave their contents in it:
+ * calls that overlap on one device (two threads, or two contexts of one
+ * device) race. */
+int smh_cg_solve_noalloc(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
+                         const smh_cg_opts *opts, smh_cg_stats *stats);
+
 /* ---- scalar 27-point stencil storage ---------------------------------------
  * Replaces TensorStorage::Stencil (src/storage.h:48) and its SpMV lowering
  * (src/lower/index_expressions/lower_indexexprs.cpp:669-711) for scalar

@@ -179,6 +179,8 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_cg_default_opts": (None, [C.POINTER(CGOptions)]),
         "smh_cg_solve": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, C.POINTER(CGOptions),
                                    C.POINTER(CGStats)]),
+        "smh_cg_solve_noalloc": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, C.POINTER(CGOptions),
+                                           C.POINTER(CGStats)]),
         "smh_cg_solve_batched": (C.c_int, [vp, C.POINTER(Bsr), u32, C.POINTER(u32), vp, vp,
                                            C.POINTER(CGOptions), C.POINTER(CGBatchStats),
                                            C.POINTER(C.c_double)]),

@@ -29,6 +29,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 namespace smh {
 
@@ -97,6 +98,35 @@ int tile_plan(smh_ctx *ctx, const uint32_t *rowptr, uint32_t nrows, uint64_t nnz
   }
   ctx->plans.push_back(p);
   *out = p;
+  return SMH_OK;
+}
+
+// ---- the allocation-free solver's workspace -------------------------------------
+// One static array per device, held until the process ends.  Host code uses
+// the array only through the device address hipGetSymbolAddress returns for
+// the current device (the name itself is a host-side stand-in).
+constexpr size_t kScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+int scratch_setup(smh_ctx *ctx) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void *ptrs[kMaxDevices] = {};
+  SMH_CHECK(ctx, ctx->device >= 0 && ctx->device < kMaxDevices, SMH_ERR_UNSUPPORTED,
+            "scratch: device %d out of range", ctx->device);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[ctx->device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(ctx, SMH_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s",
+                  hipGetErrorString(e));
+    }
+    ptrs[ctx->device] = p;
+  }
+  ctx->scratch = ptrs[ctx->device];
+  ctx->scratch_bytes = kScratchBytes;
   return SMH_OK;
 }
 
@@ -621,6 +651,19 @@ __global__ void __launch_bounds__(256) k_sum_to(const double *part, int np, doub
 constexpr int kPartSlots = 2 * kMaxTileGrid;
 
 // ---- host driver ---------------------------------------------------------------
+// Bump allocation from a caller-owned device buffer (the scratch array of the
+// allocation-free solver); 256-byte aligned pieces.
+struct Arena {
+  unsigned char *base = nullptr;
+  size_t cap = 0, used = 0;
+  void *take(size_t bytes) {
+    const size_t off = (used + 255) & ~size_t(255);
+    if (bytes > cap || off > cap - bytes) return nullptr;
+    used = off + bytes;
+    return base + off;
+  }
+};
+
 struct Work {
   void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *u = nullptr,
        *dlo = nullptr;
@@ -958,7 +1001,7 @@ int make_tile_ops(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan,
 
 template <int IP, int RP, int BD, class OPS>
 int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const double *b, double *x,
-           const smh_cg_opts *o, smh_cg_stats *st) {
+           const smh_cg_opts *o, smh_cg_stats *st, Arena *ar = nullptr) {
   using VT = typename Val<IP>::T;
   const uint64_t n = (uint64_t)nrows * BD;
   const unsigned nbe = grid_for(nrows, 256, 2048);
@@ -975,6 +1018,14 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   int nru = 0, batch = 0;
   const int z32 = learned && OPS::kU32;  // the learned correction's u stored as float
   EpiResidual<VT, BD> epi{};
+  // the solve's buffers: device allocations, or pieces of the arena
+  auto walloc = [&](void **pp, size_t bytes) -> int {
+    if (!ar) return dmalloc(ctx, pp, bytes);
+    *pp = ar->take(bytes);
+    return *pp ? SMH_OK
+               : fail(ctx, SMH_ERR_UNSUPPORTED, "CG: the %zu-byte scratch array is too small",
+                      ar->cap);
+  };
 #define TRY(x)            \
   do {                    \
     rc = (x);             \
@@ -988,24 +1039,24 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
       goto done;                                                                  \
     }                                                                             \
   } while (0)
-  TRY(dmalloc(ctx, &w.r, sizeof(VT) * n));
-  TRY(dmalloc(ctx, &w.z, sizeof(VT) * n));
-  TRY(dmalloc(ctx, &w.p, sizeof(VT) * n));
-  TRY(dmalloc(ctx, &w.q, sizeof(VT) * n));
+  TRY(walloc(&w.r, sizeof(VT) * n));
+  TRY(walloc(&w.z, sizeof(VT) * n));
+  TRY(walloc(&w.p, sizeof(VT) * n));
+  TRY(walloc(&w.q, sizeof(VT) * n));
   if (pc != SMH_PRECOND_NONE && !learned)
-    TRY(dmalloc(ctx, &w.dinv, sizeof(VT) * (uint64_t)nrows * BD * BD));
+    TRY(walloc(&w.dinv, sizeof(VT) * (uint64_t)nrows * BD * BD));
   if (learned) {
-    TRY(dmalloc(ctx, &w.u, sizeof(VT) * n));
+    TRY(walloc(&w.u, sizeof(VT) * n));
     w.dinv = nullptr;
   }
   // mixed precision: the inner operator keeps the exact block row sums
   if (o->mixed && has_lo) {
-    TRY(dmalloc(ctx, &w.dlo, sizeof(float) * (uint64_t)nrows * BD * BD));
+    TRY(walloc(&w.dlo, sizeof(float) * (uint64_t)nrows * BD * BD));
     ops.lo_rowsum(ctx->stream, (float *)w.dlo);
   }
   pdata = learned ? (const void *)o->learned : (const void *)w.dinv;
-  TRY(dmalloc(ctx, (void **)&w.S, sizeof(CGState)));
-  TRY(dmalloc(ctx, (void **)&w.part, sizeof(double) * 2 * kPartSlots + 64));
+  TRY(walloc((void **)&w.S, sizeof(CGState)));
+  TRY(walloc((void **)&w.part, sizeof(double) * 2 * kPartSlots + 64));
   // b'b, state, preconditioner
   hipLaunchKernelGGL(k_dot_wg, dim3(nbv), dim3(256), 0, ctx->stream, n, b, w.part);
   hipLaunchKernelGGL(k_sum_to, dim3(1), dim3(256), 0, ctx->stream, w.part, (int)nbv,
@@ -1102,7 +1153,7 @@ done:
 #undef TRY
 #undef LCHECK
   hipStreamSynchronize(ctx->stream);
-  w.release();
+  if (!ar) w.release();
   return rc;
 }
 
@@ -1461,7 +1512,8 @@ __global__ void __launch_bounds__(256) k_bcg_init(BChunks C, const double *cp, B
 // CG over the blocked-CSR row tiles of A
 template <int IP, int RP, int BD>
 int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const smh_tile_plan *rplan,
-             const double *b, double *x, const smh_cg_opts *o, smh_cg_stats *st) {
+             const double *b, double *x, const smh_cg_opts *o, smh_cg_stats *st,
+             Arena *ar = nullptr) {
   using VT = typename Val<IP>::T;
   constexpr int LP = BD == 3 ? (sizeof(VT) == 4 ? kFP8F : kFP8D)
                              : (sizeof(VT) == 4 ? SMH_F32 : kF32D);
@@ -1469,7 +1521,7 @@ int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const sm
   const int rc = make_tile_ops<IP, RP, BD>(ctx, A, plan, rplan, o, &ops);
   if (rc) return rc;
   return cg_run<IP, RP, BD>(ctx, ops, A->nrows, A->precision == SMH_F32X2 && A->vals_lo, b, x, o,
-                            st);
+                            st, ar);
 }
 
 // the SpMV grid of a pattern: the fastest of a few candidates, timed once
@@ -1860,7 +1912,7 @@ namespace {
 // and the first rung's status.
 template <class RUN>
 int solve_chain(smh_ctx *ctx, uint64_t n, int bdim, double *x, const smh_cg_opts *o,
-                smh_cg_stats *st, RUN &&run) {
+                smh_cg_stats *st, RUN &&run, Arena *ar = nullptr) {
   smh_cg_stats s{};
   int chain[3] = {o->precond, 0, 0}, nchain = 1;
   if (o->fallback) {
@@ -1871,11 +1923,18 @@ int solve_chain(smh_ctx *ctx, uint64_t n, int bdim, double *x, const smh_cg_opts
   double *x0 = nullptr;
   int rc = SMH_OK;
   if (nchain > 1) {
-    rc = dmalloc(ctx, (void **)&x0, sizeof(double) * (n ? n : 1));
-    if (rc) return rc;
+    if (ar) {
+      x0 = (double *)ar->take(sizeof(double) * (n ? n : 1));
+      if (!x0)
+        return fail(ctx, SMH_ERR_UNSUPPORTED, "CG: the %zu-byte scratch array is too small",
+                    ar->cap);
+    } else {
+      rc = dmalloc(ctx, (void **)&x0, sizeof(double) * (n ? n : 1));
+      if (rc) return rc;
+    }
     if (hipMemcpyAsync(x0, x, sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream) !=
         hipSuccess) {
-      dfree(x0);
+      if (!ar) dfree(x0);
       return fail(ctx, SMH_ERR_HIP, "smh_cg_solve: copy of the initial guess failed");
     }
   }
@@ -1892,7 +1951,9 @@ int solve_chain(smh_ctx *ctx, uint64_t n, int bdim, double *x, const smh_cg_opts
       break;
     }
     s = smh_cg_stats{};
+    const size_t mark = ar ? ar->used : 0;  // every rung reuses the same pieces
     rc = run(&oo, &s);
+    if (ar) ar->used = mark;
     if (rc) break;
     total += s.iters;
     nru += s.reliable_updates;
@@ -1902,7 +1963,7 @@ int solve_chain(smh_ctx *ctx, uint64_t n, int bdim, double *x, const smh_cg_opts
                        (chain[k] == SMH_PRECOND_LEARNED && s.status == SMH_ERR_NOT_CONVERGED);
     if (!retry || k + 1 >= nchain) break;
   }
-  if (x0) {
+  if (x0 && !ar) {
     (void)hipStreamSynchronize(ctx->stream);
     dfree(x0);
   }
@@ -1919,45 +1980,41 @@ int solve_chain(smh_ctx *ctx, uint64_t n, int bdim, double *x, const smh_cg_opts
 }
 }  // namespace
 
-extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
-                            const smh_cg_opts *o, smh_cg_stats *st) {
-  if (!ctx) return SMH_ERR_INVALID;
-  SMH_CHECK(ctx, A && b && x && o && A->rowptr, SMH_ERR_INVALID, "smh_cg_solve: null argument");
+namespace {
+int cg_validate(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x, const smh_cg_opts *o,
+                const char *who) {
+  SMH_CHECK(ctx, A && b && x && o && A->rowptr, SMH_ERR_INVALID, "%s: null argument", who);
   SMH_CHECK(ctx, A->bdim == 1 || A->bdim == 3, SMH_ERR_UNSUPPORTED,
-            "smh_cg_solve: block size %u (1 or 3 supported)", A->bdim);
+            "%s: block size %u (1 or 3 supported)", who, A->bdim);
   SMH_CHECK(ctx, o->precond >= 0 && o->precond <= 3, SMH_ERR_INVALID,
-            "smh_cg_solve: unknown preconditioner %d", o->precond);
+            "%s: unknown preconditioner %d", who, o->precond);
   SMH_CHECK(ctx,
             o->precond != SMH_PRECOND_LEARNED ||
                 (o->learned && o->learned_corr && ((uintptr_t)o->learned_corr & 3u) == 0),
-            SMH_ERR_INVALID, "smh_cg_solve: the learned preconditioner needs opts.learned and "
-                             "opts.learned_corr (smh_gnn_precond)");
+            SMH_ERR_INVALID, "%s: the learned preconditioner needs opts.learned and "
+                             "opts.learned_corr (smh_gnn_precond)", who);
   SMH_CHECK(ctx, A->bdim == 3 || o->precond != SMH_PRECOND_BLOCK_JACOBI, SMH_ERR_INVALID,
-            "smh_cg_solve: block Jacobi needs 3x3 blocks");
+            "%s: block Jacobi needs 3x3 blocks", who);
   SMH_CHECK(ctx, !o->mixed || A->precision == SMH_F32X2, SMH_ERR_INVALID,
-            "smh_cg_solve: mixed precision needs an SMH_F32X2 matrix");
+            "%s: mixed precision needs an SMH_F32X2 matrix", who);
   SMH_CHECK(ctx, A->precision >= SMH_F64 && A->precision <= SMH_F32D, SMH_ERR_INVALID,
-            "smh_cg_solve: unknown precision %d", A->precision);
+            "%s: unknown precision %d", who, A->precision);
   SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
-            "smh_cg_solve: SMH_F32X2 needs vals_lo");
-  SMH_CHECK(ctx, o->maxiters >= 0, SMH_ERR_INVALID, "smh_cg_solve: negative maxiters");
+            "%s: SMH_F32X2 needs vals_lo", who);
+  SMH_CHECK(ctx, o->maxiters >= 0, SMH_ERR_INVALID, "%s: negative maxiters", who);
   SMH_CHECK(ctx, o->mixed >= 0 && o->mixed <= 2, SMH_ERR_INVALID,
-            "smh_cg_solve: mixed must be 0, 1 or 2 (got %d)", o->mixed);
-  if (A->nrows == 0) {
-    if (st) std::memset(st, 0, sizeof(*st));
-    return SMH_OK;
-  }
+            "%s: mixed must be 0, 1 or 2 (got %d)", who, o->mixed);
+  return SMH_OK;
+}
+
+// the solve of smh_cg_solve / smh_cg_solve_noalloc over a tile plan
+int cg_dispatch(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const double *b,
+                double *x, const smh_cg_opts *o, smh_cg_stats *st, Arena *ar) {
   const int bd = (int)A->bdim;
-  smh_tile_plan plan, rplan;
-  int rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(bd), &plan);
-  if (!rc) rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(bd), &rplan);
-  if (rc) return rc;
-  SMH_CHECK(ctx, plan.tile_row != nullptr && rplan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
-            "smh_cg_solve: a row holds %u blocks (at most %u supported)", plan.maxrow,
-            tile_blocks_host(bd) / 2);
   auto run = [&](const smh_cg_opts *oo, smh_cg_stats *so) {
-#define SMH_CG(IP, RP) (A->bdim == 3 ? cg_tiles<IP, RP, 3>(ctx, A, &plan, &rplan, b, x, oo, so) \
-                                      : cg_tiles<IP, RP, 1>(ctx, A, &plan, &rplan, b, x, oo, so))
+#define SMH_CG(IP, RP)                                                             \
+  (A->bdim == 3 ? cg_tiles<IP, RP, 3>(ctx, A, plan, plan, b, x, oo, so, ar) \
+                : cg_tiles<IP, RP, 1>(ctx, A, plan, plan, b, x, oo, so, ar))
     if (oo->mixed == 2) return SMH_CG(kF32D, SMH_F32X2);
     if (oo->mixed) return SMH_CG(SMH_F32, SMH_F32X2);
     if (A->precision == SMH_F64) return SMH_CG(SMH_F64, SMH_F64);
@@ -1965,7 +2022,86 @@ extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, dou
     return SMH_CG(kF32D, kF32D);
 #undef SMH_CG
   };
-  return solve_chain(ctx, (uint64_t)A->nrows * bd, bd, x, o, st, run);
+  return solve_chain(ctx, (uint64_t)A->nrows * bd, bd, x, o, st, run, ar);
+}
+}  // namespace
+
+extern "C" int smh_cg_solve(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
+                            const smh_cg_opts *o, smh_cg_stats *st) {
+  if (!ctx) return SMH_ERR_INVALID;
+  int rc = cg_validate(ctx, A, b, x, o, "smh_cg_solve");
+  if (rc) return rc;
+  if (A->nrows == 0) {
+    if (st) std::memset(st, 0, sizeof(*st));
+    return SMH_OK;
+  }
+  const int bd = (int)A->bdim;
+  smh_tile_plan plan;
+  rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, tile_blocks_host(bd), &plan);
+  if (rc) return rc;
+  SMH_CHECK(ctx, plan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
+            "smh_cg_solve: a row holds %u blocks (at most %u supported)", plan.maxrow,
+            tile_blocks_host(bd) / 2);
+  return cg_dispatch(ctx, A, &plan, b, x, o, st, nullptr);
+}
+
+extern "C" int smh_cg_solve_noalloc(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x,
+                                    const smh_cg_opts *o, smh_cg_stats *st) {
+  if (!ctx) return SMH_ERR_INVALID;
+  const char *who = "smh_cg_solve_noalloc";
+  int rc = cg_validate(ctx, A, b, x, o, who);
+  if (rc) return rc;
+  SMH_CHECK(ctx, o->precond != SMH_PRECOND_LEARNED, SMH_ERR_UNSUPPORTED,
+            "%s: the learned preconditioner is not supported (none, Jacobi, block Jacobi)", who);
+  SMH_CHECK(ctx, ctx->scratch != nullptr, SMH_ERR_HIP,
+            "%s: the scratch array of this device is not available", who);
+  if (A->nrows == 0) {
+    if (st) std::memset(st, 0, sizeof(*st));
+    return SMH_OK;
+  }
+  // everything the solve needs, bounded before any work is queued
+  const int bd = (int)A->bdim;
+  const uint64_t nrows = A->nrows, n = nrows * bd, bb = (uint64_t)bd * bd;
+  const uint32_t tb = tile_blocks_host(bd);
+  const uint64_t vt = o->mixed == 1 ? 4 : 8;
+  const uint64_t max_tiles = A->nnz / (tb / 2 + 1) + 1;
+  uint64_t need = 256 + 4 * (max_tiles + 1) + 256;            // row-max counter, tile rows
+  if (o->fallback && o->precond != SMH_PRECOND_NONE) need += 8 * n + 256;  // initial guess
+  need += 4 * (vt * n + 256);                                  // r, z, p, q
+  if (o->precond != SMH_PRECOND_NONE) need += vt * nrows * bb + 256;  // (block) Jacobi
+  if (o->mixed && A->precision == SMH_F32X2) need += 4 * nrows * bb + 256;  // low row sums
+  need += sizeof(CGState) + 256 + sizeof(double) * 2 * kPartSlots + 64 + 256;
+  SMH_CHECK(ctx, need <= ctx->scratch_bytes, SMH_ERR_UNSUPPORTED,
+            "%s: this system needs %llu bytes of scratch; the maximum is %zu (use smh_cg_solve)",
+            who, (unsigned long long)need, ctx->scratch_bytes);
+  Arena ar{(unsigned char *)ctx->scratch, ctx->scratch_bytes, 0};
+  // the row-tile plan, built into the scratch array on every call
+  unsigned *dmax = (unsigned *)ar.take(sizeof(unsigned));
+  SMH_HIP(ctx, hipMemsetAsync(dmax, 0, sizeof(unsigned), ctx->stream));
+  hipLaunchKernelGGL(k_rowmax, dim3(grid_for(A->nrows, 256, 2048)), dim3(256), 0, ctx->stream,
+                     A->rowptr, A->nrows, dmax);
+  unsigned maxrow = 0;
+  SMH_HIP(ctx, hipMemcpyAsync(&maxrow, dmax, sizeof(unsigned), hipMemcpyDeviceToHost,
+                              ctx->stream));
+  SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  SMH_CHECK(ctx, maxrow <= tb / 2, SMH_ERR_UNSUPPORTED,
+            "%s: a row holds %u blocks (at most %u supported)", who, maxrow, tb / 2);
+  smh_tile_plan plan;
+  plan.rowptr = A->rowptr;
+  plan.nrows = A->nrows;
+  plan.nnz = A->nnz;
+  plan.maxrow = maxrow;
+  plan.tile_blocks = tb;
+  plan.tprime = tb - (maxrow ? maxrow : 1) + 1;
+  const uint64_t nt = std::max<uint64_t>(1, (A->nnz + plan.tprime - 1) / plan.tprime);
+  plan.ntiles = (uint32_t)nt;
+  plan.tile_row = (uint32_t *)ar.take(sizeof(uint32_t) * (nt + 1));
+  SMH_CHECK(ctx, nt <= max_tiles && plan.tile_row, SMH_ERR_UNSUPPORTED,
+            "%s: the tile plan does not fit the scratch array", who);
+  hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(nt + 1, 256)), dim3(256), 0, ctx->stream,
+                     A->rowptr, A->nrows, plan.tprime, plan.ntiles, plan.tile_row);
+  SMH_LAUNCH_CHECK(ctx);
+  return cg_dispatch(ctx, A, &plan, b, x, o, st, &ar);
 }
 
 extern "C" int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch,

@@ -197,6 +197,10 @@ extern "C" int smh_ctx_create(int device, smh_ctx **out) {
     return SMH_ERR_HIP;
   }
   c->stream = c->own_stream;
+  // the allocation-free solver's scratch array: an error here is reported
+  // by that entry point, the context is usable without it
+  (void)scratch_setup(c);
+  c->last_error.clear();
   if (hipMalloc((void **)&c->red, sizeof(double) * (kRedSlots + 64)) != hipSuccess ||
       hipMalloc((void **)&c->red_count, sizeof(unsigned) * kRedCounters) != hipSuccess ||
       hipHostMalloc((void **)&c->host_pinned, sizeof(double) * 64, hipHostMallocDefault) !=

@@ -43,6 +43,10 @@ struct smh_ctx {
   double *red = nullptr;
   unsigned *red_count = nullptr;
   double *host_pinned = nullptr;  // pinned host mirror for scalar readbacks
+  // this device's copy of the library's static scratch array (the
+  // allocation-free solver's workspace), looked up once per device
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
 };
 
 struct smh_index {
@@ -109,6 +113,8 @@ int tile_plan(smh_ctx *ctx, const uint32_t *rowptr, uint32_t nrows, uint64_t nnz
               uint32_t tile_blocks, smh_tile_plan *out);
 // drop cached plans of a rowptr buffer (before it is freed or reused)
 void forget_plans(smh_ctx *ctx, const uint32_t *rowptr);
+// device address of the static scratch array on ctx's (current) device
+int scratch_setup(smh_ctx *ctx);
 
 }  // namespace smh
 
