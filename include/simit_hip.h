@@ -59,6 +59,10 @@ const char *smh_ctx_last_error(const smh_ctx *ctx);
 int smh_ctx_set_stream(smh_ctx *ctx, void *hip_stream);
 void *smh_ctx_get_stream(const smh_ctx *ctx);
 int smh_ctx_synchronize(smh_ctx *ctx);
+/* One line naming the context's device (name, architecture, compute units,
+ * memory) into buf (NUL-terminated, truncated to len): what
+ * backend::Function::printMachine reports (backend_function.h:59-62). */
+int smh_ctx_describe(const smh_ctx *ctx, char *buf, size_t len);
 
 /* ---- device memory: replaces cuMemAlloc / cuMemcpyHtoD / cuMemcpyDtoH in
  * GPUFunction::pushSetData, pushGlobalTensor, mapArgs and unmapArgs

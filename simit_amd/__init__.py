@@ -148,6 +148,7 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_ctx_set_stream": (C.c_int, [vp, vp]),
         "smh_ctx_get_stream": (vp, [vp]),
         "smh_ctx_synchronize": (C.c_int, [vp]),
+        "smh_ctx_describe": (C.c_int, [vp, C.c_char_p, C.c_size_t]),
         "smh_malloc": (C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
         "smh_free": (C.c_int, [vp, vp]),
         "smh_memcpy_htod": (C.c_int, [vp, vp, vp, C.c_size_t]),
@@ -247,6 +248,11 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.smh_ctx_synchronize(self.h))
+
+    def describe(self) -> str:
+        buf = C.create_string_buffer(512)
+        self._check(self.lib.smh_ctx_describe(self.h, buf, len(buf)))
+        return buf.value.decode()
 
     # ---- sparsity -----------------------------------------------------------
     def build_index(self, nverts: int, endpoint_sets, force_diag: bool = False) -> "Index":

@@ -68,7 +68,7 @@ def build_host() -> Path:
     srcs = [str(CSRC / s) for s in HOST_SOURCES]
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", str(ROOT / "include"),
           "-I", str(CSRC), *srcs, "-L", str(LIBDIR), "-lsimit_hip",
-          "-Wl,-rpath,$ORIGIN", "-o", str(HOST_LIB)])
+          "-Wl,-rpath,$ORIGIN", "-ldl", "-o", str(HOST_LIB)])
     BINDIR.mkdir(exist_ok=True)
     _run(["g++", "-O2", "-std=c++17", "-I", str(ROOT / "include"), "-I", str(CSRC),
           str(CSRC / "tests_cpp" / "program_tests.cpp"), "-L", str(LIBDIR), "-lsimit",

@@ -7,6 +7,8 @@
 #include <map>
 #include <mutex>
 
+#include <dlfcn.h>
+
 #include "simit_hip.h"
 
 namespace simit {
@@ -57,9 +59,40 @@ size_t devElemBytes(FieldType t) {
   return (t == FieldType::Float64 || t == FieldType::Float32) ? 8 : fieldTypeSize(t);
 }
 
-class HipFunction : public backend::Function {
+// An extern sparse matrix tensor[rows, rows](tensor[b, b](T)) bound as CSR
+// TensorData, and an extern dense vector tensor[rows](tensor[b](T)) bound by
+// pointer.  Values are widened to double on the device and narrowed back by
+// mapArgs (as float fields are).
+struct MatrixSpec {
+  std::string name, rows;
+  int bdim;
+  ComponentType type;
+  bool output;
+};
+struct VectorSpec {
+  std::string name, rows;
+  int bdim;
+  ComponentType type;
+  bool output;
+};
+struct DevMatrix {
+  TensorData *host = nullptr;  // the caller's (borrowed) CSR arrays
+  uint32_t *rowptr = nullptr, *colidx = nullptr;
+  double *vals = nullptr;
+  uint32_t nrows = 0;
+  uint64_t nnz = 0;
+};
+
+class HipFunction : public backend::Function, public hip::SolverControl {
  public:
-  explicit HipFunction(std::vector<ExternSpec> spec) : spec(std::move(spec)) {}
+  explicit HipFunction(std::vector<ExternSpec> spec, std::vector<MatrixSpec> mats = {},
+                       std::vector<VectorSpec> vecs = {}, std::string pipeline = "")
+      : spec(std::move(spec)), mspec(std::move(mats)), vspec(std::move(vecs)),
+        pipeline(std::move(pipeline)) {
+    for (auto &e : this->spec) bindables.push_back({e.name, "set"});
+    for (auto &m : mspec) bindables.push_back({m.name, "matrix"});
+    for (auto &v : vspec) bindables.push_back({v.name, "vector"});
+  }
   ~HipFunction() override { release(); }
 
   void bind(const std::string &name, Set *set) override {
@@ -80,29 +113,82 @@ class HipFunction : public backend::Function {
     sets[name] = set;
     initialized = false;
   }
+  // a dense extern vector (host pointer) or a scalar global
   void bind(const std::string &name, void *data) override {
-    scalars[name] = data;
+    if (!data) throw SimitException("null data bound to " + name);
+    if (findVector(name)) vectors[name] = data;
+    else scalars[name] = data;
     initialized = false;
   }
-  bool isInitialized() const override {
+  // an extern sparse matrix in CSR form (function.h:58, tensor_data.h)
+  void bind(const std::string &name, TensorData &data) override {
+    const MatrixSpec *ms = findMatrix(name);
+    if (!ms) throw SimitException("no extern sparse matrix named " + name);
+    if (data.getKind() != TensorData::Sparse || !data.getRowPtr() || data.getRowLen() < 1 ||
+        data.getDataLen() < 0 || (data.getDataLen() > 0 && (!data.getColInd() || !data.getData())))
+      throw SimitException("sparse tensor data bound to " + name + " is malformed");
+    matrices[name].host = &data;
+    initialized = false;
+  }
+  bool isInitialized() override {
     if (!initialized) return false;
     for (auto &kv : sets)
       if (kv.second->structureVersion() != versions.at(kv.first)) return false;
     return true;
   }
-  std::vector<std::string> externs() const override {
-    std::vector<std::string> n;
-    for (auto &e : spec) n.push_back(e.name);
-    return n;
-  }
   void setSolverOptions(const SolverOptions &o) override { opts = o; }
   SolveStats lastSolve() const override { return stats; }
+
+  void print(std::ostream &os) const override {
+    os << "hip function {\n";
+    for (auto &e : spec) {
+      os << "  extern " << e.name << " : set";
+      if (e.cardinality) os << " of " << e.cardinality << "-element tuples";
+      os << " {";
+      for (size_t i = 0; i < e.fields.size(); ++i)
+        os << (i ? ", " : "") << e.fields[i].name << (e.fields[i].output ? " (out)" : "");
+      os << "}";
+      auto it = sets.find(e.name);
+      if (it != sets.end()) os << "  bound: " << it->second->getSize() << " elements";
+      os << "\n";
+    }
+    for (auto &m : mspec) {
+      os << "  extern " << m.name << " : matrix[" << m.rows << "," << m.rows << "] of "
+         << m.bdim << "x" << m.bdim << " blocks, CSR" << (m.output ? " (out)" : "");
+      auto it = matrices.find(m.name);
+      if (it != matrices.end() && it->second.host)
+        os << "  bound: " << it->second.host->getRowLen() - 1 << " rows, "
+           << it->second.host->getDataLen() << " blocks";
+      os << "\n";
+    }
+    for (auto &v : vspec)
+      os << "  extern " << v.name << " : vector[" << v.rows << "] of " << v.bdim
+         << "-vectors" << (v.output ? " (out)" : "") << "\n";
+    os << "  run: " << pipeline << "\n}\n";
+  }
+  void printMachine(std::ostream &os) const override {
+    hipInfo(os);
+    os << "  launches per run: " << pipeline << "\n";
+  }
 
   std::function<void()> init() override {
     release();
     for (auto &e : spec)
       if (!sets.count(e.name)) throw SimitException("extern " + e.name + " is not bound");
+    for (auto &m : mspec)
+      if (!matrices.count(m.name) || !matrices[m.name].host)
+        throw SimitException("extern " + m.name + " is not bound");
+    for (auto &v : vspec)
+      if (!vectors.count(v.name)) throw SimitException("extern " + v.name + " is not bound");
     smh_ctx *ctx = device_ctx();
+    for (auto &m : mspec) uploadMatrixStructure(ctx, m);
+    for (auto &v : vspec) {
+      DevVector d;
+      d.host = vectors[v.name];
+      d.n = (size_t)rowsOf(v.rows) * v.bdim;
+      check(smh_malloc(ctx, 8 * (d.n ? d.n : 1), &d.dev), "device allocation");
+      devVectors[v.name] = d;
+    }
     for (auto &e : spec) {
       Set *s = sets[e.name];
       versions[e.name] = s->structureVersion();
@@ -132,6 +218,14 @@ class HipFunction : public backend::Function {
   void unmapArgs(bool updated) override {
     if (!updated) return;
     smh_ctx *ctx = device_ctx();
+    for (auto &m : mspec) {
+      DevMatrix &d = matrices[m.name];
+      widenUpload(ctx, d.vals, d.host->getData(), m.type, d.nnz * m.bdim * m.bdim);
+    }
+    for (auto &v : vspec) {
+      DevVector &d = devVectors[v.name];
+      widenUpload(ctx, (double *)d.dev, d.host, v.type, d.n);
+    }
     std::vector<unsigned char> tmp;
     for (auto &kv : fields) {
       DevField &d = kv.second;
@@ -150,6 +244,16 @@ class HipFunction : public backend::Function {
   // device -> host for the fields a program writes
   void mapArgs() override {
     smh_ctx *ctx = device_ctx();
+    for (auto &m : mspec)
+      if (m.output) {
+        DevMatrix &d = matrices[m.name];
+        narrowDownload(ctx, d.host->getData(), d.vals, m.type, d.nnz * m.bdim * m.bdim);
+      }
+    for (auto &v : vspec)
+      if (v.output) {
+        DevVector &d = devVectors[v.name];
+        narrowDownload(ctx, d.host, (const double *)d.dev, v.type, d.n);
+      }
     std::vector<unsigned char> tmp;
     for (auto &kv : fields) {
       DevField &d = kv.second;
@@ -170,6 +274,94 @@ class HipFunction : public backend::Function {
  protected:
   virtual void setup(smh_ctx *ctx) = 0;
   virtual void runDevice(smh_ctx *ctx) = 0;
+
+  struct DevVector {
+    void *host = nullptr;
+    void *dev = nullptr;
+    size_t n = 0;
+  };
+  uint32_t rowsOf(const std::string &setName) {
+    auto it = sets.find(setName);
+    if (it == sets.end()) throw SimitException("set " + setName + " is not bound");
+    return (uint32_t)it->second->getSize();
+  }
+  // checks the CSR structure against its row set and uploads it (u32)
+  void uploadMatrixStructure(smh_ctx *ctx, const MatrixSpec &m) {
+    DevMatrix &d = matrices[m.name];
+    TensorData &t = *d.host;
+    const uint32_t n = rowsOf(m.rows);
+    if ((uint32_t)t.getRowLen() != n + 1)
+      throw SimitException("matrix " + m.name + ": rowLen " + std::to_string(t.getRowLen()) +
+                           " for " + std::to_string(n) + " rows of " + m.rows);
+    const int *rp = t.getRowPtr(), *ci = t.getColInd();
+    if (rp[0] != 0 || rp[n] != t.getDataLen())
+      throw SimitException("matrix " + m.name + ": rowPtr must run from 0 to dataLen");
+    for (uint32_t i = 0; i < n; ++i)
+      if (rp[i + 1] < rp[i]) throw SimitException("matrix " + m.name + ": rowPtr decreases");
+    for (int k = 0; k < t.getDataLen(); ++k)
+      if (ci[k] < 0 || (uint32_t)ci[k] >= n)
+        throw SimitException("matrix " + m.name + ": column index out of range");
+    d.nrows = n;
+    d.nnz = (uint64_t)t.getDataLen();
+    const size_t bb = (size_t)m.bdim * m.bdim;
+    void *p = nullptr;
+    check(smh_malloc(ctx, 4 * (size_t)(n + 1), &p), "device allocation");
+    d.rowptr = (uint32_t *)p;
+    check(smh_memcpy_htod(ctx, p, rp, 4 * (size_t)(n + 1)), "rowPtr upload");
+    check(smh_malloc(ctx, 4 * (d.nnz ? d.nnz : 1), &p), "device allocation");
+    d.colidx = (uint32_t *)p;
+    if (d.nnz) check(smh_memcpy_htod(ctx, p, ci, 4 * d.nnz), "colInd upload");
+    check(smh_malloc(ctx, 8 * (d.nnz ? d.nnz * bb : 1), &p), "device allocation");
+    d.vals = (double *)p;
+  }
+  static void widenUpload(smh_ctx *ctx, double *dev, const void *host, ComponentType t,
+                          size_t n) {
+    if (!n) return;
+    if (t == ComponentType::Double) {
+      check(smh_memcpy_htod(ctx, dev, host, 8 * n), "tensor upload");
+      return;
+    }
+    std::vector<double> tmp(n);
+    for (size_t i = 0; i < n; ++i)
+      tmp[i] = t == ComponentType::Float ? (double)((const float *)host)[i]
+               : t == ComponentType::Int ? (double)((const int *)host)[i]
+                                          : (double)((const bool *)host)[i];
+    check(smh_memcpy_htod(ctx, dev, tmp.data(), 8 * n), "tensor upload");
+  }
+  static void narrowDownload(smh_ctx *ctx, void *host, const double *dev, ComponentType t,
+                             size_t n) {
+    if (!n) return;
+    if (t == ComponentType::Double) {
+      check(smh_memcpy_dtoh(ctx, host, dev, 8 * n), "tensor download");
+      return;
+    }
+    std::vector<double> tmp(n);
+    check(smh_memcpy_dtoh(ctx, tmp.data(), dev, 8 * n), "tensor download");
+    for (size_t i = 0; i < n; ++i) {
+      if (t == ComponentType::Float) ((float *)host)[i] = (float)tmp[i];
+      else if (t == ComponentType::Int) ((int *)host)[i] = (int)tmp[i];
+      else ((bool *)host)[i] = tmp[i] != 0.0;
+    }
+  }
+  // the bound matrix as an FP64 blocked-CSR operand of the ABI
+  smh_bsr bsrOf(const std::string &name) {
+    const MatrixSpec *ms = findMatrix(name);
+    DevMatrix &d = matrices.at(name);
+    return smh_bsr{d.nrows, (uint32_t)ms->bdim, d.nnz, d.rowptr, d.colidx, SMH_F64, d.vals,
+                   nullptr};
+  }
+  double *devVector(const std::string &name) { return (double *)devVectors.at(name).dev; }
+  const MatrixSpec *findMatrix(const std::string &n) const {
+    for (auto &m : mspec)
+      if (m.name == n) return &m;
+    return nullptr;
+  }
+  const VectorSpec *findVector(const std::string &n) const {
+    for (auto &v : vspec)
+      if (v.name == n) return &v;
+    return nullptr;
+  }
+  static void hipInfo(std::ostream &os);
 
   void *dev(const std::string &key) { return fields.at(key).dev; }
   Set *set(const std::string &name) { return sets.at(name); }
@@ -209,16 +401,30 @@ class HipFunction : public backend::Function {
   }
   void release() {
     smh_ctx *ctx = nullptr;
-    if (!fields.empty() || !owned.empty() || index || !endpoints.empty()) ctx = device_ctx();
+    bool any = !fields.empty() || !owned.empty() || index || !endpoints.empty() ||
+               !devVectors.empty();
+    for (auto &kv : matrices) any |= kv.second.rowptr != nullptr;
+    if (any) ctx = device_ctx();
     if (!ctx) return;
     smh_ctx_synchronize(ctx);
     for (auto &kv : fields) smh_free(ctx, kv.second.dev);
     for (auto &kv : endpoints) smh_free(ctx, kv.second);
     for (void *p : owned) smh_free(ctx, p);
+    for (auto &kv : matrices) {
+      DevMatrix &d = kv.second;
+      if (d.rowptr) smh_bsr_forget(ctx, d.rowptr);
+      smh_free(ctx, d.rowptr);
+      smh_free(ctx, d.colidx);
+      smh_free(ctx, d.vals);
+      d.rowptr = d.colidx = nullptr;
+      d.vals = nullptr;
+    }
+    for (auto &kv : devVectors) smh_free(ctx, kv.second.dev);
     if (index) smh_index_destroy(ctx, index);
     fields.clear();
     endpoints.clear();
     owned.clear();
+    devVectors.clear();
     index = nullptr;
   }
   const ExternSpec *findExtern(const std::string &n) const {
@@ -228,6 +434,12 @@ class HipFunction : public backend::Function {
   }
 
   std::vector<ExternSpec> spec;
+  std::vector<MatrixSpec> mspec;
+  std::vector<VectorSpec> vspec;
+  std::string pipeline;  // the library calls of one run, for print()
+  std::map<std::string, DevMatrix> matrices;
+  std::map<std::string, void *> vectors;  // bound host pointers
+  std::map<std::string, DevVector> devVectors;
   std::map<std::string, Set *> sets;
   std::map<std::string, uint64_t> versions;
   std::map<std::string, void *> scalars;
@@ -251,7 +463,10 @@ class SpringsFunction : public HipFunction {
                        {"m", FieldType::Float64, 1, false},
                        {"fixed", FieldType::Bool, 1, false}}},
                      {"springs", 2,
-                      {{"k", FieldType::Float64, 1, false}, {"l0", FieldType::Float64, 1, false}}}}),
+                      {{"k", FieldType::Float64, 1, false}, {"l0", FieldType::Float64, 1, false}}}},
+                    {}, {},
+                    "smh_assemble_springs (k_assemble<SpringOp>, k_isolated) -> smh_cg_solve "
+                    "(tile SpMV / update / reductions per iteration) -> smh_axpby (x += h v)"),
         params(p),
         cg(o) {}
 
@@ -289,7 +504,10 @@ class CgProgramFunction : public HipFunction {
   CgProgramFunction()
       : HipFunction({{"points", 0,
                       {{"b", FieldType::Float64, 1, false}, {"c", FieldType::Float64, 1, true}}},
-                     {"springs", 2, {{"a", FieldType::Float64, 1, false}}}}) {}
+                     {"springs", 2, {{"a", FieldType::Float64, 1, false}}}},
+                    {}, {},
+                    "smh_assemble_edge_blocks (A = I - 0.01 K) -> smh_cg_solve (5 iterations, "
+                    "norm(r) > tol)") {}
 
  protected:
   void setup(smh_ctx *) override {
@@ -326,7 +544,10 @@ class FemStaticFunction : public HipFunction {
                        {"c", FieldType::Int32, 1, false},
                        {"v", FieldType::Float64, 3, true}}},
                      {"tets", 4,
-                      {{"u", FieldType::Float64, 1, false}, {"l", FieldType::Float64, 1, false}}}}) {}
+                      {{"u", FieldType::Float64, 1, false}, {"l", FieldType::Float64, 1, false}}}},
+                    {}, {},
+                    "smh_assemble_tet_linear (FP32 hi/lo) -> smh_cg_solve (mixed block-Jacobi "
+                    "PCG, FP64 reliable updates)") {}
 
  protected:
   void setup(smh_ctx *) override {
@@ -366,7 +587,9 @@ class ThermalHexFunction : public HipFunction {
       : HipFunction({{"nodes", 0,
                       {{"dirichlet", FieldType::Int32, 1, false},
                        {"T", FieldType::Float64, 1, true}}},
-                     {"hexes", 8, {{"k", FieldType::Float64, 1, false}}}}) {}
+                     {"hexes", 8, {{"k", FieldType::Float64, 1, false}}}},
+                    {}, {},
+                    "smh_assemble_hex_poisson (FP32 hi/lo) -> smh_cg_solve (mixed Jacobi PCG)") {}
 
  protected:
   void setup(smh_ctx *) override {
@@ -397,8 +620,81 @@ class ThermalHexFunction : public HipFunction {
   double *rhs = nullptr;
 };
 
+// ---- extern sparse matrices bound as CSR TensorData -------------------------------------
+// A = -A on an extern int matrix: the reference's extern-CSR function test
+// (test/function-tests.cpp:96-139, an IndexExpr -A(i,j) over extern A).
+class ExternNegFunction : public HipFunction {
+ public:
+  ExternNegFunction()
+      : HipFunction({{"V", 0, {}}}, {{"A", "V", 1, ComponentType::Int, true}}, {},
+                    "smh_axpby (A = -A over the bound CSR values)") {}
+
+ protected:
+  void setup(smh_ctx *) override {}
+  void runDevice(smh_ctx *ctx) override {
+    smh_bsr A = bsrOf("A");
+    check(smh_axpby(ctx, A.nnz, 0.0, (const double *)A.vals, -1.0, (double *)A.vals),
+          "negation");
+  }
+};
+
+// y = A * x over an extern CSR matrix of b x b blocks (the blocked row loop
+// of lower_indexexprs.cpp:630-667, in its ascending order)
+class ExternSpmvFunction : public HipFunction {
+ public:
+  explicit ExternSpmvFunction(int b)
+      : HipFunction({{"V", 0, {}}}, {{"A", "V", b, ComponentType::Double, false}},
+                    {{"x", "V", b, ComponentType::Double, false},
+                     {"y", "V", b, ComponentType::Double, true}},
+                    "smh_bsrmv (exact order) y = A x") {}
+
+ protected:
+  void setup(smh_ctx *) override {}
+  void runDevice(smh_ctx *ctx) override {
+    smh_bsr A = bsrOf("A");
+    check(smh_bsrmv(ctx, &A, devVector("x"), devVector("y"), 1), "SpMV");
+  }
+};
+
+// x = cg(A, b) over an extern CSR matrix: relative 1e-8, (block) Jacobi,
+// FP64; x holds the initial guess
+class ExternCgFunction : public HipFunction {
+ public:
+  explicit ExternCgFunction(int b)
+      : HipFunction({{"V", 0, {}}}, {{"A", "V", b, ComponentType::Double, false}},
+                    {{"b", "V", b, ComponentType::Double, false},
+                     {"x", "V", b, ComponentType::Double, true}},
+                    "smh_cg_solve (FP64 (block-)Jacobi PCG to a relative 1e-8)"),
+        bdim(b) {}
+
+ protected:
+  void setup(smh_ctx *) override {}
+  void runDevice(smh_ctx *) override {
+    smh_bsr A = bsrOf("A");
+    smh_cg_opts o;
+    smh_cg_default_opts(&o);
+    o.maxiters = 20000;
+    o.tol = 1e-8;
+    o.rel = 1;
+    o.precond = bdim == 3 ? SMH_PRECOND_BLOCK_JACOBI : SMH_PRECOND_JACOBI;
+    solve(A, devVector("b"), devVector("x"), o);
+  }
+
+ private:
+  int bdim;
+};
+
+void HipFunction::hipInfo(std::ostream &os) {
+  char buf[512] = {0};
+  if (smh_ctx_describe(device_ctx(), buf, sizeof(buf)) == SMH_OK) os << buf << "\n";
+  Dl_info info{};
+  if (dladdr((const void *)&smh_cg_solve, &info) && info.dli_fname)
+    os << "code object: " << info.dli_fname << " (hand-written HIP kernels for gfx950)\n";
+}
+
 const std::vector<std::string> kPrograms = {"isprings", "springs_implicit", "cg", "fem_static",
-                                            "thermal_hex"};
+                                            "thermal_hex", "extern_neg", "extern_spmv",
+                                            "extern_spmv_3x3", "extern_cg", "extern_cg_3x3"};
 
 std::shared_ptr<backend::Function> makeFunction(const std::string &program) {
   if (program == "isprings") {
@@ -424,6 +720,11 @@ std::shared_ptr<backend::Function> makeFunction(const std::string &program) {
   if (program == "cg") return std::make_shared<CgProgramFunction>();
   if (program == "fem_static") return std::make_shared<FemStaticFunction>();
   if (program == "thermal_hex") return std::make_shared<ThermalHexFunction>();
+  if (program == "extern_neg") return std::make_shared<ExternNegFunction>();
+  if (program == "extern_spmv") return std::make_shared<ExternSpmvFunction>(1);
+  if (program == "extern_spmv_3x3") return std::make_shared<ExternSpmvFunction>(3);
+  if (program == "extern_cg") return std::make_shared<ExternCgFunction>(1);
+  if (program == "extern_cg_3x3") return std::make_shared<ExternCgFunction>(3);
   return nullptr;
 }
 
@@ -453,6 +754,36 @@ void Function::bind(const std::string &name, void *data) {
   if (!impl) throw SimitException("bind on an undefined function");
   impl->bind(name, data);
 }
+void Function::bind(const std::string &name, const TensorType &ttype, void *data) {
+  if (!impl) throw SimitException("bind on an undefined function");
+  (void)ttype;  // dense externs are widened to double on the device whatever their type
+  impl->bind(name, data);
+}
+void Function::bind(const std::string &name, TensorData &data) {
+  if (!impl) throw SimitException("bind on an undefined function");
+  impl->bind(name, data);
+}
+void Function::print(std::ostream &os) const {
+  if (impl) impl->print(os);
+}
+void Function::printMachine(std::ostream &os) const {
+  if (impl) impl->printMachine(os);
+}
+std::ostream &operator<<(std::ostream &os, const Function &f) {
+  f.print(os);
+  return os;
+}
+bool backend::Function::hasBindable(const std::string &name) const {
+  for (auto &b : bindables)
+    if (b.name == name) return true;
+  return false;
+}
+size_t TensorType::getSize() const {
+  size_t n = 1;
+  for (int d : dimensions) n *= (size_t)d;
+  for (int d : blockDimensions) n *= (size_t)d;
+  return n;
+}
 void Function::init() {
   if (!impl) throw SimitException("init on an undefined function");
   funcPtr = impl->init();
@@ -472,9 +803,12 @@ void Function::unmapArgs(bool updated) {
   if (impl) impl->unmapArgs(updated);
 }
 void Function::setSolverOptions(const SolverOptions &o) {
-  if (impl) impl->setSolverOptions(o);
+  if (auto *c = dynamic_cast<hip::SolverControl *>(impl.get())) c->setSolverOptions(o);
 }
-SolveStats Function::lastSolve() const { return impl ? impl->lastSolve() : SolveStats{}; }
+SolveStats Function::lastSolve() const {
+  if (auto *c = dynamic_cast<const hip::SolverControl *>(impl.get())) return c->lastSolve();
+  return SolveStats{};
+}
 
 int Program::loadFile(const std::string &path) {
   std::string base = path.substr(path.find_last_of('/') == std::string::npos

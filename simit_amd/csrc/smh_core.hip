@@ -252,6 +252,18 @@ extern "C" int smh_ctx_synchronize(smh_ctx *ctx) {
   return SMH_OK;
 }
 
+extern "C" int smh_ctx_describe(const smh_ctx *ctx, char *buf, size_t len) {
+  if (!ctx || !buf || !len) return SMH_ERR_INVALID;
+  hipDeviceProp_t prop{};
+  if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) {
+    (void)hipGetLastError();
+    return SMH_ERR_HIP;
+  }
+  snprintf(buf, len, "device %d: %s (%s), %d compute units, %zu GiB", ctx->device, prop.name,
+           prop.gcnArchName, prop.multiProcessorCount, (size_t)(prop.totalGlobalMem >> 30));
+  return SMH_OK;
+}
+
 extern "C" int smh_malloc(smh_ctx *ctx, size_t bytes, void **dptr) {
   if (!ctx || !dptr) return SMH_ERR_INVALID;
   return dmalloc(ctx, dptr, bytes);

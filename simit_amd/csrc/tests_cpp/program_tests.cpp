@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -401,14 +402,148 @@ static void reorder_bunny_springs() {
   EXPECT_TRUE(scale > 0 && diff <= 1e-6 * scale);
 }
 
+// test/function-tests.cpp:96-139: an extern int matrix bound as CSR
+// TensorData and negated by the function; then the same CSR with double
+// values through y = A x, and a small SPD CSR through x = cg(A, b)
+static void extern_csr() {
+  Set V;
+  V.add();
+  V.add();
+  V.add();
+  // 1.0 2.0 0.0
+  // 0.0 3.0 4.0
+  // 0.0 0.0 0.0
+  int A_rowPtr[4] = {0, 2, 4, 4};
+  int A_colInd[4] = {0, 1, 1, 2};
+  int A_vals[4] = {1, 2, 3, 4};
+  {
+    Function f = load("extern_neg");
+    if (!f.defined()) return;
+    f.bind("V", &V);
+    TensorData data(A_rowPtr, A_colInd, A_vals, 4, 4);
+    f.bind("A", data);
+    f.runSafe();
+    EXPECT_TRUE(A_vals[0] == -1 && A_vals[1] == -2 && A_vals[2] == -3 && A_vals[3] == -4);
+    std::ostringstream os;
+    f.print(os);
+    f.printMachine(os);
+    std::printf("%s", os.str().c_str());
+    EXPECT_TRUE(os.str().find("extern A") != std::string::npos);
+    EXPECT_TRUE(os.str().find("gfx950") != std::string::npos);
+  }
+  {
+    double vals[4] = {1.0, 2.0, 3.0, 4.0}, x[3] = {1.0, 2.0, 3.0}, y[3] = {-7, -7, -7};
+    Function f = load("extern_spmv");
+    if (!f.defined()) return;
+    f.bind("V", &V);
+    TensorData data(A_rowPtr, A_colInd, vals, 4, 4);
+    f.bind("A", data);
+    f.bind("x", x);
+    f.bind("y", y);
+    f.runSafe();
+    EXPECT_NEAR(5.0, y[0], 0.0);
+    EXPECT_NEAR(18.0, y[1], 0.0);
+    EXPECT_NEAR(0.0, y[2], 0.0);
+    // a rowLen that does not match the bound set is rejected
+    int badRow[3] = {0, 2, 4};
+    TensorData bad(badRow, A_colInd, vals, 3, 4);
+    f.bind("A", bad);
+    bool threw = false;
+    try {
+      f.runSafe();
+    } catch (const SimitException &) {
+      threw = true;
+    }
+    EXPECT_TRUE(threw);
+  }
+  {
+    // [4 1 0; 1 3 1; 0 1 2] x = [1 2 3]
+    int rp[4] = {0, 2, 5, 7}, ci[7] = {0, 1, 0, 1, 2, 1, 2};
+    double vals[7] = {4, 1, 1, 3, 1, 1, 2}, b[3] = {1, 2, 3}, x[3] = {0, 0, 0};
+    Function f = load("extern_cg");
+    if (!f.defined()) return;
+    f.bind("V", &V);
+    TensorData data(rp, ci, vals, 4, 7);
+    f.bind("A", data);
+    f.bind("b", b);
+    f.bind("x", x);
+    f.runSafe();
+    // exact solution (2/9, 1/9, 13/9)
+    EXPECT_NEAR(2.0 / 9.0, x[0], 1e-7);
+    EXPECT_NEAR(1.0 / 9.0, x[1], 1e-7);
+    EXPECT_NEAR(13.0 / 9.0, x[2], 1e-7);
+  }
+}
+
+// extern_file DIR: the CSR of DIR/{rowptr,colidx}.i32, DIR/vals.f64 (3x3
+// blocks) and DIR/b.f64, bound as TensorData: y = A b written to DIR/y.f64
+// and x = cg(A, b) to DIR/x.f64 (the GPU test checks both with the oracle)
+static std::vector<char> slurp(const std::string &path) {
+  std::FILE *fp = std::fopen(path.c_str(), "rb");
+  std::vector<char> out;
+  if (!fp) return out;
+  char buf[1 << 16];
+  size_t n;
+  while ((n = std::fread(buf, 1, sizeof(buf), fp)) > 0) out.insert(out.end(), buf, buf + n);
+  std::fclose(fp);
+  return out;
+}
+static void spit(const std::string &path, const void *p, size_t bytes) {
+  std::FILE *fp = std::fopen(path.c_str(), "wb");
+  if (!fp) return;
+  std::fwrite(p, 1, bytes, fp);
+  std::fclose(fp);
+}
+static int extern_file(const std::string &dir) {
+  std::vector<char> rpb = slurp(dir + "/rowptr.i32"), cib = slurp(dir + "/colidx.i32"),
+                    vb = slurp(dir + "/vals.f64"), bb = slurp(dir + "/b.f64");
+  if (rpb.size() < 8 || vb.size() != cib.size() / 4 * 72 || bb.empty()) {
+    std::printf("  FAIL: bad input files in %s\n", dir.c_str());
+    return 1;
+  }
+  const int nrows = (int)(rpb.size() / 4) - 1, nnz = (int)(cib.size() / 4);
+  Set V;
+  for (int i = 0; i < nrows; ++i) V.add();
+  const int *rp = (const int *)rpb.data(), *ci = (const int *)cib.data();
+  double *vals = (double *)vb.data(), *b = (double *)bb.data();
+  std::vector<double> y(3 * (size_t)nrows, 0.0), x(3 * (size_t)nrows, 0.0);
+  TensorData data(rp, ci, vals, nrows + 1, nnz);
+  Function mv = load("extern_spmv_3x3");
+  mv.bind("V", &V);
+  mv.bind("A", data);
+  mv.bind("x", b);
+  mv.bind("y", y.data());
+  mv.runSafe();
+  spit(dir + "/y.f64", y.data(), 8 * y.size());
+  Function cg = load("extern_cg_3x3");
+  cg.bind("V", &V);
+  cg.bind("A", data);
+  cg.bind("b", b);
+  cg.bind("x", x.data());
+  cg.runSafe();
+  spit(dir + "/x.f64", x.data(), 8 * x.size());
+  SolveStats st = cg.lastSolve();
+  std::printf("extern_file: %d rows, %d blocks; cg iters %d status %d true_rel %.3e\n", nrows, nnz,
+              st.iters, st.status, st.true_rel);
+  return failures ? 1 : 0;
+}
+
 int main(int argc, char **argv) {
   init("hip", 8);
+  if (argc == 3 && std::string(argv[1]) == "extern_file") {
+    try {
+      return extern_file(argv[2]);
+    } catch (const std::exception &e) {
+      std::printf("  FAIL: exception %s\n", e.what());
+      return 1;
+    }
+  }
   std::vector<std::pair<std::string, std::function<void()>>> tests = {
       {"isprings", isprings}, {"cg", cg}, {"springs_lattice", springs_lattice},
       {"mesh_bunny", mesh_bunny}, {"reorder_bunny_springs", reorder_bunny_springs},
       {"reorder_int", reorder_int}, {"reorder_double", reorder_double},
       {"hilbert_curve", hilbert_curve}, {"reorder_bunny_host", reorder_bunny_host},
-      {"mesh_load", mesh_load}};
+      {"mesh_load", mesh_load}, {"extern_csr", extern_csr}};
   for (auto &t : tests) {
     bool run = argc <= 1;
     for (int i = 1; i < argc; ++i) run |= t.first == argv[i];
