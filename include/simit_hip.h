@@ -178,6 +178,38 @@ int smh_assemble_tet_linear(smh_ctx *ctx, const smh_index *idx,
                             int32_t precision, void *vals, void *vals_lo,
                             double *rhs);
 
+/* test/input/program/femTet.sim: neo-Hookean tets (PK1 and its derivative
+ * dPdF with log(det F), :27-39).  main's two maps, A = map
+ * compute_stiffness(h) to tets reduce + (:74-124: -(h*h) dH blocks on free
+ * pairs, then the lumped mass 0.25*rho*W on every diagonal) and f = map
+ * compute_force(h) to tets reduce + (:37-72: h times the element's internal
+ * force columns on free vertices, then h*fg + m*v), in one pass.  The rest
+ * shape B and volume W come from smh_femtet_precompute (initializeTet). */
+typedef struct {
+  double h;        /* time step (main: 0.01)                         */
+  double rho;      /* density (1000)                                 */
+  double grav[3];  /* gravity ((0, -10, 0))                          */
+} smh_femtet_params;
+typedef struct {
+  uint32_t nverts;
+  uint64_t ntets;
+  const int32_t *endpoints; /* device int32[ntets*4]                       */
+  const double *x, *v;      /* device double[nverts*3] (Vert.x, Vert.v)    */
+  const int32_t *c;         /* device int32[nverts]; > 0 means constrained */
+  const double *u, *l;      /* device double[ntets] (Tet.u, Tet.l)         */
+  const double *W;          /* device double[ntets] rest volume (Tet.W)    */
+  const double *B;          /* device double[ntets*9] (Tet.B, row-major)   */
+} smh_femtet_set;
+int smh_assemble_tet_neohookean(smh_ctx *ctx, const smh_index *idx,
+                                const smh_femtet_params *p, const smh_femtet_set *s,
+                                int32_t precision, void *vals, void *vals_lo, double *rhs);
+/* initializeTet (femTet.sim:126-148), verts.m = map precomputeTetMat to
+ * tets: W, B (device double[ntets], [ntets*9]) of every tet from s->x, and
+ * m (device double[nverts]) = 0.25*rho*W of the last tet holding each
+ * vertex (a map without a reduction assigns); vertices in no tet keep m. */
+int smh_femtet_precompute(smh_ctx *ctx, const smh_index *idx, const smh_femtet_params *p,
+                          const smh_femtet_set *s, double *W, double *B, double *m);
+
 /* Scalar trilinear hex Poisson (1x1 blocks): kappa_e*h*Kref on free pairs,
  * source*h^3/8 on free rows, fixed_diag on Dirichlet rows.  Corner order
  * a = dx + 2*dy + 4*dz. */

@@ -337,6 +337,194 @@ void or_tet_assemble(const or_tet_params *P, int32_t nv, int64_t nt,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Neo-Hookean tets: test/input/program/femTet.sim                           */
+/* ------------------------------------------------------------------------ */
+
+/* Natural logarithm from x = m 2^k, m in [sqrt(1/2), sqrt(2)), and
+ * log m = 2 atanh(s), s = (m - 1)/(m + 1), |s| <= 0.1716, summed as the
+ * series 2 s (1 + s^2/3 + s^4/5 + ... + s^24/25) by Horner's rule: a
+ * textbook expansion, written so that the device kernels can repeat it
+ * operation for operation (the femTet element's log(det(F)), femTet.sim:
+ * 28, 36).  Pinned against libm's log in the tests (<= 2 ulp). */
+static const double kLn2 = 0.69314718055994530942;
+double or_log(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -INFINITY : NAN;
+  if (x == INFINITY) return x;
+  int k = 0;
+  double m = frexp(x, &k);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    k = k - 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double t = 1.0 / 25.0;
+  for (int n = 11; n >= 0; --n) t = t * s2 + 1.0 / (double)(2 * n + 1);
+  return (double)k * kLn2 + 2.0 * s * t;
+}
+
+/* 3x3 helpers in the association of the .sim expressions: C = A*B sums
+ * k ascending; C = A*B' uses B's rows. */
+static void mm3(const double *A, const double *B, double *C) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+static void mm3t(const double *A, const double *B, double *C) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      C[3 * i + j] =
+          A[3 * i] * B[3 * j] + A[3 * i + 1] * B[3 * j + 1] + A[3 * i + 2] * B[3 * j + 2];
+}
+
+/* precomputeTetMat (femTet.sim:126-144) for one tet: B = 1.0*inv(M),
+ * W = -(1/6) det(M), M(jj,ii) = x_ii(jj) - x_3(jj). */
+void or_femtet_geom(const double *xs, double *B, double *W) {
+  double M[9], Mi[9];
+  for (int ii = 0; ii < 3; ++ii)
+    for (int jj = 0; jj < 3; ++jj) M[3 * jj + ii] = xs[3 * ii + jj] - xs[9 + jj];
+  or_inv3(M, Mi);
+  for (int q = 0; q < 9; ++q) B[q] = 1.0 * Mi[q];
+  *W = -(1.0 / 6.0) * or_det3(M);
+}
+
+/* initializeTet (femTet.sim:146-148): verts.m = map precomputeTetMat to
+ * tets, a map without a reduction: each write assigns, so a vertex keeps
+ * the value of the last (highest) tet that holds it; m of a vertex in no
+ * tet is left as it was. */
+void or_femtet_precompute(double rho, int32_t nv, int64_t nt, const int32_t *ep,
+                          const double *x, double *W, double *B, double *m) {
+  (void)nv;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int32_t *v = ep + 4 * t;
+    double xs[12];
+    for (int a = 0; a < 4; ++a)
+      for (int q = 0; q < 3; ++q) xs[3 * a + q] = x[3 * (int64_t)v[a] + q];
+    or_femtet_geom(xs, B + 9 * t, W + t);
+    for (int ii = 0; ii < 4; ++ii) m[v[ii]] = 0.25 * rho * W[t];
+  }
+}
+
+/* F = Ds*B with Ds(jj,ii) = x_ii(jj) - x_3(jj) (femTet.sim:49-54, 80-85) */
+static void femtet_F(const double *xs, const double *B, double *F) {
+  double Ds[9];
+  for (int ii = 0; ii < 3; ++ii)
+    for (int jj = 0; jj < 3; ++jj) Ds[3 * jj + ii] = xs[3 * ii + jj] - xs[9 + jj];
+  mm3(Ds, B, F);
+}
+
+/* The element's internal force columns H(:,ii) (H = -W * PK1(F) * B',
+ * femTet.sim:27-31, 56-57), H row-major. */
+void or_femtet_H(const double *xs, double u, double l, double W, const double *B, double *H) {
+  double F[9], Fi[9], FiT[9], P[9], Ps[9];
+  femtet_F(xs, B, F);
+  const double JJ = or_log(or_det3(F));
+  or_inv3(F, Fi);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) FiT[3 * i + j] = Fi[3 * j + i];
+  const double lJ = l * JJ;
+  for (int q = 0; q < 9; ++q) P[q] = u * (F[q] - FiT[q]) + lJ * FiT[q];
+  const double nW = -W;
+  for (int q = 0; q < 9; ++q) Ps[q] = nW * P[q];
+  mm3t(Ps, B, H);
+}
+
+/* dH = -W * dPdF(F, dF) * B' (femTet.sim:33-39, 103-104) for the dF whose
+ * only non-zero row kk is dFRow(row, :) */
+static void femtet_dH(const double *F, const double *Fi, const double *FiT, double JJ, double u,
+                      double l, double W, const double *B, const double *dFrow, int kk,
+                      double *dH) {
+  double dF[9], FidF[9], T1[9], T2[9], dP[9], dPs[9];
+  for (int q = 0; q < 9; ++q) dF[q] = 0.0;
+  for (int ll = 0; ll < 3; ++ll) dF[3 * kk + ll] = dFrow[ll];
+  mm3(Fi, dF, FidF);
+  const double s1 = u - l * JJ;
+  for (int q = 0; q < 9; ++q) T1[q] = s1 * FiT[q];
+  mm3t(T1, FidF, T2);
+  const double s2 = l * (FidF[0] + FidF[4] + FidF[8]);
+  for (int q = 0; q < 9; ++q) dP[q] = u * dF[q] + T2[q] + s2 * FiT[q];
+  const double nW = -W;
+  for (int q = 0; q < 9; ++q) dPs[q] = nW * dP[q];
+  mm3t(dPs, B, dH);
+  (void)F;
+}
+
+/* compute_stiffness (femTet.sim:74-124): block K(v(jj), v(row)) of one tet
+ * without the h*h scaling and sign: Kb(jj,:,:) for dF directions
+ * (row, kk = 0..2), out[ll*3 + kk]. */
+void or_femtet_kb(const double *xs, double u, double l, double W, const double *B, int jj,
+                  int row, double *out) {
+  double F[9], Fi[9], FiT[9], dFrow[3], dH[9];
+  femtet_F(xs, B, F);
+  const double JJ = or_log(or_det3(F));
+  or_inv3(F, Fi);
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) FiT[3 * i + j] = Fi[3 * j + i];
+  if (row < 3) {
+    for (int ll = 0; ll < 3; ++ll) dFrow[ll] = B[3 * row + ll];
+  } else {
+    for (int ii = 0; ii < 3; ++ii) dFrow[ii] = -(B[ii] + B[3 + ii] + B[6 + ii]);
+  }
+  for (int kk = 0; kk < 3; ++kk) {
+    femtet_dH(F, Fi, FiT, JJ, u, l, W, B, dFrow, kk, dH);
+    for (int ll = 0; ll < 3; ++ll)
+      out[3 * ll + kk] = jj < 3 ? dH[3 * ll + jj]
+                                : -(dH[3 * ll + 0] + dH[3 * ll + 1] + dH[3 * ll + 2]);
+  }
+}
+
+/* main's two maps (femTet.sim:150-152): f = map compute_force(h) to tets
+ * reduce +, A = map compute_stiffness(h) to tets reduce +, every element
+ * write a += in ascending element order, in the order the element function
+ * writes.  A is 3x3 blocked CSR on rowptr/colidx. */
+void or_femtet_assemble(double h, double rho, const double *grav, int32_t nv, int64_t nt,
+                        const int32_t *ep, const double *x, const double *v, const int32_t *c,
+                        const double *u, const double *l, const double *W, const double *B,
+                        const uint32_t *rowptr, const uint32_t *colidx, double *vals,
+                        double *f) {
+  const int64_t nnz = rowptr[nv];
+  memset(vals, 0, sizeof(double) * 9 * nnz);
+  memset(f, 0, sizeof(double) * 3 * nv);
+  for (int64_t t = 0; t < nt; ++t) {
+    const int32_t *vt = ep + 4 * t;
+    const double *Bt = B + 9 * t;
+    double xs[12];
+    for (int a = 0; a < 4; ++a)
+      for (int q = 0; q < 3; ++q) xs[3 * a + q] = x[3 * (int64_t)vt[a] + q];
+    /* compute_force */
+    const double mq = 0.25 * rho * W[t];
+    double fg[3], H[9];
+    for (int q = 0; q < 3; ++q) fg[q] = mq * grav[q];
+    or_femtet_H(xs, u[t], l[t], W[t], Bt, H);
+    for (int ii = 0; ii < 3; ++ii) {
+      if (c[vt[ii]] <= 0)
+        for (int q = 0; q < 3; ++q) f[3 * (int64_t)vt[ii] + q] += h * H[3 * q + ii];
+      if (c[vt[3]] <= 0)
+        for (int q = 0; q < 3; ++q) f[3 * (int64_t)vt[3] + q] += -h * H[3 * q + ii];
+    }
+    for (int ii = 0; ii < 4; ++ii)
+      if (c[vt[ii]] <= 0)
+        for (int q = 0; q < 3; ++q)
+          f[3 * (int64_t)vt[ii] + q] += h * fg[q] + mq * v[3 * (int64_t)vt[ii] + q];
+    /* compute_stiffness: stiffness of free pairs (loops row, then jj), then
+     * the lumped mass on every diagonal */
+    const double hh = h * h;
+    for (int row = 0; row < 4; ++row)
+      for (int jj = 0; jj < 4; ++jj) {
+        if (!(c[vt[jj]] <= 0 && c[vt[row]] <= 0)) continue;
+        double kb[9];
+        or_femtet_kb(xs, u[t], l[t], W[t], Bt, jj, row, kb);
+        const int32_t loc = or_loc(rowptr, colidx, vt[jj], vt[row]);
+        for (int q = 0; q < 9; ++q) vals[9 * (int64_t)loc + q] += -(hh * kb[q]);
+      }
+    for (int ii = 0; ii < 4; ++ii) {
+      const int32_t loc = or_loc(rowptr, colidx, vt[ii], vt[ii]);
+      for (int q = 0; q < 9; ++q) vals[9 * (int64_t)loc + q] += mq * ((q % 4 == 0) ? 1.0 : 0.0);
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
 /* Trilinear hex, scalar Poisson                                             */
 /* ------------------------------------------------------------------------ */
 

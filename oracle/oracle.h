@@ -20,6 +20,11 @@
  *   or_tet_*            numpy W*B'*E*B literal product (apps/fem/fem_linear.sim:
  *                       95-174), rigid-mode null space, symmetry.
  *   or_hex_*            numpy 2x2x2 Gauss quadrature of the trilinear element.
+ *   or_log              numpy/libm log on a sweep of arguments (<= 2 ulp).
+ *   or_femtet_*         test/program-fem-test.cpp:97-105 (ten implicit steps of the
+ *                       bar2k mesh reproduce the reference's positions to 1e-13),
+ *                       central finite differences of the force (the stiffness is
+ *                       its derivative), rigid translation/rotation invariance.
  *   or_bsrmv            dense numpy product; test/system-gemv-tests.cpp:854-903.
  *   or_dot              test/system-vector-tests.cpp:51-124 integer sums.
  *   or_cg_*             test/program-cg-test.cpp:41-43 fixture; scipy direct
@@ -97,6 +102,20 @@ void or_tet_assemble(const or_tet_params *P, int32_t nv, int64_t nt,
                      const double *mu, const double *lam,
                      const uint32_t *rowptr, const uint32_t *colidx,
                      double *vals, double *f);
+
+/* ---- neo-Hookean tets, test/input/program/femTet.sim --------------- */
+double or_log(double x);
+void or_femtet_geom(const double *xs, double *B, double *W);
+void or_femtet_precompute(double rho, int32_t nv, int64_t nt, const int32_t *ep,
+                          const double *x, double *W, double *B, double *m);
+void or_femtet_H(const double *xs, double u, double l, double W, const double *B, double *H);
+void or_femtet_kb(const double *xs, double u, double l, double W, const double *B, int jj,
+                  int row, double *out);
+void or_femtet_assemble(double h, double rho, const double *grav, int32_t nv, int64_t nt,
+                        const int32_t *ep, const double *x, const double *v, const int32_t *c,
+                        const double *u, const double *l, const double *W, const double *B,
+                        const uint32_t *rowptr, const uint32_t *colidx, double *vals,
+                        double *f);
 
 /* ---- trilinear hex, scalar Poisson ---------------------------------- */
 typedef struct {

@@ -97,6 +97,13 @@ def _declare(L):
                         C.POINTER(CGStats)]
     L.or_springs_step.argtypes = [C.POINTER(SpringsParams), C.POINTER(CGOpts), C.c_int32,
                                   C.c_int64] + [vp] * 9 + [C.POINTER(CGStats)]
+    L.or_log.restype = C.c_double
+    L.or_log.argtypes = [C.c_double]
+    L.or_femtet_geom.argtypes = [vp, vp, vp]
+    L.or_femtet_precompute.argtypes = [C.c_double, C.c_int32, C.c_int64] + [vp] * 5
+    L.or_femtet_H.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp, vp]
+    L.or_femtet_kb.argtypes = [vp, C.c_double, C.c_double, C.c_double, vp, C.c_int, C.c_int, vp]
+    L.or_femtet_assemble.argtypes = [C.c_double, C.c_double, vp, C.c_int32, C.c_int64] + [vp] * 12
 
 
 def _p(a):
@@ -188,6 +195,66 @@ def tet_assemble(P, prob, rowptr, colidx):
                           _p(_c(prob["lam"], np.float64)), _p(rowptr), _p(colidx), _p(vals),
                           _p(f))
     return vals[: nnz * 9], f
+
+
+# ---- neo-Hookean tets (test/input/program/femTet.sim) -----------------------------------
+def log(x):
+    return lib().or_log(float(x))
+
+
+def femtet_precompute(prob, rho=1000.0):
+    """initializeTet: per-tet W, B (row-major 3x3) and the vertex m field
+    (prob['m'] is updated in place, as the map assigns it)."""
+    ep = _c(prob["endpoints"], np.int32)
+    nt = ep.shape[0]
+    W = np.zeros(nt)
+    B = np.zeros(nt * 9)
+    m = prob["m"]
+    assert m.dtype == np.float64 and m.flags.c_contiguous
+    lib().or_femtet_precompute(rho, int(prob["nverts"]), nt, _p(ep), _p(_c(prob["x"], np.float64)),
+                               _p(W), _p(B), _p(m))
+    return W, B
+
+
+def femtet_H(xs, u, l, W, B):
+    H = np.zeros(9)
+    lib().or_femtet_H(_p(_c(xs, np.float64).ravel()), u, l, W, _p(_c(B, np.float64).ravel()),
+                      _p(H))
+    return H.reshape(3, 3)
+
+
+def femtet_kb(xs, u, l, W, B, jj, row):
+    out = np.zeros(9)
+    lib().or_femtet_kb(_p(_c(xs, np.float64).ravel()), u, l, W, _p(_c(B, np.float64).ravel()),
+                       jj, row, _p(out))
+    return out.reshape(3, 3)
+
+
+def femtet_assemble(prob, rowptr, colidx, h=0.01, rho=1000.0, grav=(0.0, -10.0, 0.0)):
+    """main's f = map compute_force(h) and A = map compute_stiffness(h)."""
+    nv = int(prob["nverts"])
+    ep = _c(prob["endpoints"], np.int32)
+    nnz = int(rowptr[-1])
+    vals = np.zeros(max(nnz, 1) * 9)
+    f = np.zeros(nv * 3)
+    g = _c(grav, np.float64)
+    lib().or_femtet_assemble(h, rho, _p(g), nv, ep.shape[0], _p(ep),
+                             _p(_c(prob["x"], np.float64)), _p(_c(prob["v"], np.float64)),
+                             _p(_c(prob["c"], np.int32)), _p(_c(prob["u"], np.float64)),
+                             _p(_c(prob["l"], np.float64)), _p(_c(prob["W"], np.float64)),
+                             _p(_c(prob["B"], np.float64)), _p(rowptr), _p(colidx), _p(vals),
+                             _p(f))
+    return vals[: nnz * 9], f
+
+
+def femtet_step(prob, rowptr, colidx, h=0.01):
+    """One femTet.sim main(): assemble, CG from v (absolute 1e-6 on r'r, 100
+    iterations), v = x, x = h*v + x.  Updates prob['x'] and prob['v']."""
+    vals, f = femtet_assemble(prob, rowptr, colidx, h)
+    x, st = cg(rowptr, colidx, vals, f, prob["v"], 3, cg_opts(maxiters=100, tol=1e-6, rel=0))
+    prob["v"][:] = x
+    prob["x"][:] = h * x + prob["x"]
+    return st
 
 
 def hex_kref(a, b):

@@ -18,7 +18,7 @@ from pathlib import Path
 
 __all__ = [
     "SimitError", "load", "Context", "Index", "Bsr", "CGOptions", "CGStats",
-    "SpringsParams", "TetParams", "HexParams", "Stencil27",
+    "SpringsParams", "TetParams", "HexParams", "FemTetParams", "Stencil27",
     "F64", "F32", "F32X2", "F32D",
     "PRECOND_NONE", "PRECOND_JACOBI", "PRECOND_BLOCK_JACOBI", "PRECOND_LEARNED",
     "INDEX_FORCE_DIAG", "LIB_PATH", "HEADER_PATH",
@@ -84,6 +84,16 @@ class TetParams(C.Structure):
 class TetSet(C.Structure):
     _fields_ = [("nverts", C.c_uint32), ("ntets", C.c_uint64), ("endpoints", C.c_void_p),
                 ("x", C.c_void_p), ("c", C.c_void_p), ("mu", C.c_void_p), ("lam", C.c_void_p)]
+
+
+class FemTetParams(C.Structure):
+    _fields_ = [("h", C.c_double), ("rho", C.c_double), ("grav", C.c_double * 3)]
+
+
+class FemTetSet(C.Structure):
+    _fields_ = [("nverts", C.c_uint32), ("ntets", C.c_uint64), ("endpoints", C.c_void_p),
+                ("x", C.c_void_p), ("v", C.c_void_p), ("c", C.c_void_p), ("u", C.c_void_p),
+                ("l", C.c_void_p), ("W", C.c_void_p), ("B", C.c_void_p)]
 
 
 class HexParams(C.Structure):
@@ -167,6 +177,10 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_assemble_hex_poisson": (C.c_int, [vp, vp, C.POINTER(HexParams), C.POINTER(HexSet),
                                                i32, vp, vp, vp]),
         "smh_assemble_edge_blocks": (C.c_int, [vp, vp, C.POINTER(EdgeBlocks), i32, vp, vp]),
+        "smh_assemble_tet_neohookean": (C.c_int, [vp, vp, C.POINTER(FemTetParams),
+                                                  C.POINTER(FemTetSet), i32, vp, vp, vp]),
+        "smh_femtet_precompute": (C.c_int, [vp, vp, C.POINTER(FemTetParams),
+                                            C.POINTER(FemTetSet), vp, vp, vp]),
         "smh_bsrmv": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, i32]),
         "smh_dot": (C.c_int, [vp, i32, u64, vp, vp, C.POINTER(C.c_double)]),
         "smh_stencil27_from_bsr": (C.c_int, [vp, C.POINTER(Bsr), u32, u32, u32, vp, vp]),
@@ -290,6 +304,26 @@ class Context:
         self._check(self.lib.smh_assemble_hex_poisson(self.h, idx.h, C.byref(params), C.byref(s),
                                                       precision, _ptr(vals), _ptr(vals_lo),
                                                       _ptr(rhs)))
+
+    def _femtet_set(self, idx, verts: dict, tets: dict):
+        return FemTetSet(idx.nrows, int(tets["endpoints"].shape[0]), _ptr(tets["endpoints"]),
+                         _ptr(verts["x"]), _ptr(verts.get("v")), _ptr(verts.get("c")),
+                         _ptr(tets.get("u")), _ptr(tets.get("l")), _ptr(tets.get("W")),
+                         _ptr(tets.get("B")))
+
+    def assemble_femtet(self, idx, params: FemTetParams, verts: dict, tets: dict,
+                        precision: int, vals, vals_lo, rhs):
+        """femTet.sim main's two maps: A (stiffness + lumped mass) and f."""
+        s = self._femtet_set(idx, verts, tets)
+        self._check(self.lib.smh_assemble_tet_neohookean(self.h, idx.h, C.byref(params),
+                                                         C.byref(s), precision, _ptr(vals),
+                                                         _ptr(vals_lo), _ptr(rhs)))
+
+    def femtet_precompute(self, idx, params: FemTetParams, verts: dict, tets: dict, W, B, m):
+        """femTet.sim initializeTet: W, B per tet and the vertex masses m."""
+        s = self._femtet_set(idx, verts, tets)
+        self._check(self.lib.smh_femtet_precompute(self.h, idx.h, C.byref(params), C.byref(s),
+                                                   _ptr(W), _ptr(B), _ptr(m)))
 
     def assemble_edge_blocks(self, idx, endpoints, w, bdim: int, sign=(1, 1, 1, 1),
                              alpha: float = 0.0, beta: float = 1.0, precision: int = F64,

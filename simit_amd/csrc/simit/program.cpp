@@ -620,6 +620,81 @@ class ThermalHexFunction : public HipFunction {
   double *rhs = nullptr;
 };
 
+// ---- test/input/program/femTet.sim: neo-Hookean tets ------------------------------------
+// initializeTet: verts.m = map precomputeTetMat to tets (Tet.B, Tet.W, Vert.m)
+class FemTetInitFunction : public HipFunction {
+ public:
+  FemTetInitFunction()
+      : HipFunction({{"verts", 0,
+                      {{"x", FieldType::Float64, 3, false}, {"m", FieldType::Float64, 1, true}}},
+                     {"tets", 4,
+                      {{"W", FieldType::Float64, 1, true}, {"B", FieldType::Float64, 9, true}}}},
+                    {}, {}, "smh_femtet_precompute (k_femtet_geom, k_femtet_mass)") {}
+
+ protected:
+  void setup(smh_ctx *) override { buildIndex("verts", "tets"); }
+  void runDevice(smh_ctx *ctx) override {
+    smh_femtet_params p{0.01, 1000.0, {0.0, -10.0, 0.0}};
+    smh_femtet_set s{(uint32_t)set("verts")->getSize(), (uint64_t)set("tets")->getSize(),
+                     endpoints.at("tets"), (const double *)dev("verts.x"), nullptr, nullptr,
+                     nullptr, nullptr, nullptr, nullptr};
+    check(smh_femtet_precompute(ctx, index, &p, &s, (double *)dev("tets.W"),
+                                (double *)dev("tets.B"), (double *)dev("verts.m")),
+          "precompute");
+  }
+};
+
+// main: A = map compute_stiffness(h), f = map compute_force(h), the inline
+// CG from v (r'r > 1e-6, 100 iterations), verts.v = x, verts.x = h*x + x
+class FemTetStepFunction : public HipFunction {
+ public:
+  FemTetStepFunction()
+      : HipFunction({{"verts", 0,
+                      {{"x", FieldType::Float64, 3, true},
+                       {"v", FieldType::Float64, 3, true},
+                       {"c", FieldType::Int32, 1, false}}},
+                     {"tets", 4,
+                      {{"u", FieldType::Float64, 1, false},
+                       {"l", FieldType::Float64, 1, false},
+                       {"W", FieldType::Float64, 1, false},
+                       {"B", FieldType::Float64, 9, false}}}},
+                    {}, {},
+                    "smh_assemble_tet_neohookean (k_assemble<FemTetOp>) -> smh_cg_solve "
+                    "(FP64, absolute r'r) -> smh_axpby (x += h v)") {}
+
+ protected:
+  void setup(smh_ctx *) override {
+    buildIndex("verts", "tets");
+    vals = (double *)alloc(sizeof(double) * 9 * (view.nnz ? view.nnz : 1));
+    rhs = (double *)alloc(sizeof(double) * 3 * (size_t)set("verts")->getSize() + 8);
+  }
+  void runDevice(smh_ctx *ctx) override {
+    const double h = 0.01;
+    smh_femtet_params p{h, 1000.0, {0.0, -10.0, 0.0}};
+    Set *vs = set("verts");
+    smh_femtet_set s{(uint32_t)vs->getSize(), (uint64_t)set("tets")->getSize(),
+                     endpoints.at("tets"), (const double *)dev("verts.x"),
+                     (const double *)dev("verts.v"), (const int32_t *)dev("verts.c"),
+                     (const double *)dev("tets.u"), (const double *)dev("tets.l"),
+                     (const double *)dev("tets.W"), (const double *)dev("tets.B")};
+    check(smh_assemble_tet_neohookean(ctx, index, &p, &s, SMH_F64, vals, nullptr, rhs),
+          "assembly");
+    smh_bsr A{view.nrows, 3, view.nnz, view.rowptr, view.colidx, SMH_F64, vals, nullptr};
+    smh_cg_opts o;
+    smh_cg_default_opts(&o);
+    o.maxiters = 100;
+    o.tol = 1e-6;
+    o.rel = 0;
+    solve(A, rhs, (double *)dev("verts.v"), o);
+    check(smh_axpby(ctx, 3 * (uint64_t)vs->getSize(), h, (const double *)dev("verts.v"), 1.0,
+                    (double *)dev("verts.x")),
+          "x update");
+  }
+
+ private:
+  double *vals = nullptr, *rhs = nullptr;
+};
+
 // ---- extern sparse matrices bound as CSR TensorData -------------------------------------
 // A = -A on an extern int matrix: the reference's extern-CSR function test
 // (test/function-tests.cpp:96-139, an IndexExpr -A(i,j) over extern A).
@@ -693,10 +768,15 @@ void HipFunction::hipInfo(std::ostream &os) {
 }
 
 const std::vector<std::string> kPrograms = {"isprings", "springs_implicit", "cg", "fem_static",
-                                            "thermal_hex", "extern_neg", "extern_spmv",
+                                            "thermal_hex", "femTet", "extern_neg", "extern_spmv",
                                             "extern_spmv_3x3", "extern_cg", "extern_cg_3x3"};
 
-std::shared_ptr<backend::Function> makeFunction(const std::string &program) {
+std::shared_ptr<backend::Function> makeFunction(const std::string &program,
+                                                const std::string &function) {
+  if (program == "femTet") {
+    if (function == "initializeTet") return std::make_shared<FemTetInitFunction>();
+    return std::make_shared<FemTetStepFunction>();
+  }
   if (program == "isprings") {
     // test/input/program/isprings.sim: h = 1e-2, viscous = 1e-1, gravity
     // -9.81 z, MD(p,p) = 1.0 for fixed points, CG tol 1e-5 on r'r, 25 iterations
@@ -824,13 +904,16 @@ int Program::loadBuiltin(const std::string &name) {
 }
 std::vector<std::string> Program::getFunctionNames() const {
   if (program.empty()) return {};
+  if (program == "femTet") return {"initializeTet", "main"};
   return {"main"};
 }
 Function Program::compile(const std::string &function) {
   if (program.empty()) throw SimitException("no program loaded");
-  if (function != "main" && function != "timestep" && function != program)
+  const std::vector<std::string> fs = getFunctionNames();
+  if (std::find(fs.begin(), fs.end(), function) == fs.end() && function != "timestep" &&
+      function != program)
     throw SimitException("no function " + function + " in program " + program);
-  return Function(makeFunction(program));
+  return Function(makeFunction(program, function));
 }
 
 }  // namespace simit

@@ -15,6 +15,9 @@
 //   "fem_static"       linear-elastic tets (fem_linear.sim element), clamped
 //                      rows, body load, mixed-precision block-Jacobi PCG
 //   "thermal_hex"      scalar trilinear hex Poisson, Dirichlet rows, PCG
+//   "femTet"           test/input/program/femTet.sim: initializeTet (rest
+//                      shape B, volume W, vertex masses) and main (neo-Hookean
+//                      stiffness and force, inline CG, position update)
 //   "extern_neg"       A = -A on an extern int matrix bound as CSR TensorData
 //                      (test/function-tests.cpp:96-139)
 //   "extern_spmv", "extern_spmv_3x3"

@@ -98,10 +98,9 @@ struct SpringOp {
     return true;
   }
   __device__ static bool extra(const Args &, const Pair &, double *) { return false; }
-  __device__ static bool rhs(const Args &, const Pair &P, double *f) {
-    if (!P.fa) return false;
-    for (int q = 0; q < 3; ++q) f[q] = P.a == 0 ? P.fe0[q] : -P.fe0[q];
-    return true;
+  __device__ static void rhs(const Args &, const Pair &P, double *f) {
+    if (!P.fa) return;
+    for (int q = 0; q < 3; ++q) f[q] += P.a == 0 ? P.fe0[q] : -P.fe0[q];
   }
   // MDK = MD + K and f = fmg + fe, with MD/fmg from compute_mass_damping
   // (isprings.sim:19-36), for the lane that owns row i's diagonal block.
@@ -243,11 +242,10 @@ struct TetOp {
     for (int q = 0; q < 9; ++q) out[q] = (q % 4 == 0) ? mm : 0.0;
     return true;
   }
-  __device__ static bool rhs(const Args &A, const Pair &P, double *f) {
-    if (!P.fa) return false;
+  __device__ static void rhs(const Args &A, const Pair &P, double *f) {
+    if (!P.fa) return;
     const double wq = 0.25 * P.W;
-    for (int q = 0; q < 3; ++q) f[q] = wq * A.p.load[q];
-    return true;
+    for (int q = 0; q < 3; ++q) f[q] += wq * A.p.load[q];
   }
   __device__ static void finish_diag(const Args &A, uint32_t i, double *blk, double *) {
     if (A.p.fixed_diag == 0.0 || A.c[i] <= 0) return;
@@ -304,10 +302,9 @@ struct HexOp {
     return true;
   }
   __device__ static bool extra(const Args &, const Pair &, double *) { return false; }
-  __device__ static bool rhs(const Args &A, const Pair &P, double *f) {
-    if (!((P.freemask >> P.a) & 1u)) return false;
-    f[0] = A.p.source * (A.p.h * A.p.h * A.p.h) * 0.125;
-    return true;
+  __device__ static void rhs(const Args &A, const Pair &P, double *f) {
+    if (!((P.freemask >> P.a) & 1u)) return;
+    f[0] += A.p.source * (A.p.h * A.p.h * A.p.h) * 0.125;
   }
   __device__ static void finish_diag(const Args &A, uint32_t i, double *blk, double *) {
     if (A.dir[i] <= 0) return;
@@ -358,7 +355,7 @@ struct EdgeOp {
     return true;
   }
   __device__ static bool extra(const Args &, const Pair &, double *) { return false; }
-  __device__ static bool rhs(const Args &, const Pair &, double *) { return false; }
+  __device__ static void rhs(const Args &, const Pair &, double *) {}
   __device__ static void finish_diag(const Args &, uint32_t, double *, double *) {}
   __device__ static void isolated_rhs(const Args &, uint32_t, double *f) {
     for (int q = 0; q < BDIM; ++q) f[q] = 0.0;
@@ -370,6 +367,201 @@ struct EdgeOp {
         blk[i * BDIM + j] = ((row == col && i == j) ? A.alpha : 0.0) + A.beta * blk[i * BDIM + j];
   }
 };
+
+// Neo-Hookean tets, test/input/program/femTet.sim: main's two maps, A = map
+// compute_stiffness(h) (:74-124) and f = map compute_force(h) (:37-72).  The
+// arithmetic repeats the oracle's operation for operation (FP64, no fused
+// multiply-add), log included, so SMH_F64 results are bit-identical.
+struct FemTetArgs {
+  smh_femtet_params p;
+  const int32_t *ep;
+  const double *x, *v, *u, *l, *W, *B;
+  const int32_t *c;
+};
+// log x = k ln 2 + 2 atanh(s), x = m 2^k with m in [sqrt(1/2), sqrt(2)),
+// s = (m - 1)/(m + 1): the series 2 s (1 + s^2/3 + ... + s^24/25)
+__device__ double fem_log(double x) {
+  if (!(x > 0.0)) return x == 0.0 ? -INFINITY : NAN;
+  if (x == INFINITY) return x;
+  int k = 0;
+  double m = frexp(x, &k);
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    k = k - 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0);
+  const double s2 = s * s;
+  double t = 1.0 / 25.0;
+#pragma unroll
+  for (int n = 11; n >= 0; --n) t = t * s2 + 1.0 / (double)(2 * n + 1);
+  return (double)k * 0.69314718055994530942 + 2.0 * s * t;
+}
+// C = A*B and C = A*B' (k ascending)
+__device__ __forceinline__ void fem_mm(const double *A, const double *B, double *C) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+__device__ __forceinline__ void fem_mmt(const double *A, const double *B, double *C) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j)
+      C[3 * i + j] =
+          A[3 * i] * B[3 * j] + A[3 * i + 1] * B[3 * j + 1] + A[3 * i + 2] * B[3 * j + 2];
+}
+struct FemTetOp {
+  static constexpr int CARD = 4, BD = 3;
+  static constexpr bool kHasRhs = true, kExtra = true;
+  using Args = FemTetArgs;
+  struct State {
+    int32_t vid[4];
+    uint32_t freemask;
+    double xs[12];
+    double B[9];
+    double u, l, W;
+  };
+  // the row's four blocks (0 where a pair is not free-free), the element's
+  // mass term, and its writes to the row's force, in program order
+  struct Pair {
+    int32_t vid[4];
+    uint32_t freemask;
+    int32_t fa, nf, pad;
+    double blk[4][9];
+    double mq;
+    double fw[4][3];
+  };
+  __device__ static void load(const Args &A, uint64_t e, State &st) {
+    st.freemask = 0;
+    for (int a = 0; a < 4; ++a) {
+      const int32_t v = A.ep[4 * e + a];
+      st.vid[a] = v;
+      st.freemask |= (A.c[v] <= 0 ? 1u : 0u) << a;
+      for (int q = 0; q < 3; ++q) st.xs[3 * a + q] = A.x[3 * (int64_t)v + q];
+    }
+    for (int q = 0; q < 9; ++q) st.B[q] = A.B[9 * e + q];
+    st.u = A.u[e];
+    st.l = A.l[e];
+    st.W = A.W[e];
+  }
+  __device__ static void pair(const Args &A, const State &st, int a, Pair &P) {
+    for (int t = 0; t < 4; ++t) P.vid[t] = st.vid[t];
+    P.freemask = st.freemask;
+    P.fa = (st.freemask >> a) & 1u;
+    const double *B = st.B;
+    const double u = st.u, l = st.l, W = st.W;
+    // F = Ds*B, Ds(jj,ii) = x_ii(jj) - x_3(jj)
+    double Ds[9], F[9], Fi[9], FiT[9];
+    for (int ii = 0; ii < 3; ++ii)
+      for (int jj = 0; jj < 3; ++jj) Ds[3 * jj + ii] = st.xs[3 * ii + jj] - st.xs[9 + jj];
+    fem_mm(Ds, B, F);
+    const double JJ = fem_log(det3(F));
+    inv3(F, Fi);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) FiT[3 * i + j] = Fi[3 * j + i];
+    const double nW = -W;
+    // compute_force: H = -W * PK1(F) * B'; f(v(a)) gets h*H(:,a) (a < 3) or
+    // -h*H(:,ii) for ii = 0..2 (a = 3), then h*fg + m*v(a)
+    {
+      const double lJ = l * JJ;
+      double Ps[9], H[9];
+      for (int q = 0; q < 9; ++q) Ps[q] = nW * (u * (F[q] - FiT[q]) + lJ * FiT[q]);
+      fem_mmt(Ps, B, H);
+      const double h = A.p.h;
+      const double mq = 0.25 * A.p.rho * W;
+      P.mq = mq;
+      int nf = 0;
+      if (a < 3) {
+        for (int q = 0; q < 3; ++q) P.fw[0][q] = h * H[3 * q + a];
+        nf = 1;
+      } else {
+        for (int ii = 0; ii < 3; ++ii)
+          for (int q = 0; q < 3; ++q) P.fw[ii][q] = -h * H[3 * q + ii];
+        nf = 3;
+      }
+      const int64_t va = st.vid[a];
+      for (int q = 0; q < 3; ++q) P.fw[nf][q] = h * (mq * A.p.grav[q]) + mq * A.v[3 * va + q];
+      P.nf = nf + 1;
+    }
+    // compute_stiffness: block (a, b) = -(h*h) Kb(a,:,:) for the dF
+    // directions (b, kk)
+    const double hh = A.p.h * A.p.h;
+    for (int b = 0; b < 4; ++b) {
+      if (!P.fa || !((st.freemask >> b) & 1u)) {
+        for (int q = 0; q < 9; ++q) P.blk[b][q] = 0.0;
+        continue;
+      }
+      double dFrow[3];
+      if (b < 3) {
+        for (int ll = 0; ll < 3; ++ll) dFrow[ll] = B[3 * b + ll];
+      } else {
+        for (int ii = 0; ii < 3; ++ii) dFrow[ii] = -(B[ii] + B[3 + ii] + B[6 + ii]);
+      }
+      for (int kk = 0; kk < 3; ++kk) {
+        double dF[9], FidF[9], T1[9], T2[9], dPs[9], dH[9];
+        for (int q = 0; q < 9; ++q) dF[q] = 0.0;
+        for (int ll = 0; ll < 3; ++ll) dF[3 * kk + ll] = dFrow[ll];
+        fem_mm(Fi, dF, FidF);
+        const double s1 = u - l * JJ;
+        for (int q = 0; q < 9; ++q) T1[q] = s1 * FiT[q];
+        fem_mmt(T1, FidF, T2);
+        const double s2 = l * (FidF[0] + FidF[4] + FidF[8]);
+        for (int q = 0; q < 9; ++q) dPs[q] = nW * (u * dF[q] + T2[q] + s2 * FiT[q]);
+        fem_mmt(dPs, B, dH);
+        for (int ll = 0; ll < 3; ++ll) {
+          const double kb = a < 3 ? dH[3 * ll + a]
+                                  : -(dH[3 * ll + 0] + dH[3 * ll + 1] + dH[3 * ll + 2]);
+          P.blk[b][3 * ll + kk] = -(hh * kb);
+        }
+      }
+    }
+  }
+  // K(v(jj), v(row)) is written iff both are free (femTet.sim:111-117)
+  __device__ static bool block(const Args &, const Pair &P, int b, double *out) {
+    if (!P.fa || !((P.freemask >> b) & 1u)) return false;
+    for (int q = 0; q < 9; ++q) out[q] = P.blk[b][q];
+    return true;
+  }
+  // the lumped mass, on every diagonal after the stiffness (:119-123)
+  __device__ static bool extra(const Args &, const Pair &P, double *out) {
+    for (int q = 0; q < 9; ++q) out[q] = P.mq * ((q % 4 == 0) ? 1.0 : 0.0);
+    return true;
+  }
+  __device__ static void rhs(const Args &, const Pair &P, double *f) {
+    if (!P.fa) return;
+    for (int k = 0; k < P.nf; ++k)
+      for (int q = 0; q < 3; ++q) f[q] += P.fw[k][q];
+  }
+  __device__ static void finish_diag(const Args &, uint32_t, double *, double *) {}
+  __device__ static void isolated_rhs(const Args &, uint32_t, double *f) {
+    f[0] = f[1] = f[2] = 0.0;
+  }
+  __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
+};
+
+// initializeTet (femTet.sim:126-148): per tet B = 1.0*inv(M), W = -(1/6)
+// det(M); then m of each vertex from the last (highest) tet holding it
+__global__ void k_femtet_geom(uint64_t nt, const int32_t *__restrict__ ep,
+                              const double *__restrict__ x, double *__restrict__ W,
+                              double *__restrict__ Bout) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  double xs[12], M[9], Mi[9];
+  for (int a = 0; a < 4; ++a) {
+    const int32_t v = ep[4 * t + a];
+    for (int q = 0; q < 3; ++q) xs[3 * a + q] = x[3 * (int64_t)v + q];
+  }
+  for (int ii = 0; ii < 3; ++ii)
+    for (int jj = 0; jj < 3; ++jj) M[3 * jj + ii] = xs[3 * ii + jj] - xs[9 + jj];
+  inv3(M, Mi);
+  for (int q = 0; q < 9; ++q) Bout[9 * t + q] = 1.0 * Mi[q];
+  W[t] = -(1.0 / 6.0) * det3(M);
+}
+__global__ void k_femtet_mass(uint32_t nv, double rho, const uint32_t *__restrict__ ve_ptr,
+                              const uint32_t *__restrict__ ve_elem, const double *__restrict__ W,
+                              double *__restrict__ m) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv || ve_ptr[i] == ve_ptr[i + 1]) return;
+  m[i] = 0.25 * rho * W[ve_elem[ve_ptr[i + 1] - 1]];
+}
 
 // ---- the pull kernel --------------------------------------------------------
 template <int PREC>
@@ -524,9 +716,7 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
         if (is_diag) {
           if (Op::kExtra && Op::extra(A, P, blk))
             for (int k = 0; k < BB; ++k) acc[k] += blk[k];
-          double fr[BD];
-          if (Op::rhs(A, P, fr))
-            for (int k = 0; k < BD; ++k) f[k] += fr[k];
+          Op::rhs(A, P, f);  // the element's writes to the row's f, in order
         }
       }
     }
@@ -691,4 +881,43 @@ extern "C" int smh_assemble_edge_blocks(smh_ctx *ctx, const smh_index *idx, cons
                                         "smh_assemble_edge_blocks");
   }
   return smh::fail(ctx, SMH_ERR_UNSUPPORTED, "smh_assemble_edge_blocks: bdim %u (1..3)", e->bdim);
+}
+
+extern "C" int smh_assemble_tet_neohookean(smh_ctx *ctx, const smh_index *idx,
+                                           const smh_femtet_params *p, const smh_femtet_set *s,
+                                           int32_t prec, void *vals, void *vals_lo, double *rhs) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, p && s, SMH_ERR_INVALID, "smh_assemble_tet_neohookean: null params or set");
+  SMH_CHECK(ctx, idx && s->nverts == idx->nrows, SMH_ERR_INVALID,
+            "smh_assemble_tet_neohookean: vertex count does not match the index rows");
+  SMH_CHECK(ctx, s->x && s->v && s->c &&
+                     (s->ntets == 0 || (s->u && s->l && s->W && s->B)),
+            SMH_ERR_INVALID, "smh_assemble_tet_neohookean: null field");
+  FemTetArgs A{*p, s->endpoints, s->x, s->v, s->u, s->l, s->W, s->B, s->c};
+  return launch_assembly<FemTetOp>(ctx, idx, s->endpoints, A, prec, vals, vals_lo, rhs,
+                                   "smh_assemble_tet_neohookean");
+}
+
+extern "C" int smh_femtet_precompute(smh_ctx *ctx, const smh_index *idx,
+                                     const smh_femtet_params *p, const smh_femtet_set *s,
+                                     double *W, double *B, double *m) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, p && s && idx && W && B && m && s->x, SMH_ERR_INVALID,
+            "smh_femtet_precompute: null argument");
+  SMH_CHECK(ctx, s->nverts == idx->nrows, SMH_ERR_INVALID,
+            "smh_femtet_precompute: vertex count does not match the index rows");
+  int set = -1;
+  for (uint32_t q = 0; q < idx->nsets; ++q)
+    if (idx->ep[q] == s->endpoints && idx->card[q] == 4u) set = (int)q;
+  SMH_CHECK(ctx, set >= 0, SMH_ERR_INVALID,
+            "smh_femtet_precompute: the tets' endpoints are not one of the index's edge sets");
+  if (s->ntets)
+    hipLaunchKernelGGL(k_femtet_geom, dim3(smh::grid_for(s->ntets, 256)), dim3(256), 0,
+                       ctx->stream, s->ntets, s->endpoints, s->x, W, B);
+  if (s->nverts)
+    hipLaunchKernelGGL(k_femtet_mass, dim3(smh::grid_for(s->nverts, 256)), dim3(256), 0,
+                       ctx->stream, s->nverts, p->rho, idx->ve_ptr[set], idx->ve_elem[set],
+                       (const double *)W, m);
+  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
 }

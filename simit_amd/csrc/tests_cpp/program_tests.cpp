@@ -475,6 +475,112 @@ static void extern_csr() {
   }
 }
 
+// test/program-fem-test.cpp:14-106: the bar2k mesh, initializeTet, ten
+// steps of femTet.sim's main; and test/reorder-tests.cpp:403-441, the same
+// after a Hilbert reordering of the vertices and tets
+static Function loadFn(const std::string &program, const std::string &fn) {
+  Program p;
+  if (p.loadFile(program + ".sim") != 0) {
+    std::printf("  FAIL: cannot load %s\n", program.c_str());
+    ++failures;
+    return Function();
+  }
+  return p.compile(fn);
+}
+static FieldRef<double, 3> initializeFem(const MeshVol &mv, Set &verts, Set &tets,
+                                         std::vector<ElementRef> &vertRefs) {
+  FieldRef<double, 3> x = verts.addField<double, 3>("x");
+  FieldRef<double, 3> v = verts.addField<double, 3>("v");
+  FieldRef<double, 3> fe = verts.addField<double, 3>("fe");
+  FieldRef<int> c = verts.addField<int>("c");
+  FieldRef<double> m = verts.addField<double>("m");
+  FieldRef<double> u = tets.addField<double>("u");
+  FieldRef<double> l = tets.addField<double>("l");
+  tets.addField<double>("W");
+  tets.addField<double, 3, 3>("B");
+  const double E = 5e3, nu = 0.45;
+  const double uval = 0.5 * E / nu, lval = E * nu / ((1 + nu) * (1 - 2 * nu));
+  const double eps = 0.0001;
+  for (size_t ii = 0; ii < mv.v.size(); ++ii) {
+    vertRefs.push_back(verts.add());
+    ElementRef p = vertRefs.back();
+    x.set(p, {mv.v[ii][0], mv.v[ii][1], mv.v[ii][2]});
+    v.set(p, {0.1, 0.0, 0.1});
+    c.set(p, 0);
+    fe.set(p, {0.0, 0.0, 0.0});
+    m.set(p, 0.0);
+    if (mv.v[ii][1] < eps) {
+      c.set(p, 1);
+      v.set(p, {0.0, 0.0, 0.0});
+    }
+  }
+  for (size_t ii = 0; ii < mv.e.size(); ++ii) {
+    ElementRef t = tets.add(vertRefs[mv.e[ii][0]], vertRefs[mv.e[ii][1]],
+                            vertRefs[mv.e[ii][2]], vertRefs[mv.e[ii][3]]);
+    u.set(t, uval);
+    l.set(t, lval);
+  }
+  return x;
+}
+static void runFem(Set &verts, Set &tets, int nSteps) {
+  Function pre = loadFn("femTet", "initializeTet");
+  if (!pre.defined()) return;
+  pre.bind("verts", &verts);
+  pre.bind("tets", &tets);
+  pre.init();
+  pre.runSafe();
+  Function step = loadFn("femTet", "main");
+  if (!step.defined()) return;
+  step.bind("verts", &verts);
+  step.bind("tets", &tets);
+  step.init();
+  for (int i = 0; i < nSteps; ++i) {
+    step.runSafe();
+    std::printf("  step %d: cg iterations %d\n", i, step.lastSolve().iters);
+  }
+}
+static void checkFem(FieldRef<double, 3> &x, ElementRef one, ElementRef two, ElementRef three) {
+  EXPECT_NEAR(0.010771915616785779, x(one)(0), 1e-13);
+  EXPECT_NEAR(0.058853573999788439, x(one)(1), 1e-13);
+  EXPECT_NEAR(0.030899457015375883, x(one)(2), 1e-13);
+  EXPECT_NEAR(0.0028221631202928516, x(two)(0), 1e-13);
+  EXPECT_NEAR(0.017969982607667911, x(two)(1), 1e-13);
+  EXPECT_NEAR(0.012885386063393013, x(two)(2), 1e-13);
+  EXPECT_NEAR(0.02411959295647129, x(three)(0), 1e-13);
+  EXPECT_NEAR(0.052036155669135678, x(three)(1), 1e-13);
+  EXPECT_NEAR(0.030173075240629205, x(three)(2), 1e-13);
+}
+static void fem_tet() {
+  MeshVol mv;
+  EXPECT_TRUE(mv.loadTet("tests/golden/fem-bar2k/bar2k.node",
+                         "tests/golden/fem-bar2k/bar2k.ele") == 0);
+  Set verts;
+  Set tets(verts, verts, verts, verts);
+  std::vector<ElementRef> vertRefs;
+  FieldRef<double, 3> x = initializeFem(mv, verts, tets, vertRefs);
+  runFem(verts, tets, 10);
+  checkFem(x, vertRefs[100], vertRefs[200], vertRefs[300]);
+}
+static void reorder_fem_tet() {
+  MeshVol mv;
+  EXPECT_TRUE(mv.loadTet("tests/golden/fem-bar2k/bar2k.node",
+                         "tests/golden/fem-bar2k/bar2k.ele") == 0);
+  Set verts;
+  Set tets(verts, verts, verts, verts);
+  std::vector<ElementRef> vertRefs;
+  FieldRef<double, 3> x = initializeFem(mv, verts, tets, vertRefs);
+  std::vector<int> vertexOrdering, edgeOrdering;
+  verts.setSpatialField("x");
+  reorder(tets, verts, edgeOrdering, vertexOrdering);
+  int moved = 0;
+  for (size_t i = 0; i < vertexOrdering.size(); ++i) moved += vertexOrdering[i] != (int)i;
+  std::printf("  %d of %zu vertices moved\n", moved, vertexOrdering.size());
+  EXPECT_TRUE(moved > 0);
+  runFem(verts, tets, 10);
+  checkFem(x, vertRefs[vertexOrdering[100]], vertRefs[vertexOrdering[200]],
+           vertRefs[vertexOrdering[300]]);
+}
+
 // extern_file DIR: the CSR of DIR/{rowptr,colidx}.i32, DIR/vals.f64 (3x3
 // blocks) and DIR/b.f64, bound as TensorData: y = A b written to DIR/y.f64
 // and x = cg(A, b) to DIR/x.f64 (the GPU test checks both with the oracle)
@@ -543,7 +649,8 @@ int main(int argc, char **argv) {
       {"mesh_bunny", mesh_bunny}, {"reorder_bunny_springs", reorder_bunny_springs},
       {"reorder_int", reorder_int}, {"reorder_double", reorder_double},
       {"hilbert_curve", hilbert_curve}, {"reorder_bunny_host", reorder_bunny_host},
-      {"mesh_load", mesh_load}, {"extern_csr", extern_csr}};
+      {"mesh_load", mesh_load}, {"extern_csr", extern_csr}, {"fem_tet", fem_tet},
+      {"reorder_fem_tet", reorder_fem_tet}};
   for (auto &t : tests) {
     bool run = argc <= 1;
     for (int i = 1; i < argc; ++i) run |= t.first == argv[i];

@@ -235,3 +235,26 @@ def tet_mesh_problem(x, tets, E=1e5, nu=0.3, clamp_frac=0.05, load=(0.0, 0.0, -9
     c = (z <= z.min() + clamp_frac * (z.max() - z.min())).astype(np.int32)
     return {"nverts": x.shape[0], "x": np.ascontiguousarray(x), "c": c, "endpoints": t,
             "mu": mu, "lam": lam, "load": load}
+
+
+def femtet_problem(node_path, ele_path):
+    """The reference's femTet fixture set-up (test/program-fem-test.cpp:14-75):
+    E = 5e3, nu = 0.45, u = 0.5 E / nu, l = E nu / ((1 + nu)(1 - 2 nu)) per
+    tet; every vertex starts with v = (0.1, 0, 0.1), c = 0, m = 0, except
+    those with y < 1e-4, which are constrained (c = 1, v = 0)."""
+    mesh = load_tetgen(node_path, ele_path)
+    x = np.ascontiguousarray(mesh["x"], dtype=np.float64)
+    nv = x.shape[0]
+    E, nu = 5e3, 0.45
+    uval = 0.5 * E / nu
+    lval = E * nu / ((1 + nu) * (1 - 2 * nu))
+    v = np.tile(np.array([0.1, 0.0, 0.1]), (nv, 1))
+    c = np.zeros(nv, dtype=np.int32)
+    low = x[:, 1] < 0.0001
+    c[low] = 1
+    v[low] = 0.0
+    ep = np.ascontiguousarray(mesh["elements"][:, :4], dtype=np.int32)
+    nt = ep.shape[0]
+    return {"nverts": nv, "x": x, "v": np.ascontiguousarray(v), "c": c,
+            "fe": np.zeros((nv, 3)), "m": np.zeros(nv), "endpoints": ep,
+            "u": np.full(nt, uval), "l": np.full(nt, lval)}

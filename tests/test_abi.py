@@ -38,6 +38,7 @@ STRUCTS = {"IndexView": "smh_index_view", "Bsr": "smh_bsr", "Stencil27": "smh_st
            "SpringsParams": "smh_springs_params", "SpringsSet": "smh_springs_set",
            "TetParams": "smh_tet_params", "TetSet": "smh_tet_set",
            "HexParams": "smh_hex_params", "HexSet": "smh_hex_set",
+           "FemTetParams": "smh_femtet_params", "FemTetSet": "smh_femtet_set",
            "EdgeBlocks": "smh_edge_blocks", "CGOptions": "smh_cg_opts",
            "CGBatchStats": "smh_cg_batch_stats", "CGStats": "smh_cg_stats"}
 

@@ -16,7 +16,8 @@ BIN = ROOT / "simit_amd" / "bin" / "simit_program_tests"
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["isprings", "cg", "springs_lattice", "mesh_bunny",
-                                  "reorder_bunny_springs", "extern_csr"])
+                                  "reorder_bunny_springs", "extern_csr", "fem_tet",
+                                  "reorder_fem_tet"])
 def test_cpp_program(name):
     import torch
     if not torch.cuda.is_available():
