@@ -70,6 +70,9 @@ def parse():
                     help="tet: 96^3 x 5 learned-PCG elasticity (default); hex: 192^3 trilinear "
                          "hex Poisson, learned PCG trained on 48^3; batch: 256 spring lattices "
                          "solved together by the batched CG")
+    ap.add_argument("--mixed", type=int, default=1, choices=(1, 2),
+                    help="tet workload: 1 = FP32 CG vectors, 2 = FP64 CG vectors (both with "
+                         "FP32 matrix values and FP64 dots and reliable updates)")
     ap.add_argument("--nmesh", type=int, default=256)
     ap.add_argument("--hex-n", type=int, default=192)
     ap.add_argument("--hex-train-n", type=int, default=48)
@@ -192,7 +195,7 @@ def main():
         assemble()
         linv, corr = net.precond(A2)
         x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
-        st = ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=1, learned=linv,
+        st = ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=args.mixed, learned=linv,
                                      learned_corr=corr))
         last["x"] = x
         return st
@@ -239,7 +242,7 @@ def main():
     def bj_mixed():
         assemble()
         x = torch.zeros(ndof, dtype=torch.float64, device="cuda")
-        return ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_BLOCK_JACOBI, mixed=1))
+        return ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_BLOCK_JACOBI, mixed=args.mixed))
 
     v64 = torch.empty(nnz * 9, dtype=torch.float64, device="cuda")
     A64 = idx.bsr(3, S.F64, v64)
@@ -301,6 +304,7 @@ def main():
                                f"log10 E = 5 + GRF(std 1, corr 8 cells), learned mixed PCG "
                                f"to a 1e-8 FP64 relative residual",
                    "nverts": nv, "ntets": nt, "nnz_blocks": nnz,
+                   "cg_vectors": "fp32" if args.mixed == 1 else "fp64",
                    "train": f"{args.train_samples} seeded {args.train_n}^3 samples"},
         "iterations": stats[-1]["iters"],
         "converged": converged,
