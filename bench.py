@@ -74,6 +74,11 @@ def parse():
                     help="tet workload: 1 = FP32 CG vectors, 2 = FP64 CG vectors (both with "
                          "FP32 matrix values and FP64 dots and reliable updates)")
     ap.add_argument("--nmesh", type=int, default=256)
+    ap.add_argument("--batch-delta", type=float, default=0.01,
+                    help="batch workload: reliable update when a lattice's residual fell by "
+                         "this factor since its last one")
+    ap.add_argument("--batch-check", type=int, default=8,
+                    help="batch workload: iterations per captured graph")
     ap.add_argument("--hex-n", type=int, default=192)
     ap.add_argument("--hex-train-n", type=int, default=48)
     ap.add_argument("--hex-train-samples", type=int, default=500)
@@ -623,8 +628,8 @@ def run_batch(args, ctx):
     # one: these lattices converge ~2x per iteration, so the default 0.3
     # replaced residuals of some lattice at nearly every iteration (23.5 ms
     # per step against 20.8 ms; the same iterations and certified residuals)
-    opts = ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, mixed=1, check_every=8,
-                          reliable_delta=0.01)
+    opts = ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, mixed=1,
+                          check_every=args.batch_check, reliable_delta=args.batch_delta)
     # the dominant kernel: the FP32 tile SpMV over the union, timed with HIP
     # events on the stream it runs on
     ctx.assemble_springs(idx, P, dev, dev, S.F32X2, hi, lo, rhs)
@@ -644,10 +649,14 @@ def run_batch(args, ctx):
     spmv_gbs = spmv_bytes / spmv_ms / 1e6
     log(f"[bench] spmv f32 {spmv_ms * 1e3:.1f} us {spmv_gbs:.0f} GB/s")
 
+    # the step: assembly and the batched CG of every lattice in one call,
+    # the assembly inside the solver's first captured graph
+    xb = torch.zeros(3 * npts, dtype=torch.float64, device="cuda")
+
     def step():
-        ctx.assemble_springs(idx, P, dev, dev, S.F32X2, hi, lo, rhs)
-        x = torch.zeros(3 * npts, dtype=torch.float64, device="cuda")
-        st, _ = ctx.cg_batched(A2, batch["rows"], rhs, x, opts)
+        xb.zero_()  # x = 0; the solver's captured graphs are kept for these buffers
+        st, _ = ctx.springs_solve_batched(idx, P, dev, dev, hi, lo, rhs, batch["rows"], xb,
+                                          opts)
         return st
 
     for w in range(args.warmup):
@@ -719,7 +728,8 @@ def run_batch(args, ctx):
         "dtype": "fp32 matrix/SpMV, fp64 dots+refinement",
         "data": "synthetic (seeded lattices, jitter and stiffness)",
         "config": {"workload": f"{args.nmesh} spring lattices n^3, n in 8..36, {npts} points, "
-                               f"assembled and solved together (batched mixed CG, one hipGraph "
+                               f"assembled and solved together (batched mixed CG: the assembly "
+                               f"and the first iterations in one captured graph, then one graph "
                                f"per chunk of iterations) to 1e-8 each", "npoints": npts,
                    "nnz_blocks": nnz},
         "converged": ok, "worst_true_rel": worst, "iterations_min": min(iters),

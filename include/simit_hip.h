@@ -403,7 +403,9 @@ int smh_cg_solve_stencil27(smh_ctx *ctx, const smh_stencil27 *A, const double *b
  * no block couples two systems (checked).  Each system runs its own CG
  * (own alpha, beta and stopping test on its own ||b_m||, as if solved alone
  * by smh_cg_solve) in lockstep; opts->check_every iterations are captured
- * once as a hipGraph and replayed.  The SpMV's workgroup count is
+ * once as a hipGraph and replayed, the replays going out ahead of the host,
+ * which watches their progress in pinned memory; the SpMV skips the row
+ * tiles of systems that have stopped.  The SpMV's workgroup count is
  * autotuned on first use of a pattern.  stats: host array of nbatch
  * entries (may be NULL); seconds (may be NULL): wall time of the solve. */
 typedef struct {
@@ -414,6 +416,20 @@ typedef struct {
 int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch, const uint32_t *rows,
                          const double *b, double *x, const smh_cg_opts *opts,
                          smh_cg_batch_stats *stats, double *seconds);
+
+/* One implicit springs step of many independent lattices (the systems of
+ * smh_cg_solve_batched) as captured graphs: the first graph holds the
+ * assembly (smh_assemble_springs into vals / vals_lo / rhs on idx's
+ * pattern; SMH_F32X2 when opts->mixed, else SMH_F64), the solver's set-up
+ * and the first check_every iterations, the second check_every more
+ * iterations and is replayed until every system has stopped.  The host
+ * learns of that from pinned memory the replays write (no copies or stream
+ * synchronisation between replays).  x: initial guess and solution. */
+int smh_springs_solve_batched(smh_ctx *ctx, const smh_index *idx, const smh_springs_params *p,
+                              const smh_springs_set *s, void *vals, void *vals_lo, double *rhs,
+                              uint32_t nbatch, const uint32_t *rows, double *x,
+                              const smh_cg_opts *opts, smh_cg_batch_stats *stats,
+                              double *seconds);
 
 /* ---- learned block-Jacobi correction (no counterpart in the reference) ----
  * A message-passing network over the matrix graph (3 layers, width 32, BF16

@@ -199,6 +199,11 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_cg_solve_batched": (C.c_int, [vp, C.POINTER(Bsr), u32, C.POINTER(u32), vp, vp,
                                            C.POINTER(CGOptions), C.POINTER(CGBatchStats),
                                            C.POINTER(C.c_double)]),
+        "smh_springs_solve_batched": (C.c_int, [vp, vp, C.POINTER(SpringsParams),
+                                                C.POINTER(SpringsSet), vp, vp, vp, u32,
+                                                C.POINTER(u32), vp, C.POINTER(CGOptions),
+                                                C.POINTER(CGBatchStats),
+                                                C.POINTER(C.c_double)]),
         "smh_gnn_create": (C.c_int, [vp, u64, C.c_double, C.POINTER(vp)]),
         "smh_gnn_destroy": (C.c_int, [vp, vp]),
         "smh_gnn_get_params": (C.c_int, [vp, vp, vp]),
@@ -409,6 +414,23 @@ class Context:
         self._check(self.lib.smh_cg_solve_batched(
             self.h, C.byref(A), nb, r.ctypes.data_as(C.POINTER(C.c_uint32)), _ptr(b), _ptr(x),
             C.byref(opts), st, C.byref(sec)))
+        return [st[m].as_dict() for m in range(nb)], sec.value
+
+    def springs_solve_batched(self, idx, params: SpringsParams, pts: dict, springs: dict,
+                              vals, vals_lo, rhs, rows, x, opts: CGOptions):
+        """Assembly of every lattice and the batched CG, in captured graphs;
+        returns (per-system stats dicts, seconds)."""
+        import numpy as np
+        s = SpringsSet(idx.nrows, int(springs["endpoints"].shape[0]), _ptr(springs["endpoints"]),
+                       _ptr(pts["x"]), _ptr(pts["v"]), _ptr(pts["m"]), _ptr(pts["fixed"]),
+                       _ptr(springs["k"]), _ptr(springs["l0"]))
+        r = np.ascontiguousarray(rows, dtype=np.uint32)
+        nb = r.size - 1
+        st = (CGBatchStats * max(nb, 1))()
+        sec = C.c_double()
+        self._check(self.lib.smh_springs_solve_batched(
+            self.h, idx.h, C.byref(params), C.byref(s), _ptr(vals), _ptr(vals_lo), _ptr(rhs), nb,
+            r.ctypes.data_as(C.POINTER(C.c_uint32)), _ptr(x), C.byref(opts), st, C.byref(sec)))
         return [st[m].as_dict() for m in range(nb)], sec.value
 
     def cg_options(self, **kw) -> CGOptions:

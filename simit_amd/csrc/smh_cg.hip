@@ -29,7 +29,10 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
+#include <memory>
 #include <mutex>
+#include <string>
 
 namespace smh {
 
@@ -95,6 +98,9 @@ int tile_plan(smh_ctx *ctx, const uint32_t *rowptr, uint32_t nrows, uint64_t nnz
     hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(p.ntiles + 1, 256)), dim3(256), 0, ctx->stream,
                        rowptr, nrows, p.tprime, p.ntiles, p.tile_row);
     SMH_LAUNCH_CHECK(ctx);
+    // a plan is complete when it is returned: its users may launch on
+    // another stream (the batched solver runs on the context's own stream)
+    SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
   ctx->plans.push_back(p);
   *out = p;
@@ -131,6 +137,19 @@ int scratch_setup(smh_ctx *ctx) {
 }
 
 void forget_plans(smh_ctx *ctx, const uint32_t *rowptr) {
+  for (size_t i = 0; i < ctx->cached.size();) {
+    if (ctx->cached[i]->rowptr == rowptr) {
+      hipStreamSynchronize(ctx->own_stream);
+      hipStreamSynchronize(ctx->stream);
+      ctx->cached.erase(ctx->cached.begin() + i);
+    } else {
+      ++i;
+    }
+  }
+  for (size_t i = 0; i < ctx->tuned.size();) {
+    if (ctx->tuned[i].rowptr == rowptr) ctx->tuned.erase(ctx->tuned.begin() + i);
+    else ++i;
+  }
   for (size_t i = 0; i < ctx->plans.size();) {
     if (ctx->plans[i].rowptr == rowptr) {
       hipStreamSynchronize(ctx->stream);
@@ -1239,14 +1258,23 @@ struct EpiPqRow {
   }
 };
 
-template <int IP, int BD>
-__global__ void __launch_bounds__(TileShape<BD>::kThreads)
+template <int IP, int BD, class SH = TileShape<BD>>
+__global__ void __launch_bounds__(SH::kThreads)
 k_bcg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP>::T *__restrict__ q,
            const float *dlo, double *prow) {
   using VT = typename Val<IP>::T;
-  __shared__ TileSmem<IP, BD> sm;
+  __shared__ TileSmem<IP, BD, SH> sm;
   uint32_t k0, k1;
   wg_tiles(A, k0, k1);
+  if (k0 < k1 && A.row_sys) {
+    // a workgroup whose rows all belong to stopped systems has nothing to do
+    const uint32_t R0 = A.tile_row[k0], R1 = A.tile_row[k1];
+    bool any = false;
+    if (R0 < R1)
+      for (uint32_t m = A.row_sys[R0]; m <= A.row_sys[R1 - 1] && !any; ++m)
+        any = A.sys_state[m] == 0;
+    if (!any) return;  // the same answer on every thread
+  }
   XPlain<VT> xg{p, BD};
   EpiPqRow<VT> epi{q, p, dlo, prow, BD};
   spmv_tiles<IP, BD>(A, k0, k1, sm, xg, epi);
@@ -1509,6 +1537,32 @@ __global__ void __launch_bounds__(256) k_bcg_init(BChunks C, const double *cp, B
   S.alpha[m] = S.beta[m] = 0.0;
 }
 
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
+  if ((int)threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+// The last kernel of every replay of the captured iterations: bumps the
+// replay counter and publishes (replay number, systems still iterating) to
+// pinned host memory, so the host learns of convergence without a copy or a
+// stream synchronisation (the count is written first, the number after a
+// system-scope fence).
+__global__ void __launch_bounds__(256) k_bcg_signal(const int *__restrict__ state, uint32_t nb,
+                                                    unsigned long long *counter,
+                                                    volatile unsigned long long *host) {
+  __shared__ unsigned s4[4];
+  unsigned n = 0;
+  for (uint32_t m = threadIdx.x; m < nb; m += 256) n += state[m] == kBActive || bflagged(state[m]);
+  for (int off = 32; off > 0; off >>= 1) n += (unsigned)__shfl_xor((int)n, off, 64);
+  if ((threadIdx.x & 63) == 0) s4[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x) return;
+  const unsigned long long c = *counter + 1;
+  *counter = c;
+  host[1] = s4[0] + s4[1] + s4[2] + s4[3];
+  __threadfence_system();
+  host[0] = c;
+}
+
 // CG over the blocked-CSR row tiles of A
 template <int IP, int RP, int BD>
 int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const smh_tile_plan *rplan,
@@ -1524,88 +1578,190 @@ int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const sm
                             st, ar);
 }
 
-// the SpMV grid of a pattern: the fastest of a few candidates, timed once
-unsigned tuned_grid(smh_ctx *ctx, hipStream_t stream, const smh_bsr *A, const smh_tile_plan *plan,
-                    void *xbuf, void *ybuf) {
+// The batched SpMV's tile shape and grid: for 3x3 blocks, every (rounds per
+// lane, threads per workgroup) shape below with 1,024, 2,048 and 4,096
+// workgroups, the kernel itself (epilogue included) timed on the pattern;
+// scalar systems keep the default shape and tune the grid only.  Chosen once
+// per pattern and precision, cached in the context.
+using BShape0 = TileShape<3>;
+using BShape1 = TileShapeR<3, 8, 64>;
+using BShape2 = TileShapeR<3, 24, 64>;
+using BShape3 = TileShapeR<3, 8, 128>;
+using BShape4 = TileShapeR<3, 16, 128>;
+constexpr int kBShapes = 5;
+inline uint32_t bshape_blocks(int k) {
+  switch (k) {
+    case 1: return BShape1::kBlocks;
+    case 2: return BShape2::kBlocks;
+    case 3: return BShape3::kBlocks;
+    case 4: return BShape4::kBlocks;
+  }
+  return BShape0::kBlocks;
+}
+inline unsigned bshape_threads(int k) {
+  switch (k) {
+    case 1: return BShape1::kThreads;
+    case 2: return BShape2::kThreads;
+    case 3: return BShape3::kThreads;
+    case 4: return BShape4::kThreads;
+  }
+  return BShape0::kThreads;
+}
+template <int IP, int BD>
+void launch_bcg_spmv(int shape, unsigned g, hipStream_t s, const TileDev &T,
+                     const typename Val<IP>::T *p, typename Val<IP>::T *q, const float *dlo,
+                     double *prow) {
+  if constexpr (BD == 3) {
+    switch (shape) {
+      case 1:
+        hipLaunchKernelGGL((k_bcg_spmv<IP, 3, BShape1>), dim3(g), dim3(BShape1::kThreads), 0, s, T,
+                           p, q, dlo, prow);
+        return;
+      case 2:
+        hipLaunchKernelGGL((k_bcg_spmv<IP, 3, BShape2>), dim3(g), dim3(BShape2::kThreads), 0, s, T,
+                           p, q, dlo, prow);
+        return;
+      case 3:
+        hipLaunchKernelGGL((k_bcg_spmv<IP, 3, BShape3>), dim3(g), dim3(BShape3::kThreads), 0, s, T,
+                           p, q, dlo, prow);
+        return;
+      case 4:
+        hipLaunchKernelGGL((k_bcg_spmv<IP, 3, BShape4>), dim3(g), dim3(BShape4::kThreads), 0, s, T,
+                           p, q, dlo, prow);
+        return;
+    }
+  }
+  hipLaunchKernelGGL((k_bcg_spmv<IP, BD>), dim3(g), dim3(tile_threads_host(BD)), 0, s, T, p, q,
+                     dlo, prow);
+}
+
+struct BTune {
+  int shape = 0;
+  unsigned grid = 0;
+  smh_tile_plan plan{};
+};
+template <int IP, int BD>
+BTune tuned_bcg(smh_ctx *ctx, hipStream_t stream, const smh_bsr *A,
+                const smh_tile_plan *default_plan, void *pbuf, void *qbuf, const float *dlo,
+                double *prow) {
+  using VT = typename Val<IP>::T;
   for (const auto &t : ctx->tuned)
-    if (t.rowptr == A->rowptr && t.nrows == A->nrows && t.nnz == A->nnz && t.bdim == A->bdim)
-      return t.grid;
-  smh_bsr Af = *A;
-  if (Af.precision == SMH_F32X2) Af.precision = SMH_F32;
-  const unsigned cands[] = {1024, 2048, 4096};
-  unsigned best = 0;
+    if (t.rowptr == A->rowptr && t.nrows == A->nrows && t.nnz == A->nnz && t.bdim == A->bdim &&
+        t.prec == IP) {
+      BTune r;
+      r.shape = t.shape;
+      r.grid = t.grid;
+      if (tile_plan(ctx, A->rowptr, A->nrows, A->nnz,
+                    BD == 3 ? bshape_blocks(t.shape) : tile_blocks_host(BD), &r.plan))
+        r.plan = *default_plan;
+      return r;
+    }
+  BTune best;
+  best.plan = *default_plan;
   float best_ms = 1e30f;
   hipEvent_t e0, e1;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 0;
-  for (unsigned g : cands) {
-    unsigned gg = 0;
-    TileDev T = tile_dev(&Af, plan, &gg);
-    gg = std::min<unsigned>(g, plan->ntiles);
-    if (gg >= 8) gg &= ~7u;
-    if (gg < 1) gg = 1;
-    T.tiles_per_wg = (uint32_t)((plan->ntiles + gg - 1) / gg);
-    float ms = 0.0f;
-    for (int rep = 0; rep < 4; ++rep) {
-      (void)hipEventRecord(e0, stream);
-      if (Af.precision == SMH_F64) {
-        if (A->bdim == 3)
-          hipLaunchKernelGGL((k_bsrmv_tiles<SMH_F64, 3>), dim3(gg), dim3(tile_threads_host(3)), 0,
-                             stream, T, (const double *)xbuf, (double *)ybuf);
-        else
-          hipLaunchKernelGGL((k_bsrmv_tiles<SMH_F64, 1>), dim3(gg), dim3(256), 0, stream, T,
-                             (const double *)xbuf, (double *)ybuf);
-      } else {
-        if (A->bdim == 3)
-          hipLaunchKernelGGL((k_bsrmv_tiles<SMH_F32, 3>), dim3(gg), dim3(tile_threads_host(3)), 0,
-                             stream, T, (const float *)xbuf, (float *)ybuf);
-        else
-          hipLaunchKernelGGL((k_bsrmv_tiles<SMH_F32, 1>), dim3(gg), dim3(256), 0, stream, T,
-                             (const float *)xbuf, (float *)ybuf);
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return best;
+  const unsigned cands[] = {1024, 2048, 4096};
+  const int nshapes = BD == 3 ? kBShapes : 1;
+  for (int k = 0; k < nshapes; ++k) {
+    smh_tile_plan pl;
+    if (tile_plan(ctx, A->rowptr, A->nrows, A->nnz,
+                  BD == 3 ? bshape_blocks(k) : tile_blocks_host(BD), &pl) != SMH_OK ||
+        !pl.tile_row)
+      continue;
+    for (unsigned g : cands) {
+      unsigned gg = 0;
+      TileDev T = tile_dev(A, &pl, &gg);
+      gg = std::min<unsigned>(g, pl.ntiles);
+      if (gg >= 8) gg &= ~7u;
+      if (gg < 1) gg = 1;
+      T.tiles_per_wg = (uint32_t)((pl.ntiles + gg - 1) / gg);
+      float ms = 0.0f;
+      for (int rep = 0; rep < 4; ++rep) {
+        (void)hipEventRecord(e0, stream);
+        launch_bcg_spmv<IP, BD>(k, gg, stream, T, (const VT *)pbuf, (VT *)qbuf, dlo, prow);
+        (void)hipEventRecord(e1, stream);
+        (void)hipEventSynchronize(e1);
+        float t = 0.0f;
+        (void)hipEventElapsedTime(&t, e0, e1);
+        if (rep) ms += t;  // the first launch warms up
       }
-      (void)hipEventRecord(e1, stream);
-      (void)hipEventSynchronize(e1);
-      float t = 0.0f;
-      (void)hipEventElapsedTime(&t, e0, e1);
-      if (rep) ms += t;  // the first launch warms up
-    }
-    if (ms < best_ms) {
-      best_ms = ms;
-      best = gg;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best.shape = k;
+        best.grid = gg;
+        best.plan = pl;
+      }
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipGetLastError();
-  ctx->tuned.push_back({A->rowptr, A->nrows, A->bdim, A->nnz, best});
+  ctx->tuned.push_back({A->rowptr, A->nrows, A->bdim, A->nnz, best.grid, best.shape, IP});
   return best;
 }
 
+// A batched solve built once and replayed: its device workspace, chunk
+// tables and three captured graphs (prologue + set-up + the first chunk of
+// iterations; a chunk of iterations; the final certification), cached in
+// the context under everything the graphs hold by value.
+struct BPlan : smh_cached {
+  // key
+  const uint32_t *colidx = nullptr;
+  const void *vals = nullptr, *vals_lo = nullptr;
+  uint32_t nrows = 0, bdim = 0;
+  uint64_t nnz = 0;
+  int prec = 0, ip = 0;
+  const double *b = nullptr;
+  double *x = nullptr;
+  std::vector<uint32_t> rows;
+  smh_cg_opts o{};
+  std::string pro_key;
+  // state
+  std::vector<void *> bufs;
+  hipGraph_t g[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t ge[3] = {nullptr, nullptr, nullptr};
+  BDev S{};
+  int check = 16;
+  ~BPlan() override {
+    for (int k = 0; k < 3; ++k) {
+      if (ge[k]) (void)hipGraphExecDestroy(ge[k]);
+      if (g[k]) (void)hipGraphDestroy(g[k]);
+    }
+    for (void *p : bufs) dfree(p);
+  }
+  bool same(const smh_bsr *A, const double *bb, const double *xx, uint32_t nb,
+            const uint32_t *hrows, const smh_cg_opts *oo, const std::string &pk, int ipp) const {
+    return rowptr == A->rowptr && colidx == A->colidx && vals == A->vals &&
+           vals_lo == A->vals_lo && nrows == A->nrows && nnz == A->nnz && bdim == A->bdim &&
+           prec == A->precision && ip == ipp && b == bb && x == xx && rows.size() == nb + 1u &&
+           std::memcmp(rows.data(), hrows, sizeof(uint32_t) * (nb + 1)) == 0 &&
+           std::memcmp(&o, oo, sizeof(smh_cg_opts)) == 0 && pro_key == pk;
+  }
+};
+
 template <int IP, int RP, int BD>
-int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t nb,
-            const uint32_t *hrows, const double *b, double *x, const smh_cg_opts *o,
-            smh_cg_batch_stats *st, double *seconds) {
+int bcg_build(smh_ctx *ctx, BPlan &P, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t nb,
+              const uint32_t *hrows, const double *b, double *x, const smh_cg_opts *o,
+              const std::function<int(hipStream_t)> &prologue) {
   using VT = typename Val<IP>::T;
   const uint32_t nrows = Am->nrows;
   const uint64_t n = (uint64_t)nrows * BD;
   const int pc = o->precond;
   const int check = o->check_every > 0 ? o->check_every : 16;
-  auto T0 = std::chrono::steady_clock::now();
-  std::vector<void *> bufs;
-  int rc = SMH_OK;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  P.check = check;
+  hipStream_t s = ctx->own_stream;
   auto alloc = [&](void **pp, size_t bytes) {
     const int r = dmalloc(ctx, pp, bytes);
-    if (!r) bufs.push_back(*pp);
+    if (!r) P.bufs.push_back(*pp);
     return r;
   };
   void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *dlo = nullptr;
   uint32_t *rows = nullptr, *mesh = nullptr, *ctab = nullptr;
   double *prz = nullptr, *prr = nullptr, *sbuf = nullptr, *cp1 = nullptr, *cp2 = nullptr;
   int *ibuf = nullptr, *err = nullptr;
-  BDev S{};
-  std::vector<int> hstate(nb), hiter(nb);
-  std::vector<double> hrr(nb), hbb(nb);
+  unsigned long long *counter = nullptr;
+  int rc = SMH_OK;
   // chunk tables: [crow (nchunks+1) | cmesh (nchunks) | mchunk (nb+1)]
   std::vector<uint32_t> hcrow, hcmesh, hmchunk(nb + 1);
   for (uint32_t m = 0; m < nb; ++m) {
@@ -1621,7 +1777,6 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
   std::vector<uint32_t> htab(hcrow);
   htab.insert(htab.end(), hcmesh.begin(), hcmesh.end());
   htab.insert(htab.end(), hmchunk.begin(), hmchunk.end());
-  BChunks CH{};
   if ((rc = alloc(&r, sizeof(VT) * n)) || (rc = alloc(&z, sizeof(VT) * n)) ||
       (rc = alloc(&p, sizeof(VT) * n)) || (rc = alloc(&q, sizeof(VT) * n)) ||
       (rc = alloc((void **)&rows, sizeof(uint32_t) * (nb + 1))) ||
@@ -1630,60 +1785,73 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
       (rc = alloc((void **)&prr, sizeof(double) * (nrows ? nrows : 1))) ||
       (rc = alloc((void **)&sbuf, sizeof(double) * 8 * nb)) ||
       (rc = alloc((void **)&ibuf, sizeof(int) * 3 * nb)) || (rc = alloc((void **)&err, 64)) ||
+      (rc = alloc((void **)&counter, 64)) ||
       (rc = alloc((void **)&ctab, sizeof(uint32_t) * htab.size())) ||
       (rc = alloc((void **)&cp1, sizeof(double) * (nch ? nch : 1))) ||
       (rc = alloc((void **)&cp2, sizeof(double) * (nch ? nch : 1))))
-    goto out;
-  CH = BChunks{ctab, ctab + nch + 1, ctab + 2 * nch + 1, nch, nb};
+    return rc;
+  const BChunks CH{ctab, ctab + nch + 1, ctab + 2 * nch + 1, nch, nb};
   if (pc != SMH_PRECOND_NONE && (rc = alloc(&dinv, sizeof(VT) * (uint64_t)nrows * BD * BD)))
-    goto out;
+    return rc;
   if (o->mixed && Am->precision == SMH_F32X2 && Am->vals_lo &&
       (rc = alloc(&dlo, sizeof(float) * (uint64_t)nrows * BD * BD)))
-    goto out;
-  S = BDev{sbuf, sbuf + nb, sbuf + 2 * nb, sbuf + 3 * nb, sbuf + 4 * nb, sbuf + 5 * nb,
-           sbuf + 6 * nb, sbuf + 7 * nb, ibuf, ibuf + nb, ibuf + 2 * nb};
-  {
-    // the iteration is captured as a graph, which the legacy default stream
-    // does not allow: run on the context's own stream, ordered after the
-    // caller's work (this solver synchronises with the host anyway)
-    hipStream_t s = ctx->own_stream;
-    (void)hipStreamSynchronize(ctx->stream);
-    rc = SMH_OK;
-    if (hipMemcpyAsync(rows, hrows, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, s) !=
-            hipSuccess ||
-        hipMemcpyAsync(ctab, htab.data(), sizeof(uint32_t) * htab.size(), hipMemcpyHostToDevice,
-                       s) != hipSuccess ||
-        hipMemsetAsync(err, 0, sizeof(int), s) != hipSuccess) {
-      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: upload failed");
-      goto out;
-    }
-    hipLaunchKernelGGL(k_bcg_mesh_rows, dim3(nb), dim3(256), 0, s, nb, (const uint32_t *)rows,
-                       mesh);
-    hipLaunchKernelGGL(k_bcg_check, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows,
-                       Am->rowptr, Am->colidx, (const uint32_t *)mesh, (const uint32_t *)rows,
-                       err);
-    int herr = 0;
-    if (hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: check failed");
-      goto out;
-    }
-    if (herr) {
-      rc = fail(ctx, SMH_ERR_INVALID,
-                "smh_cg_solve_batched: a block couples two systems (the matrix is not "
-                "block-diagonal by the given row ranges)");
-      goto out;
-    }
-    // the autotuned SpMV grid, then the kernels' shapes
-    unsigned g = tuned_grid(ctx, s, Am, plan, p, q);
-    unsigned g0 = 0;
-    TileDev T = tile_dev(Am, plan, &g0);
-    if (g) {
-      T.tiles_per_wg = (uint32_t)((plan->ntiles + g - 1) / g);
-    } else {
-      g = g0;
-    }
-    const unsigned gs = grid_for(nb, 256);  // a thread per system
+    return rc;
+  const BDev S{sbuf, sbuf + nb, sbuf + 2 * nb, sbuf + 3 * nb, sbuf + 4 * nb, sbuf + 5 * nb,
+               sbuf + 6 * nb, sbuf + 7 * nb, ibuf, ibuf + nb, ibuf + 2 * nb};
+  P.S = S;
+  SMH_HIP(ctx, hipMemcpyAsync(rows, hrows, sizeof(uint32_t) * (nb + 1), hipMemcpyHostToDevice, s));
+  SMH_HIP(ctx, hipMemcpyAsync(ctab, htab.data(), sizeof(uint32_t) * htab.size(),
+                              hipMemcpyHostToDevice, s));
+  SMH_HIP(ctx, hipMemsetAsync(err, 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_bcg_mesh_rows, dim3(nb), dim3(256), 0, s, nb, (const uint32_t *)rows, mesh);
+  hipLaunchKernelGGL(k_bcg_check, dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows, Am->rowptr,
+                     Am->colidx, (const uint32_t *)mesh, (const uint32_t *)rows, err);
+  int herr = 0;
+  SMH_HIP(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  SMH_HIP(ctx, hipStreamSynchronize(s));
+  SMH_CHECK(ctx, !herr, SMH_ERR_INVALID,
+            "smh_cg_solve_batched: a block couples two systems (the matrix is not "
+            "block-diagonal by the given row ranges)");
+  // the autotuned SpMV (tile shape and grid), then the other kernels' shapes
+  bool cached = false;
+  for (const auto &t : ctx->tuned)
+    cached |= t.rowptr == Am->rowptr && t.nrows == Am->nrows && t.nnz == Am->nnz &&
+              t.bdim == Am->bdim && t.prec == IP;
+  if (dlo && !cached)  // the tuning runs the kernel with its epilogue: finite row sums
+    SMH_HIP(ctx, hipMemsetAsync(dlo, 0, sizeof(float) * (uint64_t)nrows * BD * BD, s));
+  const BTune tune = tuned_bcg<IP, BD>(ctx, s, Am, plan, p, q, (const float *)dlo, prz);
+  unsigned g = 0, gsp = tune.grid;
+  const TileDev T = tile_dev(Am, plan, &g);  // the residual kernel's tiles (default shape)
+  unsigned gdef = 0;
+  TileDev Ts = tile_dev(Am, &tune.plan, &gdef);
+  if (gsp) Ts.tiles_per_wg = (uint32_t)((tune.plan.ntiles + gsp - 1) / gsp);
+  else gsp = gdef;
+  // the SpMV streams the tiles of systems still iterating only
+  Ts.row_sys = mesh;
+  Ts.sys_state = S.state;
+  const unsigned gs = grid_for(nb, 256);  // a thread per system
+  EpiResRow<VT, BD> epi{b, (VT *)r, (VT *)z, dinv, pc, 0, (const uint32_t *)mesh, S.state,
+                        prz, prr};
+  EpiResRow<VT, BD> epi_all = epi;
+  epi_all.all = 1;
+  // RU + S2 + RR (all: every system, else only those being replaced)
+  auto replace = [&](int all, int mode) {
+    hipLaunchKernelGGL((k_bcg_residual<RP, BD, VT>), dim3(g), dim3(tile_threads_host(BD)), 0, s,
+                       T, (const double *)x, all ? epi_all : epi);
+    hipLaunchKernelGGL(k_bcg_csum, dim3(nch), dim3(256), 0, s, CH, (const double *)prz,
+                       (const double *)prr, (const int *)S.state, all ? 0 : 1, cp1, cp2);
+    hipLaunchKernelGGL(k_bcg_rres, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1,
+                       (const double *)cp2, S, mode, pc != SMH_PRECOND_NONE ? 1 : 0);
+  };
+  auto pupd = [&]() {
+    hipLaunchKernelGGL((k_bcg_pupd<VT, BD>), dim3(nch), dim3(256), 0, s, CH, (VT *)p,
+                       (const VT *)z, S);
+  };
+  // the set-up, after the caller's prologue (e.g. the assembly):
+  // preconditioner, low-part row sums, b'b, the states, then r = b - A x,
+  // z = M r, p = z for every system; the replay counter restarts at zero
+  auto setup = [&]() {
+    hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, counter, 8, 0ull);
     if (pc != SMH_PRECOND_NONE)
       hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows,
                          Am->rowptr, Am->colidx, Am->vals, Am->vals_lo, BD, pc, (VT *)dinv);
@@ -1692,115 +1860,157 @@ int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t
                          0, s, nrows, BD * BD, Am->rowptr, (const float *)Am->vals_lo,
                          (float *)dlo);
     hipLaunchKernelGGL(k_bcg_bb<BD>, dim3(nch), dim3(256), 0, s, CH, b, cp1);
-    hipLaunchKernelGGL(k_bcg_init, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1, S,
-                       o->tol, o->rel);
-    EpiResRow<VT, BD> epi{b, (VT *)r, (VT *)z, dinv, pc, 0, (const uint32_t *)mesh, S.state,
-                          prz, prr};
-    EpiResRow<VT, BD> epi_all = epi;
-    epi_all.all = 1;
-    // RU + S2 + RR (all: every system, else only those being replaced)
-    auto replace = [&](int all, int mode) {
-      hipLaunchKernelGGL((k_bcg_residual<RP, BD, VT>), dim3(g), dim3(tile_threads_host(BD)), 0,
-                         s, T, (const double *)x, all ? epi_all : epi);
-      hipLaunchKernelGGL(k_bcg_csum, dim3(nch), dim3(256), 0, s, CH, (const double *)prz,
-                         (const double *)prr, (const int *)S.state, all ? 0 : 1, cp1, cp2);
-      hipLaunchKernelGGL(k_bcg_rres, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1,
-                         (const double *)cp2, S, mode, pc != SMH_PRECOND_NONE ? 1 : 0);
-    };
-    auto pupd = [&]() {
-      hipLaunchKernelGGL((k_bcg_pupd<VT, BD>), dim3(nch), dim3(256), 0, s, CH, (VT *)p,
-                         (const VT *)z, S);
-    };
-    // r = b - A x, z = M r, p = z for every system
+    hipLaunchKernelGGL(k_bcg_init, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1, S, o->tol,
+                       o->rel);
     replace(1, 1);
     pupd();
-    // one captured chunk of `check` iterations
-    auto iterate = [&]() {
-      for (int k = 0; k < check; ++k) {
-        hipLaunchKernelGGL((k_bcg_spmv<IP, BD>), dim3(g), dim3(tile_threads_host(BD)), 0, s, T,
-                           (const VT *)p, (VT *)q, (const float *)dlo, prz);
-        hipLaunchKernelGGL(k_bcg_csum, dim3(nch), dim3(256), 0, s, CH, (const double *)prz,
-                           (const double *)nullptr, (const int *)S.state, 0, cp1, cp2);
-        hipLaunchKernelGGL(k_bcg_alpha, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1, S);
-        hipLaunchKernelGGL((k_bcg_update<VT, BD>), dim3(nch), dim3(256), 0, s, CH, pc,
-                           (const void *)dinv, x, (VT *)r, (VT *)z, (const VT *)p, (const VT *)q,
-                           S, cp1, cp2);
-        hipLaunchKernelGGL(k_bcg_beta, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1,
-                           (const double *)cp2, S, o->maxiters, o->mixed,
-                           o->reliable_delta * o->reliable_delta, o->replace_every,
-                           pc != SMH_PRECOND_NONE ? 1 : 0);
-        replace(0, 0);
-        pupd();
-      }
-    };
-    if (hipGetLastError() != hipSuccess) {
-      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: launch failed");
-      goto out;
+  };
+  auto signal = [&]() {
+    hipLaunchKernelGGL(k_bcg_signal, dim3(1), dim3(256), 0, s, (const int *)S.state, nb,
+                       counter, ctx->signal_dev);
+  };
+  // one chunk of `check` iterations
+  auto iterate = [&]() {
+    for (int k = 0; k < check; ++k) {
+      launch_bcg_spmv<IP, BD>(tune.shape, gsp, s, Ts, (const VT *)p, (VT *)q, (const float *)dlo,
+                              prz);
+      hipLaunchKernelGGL(k_bcg_csum, dim3(nch), dim3(256), 0, s, CH, (const double *)prz,
+                         (const double *)nullptr, (const int *)S.state, 0, cp1, cp2);
+      hipLaunchKernelGGL(k_bcg_alpha, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1, S);
+      hipLaunchKernelGGL((k_bcg_update<VT, BD>), dim3(nch), dim3(256), 0, s, CH, pc,
+                         (const void *)dinv, x, (VT *)r, (VT *)z, (const VT *)p, (const VT *)q, S,
+                         cp1, cp2);
+      hipLaunchKernelGGL(k_bcg_beta, dim3(gs), dim3(256), 0, s, CH, (const double *)cp1,
+                         (const double *)cp2, S, o->maxiters, o->mixed,
+                         o->reliable_delta * o->reliable_delta, o->replace_every,
+                         pc != SMH_PRECOND_NONE ? 1 : 0);
+      replace(0, 0);
+      pupd();
     }
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: stream capture failed");
-      goto out;
+  };
+  SMH_HIP(ctx, hipGetLastError());
+  for (int k = 0; k < 3; ++k) {
+    SMH_HIP(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int prc = SMH_OK;
+    if (k == 0) {
+      if (prologue) prc = prologue(s);
+      setup();
     }
-    iterate();
-    if (hipStreamEndCapture(s, &graph) != hipSuccess ||
-        hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    if (k < 2) {
+      iterate();
+      signal();
+    } else {
+      replace(1, 2);  // certify every system against its FP64 residual
+    }
+    const hipError_t ce = hipStreamEndCapture(s, &P.g[k]);
+    if (prc) return prc;
+    if (ce != hipSuccess || hipGraphInstantiate(&P.ge[k], P.g[k], nullptr, nullptr, 0) !=
+                                hipSuccess) {
       (void)hipGetLastError();
-      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: graph instantiation failed");
-      goto out;
+      return fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: graph instantiation failed");
     }
-    // replay until no system is active (every system leaves the active state
-    // by maxiters at the latest)
-    for (;;) {
-      if (hipMemcpyAsync(hstate.data(), S.state, sizeof(int) * nb, hipMemcpyDeviceToHost, s) !=
-              hipSuccess ||
-          hipStreamSynchronize(s) != hipSuccess) {
-        rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: state readback failed");
-        goto out;
-      }
-      bool active = false;
-      for (uint32_t m = 0; m < nb; ++m) active |= hstate[m] == kBActive;
-      if (!active) break;
-      if (hipGraphLaunch(gexec, s) != hipSuccess) {
-        rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: graph launch failed");
-        goto out;
-      }
-    }
-    // certify every system against its FP64 residual
-    replace(1, 2);
-    if (hipMemcpyAsync(hstate.data(), S.state, sizeof(int) * nb, hipMemcpyDeviceToHost, s) !=
-            hipSuccess ||
-        hipMemcpyAsync(hiter.data(), S.iter, sizeof(int) * nb, hipMemcpyDeviceToHost, s) !=
-            hipSuccess ||
-        hipMemcpyAsync(hrr.data(), S.rr_true, sizeof(double) * nb, hipMemcpyDeviceToHost, s) !=
-            hipSuccess ||
-        hipMemcpyAsync(hbb.data(), S.bb, sizeof(double) * nb, hipMemcpyDeviceToHost, s) !=
-            hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: result readback failed");
-      goto out;
-    }
-    std::vector<double> hthr(nb);
-    if (hipMemcpy(hthr.data(), S.thr, sizeof(double) * nb, hipMemcpyDeviceToHost) !=
-        hipSuccess) {
-      rc = fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: result readback failed");
-      goto out;
-    }
-    if (st)
-      for (uint32_t m = 0; m < nb; ++m) {
-        st[m].iters = hiter[m];
-        st[m].true_rel = hbb[m] > 0 ? std::sqrt(hrr[m] / hbb[m]) : std::sqrt(hrr[m]);
-        int status = SMH_OK;
-        if (hstate[m] == kBBreakdown) status = SMH_ERR_BREAKDOWN;
-        else if (!(hrr[m] <= hthr[m] * (1.0 + 1e-12))) status = SMH_ERR_NOT_CONVERGED;
-        st[m].status = status;
-      }
   }
-out:
-  if (gexec) (void)hipGraphExecDestroy(gexec);
-  if (graph) (void)hipGraphDestroy(graph);
-  (void)hipStreamSynchronize(ctx->own_stream);
+  return SMH_OK;
+}
+
+// Replays go out two ahead of the host, which watches the signal in pinned
+// memory (no copies, no stream synchronisation) and stops once every system
+// has left the iteration (by maxiters at the latest); replays queued past
+// that exit at once.
+int bcg_replay(smh_ctx *ctx, BPlan &P, uint32_t nb, const smh_cg_opts *o,
+               smh_cg_batch_stats *st) {
+  hipStream_t s = ctx->own_stream;
+  const int check = P.check;
+  volatile unsigned long long *sig = ctx->signal_host;
+  const uint64_t max_replays = (uint64_t)(o->maxiters + check - 1) / check + 2;
+  sig[1] = ~0ull;  // this solve's replays count from zero
+  sig[0] = 0;
+  uint64_t launched = 0;
+  auto launch = [&](hipGraphExec_t ge) {
+    if (hipGraphLaunch(ge, s) != hipSuccess) return false;
+    ++launched;
+    return true;
+  };
+  bool ok = launch(P.ge[0]) && (max_replays < 2 || launch(P.ge[1]));
+  while (ok) {
+    // wait until all but the newest replay have finished
+    uint64_t spins = 0;
+    while (sig[0] + 1 < launched) {
+      if (++spins % 4096) continue;
+      const hipError_t q = hipStreamQuery(s);
+      if (q == hipSuccess && sig[0] + 1 < launched)
+        return fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: lost the replay signal");
+      if (q != hipSuccess && q != hipErrorNotReady)
+        return fail(ctx, SMH_ERR_HIP, "smh_cg_solve_batched: replay failed: %s",
+                    hipGetErrorString(q));
+    }
+    if (sig[1] == 0 || launched >= max_replays) break;
+    ok = launch(P.ge[1]);
+  }
+  SMH_CHECK(ctx, ok, SMH_ERR_HIP, "smh_cg_solve_batched: graph launch failed");
+  SMH_CHECK(ctx, hipGraphLaunch(P.ge[2], s) == hipSuccess, SMH_ERR_HIP,
+            "smh_cg_solve_batched: graph launch failed");
+  std::vector<int> hstate(nb), hiter(nb);
+  std::vector<double> hrr(nb), hbb(nb), hthr(nb);
+  SMH_HIP(ctx, hipMemcpyAsync(hstate.data(), P.S.state, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+  SMH_HIP(ctx, hipMemcpyAsync(hiter.data(), P.S.iter, sizeof(int) * nb, hipMemcpyDeviceToHost, s));
+  SMH_HIP(ctx, hipMemcpyAsync(hrr.data(), P.S.rr_true, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                              s));
+  SMH_HIP(ctx, hipMemcpyAsync(hbb.data(), P.S.bb, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  SMH_HIP(ctx, hipMemcpyAsync(hthr.data(), P.S.thr, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+  SMH_HIP(ctx, hipStreamSynchronize(s));
+  if (st)
+    for (uint32_t m = 0; m < nb; ++m) {
+      st[m].iters = hiter[m];
+      st[m].true_rel = hbb[m] > 0 ? std::sqrt(hrr[m] / hbb[m]) : std::sqrt(hrr[m]);
+      int status = SMH_OK;
+      if (hstate[m] == kBBreakdown) status = SMH_ERR_BREAKDOWN;
+      else if (!(hrr[m] <= hthr[m] * (1.0 + 1e-12))) status = SMH_ERR_NOT_CONVERGED;
+      st[m].status = status;
+    }
+  return SMH_OK;
+}
+
+template <int IP, int RP, int BD>
+int bcg_run(smh_ctx *ctx, const smh_bsr *Am, const smh_tile_plan *plan, uint32_t nb,
+            const uint32_t *hrows, const double *b, double *x, const smh_cg_opts *o,
+            smh_cg_batch_stats *st, double *seconds,
+            const std::function<int(hipStream_t)> &prologue, const std::string &pro_key) {
+  auto T0 = std::chrono::steady_clock::now();
+  // the graph runs on the context's own stream (capture is not allowed on the
+  // legacy default stream), ordered after the caller's work
   (void)hipStreamSynchronize(ctx->stream);
-  for (void *b : bufs) dfree(b);
+  BPlan *P = nullptr;
+  for (auto &c : ctx->cached) {
+    BPlan *bp = dynamic_cast<BPlan *>(c.get());
+    if (bp && bp->same(Am, b, x, nb, hrows, o, pro_key, IP)) P = bp;
+  }
+  int rc = SMH_OK;
+  if (!P) {
+    auto np = std::make_unique<BPlan>();
+    np->rowptr = Am->rowptr;
+    np->colidx = Am->colidx;
+    np->vals = Am->vals;
+    np->vals_lo = Am->vals_lo;
+    np->nrows = Am->nrows;
+    np->nnz = Am->nnz;
+    np->bdim = Am->bdim;
+    np->prec = Am->precision;
+    np->ip = IP;
+    np->b = b;
+    np->x = x;
+    np->rows.assign(hrows, hrows + nb + 1);
+    np->o = *o;
+    np->pro_key = pro_key;
+    rc = bcg_build<IP, RP, BD>(ctx, *np, Am, plan, nb, hrows, b, x, o, prologue);
+    (void)hipStreamSynchronize(ctx->own_stream);
+    if (rc == SMH_OK) {
+      P = np.get();
+      ctx->cached.push_back(std::move(np));
+    }
+  }
+  if (P) rc = bcg_replay(ctx, *P, nb, o, st);
+  (void)hipStreamSynchronize(ctx->own_stream);
   if (seconds)
     *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - T0).count();
   return rc;
@@ -2104,11 +2314,11 @@ extern "C" int smh_cg_solve_noalloc(smh_ctx *ctx, const smh_bsr *A, const double
   return cg_dispatch(ctx, A, &plan, b, x, o, st, &ar);
 }
 
-extern "C" int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch,
-                                    const uint32_t *rows, const double *b, double *x,
-                                    const smh_cg_opts *o, smh_cg_batch_stats *stats,
-                                    double *seconds) {
-  if (!ctx) return SMH_ERR_INVALID;
+namespace {
+int cg_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch, const uint32_t *rows,
+               const double *b, double *x, const smh_cg_opts *o, smh_cg_batch_stats *stats,
+               double *seconds, const std::function<int(hipStream_t)> &prologue,
+               const std::string &pro_key) {
   SMH_CHECK(ctx, A && rows && b && x && o && A->rowptr && nbatch > 0, SMH_ERR_INVALID,
             "smh_cg_solve_batched: null argument");
   SMH_CHECK(ctx, A->bdim == 1 || A->bdim == 3, SMH_ERR_UNSUPPORTED,
@@ -2139,14 +2349,56 @@ extern "C" int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nba
             "smh_cg_solve_batched: a row holds %u blocks (at most %u supported)", plan.maxrow,
             tile_blocks_host(bd) / 2);
 #define SMH_BCG(IP, RP)                                                                      \
-  (bd == 3 ? bcg_run<IP, RP, 3>(ctx, A, &plan, nbatch, rows, b, x, o, stats, seconds)        \
-           : bcg_run<IP, RP, 1>(ctx, A, &plan, nbatch, rows, b, x, o, stats, seconds))
+  (bd == 3 ? bcg_run<IP, RP, 3>(ctx, A, &plan, nbatch, rows, b, x, o, stats, seconds,   \
+                                 prologue, pro_key)                                          \
+           : bcg_run<IP, RP, 1>(ctx, A, &plan, nbatch, rows, b, x, o, stats, seconds,   \
+                                 prologue, pro_key))
   if (o->mixed) rc = SMH_BCG(SMH_F32, SMH_F32X2);
   else if (A->precision == SMH_F64) rc = SMH_BCG(SMH_F64, SMH_F64);
   else if (A->precision == SMH_F32X2) rc = SMH_BCG(SMH_F32X2, SMH_F32X2);
   else rc = SMH_BCG(kF32D, kF32D);
 #undef SMH_BCG
   return rc;
+}
+}  // namespace
+
+extern "C" int smh_cg_solve_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch,
+                                    const uint32_t *rows, const double *b, double *x,
+                                    const smh_cg_opts *o, smh_cg_batch_stats *stats,
+                                    double *seconds) {
+  if (!ctx) return SMH_ERR_INVALID;
+  return cg_batched(ctx, A, nbatch, rows, b, x, o, stats, seconds, nullptr, std::string());
+}
+
+extern "C" int smh_springs_solve_batched(smh_ctx *ctx, const smh_index *idx,
+                                         const smh_springs_params *p, const smh_springs_set *set,
+                                         void *vals, void *vals_lo, double *rhs, uint32_t nbatch,
+                                         const uint32_t *rows, double *x, const smh_cg_opts *o,
+                                         smh_cg_batch_stats *stats, double *seconds) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, idx && p && set && vals && rhs && o, SMH_ERR_INVALID,
+            "smh_springs_solve_batched: null argument");
+  SMH_CHECK(ctx, o->mixed == 0 || vals_lo, SMH_ERR_INVALID,
+            "smh_springs_solve_batched: mixed precision needs vals_lo (SMH_F32X2 values)");
+  const int prec = o->mixed ? SMH_F32X2 : SMH_F64;
+  const smh_bsr A{idx->nrows, 3, idx->nnz, idx->rowptr, idx->colidx, prec, vals,
+                  o->mixed ? vals_lo : nullptr};
+  // the assembly is the first work of the first captured graph
+  auto prologue = [&](hipStream_t s) {
+    hipStream_t caller = ctx->stream;
+    ctx->stream = s;
+    const int r = smh_assemble_springs(ctx, idx, p, set, prec, vals, vals_lo, rhs);
+    ctx->stream = caller;
+    return r;
+  };
+  // the graphs hold the assembly's arguments by value: they are part of the key
+  std::string key((const char *)p, sizeof(*p));
+  key.append((const char *)set, sizeof(*set));
+  key.append((const char *)&vals, sizeof(vals));
+  key.append((const char *)&vals_lo, sizeof(vals_lo));
+  key.append((const char *)&rhs, sizeof(rhs));
+  key.append((const char *)&idx, sizeof(idx));
+  return cg_batched(ctx, &A, nbatch, rows, rhs, x, o, stats, seconds, prologue, key);
 }
 
 // ---- scalar 27-point stencil storage: conversion, product and CG -------------------------

@@ -211,6 +211,21 @@ extern "C" int smh_ctx_create(int device, smh_ctx **out) {
     delete c;
     return SMH_ERR_NOMEM;
   }
+  void *sig = nullptr, *sig_dev = nullptr;
+  if (hipHostMalloc(&sig, 64, hipHostMallocCoherent | hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer(&sig_dev, sig, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    if (sig) hipHostFree(sig);
+    hipFree(c->red);
+    hipFree(c->red_count);
+    hipHostFree(c->host_pinned);
+    hipStreamDestroy(c->own_stream);
+    delete c;
+    return SMH_ERR_NOMEM;
+  }
+  std::memset(sig, 0, 64);
+  c->signal_host = (volatile unsigned long long *)sig;
+  c->signal_dev = (unsigned long long *)sig_dev;
   if (hipMemset(c->red_count, 0, sizeof(unsigned) * kRedCounters) != hipSuccess) {
     delete c;
     return SMH_ERR_HIP;
@@ -223,10 +238,13 @@ extern "C" int smh_ctx_destroy(smh_ctx *ctx) {
   if (!ctx) return SMH_ERR_INVALID;
   hipSetDevice(ctx->device);
   hipStreamSynchronize(ctx->stream);
+  if (ctx->own_stream) hipStreamSynchronize(ctx->own_stream);
+  ctx->cached.clear();  // cached solver workspaces and graphs
   for (auto &p : ctx->plans) dfree(p.tile_row);
   if (ctx->red) hipFree(ctx->red);
   if (ctx->red_count) hipFree(ctx->red_count);
   if (ctx->host_pinned) hipHostFree(ctx->host_pinned);
+  if (ctx->signal_host) hipHostFree((void *)ctx->signal_host);
   if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return SMH_OK;

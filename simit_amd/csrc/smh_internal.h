@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <string>
 #include <utility>
 #include <vector>
@@ -25,6 +26,14 @@ struct smh_tile_plan {
   uint32_t *tile_row = nullptr;  // device [ntiles+1]
 };
 
+// Something a solver builds once per pattern and keeps in the context
+// (device workspace, captured graphs); dropped by smh_bsr_forget(rowptr) and
+// with the context.
+struct smh_cached {
+  const uint32_t *rowptr = nullptr;
+  virtual ~smh_cached() = default;
+};
+
 struct smh_ctx {
   int device = 0;
   int num_cu = 256;  // compute units of the device (persistent grids)
@@ -32,17 +41,24 @@ struct smh_ctx {
   hipStream_t stream = nullptr;
   std::string last_error;
   std::vector<smh_tile_plan> plans;
+  std::vector<std::unique_ptr<smh_cached>> cached;
   struct Tuned {
     const uint32_t *rowptr;
     uint32_t nrows, bdim;
     uint64_t nnz;
     unsigned grid;
+    int shape;  // batched SpMV tile shape
+    int prec;   // the inner precision it was tuned for
   };
-  std::vector<Tuned> tuned;  // autotuned SpMV grids by pattern
+  std::vector<Tuned> tuned;  // autotuned batched SpMV shapes and grids by pattern
   // small device scratch for reductions: [0..kRedSlots) partials + counters
   double *red = nullptr;
   unsigned *red_count = nullptr;
   double *host_pinned = nullptr;  // pinned host mirror for scalar readbacks
+  // pinned host words the batched solver's replays write their progress to
+  // (host pointer and its device mapping)
+  volatile unsigned long long *signal_host = nullptr;
+  unsigned long long *signal_dev = nullptr;
   // this device's copy of the library's static scratch array (the
   // allocation-free solver's workspace), looked up once per device
   void *scratch = nullptr;

@@ -53,13 +53,14 @@ namespace smh {
 #define SMH_STREAM_LOAD(p) __builtin_nontemporal_load(p)
 #endif
 
-template <int BD>
 #ifndef SMH_TILE_THREADS3
 #define SMH_TILE_THREADS3 64
 #endif
-struct TileShape {
-  static constexpr int kThreads = BD == 3 ? SMH_TILE_THREADS3 : 256;
-  static constexpr int kRounds = BD == 3 ? SMH_TILE_ROUNDS3 : SMH_TILE_ROUNDS1;  // per lane
+// A tile shape: ROUNDS rounds per lane, THREADS threads per workgroup.
+template <int BD, int ROUNDS, int THREADS>
+struct TileShapeR {
+  static constexpr int kThreads = THREADS;
+  static constexpr int kRounds = ROUNDS;                   // per lane
   static constexpr int kWaves = kThreads / 64;
   static constexpr int kVec = BD == 1 ? SMH_TILE_VEC1 : 1;  // consecutive blocks per lane
   static constexpr int kPerWave = 64 / BD * kVec;         // blocks per wave and round
@@ -68,6 +69,10 @@ struct TileShape {
   static constexpr bool kBPL = false;                     // one block per lane
   static_assert(kThreads % 64 == 0, "tiles are made of whole wavefronts");
 };
+// the default shape of a block size
+template <int BD>
+struct TileShape : TileShapeR<BD, BD == 3 ? SMH_TILE_ROUNDS3 : SMH_TILE_ROUNDS1,
+                              BD == 3 ? SMH_TILE_THREADS3 : 256> {};
 // FP8 3x3 blocks (the learned correction) put a whole block on one lane: its
 // three 4-byte row words are one 12-byte load and 64 blocks go per wave and
 // round, instead of one word per lane and 21 blocks.
@@ -307,6 +312,11 @@ struct TileDev {
   uint32_t ntiles;
   uint32_t tiles_per_wg;
   uint32_t interleave;  // 1: workgroups of an XCD stride over its tiles (see wg_tiles)
+  // batched systems (optional): the system of each row and the systems'
+  // states; a tile whose rows all belong to systems that are no longer
+  // iterating (state != 0) is streamed as an empty tile
+  const uint32_t *row_sys = nullptr;
+  const int *sys_state = nullptr;
 };
 
 template <int PREC, int BD, class SH = TileShapeP<PREC, BD>>
@@ -358,7 +368,8 @@ inline TileDev tile_dev(const smh_bsr *A, const smh_tile_plan *p, unsigned *grid
   if (g < 1) g = 1;
   *grid = (unsigned)g;
   TileDev d{A->nrows, A->nnz, A->rowptr, A->colidx, A->vals, A->vals_lo, p->tile_row, p->ntiles,
-            (uint32_t)((p->ntiles + g - 1) / g), (interleave && g >= 8) ? 1u : 0u};
+            (uint32_t)((p->ntiles + g - 1) / g), (interleave && g >= 8) ? 1u : 0u,
+            nullptr, nullptr};
   return d;
 }
 
@@ -395,10 +406,10 @@ __device__ __forceinline__ void wg_tiles(const TileDev &A, uint32_t &k0, uint32_
 // s are loaded once, by lane s, and handed to the block's lanes by a
 // cross-lane shuffle: the contiguous gathering lanes keep the address unit's
 // work per block at one column and one x request instead of BD of each.
-template <int PREC, int BD>
+template <int PREC, int BD, class SHP = TileShape<BD>>
 struct TileRegs {
   using S = typename Val<PREC>::S;
-  static constexpr int kRounds = TileShape<BD>::kRounds;
+  static constexpr int kRounds = SHP::kRounds;
   static constexpr int kV = Val<PREC>::kFP8 ? 1 : BD;  // FP8: one word per block row
   S v[kRounds][kV];
   float vl[Val<PREC>::kLo ? kRounds : 1][BD];
@@ -406,7 +417,7 @@ struct TileRegs {
 
   __device__ __forceinline__ void load(const TileDev &A, uint32_t base, uint32_t end, int w,
                                        int lane) {
-    using SH = TileShape<BD>;
+    using SH = SHP;
     constexpr int BB = BD * BD;
     const int s = w * SH::kPerWave + lane / BD, bi = lane % BD;
     const bool ok = lane < SH::kPerWave * BD;
@@ -470,10 +481,10 @@ struct TileRegs {
 // Registers of one lane for 1x1 tiles: kVec consecutive entries per round,
 // values and columns loaded as vectors (a 4-byte value per request would
 // leave the address unit, not HBM, as the limit).
-template <int PREC>
+template <int PREC, class SHP = TileShape<1>>
 struct TileRegsV {
   using S = typename Val<PREC>::S;
-  using SH = TileShape<1>;
+  using SH = SHP;
   static constexpr int kRounds = SH::kRounds, kVec = SH::kVec;
   S v[kRounds][kVec];
   float vl[Val<PREC>::kLo ? kRounds : 1][kVec];
@@ -537,6 +548,15 @@ __device__ __forceinline__ void tile_bounds(const TileDev &A, uint32_t k, uint32
   slot[1] = R1;
   slot[2] = A.rowptr[R0];
   slot[3] = A.rowptr[R1];
+  if (A.row_sys && R1 > R0) {
+    bool active = false;
+    for (uint32_t m = A.row_sys[R0]; m <= A.row_sys[R1 - 1] && !active; ++m)
+      active = A.sys_state[m] == 0;
+    if (!active) {  // no rows, no blocks: the tile costs its loop turn only
+      slot[1] = slot[0];
+      slot[3] = slot[2];
+    }
+  }
 }
 
 // Stream tiles [k0, k1): rows handed to epi(row, s[BD]) by one thread per
@@ -557,8 +577,8 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
   }
   __syncthreads();
   using Regs = typename std::conditional<
-      BD == 1, TileRegsV<PREC>,
-      typename std::conditional<SH::kBPL, TileRegsB<PREC>, TileRegs<PREC, BD>>::type>::type;
+      BD == 1, TileRegsV<PREC, SH>,
+      typename std::conditional<SH::kBPL, TileRegsB<PREC>, TileRegs<PREC, BD, SH>>::type>::type;
   Regs cur, nxt;
   cur.load(A, sm.bnd[0][2], sm.bnd[0][3], w, lane);
   for (uint32_t k = k0; k < k1; k += ks) {

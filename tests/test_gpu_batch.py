@@ -114,3 +114,65 @@ def test_batched_rejects_coupled_rows(ctx):
     bad[-1] -= 1
     with pytest.raises(S.SimitError):
         ctx.cg_batched(A, bad, sy["rhs"], torch.zeros_like(sy["rhs"]), ctx.cg_options())
+
+
+def test_springs_solve_batched_equals_assembly_then_batched_cg(ctx):
+    """The one-call step (assembly captured in the solver's first graph) gives
+    the same x and statistics, bit for bit, as the assembly followed by the
+    batched solve."""
+    import ctypes as Cc
+    batch = meshes.springs_batch(24, 5, 14, seed=4)
+    npts = batch["npoints"]
+    d = {k: to_dev(batch[k]) for k in ("x", "v", "m", "fixed", "k", "l0", "endpoints")}
+    idx = ctx.build_index(npts, [d["endpoints"]])
+    P = S.SpringsParams(0.01, 0.0, (Cc.c_double * 3)(0.0, 0.0, -9.8), 1.0, 1)
+    o = ctx.cg_options(maxiters=2000, tol=1e-8, rel=1, mixed=1, check_every=4,
+                       reliable_delta=0.01)
+    hi = torch.empty(idx.nnz * 9, dtype=torch.float32, device="cuda")
+    lo = torch.empty_like(hi)
+    rhs = torch.empty(3 * npts, dtype=torch.float64, device="cuda")
+    ctx.assemble_springs(idx, P, d, d, S.F32X2, hi, lo, rhs)
+    x1 = torch.zeros_like(rhs)
+    st1, _ = ctx.cg_batched(idx.bsr(3, S.F32X2, hi, lo), batch["rows"], rhs, x1, o)
+    hi2, lo2, rhs2 = torch.zeros_like(hi), torch.zeros_like(lo), torch.zeros_like(rhs)
+    x2 = torch.zeros_like(rhs)
+    st2, _ = ctx.springs_solve_batched(idx, P, d, d, hi2, lo2, rhs2, batch["rows"], x2, o)
+    assert torch.equal(hi, hi2) and torch.equal(lo, lo2) and torch.equal(rhs, rhs2)
+    assert torch.equal(x1, x2)
+    assert st1 == st2
+    assert all(s["status"] == 0 and s["true_rel"] <= 1e-8 for s in st2)
+
+
+def test_springs_solve_batched_reuses_its_graphs_with_new_values(ctx):
+    """A second call with the same buffers replays the cached graphs: new
+    stiffnesses written between the calls are assembled and solved (the
+    result equals a fresh context's), and an unchanged call repeats bit for
+    bit."""
+    import ctypes as Cc
+    batch = meshes.springs_batch(12, 5, 11, seed=8)
+    npts = batch["npoints"]
+    P = S.SpringsParams(0.01, 0.0, (Cc.c_double * 3)(0.0, 0.0, -9.8), 1.0, 1)
+    o = ctx.cg_options(maxiters=2000, tol=1e-8, rel=1, mixed=1, check_every=4)
+
+    def run(c, d, idx, hi, lo, rhs, x):
+        x.zero_()
+        st, _ = c.springs_solve_batched(idx, P, d, d, hi, lo, rhs, batch["rows"], x, o)
+        return [s["iters"] for s in st], x.clone()
+
+    d = {k: to_dev(batch[k]) for k in ("x", "v", "m", "fixed", "k", "l0", "endpoints")}
+    idx = ctx.build_index(npts, [d["endpoints"]])
+    bufs = [torch.empty(idx.nnz * 9, dtype=torch.float32, device="cuda") for _ in range(2)]
+    rhs = torch.empty(3 * npts, dtype=torch.float64, device="cuda")
+    x = torch.zeros_like(rhs)
+    it1, x1 = run(ctx, d, idx, bufs[0], bufs[1], rhs, x)
+    it1b, x1b = run(ctx, d, idx, bufs[0], bufs[1], rhs, x)
+    assert it1 == it1b and torch.equal(x1, x1b)
+    d["k"].mul_(3.0)  # stiffer springs, same buffers
+    it2, x2 = run(ctx, d, idx, bufs[0], bufs[1], rhs, x)
+    assert not torch.equal(x1, x2)
+    fresh = S.Context(0)
+    idx2 = fresh.build_index(npts, [d["endpoints"]])
+    b2 = [torch.empty_like(bufs[0]) for _ in range(2)]
+    it3, x3 = run(fresh, d, idx2, b2[0], b2[1], torch.empty_like(rhs), torch.zeros_like(rhs))
+    assert it2 == it3 and torch.equal(x2, x3)
+    fresh.close()
