@@ -476,6 +476,56 @@ int smh_gnn_loss_grad(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int32_t nprobe
 int smh_gnn_train_step(smh_ctx *ctx, smh_gnn *g, const smh_bsr *A, int32_t nprobes,
                        uint64_t seed, double lr, double *loss);
 
+/* ---- compiled Simit programs ------------------------------------------------
+ * The front-end (Program::loadFile) turns each map's element function into
+ * HIP C++.  These entry points compile it at run time and launch it: the
+ * counterpart of the reference GPU backend's PTX generation,
+ * cuModuleLoadDataEx and cuLaunchKernel (GPUFunction::init,
+ * gpu_function.cpp:622-760, :930).
+ *
+ * smh_hiprtc_compile: HIP source -> code object for arch ("gfx950" when
+ * NULL), with the library's flags (-O3, no FMA contraction, no packed FP32).
+ * Needs no device.  *image is malloc'ed (free with smh_image_free); log gets
+ * the compiler's messages (errors on failure). */
+int smh_hiprtc_compile(const char *source, const char *name, const char *arch, void **image,
+                       size_t *bytes, char *log, size_t log_len);
+void smh_image_free(void *image);
+typedef struct smh_module smh_module;
+int smh_module_load(smh_ctx *ctx, const void *image, size_t bytes, smh_module **out);
+/* launches kernel over nthreads threads (256 per workgroup) on the context's
+ * stream; args is the kernel's argument struct, args_bytes long */
+int smh_module_launch(smh_ctx *ctx, smh_module *m, const char *kernel, uint64_t nthreads,
+                      const void *args, size_t args_bytes);
+int smh_module_destroy(smh_ctx *ctx, smh_module *m);
+/* A map's output tensor from its staged writes.  The element kernel stages
+ * every write in its own slot; entry i of out (bs components) takes the
+ * writes src[ptr[i]..ptr[i+1]) in the order the reference performs them
+ * (elements ascending, then the element function's execution order).
+ * `map f to E reduce +` (taken NULL): out = ((0 + w0) + w1) + ..., the
+ * compound writes of lower_maps.cpp:37-49 into the zeroed global tensor.
+ * A map without a reduction (taken = per-slot flags of the writes executed):
+ * plain writes, so out takes the last write taken and an entry no element
+ * wrote keeps its value. */
+int smh_stage_reduce(smh_ctx *ctx, uint64_t nout, uint32_t bs, const uint64_t *ptr,
+                     const uint32_t *src, const double *stage, const uint8_t *taken,
+                     double *out);
+/* The elementwise statements of an exported Simit function over system
+ * vectors (and matrix values), one rounding per operation, no contraction
+ * (the reference's scalar loops, lower_indexexprs.cpp:669-711):
+ *   LINCOMB out = a*x + b*y (y NULL: a*x)   DIVS out = x / a
+ *   EMUL out = x .* y   EDIV out = x ./ y   FILL out = a
+ *   ADDS out = x + a    SDIV out = a / x
+ * out may alias x or y. */
+#define SMH_VEC_LINCOMB 0
+#define SMH_VEC_DIVS 1
+#define SMH_VEC_EMUL 2
+#define SMH_VEC_EDIV 3
+#define SMH_VEC_FILL 4
+#define SMH_VEC_ADDS 5
+#define SMH_VEC_SDIV 6
+int smh_vec_op(smh_ctx *ctx, int32_t op, uint64_t n, double a, const double *x, double b,
+               const double *y, double *out);
+
 #ifdef __cplusplus
 }
 #endif

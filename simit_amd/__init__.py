@@ -190,6 +190,14 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
                                              C.POINTER(CGOptions), C.POINTER(CGStats)]),
         "smh_block_jacobi": (C.c_int, [vp, C.POINTER(Bsr), vp]),
         "smh_axpby": (C.c_int, [vp, u64, C.c_double, vp, C.c_double, vp]),
+        "smh_vec_op": (C.c_int, [vp, i32, u64, C.c_double, vp, C.c_double, vp, vp]),
+        "smh_stage_reduce": (C.c_int, [vp, u64, u32, vp, vp, vp, vp, vp]),
+        "smh_hiprtc_compile": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(vp),
+                                         C.POINTER(C.c_size_t), C.c_char_p, C.c_size_t]),
+        "smh_image_free": (None, [vp]),
+        "smh_module_load": (C.c_int, [vp, vp, C.c_size_t, C.POINTER(vp)]),
+        "smh_module_launch": (C.c_int, [vp, vp, C.c_char_p, u64, vp, C.c_size_t]),
+        "smh_module_destroy": (C.c_int, [vp, vp]),
         "smh_bsr_forget": (C.c_int, [vp, vp]),
         "smh_cg_default_opts": (None, [C.POINTER(CGOptions)]),
         "smh_cg_solve": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, C.POINTER(CGOptions),

@@ -17,9 +17,11 @@ LIB = LIBDIR / "libsimit_hip.so"
 HOST_LIB = LIBDIR / "libsimit.so"
 PROGRAM_TESTS = BINDIR / "simit_program_tests"
 
-HIP_SOURCES = ["smh_core.hip", "smh_index.hip", "smh_assemble.hip", "smh_cg.hip", "smh_gnn.hip"]
+HIP_SOURCES = ["smh_core.hip", "smh_index.hip", "smh_assemble.hip", "smh_cg.hip", "smh_gnn.hip",
+               "smh_jit.hip"]
 HOST_SOURCES = ["simit/graph.cpp", "simit/program.cpp", "simit/mesh.cpp",
-                "simit/reorder.cpp"]
+                "simit/reorder.cpp", "simit/parser.cpp", "simit/codegen.cpp",
+                "simit/sim_function.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
@@ -51,7 +53,8 @@ def build_hip(jobs: int = 4) -> Path:
         if len(procs) >= jobs:
             _wait(procs)
     _wait(procs)
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", str(LIB), *map(str, objs)])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", str(LIB), *map(str, objs), "-L/opt/rocm/lib",
+          "-lhiprtc", "-Wl,-rpath,/opt/rocm/lib"])
     return LIB
 
 

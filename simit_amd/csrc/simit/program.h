@@ -4,8 +4,11 @@
 // (src/program.h, program.cpp:30-107), Function (src/function.h:30-94,
 // function.cpp:22-139) and the backend plugin contract backend::Function
 // (src/backend/backend_function.h:27-88).  The reference compiles Simit source
-// to LLVM; this backend has no front-end (out of scope) and instead runs the
-// hot-path programs as fixed HIP pipelines, looked up by name:
+// to LLVM.  Program::loadFile parses Simit source (frontend.h): each map's
+// element function becomes a generated HIP kernel compiled for gfx950 at
+// init, and the exported function's statements run over the C ABI.  The
+// hot-path programs are also available as fixed pipelines of hand-written
+// kernels (Program::loadBuiltin), by name:
 //
 //   "isprings"         test/input/program/isprings.sim main (implicit springs
 //                      step: assembly of MD + K, inline CG, x += h v)
@@ -213,17 +216,33 @@ class Function {
 };
 std::ostream &operator<<(std::ostream &os, const Function &f);
 
+namespace fe {
+struct ProgramAST;
+}
+
+// Program (program.h:18-60, program.cpp:68-107): loadFile / loadString parse
+// Simit source with the front-end (frontend.h); compile(f) returns f as a
+// function of this backend: its maps as generated gfx950 kernels, its
+// statements run over the C ABI.  Status: 0 ok, 1 parse errors (see
+// getDiagnostics), 2 unreadable file.  loadBuiltin selects one of the fixed
+// pipelines listed above instead (hand-written kernels end to end).
 class Program {
  public:
-  // 0 ok, 1 errors, 2 unreadable (program.h:31-41).  A file is accepted when
-  // its name is one of the programs above (isprings.sim, cg.sim, ...).
   int loadFile(const std::string &path);
+  int loadString(const std::string &text);
   int loadBuiltin(const std::string &name);
   std::vector<std::string> getFunctionNames() const;
   Function compile(const std::string &function);
+  const std::string &getDiagnostics() const { return diagnostics; }
+  // HIP backend extensions: the exported functions of a parsed program, and
+  // the generated HIP source of a function's maps
+  std::vector<std::string> getExportedFunctionNames() const;
+  std::string getGeneratedSource(const std::string &function) const;
 
  private:
   std::string program;
+  std::shared_ptr<const fe::ProgramAST> ast;
+  std::string diagnostics;
 };
 
 }  // namespace simit

@@ -18,6 +18,7 @@
 #include "simit/mesh.h"
 #include "simit/program.h"
 #include "simit/reorder.h"
+#include "simit_hip.h"
 
 using namespace simit;
 
@@ -40,15 +41,29 @@ static int failures = 0;
     }                                                                         \
   } while (0)
 
-static Function load(const std::string &program) {
+// The programs run either as this backend's fixed pipelines (by name) or,
+// when g_programs is set, parsed from Simit source in tests/programs/ (this
+// backend's own restatements of the reference programs, same arithmetic).
+static bool g_programs = false;
+static std::string programFile(const std::string &program) {
+  if (program == "isprings") return "tests/programs/springs.sim";
+  if (program == "cg") return "tests/programs/cg_points.sim";
+  if (program == "femTet") return "tests/programs/neohookean.sim";
+  return "";
+}
+static Function loadFn(const std::string &program, const std::string &function) {
   Program p;
-  if (p.loadFile(program + ".sim") != 0) {
-    std::printf("  FAIL: cannot load %s\n", program.c_str());
+  const std::string file = g_programs ? programFile(program) : "";
+  const int rc = file.empty() ? p.loadBuiltin(program) : p.loadFile(file);
+  if (rc != 0) {
+    std::printf("  FAIL: cannot load %s: %s\n", (file.empty() ? program : file).c_str(),
+                p.getDiagnostics().c_str());
     ++failures;
     return Function();
   }
-  return p.compile("main");
+  return p.compile(function);
 }
+static Function load(const std::string &program) { return loadFn(program, "main"); }
 
 // test/program-isprings.cpp:91-175
 static void isprings() {
@@ -478,15 +493,6 @@ static void extern_csr() {
 // test/program-fem-test.cpp:14-106: the bar2k mesh, initializeTet, ten
 // steps of femTet.sim's main; and test/reorder-tests.cpp:403-441, the same
 // after a Hilbert reordering of the vertices and tets
-static Function loadFn(const std::string &program, const std::string &fn) {
-  Program p;
-  if (p.loadFile(program + ".sim") != 0) {
-    std::printf("  FAIL: cannot load %s\n", program.c_str());
-    ++failures;
-    return Function();
-  }
-  return p.compile(fn);
-}
 static FieldRef<double, 3> initializeFem(const MeshVol &mv, Set &verts, Set &tets,
                                          std::vector<ElementRef> &vertRefs) {
   FieldRef<double, 3> x = verts.addField<double, 3>("x");
@@ -636,6 +642,41 @@ static int extern_file(const std::string &dir) {
 
 int main(int argc, char **argv) {
   init("hip", 8);
+  if (argc >= 3 && std::string(argv[1]) == "compile") {
+    // compile FILE [FUNCTION...]: parse, check and generate each exported
+    // function (or the listed ones) and compile the kernels for gfx950 (no
+    // device needed)
+    Program p;
+    if (p.loadFile(argv[2]) != 0) {
+      std::printf("%s: %s", argv[2], p.getDiagnostics().c_str());
+      return 1;
+    }
+    std::vector<std::string> fns(argv + 3, argv + argc);
+    if (fns.empty()) fns = p.getExportedFunctionNames();
+    int bad = 0;
+    for (auto &f : fns) {
+      try {
+        Function fn = p.compile(f);
+        const std::string src = p.getGeneratedSource(f);
+        void *img = nullptr;
+        size_t bytes = 0;
+        std::vector<char> log(1 << 16);
+        const int rc = smh_hiprtc_compile(src.c_str(), f.c_str(), "gfx950", &img, &bytes,
+                                          log.data(), log.size());
+        smh_image_free(img);
+        std::printf("%s: %s: %zu bytes of generated HIP, code object %zu bytes%s\n", argv[2],
+                    f.c_str(), src.size(), bytes, rc ? " (COMPILE FAILED)" : "");
+        if (rc) {
+          std::printf("%s\n", log.data());
+          ++bad;
+        }
+      } catch (const std::exception &e) {
+        std::printf("%s: %s: not compiled: %s\n", argv[2], f.c_str(), e.what());
+        ++bad;
+      }
+    }
+    return bad ? 1 : 0;
+  }
   if (argc == 3 && std::string(argv[1]) == "extern_file") {
     try {
       return extern_file(argv[2]);
@@ -650,7 +691,12 @@ int main(int argc, char **argv) {
       {"reorder_int", reorder_int}, {"reorder_double", reorder_double},
       {"hilbert_curve", hilbert_curve}, {"reorder_bunny_host", reorder_bunny_host},
       {"mesh_load", mesh_load}, {"extern_csr", extern_csr}, {"fem_tet", fem_tet},
-      {"reorder_fem_tet", reorder_fem_tet}};
+      {"reorder_fem_tet", reorder_fem_tet},
+      // the same programs and asserted values, parsed from Simit source
+      {"sim_isprings", [] { g_programs = true; isprings(); g_programs = false; }},
+      {"sim_cg", [] { g_programs = true; cg(); g_programs = false; }},
+      {"sim_fem_tet", [] { g_programs = true; fem_tet(); g_programs = false; }},
+      {"sim_reorder_fem_tet", [] { g_programs = true; reorder_fem_tet(); g_programs = false; }}};
   for (auto &t : tests) {
     bool run = argc <= 1;
     for (int i = 1; i < argc; ++i) run |= t.first == argv[i];

@@ -2,7 +2,10 @@
 
 Runs the compiled program tests (simit_amd/csrc/tests_cpp/program_tests.cpp),
 which follow the reference's gtest program tests (test/program-isprings.cpp,
-test/program-cg-test.cpp) and assert the same values.
+test/program-cg-test.cpp, test/program-fem-test.cpp) and assert the same
+values.  The sim_* cases run the same tests through Simit source parsed by
+Program::loadFile (tests/programs/*.sim): element functions generated as
+HIP and compiled at init, the exported function interpreted over the C ABI.
 """
 import subprocess
 from pathlib import Path
@@ -17,7 +20,8 @@ BIN = ROOT / "simit_amd" / "bin" / "simit_program_tests"
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", ["isprings", "cg", "springs_lattice", "mesh_bunny",
                                   "reorder_bunny_springs", "extern_csr", "fem_tet",
-                                  "reorder_fem_tet"])
+                                  "reorder_fem_tet", "sim_isprings", "sim_cg", "sim_fem_tet",
+                                  "sim_reorder_fem_tet"])
 def test_cpp_program(name):
     import torch
     if not torch.cuda.is_available():
