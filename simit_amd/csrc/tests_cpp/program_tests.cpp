@@ -640,6 +640,42 @@ static int extern_file(const std::string &dir) {
   return failures ? 1 : 0;
 }
 
+// sim_fem_ops PROGRAM DIR: the bar2k set-up, PROGRAM's initializeTet and
+// operators (a parsed program that assembles main's force and stiffness and
+// writes y = A v); the fields the GPU test checks against the oracle are
+// written to DIR/<field>.f64
+static int sim_fem_ops(const std::string &program, const std::string &dir) {
+  MeshVol mv;
+  if (mv.loadTet("tests/golden/fem-bar2k/bar2k.node", "tests/golden/fem-bar2k/bar2k.ele") != 0) {
+    std::printf("  FAIL: cannot read bar2k\n");
+    return 1;
+  }
+  Set verts;
+  Set tets(verts, verts, verts, verts);
+  std::vector<ElementRef> vertRefs;
+  initializeFem(mv, verts, tets, vertRefs);
+  verts.addField<double, 3>("y");
+  verts.addField<double, 3>("f");
+  Program p;
+  if (p.loadFile(program) != 0) {
+    std::printf("  FAIL: %s: %s\n", program.c_str(), p.getDiagnostics().c_str());
+    return 1;
+  }
+  for (const char *fn : {"initializeTet", "operators"}) {
+    Function f = p.compile(fn);
+    f.bind("verts", &verts);
+    f.bind("tets", &tets);
+    f.runSafe();
+  }
+  for (const char *n : {"m", "f", "y", "v"})
+    spit(dir + "/" + n + ".f64", verts.field(n)->data(), verts.field(n)->bytes.size());
+  for (const char *n : {"W", "B"})
+    spit(dir + "/" + n + ".f64", tets.field(n)->data(), tets.field(n)->bytes.size());
+  std::printf("sim_fem_ops: %zu vertices, %zu tets\n", (size_t)verts.getSize(),
+              (size_t)tets.getSize());
+  return 0;
+}
+
 int main(int argc, char **argv) {
   init("hip", 8);
   if (argc >= 3 && std::string(argv[1]) == "compile") {
@@ -676,6 +712,14 @@ int main(int argc, char **argv) {
       }
     }
     return bad ? 1 : 0;
+  }
+  if (argc == 4 && std::string(argv[1]) == "sim_fem_ops") {
+    try {
+      return sim_fem_ops(argv[2], argv[3]);
+    } catch (const std::exception &e) {
+      std::printf("  FAIL: exception %s\n", e.what());
+      return 1;
+    }
   }
   if (argc == 3 && std::string(argv[1]) == "extern_file") {
     try {
