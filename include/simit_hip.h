@@ -376,7 +376,12 @@ typedef struct {
   int32_t precision;     /* SMH_F64, SMH_F32, SMH_F32X2, SMH_F32D          */
   const void *vals;      /* device [27*nrows] (double for SMH_F64, else float) */
   const void *vals_lo;   /* SMH_F32X2: device float[27*nrows], else NULL    */
+  uint32_t flags;        /* SMH_STENCIL_PERIODIC, or 0                       */
 } smh_stencil27;
+/* Neighbour coordinates wrap around (the rows of a grid edge set's points,
+ * whose edges are periodic: graph.h:121-168, system-gemv-tests.cpp:60-337).
+ * smh_stencil27_mv only; the solver rejects it. */
+#define SMH_STENCIL_PERIODIC 1u
 /* scatter a 1x1 blocked-CSR matrix (any precision; SMH_F32X2 copies both
  * halves) into stencil storage; SMH_ERR_INVALID if a column lies outside its
  * row's 27 offsets. */

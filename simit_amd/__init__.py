@@ -31,6 +31,7 @@ HEADER_PATH = PKG.parent / "include" / "simit_hip.h"
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM, ERR_BREAKDOWN, ERR_NOT_CONVERGED = range(7)
 F64, F32, F32X2, F32D = 0, 1, 2, 3  # F32D: float values, double vectors
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2, 3
+STENCIL_PERIODIC = 1
 INDEX_FORCE_DIAG = 1
 
 _STATUS = {0: "OK", 1: "INVALID", 2: "HIP", 3: "UNSUPPORTED", 4: "NOMEM",
@@ -60,9 +61,10 @@ class Bsr(C.Structure):
 class Stencil27(C.Structure):
     """Scalar 27-point stencil matrix of an n0 x n1 x n2 grid of rows:
     vals[s*n + i] couples row i with column i + off_s (ascending offsets,
-    s = 13 the diagonal)."""
+    s = 13 the diagonal); flags STENCIL_PERIODIC wraps the neighbours around."""
     _fields_ = [("n0", C.c_uint32), ("n1", C.c_uint32), ("n2", C.c_uint32),
-                ("precision", C.c_int32), ("vals", C.c_void_p), ("vals_lo", C.c_void_p)]
+                ("precision", C.c_int32), ("vals", C.c_void_p), ("vals_lo", C.c_void_p),
+                ("flags", C.c_uint32)]
 
 
 class SpringsParams(C.Structure):

@@ -38,7 +38,8 @@ struct CVal {
   std::string s;                               // scalar value (C expression)
   std::function<std::string(const Idx &)> at;  // array element
   std::string arr;  // a row-major C array holding the value, if there is one
-  enum Ref { None, Elem, Endpoint } ref = None;  // the element / an endpoint (s = its index)
+  // the element / an endpoint or grid neighbour / a grid link (s = its index)
+  enum Ref { None, Elem, Endpoint, Link } ref = None;
 };
 
 std::string ctype(char t) { return t == 'i' ? "int" : t == 'b' ? "bool" : "double"; }
@@ -60,7 +61,7 @@ std::string flat(const Idx &idx, const std::vector<int> &dims) {
 }
 
 struct Sym {
-  enum K { Local, Param, ArrParam, Result, ElemParam, TupleParam, Output, Loop } k = Local;
+  enum K { Local, Param, ArrParam, Result, ElemParam, TupleParam, GridParam, Output, Loop } k = Local;
   std::string c;
   Shape sh;
   int out = -1;
@@ -109,6 +110,8 @@ class Gen {
     int indent = 1;
     MapKernel *mk = nullptr;
     std::string elemSet, epSet, elemType, epType;
+    std::string gridSet, gridType;  // a map through a grid: its edge set
+    std::string dims[3];            // ... and its dimension arguments
     std::vector<LoopInfo> loops;
     int whileDepth = 0;
     std::map<const Stmt *, Site> sites;
@@ -124,6 +127,17 @@ class Gen {
 #define SIMIT_DEV __device__ __forceinline__
 // an index the generator could not bound at compile time, kept in range
 SIMIT_DEV int simit_clamp(int i, int n) { return i < 0 ? 0 : (i >= n ? n - 1 : i); }
+// the point at offset (o0, o1, o2) from point e of an n0 x n1 x n2 periodic grid
+// (first coordinate fastest)
+SIMIT_DEV unsigned long long simit_nbr(unsigned long long e, long long o0, long long o1,
+                                       long long o2, unsigned long long n0,
+                                       unsigned long long n1, unsigned long long n2) {
+  const long long m0 = (long long)n0, m1 = (long long)n1, m2 = (long long)n2;
+  const long long c0 = (((long long)(e % n0) + o0) % m0 + m0) % m0;
+  const long long c1 = (((long long)((e / n0) % n1) + o1) % m1 + m1) % m1;
+  const long long c2 = (((long long)(e / (n0 * n1)) + o2) % m2 + m2) % m2;
+  return (unsigned long long)c0 + n0 * ((unsigned long long)c1 + n1 * (unsigned long long)c2);
+}
 SIMIT_DEV double simit_det3(const double *a) {
   return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
          a[2] * (a[3] * a[7] - a[4] * a[6]);
@@ -264,6 +278,7 @@ SIMIT_DEV double simit_log(double x) {
         return v;
       }
       case Sym::TupleParam: fail({}, "a tuple of endpoints is read by index");
+      case Sym::GridParam: fail({}, "a grid's links are read as l[offset; offset]");
       case Sym::Output: fail({}, "results of a map function are written, not read");
       default: break;
     }
@@ -295,8 +310,11 @@ SIMIT_DEV double simit_log(double x) {
     } else if (base.ref == CVal::Endpoint) {
       set = F->epSet;
       type = F->epType;
+    } else if (base.ref == CVal::Link) {
+      set = F->gridSet;
+      type = F->gridType;
     } else {
-      fail(l, "only the element and its endpoints have fields");
+      fail(l, "only the element, its endpoints and grid links have fields");
     }
     const ElementDecl &ed = elementDecl(type, l);
     TypeP ft;
@@ -384,6 +402,7 @@ SIMIT_DEV double simit_log(double x) {
       case Expr::Field: return fieldRead(eval(e->args[0]), e->name, e->loc);
       case Expr::Apply: return apply(e);
       case Expr::Map: fail(e->loc, "a map inside an element function");
+      case Expr::SetRead: return setRead(e);
     }
     fail(e->loc, "unsupported expression");
   }
@@ -585,6 +604,57 @@ SIMIT_DEV double simit_log(double x) {
     if (n == "log") return scalarVal("simit_log((double)" + a[0].s + ")", 'f');
     if (n == "abs") return scalarVal("fabs((double)" + a[0].s + ")", 'f');
     return scalarVal(n + "((double)" + a[0].s + ")", 'f');
+  }
+
+  // grid offsets of a set read: constants, one per grid dimension
+  std::vector<long> offsets(const ExprP &e, size_t from, size_t count) {
+    std::vector<long> o;
+    for (size_t k = 0; k < count; ++k) o.push_back(evalConstInt(e->args[from + k], {}));
+    return o;
+  }
+  std::string nbr(const std::vector<long> &o) {
+    std::string c = "simit_nbr(e";
+    for (int k = 0; k < 3; ++k) c += ", " + std::to_string(k < (int)o.size() ? o[k] : 0) + "ll";
+    return c + ", " + F->dims[0] + ", " + F->dims[1] + ", " + F->dims[2] + ")";
+  }
+  // points[o] (a neighbour) or l[a;b] (the link between the points at a and b)
+  CVal setRead(const ExprP &e) {
+    if (!F->mk || !F->mk->gridDims) fail(e->loc, "grid reads need a map through a grid");
+    const int nd = F->mk->gridDims;
+    const ExprP &base = e->args[0];
+    const size_t n = e->args.size() - 1;
+    Sym *s = lookup(base->name);
+    CVal v;
+    if (s && s->k == Sym::GridParam) {
+      if (e->split != nd || (int)n != 2 * nd)
+        fail(e->loc, "a link is read as l[a;b] with two offsets of " + std::to_string(nd) +
+                         " coordinates");
+      const std::vector<long> a = offsets(e, 1, nd), b = offsets(e, 1 + nd, nd);
+      int axis = -1;
+      long dir = 0;
+      for (int k = 0; k < nd; ++k)
+        if (b[k] != a[k]) {
+          if (axis >= 0 || (b[k] - a[k] != 1 && b[k] - a[k] != -1))
+            fail(e->loc, "l[a;b] must name two neighbouring grid points");
+          axis = k;
+          dir = b[k] - a[k];
+        }
+      if (axis < 0) fail(e->loc, "l[a;b] must name two neighbouring grid points");
+      // the link leaves the lower of the two points along its axis
+      v.ref = CVal::Link;
+      v.s = "(" + nbr(dir > 0 ? a : b) + "*" + std::to_string(nd) + "ull+" + std::to_string(axis) +
+            "ull)";
+      return v;
+    }
+    if (!s && base->name == F->elemSet) {
+      if (e->split != (int)n || (int)n != nd)
+        fail(e->loc, base->name + "[...] takes one offset of " + std::to_string(nd) +
+                         " coordinates");
+      v.ref = CVal::Endpoint;
+      v.s = nbr(offsets(e, 1, nd));
+      return v;
+    }
+    fail(e->loc, "only the grid argument and the grid's point set are read with [...]");
   }
 
   CVal apply(const ExprP &e) {
@@ -837,7 +907,8 @@ SIMIT_DEV double simit_log(double x) {
     for (auto &i : idx) {
       if (!i) fail(lhs->loc, "a result is written at an element or an endpoint");
       CVal v = eval(i);
-      if (v.ref == CVal::None) fail(i->loc, "a result is written at an element or an endpoint");
+      if (v.ref == CVal::None || v.ref == CVal::Link)
+        fail(i->loc, "a result is written at an element or an endpoint");
     }
     if (F->whileDepth) fail(s->loc, "results written inside a while loop");
     for (auto &L : F->loops)
@@ -1097,8 +1168,27 @@ SIMIT_DEV double simit_log(double x) {
     // slots: 0 count, 1 endpoints, then stage (+ taken) per output
     mk.args.push_back({KernelArg::Count, "", "", 0});
     mk.args.push_back({KernelArg::Endpoints, "", "", 0});
+    const bool grid = !mk.req.through.empty();
+    if (grid) {
+      TypeP gt = P.externType(mk.req.through);
+      if (mk.card) fail(l, "a map through a grid runs over the grid's point set");
+      if (!gt || gt->kind != Type::Set || !gt->gridDims)
+        fail(l, mk.req.through + " is not a grid edge set");
+      if (gt->endpoints[0] != mk.req.target)
+        fail(l, "grid " + mk.req.through + " is over " + gt->endpoints[0] + ", not " +
+                    mk.req.target);
+      mk.gridDims = gt->gridDims;
+      fn.gridSet = mk.req.through;
+      fn.gridType = gt->elem;
+      fn.epSet = mk.req.target;  // neighbours are points of the same set
+      fn.epType = fn.elemType;
+      for (int k = 0; k < 3; ++k) {
+        mk.args.push_back({KernelArg::GridDim, "", "", k});
+        fn.dims[k] = "a.a" + std::to_string(mk.args.size() - 1);
+      }
+    }
     const size_t np = f->params.size();
-    const size_t expect = (size_t)mk.req.nextra + (mk.card ? 2 : 1);
+    const size_t expect = (size_t)mk.req.nextra + (mk.card ? 2 : 1) + (grid ? 1 : 0);
     if (np != expect)
       fail(f->loc, "map " + f->name + " to " + mk.req.target + ": the function takes " +
                        std::to_string(np) + " arguments, the map supplies " +
@@ -1132,6 +1222,16 @@ SIMIT_DEV double simit_log(double x) {
       s.k = Sym::TupleParam;
       fn.syms[p.name] = s;
     }
+    if (grid) {
+      const Param &p = f->params[mk.req.nextra + 1];
+      if (p.type->kind != Type::Set || p.type->gridDims != mk.gridDims ||
+          p.type->elem != fn.gridType)
+        fail(f->loc, "argument " + p.name + " must be grid[" + std::to_string(mk.gridDims) + "]{" +
+                         fn.gridType + "}(" + mk.req.target + ")");
+      Sym s;
+      s.k = Sym::GridParam;
+      fn.syms[p.name] = s;
+    }
     for (auto &r : f->results) {
       MapOutput mo;
       mo.name = r.name;
@@ -1153,6 +1253,10 @@ SIMIT_DEV double simit_log(double x) {
       if (t.comp != "float") fail(f->loc, "results must be float tensors");
       mo.bs = 1;
       for (int d : mo.blockDims) mo.bs *= d;
+      if (grid && mo.matrix) {
+        if (mo.bs != 1) fail(f->loc, "matrices over a grid hold scalar entries");
+        mo.stencil = true;
+      }
       const int oi = (int)mk.outputs.size();
       mk.outputs.push_back(mo);
       mk.args.push_back({KernelArg::Stage, "", "", oi});
@@ -1182,6 +1286,7 @@ SIMIT_DEV double simit_log(double x) {
         case KernelArg::Taken: t = "unsigned char *"; break;
         case KernelArg::Field: t = a.index == 1 ? "int *" : a.index == 2 ? "unsigned char *" : "double *"; break;
         case KernelArg::Scalar: t = "double"; break;
+        case KernelArg::GridDim: t = "unsigned long long"; break;
       }
       k << "  " << t << (a.kind == KernelArg::Scalar ? " x" : " a") << s << ";\n";
     }
@@ -1216,15 +1321,34 @@ SIMIT_DEV double simit_log(double x) {
             inst += (vals.at(site.loops[k].name) - site.loops[k].lo) * stride;
             stride *= site.loops[k].trip;
           }
+          MapOutput &mo = mk.outputs[site.out];
+          MapOutput::Write w{};
+          w.inst = (int)inst;
           int li[2] = {-1, -1};
           for (size_t k = 0; k < site.idx.size(); ++k) {
             const ExprP &x = site.idx[k];
-            if (x->kind == Expr::Apply) li[k] = (int)evalConstInt(x->args[1], vals);
-            else li[k] = -1;  // the element itself
+            if (x->kind == Expr::Apply) {
+              li[k] = (int)evalConstInt(x->args[1], vals);
+            } else if (x->kind == Expr::SetRead) {  // a grid neighbour
+              li[k] = -2;
+              for (int c = 0; c + 1 < (int)x->args.size() && c < 3; ++c)
+                w.off[k][c] = (int)evalConstInt(x->args[1 + c], vals);
+              if (mo.stencil) {
+                if (k == 0) throw CodegenError(x->loc, "grid matrices are written in their own row");
+                for (int c = 0; c < 3; ++c)
+                  if (w.off[k][c] < -1 || w.off[k][c] > 1)
+                    throw CodegenError(x->loc, "grid matrix entries lie within one step of the "
+                                               "row's point (27-point stencil)");
+              }
+            } else {
+              li[k] = -1;  // the element itself
+            }
             if (li[k] >= mk.card && mk.card)
               throw CodegenError(x->loc, "endpoint index out of range");
           }
-          mk.outputs[site.out].order.push_back({(int)inst, li[0], li[1]});
+          w.li = li[0];
+          w.lj = li[1];
+          mo.order.push_back(w);
           break;
         }
         case Stmt::If:

@@ -29,28 +29,32 @@ struct CodegenError : std::runtime_error {
 // A map as the exported function uses it.
 struct MapRequest {
   std::string func, target;
+  std::string through;  // a grid edge set over target (map ... through)
   bool reduce = false;
   int nextra = 0;  // leading scalar arguments bound at the map
 };
 
 // One slot of a kernel's argument struct (every slot is 8 bytes).
 struct KernelArg {
-  enum Kind { Count, Endpoints, Stage, Taken, Field, Scalar } kind = Count;
+  enum Kind { Count, Endpoints, Stage, Taken, Field, Scalar, GridDim } kind = Count;
   std::string set, field;  // Field
-  int index = 0;           // Stage / Taken: output; Scalar: extra argument
+  int index = 0;           // Stage / Taken: output; Scalar: extra argument; GridDim: axis
 };
 
 struct MapOutput {
   std::string name;
   bool matrix = false;        // tensor[V,V](block) (else tensor[V](block))
+  bool stencil = false;       // a matrix over a grid's points: 27-point periodic stencil
   std::string rowSet;         // V
   std::vector<int> blockDims;  // dims of one entry ([] for float entries)
   int bs = 1;                 // doubles per entry
   int ninst = 0;              // write slots per element
   // the writes in execution order: slot, local row and column endpoint
-  // (-1: the element itself; col unused for vectors)
+  // (-1: the element itself, -2: the grid point at off[0] / off[1] from it;
+  // col unused for vectors)
   struct Write {
     int inst, li, lj;
+    int off[2][3];
   };
   std::vector<Write> order;
 };
@@ -60,6 +64,7 @@ struct MapKernel {
   MapRequest req;
   int card = 0;             // endpoints per element (0: a map over a vertex set)
   std::string endpointSet;  // the set the endpoints index
+  int gridDims = 0;         // a map through a grid of this many dimensions
   std::vector<MapOutput> outputs;
   std::vector<KernelArg> args;
   std::vector<std::pair<std::string, std::string>> fieldsWritten;  // (set, field) of inout writes

@@ -43,6 +43,7 @@ struct Type {
   bool transposed = false;            // a row vector
   std::string elem;                   // Element / Set / Tuple element type
   std::vector<std::string> endpoints;  // Set
+  int gridDims = 0;                   // Set: a grid[n]{E}(points) edge set (n)
   int tupleLen = 0;                   // Tuple
   std::string str() const;
 };
@@ -62,7 +63,9 @@ struct Expr {
     Apply,      // name(args): a call, a tensor read or a tuple read (null arg = ':')
     Field,      // args[0].name
     Literal,    // dense tensor literal: lit (row-major), litDims
-    Map         // map name(args) to target [reduce +]
+    Map,        // map name(args) to target [through set] [reduce +]
+    SetRead     // args[0][args[1..split]; args[split+1..]]: a grid neighbour
+                // (one offset list) or link (two), offsets from the element
   } kind = Num;
   Loc loc;
   double num = 0.0;
@@ -72,8 +75,10 @@ struct Expr {
   std::vector<double> lit;
   std::vector<int> litDims;  // [n] (a row vector) or [rows, cols] or deeper
   bool litInt = false;
-  std::string target;  // Map
+  std::string target;   // Map
+  std::string through;  // Map over a grid: the grid edge set
   bool reduce = false;  // Map
+  int split = 0;        // SetRead: offsets in the first list
 };
 
 struct Stmt;

@@ -139,8 +139,70 @@ class Set {
   Set(const std::string &name, const Set &first, const Sets &...rest) : name(name) {
     endpointSets = {&first, &rest...};
   }
+  // A grid edge set over `points` (graph.h:121-168): points must be empty
+  // and receives d0*d1*... points, numbered with the first coordinate
+  // fastest; this set gets one edge per point and axis, edge p*ndims + k
+  // joining point p to its periodic neighbour along axis k (the numbering
+  // getGridEdge returns).
+  Set(Set &points, std::vector<int> dims) : Set(std::string(), points, std::move(dims)) {}
+  Set(const std::string &name, Set &points, std::vector<int> dims) : name(name) {
+    if (dims.empty())
+      throw SimitException("Grid Edge Set constructor takes the underlying point set and a "
+                           "vector of integer dimension sizes");
+    if (points.getSize() != 0)
+      throw SimitException("Grid Edge Set constructor must be passed an empty underlying point set");
+    for (int d : dims)
+      if (d <= 0) throw SimitException("grid dimensions must be positive");
+    kind = Grid;
+    dimensions = dims;
+    endpointSets = {&points, &points};
+    size_t total = 1;
+    for (int d : dims) total *= (size_t)d;
+    for (size_t p = 0; p < total; ++p) points.add();
+    const int nd = (int)dims.size();
+    for (size_t p = 0; p < total; ++p)
+      for (int k = 0; k < nd; ++k) {
+        // the neighbour of p one step along axis k, wrapping around
+        size_t stride = 1;
+        for (int i = 0; i < k; ++i) stride *= (size_t)dims[i];
+        const size_t c = (p / stride) % (size_t)dims[k];
+        const size_t q = c + 1 == (size_t)dims[k] ? p - c * stride : p + stride;
+        add(ElementRef((int)p), ElementRef((int)q));
+      }
+  }
   Set(const Set &) = delete;
   Set &operator=(const Set &) = delete;
+
+  enum Kind { Unstructured, Grid };
+  Kind getKind() const { return kind; }
+  // a grid edge set's dimension sizes
+  const std::vector<int> &getDimensions() const {
+    if (kind != Grid) throw SimitException("Can only retrieve dimensions for a grid edge set");
+    return dimensions;
+  }
+  // the grid point at coords (coords[0] fastest), graph.h:191-209
+  ElementRef getGridPoint(const std::vector<int> &coords) const {
+    return ElementRef((int)gridIndex(coords, "grid point"));
+  }
+  // the grid edge leaving coords along axis dir, graph.h:212-233
+  ElementRef getGridEdge(const std::vector<int> &coords, int dir) const {
+    const int64_t nd = (int64_t)dimensions.size();
+    const int64_t index = gridIndex(coords, "grid edge") * nd + dir;
+    if (dir < 0 || dir >= nd)
+      throw SimitException("Coordinates must not be negative and must fall within the grid "
+                           "dimensions");
+    return ElementRef((int)index);
+  }
+  std::vector<int> getGridPointCoords(ElementRef elt) const {
+    if (kind != Grid) throw SimitException("Cannot retrieve grid point coords of non-grid set");
+    std::vector<int> c;
+    int64_t i = elt.getIdent();
+    for (int d : dimensions) {
+      c.push_back((int)(i % d));
+      i /= d;
+    }
+    return c;
+  }
 
   const std::string &getName() const { return name; }
   int getSize() const { return numElements; }
@@ -236,6 +298,7 @@ class Set {
   // Move element i to position newPos[i] (a permutation): every field's
   // values and, for an edge set, its endpoint rows.
   void permuteElements(const std::vector<int> &newPos) {
+    if (kind == Grid) throw SimitException("a grid edge set's elements keep the grid's order");
     checkPermutation(newPos, numElements);
     for (auto &kv : fields) {
       FieldData &f = *kv.second;
@@ -277,7 +340,25 @@ class Set {
     return ElementRef(numElements++);
   }
 
+  int64_t gridIndex(const std::vector<int> &coords, const char *what) const {
+    if (kind != Grid)
+      throw SimitException(std::string("Cannot retrieve ") + what + " of non-grid set");
+    if (coords.size() != dimensions.size())
+      throw SimitException("Must provide number of coords equal to the number of dimensions");
+    int64_t index = 0, total = 1;
+    for (int i = (int)dimensions.size() - 1; i >= 0; --i) {
+      index = index * dimensions[i] + coords[i];
+      total *= dimensions[i];
+    }
+    if (index < 0 || index >= total)
+      throw SimitException("Coordinates must not be negative and must fall within the grid "
+                           "dimensions");
+    return index;
+  }
+
   std::string name;
+  Kind kind = Unstructured;
+  std::vector<int> dimensions;
   int numElements = 0;
   uint64_t version = 0;
   std::vector<const Set *> endpointSets;

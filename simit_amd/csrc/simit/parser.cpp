@@ -20,6 +20,8 @@ std::string Type::str() const {
     case Element: return elem;
     case Tuple: return "(" + elem + "*" + std::to_string(tupleLen) + ")";
     case Set: {
+      if (gridDims)
+        return "grid[" + std::to_string(gridDims) + "]{" + elem + "}(" + endpoints[0] + ")";
       std::string s = "set{" + elem + "}";
       if (!endpoints.empty()) {
         s += "(";
@@ -480,8 +482,24 @@ class Parser {
       ty->kind = Type::String;
       return ty;
     }
-    if (isKw("complex") || isKw("grid") || isKw("opaque"))
-      throw err("type '" + cur().s + "' is not supported");
+    if (isKw("complex") || isKw("opaque")) throw err("type '" + cur().s + "' is not supported");
+    if (isKw("grid")) {  // grid[n]{E}(points)
+      next();
+      ty->kind = Type::Set;
+      expect("[");
+      if (cur().k != Tok::Int) throw err("expected the grid's dimension count");
+      ty->gridDims = (int)next().v;
+      if (ty->gridDims < 1 || ty->gridDims > 3) throw err("grids have 1 to 3 dimensions");
+      expect("]");
+      expect("{");
+      ty->elem = ident();
+      expect("}");
+      expect("(");
+      const std::string pts = ident();
+      expect(")");
+      ty->endpoints = {pts, pts};
+      return ty;
+    }
     if (isKw("tensor") || isKw("vector") || isKw("matrix")) {
       const std::string k = next().s;
       ty->kind = Type::Tensor;
@@ -571,7 +589,10 @@ class Parser {
     }
     expectKw("to");
     e->target = ident();
-    if (isKw("through")) throw err("map ... through is not supported");
+    if (isKw("through")) {
+      next();
+      e->through = ident();
+    }
     if (isKw("reduce")) {
       next();
       expect("+");
@@ -675,6 +696,19 @@ class Parser {
           } while (accept(","));
         }
         expect(")");
+        a = e;
+      } else if (isOp("[") && a->kind == Expr::Var) {  // grid neighbour / link
+        ExprP e = mk(Expr::SetRead, cur().loc);
+        next();
+        e->args.push_back(a);
+        if (!isOp("]") && !isOp(";")) do e->args.push_back(expr());
+          while (accept(","));
+        e->split = (int)e->args.size() - 1;
+        if (accept(";")) {
+          do e->args.push_back(expr());
+          while (accept(","));
+        }
+        expect("]");
         a = e;
       } else if (isOp(".") && peek().k == Tok::Ident) {
         const Loc l = next().loc;

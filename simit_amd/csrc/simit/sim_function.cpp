@@ -74,6 +74,8 @@ struct Value {
   bool scaled = false;
   bool transposed = false;  // Vec'
   const Pattern *pat = nullptr;
+  bool stencil = false;  // Mat over a grid's points: 27 periodic slots per row
+  uint32_t sd[3] = {1, 1, 1};
   std::vector<Value> tuple;
   std::string str;
   bool sys() const { return k == Vec || k == Mat; }
@@ -134,7 +136,7 @@ struct Analysis {
         [&](const ExprP &e) {
           if (e->kind == Expr::Map) {
             mapIndex[e.get()] = reqs.size();
-            reqs.push_back({e->name, e->target, e->reduce, (int)e->args.size()});
+            reqs.push_back({e->name, e->target, e->through, e->reduce, (int)e->args.size()});
           }
           if (e->kind == Expr::Apply && e->args[0]->kind == Expr::Var) {
             const Func *g = P.func(e->args[0]->name);
@@ -144,7 +146,7 @@ struct Analysis {
         [&](const StmtP &s) {
           if (s->kind == Stmt::Apply) {
             mapIndex[s.get()] = reqs.size();
-            reqs.push_back({s->func, s->target, false, (int)s->args.size()});
+            reqs.push_back({s->func, s->target, "", false, (int)s->args.size()});
           }
           if (s->kind == Stmt::Assign)
             for (auto &l : s->lhs)
@@ -227,9 +229,20 @@ class SimFunction : public HipFunction {
     }
     check(smh_module_load(ctx, image.data(), image.size(), &module), "loading the element kernels");
     // the matrix patterns
+    for (auto &k : kernels) {
+      if (!k.gridDims) continue;
+      Set *g = sets.at(k.req.through);
+      if (g->getKind() != Set::Grid || (int)g->getDimensions().size() != k.gridDims)
+        throw SimitException(k.req.through + " must be bound to a grid edge set of " +
+                             std::to_string(k.gridDims) + " dimension(s)");
+      size_t n = 1;
+      for (int d : g->getDimensions()) n *= (size_t)d;
+      if (n != (size_t)sets.at(k.req.target)->getSize() || g->getEndpointSet(0) != sets.at(k.req.target))
+        throw SimitException(k.req.through + " is not the grid over the bound " + k.req.target);
+    }
     for (auto &k : kernels)
       for (auto &o : k.outputs) {
-        if (!o.matrix) continue;
+        if (!o.matrix || o.stencil) continue;
         Pattern &p = patterns[o.rowSet];
         p.set = o.rowSet;
         if (k.card && std::find(p.edgeSets.begin(), p.edgeSets.end(), k.req.target) ==
@@ -362,21 +375,48 @@ class SimFunction : public HipFunction {
     }
   }
 
+  // the grid dimensions of a map through a grid (padded with 1)
+  std::vector<uint64_t> gridDims(const MapKernel &k) {
+    std::vector<uint64_t> d(3, 1);
+    if (k.gridDims) {
+      const std::vector<int> &g = sets.at(k.req.through)->getDimensions();
+      for (size_t i = 0; i < g.size() && i < 3; ++i) d[i] = (uint64_t)g[i];
+    }
+    return d;
+  }
+  static uint64_t gridNbr(uint64_t e, const int *o, const std::vector<uint64_t> &d) {
+    int64_t c[3] = {(int64_t)(e % d[0]), (int64_t)((e / d[0]) % d[1]), (int64_t)(e / (d[0] * d[1]))};
+    for (int k = 0; k < 3; ++k) c[k] = ((c[k] + o[k]) % (int64_t)d[k] + (int64_t)d[k]) % (int64_t)d[k];
+    return (uint64_t)c[0] + d[0] * ((uint64_t)c[1] + d[1] * (uint64_t)c[2]);
+  }
+
   void buildMap(smh_ctx *c, const MapKernel &k, MapRt &rt) {
     Set *tgt = sets.at(k.req.target);
     rt.ne = (uint64_t)tgt->getSize();
     const int32_t *ep = k.card ? tgt->getEndpointsData() : nullptr;
+    const std::vector<uint64_t> gd = gridDims(k);
     for (size_t o = 0; o < k.outputs.size(); ++o) {
       const MapOutput &mo = k.outputs[o];
-      const Pattern *pat = mo.matrix ? &patterns.at(mo.rowSet) : nullptr;
-      const uint64_t nout = mo.matrix ? pat->nnz : (uint64_t)sets.at(mo.rowSet)->getSize();
+      const Pattern *pat = mo.matrix && !mo.stencil ? &patterns.at(mo.rowSet) : nullptr;
+      const uint64_t nrow = (uint64_t)sets.at(mo.rowSet)->getSize();
+      const uint64_t nout = mo.stencil ? 27 * nrow : mo.matrix ? pat->nnz : nrow;
       if (rt.ne * (uint64_t)mo.ninst >= 0xffffffffull)
         throw SimitException("map " + k.req.func + ": too many staged writes");
       auto target = [&](uint64_t e, const MapOutput::Write &w) -> uint64_t {
-        const uint32_t r = w.li < 0 ? (uint32_t)e : (uint32_t)ep[e * k.card + w.li];
+        if (mo.stencil) {  // slot-major [27][n]: slot of the column's offset, row e
+          const int *d = w.off[1];
+          const uint64_t slot = w.lj == -2 ? (uint64_t)((d[0] + 1) + 3 * (d[1] + 1) + 9 * (d[2] + 1))
+                                           : 13;
+          return slot * nrow + e;
+        }
+        const uint64_t r = w.li == -2   ? gridNbr(e, w.off[0], gd)
+                           : w.li < 0 ? e
+                                      : (uint64_t)ep[e * k.card + w.li];
         if (!mo.matrix) return r;
-        const uint32_t col = w.lj < 0 ? (uint32_t)e : (uint32_t)ep[e * k.card + w.lj];
-        return pat->loc(r, col);
+        const uint64_t col = w.lj == -2   ? gridNbr(e, w.off[1], gd)
+                             : w.lj < 0 ? e
+                                        : (uint64_t)ep[e * k.card + w.lj];
+        return pat->loc((uint32_t)r, (uint32_t)col);
       };
       std::vector<uint64_t> ptr(nout + 1, 0);
       for (uint64_t e = 0; e < rt.ne; ++e)
@@ -420,7 +460,7 @@ class SimFunction : public HipFunction {
   }
   void sameSpace(const Value &a, const Value &b, Loc l) {
     if (a.k != b.k || a.set != b.set || a.bs() != b.bs() || a.pat != b.pat ||
-        a.buf->n != b.buf->n)
+        a.stencil != b.stencil || a.buf->n != b.buf->n)
       error(l, "operands are tensors of different types");
   }
   Value zeros(const Type &t, const Frame &fr, Loc l) {
@@ -550,6 +590,7 @@ class SimFunction : public HipFunction {
           args[s] = (uint64_t)(uintptr_t)fieldOf(a.set, a.field, l).dev;
           break;
         case KernelArg::Scalar: std::memcpy(&args[s], &xs.at(a.index), 8); break;
+        case KernelArg::GridDim: args[s] = gridDims(k)[a.index]; break;
       }
     }
     check(smh_module_launch(ctx, module, k.kernel.c_str(), rt.ne, args.data(), 8 * args.size()),
@@ -561,7 +602,12 @@ class SimFunction : public HipFunction {
       v.k = mo.matrix ? Value::Mat : Value::Vec;
       v.set = mo.rowSet;
       v.block = mo.blockDims;
-      v.pat = mo.matrix ? &patterns.at(mo.rowSet) : nullptr;
+      v.stencil = mo.stencil;
+      v.pat = mo.matrix && !mo.stencil ? &patterns.at(mo.rowSet) : nullptr;
+      if (mo.stencil) {
+        const std::vector<uint64_t> gd = gridDims(k);
+        for (int i = 0; i < 3; ++i) v.sd[i] = (uint32_t)gd[i];
+      }
       v.buf = newBuf(rt.nout[o] * (size_t)mo.bs);
       if (!k.req.reduce) {  // plain writes: unwritten entries keep the target's values
         if (initSet && k.outputs.size() == 1 && !mo.matrix) {
@@ -684,6 +730,13 @@ class SimFunction : public HipFunction {
         if (a.set != b.set || a.bs() != b.bs() * b.bs() || (b.bs() != 1 && b.bs() != 3))
           error(l, "matrix and vector do not match (blocks of 1 or 3 components)");
         Value y = sysLike(b);
+        if (a.stencil) {
+          if (b.bs() != 1) error(l, "a grid matrix multiplies a vector of scalars");
+          smh_stencil27 S{a.sd[0], a.sd[1], a.sd[2], SMH_F64, a.buf->p, nullptr,
+                          SMH_STENCIL_PERIODIC};
+          check(smh_stencil27_mv(ctx, &S, b.buf->p, y.buf->p), "A*x");
+          return y;
+        }
         smh_bsr A{a.pat->nrows, (uint32_t)(b.bs() == 1 ? 1 : 3), a.pat->nnz, a.pat->rowptr,
                   a.pat->colidx, SMH_F64, a.buf->p, nullptr};
         check(smh_bsrmv(ctx, &A, b.buf->p, y.buf->p, 1), "A*x");

@@ -704,7 +704,19 @@ struct StDev {
   int32_t off[27];
   const void *vals;
   const void *vals_lo;
+  uint32_t n0, n1, n2;  // the grid of rows (periodic neighbours)
 };
+
+// row j of slot s of row i on a periodic n0 x n1 x n2 grid (offsets of slot s:
+// s % 3 - 1, s / 3 % 3 - 1, s / 9 - 1 along the three axes)
+__device__ __forceinline__ uint32_t periodic_nbr(const StDev &A, uint32_t i, int s) {
+  auto wrap = [](uint32_t c, int d, uint32_t n) -> uint32_t {
+    return d < 0 ? (c == 0 ? n - 1 : c - 1) : d > 0 ? (c + 1 == n ? 0u : c + 1) : c;
+  };
+  const uint32_t c0 = i % A.n0, c1 = (i / A.n0) % A.n1, c2 = i / (A.n0 * A.n1);
+  return wrap(c0, s % 3 - 1, A.n0) +
+         A.n0 * (wrap(c1, s / 3 % 3 - 1, A.n1) + A.n1 * wrap(c2, s / 9 - 1, A.n2));
+}
 
 // One thread per row: the 27 products in ascending column order, each
 // rounded before it is added -- the order and rounding of the tile loop, so
@@ -717,7 +729,7 @@ struct StDev {
 // 256-row block in LDS (9 global loads per row instead of 27 gathers; no
 // faster, the solver iteration slower) and non-temporal value loads
 // (239 us).
-template <int PREC, class XG, class EPI>
+template <int PREC, class XG, class EPI, bool PER = false>
 __device__ __forceinline__ void stencil_rows(const StDev &A, const XG &xg, EPI &epi) {
 #pragma clang fp contract(off)
   using T = typename Val<PREC>::T;
@@ -737,9 +749,13 @@ __device__ __forceinline__ void stencil_rows(const StDev &A, const XG &xg, EPI &
     }
 #pragma unroll
     for (int s = 0; s < 27; ++s) {
-      const int64_t j = (int64_t)i + A.off[s];
-      const uint32_t jc = j < 0 ? 0u : (j >= (int64_t)n ? (uint32_t)(n - 1) : (uint32_t)j);
-      xs[s] = xg(jc, 0);
+      if constexpr (PER) {
+        xs[s] = xg(periodic_nbr(A, (uint32_t)i, s), 0);
+      } else {
+        const int64_t j = (int64_t)i + A.off[s];
+        const uint32_t jc = j < 0 ? 0u : (j >= (int64_t)n ? (uint32_t)(n - 1) : (uint32_t)j);
+        xs[s] = xg(jc, 0);
+      }
     }
     T acc = 0;
 #pragma unroll
@@ -753,13 +769,13 @@ __device__ __forceinline__ void stencil_rows(const StDev &A, const XG &xg, EPI &
   }
 }
 
-template <int PREC>
+template <int PREC, bool PER = false>
 __global__ void __launch_bounds__(256) k_stmv(StDev A, const typename Val<PREC>::T *__restrict__ x,
                                              typename Val<PREC>::T *__restrict__ y) {
   using T = typename Val<PREC>::T;
   XPlain<T> xg{x, 1};
   EpiStore<T, 1> epi{y};
-  stencil_rows<PREC>(A, xg, epi);
+  stencil_rows<PREC, XPlain<T>, EpiStore<T, 1>, PER>(A, xg, epi);
 }
 
 template <int IP>
@@ -947,6 +963,9 @@ StDev st_dev(const smh_stencil27 *S, const void *vals, const void *vals_lo) {
         d.off[q++] = d0 + (int32_t)S->n0 * d1 + (int32_t)(S->n0 * S->n1) * d2;
   d.vals = vals;
   d.vals_lo = vals_lo;
+  d.n0 = S->n0;
+  d.n1 = S->n1;
+  d.n2 = S->n2;
   return d;
 }
 
@@ -2412,6 +2431,8 @@ int check_stencil(smh_ctx *ctx, const smh_stencil27 *S, const char *who) {
             "%s: unknown precision %d", who, S->precision);
   SMH_CHECK(ctx, S->precision != SMH_F32X2 || S->vals_lo, SMH_ERR_INVALID,
             "%s: SMH_F32X2 needs vals_lo", who);
+  SMH_CHECK(ctx, (S->flags & ~SMH_STENCIL_PERIODIC) == 0, SMH_ERR_INVALID, "%s: unknown flags %u",
+            who, S->flags);
   return SMH_OK;
 }
 }  // namespace
@@ -2494,6 +2515,18 @@ extern "C" int smh_stencil27_mv(smh_ctx *ctx, const smh_stencil27 *S, const void
   SMH_CHECK(ctx, x && y, SMH_ERR_INVALID, "smh_stencil27_mv: null vector");
   const StDev D = st_dev(S, S->vals, S->vals_lo);
   const unsigned g = grid_for(D.n, 256, 4096);
+  if (S->flags & SMH_STENCIL_PERIODIC) {
+    SMH_CHECK(ctx, S->precision == SMH_F64 || S->precision == SMH_F32, SMH_ERR_UNSUPPORTED,
+              "smh_stencil27_mv: periodic stencils hold SMH_F64 or SMH_F32 values");
+    if (S->precision == SMH_F64)
+      hipLaunchKernelGGL((k_stmv<SMH_F64, true>), dim3(g), dim3(256), 0, ctx->stream, D,
+                         (const double *)x, (double *)y);
+    else
+      hipLaunchKernelGGL((k_stmv<SMH_F32, true>), dim3(g), dim3(256), 0, ctx->stream, D,
+                         (const float *)x, (float *)y);
+    SMH_LAUNCH_CHECK(ctx);
+    return SMH_OK;
+  }
   if (S->precision == SMH_F64)
     hipLaunchKernelGGL((k_stmv<SMH_F64>), dim3(g), dim3(256), 0, ctx->stream, D,
                        (const double *)x, (double *)y);
@@ -2532,6 +2565,8 @@ extern "C" int smh_cg_solve_stencil27(smh_ctx *ctx, const smh_stencil27 *S, cons
   int rc = check_stencil(ctx, S, "smh_cg_solve_stencil27");
   if (rc) return rc;
   SMH_CHECK(ctx, b && x && o, SMH_ERR_INVALID, "smh_cg_solve_stencil27: null argument");
+  SMH_CHECK(ctx, !(S->flags & SMH_STENCIL_PERIODIC), SMH_ERR_UNSUPPORTED,
+            "smh_cg_solve_stencil27: periodic stencils are not supported by the solver");
   SMH_CHECK(ctx, o->precond >= 0 && o->precond <= 3, SMH_ERR_INVALID,
             "smh_cg_solve_stencil27: unknown preconditioner %d", o->precond);
   SMH_CHECK(ctx,

@@ -640,6 +640,140 @@ static int extern_file(const std::string &dir) {
   return failures ? 1 : 0;
 }
 
+// test/system-gemv-tests.cpp, run from this backend's own programs in
+// tests/programs/: c = A b on an edge set (:12-58), on a periodic 1-D grid
+// edge set (:60-110) and on a periodic 3 x 2 grid (:171-251); the inputs are
+// kept and the outputs are exact.
+static Function loadProgram(const std::string &file) {
+  Program p;
+  if (p.loadFile(file) != 0) {
+    std::printf("  FAIL: cannot load %s: %s\n", file.c_str(), p.getDiagnostics().c_str());
+    ++failures;
+    return Function();
+  }
+  return p.compile("main");
+}
+static void sim_gemv() {
+  Set points;
+  FieldRef<double> b = points.addField<double>("b");
+  FieldRef<double> c = points.addField<double>("c");
+  ElementRef p0 = points.add(), p1 = points.add(), p2 = points.add();
+  b.set(p0, 1.0);
+  b.set(p1, 2.0);
+  b.set(p2, 3.0);
+  c.set(p0, 42.0);
+  c.set(p2, 42.0);
+  Set springs(points, points);
+  FieldRef<double> a = springs.addField<double>("a");
+  a.set(springs.add(p0, p1), 1.0);
+  a.set(springs.add(p1, p2), 2.0);
+  Function func = loadProgram("tests/programs/gemv.sim");
+  if (!func.defined()) return;
+  func.bind("points", &points);
+  func.bind("springs", &springs);
+  func.runSafe();
+  EXPECT_NEAR(1.0, b.get(p0), 0.0);
+  EXPECT_NEAR(2.0, b.get(p1), 0.0);
+  EXPECT_NEAR(3.0, b.get(p2), 0.0);
+  EXPECT_NEAR(3.0, c.get(p0), 0.0);
+  EXPECT_NEAR(13.0, c.get(p1), 0.0);
+  EXPECT_NEAR(10.0, c.get(p2), 0.0);
+}
+static void sim_gemv_grid() {
+  Set points;
+  FieldRef<double> b = points.addField<double>("b");
+  FieldRef<double> c = points.addField<double>("c");
+  Set springs(points, {3});
+  FieldRef<double> a = springs.addField<double>("a");
+  ElementRef p0 = springs.getGridPoint({0}), p1 = springs.getGridPoint({1}),
+             p2 = springs.getGridPoint({2});
+  b.set(p0, 1.0);
+  b.set(p1, 2.0);
+  b.set(p2, 3.0);
+  c.set(p0, 42.0);
+  c.set(p2, 42.0);
+  a.set(springs.getGridEdge({0}, 0), 1.0);
+  a.set(springs.getGridEdge({1}, 0), 2.0);
+  a.set(springs.getGridEdge({2}, 0), 0.0);
+  Function func = loadProgram("tests/programs/grid_gemv.sim");
+  if (!func.defined()) return;
+  func.bind("points", &points);
+  func.bind("springs", &springs);
+  func.runSafe();
+  EXPECT_NEAR(1.0, b.get(p0), 0.0);
+  EXPECT_NEAR(2.0, b.get(p1), 0.0);
+  EXPECT_NEAR(3.0, b.get(p2), 0.0);
+  EXPECT_NEAR(3.0, c.get(p0), 0.0);
+  EXPECT_NEAR(13.0, c.get(p1), 0.0);
+  EXPECT_NEAR(10.0, c.get(p2), 0.0);
+}
+static void sim_gemv_grid_2d() {
+  Set points;
+  FieldRef<double> b = points.addField<double>("b");
+  FieldRef<double> c = points.addField<double>("c");
+  Set springs(points, {3, 2});
+  FieldRef<double> a = springs.addField<double>("a");
+  std::vector<ElementRef> p;
+  for (int y = 0; y < 2; ++y)
+    for (int x = 0; x < 3; ++x) p.push_back(springs.getGridPoint({x, y}));
+  for (int i = 0; i < 6; ++i) b.set(p[i], 1.0 + i);
+  c.set(p[0], 42.0);
+  c.set(p[5], 42.0);
+  // weights 1..6 on the x-direction links, 7..12 on the y-direction links
+  for (int dir = 0; dir < 2; ++dir)
+    for (int y = 0; y < 2; ++y)
+      for (int x = 0; x < 3; ++x) a.set(springs.getGridEdge({x, y}, dir), 1.0 + 6 * dir + 3 * y + x);
+  Function func = loadProgram("tests/programs/grid_gemv_2d.sim");
+  if (!func.defined()) return;
+  func.bind("points", &points);
+  func.bind("springs", &springs);
+  func.runSafe();
+  const double want[6] = {100.0, 146.0, 211.0, 181.0, 224.0, 304.0};
+  for (int i = 0; i < 6; ++i) {
+    EXPECT_NEAR(1.0 + i, b.get(p[i]), 0.0);
+    EXPECT_NEAR(want[i], c.get(p[i]), 0.0);
+  }
+}
+// the grid edge set API on the host (graph.h:121-240)
+static void grid_set() {
+  Set points;
+  points.addField<double>("b");
+  Set grid(points, {3, 2});
+  EXPECT_TRUE(points.getSize() == 6 && grid.getSize() == 12 && grid.getCardinality() == 2);
+  EXPECT_TRUE(grid.getKind() == Set::Grid && points.getKind() == Set::Unstructured);
+  EXPECT_TRUE(grid.getDimensions() == std::vector<int>({3, 2}));
+  EXPECT_TRUE(grid.getGridPoint({2, 1}).getIdent() == 5);
+  EXPECT_TRUE(grid.getGridEdge({1, 1}, 1).getIdent() == 9);
+  EXPECT_TRUE(grid.getGridPointCoords(ElementRef(4)) == std::vector<int>({1, 1}));
+  // edge p*2 + k joins p to its periodic neighbour along k
+  EXPECT_TRUE(grid.getEndpoint(grid.getGridEdge({2, 0}, 0), 0).getIdent() == 2);
+  EXPECT_TRUE(grid.getEndpoint(grid.getGridEdge({2, 0}, 0), 1).getIdent() == 0);
+  EXPECT_TRUE(grid.getEndpoint(grid.getGridEdge({1, 1}, 1), 1).getIdent() == 1);
+  bool threw = false;
+  try {
+    grid.getGridPoint({3, 1});
+  } catch (const SimitException &) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+  threw = false;
+  try {
+    Set again(points, {2});  // points is no longer empty
+  } catch (const SimitException &) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+  threw = false;
+  try {
+    std::vector<int> perm(12);
+    for (int i = 0; i < 12; ++i) perm[i] = 11 - i;
+    grid.permuteElements(perm);
+  } catch (const SimitException &) {
+    threw = true;
+  }
+  EXPECT_TRUE(threw);
+}
+
 // sim_fem_ops PROGRAM DIR: the bar2k set-up, PROGRAM's initializeTet and
 // operators (a parsed program that assembles main's force and stiffness and
 // writes y = A v); the fields the GPU test checks against the oracle are
@@ -740,7 +874,9 @@ int main(int argc, char **argv) {
       {"sim_isprings", [] { g_programs = true; isprings(); g_programs = false; }},
       {"sim_cg", [] { g_programs = true; cg(); g_programs = false; }},
       {"sim_fem_tet", [] { g_programs = true; fem_tet(); g_programs = false; }},
-      {"sim_reorder_fem_tet", [] { g_programs = true; reorder_fem_tet(); g_programs = false; }}};
+      {"sim_reorder_fem_tet", [] { g_programs = true; reorder_fem_tet(); g_programs = false; }},
+      {"sim_gemv", sim_gemv}, {"sim_gemv_grid", sim_gemv_grid},
+      {"sim_gemv_grid_2d", sim_gemv_grid_2d}, {"grid_set", grid_set}};
   for (auto &t : tests) {
     bool run = argc <= 1;
     for (int i = 1; i < argc; ++i) run |= t.first == argv[i];

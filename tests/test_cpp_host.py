@@ -11,7 +11,7 @@ BIN = ROOT / "simit_amd" / "bin" / "simit_program_tests"
 
 
 @pytest.mark.parametrize("name", ["reorder_int", "reorder_double", "hilbert_curve",
-                                  "reorder_bunny_host", "mesh_load"])
+                                  "reorder_bunny_host", "mesh_load", "grid_set"])
 def test_cpp_host(name):
     if not BIN.exists():
         pytest.skip(f"{BIN} is not built (run __graft_entry__.build())")

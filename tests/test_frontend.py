@@ -36,6 +36,9 @@ def _built():
     ("springs.sim", ["main"]),
     ("cg_points.sim", ["main"]),
     ("neohookean.sim", ["initializeTet", "main"]),
+    ("gemv.sim", ["main"]),
+    ("grid_gemv.sim", ["main"]),
+    ("grid_gemv_2d.sim", ["main"]),
 ])
 def test_own_programs_compile(name, functions):
     rc, out = compile_program(ROOT / "tests" / "programs" / name)
@@ -54,6 +57,11 @@ REF_PROGRAMS = [
     "apps/fem/fem_neohookean.sim",
     "apps/springs/isprings.sim",
     "apps/springs/esprings.sim",
+    "test/input/system/gemv.sim",
+    "test/input/system/gemv_stencil.sim",
+    "test/input/system/gemv_stencil_2d.sim",
+    "test/input/system/gemv_stencil_indexless.sim",
+    "test/input/system/gemv_stencil_2d_indexless.sim",
 ]
 
 
@@ -92,3 +100,21 @@ def test_unsupported_or_wrong_element_code_is_rejected(tmp_path, body, message):
     rc, out = compile_program(None, "main", text=text, tmp=tmp_path)
     assert rc == 1, out
     assert message in out, out
+
+
+def test_grid_reads_are_checked(tmp_path):
+    """A link must join two neighbouring grid points; grid matrices are
+    written in their own row within one step (27-point stencil)."""
+    head = ("element P\n  b : float;\nend\nelement L\n  a : float;\nend\n"
+            "extern ps : set{P};\nextern g : grid[2]{L}(ps);\n")
+    tail = "export func main()\n  B = map f to ps through g;\n  ps.b = B*ps.b;\nend\n"
+    cases = [
+        ("  B(p,p) = l[0,0;1,1].a;\n", "neighbouring grid points"),
+        ("  B(p,ps[2,0]) = l[0,0;1,0].a;\n", "within one step"),
+        ("  B(ps[1,0],p) = l[0,0;1,0].a;\n", "own row"),
+    ]
+    for body, message in cases:
+        text = (head + "func f(p : P, l : grid[2]{L}(ps)) -> (B : tensor[ps,ps](float))\n" + body +
+                "end\n" + tail)
+        rc, out = compile_program(None, "main", text=text, tmp=tmp_path)
+        assert rc == 1 and message in out, out

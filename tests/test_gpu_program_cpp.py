@@ -21,7 +21,8 @@ BIN = ROOT / "simit_amd" / "bin" / "simit_program_tests"
 @pytest.mark.parametrize("name", ["isprings", "cg", "springs_lattice", "mesh_bunny",
                                   "reorder_bunny_springs", "extern_csr", "fem_tet",
                                   "reorder_fem_tet", "sim_isprings", "sim_cg", "sim_fem_tet",
-                                  "sim_reorder_fem_tet"])
+                                  "sim_reorder_fem_tet", "sim_gemv", "sim_gemv_grid",
+                                  "sim_gemv_grid_2d"])
 def test_cpp_program(name):
     import torch
     if not torch.cuda.is_available():

@@ -167,3 +167,38 @@ def test_stencil_learned_correction(ctx, mixed):
     st = ctx.cg_stencil27(St, sy["rhs"], x, o)
     assert st.status == 0 and st.true_rel <= 1e-8, st.as_dict()
     net.close()
+
+
+@pytest.mark.parametrize("dims", [(3, 1, 1), (3, 2, 1), (5, 4, 3), (2, 2, 2), (1, 1, 1), (17, 9, 6)])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_periodic_stencil_product(ctx, dims, prec):
+    """Stencil storage with SMH_STENCIL_PERIODIC (the matrices of maps
+    through a grid edge set): neighbour coordinates wrap around, every slot
+    is its own term (slots that reach the same point on a grid of size 1 or
+    2 are not merged), the 27 products summed in slot order from 0."""
+    import ctypes as C
+    n0, n1, n2 = dims
+    n = n0 * n1 * n2
+    rng = np.random.default_rng(sum(dims))
+    dt = np.float64 if prec == "f64" else np.float32
+    vals = rng.standard_normal((27, n)).astype(dt)
+    x = rng.standard_normal(n).astype(dt)
+    i = np.arange(n)
+    c = np.stack([i % n0, (i // n0) % n1, i // (n0 * n1)])
+    want = np.zeros(n, dtype=dt)
+    for s in range(27):
+        o = np.array([s % 3 - 1, s // 3 % 3 - 1, s // 9 - 1])
+        w = (c + o[:, None]) % np.array(dims)[:, None]
+        j = w[0] + n0 * (w[1] + n1 * w[2])
+        want = (want + vals[s] * x[j]).astype(dt)
+    dv, dx = to_dev(vals.ravel()), to_dev(x)
+    y = torch.empty_like(dx)
+    S_ = S.Stencil27(n0, n1, n2, S.F64 if prec == "f64" else S.F32, dv.data_ptr(), None,
+                     S.STENCIL_PERIODIC)
+    ctx.stencil27_mv(S_, dx, y)
+    np.testing.assert_array_equal(y.cpu().numpy(), want)
+    # the solver refuses a periodic stencil
+    with pytest.raises(S.SimitError):
+        ctx.cg_stencil27(S_, to_dev(x.astype(np.float64)), torch.zeros(n, dtype=torch.float64,
+                                                                        device="cuda"),
+                         ctx.cg_options(maxiters=5))
