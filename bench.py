@@ -188,7 +188,8 @@ def main():
     tr = learned.train(ctx, net, n=args.train_n, nsamples=args.train_samples, log_every=500,
                        log=log)
     log(f"[bench] trained on {tr['steps']} samples of {args.train_n}^3 in {tr['seconds']:.1f}s "
-        f"(host field draws {tr['sample_seconds']:.1f}s, GPU steps {tr['step_seconds']:.1f}s)")
+        f"(samples drawn and assembled on the device {tr['sample_seconds']:.1f}s, "
+        f"training steps {tr['step_seconds']:.1f}s)")
 
     def opts(**kw):
         return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
@@ -304,7 +305,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 matrix/SpMV, fp64 dots+refinement, bf16 network",
-        "data": "synthetic (seeded Gaussian random field material, generated on the host)",
+        "data": ("synthetic (seeded Gaussian random field material: the solved system's "
+                 "generated on the host, the training samples' on the device)"),
         "config": {"workload": f"tet5 linear elasticity {n}^3 cubes x 5 tets, {ndof} DOF, "
                                f"log10 E = 5 + GRF(std 1, corr 8 cells), learned mixed PCG "
                                f"to a 1e-8 FP64 relative residual",
