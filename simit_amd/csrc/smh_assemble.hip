@@ -21,6 +21,7 @@
 // 64 results are staged through LDS so the stores are coalesced.  No atomics,
 // so repeated runs are bitwise identical.
 #include <algorithm>
+#include <type_traits>
 #include <cstdlib>
 
 #include "smh_internal.h"
@@ -36,6 +37,9 @@ constexpr int kWave = 64;
 // (LDS, one per staged pair) made for the row's endpoint slot a; block()
 // forms the row's block with endpoint slot b from a Pair; extra() and rhs()
 // the diagonal extra term and the right-hand-side contribution of the row.
+// The *_entry() forms give one value q of a block (q = r*BD + s) or one
+// component r of the right-hand side with exactly the arithmetic of the full
+// forms, for the lanes that sum a diagonal block entry by entry.
 
 // test/input/program/isprings.sim:19-69
 struct SpringArgs {
@@ -127,6 +131,31 @@ struct SpringOp {
       f[q] = mass * A.v[3 * (int64_t)i + q] + A.p.h * mass * A.p.grav[q] + 0.0;
   }
   __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
+  __device__ static bool block_entry(const Args &, const Pair &P, int b, int q, double &v) {
+    if (!P.fa || (P.a != b && !P.fb)) return false;
+    v = P.a == b ? P.kb[q] : -P.kb[q];
+    return true;
+  }
+  __device__ static bool extra_entry(const Args &, const Pair &, int, double &) { return false; }
+  __device__ static void rhs_entry(const Args &, const Pair &P, int r, double &f) {
+    if (!P.fa) return;
+    f += P.a == 0 ? P.fe0[r] : -P.fe0[r];
+  }
+  __device__ static void finish_diag_entry(const Args &A, uint32_t i, int q, double &v) {
+    double md;
+    if (A.fixed[i]) md = (A.p.fixed_identity && (q % 4 != 0)) ? 0.0 : A.p.fixed_fill;
+    else md = (q % 4 == 0) ? A.m[i] + A.p.h * A.p.viscous : 0.0;
+    v = md + v;
+  }
+  __device__ static void finish_rhs_entry(const Args &A, uint32_t i, int r, double &f) {
+    double fmg = 0.0;
+    if (!A.fixed[i]) {
+      const double mass = A.m[i];
+      fmg = mass * A.v[3 * (int64_t)i + r] + A.p.h * mass * A.p.grav[r];
+    }
+    f = fmg + f;
+  }
+  __device__ static void finish_block_entry(const Args &, uint32_t, uint32_t, int, double &) {}
 };
 
 // apps/fem/fem_linear.sim:95-194
@@ -173,15 +202,23 @@ struct TetOp {
     double dN[12];
     double W, mu, lam;
   };
-  // u = (s*W*lam) g_a and w = (s*W*mu) g_a for the row's slot a, and all four
-  // shape gradients g_b
+  // the four shape gradients, the row's own g_a first and then the other
+  // slots ascending; the row's slot a and the coefficients cl = s*W*lam,
+  // cm = s*W*mu of u = cl g_a and w = cm g_a (144 bytes)
+  static constexpr uint32_t kFa = 16u;
   struct Pair {
     int32_t vid[4];
-    uint32_t freemask;
-    int32_t fa;
-    double u[3], w[3];
+    uint32_t bits;  // freemask (bits 0-3), fa (bit 4), a (bits 8-9)
+    int32_t pad;
+    double cl, cm, W;
     double g[12];
-    double W;
+    __device__ uint32_t freemask() const { return bits & 15u; }
+    __device__ bool fa() const { return bits & kFa; }
+    __device__ int a() const { return (int)((bits >> 8) & 3u); }
+    // where slot b's gradient is stored
+    __device__ int pos(int b) const { return b == a() ? 0 : (b < a() ? b + 1 : b); }
+    __device__ double u(int i) const { return cl * g[i]; }
+    __device__ double w(int i) const { return cm * g[i]; }
   };
   __device__ static void load(const Args &A, uint64_t e, State &st) {
     double xs[12];
@@ -211,25 +248,31 @@ struct TetOp {
       P.vid[t] = st.vid[t];
       fm |= (st.free_[t] ? 1u : 0u) << t;
     }
-    P.freemask = fm;
-    P.fa = (fm >> a) & 1u;
-    const double sw = A.p.stiff_scale * st.W, cl = sw * st.lam, cm = sw * st.mu;
-    for (int i = 0; i < 3; ++i) {
-      const double ga = pick4(st.dN + i, a, 3);
-      P.u[i] = cl * ga;
-      P.w[i] = cm * ga;
-    }
-    for (int q = 0; q < 12; ++q) P.g[q] = st.dN[q];
+    P.bits = fm | (((fm >> a) & 1u) << 4) | ((uint32_t)a << 8);
+    const double sw = A.p.stiff_scale * st.W;
+    P.cl = sw * st.lam;
+    P.cm = sw * st.mu;
     P.W = st.W;
+    for (int i = 0; i < 3; ++i) P.g[i] = pick4(st.dN + i, a, 3);
+    for (int t = 0; t < 4; ++t) {
+      if (t == a) continue;
+      const int o = 3 * (t < a ? t + 1 : t);
+      for (int i = 0; i < 3; ++i) P.g[o + i] = st.dN[3 * t + i];
+    }
   }
   // blk[i][j] = u_i gb_j + w_j gb_i + [i==j] (w . gb)  (s * W * B'EB, closed form)
   __device__ static bool block(const Args &, const Pair &P, int b, double *out) {
-    if (!P.fa || !((P.freemask >> b) & 1u)) return false;
-    const double *gb = P.g + 3 * b;
-    const double wg = P.w[0] * gb[0] + P.w[1] * gb[1] + P.w[2] * gb[2];
+    if (!P.fa() || !((P.freemask() >> b) & 1u)) return false;
+    const double *gb = P.g + 3 * P.pos(b);
+    double u[3], w[3];
+    for (int i = 0; i < 3; ++i) {
+      u[i] = P.u(i);
+      w[i] = P.w(i);
+    }
+    const double wg = w[0] * gb[0] + w[1] * gb[1] + w[2] * gb[2];
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
-        double t = P.u[i] * gb[j] + P.w[j] * gb[i];
+        double t = u[i] * gb[j] + w[j] * gb[i];
         if (i == j) t = t + wg;
         out[3 * i + j] = t;
       }
@@ -243,7 +286,7 @@ struct TetOp {
     return true;
   }
   __device__ static void rhs(const Args &A, const Pair &P, double *f) {
-    if (!P.fa) return;
+    if (!P.fa()) return;
     const double wq = 0.25 * P.W;
     for (int q = 0; q < 3; ++q) f[q] += wq * A.p.load[q];
   }
@@ -255,7 +298,72 @@ struct TetOp {
     f[0] = f[1] = f[2] = 0.0;
   }
   __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
+  __device__ static bool block_entry(const Args &, const Pair &P, int b, int q, double &v) {
+    if (!P.fa() || !((P.freemask() >> b) & 1u)) return false;
+    const double *gb = P.g + 3 * P.pos(b);
+    const int i = q / 3, j = q - 3 * (q / 3);
+    double t = P.u(i) * gb[j] + P.w(j) * gb[i];
+    if (i == j) t = t + (P.w(0) * gb[0] + P.w(1) * gb[1] + P.w(2) * gb[2]);
+    v = t;
+    return true;
+  }
+  __device__ static bool extra_entry(const Args &A, const Pair &P, int q, double &v) {
+    if (A.p.rho == 0.0) return false;
+    v = (q % 4 == 0) ? 0.25 * A.p.rho * P.W : 0.0;
+    return true;
+  }
+  __device__ static void rhs_entry(const Args &A, const Pair &P, int r, double &f) {
+    if (!P.fa()) return;
+    // (selects keep the kernel argument a uniform load for a per-lane r)
+    const double L = r == 0 ? A.p.load[0] : (r == 1 ? A.p.load[1] : A.p.load[2]);
+    f += 0.25 * P.W * L;
+  }
+  __device__ static void finish_diag_entry(const Args &A, uint32_t i, int q, double &v) {
+    if (A.p.fixed_diag == 0.0 || A.c[i] <= 0) return;
+    v = ((q % 4 == 0) ? A.p.fixed_diag : 0.0) + v;
+  }
+  __device__ static void finish_rhs_entry(const Args &, uint32_t, int, double &) {}
+  __device__ static void finish_block_entry(const Args &, uint32_t, uint32_t, int, double &) {}
+  // After the off-diagonal blocks are formed, the pair's record is turned
+  // in place into its writes to the row's diagonal: g[q] = block(a, a)[q]
+  // (0 when not written), g[9 + r] = the rhs term of component r, cl = the
+  // lumped mass (added to q = 0, 4, 8 when rho != 0).  Same arithmetic as
+  // block()/extra()/rhs(); adding the 0 of an unwritten term leaves a sum
+  // unchanged (a sum that starts at +0 is never -0).
+  __device__ static void to_diag(const Args &A, Pair &P) {
+    const double cl = P.cl, cm = P.cm, W = P.W;
+    const bool fa = P.bits & kFa;
+    const double g0 = P.g[0], g1 = P.g[1], g2 = P.g[2];
+    const double g[3] = {g0, g1, g2};
+    const double wg = (cm * g0) * g0 + (cm * g1) * g1 + (cm * g2) * g2;
+    double D[9];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double t = (cl * g[i]) * g[j] + (cm * g[j]) * g[i];
+        if (i == j) t = t + wg;
+        D[3 * i + j] = fa ? t : 0.0;
+      }
+    const double wq = 0.25 * W;
+    for (int q = 0; q < 9; ++q) P.g[q] = D[q];
+    for (int r = 0; r < 3; ++r) P.g[9 + r] = fa ? wq * A.p.load[r] : 0.0;
+    P.cl = 0.25 * A.p.rho * W;
+  }
+  // entry q of the diagonal (and rhs component q/4 for q = 0, 4, 8) from a
+  // record made by to_diag
+  __device__ static void diag_sum(const Args &A, const Pair &P, int q, double &acc, double &f) {
+    acc += P.g[q];
+    if (q % 4 == 0) {
+      if (A.p.rho != 0.0) acc += P.cl;
+      f += P.g[9 + q / 4];
+    }
+  }
 };
+
+// ops that turn a record into its diagonal terms (to_diag / diag_sum)
+template <class Op, class = void>
+struct HasToDiag : std::false_type {};
+template <class Op>
+struct HasToDiag<Op, std::void_t<decltype(&Op::to_diag)>> : std::true_type {};
 
 // scalar trilinear hex Poisson, corner a = dx + 2dy + 4dz
 struct HexArgs {
@@ -312,6 +420,17 @@ struct HexOp {
   }
   __device__ static void isolated_rhs(const Args &, uint32_t, double *f) { f[0] = 0.0; }
   __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
+  __device__ static bool block_entry(const Args &A, const Pair &P, int b, int, double &v) {
+    return block(A, P, b, &v);
+  }
+  __device__ static bool extra_entry(const Args &, const Pair &, int, double &) { return false; }
+  __device__ static void rhs_entry(const Args &A, const Pair &P, int, double &f) { rhs(A, P, &f); }
+  __device__ static void finish_diag_entry(const Args &A, uint32_t i, int, double &v) {
+    if (A.dir[i] <= 0) return;
+    v = A.p.fixed_diag + v;
+  }
+  __device__ static void finish_rhs_entry(const Args &, uint32_t, int, double &) {}
+  __device__ static void finish_block_entry(const Args &, uint32_t, uint32_t, int, double &) {}
 };
 
 // one per-edge block written into the four endpoint blocks with signs, then
@@ -349,8 +468,14 @@ struct EdgeOp {
     P.a = a;
     for (int q = 0; q < BDIM * BDIM; ++q) P.w[q] = st.w[q];
   }
+  // A.sign[k] by selects (a uniform kernel-argument load for a per-lane k)
+  __device__ static double sign_of(const Args &A, int k) {
+    const double s01 = k == 0 ? A.sign[0] : A.sign[1];
+    const double s23 = k == 2 ? A.sign[2] : A.sign[3];
+    return k < 2 ? s01 : s23;
+  }
   __device__ static bool block(const Args &A, const Pair &P, int b, double *out) {
-    const double s = A.sign[2 * P.a + b];
+    const double s = sign_of(A, 2 * P.a + b);
     for (int q = 0; q < BDIM * BDIM; ++q) out[q] = s * P.w[q];
     return true;
   }
@@ -365,6 +490,20 @@ struct EdgeOp {
     for (int i = 0; i < BDIM; ++i)
       for (int j = 0; j < BDIM; ++j)
         blk[i * BDIM + j] = ((row == col && i == j) ? A.alpha : 0.0) + A.beta * blk[i * BDIM + j];
+  }
+  __device__ static bool block_entry(const Args &A, const Pair &P, int b, int q, double &v) {
+    v = sign_of(A, 2 * P.a + b) * P.w[q];
+    return true;
+  }
+  __device__ static bool extra_entry(const Args &, const Pair &, int, double &) { return false; }
+  __device__ static void rhs_entry(const Args &, const Pair &, int, double &) {}
+  __device__ static void finish_diag_entry(const Args &, uint32_t, int, double &) {}
+  __device__ static void finish_rhs_entry(const Args &, uint32_t, int, double &) {}
+  __device__ static void finish_block_entry(const Args &A, uint32_t row, uint32_t col, int q,
+                                            double &v) {
+    if (!A.affine) return;
+    const int i = q / BDIM, j = q - BDIM * (q / BDIM);
+    v = ((row == col && i == j) ? A.alpha : 0.0) + A.beta * v;
   }
 };
 
@@ -535,6 +674,22 @@ struct FemTetOp {
     f[0] = f[1] = f[2] = 0.0;
   }
   __device__ static void finish_block(const Args &, uint32_t, uint32_t, double *) {}
+  __device__ static bool block_entry(const Args &, const Pair &P, int b, int q, double &v) {
+    if (!P.fa || !((P.freemask >> b) & 1u)) return false;
+    v = P.blk[b][q];
+    return true;
+  }
+  __device__ static bool extra_entry(const Args &, const Pair &P, int q, double &v) {
+    v = P.mq * ((q % 4 == 0) ? 1.0 : 0.0);
+    return true;
+  }
+  __device__ static void rhs_entry(const Args &, const Pair &P, int r, double &f) {
+    if (!P.fa) return;
+    for (int k = 0; k < P.nf; ++k) f += P.fw[k][r];
+  }
+  __device__ static void finish_diag_entry(const Args &, uint32_t, int, double &) {}
+  __device__ static void finish_rhs_entry(const Args &, uint32_t, int, double &) {}
+  __device__ static void finish_block_entry(const Args &, uint32_t, uint32_t, int, double &) {}
 };
 
 // initializeTet (femTet.sim:126-148): per tet B = 1.0*inv(M), W = -(1/6)
@@ -564,6 +719,13 @@ __global__ void k_femtet_mass(uint32_t nv, double rho, const uint32_t *__restric
 }
 
 // ---- the pull kernel --------------------------------------------------------
+// A wave the fast kernel cannot take: marked by the plan (nd == 255: its
+// rows do not fit the 64-row window, a row has more pairs than mask bits, an
+// element holds a row twice), or its diagonal blocks have more than 128
+// entries to sum.
+__device__ __forceinline__ bool generic_wave(uint32_t nd, int bb) {
+  return nd == 255u || nd * (uint32_t)bb > 2u * kWave;
+}
 template <int PREC>
 __device__ __forceinline__ void store_val(void *vals, void *lo, uint64_t i, double v) {
   if (PREC == SMH_F64) {
@@ -584,7 +746,7 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
            const uint32_t *__restrict__ wave_row,
            const uint32_t *__restrict__ ve_ptr, const uint32_t *__restrict__ ve_elem,
            void *__restrict__ vals, void *__restrict__ vals_lo, double *__restrict__ rhs,
-           uint64_t nwaves) {
+           uint64_t nwaves, const uint8_t *__restrict__ wnd) {
   constexpr int BD = Op::BD, BB = BD * BD, CARD = Op::CARD;
   using Pair = typename Op::Pair;
   // per staged pair: its record and flags (bit 0: the row's slot was found);
@@ -605,6 +767,7 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
   const uint32_t xcd = blockIdx.x & 7u, per = gridDim.x >> 3;
   const uint64_t w1 = nwaves * (xcd + 1) / 8;
   for (uint64_t w = nwaves * xcd / 8 + (blockIdx.x >> 3); w < w1; w += per) {
+  if (wnd && !generic_wave(wnd[w], BB)) continue;  // the fast kernel's wave
   const uint64_t b0 = w * kWave;
   const uint64_t bk = b0 + lane;
   const bool valid = bk < nnz;
@@ -741,6 +904,297 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
   }
 }
 
+// ---- the per-pattern plan and the fast kernel ----------------------------------
+// Row i's pairs are ve_elem[ve_ptr[i] .. ve_ptr[i+1]), elements ascending (an
+// element holding i k times is listed k times in a row).  Per pair: the
+// endpoint slot it stands for (the occ-th slot equal to i; bit 7 set when
+// the element holds i more than once).  Per
+// off-diagonal block (i, j): bit k set when the element of the row's k-th
+// pair holds j.  One thread per row, which owns the row's blocks.
+template <typename MT>
+__global__ void k_asm_plan(uint32_t nrows, uint32_t card, const int32_t *__restrict__ ep,
+                           const uint32_t *__restrict__ rowptr,
+                           const uint32_t *__restrict__ colidx,
+                           const uint32_t *__restrict__ ve_ptr,
+                           const uint32_t *__restrict__ ve_elem, uint8_t *__restrict__ slot,
+                           MT *__restrict__ mask) {
+  constexpr uint32_t kBits = 8 * sizeof(MT);
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrows) return;
+  const uint32_t c0 = rowptr[i], c1 = rowptr[i + 1];
+  for (uint32_t q = c0; q < c1; ++q) mask[q] = 0;
+  const uint32_t rb = ve_ptr[i], re = ve_ptr[i + 1];
+  uint32_t prev = 0xffffffffu;
+  uint32_t occ = 0;
+  for (uint32_t p = rb; p < re; ++p) {
+    const uint32_t e = ve_elem[p];
+    occ = e == prev ? occ + 1 : 0;
+    prev = e;
+    const int32_t *v = ep + (uint64_t)e * card;
+    uint32_t a = 255, seen = 0;
+    for (uint32_t t = 0; t < card; ++t)
+      if ((uint32_t)v[t] == i) {
+        if (seen == occ) a = t;
+        ++seen;
+      }
+    // 0x80: the element holds the row more than once (its waves take the
+    // generic kernel)
+    slot[p] = (uint8_t)(seen > 1 && a < 128 ? a | 0x80u : a);
+    const uint32_t k = p - rb;
+    if (k >= kBits) continue;
+    for (uint32_t t = 0; t < card; ++t) {
+      const uint32_t j = (uint32_t)v[t];
+      if (j == i) continue;
+      uint32_t lo = c0, hi = c1;  // first position with colidx >= j
+      while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (colidx[mid] < j) lo = mid + 1; else hi = mid;
+      }
+      if (lo < c1 && colidx[lo] == j) mask[lo] |= (MT)1 << k;
+    }
+  }
+}
+
+__global__ void k_asm_maxdeg(uint32_t nrows, const uint32_t *__restrict__ ve_ptr,
+                             uint32_t *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nrows) atomicMax(out, ve_ptr[i + 1] - ve_ptr[i]);
+}
+
+// Per wave of 64 blocks: the number of diagonal blocks it holds, or 255 when
+// its rows span more than the 64-row window, a row has more pairs than the
+// mask has bits, or a pair has no slot or a repeated one.  cnt[0..2]: generic waves for 1x1, 2x2
+// and 3x3 blocks.
+__global__ void k_asm_waves(uint64_t nwaves, uint64_t nnz, uint32_t nrows, uint32_t bits,
+                            uint32_t card, const uint32_t *__restrict__ rowptr,
+                            const uint32_t *__restrict__ colidx,
+                            const uint32_t *__restrict__ wave_row,
+                            const uint32_t *__restrict__ ve_ptr,
+                            const uint8_t *__restrict__ slot, uint8_t *__restrict__ wnd,
+                            unsigned long long *__restrict__ cnt) {
+  const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= nwaves) return;
+  const uint64_t b0 = w * kWave, blast = min(b0 + kWave, nnz) - 1;
+  const uint32_t rfirst = wave_row[w];
+  uint32_t lo = rfirst, hi = nrows;  // last row with rowptr <= blast
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if ((uint64_t)rowptr[mid] <= blast) lo = mid; else hi = mid;
+  }
+  const uint32_t rlast = lo;
+  uint32_t nd = 0;
+  bool slow = rlast - rfirst + 1 > (uint32_t)kWave;
+  for (uint32_t r = rfirst; !slow && r <= rlast; ++r) {
+    if (ve_ptr[r + 1] - ve_ptr[r] > bits) slow = true;
+    // a pair without its slot (an index that does not match the endpoints),
+    // or of an element that holds the row more than once
+    for (uint32_t p = ve_ptr[r]; p < ve_ptr[r + 1]; ++p)
+      if (slot[p] >= card) slow = true;
+    uint32_t a = rowptr[r], b = rowptr[r + 1];
+    while (a < b) {
+      const uint32_t mid = a + (b - a) / 2;
+      if (colidx[mid] < r) a = mid + 1; else b = mid;
+    }
+    if (a < rowptr[r + 1] && colidx[a] == r && a >= b0 && a <= blast) ++nd;
+  }
+  wnd[w] = slow ? 255 : (uint8_t)nd;
+  for (int c = 0; c < 3; ++c)
+    if (generic_wave(slow ? 255u : nd, (c + 1) * (c + 1))) atomicAdd(cnt + c, 1ull);
+}
+
+template <typename MT>
+__device__ __forceinline__ int ctz_mask(MT m) {
+  return sizeof(MT) == 8 ? __builtin_ctzll((unsigned long long)m) : __builtin_ctz((unsigned)m);
+}
+
+// The fast kernel: the same waves of 64 blocks and the same sums as the
+// generic kernel, with the plan in place of searches.
+//   A. lane l builds the record of pair c0 + l from its element and slot;
+//   D. the diagonal blocks are summed one entry per lane (a lane for each of
+//      the BB entries of each diagonal in the wave, two rounds of 64): every
+//      pair of the row adds to every entry of the diagonal, so these sums are
+//      the long ones (a row's pair count) and need all the lanes;
+//   B. an off-diagonal block lane adds only the pairs its mask names (at most
+//      a few, the elements holding both vertices).
+// Ops with to_diag() run D after B: each pair lane first turns its record
+// into its diagonal terms (full lanes), so D is a load and an add per pair.
+// Waves with an element that holds a row twice take the generic kernel.
+// Every value is summed by one lane in the generic kernel's (the CPU loop's)
+// order with the same arithmetic, so the results are bit-identical.
+template <class Op, int PREC, typename MT>
+__global__ void __launch_bounds__(kWave)
+k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
+                const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ colidx,
+                const uint32_t *__restrict__ wave_row, const uint32_t *__restrict__ ve_ptr,
+                const uint32_t *__restrict__ ve_elem, const uint8_t *__restrict__ slot,
+                const MT *__restrict__ mask, const uint8_t *__restrict__ wnd,
+                void *__restrict__ vals, void *__restrict__ vals_lo, double *__restrict__ rhs,
+                uint64_t nwaves) {
+  constexpr int BD = Op::BD, BB = BD * BD, CARD = Op::CARD;
+  constexpr int64_t kBits = 8 * (int64_t)sizeof(MT);
+  using Pair = typename Op::Pair;
+  constexpr size_t kPairBytes = sizeof(Pair) * kWave;
+  constexpr size_t kOutBytes = sizeof(double) * kWave * BB;
+  __shared__ __align__(16) unsigned char smem[kPairBytes > kOutBytes ? kPairBytes : kOutBytes];
+  Pair *PP = reinterpret_cast<Pair *>(smem);
+  __shared__ uint32_t R[kWave + 1];   // rowptr of the wave's rows
+  __shared__ uint32_t VP[kWave + 1];  // ve_ptr of the wave's rows
+  __shared__ uint8_t DR[kWave];       // local row of the wave's d-th diagonal block
+  __shared__ uint8_t DL[kWave];       // and its lane
+
+  const int lane = threadIdx.x;
+  const uint32_t xcd = blockIdx.x & 7u, per = gridDim.x >> 3;
+  const uint64_t w1 = nwaves * (xcd + 1) / 8;
+  for (uint64_t w = nwaves * xcd / 8 + (blockIdx.x >> 3); w < w1; w += per) {
+    if (generic_wave(wnd[w], BB)) continue;
+    const uint64_t b0 = w * kWave, bk = b0 + lane;
+    const bool valid = bk < nnz;
+    const uint32_t rfirst = wave_row[w];
+    R[lane] = rowptr[min(rfirst + (uint32_t)lane, nrows)];
+    VP[lane] = ve_ptr[min(rfirst + (uint32_t)lane, nrows)];
+    if (lane == 0) {
+      R[kWave] = rowptr[min(rfirst + (uint32_t)kWave, nrows)];
+      VP[kWave] = ve_ptr[min(rfirst + (uint32_t)kWave, nrows)];
+    }
+    const uint32_t col = valid ? colidx[bk] : 0xffffffffu;
+    const MT bm = valid ? mask[bk] : (MT)0;
+    __syncthreads();
+    // local row: last l with R[l] <= bk (the plan put the wave's rows in the window)
+    int lr = 0;
+    {
+      int hi = kWave + 1;
+      while (hi - lr > 1) {
+        const int mid = (lr + hi) >> 1;
+        if ((uint64_t)R[mid] <= bk) lr = mid; else hi = mid;
+      }
+    }
+    if (!valid) lr = 0;
+    const uint32_t row = rfirst + (uint32_t)lr;
+    const bool is_diag = valid && col == row;
+    int lrl = lr;
+    for (int off = 32; off > 0; off >>= 1) lrl = max(lrl, __shfl_xor(lrl, off, 64));
+    const uint32_t P0 = VP[0], P1 = VP[lrl + 1];
+    const uint32_t myb = VP[lr];
+    const unsigned long long dball = __ballot(is_diag);
+    const int nd = __popcll(dball);
+    if (is_diag) {
+      const int rank = __popcll(dball & ((1ull << lane) - 1ull));
+      DR[rank] = (uint8_t)lr;
+      DL[rank] = (uint8_t)lane;
+    }
+    __syncthreads();
+    // diagonal-entry roles: t = g*64 + lane is entry t % BB of diagonal t / BB
+    bool dact[2];
+    int dq[2];
+    uint32_t drow[2], db[2], de[2];
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      const int t = g * kWave + lane, d = t / BB;
+      dact[g] = d < nd;
+      dq[g] = t - d * BB;
+      const int l = dact[g] ? DR[d] : 0;
+      drow[g] = rfirst + (uint32_t)l;
+      db[g] = VP[l];
+      de[g] = VP[l + 1];
+    }
+
+    double acc[BB];
+    for (int q = 0; q < BB; ++q) acc[q] = 0.0;
+    double a2[2] = {0.0, 0.0}, f2[2] = {0.0, 0.0};
+    for (uint32_t c0 = P0; c0 < P1; c0 += kWave) {
+      const uint32_t p = c0 + lane;
+      if (p < P1) {
+        typename Op::State st;
+        Op::load(A, ve_elem[p], st);
+        Op::pair(A, st, slot[p], PP[lane]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int g = 0; g < 2; ++g) {
+        if (HasToDiag<Op>::value || !dact[g]) continue;  // (summed after phase B)
+        const int q = dq[g];
+        const bool rq = Op::kHasRhs && q % (BD + 1) == 0;  // also rhs component q/(BD+1)
+        const uint32_t lo = max(db[g], c0), hi = min(de[g], c0 + kWave);
+        for (uint32_t k = lo; k < hi; ++k) {
+          const Pair &P = PP[k - c0];
+          double v;
+          // the slots holding the row (one, unless the element repeats it)
+          uint32_t match = 0;
+          for (int b = 0; b < CARD; ++b) match |= ((uint32_t)P.vid[b] == drow[g] ? 1u : 0u) << b;
+          while (match) {
+            const int b = __builtin_ctz(match);
+            match &= match - 1;
+            if (Op::block_entry(A, P, b, q, v)) a2[g] += v;
+          }
+          if (Op::kExtra && Op::extra_entry(A, P, q, v)) a2[g] += v;
+          if (rq) Op::rhs_entry(A, P, q / (BD + 1), f2[g]);
+        }
+      }
+      if (valid && !is_diag) {
+        // bits k of the row's pairs myb + k inside this chunk
+        const int64_t s0 = (int64_t)c0 - (int64_t)myb, s1 = s0 + kWave;
+        MT m = bm;
+        if (s0 > 0) m = s0 >= kBits ? (MT)0 : (MT)(m & (~(MT)0 << s0));
+        if (s1 < kBits) m = s1 <= 0 ? (MT)0 : (MT)(m & (((MT)1 << s1) - (MT)1));
+        while (m) {
+          const int k = ctz_mask(m);
+          m &= m - (MT)1;
+          const Pair &P = PP[myb + (uint32_t)k - c0];
+          uint32_t match = 0;
+          for (int b = 0; b < CARD; ++b) match |= ((uint32_t)P.vid[b] == col ? 1u : 0u) << b;
+          while (match) {
+            const int b = __builtin_ctz(match);
+            match &= match - 1;
+            double blk[BB];
+            if (Op::block(A, P, b, blk))
+              for (int q = 0; q < BB; ++q) acc[q] += blk[q];
+          }
+        }
+      }
+      __syncthreads();
+      if constexpr (HasToDiag<Op>::value) {
+        // each record becomes its pair's diagonal terms, then the diagonal
+        // entries are summed from them in list order
+        if (p < P1) Op::to_diag(A, PP[lane]);
+        __syncthreads();
+#pragma unroll
+        for (int g = 0; g < 2; ++g) {
+          if (!dact[g]) continue;
+          const uint32_t lo = max(db[g], c0), hi = min(de[g], c0 + kWave);
+          for (uint32_t k = lo; k < hi; ++k) Op::diag_sum(A, PP[k - c0], dq[g], a2[g], f2[g]);
+        }
+        __syncthreads();
+      }
+    }
+    double *outbuf = reinterpret_cast<double *>(smem);
+    if (valid && !is_diag) {
+      Op::finish_block(A, row, col, acc);
+      for (int q = 0; q < BB; ++q) outbuf[lane * BB + q] = acc[q];
+    }
+#pragma unroll
+    for (int g = 0; g < 2; ++g) {
+      if (!dact[g]) continue;
+      const int q = dq[g], d = (g * kWave + lane) / BB;
+      double v = a2[g];
+      Op::finish_block_entry(A, drow[g], drow[g], q, v);
+      Op::finish_diag_entry(A, drow[g], q, v);
+      outbuf[DL[d] * BB + q] = v;
+      if (Op::kHasRhs && rhs && q % (BD + 1) == 0) {
+        double f = f2[g];
+        Op::finish_rhs_entry(A, drow[g], q / (BD + 1), f);
+        rhs[(uint64_t)drow[g] * BD + q / (BD + 1)] = f;
+      }
+    }
+    __syncthreads();
+    const uint64_t nval = (min(b0 + kWave, nnz) - b0) * BB;
+    for (int q = 0; q < BB; ++q) {
+      const uint64_t i = (uint64_t)q * kWave + lane;
+      if (i < nval) store_val<PREC>(vals, vals_lo, b0 * BB + i, outbuf[i]);
+    }
+    __syncthreads();  // LDS is reused by the next wave
+  }
+}
+
 // right-hand side of vertices that belong to no element (empty rows)
 template <class Op>
 __global__ void k_isolated(typename Op::Args A, uint32_t nrows, const uint32_t *__restrict__ rowptr,
@@ -774,6 +1228,92 @@ unsigned assembly_wgs_per_cu() {
   return n;
 }
 
+template <class Op, int P, typename MT>
+unsigned fast_wgs_per_cu() {
+  static const unsigned n = [] {
+    if (const char *e = getenv("SMH_ASSEMBLY_WGS_PER_CU")) {
+      const int v = atoi(e);
+      if (v > 0) return (unsigned)v;
+    }
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_assemble_fast<Op, P, MT>, kWave, 0) !=
+            hipSuccess ||
+        occ < 1) {
+      (void)hipGetLastError();
+      occ = 4;
+    }
+    return (unsigned)occ;
+  }();
+  return n;
+}
+
+// The set's plan (slots, masks, wave classes), built on first use and kept
+// with the index; one-time work that synchronizes the stream.
+int asm_plan(smh_ctx *ctx, const smh_index *idx, int set, smh_index::AsmPlan **out) {
+  using namespace smh;
+  if (idx->asm_plan.size() < idx->nsets) idx->asm_plan.resize(idx->nsets);
+  smh_index::AsmPlan &pl = idx->asm_plan[set];
+  *out = &pl;
+  if (pl.bits) return SMH_OK;
+  const uint32_t nrows = idx->nrows;
+  const uint64_t nnz = idx->nnz, ninc = idx->n_inc[set];
+  const uint64_t nwaves = (nnz + kWave - 1) / kWave;
+  const uint32_t card = idx->card[set];
+  unsigned long long *dcnt = nullptr;
+  SMH_CHECK(ctx, dmalloc(ctx, (void **)&dcnt, 4 * sizeof(unsigned long long)) == SMH_OK,
+            SMH_ERR_HIP, "assembly plan: out of device memory");
+  unsigned long long hcnt[4] = {0, 0, 0, 0};
+  int rc = SMH_OK;
+  auto fail_free = [&](int code) {
+    dfree(dcnt);
+    dfree(pl.slot);
+    dfree(pl.mask);
+    dfree(pl.wnd);
+    pl.slot = nullptr;
+    pl.mask = nullptr;
+    pl.wnd = nullptr;
+    return code;
+  };
+  SMH_HIP(ctx, hipMemsetAsync(dcnt, 0, 4 * sizeof(unsigned long long), ctx->stream));
+  hipLaunchKernelGGL(k_asm_maxdeg, dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream, nrows,
+                     (const uint32_t *)idx->ve_ptr[set], (uint32_t *)(dcnt + 3));
+  if (hipMemcpyAsync(hcnt, dcnt, sizeof(hcnt), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return fail_free(fail(ctx, SMH_ERR_HIP, "assembly plan: device copy failed"));
+  const uint32_t maxdeg = (uint32_t)(hcnt[3] & 0xffffffffu);
+  const int bits = maxdeg <= 32 ? 32 : 64;
+  if ((rc = dmalloc(ctx, (void **)&pl.slot, ninc ? ninc : 1)) != SMH_OK ||
+      (rc = dmalloc(ctx, &pl.mask, (bits / 8) * (nnz ? nnz : 1))) != SMH_OK ||
+      (rc = dmalloc(ctx, (void **)&pl.wnd, nwaves ? nwaves : 1)) != SMH_OK)
+    return fail_free(rc);
+  if (nrows) {
+    if (bits == 32)
+      hipLaunchKernelGGL(k_asm_plan<uint32_t>, dim3(grid_for(nrows, 256)), dim3(256), 0,
+                         ctx->stream, nrows, card, idx->ep[set], (const uint32_t *)idx->rowptr,
+                         (const uint32_t *)idx->colidx, (const uint32_t *)idx->ve_ptr[set],
+                         (const uint32_t *)idx->ve_elem[set], pl.slot, (uint32_t *)pl.mask);
+    else
+      hipLaunchKernelGGL(k_asm_plan<uint64_t>, dim3(grid_for(nrows, 256)), dim3(256), 0,
+                         ctx->stream, nrows, card, idx->ep[set], (const uint32_t *)idx->rowptr,
+                         (const uint32_t *)idx->colidx, (const uint32_t *)idx->ve_ptr[set],
+                         (const uint32_t *)idx->ve_elem[set], pl.slot, (uint64_t *)pl.mask);
+  }
+  if (nwaves)
+    hipLaunchKernelGGL(k_asm_waves, dim3(grid_for(nwaves, 256)), dim3(256), 0, ctx->stream,
+                       nwaves, nnz, nrows, (uint32_t)bits, card, (const uint32_t *)idx->rowptr,
+                       (const uint32_t *)idx->colidx, (const uint32_t *)idx->wave_row,
+                       (const uint32_t *)idx->ve_ptr[set], (const uint8_t *)pl.slot, pl.wnd,
+                       dcnt);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(hcnt, dcnt, sizeof(hcnt), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
+      hipStreamSynchronize(ctx->stream) != hipSuccess)
+    return fail_free(fail(ctx, SMH_ERR_HIP, "assembly plan: kernels failed"));
+  dfree(dcnt);
+  for (int c = 0; c < 3; ++c) pl.nslow[c] = hcnt[c];
+  pl.bits = bits;
+  return SMH_OK;
+}
+
 template <class Op>
 int launch_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, const typename Op::Args &A,
                     int32_t prec, void *vals, void *vals_lo, double *rhs, const char *name) {
@@ -795,21 +1335,59 @@ int launch_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, const
   if (nrows == 0) return SMH_OK;
   const uint64_t nwaves = (idx->nnz + kWave - 1) / kWave;
   if (nwaves) {
+    // SMH_ASSEMBLY_GENERIC=1: every wave through the generic kernel (tests)
+    // (a plan not built before a stream capture is not built inside it)
+    const char *gen = getenv("SMH_ASSEMBLY_GENERIC");
+    smh_index::AsmPlan *pl = nullptr;
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cap) != hipSuccess) {
+      (void)hipGetLastError();
+      cap = hipStreamCaptureStatusNone;
+    }
+    const bool built = (uint32_t)set < idx->asm_plan.size() && idx->asm_plan[set].bits;
+    if (!(gen && atoi(gen) == 1) && (built || cap == hipStreamCaptureStatusNone)) {
+      const int rc = asm_plan(ctx, idx, set, &pl);
+      if (rc != SMH_OK) return rc;
+    }
+    constexpr int BB = Op::BD * Op::BD;
+    const uint64_t nslow = pl ? pl->nslow[Op::BD - 1] : nwaves;
     // grid: a multiple of 8 (one share per XCD), no more than fills the chip
-    auto grid = [&](unsigned per_cu) {
+    auto grid = [&](unsigned per_cu, uint64_t n) {
       const uint64_t full = (uint64_t)ctx->num_cu * per_cu;
-      const uint64_t g = std::min<uint64_t>(full, nwaves);
+      const uint64_t g = std::min<uint64_t>(full, n);
       return (unsigned)((g + 7) / 8 * 8);
     };
+    static_assert(BB == 1 || BB == 4 || BB == 9, "1x1, 2x2 or 3x3 blocks");
+    if (pl && nslow < nwaves) {
+#define SMH_FAST(P, MT)                                                                      \
+  hipLaunchKernelGGL((k_assemble_fast<Op, P, MT>), dim3(grid(fast_wgs_per_cu<Op, P, MT>(),   \
+                                                            nwaves - nslow)),                \
+                     dim3(kWave), 0, ctx->stream, A, nrows, idx->nnz, idx->rowptr,           \
+                     idx->colidx, idx->wave_row, idx->ve_ptr[set], idx->ve_elem[set],        \
+                     pl->slot, (const MT *)pl->mask, pl->wnd, vals, vals_lo, rhs, nwaves)
+#define SMH_FAST_P(MT)                            \
+  do {                                            \
+    if (prec == SMH_F64) SMH_FAST(SMH_F64, MT);   \
+    else if (prec == SMH_F32) SMH_FAST(SMH_F32, MT); \
+    else SMH_FAST(SMH_F32X2, MT);                 \
+  } while (0)
+      if (pl->bits == 32) SMH_FAST_P(uint32_t);
+      else SMH_FAST_P(uint64_t);
+#undef SMH_FAST_P
+#undef SMH_FAST
+    }
+    if (nslow) {
+      const uint8_t *wnd = pl ? pl->wnd : nullptr;
 #define SMH_ASM(P)                                                                          \
-  hipLaunchKernelGGL((k_assemble<Op, P>), dim3(grid(assembly_wgs_per_cu<Op, P>())),         \
+  hipLaunchKernelGGL((k_assemble<Op, P>), dim3(grid(assembly_wgs_per_cu<Op, P>(), nslow)),  \
                      dim3(kWave), 0, ctx->stream, A, nrows, idx->nnz, idx->rowptr,          \
                      idx->colidx, idx->wave_row, idx->ve_ptr[set], idx->ve_elem[set], vals, \
-                     vals_lo, rhs, nwaves)
-    if (prec == SMH_F64) SMH_ASM(SMH_F64);
-    else if (prec == SMH_F32) SMH_ASM(SMH_F32);
-    else SMH_ASM(SMH_F32X2);
+                     vals_lo, rhs, nwaves, wnd)
+      if (prec == SMH_F64) SMH_ASM(SMH_F64);
+      else if (prec == SMH_F32) SMH_ASM(SMH_F32);
+      else SMH_ASM(SMH_F32X2);
 #undef SMH_ASM
+    }
   }
   hipLaunchKernelGGL(k_isolated<Op>, dim3(grid_for(nrows, 256)), dim3(256), 0, ctx->stream, A,
                      nrows, idx->rowptr, rhs);
@@ -818,6 +1396,17 @@ int launch_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, const
 }
 
 }  // namespace
+
+namespace smh {
+int prepare_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, uint32_t card) {
+  for (uint32_t s = 0; s < idx->nsets; ++s)
+    if (idx->ep[s] == ep && idx->card[s] == card) {
+      smh_index::AsmPlan *pl = nullptr;
+      return idx->nnz ? asm_plan(ctx, idx, (int)s, &pl) : SMH_OK;
+    }
+  return SMH_OK;
+}
+}  // namespace smh
 
 extern "C" int smh_assemble_springs(smh_ctx *ctx, const smh_index *idx, const smh_springs_params *p,
                                     const smh_springs_set *s, int32_t prec, void *vals,

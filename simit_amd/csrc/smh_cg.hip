@@ -2402,7 +2402,9 @@ extern "C" int smh_springs_solve_batched(smh_ctx *ctx, const smh_index *idx,
   const int prec = o->mixed ? SMH_F32X2 : SMH_F64;
   const smh_bsr A{idx->nrows, 3, idx->nnz, idx->rowptr, idx->colidx, prec, vals,
                   o->mixed ? vals_lo : nullptr};
-  // the assembly is the first work of the first captured graph
+  // the assembly is the first work of the first captured graph (its plan is
+  // built before the capture)
+  if (const int pr = smh::prepare_assembly(ctx, idx, set->endpoints, 2)) return pr;
   auto prologue = [&](hipStream_t s) {
     hipStream_t caller = ctx->stream;
     ctx->stream = s;

@@ -255,6 +255,11 @@ void free_index(smh_index *ix) {
   smh::dfree(ix->wave_row);
   for (auto p : ix->ve_ptr) smh::dfree(p);
   for (auto p : ix->ve_elem) smh::dfree(p);
+  for (auto &a : ix->asm_plan) {
+    smh::dfree(a.slot);
+    smh::dfree(a.mask);
+    smh::dfree(a.wnd);
+  }
   delete ix;
 }
 

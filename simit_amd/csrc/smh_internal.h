@@ -78,6 +78,17 @@ struct smh_index {
   std::vector<uint64_t> ne;         // per set element count
   std::vector<uint32_t> card;       // per set cardinality
   std::vector<const int32_t *> ep;  // per set endpoints (borrowed, device)
+  // Per set, made by the first assembly over the set and kept with the pattern:
+  // each incidence's endpoint slot, each block's contributing incidences, and
+  // which 64-block waves the fast assembly path takes.
+  struct AsmPlan {
+    uint8_t *slot = nullptr;   // [n_inc] slot of the row in the incidence's element
+    void *mask = nullptr;      // [nnz] bit k: the row's k-th incidence writes the block
+    uint8_t *wnd = nullptr;    // [waves] diagonal blocks in the wave (255: generic path)
+    int bits = 0;              // 32 or 64 mask bits (0: not built yet)
+    uint64_t nslow[3] = {0, 0, 0};  // generic waves for 1x1, 2x2 and 3x3 blocks
+  };
+  mutable std::vector<AsmPlan> asm_plan;
 };
 
 namespace smh {
@@ -129,6 +140,9 @@ int tile_plan(smh_ctx *ctx, const uint32_t *rowptr, uint32_t nrows, uint64_t nnz
               uint32_t tile_blocks, smh_tile_plan *out);
 // drop cached plans of a rowptr buffer (before it is freed or reused)
 void forget_plans(smh_ctx *ctx, const uint32_t *rowptr);
+// Builds the index's assembly plan for the set with these endpoints (before a
+// stream capture that records the assembly).
+int prepare_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, uint32_t card);
 // device address of the static scratch array on ctx's (current) device
 int scratch_setup(smh_ctx *ctx);
 
