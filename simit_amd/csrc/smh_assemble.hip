@@ -208,7 +208,9 @@ struct TetOp {
   static constexpr uint32_t kFa = 16u;
   struct Pair {
     int32_t vid[4];
-    uint32_t bits;  // freemask (bits 0-3), fa (bit 4), a (bits 8-9)
+    // freemask (bits 0-3), fa (bit 4), a (bits 8-9), freemask in stored
+    // order (bits 12-15: the own slot, then the others ascending)
+    uint32_t bits;
     int32_t pad;
     double cl, cm, W;
     double g[12];
@@ -248,7 +250,10 @@ struct TetOp {
       P.vid[t] = st.vid[t];
       fm |= (st.free_[t] ? 1u : 0u) << t;
     }
-    P.bits = fm | (((fm >> a) & 1u) << 4) | ((uint32_t)a << 8);
+    uint32_t fmp = (fm >> a) & 1u;
+    for (int t = 0; t < 4; ++t)
+      if (t != a) fmp |= ((fm >> t) & 1u) << (t < a ? t + 1 : t);
+    P.bits = fm | (((fm >> a) & 1u) << 4) | ((uint32_t)a << 8) | (fmp << 12);
     const double sw = A.p.stiff_scale * st.W;
     P.cl = sw * st.lam;
     P.cm = sw * st.mu;
@@ -273,6 +278,27 @@ struct TetOp {
     for (int i = 0; i < 3; ++i)
       for (int j = 0; j < 3; ++j) {
         double t = u[i] * gb[j] + w[j] * gb[i];
+        if (i == j) t = t + wg;
+        out[3 * i + j] = t;
+      }
+    return true;
+  }
+  // block() for the slot of rank r among the slots other than the row's
+  // (stored at position r + 1): the fast kernel's plan names r, so the
+  // gradients are read at offsets known before the record is
+  __device__ static bool block_rank(const Args &, const Pair &P, int r, double *out) {
+    const uint32_t bits = P.bits;
+    const double cl = P.cl, cm = P.cm;
+    const double g0 = P.g[0], g1 = P.g[1], g2 = P.g[2];
+    const double *gb = P.g + 3 * (r + 1);
+    const double b0 = gb[0], b1 = gb[1], b2 = gb[2];
+    if (!(bits & kFa) || !((bits >> (13 + r)) & 1u)) return false;
+    const double u[3] = {cl * g0, cl * g1, cl * g2}, w[3] = {cm * g0, cm * g1, cm * g2};
+    const double gbv[3] = {b0, b1, b2};
+    const double wg = w[0] * b0 + w[1] * b1 + w[2] * b2;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) {
+        double t = u[i] * gbv[j] + w[j] * gbv[i];
         if (i == j) t = t + wg;
         out[3 * i + j] = t;
       }
@@ -364,6 +390,11 @@ template <class Op, class = void>
 struct HasToDiag : std::false_type {};
 template <class Op>
 struct HasToDiag<Op, std::void_t<decltype(&Op::to_diag)>> : std::true_type {};
+// ops whose off-diagonal blocks are formed from the plan's slot ranks
+template <class Op, class = void>
+struct HasBlockRank : std::false_type {};
+template <class Op>
+struct HasBlockRank<Op, std::void_t<decltype(&Op::block_rank)>> : std::true_type {};
 
 // scalar trilinear hex Poisson, corner a = dx + 2dy + 4dz
 struct HexArgs {
@@ -907,17 +938,21 @@ k_assemble(typename Op::Args A, uint32_t nrows, uint64_t nnz,
 // ---- the per-pattern plan and the fast kernel ----------------------------------
 // Row i's pairs are ve_elem[ve_ptr[i] .. ve_ptr[i+1]), elements ascending (an
 // element holding i k times is listed k times in a row).  Per pair: the
-// endpoint slot it stands for (the occ-th slot equal to i; bit 7 set when
-// the element holds i more than once).  Per
-// off-diagonal block (i, j): bit k set when the element of the row's k-th
-// pair holds j.  One thread per row, which owns the row's blocks.
+// endpoint slot it stands for (the occ-th slot equal to i), with bit 7 set
+// when the element repeats a vertex and bit 6 when one of the row's blocks
+// has more contributions than the rank list holds (their waves take the
+// generic kernel).  Per off-diagonal block (i, j): bit k set when the
+// element of the row's k-th pair holds j, and (rank != null, elements of at
+// most 4 slots) for its contributions in bit order 2 bits each: the rank of
+// j's slot b among the element's slots other than the row's slot a
+// (b - (b > a)).  One thread per row, which owns the row's blocks.
 template <typename MT>
 __global__ void k_asm_plan(uint32_t nrows, uint32_t card, const int32_t *__restrict__ ep,
                            const uint32_t *__restrict__ rowptr,
                            const uint32_t *__restrict__ colidx,
                            const uint32_t *__restrict__ ve_ptr,
                            const uint32_t *__restrict__ ve_elem, uint8_t *__restrict__ slot,
-                           MT *__restrict__ mask) {
+                           MT *__restrict__ mask, uint32_t *__restrict__ rank) {
   constexpr uint32_t kBits = 8 * sizeof(MT);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nrows) return;
@@ -932,14 +967,15 @@ __global__ void k_asm_plan(uint32_t nrows, uint32_t card, const int32_t *__restr
     prev = e;
     const int32_t *v = ep + (uint64_t)e * card;
     uint32_t a = 255, seen = 0;
-    for (uint32_t t = 0; t < card; ++t)
+    bool repeat = false;
+    for (uint32_t t = 0; t < card; ++t) {
       if ((uint32_t)v[t] == i) {
         if (seen == occ) a = t;
         ++seen;
       }
-    // 0x80: the element holds the row more than once (its waves take the
-    // generic kernel)
-    slot[p] = (uint8_t)(seen > 1 && a < 128 ? a | 0x80u : a);
+      for (uint32_t t2 = 0; t2 < t; ++t2) repeat = repeat || v[t2] == v[t];
+    }
+    slot[p] = (uint8_t)(repeat ? (a | 0x80u) : a);
     const uint32_t k = p - rb;
     if (k >= kBits) continue;
     for (uint32_t t = 0; t < card; ++t) {
@@ -953,6 +989,28 @@ __global__ void k_asm_plan(uint32_t nrows, uint32_t card, const int32_t *__restr
       if (lo < c1 && colidx[lo] == j) mask[lo] |= (MT)1 << k;
     }
   }
+  if (!rank) return;
+  bool over = false;
+  for (uint32_t q = c0; q < c1; ++q) {
+    const uint32_t j = colidx[q];
+    MT m = mask[q];
+    uint32_t rk = 0, n = 0;
+    while (m) {
+      const uint32_t k = sizeof(MT) == 8 ? (uint32_t)__builtin_ctzll((unsigned long long)m)
+                                         : (uint32_t)__builtin_ctz((unsigned)m);
+      m &= m - (MT)1;
+      const uint32_t a = slot[rb + k] & 0x3fu;
+      const int32_t *v = ep + (uint64_t)ve_elem[rb + k] * card;
+      uint32_t b = 0;
+      while (b < card && (uint32_t)v[b] != j) ++b;
+      if (n < 16) rk |= ((b > a ? b - 1 : b) & 3u) << (2 * n);
+      ++n;
+    }
+    rank[q] = rk;
+    over = over || n > 16;
+  }
+  if (over)
+    for (uint32_t p = rb; p < re; ++p) slot[p] = (uint8_t)(slot[p] | 0x40u);
 }
 
 __global__ void k_asm_maxdeg(uint32_t nrows, const uint32_t *__restrict__ ve_ptr,
@@ -963,15 +1021,16 @@ __global__ void k_asm_maxdeg(uint32_t nrows, const uint32_t *__restrict__ ve_ptr
 
 // Per wave of 64 blocks: the number of diagonal blocks it holds, or 255 when
 // its rows span more than the 64-row window, a row has more pairs than the
-// mask has bits, or a pair has no slot or a repeated one.  cnt[0..2]: generic waves for 1x1, 2x2
-// and 3x3 blocks.
+// mask has bits, or a pair has no slot or a repeated one; and the number of
+// LDS chunks its pairs take.  cnt[0..2]: generic waves for 1x1, 2x2 and 3x3
+// blocks.
 __global__ void k_asm_waves(uint64_t nwaves, uint64_t nnz, uint32_t nrows, uint32_t bits,
                             uint32_t card, const uint32_t *__restrict__ rowptr,
                             const uint32_t *__restrict__ colidx,
                             const uint32_t *__restrict__ wave_row,
                             const uint32_t *__restrict__ ve_ptr,
                             const uint8_t *__restrict__ slot, uint8_t *__restrict__ wnd,
-                            unsigned long long *__restrict__ cnt) {
+                            uint8_t *__restrict__ wnc, unsigned long long *__restrict__ cnt) {
   const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= nwaves) return;
   const uint64_t b0 = w * kWave, blast = min(b0 + kWave, nnz) - 1;
@@ -998,6 +1057,27 @@ __global__ void k_asm_waves(uint64_t nwaves, uint64_t nnz, uint32_t nrows, uint3
     if (a < rowptr[r + 1] && colidx[a] == r && a >= b0 && a <= blast) ++nd;
   }
   wnd[w] = slow ? 255 : (uint8_t)nd;
+  // chunks: the fewest n such that, with each row's list cut into n parts
+  // ([c*deg/n, (c+1)*deg/n) for part c), every chunk c (part c of each row)
+  // holds at most 64 pairs
+  uint32_t n = 1;
+  if (!slow) {
+    const uint32_t T = ve_ptr[rlast + 1] - ve_ptr[rfirst];
+    n = T > 0 ? (T + kWave - 1) / kWave : 1;
+    for (;; ++n) {
+      uint32_t worst = 0;
+      for (uint32_t c = 0; c < n; ++c) {
+        uint32_t sz = 0;
+        for (uint32_t r = rfirst; r <= rlast; ++r) {
+          const uint32_t d = ve_ptr[r + 1] - ve_ptr[r];
+          sz += (c + 1) * d / n - c * d / n;
+        }
+        worst = max(worst, sz);
+      }
+      if (worst <= (uint32_t)kWave || n >= 255) break;
+    }
+  }
+  wnc[w] = (uint8_t)n;
   for (int c = 0; c < 3; ++c)
     if (generic_wave(slow ? 255u : nd, (c + 1) * (c + 1))) atomicAdd(cnt + c, 1ull);
 }
@@ -1021,15 +1101,21 @@ __device__ __forceinline__ int ctz_mask(MT m) {
 // Waves with an element that holds a row twice take the generic kernel.
 // Every value is summed by one lane in the generic kernel's (the CPU loop's)
 // order with the same arithmetic, so the results are bit-identical.
+// resident waves per SIMD the fast kernel is compiled for (registers capped
+// to fit them): linear tets hold no more than 128 without spilling
+template <class Op> struct FastWaves { static constexpr int value = 1; };
+template <> struct FastWaves<TetOp> { static constexpr int value = 4; };
+
 template <class Op, int PREC, typename MT>
-__global__ void __launch_bounds__(kWave)
+__global__ void __launch_bounds__(kWave, FastWaves<Op>::value)
 k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
                 const uint32_t *__restrict__ rowptr, const uint32_t *__restrict__ colidx,
                 const uint32_t *__restrict__ wave_row, const uint32_t *__restrict__ ve_ptr,
                 const uint32_t *__restrict__ ve_elem, const uint8_t *__restrict__ slot,
-                const MT *__restrict__ mask, const uint8_t *__restrict__ wnd,
-                void *__restrict__ vals, void *__restrict__ vals_lo, double *__restrict__ rhs,
-                uint64_t nwaves) {
+                const MT *__restrict__ mask, const uint32_t *__restrict__ rank,
+                const uint8_t *__restrict__ wnd, const uint8_t *__restrict__ wnc,
+                void *__restrict__ vals,
+                void *__restrict__ vals_lo, double *__restrict__ rhs, uint64_t nwaves) {
   constexpr int BD = Op::BD, BB = BD * BD, CARD = Op::CARD;
   constexpr int64_t kBits = 8 * (int64_t)sizeof(MT);
   using Pair = typename Op::Pair;
@@ -1041,12 +1127,18 @@ k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
   __shared__ uint32_t VP[kWave + 1];  // ve_ptr of the wave's rows
   __shared__ uint8_t DR[kWave];       // local row of the wave's d-th diagonal block
   __shared__ uint8_t DL[kWave];       // and its lane
+  __shared__ uint8_t CW[kWave + 1];   // per row: its first slot in the current chunk
+  __shared__ uint8_t CS[kWave];       // per row: the rank of that pair in the row's list
 
   const int lane = threadIdx.x;
-  const uint32_t xcd = blockIdx.x & 7u, per = gridDim.x >> 3;
-  const uint64_t w1 = nwaves * (xcd + 1) / 8;
-  for (uint64_t w = nwaves * xcd / 8 + (blockIdx.x >> 3); w < w1; w += per) {
-    if (generic_wave(wnd[w], BB)) continue;
+  // One workgroup per wave of blocks (a grid of 8 * ceil(nwaves / 8)).
+  // Workgroups are dealt to the 8 XCDs round-robin; XCD x takes the x-th
+  // eighth of the waves, so neighbouring waves, which gather the same
+  // vertices and elements, share that XCD's L2.
+  const uint32_t xcd = blockIdx.x & 7u;
+  const uint64_t w = nwaves * xcd / 8 + (blockIdx.x >> 3);
+  if (w >= nwaves * (xcd + 1) / 8 || generic_wave(wnd[w], BB)) return;
+  {
     const uint64_t b0 = w * kWave, bk = b0 + lane;
     const bool valid = bk < nnz;
     const uint32_t rfirst = wave_row[w];
@@ -1058,6 +1150,7 @@ k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
     }
     const uint32_t col = valid ? colidx[bk] : 0xffffffffu;
     const MT bm = valid ? mask[bk] : (MT)0;
+    uint32_t rk = HasBlockRank<Op>::value && valid ? rank[bk] : 0u;
     __syncthreads();
     // local row: last l with R[l] <= bk (the plan put the wave's rows in the window)
     int lr = 0;
@@ -1073,8 +1166,6 @@ k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
     const bool is_diag = valid && col == row;
     int lrl = lr;
     for (int off = 32; off > 0; off >>= 1) lrl = max(lrl, __shfl_xor(lrl, off, 64));
-    const uint32_t P0 = VP[0], P1 = VP[lrl + 1];
-    const uint32_t myb = VP[lr];
     const unsigned long long dball = __ballot(is_diag);
     const int nd = __popcll(dball);
     if (is_diag) {
@@ -1085,25 +1176,51 @@ k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
     __syncthreads();
     // diagonal-entry roles: t = g*64 + lane is entry t % BB of diagonal t / BB
     bool dact[2];
-    int dq[2];
-    uint32_t drow[2], db[2], de[2];
+    int dq[2], dl[2];
+    uint32_t drow[2];
 #pragma unroll
     for (int g = 0; g < 2; ++g) {
       const int t = g * kWave + lane, d = t / BB;
       dact[g] = d < nd;
       dq[g] = t - d * BB;
-      const int l = dact[g] ? DR[d] : 0;
-      drow[g] = rfirst + (uint32_t)l;
-      db[g] = VP[l];
-      de[g] = VP[l + 1];
+      dl[g] = dact[g] ? DR[d] : 0;
+      drow[g] = rfirst + (uint32_t)dl[g];
     }
 
+    // The wave's pairs go through LDS in nch chunks of at most 64: chunk c
+    // holds part c of every row's list (ranks [c*deg/nch, (c+1)*deg/nch)),
+    // rows in order.  A row's pairs stay in list order across the chunks, and
+    // every row has about deg/nch of its pairs in each chunk, which bounds the
+    // per-row loops below.
+    const int nwr = lrl + 1;
+    const uint32_t nch = wnc[w];
+    const uint32_t rdeg = lane < nwr ? VP[lane + 1] - VP[lane] : 0u;  // lane l: row l
     double acc[BB];
     for (int q = 0; q < BB; ++q) acc[q] = 0.0;
     double a2[2] = {0.0, 0.0}, f2[2] = {0.0, 0.0};
-    for (uint32_t c0 = P0; c0 < P1; c0 += kWave) {
-      const uint32_t p = c0 + lane;
-      if (p < P1) {
+    uint32_t rs = 0;  // lane l < nwr: first rank of row l in this chunk
+    for (uint32_t c = 0; c < nch; ++c) {
+      const uint32_t re = (c + 1) * rdeg / nch, part = re - rs;
+      uint32_t incl = part;  // inclusive prefix over rows -> chunk slots
+      for (int off = 1; off < kWave; off <<= 1) {
+        const uint32_t t = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += t;
+      }
+      if (lane < nwr) {
+        CW[lane] = (uint8_t)(incl - part);
+        CS[lane] = (uint8_t)rs;
+      }
+      if (lane == nwr - 1) CW[nwr] = (uint8_t)incl;
+      rs = re;
+      __syncthreads();
+      const uint32_t csz = CW[nwr];
+      if ((uint32_t)lane < csz) {
+        int l = 0, hi = nwr;  // the row of slot lane: last l with CW[l] <= lane
+        while (hi - l > 1) {
+          const int mid = (l + hi) >> 1;
+          if (CW[mid] <= (uint32_t)lane) l = mid; else hi = mid;
+        }
+        const uint32_t p = VP[l] + CS[l] + ((uint32_t)lane - CW[l]);
         typename Op::State st;
         Op::load(A, ve_elem[p], st);
         Op::pair(A, st, slot[p], PP[lane]);
@@ -1114,9 +1231,9 @@ k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
         if (HasToDiag<Op>::value || !dact[g]) continue;  // (summed after phase B)
         const int q = dq[g];
         const bool rq = Op::kHasRhs && q % (BD + 1) == 0;  // also rhs component q/(BD+1)
-        const uint32_t lo = max(db[g], c0), hi = min(de[g], c0 + kWave);
+        const uint32_t lo = CW[dl[g]], hi = CW[dl[g] + 1];
         for (uint32_t k = lo; k < hi; ++k) {
-          const Pair &P = PP[k - c0];
+          const Pair &P = PP[k];
           double v;
           // the slots holding the row (one, unless the element repeats it)
           uint32_t match = 0;
@@ -1131,15 +1248,24 @@ k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
         }
       }
       if (valid && !is_diag) {
-        // bits k of the row's pairs myb + k inside this chunk
-        const int64_t s0 = (int64_t)c0 - (int64_t)myb, s1 = s0 + kWave;
+        // the mask bits (row ranks) of this chunk: [k0, k1), at slots CW[lr] + k - k0
+        const uint32_t base = CW[lr], k0 = CS[lr], k1 = k0 + (CW[lr + 1] - base);
         MT m = bm;
-        if (s0 > 0) m = s0 >= kBits ? (MT)0 : (MT)(m & (~(MT)0 << s0));
-        if (s1 < kBits) m = s1 <= 0 ? (MT)0 : (MT)(m & (((MT)1 << s1) - (MT)1));
+        if (k1 < (uint32_t)kBits) m &= ((MT)1 << k1) - (MT)1;
+        m = k0 >= (uint32_t)kBits ? (MT)0 : (MT)(m & (~(MT)0 << k0));
         while (m) {
           const int k = ctz_mask(m);
           m &= m - (MT)1;
-          const Pair &P = PP[myb + (uint32_t)k - c0];
+          const Pair &P = PP[base + (uint32_t)k - k0];
+          if constexpr (HasBlockRank<Op>::value) {
+            // contributions come in mask order, each with the next 2 bits
+            double blk[BB];
+            const int r = (int)(rk & 3u);
+            rk >>= 2;
+            if (Op::block_rank(A, P, r, blk))
+              for (int q = 0; q < BB; ++q) acc[q] += blk[q];
+            continue;
+          }
           uint32_t match = 0;
           for (int b = 0; b < CARD; ++b) match |= ((uint32_t)P.vid[b] == col ? 1u : 0u) << b;
           while (match) {
@@ -1155,13 +1281,13 @@ k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
       if constexpr (HasToDiag<Op>::value) {
         // each record becomes its pair's diagonal terms, then the diagonal
         // entries are summed from them in list order
-        if (p < P1) Op::to_diag(A, PP[lane]);
+        if ((uint32_t)lane < csz) Op::to_diag(A, PP[lane]);
         __syncthreads();
 #pragma unroll
         for (int g = 0; g < 2; ++g) {
           if (!dact[g]) continue;
-          const uint32_t lo = max(db[g], c0), hi = min(de[g], c0 + kWave);
-          for (uint32_t k = lo; k < hi; ++k) Op::diag_sum(A, PP[k - c0], dq[g], a2[g], f2[g]);
+          const uint32_t lo = CW[dl[g]], hi = CW[dl[g] + 1];
+          for (uint32_t k = lo; k < hi; ++k) Op::diag_sum(A, PP[k], dq[g], a2[g], f2[g]);
         }
         __syncthreads();
       }
@@ -1191,7 +1317,6 @@ k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
       const uint64_t i = (uint64_t)q * kWave + lane;
       if (i < nval) store_val<PREC>(vals, vals_lo, b0 * BB + i, outbuf[i]);
     }
-    __syncthreads();  // LDS is reused by the next wave
   }
 }
 
@@ -1228,25 +1353,6 @@ unsigned assembly_wgs_per_cu() {
   return n;
 }
 
-template <class Op, int P, typename MT>
-unsigned fast_wgs_per_cu() {
-  static const unsigned n = [] {
-    if (const char *e = getenv("SMH_ASSEMBLY_WGS_PER_CU")) {
-      const int v = atoi(e);
-      if (v > 0) return (unsigned)v;
-    }
-    int occ = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_assemble_fast<Op, P, MT>, kWave, 0) !=
-            hipSuccess ||
-        occ < 1) {
-      (void)hipGetLastError();
-      occ = 4;
-    }
-    return (unsigned)occ;
-  }();
-  return n;
-}
-
 // The set's plan (slots, masks, wave classes), built on first use and kept
 // with the index; one-time work that synchronizes the stream.
 int asm_plan(smh_ctx *ctx, const smh_index *idx, int set, smh_index::AsmPlan **out) {
@@ -1269,9 +1375,13 @@ int asm_plan(smh_ctx *ctx, const smh_index *idx, int set, smh_index::AsmPlan **o
     dfree(pl.slot);
     dfree(pl.mask);
     dfree(pl.wnd);
+    dfree(pl.rank);
+    dfree(pl.wnc);
+    pl.wnc = nullptr;
     pl.slot = nullptr;
     pl.mask = nullptr;
     pl.wnd = nullptr;
+    pl.rank = nullptr;
     return code;
   };
   SMH_HIP(ctx, hipMemsetAsync(dcnt, 0, 4 * sizeof(unsigned long long), ctx->stream));
@@ -1284,26 +1394,30 @@ int asm_plan(smh_ctx *ctx, const smh_index *idx, int set, smh_index::AsmPlan **o
   const int bits = maxdeg <= 32 ? 32 : 64;
   if ((rc = dmalloc(ctx, (void **)&pl.slot, ninc ? ninc : 1)) != SMH_OK ||
       (rc = dmalloc(ctx, &pl.mask, (bits / 8) * (nnz ? nnz : 1))) != SMH_OK ||
-      (rc = dmalloc(ctx, (void **)&pl.wnd, nwaves ? nwaves : 1)) != SMH_OK)
+      (rc = dmalloc(ctx, (void **)&pl.wnd, nwaves ? nwaves : 1)) != SMH_OK ||
+      (rc = dmalloc(ctx, (void **)&pl.wnc, nwaves ? nwaves : 1)) != SMH_OK ||
+      (card <= 4 && (rc = dmalloc(ctx, (void **)&pl.rank, 4 * (nnz ? nnz : 1))) != SMH_OK))
     return fail_free(rc);
   if (nrows) {
     if (bits == 32)
       hipLaunchKernelGGL(k_asm_plan<uint32_t>, dim3(grid_for(nrows, 256)), dim3(256), 0,
                          ctx->stream, nrows, card, idx->ep[set], (const uint32_t *)idx->rowptr,
                          (const uint32_t *)idx->colidx, (const uint32_t *)idx->ve_ptr[set],
-                         (const uint32_t *)idx->ve_elem[set], pl.slot, (uint32_t *)pl.mask);
+                         (const uint32_t *)idx->ve_elem[set], pl.slot, (uint32_t *)pl.mask,
+                         pl.rank);
     else
       hipLaunchKernelGGL(k_asm_plan<uint64_t>, dim3(grid_for(nrows, 256)), dim3(256), 0,
                          ctx->stream, nrows, card, idx->ep[set], (const uint32_t *)idx->rowptr,
                          (const uint32_t *)idx->colidx, (const uint32_t *)idx->ve_ptr[set],
-                         (const uint32_t *)idx->ve_elem[set], pl.slot, (uint64_t *)pl.mask);
+                         (const uint32_t *)idx->ve_elem[set], pl.slot, (uint64_t *)pl.mask,
+                         pl.rank);
   }
   if (nwaves)
     hipLaunchKernelGGL(k_asm_waves, dim3(grid_for(nwaves, 256)), dim3(256), 0, ctx->stream,
                        nwaves, nnz, nrows, (uint32_t)bits, card, (const uint32_t *)idx->rowptr,
                        (const uint32_t *)idx->colidx, (const uint32_t *)idx->wave_row,
                        (const uint32_t *)idx->ve_ptr[set], (const uint8_t *)pl.slot, pl.wnd,
-                       dcnt);
+                       pl.wnc, dcnt);
   if (hipGetLastError() != hipSuccess ||
       hipMemcpyAsync(hcnt, dcnt, sizeof(hcnt), hipMemcpyDeviceToHost, ctx->stream) != hipSuccess ||
       hipStreamSynchronize(ctx->stream) != hipSuccess)
@@ -1360,11 +1474,11 @@ int launch_assembly(smh_ctx *ctx, const smh_index *idx, const int32_t *ep, const
     static_assert(BB == 1 || BB == 4 || BB == 9, "1x1, 2x2 or 3x3 blocks");
     if (pl && nslow < nwaves) {
 #define SMH_FAST(P, MT)                                                                      \
-  hipLaunchKernelGGL((k_assemble_fast<Op, P, MT>), dim3(grid(fast_wgs_per_cu<Op, P, MT>(),   \
-                                                            nwaves - nslow)),                \
+  hipLaunchKernelGGL((k_assemble_fast<Op, P, MT>), dim3((unsigned)((nwaves + 7) / 8 * 8)),   \
                      dim3(kWave), 0, ctx->stream, A, nrows, idx->nnz, idx->rowptr,           \
                      idx->colidx, idx->wave_row, idx->ve_ptr[set], idx->ve_elem[set],        \
-                     pl->slot, (const MT *)pl->mask, pl->wnd, vals, vals_lo, rhs, nwaves)
+                     pl->slot, (const MT *)pl->mask, pl->rank, pl->wnd, pl->wnc, vals,        \
+                     vals_lo, rhs, nwaves)
 #define SMH_FAST_P(MT)                            \
   do {                                            \
     if (prec == SMH_F64) SMH_FAST(SMH_F64, MT);   \

@@ -258,7 +258,9 @@ void free_index(smh_index *ix) {
   for (auto &a : ix->asm_plan) {
     smh::dfree(a.slot);
     smh::dfree(a.mask);
+    smh::dfree(a.rank);
     smh::dfree(a.wnd);
+    smh::dfree(a.wnc);
   }
   delete ix;
 }

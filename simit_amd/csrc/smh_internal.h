@@ -84,7 +84,10 @@ struct smh_index {
   struct AsmPlan {
     uint8_t *slot = nullptr;   // [n_inc] slot of the row in the incidence's element
     void *mask = nullptr;      // [nnz] bit k: the row's k-th incidence writes the block
+    uint32_t *rank = nullptr;  // [nnz] per contribution (mask order, 2 bits): the rank of
+                               // the column's slot among the element's other slots
     uint8_t *wnd = nullptr;    // [waves] diagonal blocks in the wave (255: generic path)
+    uint8_t *wnc = nullptr;    // [waves] LDS chunks the wave's pairs are split into
     int bits = 0;              // 32 or 64 mask bits (0: not built yet)
     uint64_t nslow[3] = {0, 0, 0};  // generic waves for 1x1, 2x2 and 3x3 blocks
   };
