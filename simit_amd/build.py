@@ -79,10 +79,29 @@ def build_host() -> Path:
     return HOST_LIB
 
 
+PROGRAM_TESTS_ASAN = BINDIR / "simit_program_tests_asan"
+
+
+def build_host_asan() -> Path:
+    """The host layer (sets, graphs, reordering, mesh loading, parser, code
+    generator, programs) and its test driver built with AddressSanitizer and
+    UndefinedBehaviorSanitizer, as one executable on the unsanitized device
+    library.  tests/test_host_asan.py runs its host-only cases."""
+    BINDIR.mkdir(exist_ok=True)
+    srcs = [str(CSRC / s) for s in HOST_SOURCES]
+    _run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+          "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined",
+          "-I", str(ROOT / "include"), "-I", str(CSRC), *srcs,
+          str(CSRC / "tests_cpp" / "program_tests.cpp"), "-L", str(LIBDIR), "-lsimit_hip",
+          "-Wl,-rpath,$ORIGIN/../lib", "-ldl", "-o", str(PROGRAM_TESTS_ASAN)])
+    return PROGRAM_TESTS_ASAN
+
+
 def build_all():
     build_hip()
     if (CSRC / "simit" / "program.cpp").exists():
         build_host()
+        build_host_asan()
 
 
 if __name__ == "__main__":

@@ -32,14 +32,17 @@ def _built():
         pytest.fail(f"{BIN} is not built (run __graft_entry__.build())")
 
 
-@pytest.mark.parametrize("name,functions", [
+OWN_PROGRAMS = [
     ("springs.sim", ["main"]),
     ("cg_points.sim", ["main"]),
     ("neohookean.sim", ["initializeTet", "main"]),
     ("gemv.sim", ["main"]),
     ("grid_gemv.sim", ["main"]),
     ("grid_gemv_2d.sim", ["main"]),
-])
+]
+
+
+@pytest.mark.parametrize("name,functions", OWN_PROGRAMS)
 def test_own_programs_compile(name, functions):
     rc, out = compile_program(ROOT / "tests" / "programs" / name)
     assert rc == 0, out
