@@ -16,7 +16,8 @@ x = 0.  Block-Jacobi PCG (mixed and FP64) and plain FP64 CG on the same GPU
 are timed beside it, each from assembly to a certified 1e-8 residual.
 
 The roofline object is for the dominant kernel, the FP32 3x3 blocked SpMV
-(k_bsrmv_tiles; the CG's k_cg_spmv runs the same tile loop): algorithmic
+over the lane-planar copy of the values (k_bsrmv_planar; the mixed CG's
+inner k_cg_spmv_pl runs the same tile loop): algorithmic
 bytes per launch = 36*nnz (values) + 4*nnz (colidx) + 4*(N+1) (rowptr) +
 12*N (x) + 12*N (y), timed with HIP events on the stream it runs on.
 """
@@ -48,6 +49,13 @@ SPMV_TRAFFIC_HEX192 = 1_743_427_658
 # Cache serves as well as HBM reads)
 STENCIL_TRAFFIC_HEX192 = 1_364_080_041
 METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
+# The mixed solvers certify ||b - A x|| / ||b|| in FP64 on the hi + lo split
+# of the matrix, which is the FP64 matrix to ~2^-48 per entry; on the stiff
+# 96^3 system that difference moved the residual by ~2e-11 against the
+# oracle's FP64 matrix (9.989e-9 certified, 1.0011e-8 by the oracle).  The
+# solves therefore certify at 0.99e-8, so that the residual against the
+# oracle's FP64 matrix, reported beside it, also meets 1e-8.
+CERT_TOL = 0.99e-8
 
 
 def parse():
@@ -166,10 +174,18 @@ def main():
     xs = torch.randn(ndof, dtype=torch.float32, device="cuda")
     ys = torch.empty_like(xs)
     ctx.bsrmv(A32, xs, ys)
-    spmv_ms = timed(lambda: ctx.bsrmv(A32, xs, ys), args.spmv_reps)
+    spmv_rm_ms = timed(lambda: ctx.bsrmv(A32, xs, ys), args.spmv_reps)
     spmv_bytes = 40 * nnz + 4 * (nv + 1) + 24 * nv
+    # the same product over the lane-planar copy of the values (the layout the
+    # mixed CG's inner SpMV streams): the same bits
+    Ap = ctx.bsr_planar(A32)
+    yp = torch.empty_like(xs)
+    ctx.bsrmv(Ap, xs, yp)
+    assert torch.equal(yp, ys), "planar SpMV differs from the row-major one"
+    spmv_ms = timed(lambda: ctx.bsrmv(Ap, xs, yp), args.spmv_reps)
     spmv_gbs = spmv_bytes / spmv_ms / 1e6
-    log(f"[bench] spmv f32 {spmv_ms * 1e3:.1f} us  {spmv_gbs:.0f} GB/s")
+    log(f"[bench] spmv f32 planar {spmv_ms * 1e3:.1f} us  {spmv_gbs:.0f} GB/s; row-major "
+        f"{spmv_rm_ms * 1e3:.1f} us  {spmv_bytes / spmv_rm_ms / 1e6:.0f} GB/s")
 
     if args.kernels_only:
         # a copy of known size for the counter calibration: y = 1*x + 0*y on
@@ -179,7 +195,8 @@ def main():
         copy_ms = timed(lambda: ctx.axpby(1.0, xa, 0.0, ya), 5)
         print(json.dumps({"kernels_only": True, "index_ms": index_ms, "assembly_ms": asm_ms,
                           "assembly_GBps": asm_gbs, "spmv_us": spmv_ms * 1e3,
-                          "spmv_GBps": spmv_gbs, "axpby_2GiB_read_1GiB_write_ms": copy_ms}))
+                          "spmv_GBps": spmv_gbs, "spmv_rowmajor_us": spmv_rm_ms * 1e3,
+                          "axpby_2GiB_read_1GiB_write_ms": copy_ms}))
         ctx.close()
         return
 
@@ -192,7 +209,8 @@ def main():
         f"training steps {tr['step_seconds']:.1f}s)")
 
     def opts(**kw):
-        return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
+        return ctx.cg_options(maxiters=args.cg_maxiters, tol=CERT_TOL, rel=1, check_every=64,
+                              **kw)
 
     # ---- the step: assemble + network + learned mixed PCG to 1e-8 ---------------------------------
     last = {}
@@ -333,7 +351,10 @@ def main():
         "assembly_GBps": round(asm_gbs, 1),
         "spmv_us": round(spmv_ms * 1e3, 2),
         "spmv_GBps": round(spmv_gbs, 1),
-        "roofline": {"bound": "hbm", "kernel": "k_bsrmv_tiles<SMH_F32,3> (FP32 3x3 BSR SpMV)",
+        "spmv_rowmajor_us": round(spmv_rm_ms * 1e3, 2),
+        "roofline": {"bound": "hbm",
+                     "kernel": ("k_bsrmv_planar (FP32 3x3 BSR SpMV over lane-planar values; "
+                                "the mixed CG's inner k_cg_spmv_pl runs the same tile loop)"),
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
                      "traffic": SPMV_TRAFFIC_96 if n == 96 else None,
@@ -440,7 +461,8 @@ def run_hex(args, ctx):
         f"{tr['seconds']:.1f}s")
 
     def opts(**kw):
-        return ctx.cg_options(maxiters=args.cg_maxiters, tol=1e-8, rel=1, check_every=64, **kw)
+        return ctx.cg_options(maxiters=args.cg_maxiters, tol=CERT_TOL, rel=1, check_every=64,
+                              **kw)
 
     corr_st = torch.empty(27 * nv, dtype=torch.int16, device="cuda")  # FP16 slots
     last = {}

@@ -40,6 +40,8 @@ extern "C" {
 #define SMH_F32 1   /* float values                                          */
 #define SMH_F32X2 2 /* float hi + float lo = the FP64 value to ~2^-48        */
 #define SMH_F32D 3  /* float values, double vectors (smh_bsrmv, smh_cg_solve) */
+#define SMH_F32P 4  /* float 3x3 values in the lane-planar layout of
+                       smh_bsr_planar, float vectors (smh_bsrmv only)       */
 
 /* index build flags */
 #define SMH_INDEX_FORCE_DIAG 1u /* every row holds its own vertex            */
@@ -251,7 +253,8 @@ int smh_assemble_edge_blocks(smh_ctx *ctx, const smh_index *idx,
 
 /* ---- linear algebra --------------------------------------------------------
  * y = A*x: the blocked row loop of lower_indexexprs.cpp:630-667.  x and y
- * are float for SMH_F32 and double for SMH_F64 / SMH_F32X2.  exact_order=1
+ * are float for SMH_F32 / SMH_F32P and double for SMH_F64 / SMH_F32X2 /
+ * SMH_F32D.  exact_order=1
  * (SMH_F64 only) sums each row in the reference's ascending ij, bj order
  * with no fused multiply-add: bit-identical to the CPU loop. */
 int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
@@ -262,6 +265,16 @@ int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
  * precision SMH_F64: double inputs; SMH_F32: float inputs. */
 int smh_dot(smh_ctx *ctx, int32_t precision, uint64_t n, const void *x,
             const void *y, double *result);
+/* The lane-planar layout of FP32 3x3 values (precision SMH_F32P), for the
+ * SpMV's loads.  No counterpart in the reference, which keeps the row-major
+ * block layout above (src/runtime.h:21-44): the same values, reordered.  The
+ * blocks are cut into groups of 64 consecutive blocks; group g occupies the
+ * 576 floats at d_out + 576*g: entries 0..3 of its 64 blocks (4 floats per
+ * block), then entries 4..7, then entry 8 (entry 3*bi + bj of a block).
+ * d_out holds 576*ceil(nnz/64) floats.  A: bdim 3, SMH_F32 (or the hi part
+ * of SMH_F32X2).  smh_cg_solve with mixed = 1 builds such a copy of the hi
+ * values itself (kept per pattern in the context until smh_bsr_forget). */
+int smh_bsr_planar(smh_ctx *ctx, const smh_bsr *A, float *d_out);
 /* smh_bsrmv and smh_cg_solve cut a pattern into row tiles on first use and
  * cache that plan per context, keyed by (rowptr pointer, nrows, nnz).  Call
  * this before a rowptr buffer is freed or refilled with another pattern of

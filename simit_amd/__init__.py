@@ -19,7 +19,7 @@ from pathlib import Path
 __all__ = [
     "SimitError", "load", "Context", "Index", "Bsr", "CGOptions", "CGStats",
     "SpringsParams", "TetParams", "HexParams", "FemTetParams", "Stencil27",
-    "F64", "F32", "F32X2", "F32D",
+    "F64", "F32", "F32X2", "F32D", "F32P",
     "PRECOND_NONE", "PRECOND_JACOBI", "PRECOND_BLOCK_JACOBI", "PRECOND_LEARNED",
     "INDEX_FORCE_DIAG", "LIB_PATH", "HEADER_PATH",
 ]
@@ -29,7 +29,8 @@ LIB_PATH = PKG / "lib" / "libsimit_hip.so"
 HEADER_PATH = PKG.parent / "include" / "simit_hip.h"
 
 OK, ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_NOMEM, ERR_BREAKDOWN, ERR_NOT_CONVERGED = range(7)
-F64, F32, F32X2, F32D = 0, 1, 2, 3  # F32D: float values, double vectors
+F64, F32, F32X2, F32D, F32P = 0, 1, 2, 3, 4  # F32D: float values, double vectors
+# F32P: FP32 3x3 values in the lane-planar layout (smh_bsr_planar), float vectors
 PRECOND_NONE, PRECOND_JACOBI, PRECOND_BLOCK_JACOBI, PRECOND_LEARNED = 0, 1, 2, 3
 STENCIL_PERIODIC = 1
 INDEX_FORCE_DIAG = 1
@@ -184,6 +185,7 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
         "smh_femtet_precompute": (C.c_int, [vp, vp, C.POINTER(FemTetParams),
                                             C.POINTER(FemTetSet), vp, vp, vp]),
         "smh_bsrmv": (C.c_int, [vp, C.POINTER(Bsr), vp, vp, i32]),
+        "smh_bsr_planar": (C.c_int, [vp, C.POINTER(Bsr), vp]),
         "smh_dot": (C.c_int, [vp, i32, u64, vp, vp, C.POINTER(C.c_double)]),
         "smh_stencil27_from_bsr": (C.c_int, [vp, C.POINTER(Bsr), u32, u32, u32, vp, vp]),
         "smh_stencil27_correction": (C.c_int, [vp, C.POINTER(Bsr), u32, u32, u32, vp]),
@@ -351,6 +353,18 @@ class Context:
     # ---- linear algebra -----------------------------------------------------
     def bsrmv(self, A: Bsr, x, y, exact_order: bool = False):
         self._check(self.lib.smh_bsrmv(self.h, C.byref(A), _ptr(x), _ptr(y), int(exact_order)))
+
+    def bsr_planar(self, A: Bsr, out=None):
+        """The lane-planar copy of A's FP32 3x3 values (a float32 tensor of
+        576 * ceil(nnz / 64) entries) and the SMH_F32P matrix over it."""
+        import torch
+        n = ((A.nnz + 63) // 64) * 576
+        if out is None:
+            out = torch.empty(max(n, 1), dtype=torch.float32, device="cuda")
+        self._check(self.lib.smh_bsr_planar(self.h, C.byref(A), _ptr(out)))
+        P = Bsr(A.nrows, A.bdim, A.nnz, A.rowptr, A.colidx, F32P, _ptr(out), None)
+        P._keep = (out, A)
+        return P
 
     def dot(self, x, y, precision: int | None = None) -> float:
         import torch

@@ -161,6 +161,33 @@ void forget_plans(smh_ctx *ctx, const uint32_t *rowptr) {
   }
 }
 
+// The lane-planar copy of a pattern's FP32 3x3 values that the CG solver's
+// inner SpMV streams: one device buffer per pattern, kept in the context
+// (dropped with the pattern's other plans) and refilled by each solve.
+struct PlanarBuf : smh_cached {
+  uint64_t nnz = 0;
+  float *vals = nullptr;
+  ~PlanarBuf() override { dfree(vals); }
+};
+
+int planar_buffer(smh_ctx *ctx, const smh_bsr *A, float **out) {
+  for (auto &c : ctx->cached) {
+    auto *pb = dynamic_cast<PlanarBuf *>(c.get());
+    if (pb && pb->rowptr == A->rowptr && pb->nnz == A->nnz) {
+      *out = pb->vals;
+      return SMH_OK;
+    }
+  }
+  std::unique_ptr<PlanarBuf> pb(new PlanarBuf);
+  pb->rowptr = A->rowptr;
+  pb->nnz = A->nnz;
+  const int rc = dmalloc(ctx, (void **)&pb->vals, sizeof(float) * planar_floats(A->nnz));
+  if (rc) return rc;
+  *out = pb->vals;
+  ctx->cached.push_back(std::move(pb));
+  return SMH_OK;
+}
+
 }  // namespace smh
 
 namespace {
@@ -213,6 +240,37 @@ k_bsrmv_tiles(TileDev A, const typename Val<PREC>::T *__restrict__ x,
   spmv_tiles<PREC, BD>(A, k0, k1, sm, xg, epi);
 }
 
+// y = A x over lane-planar FP32 3x3 values (SMH_F32P), float vectors
+__global__ void __launch_bounds__(TileShapePl::kThreads)
+k_bsrmv_planar(TileDev A, const float *__restrict__ x, float *__restrict__ y) {
+  __shared__ TileSmem<SMH_F32, 3, TileShapePl> sm;
+  uint32_t k0, k1;
+  wg_tiles(A, k0, k1);
+  XPlain<float> xg{x, 3};
+  EpiStore<float, 3> epi{y};
+  spmv_tiles<SMH_F32, 3>(A, k0, k1, sm, xg, epi);
+}
+
+// row-major FP32 3x3 values -> lane-planar (one thread per block; the
+// blocks past nnz in the last group are written as zeros)
+__global__ void __launch_bounds__(256)
+k_to_planar(uint64_t nnz, const float *__restrict__ v, float *__restrict__ out) {
+  const uint64_t b = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (b >= ((nnz + 63) & ~63ull)) return;
+  float e[9];
+  if (b < nnz) {
+    for (int k = 0; k < 9; ++k) e[k] = v[b * 9 + k];
+  } else {
+    for (int k = 0; k < 9; ++k) e[k] = 0.0f;
+  }
+  typedef float v4 __attribute__((ext_vector_type(4)));
+  float *g = out + (b >> 6) * kPlanarGroup;
+  const uint64_t l = b & 63u;
+  *(v4 *)(g + 4 * l) = v4{e[0], e[1], e[2], e[3]};
+  *(v4 *)(g + 256 + 4 * l) = v4{e[4], e[5], e[6], e[7]};
+  g[512 + l] = e[8];
+}
+
 // ---- CG device state ------------------------------------------------------------
 struct CGState {
   double rz, pq, alpha, beta, rr, rr_true, thr, rr_ref, bb, delta2;
@@ -222,11 +280,12 @@ struct CGState {
 
 __device__ __forceinline__ bool finite_pos(double v) { return v > 0.0 && v < INFINITY; }
 
-// fixed-order sum of np values (stride st) by one 256-thread workgroup
+// fixed-order sum of np values (stride st) by one 256-thread workgroup (the
+// update kernels sum the SpMV's 8,192 partials this way in every workgroup:
+// with one load in flight per thread that held each workgroup ~4 us before
+// its first streaming load)
 __device__ double wg_sum(const double *part, int np, int st, double *s4) {
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += 256) acc += part[(size_t)i * st];
-  return block_sum256(acc, s4);
+  return block_sum256(thread_sum(part, np, st), s4);
 }
 
 template <int BD>
@@ -315,6 +374,24 @@ k_cg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP>
   EpiPq<VT> epi{q, p, dlo, 0.0, BD};
   spmv_tiles<IP, BD>(A, k0, k1, sm, xg, epi);
   const double t = block_sum<TileShape<BD>::kWaves>(epi.d, sm.red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// K1 over lane-planar FP32 3x3 values (the mixed solver's inner SpMV)
+template <int IP>
+__global__ void __launch_bounds__(TileShapePl::kThreads)
+k_cg_spmv_pl(TileDev A, const typename Val<IP>::T *__restrict__ p,
+             typename Val<IP>::T *__restrict__ q, const CGState *S, const float *dlo,
+             double *part) {
+  using VT = typename Val<IP>::T;
+  __shared__ TileSmem<IP, 3, TileShapePl> sm;
+  if (S->done) return;
+  uint32_t k0, k1;
+  wg_tiles(A, k0, k1);
+  XPlain<VT> xg{p, 3};
+  EpiPq<VT> epi{q, p, dlo, 0.0, 3};
+  spmv_tiles<IP, 3>(A, k0, k1, sm, xg, epi);
+  const double t = block_sum<TileShapePl::kWaves>(epi.d, sm.red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
@@ -976,14 +1053,25 @@ StDev st_dev(const smh_stencil27 *S, const void *vals, const void *vals_lo) {
 template <int IP, int RP, int LP, int BD>
 struct TileOps {
   using VT = typename Val<IP>::T;
+  // the inner SpMV of FP32 3x3 values may run over a lane-planar copy
+  static constexpr bool kPlanar = BD == 3 && IP == SMH_F32;
   const smh_bsr *A = nullptr;
   TileDev T{}, TR{}, TC{};
-  unsigned gt = 0, grt = 0, gc = 0;
-  unsigned parts_spmv() const { return gt; }
+  TileDev TP{};  // lane-planar tiles (TP.vals set: the inner SpMV uses them)
+  unsigned gt = 0, grt = 0, gc = 0, gp = 0;
+  bool planar() const { return kPlanar && TP.vals != nullptr; }
+  unsigned parts_spmv() const { return planar() ? gp : gt; }
   unsigned parts_res() const { return grt; }
   unsigned parts_corr() const { return gc; }
   void spmv(hipStream_t s, const VT *p, VT *q, const CGState *S, const float *dlo,
             double *part) const {
+    if constexpr (kPlanar) {
+      if (planar()) {
+        hipLaunchKernelGGL((k_cg_spmv_pl<IP>), dim3(gp), dim3(TileShapePl::kThreads), 0, s, TP,
+                           p, q, S, dlo, part);
+        return;
+      }
+    }
     hipLaunchKernelGGL((k_cg_spmv<IP, BD>), dim3(gt), dim3(tile_threads_host(BD)), 0, s, T, p,
                        q, S, dlo, part);
   }
@@ -1331,9 +1419,19 @@ __global__ void __launch_bounds__(256) k_bcg_csum(BChunks C, const double *__res
 }
 
 // a system's total over its chunks, in chunk order (one thread)
+// (loads issued eight at a time, added in chunk order)
 __device__ __forceinline__ double sys_sum(const BChunks &C, uint32_t m, const double *cp) {
   double t = 0.0;
-  for (uint32_t c = C.mchunk[m]; c < C.mchunk[m + 1]; ++c) t += cp[c];
+  uint32_t c = C.mchunk[m];
+  const uint32_t e = C.mchunk[m + 1];
+  for (; c + 8 <= e; c += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) v[u] = cp[c + u];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) t += v[u];
+  }
+  for (; c < e; ++c) t += cp[c];
   return t;
 }
 
@@ -1582,17 +1680,52 @@ __global__ void __launch_bounds__(256) k_bcg_signal(const int *__restrict__ stat
   host[0] = c;
 }
 
+// The lane-planar inner SpMV's buffer and tile plan when the caller (the
+// allocation-free solver) provides them
+struct PlanarIn {
+  const smh_tile_plan *plan = nullptr;
+  float *vals = nullptr;
+};
+
 // CG over the blocked-CSR row tiles of A
 template <int IP, int RP, int BD>
 int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const smh_tile_plan *rplan,
              const double *b, double *x, const smh_cg_opts *o, smh_cg_stats *st,
-             Arena *ar = nullptr) {
+             Arena *ar = nullptr, const PlanarIn *pin = nullptr) {
   using VT = typename Val<IP>::T;
   constexpr int LP = BD == 3 ? (sizeof(VT) == 4 ? kFP8F : kFP8D)
                              : (sizeof(VT) == 4 ? SMH_F32 : kF32D);
   TileOps<IP, RP, LP, BD> ops;
-  const int rc = make_tile_ops<IP, RP, BD>(ctx, A, plan, rplan, o, &ops);
+  int rc = make_tile_ops<IP, RP, BD>(ctx, A, plan, rplan, o, &ops);
   if (rc) return rc;
+  if constexpr (decltype(ops)::kPlanar) {
+    // the inner SpMV streams a lane-planar copy of the FP32 values, kept per
+    // pattern in the context (refilled by every solve), or in the buffer the
+    // allocation-free solver carves from its scratch array
+    if (A->nnz > 0 && (!ar || (pin && pin->vals))) {
+      float *pv = nullptr;
+      smh_tile_plan pplan;
+      if (ar) {
+        pv = pin->vals;
+        pplan = *pin->plan;
+      } else {
+        rc = planar_buffer(ctx, A, &pv);
+        if (rc) return rc;
+        rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, (uint32_t)TileShapePl::kBlocks,
+                       &pplan);
+        if (rc) return rc;
+      }
+      if (pplan.tile_row) {
+        hipLaunchKernelGGL(k_to_planar, dim3(grid_for((A->nnz + 63) & ~63ull, 256)), dim3(256),
+                           0, ctx->stream, A->nnz, (const float *)A->vals, pv);
+        SMH_LAUNCH_CHECK(ctx);
+        smh_bsr Pb = *A;
+        Pb.vals = pv;
+        Pb.vals_lo = nullptr;
+        ops.TP = tile_dev(&Pb, &pplan, &ops.gp, false, SMH_TILE_GRID_PL);
+      }
+    }
+  }
   return cg_run<IP, RP, BD>(ctx, ops, A->nrows, A->precision == SMH_F32X2 && A->vals_lo, b, x, o,
                             st, ar);
 }
@@ -2043,6 +2176,20 @@ extern "C" int smh_bsr_forget(smh_ctx *ctx, const uint32_t *rowptr) {
   return SMH_OK;
 }
 
+extern "C" int smh_bsr_planar(smh_ctx *ctx, const smh_bsr *A, float *d_out) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, A && d_out && A->rowptr && (A->nnz == 0 || A->vals), SMH_ERR_INVALID,
+            "smh_bsr_planar: null argument");
+  SMH_CHECK(ctx, A->bdim == 3 && (A->precision == SMH_F32 || A->precision == SMH_F32X2),
+            SMH_ERR_INVALID,
+            "smh_bsr_planar: needs 3x3 float values (SMH_F32, or the hi part of SMH_F32X2)");
+  if (A->nnz == 0) return SMH_OK;
+  hipLaunchKernelGGL(k_to_planar, dim3(grid_for((A->nnz + 63) & ~63ull, 256)), dim3(256), 0,
+                     ctx->stream, A->nnz, (const float *)A->vals, d_out);
+  SMH_LAUNCH_CHECK(ctx);
+  return SMH_OK;
+}
+
 extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
                          int32_t exact_order) {
   if (!ctx) return SMH_ERR_INVALID;
@@ -2051,11 +2198,28 @@ extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
   SMH_CHECK(ctx, A->bdim == 1 || A->bdim == 3 || (exact_order && A->bdim >= 1 && A->bdim <= 8),
             SMH_ERR_UNSUPPORTED,
             "smh_bsrmv: block size %u (1 or 3; 1..8 with exact_order)", A->bdim);
-  SMH_CHECK(ctx, A->precision >= SMH_F64 && A->precision <= SMH_F32D, SMH_ERR_INVALID,
+  SMH_CHECK(ctx, A->precision >= SMH_F64 && A->precision <= SMH_F32P, SMH_ERR_INVALID,
             "smh_bsrmv: unknown precision %d", A->precision);
   SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
             "smh_bsrmv: SMH_F32X2 needs vals_lo");
+  SMH_CHECK(ctx, A->precision != SMH_F32P || (A->bdim == 3 && !exact_order), SMH_ERR_INVALID,
+            "smh_bsrmv: SMH_F32P holds 3x3 blocks (and has no exact_order form)");
   if (A->nrows == 0) return SMH_OK;
+  if (A->precision == SMH_F32P) {
+    smh_tile_plan pplan;
+    const uint32_t nb = (uint32_t)TileShapePl::kBlocks;
+    int rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, nb, &pplan);
+    if (rc) return rc;
+    SMH_CHECK(ctx, pplan.tile_row != nullptr, SMH_ERR_UNSUPPORTED,
+              "smh_bsrmv: a row holds %u blocks (at most %u with SMH_F32P)", pplan.maxrow,
+              nb / 2);
+    unsigned g = 0;
+    const TileDev T = tile_dev(A, &pplan, &g, false, SMH_TILE_GRID_PL);
+    hipLaunchKernelGGL(k_bsrmv_planar, dim3(g), dim3(TileShapePl::kThreads), 0, ctx->stream, T,
+                       (const float *)x, (float *)y);
+    SMH_LAUNCH_CHECK(ctx);
+    return SMH_OK;
+  }
   if (exact_order) {
     SMH_CHECK(ctx, A->precision == SMH_F64, SMH_ERR_INVALID,
               "smh_bsrmv: exact_order needs SMH_F64 values");
@@ -2238,12 +2402,13 @@ int cg_validate(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x, cons
 
 // the solve of smh_cg_solve / smh_cg_solve_noalloc over a tile plan
 int cg_dispatch(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const double *b,
-                double *x, const smh_cg_opts *o, smh_cg_stats *st, Arena *ar) {
+                double *x, const smh_cg_opts *o, smh_cg_stats *st, Arena *ar,
+                const PlanarIn *pin = nullptr) {
   const int bd = (int)A->bdim;
   auto run = [&](const smh_cg_opts *oo, smh_cg_stats *so) {
-#define SMH_CG(IP, RP)                                                             \
-  (A->bdim == 3 ? cg_tiles<IP, RP, 3>(ctx, A, plan, plan, b, x, oo, so, ar) \
-                : cg_tiles<IP, RP, 1>(ctx, A, plan, plan, b, x, oo, so, ar))
+#define SMH_CG(IP, RP)                                                                  \
+  (A->bdim == 3 ? cg_tiles<IP, RP, 3>(ctx, A, plan, plan, b, x, oo, so, ar, pin) \
+                : cg_tiles<IP, RP, 1>(ctx, A, plan, plan, b, x, oo, so, ar, pin))
     if (oo->mixed == 2) return SMH_CG(kF32D, SMH_F32X2);
     if (oo->mixed) return SMH_CG(SMH_F32, SMH_F32X2);
     if (A->precision == SMH_F64) return SMH_CG(SMH_F64, SMH_F64);
@@ -2300,6 +2465,12 @@ extern "C" int smh_cg_solve_noalloc(smh_ctx *ctx, const smh_bsr *A, const double
   if (o->precond != SMH_PRECOND_NONE) need += vt * nrows * bb + 256;  // (block) Jacobi
   if (o->mixed && A->precision == SMH_F32X2) need += 4 * nrows * bb + 256;  // low row sums
   need += sizeof(CGState) + 256 + sizeof(double) * 2 * kPartSlots + 64 + 256;
+  // mixed 3x3 (FP32 vectors): the lane-planar copy of the values and its tile
+  // rows, as smh_cg_solve streams them
+  const bool want_pl = bd == 3 && o->mixed == 1;
+  const uint32_t tbp = (uint32_t)TileShapePl::kBlocks;
+  const uint64_t max_tiles_pl = A->nnz / (tbp / 2 + 1) + 1;
+  if (want_pl) need += sizeof(float) * planar_floats(A->nnz) + 256 + 4 * (max_tiles_pl + 1) + 256;
   SMH_CHECK(ctx, need <= ctx->scratch_bytes, SMH_ERR_UNSUPPORTED,
             "%s: this system needs %llu bytes of scratch; the maximum is %zu (use smh_cg_solve)",
             who, (unsigned long long)need, ctx->scratch_bytes);
@@ -2330,7 +2501,24 @@ extern "C" int smh_cg_solve_noalloc(smh_ctx *ctx, const smh_bsr *A, const double
   hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(nt + 1, 256)), dim3(256), 0, ctx->stream,
                      A->rowptr, A->nrows, plan.tprime, plan.ntiles, plan.tile_row);
   SMH_LAUNCH_CHECK(ctx);
-  return cg_dispatch(ctx, A, &plan, b, x, o, st, &ar);
+  smh_tile_plan pplan;
+  PlanarIn pin;
+  if (want_pl && A->nnz > 0 && maxrow <= tbp / 2) {
+    pplan = plan;
+    pplan.tile_blocks = tbp;
+    pplan.tprime = tbp - (maxrow ? maxrow : 1) + 1;
+    const uint64_t ntp = std::max<uint64_t>(1, (A->nnz + pplan.tprime - 1) / pplan.tprime);
+    pplan.ntiles = (uint32_t)ntp;
+    pplan.tile_row = (uint32_t *)ar.take(sizeof(uint32_t) * (ntp + 1));
+    pin.vals = (float *)ar.take(sizeof(float) * planar_floats(A->nnz));
+    SMH_CHECK(ctx, ntp <= max_tiles_pl && pplan.tile_row && pin.vals, SMH_ERR_UNSUPPORTED,
+              "%s: the planar values do not fit the scratch array", who);
+    hipLaunchKernelGGL(k_tile_rows, dim3(grid_for(ntp + 1, 256)), dim3(256), 0, ctx->stream,
+                       A->rowptr, A->nrows, pplan.tprime, pplan.ntiles, pplan.tile_row);
+    SMH_LAUNCH_CHECK(ctx);
+    pin.plan = &pplan;
+  }
+  return cg_dispatch(ctx, A, &plan, b, x, o, st, &ar, &pin);
 }
 
 namespace {

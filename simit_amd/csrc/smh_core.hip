@@ -163,8 +163,7 @@ k_axpby(uint64_t n, double a, const double *__restrict__ x, double b, double *__
 __global__ void __launch_bounds__(256)
 k_sum_partials(const double *__restrict__ part, int np, double *__restrict__ out) {
   __shared__ double s[4];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += 256) acc += part[i];
+  double acc = thread_sum(part, np);
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = acc;
   __syncthreads();

@@ -1009,9 +1009,7 @@ __global__ void k_gnn_wt(int in, int out, const float *__restrict__ W, uint16_t 
 
 __global__ void k_gnn_sum(int n, const double *__restrict__ part, double *__restrict__ out) {
   __shared__ double s4[4];
-  double a = 0.0;
-  for (int i = threadIdx.x; i < n; i += 256) a += part[i];
-  const double t = block_sum256(a, s4);
+  const double t = block_sum256(thread_sum(part, n), s4);
   if (threadIdx.x == 0) *out = t;
 }
 
@@ -1458,9 +1456,7 @@ k_gnn_cmv(TileDev C, const float *__restrict__ v, float *__restrict__ y,
 __global__ void __launch_bounds__(256) k_lz_sum(const double *part, int np, double *ab, int j,
                                                 int root) {
   __shared__ double s4[4];
-  double acc = 0.0;
-  for (int i = threadIdx.x; i < np; i += 256) acc += part[i];
-  const double t = block_sum256(acc, s4);
+  const double t = block_sum256(thread_sum(part, np), s4);
   if (threadIdx.x == 0) ab[j] = root ? sqrt(t) : t;
 }
 

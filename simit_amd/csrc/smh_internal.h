@@ -151,6 +151,26 @@ int scratch_setup(smh_ctx *ctx);
 
 }  // namespace smh
 
+// ---- fixed-order partial sums ----------------------------------------------
+// This thread's share of a sum over part[i * st]: i = threadIdx.x,
+// threadIdx.x + nt, ... added in that order, the loads issued 16 at a time
+// (a load-add loop waits one L2 round trip per value).
+__device__ __forceinline__ double thread_sum(const double *part, int np, int st = 1,
+                                             int nt = 256) {
+  constexpr int U = 16;
+  double acc = 0.0;
+  int i = threadIdx.x;
+  for (; i + (U - 1) * nt < np; i += U * nt) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = part[(size_t)(i + u * nt) * st];
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += v[u];
+  }
+  for (; i < np; i += nt) acc += part[(size_t)i * st];
+  return acc;
+}
+
 // ---- wave-level helpers (64-wide wavefronts) ------------------------------
 __device__ __forceinline__ double wave_sum(double v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);

@@ -83,6 +83,63 @@ def test_fast_spmv_all_precisions(ctx, oracle, kind, n):
     assert (np.abs(yh.cpu().numpy() - ref) <= 2.0 ** -45 * absrow).all()
 
 
+def planar_host(v32, nnz):
+    """The lane-planar layout of FP32 3x3 values restated on the host: groups
+    of 64 blocks, entries 0..3 of the group's blocks, then 4..7, then 8."""
+    G = (nnz + 63) // 64
+    A = np.zeros((G * 64, 9), dtype=np.float32)
+    A[:nnz] = v32.reshape(nnz, 9)
+    A = A.reshape(G, 64, 9)
+    return np.concatenate([A[:, :, 0:4].reshape(G, 256), A[:, :, 4:8].reshape(G, 256),
+                           A[:, :, 8]], axis=1).ravel()
+
+
+@pytest.mark.parametrize("kind,n", [("box", 10), ("tet", 8), ("tet", 40), ("ragged3", 20),
+                                    ("ragged3", 2), ("box", 2)])
+def test_planar_spmv(ctx, oracle, kind, n):
+    """SMH_F32P (the layout the mixed CG's inner SpMV streams) gives the same
+    products as the row-major FP32 tiles, bit for bit: each block row is
+    summed in the same order and each row's blocks in ascending order."""
+    idx, rp, ci, vals, xv, bd = random_bsr(ctx, oracle, kind, n)
+    assert bd == 3
+    nnz = len(ci)
+    v32, x32 = vals.astype(np.float32), xv.astype(np.float32)
+    A = idx.bsr(3, S.F32, to_dev(v32))
+    P = ctx.bsr_planar(A)
+    pv = P._keep[0].cpu().numpy()
+    np.testing.assert_array_equal(pv, planar_host(v32, nnz))
+    y32 = torch.empty(len(xv), dtype=torch.float32, device="cuda")
+    yp = torch.full_like(y32, float("nan"))
+    ctx.bsrmv(P, to_dev(x32), yp)
+    if np.diff(rp).max() <= 168:
+        ctx.bsrmv(A, to_dev(x32), y32)
+        assert torch.equal(yp, y32)
+    else:
+        # planar tiles hold 512 blocks (rows up to 256); row-major ones 336
+        with pytest.raises(S.SimitError) as e:
+            ctx.bsrmv(A, to_dev(x32), y32)
+        assert e.value.code == S.ERR_UNSUPPORTED
+    ref32 = oracle.bsrmv(rp, ci, v32.astype(np.float64), x32.astype(np.float64), 3)
+    absrow = oracle.bsrmv(rp, ci, np.abs(v32).astype(np.float64),
+                          np.abs(x32).astype(np.float64), 3)
+    assert (np.abs(yp.cpu().numpy() - ref32) <= 128 * 2.0 ** -24 * absrow + 1e-30).all()
+    # the planar layout holds 3x3 blocks and has no exact-order form
+    with pytest.raises(S.SimitError) as e:
+        ctx.bsrmv(P, to_dev(x32), yp, exact_order=True)
+    assert e.value.code == S.ERR_INVALID
+
+
+def test_planar_rejects_other_values(ctx, oracle):
+    idx, rp, ci, vals, xv, bd = random_bsr(ctx, oracle, "hex", 4)
+    with pytest.raises(S.SimitError) as e:
+        ctx.bsr_planar(idx.bsr(1, S.F32, to_dev(vals.astype(np.float32))))
+    assert e.value.code == S.ERR_INVALID
+    idx, rp, ci, vals, xv, bd = random_bsr(ctx, oracle, "tet", 3)
+    with pytest.raises(S.SimitError) as e:
+        ctx.bsr_planar(idx.bsr(3, S.F64, to_dev(vals)))
+    assert e.value.code == S.ERR_INVALID
+
+
 def test_gemv_blocked_fixture(ctx, oracle, fixtures):
     fx = fixtures["gemv_blocked"]
     ep = np.array(fx["springs"], np.int32)
