@@ -1722,7 +1722,7 @@ int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const sm
         smh_bsr Pb = *A;
         Pb.vals = pv;
         Pb.vals_lo = nullptr;
-        ops.TP = tile_dev(&Pb, &pplan, &ops.gp, false, SMH_TILE_GRID_PL);
+        ops.TP = tile_dev(&Pb, &pplan, &ops.gp, true, SMH_TILE_GRID_PL);
       }
     }
   }
@@ -2214,7 +2214,7 @@ extern "C" int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
               "smh_bsrmv: a row holds %u blocks (at most %u with SMH_F32P)", pplan.maxrow,
               nb / 2);
     unsigned g = 0;
-    const TileDev T = tile_dev(A, &pplan, &g, false, SMH_TILE_GRID_PL);
+    const TileDev T = tile_dev(A, &pplan, &g, true, SMH_TILE_GRID_PL);
     hipLaunchKernelGGL(k_bsrmv_planar, dim3(g), dim3(TileShapePl::kThreads), 0, ctx->stream, T,
                        (const float *)x, (float *)y);
     SMH_LAUNCH_CHECK(ctx);

@@ -103,8 +103,12 @@ struct TileShapeB {
 #ifndef SMH_TILE_ROUNDS_PL
 #define SMH_TILE_ROUNDS_PL 8
 #endif
-// persistent grid of the planar kernels (8 rounds: 8,192 workgroups measured
-// best against 2,048 - 16,384 at 96^3 x 5)
+// Grid of the planar kernels.  Their workgroups stride over their XCD's
+// eighth of the tiles (TileDev::interleave), so the tiles in flight on one
+// XCD form one moving window (its L2 holds the x they gather) and the load
+// stays even for any grid.  Measured at 96^3 x 5 tets: SpMV 91 -> 87 us at
+// 8,192 workgroups (84 us at 16,384, but the solver's update kernel sums one
+// partial per SpMV workgroup: 8,192 gave the shortest CG iteration).
 #ifndef SMH_TILE_GRID_PL
 #define SMH_TILE_GRID_PL 8192
 #endif
