@@ -663,15 +663,27 @@ def run_batch(args, ctx):
     ctx.bsrmv(A32, xs, ys)
     stream = torch.cuda.current_stream()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    for _ in range(20):
-        ctx.bsrmv(A32, xs, ys)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    spmv_ms = e0.elapsed_time(e1) / 20
+
+    def time_mv(A, y):
+        e0.record(stream)
+        for _ in range(20):
+            ctx.bsrmv(A, xs, y)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 20
+    spmv_rm_ms = time_mv(A32, ys)
+    # the lane-planar copy (the shape the batched solver's autotuner picks for
+    # these lattices): the same bits
+    Ap = ctx.bsr_planar(A32)
+    yp = torch.empty_like(xs)
+    ctx.bsrmv(Ap, xs, yp)
+    assert torch.equal(yp, ys), "planar SpMV differs from the row-major one"
+    spmv_ms = time_mv(Ap, yp)
     spmv_bytes = 40 * nnz + 4 * (npts + 1) + 24 * npts
     spmv_gbs = spmv_bytes / spmv_ms / 1e6
-    log(f"[bench] spmv f32 {spmv_ms * 1e3:.1f} us {spmv_gbs:.0f} GB/s")
+    log(f"[bench] spmv f32 planar {spmv_ms * 1e3:.1f} us {spmv_gbs:.0f} GB/s; row-major "
+        f"{spmv_rm_ms * 1e3:.1f} us {spmv_bytes / spmv_rm_ms / 1e6:.0f} GB/s")
+    del Ap
 
     # the step: assembly and the batched CG of every lattice in one call,
     # the assembly inside the solver's first captured graph
@@ -760,7 +772,10 @@ def run_batch(args, ctx):
         "iterations_max": max(iters), "one_at_a_time_seconds": round(seq_s, 4),
         "speedup_vs_one_at_a_time": round(seq_s / step_s, 2), "index_build_ms": round(index_ms, 2),
         "spmv_us": round(spmv_ms * 1e3, 1), "spmv_GBps": round(spmv_gbs, 1),
-        "roofline": {"bound": "hbm", "kernel": "k_bsrmv_tiles<SMH_F32,3> (FP32 3x3 BSR SpMV)",
+        "spmv_rowmajor_us": round(spmv_rm_ms * 1e3, 1),
+        "roofline": {"bound": "hbm",
+                     "kernel": ("k_bsrmv_planar (FP32 3x3 BSR SpMV over lane-planar values; "
+                                "the batched solver's k_bcg_spmv_pl runs the same tile loop)"),
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes": spmv_bytes},

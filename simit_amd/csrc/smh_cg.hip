@@ -1121,7 +1121,9 @@ int make_tile_ops(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan,
               "smh_cg_solve: a row holds %u blocks (the learned correction supports %u)",
               lplan.maxrow, tile_blocks_of<LP, BD>() / 2);
   }
-  ops.TC = tile_dev(&Cb, &lplan, &ops.gc, kIl);
+  // the 3x3 correction pass strides over its XCD's tiles too (96^3 x 5:
+  // 59 -> 57 us per pass)
+  ops.TC = tile_dev(&Cb, &lplan, &ops.gc, kIl || BD == 3);
   return SMH_OK;
 }
 
@@ -1385,6 +1387,21 @@ k_bcg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP
   XPlain<VT> xg{p, BD};
   EpiPqRow<VT> epi{q, p, dlo, prow, BD};
   spmv_tiles<IP, BD>(A, k0, k1, sm, xg, epi);
+}
+
+// K1 over lane-planar FP32 3x3 values (the batched mixed solver's planar
+// shape)
+template <int IP>
+__global__ void __launch_bounds__(TileShapePl::kThreads)
+k_bcg_spmv_pl(TileDev A, const typename Val<IP>::T *__restrict__ p,
+              typename Val<IP>::T *__restrict__ q, const float *dlo, double *prow) {
+  using VT = typename Val<IP>::T;
+  __shared__ TileSmem<IP, 3, TileShapePl> sm;
+  uint32_t k0, k1;
+  wg_tiles(A, k0, k1);
+  XPlain<VT> xg{p, 3};
+  EpiPqRow<VT> epi{q, p, dlo, prow, 3};
+  spmv_tiles<IP, 3>(A, k0, k1, sm, xg, epi);
 }
 
 // Chunks: crow[c]..crow[c+1] are the rows of chunk c, all in system cmesh[c];
@@ -1732,21 +1749,26 @@ int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const sm
 
 // The batched SpMV's tile shape and grid: for 3x3 blocks, every (rounds per
 // lane, threads per workgroup) shape below with 1,024, 2,048 and 4,096
-// workgroups, the kernel itself (epilogue included) timed on the pattern;
-// scalar systems keep the default shape and tune the grid only.  Chosen once
-// per pattern and precision, cached in the context.
+// workgroups, and for FP32 inner products the lane-planar shape with 4,096,
+// 8,192 and 12,288; the kernel itself (epilogue included) timed on the
+// pattern (five launches after a warm-up); scalar systems keep the default
+// shape and tune the grid only.  Chosen once per pattern and precision,
+// cached in the context.
 using BShape0 = TileShape<3>;
 using BShape1 = TileShapeR<3, 8, 64>;
 using BShape2 = TileShapeR<3, 24, 64>;
 using BShape3 = TileShapeR<3, 8, 128>;
 using BShape4 = TileShapeR<3, 16, 128>;
-constexpr int kBShapes = 5;
+// shape 5: lane-planar FP32 values (TileShapePl; FP32 inner products only),
+// with workgroups striding over their XCD's tiles
+constexpr int kBShapes = 5, kBShapePlanar = 5;
 inline uint32_t bshape_blocks(int k) {
   switch (k) {
     case 1: return BShape1::kBlocks;
     case 2: return BShape2::kBlocks;
     case 3: return BShape3::kBlocks;
     case 4: return BShape4::kBlocks;
+    case kBShapePlanar: return TileShapePl::kBlocks;
   }
   return BShape0::kBlocks;
 }
@@ -1756,6 +1778,7 @@ inline unsigned bshape_threads(int k) {
     case 2: return BShape2::kThreads;
     case 3: return BShape3::kThreads;
     case 4: return BShape4::kThreads;
+    case kBShapePlanar: return TileShapePl::kThreads;
   }
   return BShape0::kThreads;
 }
@@ -1781,6 +1804,13 @@ void launch_bcg_spmv(int shape, unsigned g, hipStream_t s, const TileDev &T,
         hipLaunchKernelGGL((k_bcg_spmv<IP, 3, BShape4>), dim3(g), dim3(BShape4::kThreads), 0, s, T,
                            p, q, dlo, prow);
         return;
+      case kBShapePlanar:
+        if constexpr (IP == SMH_F32) {
+          hipLaunchKernelGGL((k_bcg_spmv_pl<IP>), dim3(g), dim3(TileShapePl::kThreads), 0, s, T,
+                             p, q, dlo, prow);
+          return;
+        }
+        break;
     }
   }
   hipLaunchKernelGGL((k_bcg_spmv<IP, BD>), dim3(g), dim3(tile_threads_host(BD)), 0, s, T, p, q,
@@ -1795,7 +1825,7 @@ struct BTune {
 template <int IP, int BD>
 BTune tuned_bcg(smh_ctx *ctx, hipStream_t stream, const smh_bsr *A,
                 const smh_tile_plan *default_plan, void *pbuf, void *qbuf, const float *dlo,
-                double *prow) {
+                double *prow, const float *planar) {
   using VT = typename Val<IP>::T;
   for (const auto &t : ctx->tuned)
     if (t.rowptr == A->rowptr && t.nrows == A->nrows && t.nnz == A->nnz && t.bdim == A->bdim &&
@@ -1814,22 +1844,30 @@ BTune tuned_bcg(smh_ctx *ctx, hipStream_t stream, const smh_bsr *A,
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return best;
   const unsigned cands[] = {1024, 2048, 4096};
-  const int nshapes = BD == 3 ? kBShapes : 1;
+  const unsigned cands_pl[] = {4096, 8192, 12288};
+  const int nshapes = BD == 3 ? kBShapes + (planar ? 1 : 0) : 1;
   for (int k = 0; k < nshapes; ++k) {
     smh_tile_plan pl;
     if (tile_plan(ctx, A->rowptr, A->nrows, A->nnz,
                   BD == 3 ? bshape_blocks(k) : tile_blocks_host(BD), &pl) != SMH_OK ||
         !pl.tile_row)
       continue;
-    for (unsigned g : cands) {
+    const bool is_pl = k == kBShapePlanar;
+    for (unsigned g : is_pl ? cands_pl : cands) {
       unsigned gg = 0;
-      TileDev T = tile_dev(A, &pl, &gg);
+      smh_bsr Ap = *A;
+      if (is_pl) {
+        Ap.vals = planar;
+        Ap.vals_lo = nullptr;
+      }
+      TileDev T = tile_dev(&Ap, &pl, &gg, is_pl);
       gg = std::min<unsigned>(g, pl.ntiles);
       if (gg >= 8) gg &= ~7u;
       if (gg < 1) gg = 1;
+      T.interleave = is_pl && gg >= 8 ? 1u : 0u;
       T.tiles_per_wg = (uint32_t)((pl.ntiles + gg - 1) / gg);
       float ms = 0.0f;
-      for (int rep = 0; rep < 4; ++rep) {
+      for (int rep = 0; rep < 6; ++rep) {
         (void)hipEventRecord(e0, stream);
         launch_bcg_spmv<IP, BD>(k, gg, stream, T, (const VT *)pbuf, (VT *)qbuf, dlo, prow);
         (void)hipEventRecord(e1, stream);
@@ -1948,6 +1986,17 @@ int bcg_build(smh_ctx *ctx, BPlan &P, const smh_bsr *Am, const smh_tile_plan *pl
   if (o->mixed && Am->precision == SMH_F32X2 && Am->vals_lo &&
       (rc = alloc(&dlo, sizeof(float) * (uint64_t)nrows * BD * BD)))
     return rc;
+  // FP32 3x3 inner products may stream a lane-planar copy of the values
+  // (made from the prologue's values by the set-up of every replayed solve)
+  void *planar = nullptr;
+  if (BD == 3 && IP == SMH_F32 && Am->nnz > 0 &&
+      (rc = alloc(&planar, sizeof(float) * planar_floats(Am->nnz))))
+    return rc;
+  auto to_planar = [&]() {
+    if (planar)
+      hipLaunchKernelGGL(k_to_planar, dim3(grid_for((Am->nnz + 63) & ~63ull, 256)), dim3(256), 0,
+                         s, Am->nnz, (const float *)Am->vals, (float *)planar);
+  };
   const BDev S{sbuf, sbuf + nb, sbuf + 2 * nb, sbuf + 3 * nb, sbuf + 4 * nb, sbuf + 5 * nb,
                sbuf + 6 * nb, sbuf + 7 * nb, ibuf, ibuf + nb, ibuf + 2 * nb};
   P.S = S;
@@ -1971,13 +2020,22 @@ int bcg_build(smh_ctx *ctx, BPlan &P, const smh_bsr *Am, const smh_tile_plan *pl
               t.bdim == Am->bdim && t.prec == IP;
   if (dlo && !cached)  // the tuning runs the kernel with its epilogue: finite row sums
     SMH_HIP(ctx, hipMemsetAsync(dlo, 0, sizeof(float) * (uint64_t)nrows * BD * BD, s));
-  const BTune tune = tuned_bcg<IP, BD>(ctx, s, Am, plan, p, q, (const float *)dlo, prz);
+  to_planar();
+  const BTune tune =
+      tuned_bcg<IP, BD>(ctx, s, Am, plan, p, q, (const float *)dlo, prz, (const float *)planar);
   unsigned g = 0, gsp = tune.grid;
   const TileDev T = tile_dev(Am, plan, &g);  // the residual kernel's tiles (default shape)
   unsigned gdef = 0;
-  TileDev Ts = tile_dev(Am, &tune.plan, &gdef);
+  const bool use_pl = tune.shape == kBShapePlanar && planar;
+  smh_bsr Asp = *Am;
+  if (use_pl) {
+    Asp.vals = planar;
+    Asp.vals_lo = nullptr;
+  }
+  TileDev Ts = tile_dev(&Asp, &tune.plan, &gdef, use_pl);
   if (gsp) Ts.tiles_per_wg = (uint32_t)((tune.plan.ntiles + gsp - 1) / gsp);
   else gsp = gdef;
+  Ts.interleave = use_pl && gsp >= 8 ? 1u : 0u;
   // the SpMV streams the tiles of systems still iterating only
   Ts.row_sys = mesh;
   Ts.sys_state = S.state;
@@ -2004,6 +2062,7 @@ int bcg_build(smh_ctx *ctx, BPlan &P, const smh_bsr *Am, const smh_tile_plan *pl
   // z = M r, p = z for every system; the replay counter restarts at zero
   auto setup = [&]() {
     hipLaunchKernelGGL(k_fill_u64, dim3(1), dim3(64), 0, s, counter, 8, 0ull);
+    if (use_pl) to_planar();
     if (pc != SMH_PRECOND_NONE)
       hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(nrows, 256)), dim3(256), 0, s, nrows,
                          Am->rowptr, Am->colidx, Am->vals, Am->vals_lo, BD, pc, (VT *)dinv);
