@@ -587,22 +587,6 @@ void or_bsrmv(int32_t nrows, int32_t bdim, const uint32_t *rowptr,
   }
 }
 
-void or_bsrmv_f32(int32_t nrows, int32_t bdim, const uint32_t *rowptr,
-                  const uint32_t *colidx, const float *vals, const float *x,
-                  float *y) {
-  const int bb = bdim * bdim;
-  for (int32_t i = 0; i < nrows; ++i) {
-    float tmp[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t ij = rowptr[i]; ij < rowptr[i + 1]; ++ij) {
-      uint32_t j = colidx[ij];
-      for (int bi = 0; bi < bdim; ++bi)
-        for (int bj = 0; bj < bdim; ++bj)
-          tmp[bi] += vals[(int64_t)ij * bb + bi * bdim + bj] * x[(int64_t)j * bdim + bj];
-    }
-    for (int bi = 0; bi < bdim; ++bi) y[(int64_t)i * bdim + bi] = tmp[bi];
-  }
-}
-
 /* flatten.cpp:286-306 -> ir_builder.cpp:130-140: sequential inner product */
 double or_dot(int64_t n, const double *x, const double *y) {
   double s = 0.0;

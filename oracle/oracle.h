@@ -133,9 +133,6 @@ void or_hex_assemble(const or_hex_params *P, int32_t nv, int64_t nh,
 void or_bsrmv(int32_t nrows, int32_t bdim, const uint32_t *rowptr,
               const uint32_t *colidx, const double *vals, const double *x,
               double *y);
-void or_bsrmv_f32(int32_t nrows, int32_t bdim, const uint32_t *rowptr,
-                  const uint32_t *colidx, const float *vals, const float *x,
-                  float *y);
 double or_dot(int64_t n, const double *x, const double *y);
 
 /* ---- CG: isprings.sim:84-103 / fem_linear.sim:211-230 / cg.sim:25-53 /

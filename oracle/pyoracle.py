@@ -90,7 +90,6 @@ def _declare(L):
     L.or_hex_kref.argtypes = [C.c_int, C.c_int]
     L.or_hex_assemble.argtypes = [C.POINTER(HexParams), C.c_int32, C.c_int64] + [vp] * 7
     L.or_bsrmv.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, vp, vp]
-    L.or_bsrmv_f32.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, vp, vp]
     L.or_dot.restype = C.c_double
     L.or_dot.argtypes = [C.c_int64, vp, vp]
     L.or_cg.argtypes = [C.c_int32, C.c_int32, vp, vp, vp, vp, vp, C.POINTER(CGOpts),
@@ -279,14 +278,6 @@ def bsrmv(rowptr, colidx, vals, x, bdim):
     y = np.zeros(n * bdim)
     lib().or_bsrmv(n, bdim, _p(rowptr), _p(colidx), _p(_c(vals, np.float64)),
                    _p(_c(x, np.float64)), _p(y))
-    return y
-
-
-def bsrmv_f32(rowptr, colidx, vals, x, bdim):
-    n = len(rowptr) - 1
-    y = np.zeros(n * bdim, dtype=np.float32)
-    lib().or_bsrmv_f32(n, bdim, _p(rowptr), _p(colidx), _p(_c(vals, np.float32)),
-                       _p(_c(x, np.float32)), _p(y))
     return y
 
 

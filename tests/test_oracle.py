@@ -303,3 +303,33 @@ def test_cg_matches_direct_solve(oracle):
                           oracle.cg_opts(maxiters=5000, tol=tol, precond=pc))
         assert st["true_rel"] < 1e-9
         np.testing.assert_allclose(x, xd, rtol=1e-6, atol=1e-8 * np.abs(xd).max())
+
+
+def test_cg_residual_replacement(oracle):
+    """or_cg's periodic r = b - A x (thermal.sim:75-99): with replace_every = k
+    the iterates still solve the system (direct solve), the iteration count
+    stays that of the plain recurrence (they agree in exact arithmetic), and
+    after a replacing iteration the iterated r'r is the true residual's."""
+    prob = meshes.tet_problem(3, seed=1)
+    rp, ci = oracle.index_vev(prob["nverts"], [prob["endpoints"]])
+    vals, f = oracle.tet_assemble(oracle.tet_params(), prob, rp, ci)
+    n = prob["nverts"]
+    A = np.zeros((3 * n, 3 * n))
+    for i in range(n):
+        for l in range(rp[i], rp[i + 1]):
+            A[3 * i:3 * i + 3, 3 * ci[l]:3 * ci[l] + 3] = vals[9 * l:9 * l + 9].reshape(3, 3)
+    xd = np.linalg.solve(A, f)
+    bn = np.linalg.norm(f)
+    for pc, tol in ((0, 1e-10), (2, 1e-13)):
+        _, st0 = oracle.cg(rp, ci, vals, f, np.zeros(3 * n), 3,
+                           oracle.cg_opts(maxiters=5000, tol=tol, precond=pc))
+        for every in (1, 2, 7):
+            x, st = oracle.cg(rp, ci, vals, f, np.zeros(3 * n), 3,
+                              oracle.cg_opts(maxiters=5000, tol=tol, precond=pc,
+                                             replace_every=every))
+            np.testing.assert_allclose(x, xd, rtol=1e-6, atol=1e-8 * np.abs(xd).max())
+            assert abs(st["iters"] - st0["iters"]) <= 0.1 * st0["iters"] + 2, (pc, every)
+            if st["iters"] % every == 0:
+                true_rr = (st["true_rel"] * bn) ** 2
+                assert abs(st["rr"] - true_rr) <= 1e-10 * true_rr, (pc, every, st)
+
