@@ -24,9 +24,12 @@
 // with BF16 copies stored transposed ([out][in]) so one lane's four
 // consecutive k values of an MFMA B fragment are one 8-byte load.
 //
-// MFMA: __builtin_amdgcn_mfma_f32_16x16x16bf16_1k.  Lane l feeds
-// A[l % 16][4 (l / 16) + i] and B[4 (l / 16) + i][l % 16] (i = 0..3) and
-// receives D[4 (l / 16) + i][l % 16].  One wave multiplies 16 rows by a
+// MFMA: gfx950's __builtin_amdgcn_mfma_f32_16x16x32_bf16 for the K = 64 and
+// K = 96 layers (one instruction per 32 of K): lane l feeds
+// A[l % 16][8 (l / 16) + i] and B[8 (l / 16) + i][l % 16] (i = 0..7, one
+// 16-byte load each) and receives D[4 (l / 16) + v][l % 16].  The K = 16
+// encoders use __builtin_amdgcn_mfma_f32_16x16x16bf16_1k (lane l feeds
+// A[l % 16][4 (l / 16) + i], i = 0..3).  One wave multiplies 16 rows by a
 // K x 32 (or K x 16) weight matrix per tile.
 #include <cmath>
 #include <cstring>
@@ -40,6 +43,8 @@ namespace {
 using namespace smh;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 mfma_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kH = 32;
@@ -407,6 +412,32 @@ __device__ __forceinline__ void tile_mma(const bf16x4 (&a)[KB], const BFrag<KB, 
       acc[nb] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[kb], w.b[nb][kb], acc[nb], 0, 0, 0);
 }
 
+// The same with K in blocks of 32 (16x16x32 MFMA): acc[nb] += A(16 x 32 KB)
+// * W(32 KB x 16 NB), W transposed (Wt[c][k]); lane l holds k = 32 kb +
+// 8 (l / 16) + i of its row (A) or column (B)
+template <int KB, int NB>
+struct BFrag32 {
+  bf16x8 b[NB][KB];
+  __device__ void load(const uint16_t *Wt, int lane) {
+    for (int nb = 0; nb < NB; ++nb)
+      for (int kb = 0; kb < KB; ++kb)
+        b[nb][kb] = *reinterpret_cast<const bf16x8 *>(
+            Wt + (16 * nb + (lane & 15)) * (32 * KB) + 32 * kb + 8 * (lane >> 4));
+  }
+};
+__device__ __forceinline__ bf16x8 load_bf8(const uint16_t *p) {
+  return *reinterpret_cast<const bf16x8 *>(p);
+}
+template <int KB, int NB>
+__device__ __forceinline__ void tile_mma32(const bf16x8 (&a)[KB], const BFrag32<KB, NB> &w,
+                                           f32x4 (&acc)[NB]) {
+  for (int nb = 0; nb < NB; ++nb)
+    for (int kb = 0; kb < KB; ++kb)
+      acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+          __builtin_bit_cast(mfma_bf16x8, a[kb]), __builtin_bit_cast(mfma_bf16x8, w.b[nb][kb]),
+          acc[nb], 0, 0, 0);
+}
+
 // Orders a wave's LDS accesses across its lanes: a wave's LDS instructions
 // execute in order, so it only has to keep the compiler from moving them
 // (no workgroup barrier: the waves of a workgroup run different numbers of
@@ -498,22 +529,20 @@ __device__ __forceinline__ TileRange edge_tiles(uint64_t ntiles) {
   return {(uint64_t)blockIdx.x * 4 + wave, ntiles, (uint64_t)gridDim.x * 4};
 }
 
-// A fragment of [h_i h_j e_k] for edge row r (hb: the BF16 vertex states)
-__device__ __forceinline__ void gather_edge_x(bf16x4 (&a)[6], uint64_t r, bool valid,
+// A fragment of [h_i h_j e_k] for edge row r (hb: the BF16 vertex states):
+// one 16-byte load of each of the three 32-wide parts
+__device__ __forceinline__ void gather_edge_x(bf16x8 (&a)[3], uint64_t r, bool valid,
                                               const uint32_t *brow, const uint32_t *ci,
                                               const uint16_t *hb, const uint16_t *e, int lane) {
-  const int k4 = 4 * (lane >> 4);
+  const int k8 = 8 * (lane >> 4);
   if (!valid) {
-    for (int kb = 0; kb < 6; ++kb) a[kb] = bf16x4{0, 0, 0, 0};
+    for (int kb = 0; kb < 3; ++kb) a[kb] = bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
     return;
   }
   const uint32_t i = brow[r], j = ci[r];
-  a[0] = load_bf(hb + (uint64_t)i * kH + k4);
-  a[1] = load_bf(hb + (uint64_t)i * kH + 16 + k4);
-  a[2] = load_bf(hb + (uint64_t)j * kH + k4);
-  a[3] = load_bf(hb + (uint64_t)j * kH + 16 + k4);
-  a[4] = load_bf(e + r * kH + k4);
-  a[5] = load_bf(e + r * kH + 16 + k4);
+  a[0] = load_bf8(hb + (uint64_t)i * kH + k8);
+  a[1] = load_bf8(hb + (uint64_t)j * kH + k8);
+  a[2] = load_bf8(e + r * kH + k8);
 }
 
 // m_k = relu([h_i h_j e_k] W_m + b_m) (BF16 out).  The weights stay in
@@ -526,15 +555,15 @@ k_gnn_message(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *_
               uint16_t *__restrict__ m) {
   const int lane = threadIdx.x & 63;
   __shared__ __align__(16) uint16_t lt[4][16 * kH];
-  BFrag<6, 2> w;
+  BFrag32<3, 2> w;
   w.load(Wt, lane);
   for (TileRange T = edge_tiles((nnz + 15) / 16); T.t < T.end; T.t += T.step) {
     const uint64_t tile = T.t;
     const uint64_t r = tile * 16 + (lane & 15);
-    bf16x4 a[6];
+    bf16x8 a[3];
     gather_edge_x(a, r, r < nnz, brow, ci, hb, e, lane);
     f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    tile_mma<6, 2>(a, w, acc);
+    tile_mma32<3, 2>(a, w, acc);
     store_tile_bf16(lt[threadIdx.x >> 6], acc, ReluBias{bias}, m, tile, nnz, lane);
   }
 }
@@ -548,40 +577,39 @@ k_gnn_update(uint32_t n, const uint32_t *__restrict__ rp, const float *__restric
              const float *__restrict__ bias, float *__restrict__ hn, uint16_t *__restrict__ hbn,
              float *__restrict__ Uout, float *__restrict__ aout) {
   const int lane = threadIdx.x & 63;
-  BFrag<4, 2> w;
+  BFrag32<2, 2> w;
   w.load(Wt, lane);
-  const int k4 = 4 * (lane >> 4);
+  const int k8 = 8 * (lane >> 4);
   const uint32_t ntiles = (n + 15) / 16;
   for (uint32_t tile = blockIdx.x * 4 + (threadIdx.x >> 6); tile < ntiles;
        tile += gridDim.x * 4) {
     const uint32_t r = tile * 16 + (lane & 15);
-    bf16x4 a[4] = {{0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    bf16x8 a[2] = {{0, 0, 0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0, 0, 0}};
     if (r < n) {
-      a[0] = load_f32_as_bf(h + (uint64_t)r * kH + k4);
-      a[1] = load_f32_as_bf(h + (uint64_t)r * kH + 16 + k4);
+      const bf16x4 h0 = load_f32_as_bf(h + (uint64_t)r * kH + k8);
+      const bf16x4 h1 = load_f32_as_bf(h + (uint64_t)r * kH + k8 + 4);
+      for (int q = 0; q < 4; ++q) {
+        a[0][q] = h0[q];
+        a[0][4 + q] = h1[q];
+      }
       float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
       const uint32_t k0 = rp[r], k1 = rp[r + 1];
       for (uint32_t k = k0; k < k1; ++k) {
-        const bf16x4 x0 = load_bf(m + (uint64_t)k * kH + k4);
-        const bf16x4 x1 = load_bf(m + (uint64_t)k * kH + 16 + k4);
-        for (int q = 0; q < 4; ++q) {
-          s[q] += bf2f((uint16_t)x0[q]);
-          s[4 + q] += bf2f((uint16_t)x1[q]);
-        }
+        const bf16x8 x = load_bf8(m + (uint64_t)k * kH + k8);
+        for (int q = 0; q < 8; ++q) s[q] += bf2f((uint16_t)x[q]);
       }
       const float inv = 1.0f / (float)(k1 > k0 ? k1 - k0 : 1u);
       for (int q = 0; q < 8; ++q) s[q] *= inv;
-      a[2] = pack4(s[0], s[1], s[2], s[3]);
-      a[3] = pack4(s[4], s[5], s[6], s[7]);
-      if (aout) {
-        for (int q = 0; q < 4; ++q) {
-          aout[(uint64_t)r * kH + k4 + q] = s[q];
-          aout[(uint64_t)r * kH + 16 + k4 + q] = s[4 + q];
-        }
+      const bf16x4 s0 = pack4(s[0], s[1], s[2], s[3]), s1 = pack4(s[4], s[5], s[6], s[7]);
+      for (int q = 0; q < 4; ++q) {
+        a[1][q] = s0[q];
+        a[1][4 + q] = s1[q];
       }
+      if (aout)
+        for (int q = 0; q < 8; ++q) aout[(uint64_t)r * kH + k8 + q] = s[q];
     }
     f32x4 acc[2] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-    tile_mma<4, 2>(a, w, acc);
+    tile_mma32<2, 2>(a, w, acc);
     for (int v = 0; v < 4; ++v) {
       const uint32_t row = tile * 16 + 4 * (lane >> 4) + v;
       if (row >= n) continue;
@@ -609,15 +637,15 @@ k_gnn_decode(uint64_t nnz, const uint32_t *__restrict__ brow, const uint32_t *__
   const int lane = threadIdx.x & 63;
   __shared__ __align__(16) float lt[4][16 * 9];
   float *l = lt[threadIdx.x >> 6];
-  BFrag<6, 1> w;
+  BFrag32<3, 1> w;
   w.load(Wt, lane);
   for (TileRange T = edge_tiles((nnz + 15) / 16); T.t < T.end; T.t += T.step) {
     const uint64_t tile = T.t;
     const uint64_t r = tile * 16 + (lane & 15);
-    bf16x4 a[6];
+    bf16x8 a[3];
     gather_edge_x(a, r, r < nnz, brow, ci, hb, e, lane);
     f32x4 acc[1] = {{0, 0, 0, 0}};
-    tile_mma<6, 1>(a, w, acc);
+    tile_mma32<3, 1>(a, w, acc);
     if constexpr (RS == 1) {
       // output 0 of rows 4 (lane / 16) + v sits in lanes 0, 16, 32, 48: one
       // 16-byte store each
