@@ -288,6 +288,19 @@ __device__ double wg_sum(const double *part, int np, int st, double *s4) {
   return block_sum256(thread_sum(part, np, st), s4);
 }
 
+// entry (a, b), b <= a, of a lower-triangular 3x3 block stored as its six
+// entries row by row
+__host__ __device__ constexpr int tri_idx(int a, int b) { return a * (a + 1) / 2 + b; }
+
+// Linv [n][9] (row-major, zero above the diagonal) -> [n][6] (tri_idx)
+__global__ void k_pack_linv(uint32_t n, const float *__restrict__ L9, float *__restrict__ L6) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (uint64_t)n * 6) return;
+  const uint64_t i = t / 6;
+  const int k = (int)(t - i * 6), a = k >= 3 ? 2 : (k >= 1 ? 1 : 0), b = k - tri_idx(a, 0);
+  L6[t] = L9[i * 9 + 3 * a + b];
+}
+
 template <int BD>
 __device__ __forceinline__ void apply_precond(int pc, const double *r, double *z,
                                               const void *dinv, bool f32, uint64_t i) {
@@ -297,11 +310,12 @@ __device__ __forceinline__ void apply_precond(int pc, const double *r, double *z
     z[0] = (double)((const float *)dinv)[i] * r[0];  // u = Linv r, Linv = 1/sqrt(a_ii)
   } else if (BD == 3 && pc == SMH_PRECOND_LEARNED) {
     // first half of the learned preconditioner: u = Linv r (the rest,
-    // z = Linv^T (u + C u), needs the neighbours' u: k_cg_lapply)
-    const float *L = (const float *)dinv + i * 9;
+    // z = Linv^T (u + C u), needs the neighbours' u: k_cg_lapply); Linv
+    // packed as its six lower-triangular entries (tri_idx)
+    const float *L = (const float *)dinv + i * 6;
     for (int a = 0; a < 3; ++a) {
       double s = 0.0;
-      for (int b = 0; b <= a; ++b) s += (double)L[3 * a + b] * r[b];
+      for (int b = 0; b <= a; ++b) s += (double)L[tri_idx(a, b)] * r[b];
       z[a] = s;
     }
   } else if (BD == 1 || pc == SMH_PRECOND_JACOBI) {
@@ -475,10 +489,10 @@ struct EpiLearned {
     }
     double y[3];
     for (int a = 0; a < 3; ++a) y[a] = (double)u[(uint64_t)row * 3 + a] + (double)s[a];
-    const float *L = linv + (uint64_t)row * 9;
+    const float *L = linv + (uint64_t)row * 6;  // packed (tri_idx)
     for (int a = 0; a < 3; ++a) {
       double t = 0.0;
-      for (int b = a; b < 3; ++b) t += (double)L[3 * b + a] * y[b];
+      for (int b = a; b < 3; ++b) t += (double)L[tri_idx(b, a)] * y[b];
       const uint64_t o = (uint64_t)row * 3 + a;
       const VT zz = (VT)t;
       z[o] = zz;
@@ -762,12 +776,12 @@ struct Arena {
 
 struct Work {
   void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *dinv = nullptr, *u = nullptr,
-       *dlo = nullptr;
+       *dlo = nullptr, *linv = nullptr;
   CGState *S = nullptr;
   double *part = nullptr;
   void release() {
-    dfree(r); dfree(z); dfree(p); dfree(q); dfree(dinv); dfree(u); dfree(dlo); dfree(S);
-    dfree(part);
+    dfree(r); dfree(z); dfree(p); dfree(q); dfree(dinv); dfree(u); dfree(dlo); dfree(linv);
+    dfree(S); dfree(part);
   }
 };
 
@@ -1139,6 +1153,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   const unsigned gt = ops.parts_spmv(), grt = ops.parts_res(), gc = ops.parts_corr();
   const int check = o->check_every > 0 ? o->check_every : 16;
   const void *pdata = nullptr;  // the preconditioner data K2 reads
+  const float *linv = nullptr;   // the learned preconditioner's Linv blocks
   Work w;
   int rc = SMH_OK;
   auto T0 = std::chrono::steady_clock::now();
@@ -1176,13 +1191,21 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   if (learned) {
     TRY(walloc(&w.u, sizeof(VT) * n));
     w.dinv = nullptr;
+    if (BD == 3) {
+      // Linv's six lower-triangular entries per row: what the update and the
+      // correction pass read (24 instead of 36 bytes a row)
+      TRY(walloc(&w.linv, sizeof(float) * 6 * (uint64_t)nrows));
+      hipLaunchKernelGGL(k_pack_linv, dim3(grid_for((uint64_t)nrows * 6, 256)), dim3(256), 0,
+                         ctx->stream, nrows, (const float *)o->learned, (float *)w.linv);
+    }
   }
+  linv = learned ? (BD == 3 ? (const float *)w.linv : (const float *)o->learned) : nullptr;
   // mixed precision: the inner operator keeps the exact block row sums
   if (o->mixed && has_lo) {
     TRY(walloc(&w.dlo, sizeof(float) * (uint64_t)nrows * BD * BD));
     ops.lo_rowsum(ctx->stream, (float *)w.dlo);
   }
-  pdata = learned ? (const void *)o->learned : (const void *)w.dinv;
+  pdata = learned ? (const void *)linv : (const void *)w.dinv;
   TRY(walloc((void **)&w.S, sizeof(CGState)));
   TRY(walloc((void **)&w.part, sizeof(double) * 2 * kPartSlots + 64));
   // b'b, state, preconditioner
@@ -1205,7 +1228,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   epi.init = learned ? 0 : 1;
   ops.residual(ctx->stream, b, (const double *)x, epi, w.part);
   if (learned) {
-    ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
+    ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, linv,
              (const CGState *)w.S, 0, w.part);
     hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
                        (const double *)w.part, (int)gc, 0);
@@ -1230,7 +1253,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
       batch = std::max(2, batch / 4);
       ops.residual(ctx->stream, b, (const double *)x, epi, w.part);
       if (learned)
-        ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
+        ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, linv,
                  (const CGState *)w.S, 0, w.part);
       hipLaunchKernelGGL(k_cg_rres, dim3(1), dim3(256), 0, ctx->stream, w.S,
                          (const double *)w.part, (int)(learned ? gc : grt), 0);
@@ -1248,7 +1271,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
                          (const VT *)w.q, w.S, (const double *)w.part, (int)gt,
                          w.part + kPartSlots);
       if (learned)
-        ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, (const float *)o->learned,
+        ops.corr(ctx->stream, w.u, (const VT *)w.r, (VT *)w.z, linv,
                  (const CGState *)w.S, 1, w.part + kPartSlots);
       hipLaunchKernelGGL(k_cg_r2, dim3(1), dim3(256), 0, ctx->stream, w.S,
                          (const double *)(w.part + kPartSlots), (int)(learned ? gc : nbe));
