@@ -259,6 +259,12 @@ int smh_assemble_edge_blocks(smh_ctx *ctx, const smh_index *idx,
  * with no fused multiply-add: bit-identical to the CPU loop. */
 int smh_bsrmv(smh_ctx *ctx, const smh_bsr *A, const void *x, void *y,
               int32_t exact_order);
+/* The SMH_F32X2 pair of n FP64 values: hi[i] = fl32(v[i]), lo[i] =
+ * fl32(v[i] - hi[i]), the rounding the assemblies apply when they write
+ * SMH_F32X2.  No counterpart in the reference (one global float type there):
+ * a matrix assembled once in FP64 gives both the mixed solver's inner
+ * operator and the FP64 values it certifies with (smh_cg_opts.vals_f64). */
+int smh_split_hilo(smh_ctx *ctx, uint64_t n, const double *v, float *hi, float *lo);
 /* r'r style inner product (flatten.cpp:286-306; the reference GPU path used
  * one global atomicAdd, gpu_backend.cpp:339-365).  Deterministic two-level
  * reduction in FP64; synchronises and writes *result on the host.
@@ -338,6 +344,17 @@ typedef struct {
                             Jacobi (after the learned one), then with no
                             preconditioner; each rung is certified by its
                             own FP64 residual.  0: report the first status */
+  const double *vals_f64; /* mixed, optional: device FP64 values of the same
+                            matrix in A's layout (blocked CSR: [nnz*b*b];
+                            stencil: [27*n]), e.g. the assembly's FP64
+                            output that hi and lo were split from
+                            (smh_split_hilo).  When set, the reliable
+                            updates, the (block) Jacobi inverse and the
+                            final certification use these values instead
+                            of hi + lo, which carries each value only to
+                            ~2^-48: on stiff systems that difference alone
+                            can move ||b - A x|| / ||b|| past 1e-8.  NULL:
+                            hi + lo.  Not taken by the batched solver.    */
 } smh_cg_opts;
 typedef struct {
   int32_t iters;         /* iterations, summed over the rungs of the chain  */

@@ -120,7 +120,8 @@ class CGOptions(C.Structure):
                 ("precond", C.c_int32), ("mixed", C.c_int32),
                 ("reliable_delta", C.c_double), ("check_every", C.c_int32),
                 ("learned", C.c_void_p), ("zr_stop", C.c_int32),
-                ("learned_corr", C.c_void_p), ("fallback", C.c_int32)]
+                ("learned_corr", C.c_void_p), ("fallback", C.c_int32),
+                ("vals_f64", C.c_void_p)]
 
 
 class CGBatchStats(C.Structure):
@@ -194,6 +195,7 @@ def load(path: os.PathLike | None = None) -> C.CDLL:
                                              C.POINTER(CGOptions), C.POINTER(CGStats)]),
         "smh_block_jacobi": (C.c_int, [vp, C.POINTER(Bsr), vp]),
         "smh_axpby": (C.c_int, [vp, u64, C.c_double, vp, C.c_double, vp]),
+        "smh_split_hilo": (C.c_int, [vp, u64, vp, vp, vp]),
         "smh_vec_op": (C.c_int, [vp, i32, u64, C.c_double, vp, C.c_double, vp, vp]),
         "smh_stage_reduce": (C.c_int, [vp, u64, u32, vp, vp, vp, vp, vp]),
         "smh_hiprtc_compile": (C.c_int, [C.c_char_p, C.c_char_p, C.c_char_p, C.POINTER(vp),
@@ -354,6 +356,11 @@ class Context:
     def bsrmv(self, A: Bsr, x, y, exact_order: bool = False):
         self._check(self.lib.smh_bsrmv(self.h, C.byref(A), _ptr(x), _ptr(y), int(exact_order)))
 
+    def split_hilo(self, v, hi, lo):
+        """hi = fl32(v), lo = fl32(v - hi) for FP64 values v (the SMH_F32X2 pair
+        the assemblies would write)."""
+        self._check(self.lib.smh_split_hilo(self.h, v.numel(), _ptr(v), _ptr(hi), _ptr(lo)))
+
     def bsr_planar(self, A: Bsr, out=None):
         """The lane-planar copy of A's FP32 3x3 values (a float32 tensor of
         576 * ceil(nnz / 64) entries) and the SMH_F32P matrix over it."""
@@ -462,7 +469,7 @@ class Context:
         self.lib.smh_cg_default_opts(C.byref(o))
         keep = []
         for k, v in kw.items():
-            if k in ("learned", "learned_corr"):
+            if k in ("learned", "learned_corr", "vals_f64"):
                 keep.append(v)  # the struct holds a raw pointer: keep the buffer alive
                 v = _ptr(v)
             setattr(o, k, v)

@@ -1013,6 +1013,7 @@ template <int IP, int RP, int LP>
 struct StencilOps {
   using VT = typename Val<IP>::T;
   StDev A{}, C{};
+  StDev R{};  // the residual's (and the Jacobi inverse's) values, precision RP
   unsigned g = 0;
   unsigned parts_spmv() const { return g; }
   unsigned parts_res() const { return g; }
@@ -1023,7 +1024,7 @@ struct StencilOps {
   }
   void residual(hipStream_t s, const double *b, const double *x, const EpiResidual<VT, 1> &epi,
                 double *part) const {
-    hipLaunchKernelGGL((k_residual_st<RP, VT>), dim3(g), dim3(256), 0, s, A, b, x, epi, part);
+    hipLaunchKernelGGL((k_residual_st<RP, VT>), dim3(g), dim3(256), 0, s, R, b, x, epi, part);
   }
   // the correction pass reads u in the correction's vector type
   static constexpr bool kU32 = sizeof(typename Val<LP>::T) == 4 && sizeof(VT) == 8;
@@ -1037,8 +1038,8 @@ struct StencilOps {
                        (const float *)A.vals_lo, dlo);
   }
   void dinv(hipStream_t s, int, VT *out) const {
-    hipLaunchKernelGGL((k_dinv_st<RP, VT>), dim3(grid_for(A.n, 256)), dim3(256), 0, s, A.n,
-                       A.vals, A.vals_lo, out);
+    hipLaunchKernelGGL((k_dinv_st<RP, VT>), dim3(grid_for(R.n, 256)), dim3(256), 0, s, R.n,
+                       R.vals, R.vals_lo, out);
   }
 };
 
@@ -1106,7 +1107,7 @@ struct TileOps {
   }
   void dinv(hipStream_t s, int pc, VT *out) const {
     hipLaunchKernelGGL((k_dinv<RP, VT>), dim3(grid_for(A->nrows, 256)), dim3(256), 0, s,
-                       A->nrows, A->rowptr, A->colidx, A->vals, A->vals_lo, BD, pc, out);
+                       A->nrows, A->rowptr, A->colidx, TR.vals, TR.vals_lo, BD, pc, out);
   }
 };
 
@@ -1122,7 +1123,13 @@ int make_tile_ops(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan,
   TileOps<IP, RP, LP, BD> &ops = *(TileOps<IP, RP, LP, BD> *)out;
   ops.A = A;
   ops.T = tile_dev(A, plan, &ops.gt, kIl);
-  ops.TR = tile_dev(A, rplan, &ops.grt, kIl);
+  // residuals (and the Jacobi inverse) over the caller's FP64 values when given
+  smh_bsr Ar = *A;
+  if (RP == SMH_F64 && o->vals_f64) {
+    Ar.vals = o->vals_f64;
+    Ar.vals_lo = nullptr;
+  }
+  ops.TR = tile_dev(&Ar, rplan, &ops.grt, kIl);
   smh_bsr Cb = *A;
   const bool learned = o->precond == SMH_PRECOND_LEARNED;
   Cb.vals = learned ? (const void *)o->learned_corr : A->vals;
@@ -2479,6 +2486,8 @@ int cg_validate(smh_ctx *ctx, const smh_bsr *A, const double *b, double *x, cons
   SMH_CHECK(ctx, o->maxiters >= 0, SMH_ERR_INVALID, "%s: negative maxiters", who);
   SMH_CHECK(ctx, o->mixed >= 0 && o->mixed <= 2, SMH_ERR_INVALID,
             "%s: mixed must be 0, 1 or 2 (got %d)", who, o->mixed);
+  SMH_CHECK(ctx, !o->vals_f64 || (o->mixed && ((uintptr_t)o->vals_f64 & 7u) == 0),
+            SMH_ERR_INVALID, "%s: vals_f64 (8-byte aligned) goes with mixed precision", who);
   return SMH_OK;
 }
 
@@ -2491,8 +2500,8 @@ int cg_dispatch(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const
 #define SMH_CG(IP, RP)                                                                  \
   (A->bdim == 3 ? cg_tiles<IP, RP, 3>(ctx, A, plan, plan, b, x, oo, so, ar, pin) \
                 : cg_tiles<IP, RP, 1>(ctx, A, plan, plan, b, x, oo, so, ar, pin))
-    if (oo->mixed == 2) return SMH_CG(kF32D, SMH_F32X2);
-    if (oo->mixed) return SMH_CG(SMH_F32, SMH_F32X2);
+    if (oo->mixed == 2) return oo->vals_f64 ? SMH_CG(kF32D, SMH_F64) : SMH_CG(kF32D, SMH_F32X2);
+    if (oo->mixed) return oo->vals_f64 ? SMH_CG(SMH_F32, SMH_F64) : SMH_CG(SMH_F32, SMH_F32X2);
     if (A->precision == SMH_F64) return SMH_CG(SMH_F64, SMH_F64);
     if (A->precision == SMH_F32X2) return SMH_CG(SMH_F32X2, SMH_F32X2);
     return SMH_CG(kF32D, kF32D);
@@ -2620,6 +2629,8 @@ int cg_batched(smh_ctx *ctx, const smh_bsr *A, uint32_t nbatch, const uint32_t *
             "smh_cg_solve_batched: mixed precision needs an SMH_F32X2 matrix");
   SMH_CHECK(ctx, o->mixed == 0 || o->mixed == 1, SMH_ERR_INVALID,
             "smh_cg_solve_batched: mixed must be 0 or 1");
+  SMH_CHECK(ctx, !o->vals_f64, SMH_ERR_UNSUPPORTED,
+            "smh_cg_solve_batched: vals_f64 is not supported (residuals use hi + lo)");
   SMH_CHECK(ctx, A->precision >= SMH_F64 && A->precision <= SMH_F32D, SMH_ERR_INVALID,
             "smh_cg_solve_batched: unknown precision %d", A->precision);
   SMH_CHECK(ctx, A->precision != SMH_F32X2 || A->vals_lo, SMH_ERR_INVALID,
@@ -2824,6 +2835,7 @@ int cg_stencil(smh_ctx *ctx, const smh_stencil27 *S, const double *b, double *x,
   constexpr int LP = IP == SMH_F64 ? kF16D : kF16F;
   StencilOps<IP, RP, LP> ops;
   ops.A = st_dev(S, S->vals, S->vals_lo);
+  ops.R = RP == SMH_F64 && o->vals_f64 ? st_dev(S, o->vals_f64, nullptr) : ops.A;
   ops.C = st_dev(S, o->precond == SMH_PRECOND_LEARNED ? o->learned_corr : S->vals, nullptr);
   ops.g = grid_for(ops.A.n, 256, 4096);
   return cg_run<IP, RP, 1>(ctx, ops, ops.A.n, S->precision == SMH_F32X2 && S->vals_lo, b, x, o,
@@ -2852,9 +2864,16 @@ extern "C" int smh_cg_solve_stencil27(smh_ctx *ctx, const smh_stencil27 *S, cons
   SMH_CHECK(ctx, o->maxiters >= 0, SMH_ERR_INVALID, "smh_cg_solve_stencil27: negative maxiters");
   SMH_CHECK(ctx, o->mixed >= 0 && o->mixed <= 2, SMH_ERR_INVALID,
             "smh_cg_solve_stencil27: mixed must be 0, 1 or 2 (got %d)", o->mixed);
+  SMH_CHECK(ctx, !o->vals_f64 || (o->mixed && ((uintptr_t)o->vals_f64 & 7u) == 0),
+            SMH_ERR_INVALID,
+            "smh_cg_solve_stencil27: vals_f64 (8-byte aligned) goes with mixed precision");
   auto run = [&](const smh_cg_opts *oo, smh_cg_stats *so) {
-    if (oo->mixed == 2) return cg_stencil<kF32D, SMH_F32X2>(ctx, S, b, x, oo, so);
-    if (oo->mixed) return cg_stencil<SMH_F32, SMH_F32X2>(ctx, S, b, x, oo, so);
+    if (oo->mixed == 2)
+      return oo->vals_f64 ? cg_stencil<kF32D, SMH_F64>(ctx, S, b, x, oo, so)
+                          : cg_stencil<kF32D, SMH_F32X2>(ctx, S, b, x, oo, so);
+    if (oo->mixed)
+      return oo->vals_f64 ? cg_stencil<SMH_F32, SMH_F64>(ctx, S, b, x, oo, so)
+                          : cg_stencil<SMH_F32, SMH_F32X2>(ctx, S, b, x, oo, so);
     if (S->precision == SMH_F64) return cg_stencil<SMH_F64, SMH_F64>(ctx, S, b, x, oo, so);
     if (S->precision == SMH_F32X2) return cg_stencil<SMH_F32X2, SMH_F32X2>(ctx, S, b, x, oo, so);
     return cg_stencil<kF32D, kF32D>(ctx, S, b, x, oo, so);

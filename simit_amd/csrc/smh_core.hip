@@ -153,6 +153,19 @@ k_dot_partial(uint64_t n, const T *__restrict__ x, const T *__restrict__ y,
   if (threadIdx.x == 0) part[blockIdx.x] = ((s[0] + s[1]) + s[2]) + s[3];
 }
 
+// hi = fl32(v), lo = fl32(v - hi): the assemblies' SMH_F32X2 rounding
+__global__ void __launch_bounds__(256)
+k_split_hilo(uint64_t n, const double *__restrict__ v, float *__restrict__ hi,
+             float *__restrict__ lo) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * 256) {
+    const double x = v[i];
+    const float h = (float)x;
+    hi[i] = h;
+    lo[i] = (float)(x - (double)h);
+  }
+}
+
 __global__ void __launch_bounds__(256)
 k_axpby(uint64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n;
@@ -334,6 +347,16 @@ extern "C" int smh_dot(smh_ctx *ctx, int32_t precision, uint64_t n, const void *
                               hipMemcpyDeviceToHost, ctx->stream));
   SMH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   *result = ctx->host_pinned[0];
+  return SMH_OK;
+}
+
+extern "C" int smh_split_hilo(smh_ctx *ctx, uint64_t n, const double *v, float *hi, float *lo) {
+  if (!ctx) return SMH_ERR_INVALID;
+  SMH_CHECK(ctx, n == 0 || (v && hi && lo), SMH_ERR_INVALID, "smh_split_hilo: null argument");
+  if (!n) return SMH_OK;
+  hipLaunchKernelGGL(k_split_hilo, dim3(grid_for(n, 256, 8192)), dim3(256), 0, ctx->stream, n, v,
+                     hi, lo);
+  SMH_LAUNCH_CHECK(ctx);
   return SMH_OK;
 }
 
