@@ -1069,7 +1069,7 @@ template <int IP, int RP, int LP, int BD>
 struct TileOps {
   using VT = typename Val<IP>::T;
   // the inner SpMV of FP32 3x3 values may run over a lane-planar copy
-  static constexpr bool kPlanar = BD == 3 && IP == SMH_F32;
+  static constexpr bool kPlanar = BD == 3 && (IP == SMH_F32 || IP == kF32D);
   const smh_bsr *A = nullptr;
   TileDev T{}, TR{}, TC{};
   TileDev TP{};  // lane-planar tiles (TP.vals set: the inner SpMV uses them)
@@ -2556,9 +2556,9 @@ extern "C" int smh_cg_solve_noalloc(smh_ctx *ctx, const smh_bsr *A, const double
   if (o->precond != SMH_PRECOND_NONE) need += vt * nrows * bb + 256;  // (block) Jacobi
   if (o->mixed && A->precision == SMH_F32X2) need += 4 * nrows * bb + 256;  // low row sums
   need += sizeof(CGState) + 256 + sizeof(double) * 2 * kPartSlots + 64 + 256;
-  // mixed 3x3 (FP32 vectors): the lane-planar copy of the values and its tile
-  // rows, as smh_cg_solve streams them
-  const bool want_pl = bd == 3 && o->mixed == 1;
+  // mixed 3x3: the lane-planar copy of the values and its tile rows, as
+  // smh_cg_solve streams them
+  const bool want_pl = bd == 3 && (o->mixed == 1 || o->mixed == 2);
   const uint32_t tbp = (uint32_t)TileShapePl::kBlocks;
   const uint64_t max_tiles_pl = A->nnz / (tbp / 2 + 1) + 1;
   if (want_pl) need += sizeof(float) * planar_floats(A->nnz) + 256 + 4 * (max_tiles_pl + 1) + 256;
