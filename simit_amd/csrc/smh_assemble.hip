@@ -375,13 +375,29 @@ struct TetOp {
     P.cl = 0.25 * A.p.rho * W;
   }
   // entry q of the diagonal (and rhs component q/4 for q = 0, 4, 8) from a
-  // record made by to_diag
-  __device__ static void diag_sum(const Args &A, const Pair &P, int q, double &acc, double &f) {
-    acc += P.g[q];
+  // record made by to_diag, as its loads and its adds: the fast kernel
+  // issues several records' loads before their adds (the same adds, in the
+  // same order)
+  struct DiagTerm {
+    double v, c, f;
+  };
+  __device__ static DiagTerm diag_load(const Pair &P, int q) {
+    DiagTerm t;
+    t.v = P.g[q];
+    t.c = q % 4 == 0 ? P.cl : 0.0;
+    t.f = q % 4 == 0 ? P.g[9 + q / 4] : 0.0;
+    return t;
+  }
+  __device__ static void diag_add(const Args &A, const DiagTerm &t, int q, double &acc,
+                                  double &f) {
+    acc += t.v;
     if (q % 4 == 0) {
-      if (A.p.rho != 0.0) acc += P.cl;
-      f += P.g[9 + q / 4];
+      if (A.p.rho != 0.0) acc += t.c;
+      f += t.f;
     }
+  }
+  __device__ static void diag_sum(const Args &A, const Pair &P, int q, double &acc, double &f) {
+    diag_add(A, diag_load(P, q), q, acc, f);
   }
 };
 
@@ -390,6 +406,11 @@ template <class Op, class = void>
 struct HasToDiag : std::false_type {};
 template <class Op>
 struct HasToDiag<Op, std::void_t<decltype(&Op::to_diag)>> : std::true_type {};
+// ops whose diag_sum comes as diag_load + diag_add
+template <class Op, class = void>
+struct HasDiagLoad : std::false_type {};
+template <class Op>
+struct HasDiagLoad<Op, std::void_t<decltype(&Op::diag_load)>> : std::true_type {};
 // ops whose off-diagonal blocks are formed from the plan's slot ranks
 template <class Op, class = void>
 struct HasBlockRank : std::false_type {};
@@ -1287,7 +1308,21 @@ k_assemble_fast(typename Op::Args A, uint32_t nrows, uint64_t nnz,
         for (int g = 0; g < 2; ++g) {
           if (!dact[g]) continue;
           const uint32_t lo = CW[dl[g]], hi = CW[dl[g] + 1];
-          for (uint32_t k = lo; k < hi; ++k) Op::diag_sum(A, PP[k], dq[g], a2[g], f2[g]);
+          if constexpr (HasDiagLoad<Op>::value) {
+            // four records' loads in flight, then their adds in list order
+            const int q = dq[g];
+            uint32_t k = lo;
+            for (; k + 4 <= hi; k += 4) {
+              typename Op::DiagTerm t[4];
+#pragma unroll
+              for (int j = 0; j < 4; ++j) t[j] = Op::diag_load(PP[k + j], q);
+#pragma unroll
+              for (int j = 0; j < 4; ++j) Op::diag_add(A, t[j], q, a2[g], f2[g]);
+            }
+            for (; k < hi; ++k) Op::diag_add(A, Op::diag_load(PP[k], q), q, a2[g], f2[g]);
+          } else {
+            for (uint32_t k = lo; k < hi; ++k) Op::diag_sum(A, PP[k], dq[g], a2[g], f2[g]);
+          }
         }
         __syncthreads();
       }
