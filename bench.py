@@ -44,10 +44,10 @@ SPMV_TRAFFIC_96 = 510_694_370
 # 192^3 hexes, blocked CSR (scalar, FP32 values, FP64 vectors): FETCH 803.20 MiB x 2.0 +
 # WRITE 56.27 MiB x 1.0
 SPMV_TRAFFIC_HEX192 = 1_743_427_658
-# the same matrix in 27-point stencil storage: FETCH 623.03 MiB x 2.0 + WRITE 54.85 MiB x 1.0
-# (1.54x the algorithmic bytes: FETCH_SIZE counts the x re-reads the Infinity
+# the same matrix in 27-point stencil storage: FETCH 501.14 MiB x 2.0 + WRITE 54.85 MiB x 1.0
+# (1.24x the algorithmic bytes: FETCH_SIZE counts the x re-reads the Infinity
 # Cache serves as well as HBM reads)
-STENCIL_TRAFFIC_HEX192 = 1_364_080_041
+STENCIL_TRAFFIC_HEX192 = 1_108_476_741
 METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
 # The mixed solves are certified on the FP64 matrix (smh_cg_opts.vals_f64):
 # the assembly writes FP64 values once and the inner FP32 operator's hi/lo

@@ -812,14 +812,16 @@ __device__ __forceinline__ uint32_t periodic_nbr(const StDev &A, uint32_t i, int
 // One thread per row: the 27 products in ascending column order, each
 // rounded before it is added -- the order and rounding of the tile loop, so
 // the result is the blocked-CSR product bit for bit (zero slots add exact
-// zeros; out-of-range columns read a clamped x times a zero value).
+// zeros; out-of-range columns read a clamped x times a zero value).  The
+// periodic storage runs this loop; the non-periodic one runs stencil_rows_n
+// below, which groups adjacent rows with the same per-row arithmetic.
 //
 // A plain grid-stride loop over the rows with plain loads.  Measured on the
-// 192^3 hex system and slower than it: giving each XCD a contiguous eighth
-// of the rows (226 us against 212 us), staging the nine x pencils of a
-// 256-row block in LDS (9 global loads per row instead of 27 gathers; no
-// faster, the solver iteration slower) and non-temporal value loads
-// (239 us).
+// 192^3 hex system with one row per thread and slower than it: giving each
+// XCD a contiguous eighth of the rows (226 us against 212 us), staging the
+// nine x pencils of a 256-row block in LDS (9 global loads per row instead
+// of 27 gathers; no faster, the solver iteration slower) and non-temporal
+// value loads (239 us).
 template <int PREC, class XG, class EPI, bool PER = false>
 __device__ __forceinline__ void stencil_rows(const StDev &A, const XG &xg, EPI &epi) {
 #pragma clang fp contract(off)
@@ -860,13 +862,108 @@ __device__ __forceinline__ void stencil_rows(const StDev &A, const XG &xg, EPI &
   }
 }
 
+// SMH_ST_ROWS adjacent rows per thread (non-periodic storage): per slot the
+// rows' values are one load, and per pencil (d1, d2) the R + 2 entries
+// x[i + o - 1 .. i + o + R] the rows gather (d0 = -1, 0, +1) are one load,
+// so a row costs 27 / R value loads and 9 (R + 2) / R x loads instead of 27
+// and 27.  Each row's 27 products are added in slot order with the one-row
+// loop's rounding, so the results are the same bits (windows that leave
+// [0, n) load entry by entry, clamped; a ragged last group runs row by row).
+// 192^3 hex system, FP32 values and FP64 vectors: one row 211 us, two rows
+// 179 us (the solver's SpMV 231 -> 203 us), four rows 171 us but the
+// solver's SpMV 210 us (its epilogue pushes the kernel to 236 VGPRs); each
+// XCD walking a contiguous eighth of the rows was slower again (189 us).
+#ifndef SMH_ST_ROWS
+#define SMH_ST_ROWS 2
+#endif
+template <typename S, int R>
+__device__ __forceinline__ void load_run(const S *p, S *o) {
+  typedef S vR __attribute__((ext_vector_type(R), aligned(sizeof(S))));
+  const vR r = *(const vR *)p;
+#pragma unroll
+  for (int k = 0; k < R; ++k) o[k] = r[k];
+}
+template <typename T, int W>
+__device__ __forceinline__ void load_window(const T *x, int64_t j0, uint64_t n, T *w) {
+  static_assert(W % 2 == 0, "windows load in pairs");
+  if (j0 >= 0 && (uint64_t)j0 + W <= n) {
+#pragma unroll
+    for (int k = 0; k < W; k += 2) load_run<T, 2>(x + j0 + k, w + k);
+  } else {
+#pragma unroll
+    for (int k = 0; k < W; ++k) {
+      const int64_t j = j0 + k;
+      w[k] = x[j < 0 ? 0 : (j >= (int64_t)n ? (int64_t)n - 1 : j)];
+    }
+  }
+}
+template <int PREC, class EPI>
+__device__ __forceinline__ void stencil_rows_n(const StDev &A, const typename Val<PREC>::T *x,
+                                               EPI &epi) {
+#pragma clang fp contract(off)
+  using T = typename Val<PREC>::T;
+  using S = typename Val<PREC>::S;
+  constexpr int R = SMH_ST_ROWS;
+  const uint64_t n = A.n, ng = (n + R - 1) / R;
+  for (uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < ng;
+       gi += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t i = R * gi;
+    if (i + R > n) {  // the ragged last group: row by row, as stencil_rows
+      for (uint64_t r = i; r < n; ++r) {
+        T acc = 0;
+#pragma unroll
+        for (int s = 0; s < 27; ++s) {
+          T v = val_of<PREC>(((const S *)A.vals)[(uint64_t)s * n + r]);
+          if constexpr (Val<PREC>::kLo) v += (T)((const float *)A.vals_lo)[(uint64_t)s * n + r];
+          const int64_t j = (int64_t)r + A.off[s];
+          const uint64_t jc = j < 0 ? 0 : (j >= (int64_t)n ? n - 1 : (uint64_t)j);
+          const T prod = v * x[jc];
+          acc = acc + prod;
+        }
+        epi((uint32_t)r, &acc);
+      }
+      continue;
+    }
+    S v[27][R];
+    float l[Val<PREC>::kLo ? 27 : 1][R];
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+      load_run<S, R>((const S *)A.vals + (uint64_t)s * n + i, v[s]);
+      if constexpr (Val<PREC>::kLo) load_run<float, R>((const float *)A.vals_lo + (uint64_t)s * n + i, l[s]);
+    }
+    T w[9][R + 2];
+#pragma unroll
+    for (int pc = 0; pc < 9; ++pc)
+      load_window<T, R + 2>(x, (int64_t)i + A.off[3 * pc + 1] - 1, n, w[pc]);
+    T acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0;
+#pragma unroll
+    for (int pc = 0; pc < 9; ++pc)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          T vv = val_of<PREC>(v[3 * pc + k][r]);
+          if constexpr (Val<PREC>::kLo) vv += (T)l[3 * pc + k][r];
+          const T prod = vv * w[pc][k + r];
+          acc[r] = acc[r] + prod;
+        }
+#pragma unroll
+    for (int r = 0; r < R; ++r) epi((uint32_t)(i + r), &acc[r]);
+  }
+}
+
 template <int PREC, bool PER = false>
 __global__ void __launch_bounds__(256) k_stmv(StDev A, const typename Val<PREC>::T *__restrict__ x,
                                              typename Val<PREC>::T *__restrict__ y) {
   using T = typename Val<PREC>::T;
   XPlain<T> xg{x, 1};
   EpiStore<T, 1> epi{y};
-  stencil_rows<PREC, XPlain<T>, EpiStore<T, 1>, PER>(A, xg, epi);
+  if constexpr (PER)
+    stencil_rows<PREC, XPlain<T>, EpiStore<T, 1>, PER>(A, xg, epi);
+  else
+    stencil_rows_n<PREC>(A, x, epi);
 }
 
 template <int IP>
@@ -877,9 +974,8 @@ k_cg_spmv_st(StDev A, const typename Val<IP>::T *__restrict__ p,
   using VT = typename Val<IP>::T;
   __shared__ double red[4];
   if (S->done) return;
-  XPlain<VT> xg{p, 1};
   EpiPq<VT> epi{q, p, dlo, 0.0, 1};
-  stencil_rows<IP>(A, xg, epi);
+  stencil_rows_n<IP>(A, p, epi);
   const double t = block_sum256(epi.d, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -889,10 +985,9 @@ __global__ void __launch_bounds__(256)
 k_residual_st(StDev A, const double *__restrict__ b, const double *__restrict__ x,
               EpiResidual<VT, 1> epi, double *part) {
   __shared__ double red[4];
-  XPlain<double> xg{x, 1};
   epi.drz = 0.0;
   epi.drr = 0.0;
-  stencil_rows<RP>(A, xg, epi);
+  stencil_rows_n<RP>(A, x, epi);
   const double t1 = block_sum256(epi.drz, red);
   const double t2 = block_sum256(epi.drr, red);
   if (threadIdx.x == 0) {
@@ -911,9 +1006,8 @@ k_cg_lapply_st(StDev C, const typename Val<LP>::T *__restrict__ u, const VT *__r
   using UT = typename Val<LP>::T;
   __shared__ double red[4];
   if (check_done && S->done) return;
-  XPlain<UT> xg{u, 1};
   EpiLearned<VT, 1, UT> epi{u, r, z, linv, 0.0, 0.0};
-  stencil_rows<LP>(C, xg, epi);
+  stencil_rows_n<LP>(C, u, epi);
   const double t1 = block_sum256(epi.drz, red);
   const double t2 = block_sum256(epi.drr, red);
   if (threadIdx.x == 0) {
