@@ -345,19 +345,47 @@ struct EpiPq {
   const float *dlo;
   double d;
   int bd;
-  __device__ void operator()(uint32_t r, const VT *s) {
-    VT sv[3];
+  // AT: the row sums' type (double for FP32 vectors: q is rounded once, after
+  // the low parts' term)
+  template <typename AT>
+  __device__ void operator()(uint32_t r, const AT *s) {
+    AT sv[3];
     for (int a = 0; a < bd; ++a) sv[a] = s[a];
     if (dlo) {
       const float *D = dlo + (uint64_t)r * bd * bd;
       for (int a = 0; a < bd; ++a)
-        for (int b = 0; b < bd; ++b) sv[a] += (VT)D[bd * a + b] * p[(uint64_t)r * bd + b];
+        for (int b = 0; b < bd; ++b) sv[a] += (AT)D[bd * a + b] * (AT)p[(uint64_t)r * bd + b];
     }
     for (int a = 0; a < bd; ++a) {
       const uint64_t o = (uint64_t)r * bd + a;
-      q[o] = sv[a];
-      d += (double)sv[a] * (double)p[o];
+      const VT qv = (VT)sv[a];
+      q[o] = qv;
+      d += (double)qv * (double)p[o];
     }
+  }
+};
+
+// The CG inner SpMV's accumulation precision: FP32 values times FP32 vectors
+// are summed in FP64 (kF32D's loop: every FP32 x FP32 product is exact in
+// FP64, so a row's sum carries one rounding, that of q).  FP32 sums lose the
+// low modes' small A p to the cancellation of large terms: on the 32^3 x 5
+// and 48^3 x 5 tet systems they cost 27% and 41% more PCG iterations than
+// FP64 sums, which match FP64 vectors' counts.
+template <int IP>
+constexpr int cg_acc_prec() {
+  return IP == SMH_F32 ? kF32D : IP;
+}
+// Gather of FP32/FP64 vector entries widened to the loop's type AT.
+template <typename VT, typename AT>
+struct XWide {
+  const VT *x;
+  int bd;
+  __device__ AT operator()(uint32_t j, int b) const { return (AT)x[(uint64_t)j * bd + b]; }
+  template <int BD>
+  __device__ void row(uint32_t j, AT *out) const {
+    VT t[BD];
+    load_row<VT, BD, false>(x + (uint64_t)j * BD, t);
+    for (int b = 0; b < BD; ++b) out[b] = (AT)t[b];
   }
 };
 
@@ -380,13 +408,14 @@ __global__ void __launch_bounds__(TileShape<BD>::kThreads)
 k_cg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP>::T *__restrict__ q,
           const CGState *S, const float *dlo, double *part) {
   using VT = typename Val<IP>::T;
-  __shared__ TileSmem<IP, BD> sm;
+  constexpr int AP = cg_acc_prec<IP>();
+  __shared__ TileSmem<AP, BD, TileShape<BD>> sm;
   if (S->done) return;
   uint32_t k0, k1;
   wg_tiles(A, k0, k1);
-  XPlain<VT> xg{p, BD};
+  XWide<VT, typename Val<AP>::T> xg{p, BD};
   EpiPq<VT> epi{q, p, dlo, 0.0, BD};
-  spmv_tiles<IP, BD>(A, k0, k1, sm, xg, epi);
+  spmv_tiles<AP, BD>(A, k0, k1, sm, xg, epi);
   const double t = block_sum<TileShape<BD>::kWaves>(epi.d, sm.red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -398,13 +427,14 @@ k_cg_spmv_pl(TileDev A, const typename Val<IP>::T *__restrict__ p,
              typename Val<IP>::T *__restrict__ q, const CGState *S, const float *dlo,
              double *part) {
   using VT = typename Val<IP>::T;
-  __shared__ TileSmem<IP, 3, TileShapePl> sm;
+  constexpr int AP = cg_acc_prec<IP>();
+  __shared__ TileSmem<AP, 3, TileShapePl> sm;
   if (S->done) return;
   uint32_t k0, k1;
   wg_tiles(A, k0, k1);
-  XPlain<VT> xg{p, 3};
+  XWide<VT, typename Val<AP>::T> xg{p, 3};
   EpiPq<VT> epi{q, p, dlo, 0.0, 3};
-  spmv_tiles<IP, 3>(A, k0, k1, sm, xg, epi);
+  spmv_tiles<AP, 3>(A, k0, k1, sm, xg, epi);
   const double t = block_sum<TileShapePl::kWaves>(epi.d, sm.red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -897,7 +927,8 @@ __device__ __forceinline__ void load_window(const T *x, int64_t j0, uint64_t n, 
     }
   }
 }
-template <int PREC, class EPI>
+// AT: the row sums' type (the CG inner SpMV sums FP32 products in FP64)
+template <int PREC, class EPI, typename AT = typename Val<PREC>::T>
 __device__ __forceinline__ void stencil_rows_n(const StDev &A, const typename Val<PREC>::T *x,
                                                EPI &epi) {
 #pragma clang fp contract(off)
@@ -910,14 +941,14 @@ __device__ __forceinline__ void stencil_rows_n(const StDev &A, const typename Va
     const uint64_t i = R * gi;
     if (i + R > n) {  // the ragged last group: row by row, as stencil_rows
       for (uint64_t r = i; r < n; ++r) {
-        T acc = 0;
+        AT acc = 0;
 #pragma unroll
         for (int s = 0; s < 27; ++s) {
           T v = val_of<PREC>(((const S *)A.vals)[(uint64_t)s * n + r]);
           if constexpr (Val<PREC>::kLo) v += (T)((const float *)A.vals_lo)[(uint64_t)s * n + r];
           const int64_t j = (int64_t)r + A.off[s];
           const uint64_t jc = j < 0 ? 0 : (j >= (int64_t)n ? n - 1 : (uint64_t)j);
-          const T prod = v * x[jc];
+          const AT prod = (AT)v * (AT)x[jc];
           acc = acc + prod;
         }
         epi((uint32_t)r, &acc);
@@ -935,7 +966,7 @@ __device__ __forceinline__ void stencil_rows_n(const StDev &A, const typename Va
 #pragma unroll
     for (int pc = 0; pc < 9; ++pc)
       load_window<T, R + 2>(x, (int64_t)i + A.off[3 * pc + 1] - 1, n, w[pc]);
-    T acc[R];
+    AT acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0;
 #pragma unroll
@@ -946,7 +977,7 @@ __device__ __forceinline__ void stencil_rows_n(const StDev &A, const typename Va
         for (int r = 0; r < R; ++r) {
           T vv = val_of<PREC>(v[3 * pc + k][r]);
           if constexpr (Val<PREC>::kLo) vv += (T)l[3 * pc + k][r];
-          const T prod = vv * w[pc][k + r];
+          const AT prod = (AT)vv * (AT)w[pc][k + r];
           acc[r] = acc[r] + prod;
         }
 #pragma unroll
@@ -975,7 +1006,7 @@ k_cg_spmv_st(StDev A, const typename Val<IP>::T *__restrict__ p,
   __shared__ double red[4];
   if (S->done) return;
   EpiPq<VT> epi{q, p, dlo, 0.0, 1};
-  stencil_rows_n<IP>(A, p, epi);
+  stencil_rows_n<IP, EpiPq<VT>, typename Val<cg_acc_prec<IP>()>::T>(A, p, epi);
   const double t = block_sum256(epi.d, red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -1473,19 +1504,21 @@ struct EpiPqRow {
   const float *dlo;
   double *prow;
   int bd;
-  __device__ void operator()(uint32_t r, const VT *s) {
-    VT sv[3];
+  template <typename AT>  // the row sums' type, as EpiPq
+  __device__ void operator()(uint32_t r, const AT *s) {
+    AT sv[3];
     for (int a = 0; a < bd; ++a) sv[a] = s[a];
     if (dlo) {
       const float *D = dlo + (uint64_t)r * bd * bd;
       for (int a = 0; a < bd; ++a)
-        for (int b = 0; b < bd; ++b) sv[a] += (VT)D[bd * a + b] * p[(uint64_t)r * bd + b];
+        for (int b = 0; b < bd; ++b) sv[a] += (AT)D[bd * a + b] * (AT)p[(uint64_t)r * bd + b];
     }
     double d = 0.0;
     for (int a = 0; a < bd; ++a) {
       const uint64_t o = (uint64_t)r * bd + a;
-      q[o] = sv[a];
-      d += (double)sv[a] * (double)p[o];
+      const VT qv = (VT)sv[a];
+      q[o] = qv;
+      d += (double)qv * (double)p[o];
     }
     prow[r] = d;
   }
@@ -1496,7 +1529,8 @@ __global__ void __launch_bounds__(SH::kThreads)
 k_bcg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP>::T *__restrict__ q,
            const float *dlo, double *prow) {
   using VT = typename Val<IP>::T;
-  __shared__ TileSmem<IP, BD, SH> sm;
+  constexpr int AP = cg_acc_prec<IP>();
+  __shared__ TileSmem<AP, BD, SH> sm;
   uint32_t k0, k1;
   wg_tiles(A, k0, k1);
   if (k0 < k1 && A.row_sys) {
@@ -1508,9 +1542,9 @@ k_bcg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP
         any = A.sys_state[m] == 0;
     if (!any) return;  // the same answer on every thread
   }
-  XPlain<VT> xg{p, BD};
+  XWide<VT, typename Val<AP>::T> xg{p, BD};
   EpiPqRow<VT> epi{q, p, dlo, prow, BD};
-  spmv_tiles<IP, BD>(A, k0, k1, sm, xg, epi);
+  spmv_tiles<AP, BD>(A, k0, k1, sm, xg, epi);
 }
 
 // K1 over lane-planar FP32 3x3 values (the batched mixed solver's planar
@@ -1520,12 +1554,13 @@ __global__ void __launch_bounds__(TileShapePl::kThreads)
 k_bcg_spmv_pl(TileDev A, const typename Val<IP>::T *__restrict__ p,
               typename Val<IP>::T *__restrict__ q, const float *dlo, double *prow) {
   using VT = typename Val<IP>::T;
-  __shared__ TileSmem<IP, 3, TileShapePl> sm;
+  constexpr int AP = cg_acc_prec<IP>();
+  __shared__ TileSmem<AP, 3, TileShapePl> sm;
   uint32_t k0, k1;
   wg_tiles(A, k0, k1);
-  XPlain<VT> xg{p, 3};
+  XWide<VT, typename Val<AP>::T> xg{p, 3};
   EpiPqRow<VT> epi{q, p, dlo, prow, 3};
-  spmv_tiles<IP, 3>(A, k0, k1, sm, xg, epi);
+  spmv_tiles<AP, 3>(A, k0, k1, sm, xg, epi);
 }
 
 // Chunks: crow[c]..crow[c+1] are the rows of chunk c, all in system cmesh[c];
