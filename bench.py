@@ -11,13 +11,15 @@ forward/backward kernels, on 2000 seeded 32^3 samples of the same physics.
 One step (the metric) = assemble the FP64 blocked-CSR matrix and the
 right-hand side from the resident mesh, split it into the FP32 hi/lo pair,
 run the network to get the learned preconditioner, then solve to a 1e-8
-relative residual with the mixed-precision learned PCG (FP32 SpMV, FP64 dots,
-reliable updates and certification on the FP64 matrix) from x = 0.  Block-Jacobi PCG (mixed and FP64) and plain FP64 CG on the same GPU
-are timed beside it, each from assembly to a certified 1e-8 residual.
+relative residual with the mixed-precision learned PCG (FP32 SpMV values and
+vectors with FP64 row sums, FP64 dots, reliable updates and certification on
+the FP64 matrix) from x = 0.  Block-Jacobi PCG (mixed and FP64) and plain
+FP64 CG on the same GPU are timed beside it, each from assembly to a
+certified 1e-8 residual.
 
 The roofline object is for the dominant kernel, the FP32 3x3 blocked SpMV
 over the lane-planar copy of the values (k_bsrmv_planar; the mixed CG's
-inner k_cg_spmv_pl runs the same tile loop): algorithmic
+inner k_cg_spmv_pl runs the same tile loop, with FP64 row sums): algorithmic
 bytes per launch = 36*nnz (values) + 4*nnz (colidx) + 4*(N+1) (rowptr) +
 12*N (x) + 12*N (y), timed with HIP events on the stream it runs on.
 """

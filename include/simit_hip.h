@@ -317,7 +317,9 @@ typedef struct {
   int32_t precond;
   int32_t mixed;         /* 1: FP32 inner iteration on vals (hi part) with
                             FP64 reliable updates r = b - A x on hi+lo
-                            (needs SMH_F32X2); 2: the same with the inner
+                            (needs SMH_F32X2); its SpMV multiplies FP32
+                            values by FP32 vectors and sums each row in
+                            FP64, rounding q once; 2: the same with the inner
                             vectors (p, q, r, z) kept in FP64 (over
                             stencil storage the learned correction's
                             input u = Linv r is gathered in FP32)         */
