@@ -88,6 +88,8 @@ def parse():
                          "this factor since its last one")
     ap.add_argument("--batch-check", type=int, default=8,
                     help="batch workload: iterations per captured graph")
+    ap.add_argument("--hex-mixed", type=int, default=2, choices=(1, 2),
+                    help="hex workload: 1 = FP32 CG vectors, 2 = FP64 CG vectors")
     ap.add_argument("--hex-n", type=int, default=192)
     ap.add_argument("--hex-train-n", type=int, default=48)
     ap.add_argument("--hex-train-samples", type=int, default=500)
@@ -488,7 +490,7 @@ def run_hex(args, ctx):
         ctx.stencil27_correction(idx, corr, dims, corr_st)
         x = torch.zeros(nv, dtype=torch.float64, device="cuda")
         last["x"] = x
-        return ctx.cg_stencil27(St, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=2,
+        return ctx.cg_stencil27(St, rhs, x, opts(precond=S.PRECOND_LEARNED, mixed=args.hex_mixed,
                                                  learned=linv, learned_corr=corr_st,
                                                  vals_f64=st64))
 
@@ -522,10 +524,11 @@ def run_hex(args, ctx):
         if stencil:
             ctx.stencil27_from_bsr(A2, dims, st_hi, st_lo)
             ctx.stencil27_from_bsr(A64, dims, st64)
-            st = ctx.cg_stencil27(St, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=2,
+            st = ctx.cg_stencil27(St, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=args.hex_mixed,
                                                    vals_f64=st64))
         else:
-            st = ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=2, vals_f64=v64))
+            st = ctx.cg(A2, rhs, x, opts(precond=S.PRECOND_JACOBI, mixed=args.hex_mixed,
+                                         vals_f64=v64))
         torch.cuda.synchronize()
         r = {"seconds": round(time.perf_counter() - t, 3), "iters": st.iters,
              "reliable_updates": st.reliable_updates, "true_rel": st.true_rel,
