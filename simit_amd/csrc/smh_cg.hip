@@ -422,20 +422,20 @@ k_cg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP>
 
 // K1 over lane-planar FP32 3x3 values (the mixed solver's inner SpMV)
 template <int IP>
-__global__ void __launch_bounds__(TileShapePl::kThreads)
+__global__ void __launch_bounds__(TileShapePlS::kThreads)
 k_cg_spmv_pl(TileDev A, const typename Val<IP>::T *__restrict__ p,
              typename Val<IP>::T *__restrict__ q, const CGState *S, const float *dlo,
              double *part) {
   using VT = typename Val<IP>::T;
   constexpr int AP = cg_acc_prec<IP>();
-  __shared__ TileSmem<AP, 3, TileShapePl> sm;
+  __shared__ TileSmem<AP, 3, TileShapePlS> sm;
   if (S->done) return;
   uint32_t k0, k1;
   wg_tiles(A, k0, k1);
   XWide<VT, typename Val<AP>::T> xg{p, 3};
   EpiPq<VT> epi{q, p, dlo, 0.0, 3};
   spmv_tiles<AP, 3>(A, k0, k1, sm, xg, epi);
-  const double t = block_sum<TileShapePl::kWaves>(epi.d, sm.red);
+  const double t = block_sum<TileShapePlS::kWaves>(epi.d, sm.red);
   if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
@@ -1207,7 +1207,7 @@ struct TileOps {
             double *part) const {
     if constexpr (kPlanar) {
       if (planar()) {
-        hipLaunchKernelGGL((k_cg_spmv_pl<IP>), dim3(gp), dim3(TileShapePl::kThreads), 0, s, TP,
+        hipLaunchKernelGGL((k_cg_spmv_pl<IP>), dim3(gp), dim3(TileShapePlS::kThreads), 0, s, TP,
                            p, q, S, dlo, part);
         return;
       }
@@ -1550,12 +1550,12 @@ k_bcg_spmv(TileDev A, const typename Val<IP>::T *__restrict__ p, typename Val<IP
 // K1 over lane-planar FP32 3x3 values (the batched mixed solver's planar
 // shape)
 template <int IP>
-__global__ void __launch_bounds__(TileShapePl::kThreads)
+__global__ void __launch_bounds__(TileShapePlS::kThreads)
 k_bcg_spmv_pl(TileDev A, const typename Val<IP>::T *__restrict__ p,
               typename Val<IP>::T *__restrict__ q, const float *dlo, double *prow) {
   using VT = typename Val<IP>::T;
   constexpr int AP = cg_acc_prec<IP>();
-  __shared__ TileSmem<AP, 3, TileShapePl> sm;
+  __shared__ TileSmem<AP, 3, TileShapePlS> sm;
   uint32_t k0, k1;
   wg_tiles(A, k0, k1);
   XWide<VT, typename Val<AP>::T> xg{p, 3};
@@ -1887,7 +1887,7 @@ int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const sm
       } else {
         rc = planar_buffer(ctx, A, &pv);
         if (rc) return rc;
-        rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, (uint32_t)TileShapePl::kBlocks,
+        rc = tile_plan(ctx, A->rowptr, A->nrows, A->nnz, (uint32_t)TileShapePlS::kBlocks,
                        &pplan);
         if (rc) return rc;
       }
@@ -1918,7 +1918,7 @@ using BShape1 = TileShapeR<3, 8, 64>;
 using BShape2 = TileShapeR<3, 24, 64>;
 using BShape3 = TileShapeR<3, 8, 128>;
 using BShape4 = TileShapeR<3, 16, 128>;
-// shape 5: lane-planar FP32 values (TileShapePl; FP32 inner products only),
+// shape 5: lane-planar FP32 values (TileShapePlS; FP32 inner products only),
 // with workgroups striding over their XCD's tiles
 constexpr int kBShapes = 5, kBShapePlanar = 5;
 inline uint32_t bshape_blocks(int k) {
@@ -1927,7 +1927,7 @@ inline uint32_t bshape_blocks(int k) {
     case 2: return BShape2::kBlocks;
     case 3: return BShape3::kBlocks;
     case 4: return BShape4::kBlocks;
-    case kBShapePlanar: return TileShapePl::kBlocks;
+    case kBShapePlanar: return TileShapePlS::kBlocks;
   }
   return BShape0::kBlocks;
 }
@@ -1937,7 +1937,7 @@ inline unsigned bshape_threads(int k) {
     case 2: return BShape2::kThreads;
     case 3: return BShape3::kThreads;
     case 4: return BShape4::kThreads;
-    case kBShapePlanar: return TileShapePl::kThreads;
+    case kBShapePlanar: return TileShapePlS::kThreads;
   }
   return BShape0::kThreads;
 }
@@ -1965,7 +1965,7 @@ void launch_bcg_spmv(int shape, unsigned g, hipStream_t s, const TileDev &T,
         return;
       case kBShapePlanar:
         if constexpr (IP == SMH_F32) {
-          hipLaunchKernelGGL((k_bcg_spmv_pl<IP>), dim3(g), dim3(TileShapePl::kThreads), 0, s, T,
+          hipLaunchKernelGGL((k_bcg_spmv_pl<IP>), dim3(g), dim3(TileShapePlS::kThreads), 0, s, T,
                              p, q, dlo, prow);
           return;
         }
@@ -2688,7 +2688,7 @@ extern "C" int smh_cg_solve_noalloc(smh_ctx *ctx, const smh_bsr *A, const double
   // mixed 3x3: the lane-planar copy of the values and its tile rows, as
   // smh_cg_solve streams them
   const bool want_pl = bd == 3 && (o->mixed == 1 || o->mixed == 2);
-  const uint32_t tbp = (uint32_t)TileShapePl::kBlocks;
+  const uint32_t tbp = (uint32_t)TileShapePlS::kBlocks;
   const uint64_t max_tiles_pl = A->nnz / (tbp / 2 + 1) + 1;
   if (want_pl) need += sizeof(float) * planar_floats(A->nnz) + 256 + 4 * (max_tiles_pl + 1) + 256;
   SMH_CHECK(ctx, need <= ctx->scratch_bytes, SMH_ERR_UNSUPPORTED,

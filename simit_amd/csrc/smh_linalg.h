@@ -99,9 +99,17 @@ struct TileShapeB {
 // tile starts inside a group), against one 12-byte load per block row in the
 // row-major layout.  Measured at 96^3 x 5 tets (FP32, 11.7M blocks): 91 us
 // against 104 us for the same products, bitwise equal (each block row is
-// summed in the same order).
+// summed in the same order).  Rounds per lane (tiles of 64 x rounds
+// blocks), measured at 96^3 x 5 tets: the plain FP32 product 88.1-88.8 us at
+// 8 rounds across boxes (87.6-92.3 at 12, 93.9 at 10, 99.1 at 16); the
+// solvers' SpMV with FP64 row sums (TileShapePlS) 96-98 us at 12 rounds
+// against 104-106 at 8, 107 at 10 and 129 at 16 (12 rounds hold 220
+// registers, 2 waves per SIMD).
 #ifndef SMH_TILE_ROUNDS_PL
 #define SMH_TILE_ROUNDS_PL 8
+#endif
+#ifndef SMH_TILE_ROUNDS_PLS
+#define SMH_TILE_ROUNDS_PLS 12
 #endif
 // Grid of the planar kernels.  Their workgroups stride over their XCD's
 // eighth of the tiles (TileDev::interleave), so the tiles in flight on one
@@ -116,9 +124,10 @@ constexpr uint64_t kPlanarGroup = 576;  // floats per group of 64 blocks
 __host__ __device__ inline uint64_t planar_floats(uint64_t nnz) {
   return ((nnz + 63) / 64) * kPlanarGroup;
 }
-struct TileShapePl {
+template <int ROUNDS>
+struct TileShapePlR {
   static constexpr int kThreads = 64;
-  static constexpr int kRounds = SMH_TILE_ROUNDS_PL;
+  static constexpr int kRounds = ROUNDS;
   static constexpr int kWaves = 1;
   static constexpr int kVec = 1;
   static constexpr int kPerWave = 64;
@@ -127,6 +136,8 @@ struct TileShapePl {
   static constexpr bool kBPL = true;
   static constexpr bool kPlanar = true;
 };
+using TileShapePl = TileShapePlR<SMH_TILE_ROUNDS_PL>;    // smh_bsrmv (FP32 row sums)
+using TileShapePlS = TileShapePlR<SMH_TILE_ROUNDS_PLS>;  // the solvers' inner SpMV
 inline uint32_t tile_blocks_host(int bd) {
   return bd == 3 ? (SMH_TILE_THREADS3 / 64) * 21 * SMH_TILE_ROUNDS3
                  : 256 * SMH_TILE_VEC1 * SMH_TILE_ROUNDS1;
@@ -588,9 +599,8 @@ struct TileRegsB {
 
 // Registers of one lane for lane-planar FP32 3x3 tiles (TileShapePl): one
 // whole block per round, its nine values and its column.
-template <int PREC>
+template <int PREC, class SH>
 struct TileRegsPl {
-  using SH = TileShapePl;
   static_assert(std::is_same<typename Val<PREC>::S, float>::value && !Val<PREC>::kLo,
                 "lane-planar tiles hold plain float values");
   static constexpr int kRounds = SH::kRounds;
@@ -665,7 +675,7 @@ __device__ __forceinline__ void spmv_tiles(const TileDev &A, uint32_t k0, uint32
   using Regs = typename std::conditional<
       BD == 1, TileRegsV<PREC, SH>,
       typename std::conditional<
-          SH::kPlanar, TileRegsPl<PREC>,
+          SH::kPlanar, TileRegsPl<PREC, SH>,
           typename std::conditional<SH::kBPL, TileRegsB<PREC>,
                                     TileRegs<PREC, BD, SH>>::type>::type>::type;
   Regs cur, nxt;
