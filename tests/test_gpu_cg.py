@@ -90,6 +90,40 @@ def test_tet_cg_modes_reach_1e8(ctx, oracle, mode):
     assert np.linalg.norm(xv - xo) <= 1e-5 * np.linalg.norm(xo)
 
 
+def test_mixed_inner_spmv_sums_rows_in_fp64(ctx, oracle):
+    """mixed = 1 keeps FP32 vectors, but its inner SpMV sums each row in FP64
+    (the FP32 x FP32 products are exact there) and rounds q once.  On a stiff
+    tet system (log10 E a random field with std 1) that keeps its block-Jacobi
+    PCG within a few percent of mixed = 2's iteration count; FP32 row sums lose
+    the small A p of the smooth modes to the cancellation of large terms (a
+    numpy replay of the solver on this system: 1,572 iterations with FP32 row
+    sums, 1,264 with FP64 row sums, 1,181 with FP64 vectors)."""
+    prob = meshes.tet_problem(32, seed=0, field="grf", std_log10=1.0, corr_len=8.0)
+    d = {k: to_dev(prob[k]) for k in ("x", "c", "mu", "lam", "endpoints")}
+    idx = ctx.build_index(prob["nverts"], [d["endpoints"]])
+    P = S.TetParams(1.0, 0.0, (C.c_double * 3)(*prob["load"]), 1.0)
+    v64 = torch.empty(idx.nnz * 9, dtype=torch.float64, device="cuda")
+    rhs = torch.empty(prob["nverts"] * 3, dtype=torch.float64, device="cuda")
+    ctx.assemble_tets(idx, P, d, d, S.F64, v64, None, rhs)
+    hi = torch.empty(idx.nnz * 9, dtype=torch.float32, device="cuda")
+    lo = torch.empty_like(hi)
+    ctx.split_hilo(v64, hi, lo)
+    A = idx.bsr(3, S.F32X2, hi, lo)
+    rp, ci = idx.to_host()
+    ov, of = oracle.tet_assemble(oracle.tet_params(load=prob["load"]), prob, rp, ci)
+    its = {}
+    for mixed in (1, 2):
+        x = torch.zeros_like(rhs)
+        st = ctx.cg(A, rhs, x, ctx.cg_options(maxiters=20000, tol=1e-8, rel=1, mixed=mixed,
+                                              precond=S.PRECOND_BLOCK_JACOBI, vals_f64=v64))
+        print(mixed, st.as_dict(), flush=True)
+        assert st.status == 0 and st.true_rel <= 1e-8, (mixed, st.as_dict())
+        r = of - oracle.bsrmv(rp, ci, ov, x.cpu().numpy(), 3)
+        assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(of)
+        its[mixed] = st.iters
+    assert its[1] <= 1.15 * its[2], its
+
+
 def test_cg_repeatable_bitwise(ctx):
     _, idx, v64, hi, lo, rhs, _ = tet_system(ctx, 8, seed=1)
     outs = []
