@@ -1269,7 +1269,7 @@ int make_tile_ops(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan,
   }
   // the 3x3 correction pass strides over its XCD's tiles too (96^3 x 5:
   // 59 -> 57 us per pass)
-  ops.TC = tile_dev(&Cb, &lplan, &ops.gc, kIl || BD == 3);
+  ops.TC = tile_dev(&Cb, &lplan, &ops.gc, kIl || BD == 3, SMH_TILE_GRID_C);
   return SMH_OK;
 }
 
@@ -1278,7 +1278,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
            const smh_cg_opts *o, smh_cg_stats *st, Arena *ar = nullptr) {
   using VT = typename Val<IP>::T;
   const uint64_t n = (uint64_t)nrows * BD;
-  const unsigned nbe = grid_for(nrows, 256, 2048);
+  const unsigned nbe = grid_for(nrows, 256, SMH_CG_UPDATE_GRID);
   const unsigned nbv = grid_for(n, 256, 2048);
   const int pc = o->precond;
   const bool learned = pc == SMH_PRECOND_LEARNED;

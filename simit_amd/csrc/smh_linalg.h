@@ -42,11 +42,13 @@ namespace smh {
 #ifndef SMH_TILE_ROUNDS1
 #define SMH_TILE_ROUNDS1 1
 #endif
-// FP8 3x3 tiles (one block per lane): 4 rounds, whose gathers go out as one
-// group (measured at 96^3 x 5: 58 us per correction pass against 67-86 us for
-// 1-3 rounds and 75-93 us for 6-16)
+// Rounds per lane of the FP8 3x3 correction pass (one block per lane),
+// measured at 96^3 x 5 tets with the row sums on one thread per row: 54 / 49
+// / 47 / 60 / 54 / 59 us at 4 / 6 / 8 / 10 / 12 / 16 rounds (6 and 10 gather
+// in groups of three and two instead of four; with (row, entry) sums staged
+// through LDS, 4 rounds had been best: 58 us)
 #ifndef SMH_TILE_ROUNDS8
-#define SMH_TILE_ROUNDS8 4
+#define SMH_TILE_ROUNDS8 8
 #endif
 // matrix values are read once per product: streaming (non-temporal) loads
 #ifndef SMH_STREAM_LOAD
@@ -401,6 +403,16 @@ struct XPlain {
 // Tile kernels run persistent workgroups (16 per CU on 256 CUs by default),
 // each over a contiguous range of tiles; at most kMaxTileGrid of them.
 constexpr int kMaxTileGrid = 16384;
+// Grid of the learned correction pass (8,192: 53.4 us against 54.3 at 4 rounds)
+#ifndef SMH_TILE_GRID_C
+#define SMH_TILE_GRID_C 4096
+#endif
+// Grid of the CG x/r update, whose every workgroup sums the SpMV's partials:
+// 512 / 1,024 / 2,048 workgroups took 22.6 / 23.4 / 24.2 us at 96^3 x 5 tets
+// and 91.5 / 74.2 / 82.5 us on the 192^3 hex system
+#ifndef SMH_CG_UPDATE_GRID
+#define SMH_CG_UPDATE_GRID 1024
+#endif
 #ifndef SMH_TILE_GRID
 #define SMH_TILE_GRID 4096
 #endif
