@@ -1279,7 +1279,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   using VT = typename Val<IP>::T;
   const uint64_t n = (uint64_t)nrows * BD;
   const unsigned nbe = grid_for(nrows, 256, SMH_CG_UPDATE_GRID);
-  const unsigned nbv = grid_for(n, 256, 2048);
+  const unsigned nbv = grid_for(n, 256, SMH_CG_VEC_GRID);
   const int pc = o->precond;
   const bool learned = pc == SMH_PRECOND_LEARNED;
   const unsigned gt = ops.parts_spmv(), grt = ops.parts_res(), gc = ops.parts_corr();
@@ -1898,7 +1898,7 @@ int cg_tiles(smh_ctx *ctx, const smh_bsr *A, const smh_tile_plan *plan, const sm
         smh_bsr Pb = *A;
         Pb.vals = pv;
         Pb.vals_lo = nullptr;
-        ops.TP = tile_dev(&Pb, &pplan, &ops.gp, true, SMH_TILE_GRID_PL);
+        ops.TP = tile_dev(&Pb, &pplan, &ops.gp, true, SMH_TILE_GRID_PLS);
       }
     }
   }
@@ -2926,7 +2926,7 @@ extern "C" int smh_stencil27_mv(smh_ctx *ctx, const smh_stencil27 *S, const void
   if (rc) return rc;
   SMH_CHECK(ctx, x && y, SMH_ERR_INVALID, "smh_stencil27_mv: null vector");
   const StDev D = st_dev(S, S->vals, S->vals_lo);
-  const unsigned g = grid_for(D.n, 256, 4096);
+  const unsigned g = grid_for(D.n, 256, SMH_ST_GRID);
   if (S->flags & SMH_STENCIL_PERIODIC) {
     SMH_CHECK(ctx, S->precision == SMH_F64 || S->precision == SMH_F32, SMH_ERR_UNSUPPORTED,
               "smh_stencil27_mv: periodic stencils hold SMH_F64 or SMH_F32 values");
@@ -2966,7 +2966,7 @@ int cg_stencil(smh_ctx *ctx, const smh_stencil27 *S, const double *b, double *x,
   ops.A = st_dev(S, S->vals, S->vals_lo);
   ops.R = RP == SMH_F64 && o->vals_f64 ? st_dev(S, o->vals_f64, nullptr) : ops.A;
   ops.C = st_dev(S, o->precond == SMH_PRECOND_LEARNED ? o->learned_corr : S->vals, nullptr);
-  ops.g = grid_for(ops.A.n, 256, 4096);
+  ops.g = grid_for(ops.A.n, 256, SMH_ST_GRID);
   return cg_run<IP, RP, 1>(ctx, ops, ops.A.n, S->precision == SMH_F32X2 && S->vals_lo, b, x, o,
                            st);
 }

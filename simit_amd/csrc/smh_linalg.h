@@ -413,10 +413,30 @@ constexpr int kMaxTileGrid = 16384;
 #ifndef SMH_CG_UPDATE_GRID
 #define SMH_CG_UPDATE_GRID 1024
 #endif
+// Grid of the CG p update and the other vector passes (1,024: 6.7 / 28.0 us
+// against 6.0 / 30.8 us at 2,048 on 96^3 x 5 tets / the 192^3 hex system)
+#ifndef SMH_CG_VEC_GRID
+#define SMH_CG_VEC_GRID 2048
+#endif
+// Grid of the solvers' planar inner SpMV at 96^3 x 5 tets: 8,192 / 12,288 /
+// 16,384 workgroups 98.4 / 95.0 / 95.4 us, the update kernel that sums their
+// partials 23.4 / 24.6 / 28.3 us
+#ifndef SMH_TILE_GRID_PLS
+#define SMH_TILE_GRID_PLS 12288
+#endif
+// Grid of the stencil row kernels on the 192^3 hex system: 2,048 / 4,096 /
+// 8,192 workgroups, the solver's SpMV 206.7 / 204.4 / 198.3 us, its
+// correction pass 121.8 / 120.6 / 117.2 us
+#ifndef SMH_ST_GRID
+#define SMH_ST_GRID 8192
+#endif
 #ifndef SMH_TILE_GRID
 #define SMH_TILE_GRID 4096
 #endif
 static_assert(SMH_TILE_GRID <= kMaxTileGrid, "tile grid above kMaxTileGrid");
+static_assert(SMH_TILE_GRID_PLS <= kMaxTileGrid && SMH_TILE_GRID_C <= kMaxTileGrid &&
+                  SMH_ST_GRID <= kMaxTileGrid,
+              "solver grids above kMaxTileGrid");
 static_assert(SMH_TILE_GRID_PL <= kMaxTileGrid, "tile grid above kMaxTileGrid");
 
 // Scalar (1x1) products of a single system stride over the tiles (below):
