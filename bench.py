@@ -43,13 +43,13 @@ HBM_PEAK_GBS = 8000.0
 # summaries and commands in profiles/.
 # 96^3 x 5 tets, the planar SpMV (k_bsrmv_planar): FETCH 238.61 MiB x 2.0 + WRITE 10.51 MiB x 1.0
 SPMV_TRAFFIC_96 = 511_424_343
-# 192^3 hexes, blocked CSR (scalar, FP32 values, FP64 vectors): FETCH 803.20 MiB x 2.0 +
-# WRITE 56.27 MiB x 1.0
-SPMV_TRAFFIC_HEX192 = 1_743_427_658
-# the same matrix in 27-point stencil storage: FETCH 501.14 MiB x 2.0 + WRITE 54.85 MiB x 1.0
-# (1.24x the algorithmic bytes: FETCH_SIZE counts the x re-reads the Infinity
+# 192^3 hexes, blocked CSR (scalar, FP32 values, FP64 vectors): FETCH 814.36 MiB x 2.0 +
+# WRITE 56.22 MiB x 1.0
+SPMV_TRAFFIC_HEX192 = 1_766_785_233
+# the same matrix in 27-point stencil storage: FETCH 489.33 MiB x 2.0 + WRITE 54.85 MiB x 1.0
+# (1.22x the algorithmic bytes: FETCH_SIZE counts the x re-reads the Infinity
 # Cache serves as well as HBM reads)
-STENCIL_TRAFFIC_HEX192 = 1_108_476_741
+STENCIL_TRAFFIC_HEX192 = 1_083_722_327
 METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
 # The mixed solves are certified on the FP64 matrix (smh_cg_opts.vals_f64):
 # the assembly writes FP64 values once and the inner FP32 operator's hi/lo
