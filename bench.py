@@ -82,6 +82,9 @@ def parse():
     ap.add_argument("--mixed", type=int, default=1, choices=(1, 2),
                     help="tet workload: 1 = FP32 CG vectors, 2 = FP64 CG vectors (both with "
                          "FP32 matrix values and FP64 dots and reliable updates)")
+    ap.add_argument("--reliable-delta", type=float, default=None,
+                    help="tet workload: reliable update when the iterated residual fell by this "
+                         "factor since the last one (the library's default: 0.3)")
     ap.add_argument("--nmesh", type=int, default=256)
     ap.add_argument("--batch-delta", type=float, default=0.01,
                     help="batch workload: reliable update when a lattice's residual fell by "
@@ -219,6 +222,8 @@ def main():
         f"training steps {tr['step_seconds']:.1f}s)")
 
     def opts(**kw):
+        if args.reliable_delta is not None:
+            kw["reliable_delta"] = args.reliable_delta
         return ctx.cg_options(maxiters=args.cg_maxiters, tol=CERT_TOL, rel=1, check_every=64,
                               **kw)
 
