@@ -1279,7 +1279,7 @@ int cg_run(smh_ctx *ctx, const OPS &ops, uint32_t nrows, bool has_lo, const doub
   using VT = typename Val<IP>::T;
   const uint64_t n = (uint64_t)nrows * BD;
   const unsigned nbe = grid_for(nrows, 256, SMH_CG_UPDATE_GRID);
-  const unsigned nbv = grid_for(n, 256, SMH_CG_VEC_GRID);
+  const unsigned nbv = grid_for(n, 256, sizeof(VT) == 8 ? SMH_CG_VEC_GRID64 : SMH_CG_VEC_GRID);
   const int pc = o->precond;
   const bool learned = pc == SMH_PRECOND_LEARNED;
   const unsigned gt = ops.parts_spmv(), grt = ops.parts_res(), gc = ops.parts_corr();

@@ -403,7 +403,8 @@ struct XPlain {
 // Tile kernels run persistent workgroups (16 per CU on 256 CUs by default),
 // each over a contiguous range of tiles; at most kMaxTileGrid of them.
 constexpr int kMaxTileGrid = 16384;
-// Grid of the learned correction pass (8,192: 53.4 us against 54.3 at 4 rounds)
+// Grid of the learned correction pass (8,192: 53.4 us against 54.3 at 4
+// rounds per tile, 48.0 against 47.0 at 8)
 #ifndef SMH_TILE_GRID_C
 #define SMH_TILE_GRID_C 4096
 #endif
@@ -413,10 +414,14 @@ constexpr int kMaxTileGrid = 16384;
 #ifndef SMH_CG_UPDATE_GRID
 #define SMH_CG_UPDATE_GRID 1024
 #endif
-// Grid of the CG p update and the other vector passes (1,024: 6.7 / 28.0 us
-// against 6.0 / 30.8 us at 2,048 on 96^3 x 5 tets / the 192^3 hex system)
+// Grid of the CG p update and the other vector passes: FP32 vectors 2,048
+// (1,024: 6.7 against 6.0 us at 96^3 x 5 tets), FP64 vectors 1,024 (27.6
+// against 30.5 us on the 192^3 hex system)
 #ifndef SMH_CG_VEC_GRID
 #define SMH_CG_VEC_GRID 2048
+#endif
+#ifndef SMH_CG_VEC_GRID64
+#define SMH_CG_VEC_GRID64 1024
 #endif
 // Grid of the solvers' planar inner SpMV at 96^3 x 5 tets: 8,192 / 12,288 /
 // 16,384 workgroups 98.4 / 95.0 / 95.4 us, the update kernel that sums their
