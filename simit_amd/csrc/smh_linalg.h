@@ -634,8 +634,8 @@ struct TileRegsB {
   }
 };
 
-// Registers of one lane for lane-planar FP32 3x3 tiles (TileShapePl): one
-// whole block per round, its nine values and its column.
+// Registers of one lane for lane-planar FP32 3x3 tiles (TileShapePl,
+// TileShapePlS): one whole block per round, its nine values and its column.
 template <int PREC, class SH>
 struct TileRegsPl {
   static_assert(std::is_same<typename Val<PREC>::S, float>::value && !Val<PREC>::kLo,
