@@ -46,10 +46,10 @@ SPMV_TRAFFIC_96 = 511_424_343
 # 192^3 hexes, blocked CSR (scalar, FP32 values, FP64 vectors): FETCH 814.36 MiB x 2.0 +
 # WRITE 56.22 MiB x 1.0
 SPMV_TRAFFIC_HEX192 = 1_766_785_233
-# the same matrix in 27-point stencil storage: FETCH 489.33 MiB x 2.0 + WRITE 54.85 MiB x 1.0
-# (1.22x the algorithmic bytes: FETCH_SIZE counts the x re-reads the Infinity
+# the same matrix in 27-point stencil storage: FETCH 479.32 MiB x 2.0 + WRITE 54.85 MiB x 1.0
+# (1.19x the algorithmic bytes: FETCH_SIZE counts the x re-reads the Infinity
 # Cache serves as well as HBM reads)
-STENCIL_TRAFFIC_HEX192 = 1_083_722_327
+STENCIL_TRAFFIC_HEX192 = 1_062_728_559
 METRIC = "time-to-1e-8 solve: learned-PCG vs plain CG; assembly+SpMV HBM GB/s"
 # The mixed solves are certified on the FP64 matrix (smh_cg_opts.vals_f64):
 # the assembly writes FP64 values once and the inner FP32 operator's hi/lo
@@ -614,7 +614,8 @@ def run_hex(args, ctx):
         "csr_spmv_traffic": SPMV_TRAFFIC_HEX192 if n == 192 else None,
         "roofline": {"bound": "hbm",
                      "kernel": "k_stmv<SMH_F32D> (27-point stencil SpMV: FP32 values, FP64 "
-                               "vectors; the solver's k_cg_spmv_st runs the same row loop)",
+                               "vectors, four rows per thread; the solver's k_cg_spmv_st runs "
+                               "the same row loop with two)",
                      "achieved": round(st_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(st_gbs / HBM_PEAK_GBS, 4),
                      "traffic": STENCIL_TRAFFIC_HEX192 if n == 192 else None,

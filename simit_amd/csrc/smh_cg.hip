@@ -899,12 +899,18 @@ __device__ __forceinline__ void stencil_rows(const StDev &A, const XG &xg, EPI &
 // and 27.  Each row's 27 products are added in slot order with the one-row
 // loop's rounding, so the results are the same bits (windows that leave
 // [0, n) load entry by entry, clamped; a ragged last group runs row by row).
-// 192^3 hex system, FP32 values and FP64 vectors: one row 211 us, two rows
-// 179 us (the solver's SpMV 231 -> 203 us), four rows 171 us but the
-// solver's SpMV 210 us (its epilogue pushes the kernel to 236 VGPRs); each
-// XCD walking a contiguous eighth of the rows was slower again (189 us).
+// 192^3 hex system, FP32 values and FP64 vectors, 4,096 workgroups: one row
+// 211 us, two rows 179 us (the solver's SpMV 231 -> 203 us), four rows
+// 171 us but the solver's SpMV 210 us (its epilogue pushes the kernel to 236
+// VGPRs); each XCD walking a contiguous eighth of the rows was slower again
+// (189 us).  At 8,192 workgroups: two rows 174 us (the solver's 197 us), four
+// rows 163 us (the solver's 206 us).  So the plain product (smh_stencil27_mv)
+// takes four rows and the solver's kernels two.
 #ifndef SMH_ST_ROWS
 #define SMH_ST_ROWS 2
+#endif
+#ifndef SMH_ST_ROWS_MV
+#define SMH_ST_ROWS_MV 4
 #endif
 template <typename S, int R>
 __device__ __forceinline__ void load_run(const S *p, S *o) {
@@ -927,14 +933,14 @@ __device__ __forceinline__ void load_window(const T *x, int64_t j0, uint64_t n, 
     }
   }
 }
-// AT: the row sums' type (the CG inner SpMV sums FP32 products in FP64)
-template <int PREC, class EPI, typename AT = typename Val<PREC>::T>
+// AT: the row sums' type (the CG inner SpMV sums FP32 products in FP64); R:
+// rows per thread
+template <int PREC, class EPI, typename AT = typename Val<PREC>::T, int R = SMH_ST_ROWS>
 __device__ __forceinline__ void stencil_rows_n(const StDev &A, const typename Val<PREC>::T *x,
                                                EPI &epi) {
 #pragma clang fp contract(off)
   using T = typename Val<PREC>::T;
   using S = typename Val<PREC>::S;
-  constexpr int R = SMH_ST_ROWS;
   const uint64_t n = A.n, ng = (n + R - 1) / R;
   for (uint64_t gi = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; gi < ng;
        gi += (uint64_t)gridDim.x * blockDim.x) {
@@ -994,7 +1000,7 @@ __global__ void __launch_bounds__(256) k_stmv(StDev A, const typename Val<PREC>:
   if constexpr (PER)
     stencil_rows<PREC, XPlain<T>, EpiStore<T, 1>, PER>(A, xg, epi);
   else
-    stencil_rows_n<PREC>(A, x, epi);
+    stencil_rows_n<PREC, EpiStore<T, 1>, T, SMH_ST_ROWS_MV>(A, x, epi);
 }
 
 template <int IP>
