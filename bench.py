@@ -366,7 +366,8 @@ def main():
         "spmv_rowmajor_us": round(spmv_rm_ms * 1e3, 2),
         "roofline": {"bound": "hbm",
                      "kernel": ("k_bsrmv_planar (FP32 3x3 BSR SpMV over lane-planar values; "
-                                "the mixed CG's inner k_cg_spmv_pl runs the same tile loop)"),
+                                "the mixed CG's inner k_cg_spmv_pl runs the same tile loop "
+                                "with 12 rounds per tile and FP64 row sums)"),
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4),
                      "traffic": SPMV_TRAFFIC_96 if n == 96 else None,
