@@ -41,8 +41,8 @@ HBM_PEAK_GBS = 8000.0
 # then WRITE_SIZE) of `bench.py --kernels-only`, each counter scaled by what it
 # reads for a copy of known size (y = x over 2^27 doubles: factors 2.0 and 1.0);
 # summaries and commands in profiles/.
-# 96^3 x 5 tets, the planar SpMV (k_bsrmv_planar): FETCH 238.61 MiB x 2.0 + WRITE 10.51 MiB x 1.0
-SPMV_TRAFFIC_96 = 511_424_343
+# 96^3 x 5 tets, the planar SpMV (k_bsrmv_planar): FETCH 237.44 MiB x 2.0 + WRITE 10.46 MiB x 1.0
+SPMV_TRAFFIC_96 = 508_920_714
 # 192^3 hexes, blocked CSR (scalar, FP32 values, FP64 vectors): FETCH 814.36 MiB x 2.0 +
 # WRITE 56.22 MiB x 1.0
 SPMV_TRAFFIC_HEX192 = 1_766_785_233

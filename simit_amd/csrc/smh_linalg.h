@@ -117,10 +117,11 @@ struct TileShapeB {
 // eighth of the tiles (TileDev::interleave), so the tiles in flight on one
 // XCD form one moving window (its L2 holds the x they gather) and the load
 // stays even for any grid.  Measured at 96^3 x 5 tets: SpMV 91 -> 87 us at
-// 8,192 workgroups (84 us at 16,384, but the solver's update kernel sums one
-// partial per SpMV workgroup: 8,192 gave the shortest CG iteration).
+// 8,192 workgroups, 86.4-87.4 us at 16,384 against 89.1-89.6 at 8,192 on one
+// box (the solvers' copy has its own grid, SMH_TILE_GRID_PLS: their update
+// kernel sums one partial per SpMV workgroup).
 #ifndef SMH_TILE_GRID_PL
-#define SMH_TILE_GRID_PL 8192
+#define SMH_TILE_GRID_PL 16384
 #endif
 constexpr uint64_t kPlanarGroup = 576;  // floats per group of 64 blocks
 __host__ __device__ inline uint64_t planar_floats(uint64_t nnz) {
